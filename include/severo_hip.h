@@ -1,0 +1,187 @@
+/*
+ * severo_hip.h -- C ABI of libsevero_hip.so, the MI355X (gfx950) engine for
+ * Severo.jl's embedding-and-graph path:
+ *
+ *   normalize_cells -> feature selection -> scale_features -> PCA (IRLBA)
+ *     -> nearest_neighbours (exact Euclidean kNN) -> Jaccard SNN graph
+ *
+ * Conventions
+ *   - Every entry point returns an int status (SEV_OK == 0) unless it mirrors
+ *     a reference native symbol with another signature; sev_last_error()
+ *     returns the message of the calling thread's last failure.
+ *   - "Host" entry points take caller-owned host buffers in Severo's Julia
+ *     layout (cells x genes SparseMatrixCSC{T,Int64}: Int64 colptr/rowval,
+ *     1-based; column-major dense matrices; 1-based neighbour indices) and
+ *     write results in place, exactly as the reference's ccall sites do.
+ *   - The sev_pipeline_* handle keeps a dataset resident in device memory so
+ *     the whole path runs without host round trips.  A handle is not
+ *     thread-safe; use one per thread.
+ *   - No entry point falls back to CPU code: without a usable gfx950 device
+ *     every call fails with SEV_ERR_NODEV.
+ */
+#ifndef SEVERO_HIP_H
+#define SEVERO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEV_OK 0
+#define SEV_ERR_ARG (-1)
+#define SEV_ERR_HIP (-2)
+#define SEV_ERR_CONVERGENCE (-3)
+#define SEV_ERR_NOMEM (-4)
+#define SEV_ERR_NODEV (-5)
+#define SEV_ERR_INTERNAL (-6)
+
+/* dtype codes */
+#define SEV_FLOAT32 0
+#define SEV_FLOAT64 1
+/* count element types */
+#define SEV_INT32 0
+#define SEV_INT64 1
+/* normalisation methods (src/normalize.jl:45-51) */
+#define SEV_LOGNORMALIZE 0
+#define SEV_RELATIVECOUNTS 1
+/* precision forms */
+#define SEV_PREC_FP32 0 /* plain form */
+#define SEV_PREC_BF16 1 /* BF16 values / BF16 MFMA candidates, exact re-rank */
+
+/* ---- library ----------------------------------------------------------- */
+const char *sev_last_error(void);
+int sev_version(void);
+/* Selects the device for the calling thread and checks it is usable. */
+int sev_device_init(int device);
+
+/* ---- drop-ins over host buffers --------------------------------------- */
+
+/* normalize_cells (src/normalize.jl:40-55, row_norm :17-32, log_norm :34-38).
+ * m cells x n genes; colptr[n+1], rowval[nnz] 1-based Int64; nzval is Int32
+ * (count_type SEV_INT32) or Int64 counts.  Per cell c: s_c = sum of its
+ * counts; out = sf * x / s_c in dtype, then log1p for SEV_LOGNORMALIZE.
+ * out_nzval has nnz elements of dtype (the structure is unchanged). */
+int sev_normalize_cells(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int count_type, int method, double scale_factor,
+                        int dtype, void *out_nzval);
+
+/* scale_features -> scale_data (src/scaling.jl:199-217, :335-357) with the
+ * Welford mean/var of src/scaling.jl:18-34 in dtype.  Writes the clipped
+ * scaled values (same structure) and mu' = mu / sigma per column (the
+ * CenteredMatrix mu).  scale_max may be +Inf. */
+int sev_scale_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                       const void *nzval, int dtype, double scale_max, void *out_nzval,
+                       void *out_mu);
+
+/* CenteredMatrix mul! (src/scaling.jl:245-257):
+ *   trans == 'N':  y = alpha*(A x - (mu.x) 1) + beta*y     (y length m)
+ *   trans == 'T':  y = alpha*(A' x - (sum x) mu) + beta*y  (y length n)
+ * A is m x n CSC with dtype values (1-based Int64 structure); mu may be NULL
+ * for a plain sparse product. */
+int sev_centered_mul(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                     const void *nzval, const void *mu, int dtype, char trans, const double *x,
+                     double *y, double alpha, double beta);
+
+/* IRLBA on the centered matrix held on the device (replaces the libcell
+ * irlba call of src/irlba.jl:66-71 for the CenteredMatrix operator of
+ * src/scaling.jl:219-257, without per-product callbacks).  B (dtype values,
+ * 1-based Int64 structure) and mu (dtype, length n, or NULL for an
+ * uncentered matrix) describe A = B - 1 mu'.  work = nu + 7 in the reference
+ * (src/irlba.jl:50); init has n entries (src/irlba.jl:63).  Outputs s[nu],
+ * U (m x nu), V (n x nu) column-major FP64.  precision: SEV_PREC_FP32 reads
+ * B in its dtype, SEV_PREC_BF16 as BF16 (Float32 input only); vectors and
+ * reorthogonalisation are FP64 either way.  Returns SEV_ERR_CONVERGENCE
+ * where the reference raises "convergence failed". */
+int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                           const void *B, const void *mu, int dtype, int64_t nu, int64_t work,
+                           int64_t maxit, double tol, const double *init, double *s, double *U,
+                           double *V, int precision, int64_t *iters, int64_t *nmult);
+
+/* Exact Euclidean kNN with the native libcell signature
+ * (src/neighbours.jl:32-39 and :54-61).  X is n x d column-major with
+ * leading dimension ld; nn_index (1-based) and distances are n x k
+ * column-major.  ntables and seed are accepted and ignored (the search is
+ * exact: D = sum_t (x_it - x_jt)^2 accumulated in t order, ties by lower
+ * index, distance sqrt(D)).  Errors are reported via sev_last_error(). */
+void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int ntables,
+                               int include_self, int seed, int32_t *nn_index, float *distances);
+void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int ntables,
+                               int include_self, int seed, int32_t *nn_index, double *distances);
+
+/* Same search with a status and a precision form.  precision SEV_PREC_FP32
+ * or SEV_PREC_BF16 (candidates on MFMA, then exact FP32 re-rank; the result
+ * equals the FP32 exact search).  *n_fallback (may be NULL) receives the
+ * number of query rows that needed the exhaustive exact search. */
+int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
+            int include_self, int precision, int32_t *nn_index, void *distances,
+            int64_t *n_fallback);
+
+/* Jaccard SNN graph (src/neighbours.jl:79, :88-93): from nn_index (n x k
+ * column-major, 1-based), S = nn'nn, w = x/(k + (k - x)) in dtype, entries
+ * with w <= prune dropped (droptol!).  Two calls: with colptr == NULL only
+ * *nnz is set; then colptr[n+1], rowval[nnz] (1-based Int64, rows sorted)
+ * and nzval[nnz] of dtype are filled. */
+int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,
+                    int64_t *nnz, int64_t *colptr, int64_t *rowval, void *nzval);
+
+/* ---- device-resident pipeline ----------------------------------------- */
+typedef struct sev_pipeline sev_pipeline;
+
+int sev_pipeline_create(sev_pipeline **out);
+void sev_pipeline_destroy(sev_pipeline *p);
+int sev_pipeline_synchronize(sev_pipeline *p);
+
+/* Load a cells x genes count matrix (1-based Int64 structure, Int32 counts). */
+int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_t *colptr,
+                             const int64_t *rowval, const int32_t *nzval);
+/* Generate the seeded negative-binomial matrix on the device: entry (g, c)
+ * is 0 when u < p0[2g+class(c)], else drawn by inverse CDF with
+ * q[2g+class(c)] and size r; class(c) = 1 for a boost_frac share of cells. */
+int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t seed, double r,
+                              const double *p0, const double *q, double boost_frac);
+int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *nnz);
+int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, int32_t *nzval);
+
+/* Stage 1: normalize_cells in Float32 (method SEV_LOGNORMALIZE or
+ * SEV_RELATIVECOUNTS). */
+int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor);
+int sev_pipeline_get_normalized(sev_pipeline *p, float *nzval);
+
+/* Stage 2: Y[:, features] then scale_features(scale_max) in Float32.
+ * features are 1-based gene indices in the order to keep. */
+int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *features,
+                              double scale_max);
+/* nnz of the selected matrix, then its 1-based structure, scaled values
+ * and mu' (length nfeat). */
+int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz);
+int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, float *nzval,
+                            float *mu);
+
+/* Stage 3: embedding(S, npcs; method=:pca, algorithm=:irlba)
+ * (src/embedding.jl:46-94): IRLBA with work = npcs + 7, then
+ * Z = U diag(s) kept on the device in Float32 for the neighbour search. */
+int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+                     const double *init, int precision, int64_t *iters, int64_t *nmult);
+/* s[npcs]; U (m x npcs) and V (nfeat x npcs) column-major FP64; Z (m x npcs
+ * column-major Float32).  Any pointer may be NULL. */
+int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float *Z);
+/* Replace the embedding with a caller-supplied m x d Float32 matrix
+ * (column-major, ld >= m), e.g. to search neighbours of given coordinates. */
+int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const float *Z, int64_t ld);
+
+/* Stage 4: exact kNN on the first ndims embedding dimensions (ndims <= 0:
+ * all). */
+int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                            int precision, int64_t *n_fallback);
+int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances);
+
+/* Stage 5: Jaccard SNN of the current neighbours (dtype of the weights). */
+int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz);
+int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void *nzval);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEVERO_HIP_H */

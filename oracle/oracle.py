@@ -1,0 +1,238 @@
+"""ctypes front end of the CPU oracle (``libsevero_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()``
+and bench.py's ``cpu_baseline`` leg, always as the checker.  The product
+library never calls into it.
+
+Every function takes and returns numpy arrays in the layouts Severo.jl uses
+(cells x genes CSC, column-major dense matrices, 1-based neighbour indices)
+except that CSC row indices are 0-based int32 here, as on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i64 = C.c_int64
+_i32 = C.c_int32
+_u64 = C.c_uint64
+_dbl = C.c_double
+_flt = C.c_float
+_p = C.c_void_p
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libsevero_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
+        L.so_u01.restype = _dbl
+        L.so_u01.argtypes = [_u64, C.c_uint32, C.c_uint32]
+        L.so_synth_nb_csc.restype = _i64
+        L.so_synth_nb_csc.argtypes = [_i64, _i64, _u64, _dbl, _p, _p, _dbl, _p, _p, _p]
+        L.so_normalize_f32.argtypes = [_i64, _i64, _p, _p, _p, _flt, C.c_int, _p]
+        L.so_normalize_f64.argtypes = [_i64, _i64, _p, _p, _p, _dbl, C.c_int, _p]
+        L.so_gather_cols.restype = _i64
+        L.so_gather_cols.argtypes = [_i64, _p, _p, _p, _p, _p, _p, _p]
+        L.so_scale_f32.argtypes = [_i64, _i64, _p, _p, _flt, _p, _p, _p]
+        L.so_scale_f64.argtypes = [_i64, _i64, _p, _p, _dbl, _p, _p, _p]
+        L.so_centered_mul.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, C.c_int, _p, _p,
+                                      _dbl, _dbl]
+        L.so_svd_jacobi.argtypes = [_i64, _p, _p, _p, _p]
+        L.so_irlba_csc.restype = C.c_int
+        L.so_irlba_csc.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, _i64, _i64, _i64, _dbl,
+                                   _dbl, _p, _p, _p, _p, _p, _p]
+        L.so_irlba_dense.restype = C.c_int
+        L.so_irlba_dense.argtypes = [_i64, _i64, _p, _p, _i64, _i64, _i64, _dbl, _dbl, _p, _p,
+                                     _p, _p, _p, _p]
+        for name in ("so_knn_f32", "so_knn_f64"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = [_p, _i64, _i64, _i64, _i64, C.c_int, _p, _i64, _p, _p, C.c_int]
+        L.so_snn_jaccard.restype = _i64
+        L.so_snn_jaccard.argtypes = [_i64, _i64, _p, _dbl, C.c_int, _p, _p, _p]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def u01(seed, stream, index):
+    return lib().so_u01(seed, stream, index)
+
+
+def synth_csc(n_cells, tables, seed=None):
+    """cells x genes CSC of NB counts: (colptr int64, rowval int32 0-based, counts int32)."""
+    seed = tables.seed if seed is None else seed
+    G = tables.n_genes
+    colptr = np.zeros(G + 1, np.int64)
+    L = lib()
+    nnz = L.so_synth_nb_csc(n_cells, G, seed, tables.r, _ptr(tables.p0), _ptr(tables.q),
+                            tables.boost_frac, _ptr(colptr), None, None)
+    rowval = np.empty(nnz, np.int32)
+    counts = np.empty(nnz, np.int32)
+    L.so_synth_nb_csc(n_cells, G, seed, tables.r, _ptr(tables.p0), _ptr(tables.q),
+                      tables.boost_frac, _ptr(colptr), _ptr(rowval), _ptr(counts))
+    return colptr, rowval, counts
+
+
+def normalize(m, colptr, rowval, counts, scale_factor=1e4, log=True, dtype=np.float32):
+    n = len(colptr) - 1
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowval = np.ascontiguousarray(rowval, np.int32)
+    counts = np.ascontiguousarray(counts, np.int32)
+    out = np.empty(len(rowval), dtype)
+    if dtype == np.float32:
+        lib().so_normalize_f32(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts), scale_factor,
+                               int(log), _ptr(out))
+    else:
+        lib().so_normalize_f64(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts), scale_factor,
+                               int(log), _ptr(out))
+    return out
+
+
+def gather(feats, colptr, rowval, vals):
+    feats = np.ascontiguousarray(feats, np.int64)
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowval = np.ascontiguousarray(rowval, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    out_ptr = np.zeros(len(feats) + 1, np.int64)
+    L = lib()
+    nnz = L.so_gather_cols(len(feats), _ptr(feats), _ptr(colptr), None, None, _ptr(out_ptr),
+                           None, None)
+    r = np.empty(nnz, np.int32)
+    v = np.empty(nnz, np.float32)
+    L.so_gather_cols(len(feats), _ptr(feats), _ptr(colptr), _ptr(rowval), _ptr(vals),
+                     _ptr(out_ptr), _ptr(r), _ptr(v))
+    return out_ptr, r, v
+
+
+def scale(m, colptr, vals, scale_max=np.inf):
+    """Returns (B values, mu/sigma, sigma) in the dtype of vals."""
+    n = len(colptr) - 1
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    vals = np.ascontiguousarray(vals)
+    out = np.empty_like(vals)
+    mu = np.empty(n, vals.dtype)
+    sd = np.empty(n, vals.dtype)
+    if vals.dtype == np.float32:
+        lib().so_scale_f32(m, n, _ptr(colptr), _ptr(vals), scale_max, _ptr(out), _ptr(mu),
+                           _ptr(sd))
+    else:
+        lib().so_scale_f64(m, n, _ptr(colptr), _ptr(vals), scale_max, _ptr(out), _ptr(mu),
+                           _ptr(sd))
+    return out, mu, sd
+
+
+def centered_mul(m, colptr, rowval, vals, mu, x, trans=False, y=None, alpha=1.0, beta=0.0):
+    n = len(colptr) - 1
+    vals = np.ascontiguousarray(vals)
+    f64 = int(vals.dtype == np.float64)
+    if mu is not None:
+        mu = np.ascontiguousarray(mu, vals.dtype)
+    x = np.ascontiguousarray(x, np.float64)
+    if y is None:
+        y = np.zeros(n if trans else m, np.float64)
+    else:
+        y = np.array(y, np.float64, copy=True)
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowval = np.ascontiguousarray(rowval, np.int32)
+    lib().so_centered_mul(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), f64, _ptr(mu),
+                          int(trans), _ptr(x), _ptr(y), alpha, beta)
+    return y
+
+
+def svd_jacobi(A):
+    A = np.asfortranarray(A, np.float64)
+    n = A.shape[0]
+    U = np.zeros((n, n), order="F")
+    V = np.zeros((n, n), order="F")
+    s = np.zeros(n)
+    lib().so_svd_jacobi(n, _ptr(A), _ptr(U), _ptr(s), _ptr(V))
+    return U, s, V
+
+
+def irlba_csc(m, colptr, rowval, vals, mu, nu, init, work=None, maxit=1000, tol=1e-5,
+              svtol=None):
+    n = len(colptr) - 1
+    work = nu + 7 if work is None else work
+    work = min(work, m, n)
+    svtol = tol if svtol is None else svtol
+    vals = np.ascontiguousarray(vals)
+    f64 = int(vals.dtype == np.float64)
+    if mu is not None:
+        mu = np.ascontiguousarray(mu, vals.dtype)
+    s = np.zeros(nu)
+    U = np.zeros((m, nu), order="F")
+    V = np.zeros((n, nu), order="F")
+    it = np.zeros(1, np.int64)
+    nm = np.zeros(1, np.int64)
+    init = np.ascontiguousarray(init, np.float64)
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowval = np.ascontiguousarray(rowval, np.int32)
+    rc = lib().so_irlba_csc(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), f64, _ptr(mu), nu, work, maxit, tol, svtol, _ptr(init), _ptr(s), _ptr(U),
+                            _ptr(V), _ptr(it), _ptr(nm))
+    return rc, s, U, V, int(it[0]), int(nm[0])
+
+
+def irlba_dense(A, nu, init, mu=None, work=None, maxit=1000, tol=1e-5, svtol=None):
+    A = np.asfortranarray(A, np.float64)
+    m, n = A.shape
+    work = nu + 7 if work is None else work
+    work = min(work, m, n)
+    svtol = tol if svtol is None else svtol
+    if mu is not None:
+        mu = np.ascontiguousarray(mu, np.float64)
+    s = np.zeros(nu)
+    U = np.zeros((m, nu), order="F")
+    V = np.zeros((n, nu), order="F")
+    it = np.zeros(1, np.int64)
+    nm = np.zeros(1, np.int64)
+    init = np.ascontiguousarray(init, np.float64)
+    rc = lib().so_irlba_dense(m, n, _ptr(A), _ptr(mu), nu, work, maxit, tol, svtol, _ptr(init),
+                              _ptr(s), _ptr(U), _ptr(V), _ptr(it), _ptr(nm))
+    return rc, s, U, V, int(it[0]), int(nm[0])
+
+
+def knn(X, k, include_self=True, qidx=None, nthreads=None):
+    """Exact kNN of the rows of X (n x d).  Returns (nn_index n x k 1-based
+    int32, distances n x k), both column-major like Severo's ann()."""
+    X = np.asfortranarray(X)
+    n, d = X.shape
+    nthreads = nthreads or os.cpu_count() or 1
+    nrow = n if qidx is None else len(qidx)
+    if qidx is not None:
+        qidx = np.ascontiguousarray(qidx, np.int64)
+    idx = np.zeros((nrow, k), np.int32, order="F")
+    dist = np.zeros((nrow, k), X.dtype, order="F")
+    f = lib().so_knn_f32 if X.dtype == np.float32 else lib().so_knn_f64
+    rc = f(_ptr(X), n, d, n, k, int(include_self), _ptr(qidx), nrow, _ptr(idx), _ptr(dist),
+           int(nthreads))
+    if rc != 0:
+        raise ValueError(f"oracle knn failed ({rc})")
+    return idx, dist
+
+
+def snn(nn_index, k, prune=1.0 / 15, dtype=np.float64):
+    """Jaccard SNN graph as CSC (colptr, rowval 1-based int64, nzval)."""
+    nn_index = np.asfortranarray(nn_index, np.int32)
+    n = nn_index.shape[0]
+    f64 = int(dtype == np.float64)
+    L = lib()
+    nnz = L.so_snn_jaccard(n, k, _ptr(nn_index), prune, f64, None, None, None)
+    if nnz < 0:
+        raise ValueError("bad neighbour index")
+    colptr = np.zeros(n + 1, np.int64)
+    rowval = np.zeros(nnz, np.int64)
+    nzval = np.zeros(nnz, dtype)
+    L.so_snn_jaccard(n, k, _ptr(nn_index), prune, f64, _ptr(colptr), _ptr(rowval), _ptr(nzval))
+    return colptr, rowval, nzval

@@ -1,0 +1,300 @@
+"""Severo.jl's embedding-and-graph API over the MI355X library.
+
+The functions mirror the Julia API of the reference (same names, argument
+meaning, defaults and error behaviour) with scipy CSC matrices standing in for
+``SparseMatrixCSC{T,Int64}`` (cells x genes, one CSC column per gene) and
+numpy arrays for dense matrices.  Every computation runs in libsevero_hip.so
+on the GPU; nothing here computes results on the CPU beyond re-wrapping them.
+
+Reference files: src/normalize.jl, src/scaling.jl, src/irlba.jl,
+src/embedding.jl, src/neighbours.jl.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import check
+
+
+def _csc_julia(X: sp.spmatrix):
+    """1-based Int64 colptr/rowval of a scipy matrix (Julia's layout)."""
+    X = sp.csc_matrix(X)
+    X.sort_indices()
+    colptr = np.ascontiguousarray(X.indptr, np.int64) + 1
+    rowval = np.ascontiguousarray(X.indices, np.int64) + 1
+    return X, colptr, rowval
+
+
+def _dtype_code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return _lib.SEV_FLOAT32
+    if dt == np.float64:
+        return _lib.SEV_FLOAT64
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# normalize_cells  (src/normalize.jl:40-55, :76-79)
+# ---------------------------------------------------------------------------
+def normalize_cells(X, method="lognormalize", scale_factor=1.0, dtype=np.float64):
+    """Divide each cell's counts by its total, multiply by ``scale_factor``
+    and (``:lognormalize``) apply log1p.  Returns a CSC matrix of ``dtype``
+    with the structure of ``X``."""
+    if isinstance(method, str):
+        method = method.lstrip(":")
+    codes = {"lognormalize": _lib.SEV_LOGNORMALIZE, "relativecounts": _lib.SEV_RELATIVECOUNTS}
+    if method not in codes:
+        raise ValueError(f"unknown normalization method: {method}")
+    X, colptr, rowval = _csc_julia(X)
+    if not np.issubdtype(X.dtype, np.integer):
+        raise TypeError("normalize_cells expects an integer count matrix")
+    counts = np.ascontiguousarray(X.data, np.int64)
+    out = np.empty(X.nnz, np.dtype(dtype))
+    m, n = X.shape
+    L = _lib.load()
+    check(L.sev_normalize_cells(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts), _lib.SEV_INT64,
+                                codes[method], float(scale_factor), _dtype_code(dtype), _ptr(out)))
+    return sp.csc_matrix((out, X.indices.copy(), X.indptr.copy()), shape=X.shape)
+
+
+# ---------------------------------------------------------------------------
+# CenteredMatrix / scale_features  (src/scaling.jl:219-357)
+# ---------------------------------------------------------------------------
+class CenteredMatrix:
+    """Lazy ``A - 1 mu'`` (src/scaling.jl:219-233): never densified."""
+
+    def __init__(self, A, mu):
+        A = sp.csc_matrix(A)
+        mu = np.asarray(mu)
+        if A.shape[1] != mu.shape[0]:
+            raise ValueError("length(mu) must equal size(A, 2)")
+        self.A = A
+        self.mu = mu
+
+    @property
+    def shape(self):
+        return self.A.shape
+
+    @property
+    def dtype(self):
+        return np.promote_types(self.A.dtype, self.mu.dtype)
+
+    def mul(self, y, x, alpha=1.0, beta=0.0, trans=False):
+        """mul!(y, C, x, alpha, beta) (src/scaling.jl:245-250) or, with
+        ``trans``, mul!(y, C', x, alpha, beta) (:252-257).  FP64 vectors."""
+        A, colptr, rowval = _csc_julia(self.A)
+        dt = np.dtype(self.dtype)
+        vals = np.ascontiguousarray(A.data, dt)
+        mu = np.ascontiguousarray(self.mu, dt)
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        m, n = A.shape
+        if x.shape[0] != (m if trans else n) or y.shape[0] != (n if trans else m):
+            raise ValueError("dimension mismatch")
+        check(_lib.load().sev_centered_mul(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals),
+                                           _ptr(mu), _dtype_code(dt), b"T" if trans else b"N",
+                                           _ptr(x), _ptr(y), float(alpha), float(beta)))
+        return y
+
+    def __matmul__(self, x):
+        return self.mul(np.zeros(self.shape[0]), x)
+
+    def rmatvec(self, x):
+        return self.mul(np.zeros(self.shape[1]), x, trans=True)
+
+    def to_dense(self):
+        """convert(Matrix, C) (src/scaling.jl:298-304)."""
+        return self.A.toarray() - self.mu[None, :]
+
+
+def scale_features(X, scale_max=np.inf, dtype=None, features=None):
+    """Standardise each feature (column) with the sequential Welford
+    mean/variance of src/scaling.jl:18-34, divide by sigma and clip at
+    ``scale_max + mu/sigma``; returns the lazy CenteredMatrix(B, mu/sigma).
+    ``features`` are 0-based column indices (Julia's are 1-based)."""
+    X = sp.csc_matrix(X)
+    if features is not None:
+        X = X[:, np.asarray(features)]
+    if dtype is None:
+        dtype = X.dtype if np.issubdtype(X.dtype, np.floating) else np.float64
+    X, colptr, rowval = _csc_julia(X)
+    vals = np.ascontiguousarray(X.data, np.dtype(dtype))
+    out = np.empty_like(vals)
+    mu = np.empty(X.shape[1], np.dtype(dtype))
+    m, n = X.shape
+    check(_lib.load().sev_scale_features(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals),
+                                         _dtype_code(dtype), float(scale_max), _ptr(out),
+                                         _ptr(mu)))
+    B = sp.csc_matrix((out, X.indices.copy(), X.indptr.copy()), shape=X.shape)
+    return CenteredMatrix(B, mu)
+
+
+# ---------------------------------------------------------------------------
+# irlba / pca / embedding  (src/irlba.jl:47-99, src/embedding.jl:46-94)
+# ---------------------------------------------------------------------------
+@dataclass
+class SVD:
+    U: np.ndarray
+    S: np.ndarray
+    V: np.ndarray
+    iters: int = 0
+    nmult: int = 0
+
+
+def irlba(A, nu: int, tol=1e-5, maxit=1000, init=None, rng=None, precision="fp32",
+          work=None) -> SVD:
+    """Truncated SVD by augmented restarted Lanczos bidiagonalisation on the
+    GPU (src/irlba.jl:47-85).  ``A`` is a CenteredMatrix or any sparse/dense
+    matrix; work defaults to nu + 7 (src/irlba.jl:50); ``init`` defaults to a
+    standard-normal vector (src/irlba.jl:63).  Raises ConvergenceError like
+    the reference's ``error("convergence failed")``."""
+    mu = None
+    if isinstance(A, CenteredMatrix):
+        mu, A = A.mu, A.A
+    Am = sp.csc_matrix(A)
+    dt = np.float32 if (Am.dtype == np.float32 and (mu is None or mu.dtype == np.float32)) \
+        else np.float64
+    Am, colptr, rowval = _csc_julia(Am)
+    m, n = Am.shape
+    if init is None:
+        rng = np.random.default_rng() if rng is None else rng
+        init = rng.standard_normal(n)
+    init = np.ascontiguousarray(init, np.float64)
+    vals = np.ascontiguousarray(Am.data, dt)
+    mu = None if mu is None else np.ascontiguousarray(mu, dt)
+    s = np.zeros(nu)
+    U = np.zeros((m, nu), order="F")
+    V = np.zeros((n, nu), order="F")
+    it = C.c_int64(0)
+    nm = C.c_int64(0)
+    prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16}[precision]
+    work = nu + 7 if work is None else work
+    check(_lib.load().sev_irlba_centered_csc(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals),
+                                             _ptr(mu), _dtype_code(dt), nu, work, maxit,
+                                             float(tol), _ptr(init), _ptr(s), _ptr(U), _ptr(V),
+                                             prec, C.byref(it), C.byref(nm)))
+    return SVD(U, s, V, it.value, nm.value)
+
+
+@dataclass
+class LinearEmbedding:
+    """src/embedding.jl:20-25."""
+    coordinates: np.ndarray  # C x npcs FP64 (U * diag(S))
+    stdev: np.ndarray
+    basis: np.ndarray  # F x npcs (loadings)
+
+
+def pca(X: CenteredMatrix, npcs: int = 50, tol=1e-5, maxit=1000, init=None, rng=None,
+        precision="fp32") -> LinearEmbedding:
+    """_pca with algorithm=:irlba (src/embedding.jl:46-76)."""
+    m, n = X.shape
+    npcs = min(min(m, n), npcs)
+    S = irlba(X, npcs, tol=tol, maxit=maxit, init=init, rng=rng, precision=precision)
+    Z = S.U[:, :npcs] * S.S[:npcs][None, :]
+    stdev = S.S[:npcs] / np.sqrt(max(1, m - 1))
+    return LinearEmbedding(Z, stdev, S.V[:, :npcs])
+
+
+def embedding(X, ncomponents: int = 50, method="pca", **kw) -> LinearEmbedding:
+    """src/embedding.jl:202-212."""
+    if isinstance(method, str):
+        method = method.lstrip(":")
+    if method != "pca":
+        raise ValueError(f"unknown reduction method: {method}")
+    kw.pop("algorithm", None)
+    return pca(X, ncomponents, **kw)
+
+
+# ---------------------------------------------------------------------------
+# ann / nearest_neighbours / jaccard_index / shared_nearest_neighbours
+# (src/neighbours.jl:19-108, 127-349)
+# ---------------------------------------------------------------------------
+def _coords(X, dims):
+    if isinstance(X, LinearEmbedding):
+        X = X.coordinates
+    X = np.asarray(X)
+    if dims is not None:
+        X = X[:, dims]
+    return X
+
+
+def ann(X, k: int, include_self=True, ntables=None, precision="fp32"):
+    """Exact Euclidean kNN (the search behind ann!/FindNeighboursEuclidean,
+    src/neighbours.jl:19-63).  Returns (nn_index n x k 1-based Int32,
+    distances n x k), column-major like the reference."""
+    X = np.asfortranarray(X)
+    if X.dtype not in (np.float32, np.float64):
+        X = X.astype(np.float64, order="F")
+    n, d = X.shape
+    idx = np.zeros((n, k), np.int32, order="F")
+    dist = np.zeros((n, k), X.dtype, order="F")
+    nfb = C.c_int64(0)
+    prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16}[precision]
+    check(_lib.load().sev_knn(_ptr(X), n, d, n, _dtype_code(X.dtype), k, int(include_self), prec,
+                              _ptr(idx), _ptr(dist), C.byref(nfb)))
+    return idx, dist
+
+
+def _nn_matrix(nn_index):
+    """sparse(vec(nn_index'), repeat(1:n, inner=k), true) (src/neighbours.jl:79):
+    column j holds N(j)."""
+    n, k = nn_index.shape
+    rows = np.asarray(nn_index, np.int64).ravel(order="C") - 1
+    cols = np.repeat(np.arange(n), k)
+    M = sp.csc_matrix((np.ones(n * k, bool), (rows, cols)), shape=(n, n))
+    M.sort_indices()
+    return M
+
+
+def nearest_neighbours(X, k: int, dims=None, include_self=True, ntables=None):
+    """k-NN indicator graph, column j holding N(j) (src/neighbours.jl:76-86)."""
+    idx, _ = ann(_coords(X, dims), k, include_self=include_self)
+    return _nn_matrix(idx)
+
+
+def _snn_from_index(nn_index, k, prune, dtype):
+    n = nn_index.shape[0]
+    nn_index = np.asfortranarray(nn_index, np.int32)
+    nnz = C.c_int64(0)
+    L = _lib.load()
+    dt = _dtype_code(dtype)
+    check(L.sev_snn_jaccard(n, k, _ptr(nn_index), float(prune), dt, C.byref(nnz), None, None,
+                            None))
+    colptr = np.zeros(n + 1, np.int64)
+    rowval = np.zeros(nnz.value, np.int64)
+    vals = np.zeros(nnz.value, np.dtype(dtype))
+    check(L.sev_snn_jaccard(n, k, _ptr(nn_index), float(prune), dt, C.byref(nnz), _ptr(colptr),
+                            _ptr(rowval), _ptr(vals)))
+    return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(n, n))
+
+
+def jaccard_index(nn, prune=1.0 / 15, dtype=np.float64):
+    """Jaccard graph of a k-NN indicator matrix (src/neighbours.jl:96-108,
+    :127-131); k is taken from the column counts, which must all be equal."""
+    nn = sp.csc_matrix(nn)
+    counts = np.diff(nn.indptr)
+    if len(counts) == 0 or not np.all(counts == counts[0]):
+        raise ValueError("jaccard_index: every column of nn must hold k neighbours")
+    k = int(counts[0])
+    n = nn.shape[1]
+    nn.sort_indices()
+    idx = (nn.indices.reshape(n, k) + 1).astype(np.int32)
+    return _snn_from_index(np.asfortranarray(idx), k, prune, dtype)
+
+
+def shared_nearest_neighbours(X, k: int, dims=None, include_self=True, ntables=None,
+                              prune=1.0 / 15):
+    """kNN then Jaccard graph with weights in eltype(X) (src/neighbours.jl:272-278)."""
+    Xc = _coords(X, dims)
+    idx, _ = ann(Xc, k, include_self=include_self)
+    return _snn_from_index(idx, k, prune, Xc.dtype)

@@ -1,0 +1,836 @@
+// C ABI of libsevero_hip.so (declared in include/severo_hip.h).
+//
+// Host entry points convert Severo's Julia layout (1-based Int64 CSC,
+// column-major dense, 1-based neighbour indices) to the device layout
+// (0-based int64 colptr, int32 rows, row-major neighbour lists), run the
+// device kernels and convert back.  The sev_pipeline handle keeps every
+// stage resident on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/severo_hip.h"
+#include "sev_kernels.hpp"
+
+using namespace sev;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return SEV_OK;
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return SEV_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return SEV_ERR_INTERNAL;
+    } catch (...) {
+        g_err = "unknown error";
+        return SEV_ERR_INTERNAL;
+    }
+}
+
+// The library's code objects are built for gfx950 only; refuse anything else
+// (and a machine without a device) instead of falling back.
+void ensure_device() {
+    thread_local bool ok = false;
+    if (ok) return;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) fail(kErrNoDevice, "no HIP device available");
+    int dev = 0;
+    SEV_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SEV_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        fail(kErrNoDevice, std::string("libsevero_hip is built for gfx950, device is ") +
+                               prop.gcnArchName);
+    ok = true;
+}
+
+// Julia 1-based Int64 CSC structure -> device 0-based int64 colptr and int32
+// rows (validated).
+void csc_to_device(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                   DevBuf<int64_t> &dcp, DevBuf<int32_t> &drow, int64_t &nnz, hipStream_t st) {
+    SEV_REQUIRE(m >= 0 && n >= 0, "negative dimension");
+    SEV_REQUIRE(m < (int64_t)std::numeric_limits<int32_t>::max(), "too many rows for int32 rows");
+    SEV_REQUIRE(colptr != nullptr, "colptr is NULL");
+    SEV_REQUIRE(colptr[0] == 1, "colptr must be 1-based (colptr[0] == 1)");
+    std::vector<int64_t> cp((size_t)n + 1);
+    for (int64_t j = 0; j <= n; ++j) {
+        cp[(size_t)j] = colptr[j] - 1;
+        if (j > 0) SEV_REQUIRE(cp[(size_t)j] >= cp[(size_t)j - 1], "colptr not nondecreasing");
+    }
+    nnz = cp[(size_t)n];
+    std::vector<int32_t> rv((size_t)nnz);
+    for (int64_t p = 0; p < nnz; ++p) {
+        const int64_t r = rowval[p] - 1;
+        SEV_REQUIRE(r >= 0 && r < m, "row index out of range");
+        rv[(size_t)p] = (int32_t)r;
+    }
+    dcp.ensure((size_t)n + 1);
+    drow.ensure((size_t)nnz);
+    SEV_HIP(hipMemcpyAsync(dcp.get(), cp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
+                           st));
+    if (nnz)
+        SEV_HIP(hipMemcpyAsync(drow.get(), rv.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice,
+                               st));
+    SEV_HIP(hipStreamSynchronize(st));
+}
+
+struct Stream {
+    hipStream_t st = nullptr;
+    Stream() { SEV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); }
+    ~Stream() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+template <class T>
+void upload(DevBuf<T> &b, const T *h, size_t n, hipStream_t st) {
+    b.ensure(n);
+    if (n) SEV_HIP(hipMemcpyAsync(b.get(), h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+}
+
+template <class T>
+void download(T *h, const T *d, size_t n, hipStream_t st) {
+    if (n) SEV_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+}
+
+// xmin and weight table for the Jaccard graph: w[x] = x/(k + (k - x)) in T,
+// keep iff |w| > T(prune) (droptol!).
+template <class T>
+int snn_table(int64_t k, double prune, std::vector<T> &w) {
+    w.resize((size_t)k + 1);
+    const T kk = (T)k, pr = (T)prune;
+    int xmin = (int)k + 1;
+    for (int64_t x = 0; x <= k; ++x) {
+        const T xf = (T)x;
+        w[(size_t)x] = xf / (kk + (kk - xf));
+        if (std::fabs(w[(size_t)x]) > pr && xmin > (int)x) xmin = (int)x;
+    }
+    // w is increasing in x, so keep iff x >= xmin; a count of 0 never occurs
+    if (xmin < 1) xmin = 1;
+    return xmin;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Pipeline handle
+// ---------------------------------------------------------------------------
+struct sev_pipeline {
+    Stream stream;
+    // counts (cells x genes CSC)
+    int64_t m = 0, g = 0, nnz = 0;
+    DevBuf<int64_t> cp;
+    DevBuf<int32_t> row, cnt;
+    bool have_counts = false;
+    // normalised values
+    DevBuf<uint32_t> tot;
+    DevBuf<float> norm;
+    bool have_norm = false;
+    // selection + scaling
+    int64_t F = 0, fnnz = 0;
+    DevBuf<int64_t> feat, fcp, flen;
+    DevBuf<int32_t> frow;
+    DevBuf<float> fval, bval, mu, sd, clip;
+    bool have_scaled = false;
+    // PCA
+    SlabCsc op;
+    IrlbaWork iw;
+    int64_t npcs = 0;
+    std::vector<double> s;
+    DevBuf<double> U, V, s_d;
+    bool have_pca = false;
+    // embedding (column-major m x zdim Float32)
+    int64_t zdim = 0;
+    DevBuf<float> Z;
+    bool have_embedding = false;
+    // neighbours (row-major m x k, 0-based)
+    int64_t k = 0;
+    DevBuf<int32_t> nn;
+    DevBuf<float> dist;
+    bool have_nn = false;
+    // SNN
+    SnnWork sw;
+    int64_t snn_nnz = 0;
+    int snn_dtype = SEV_FLOAT32;
+    DevBuf<int64_t> scp;
+    DevBuf<int32_t> srow;
+    DevBuf<float> sval32, wtab32;
+    DevBuf<double> sval64, wtab64;
+    bool have_snn = false;
+    // scratch
+    DevBuf<int64_t> scan_scratch, chunk_cnt, chunk_off;
+    DevBuf<uint8_t> cls;
+    DevBuf<double> p0, q;
+
+    hipStream_t st() const { return stream.st; }
+
+    void invalidate_from_counts() {
+        have_norm = have_scaled = have_pca = have_embedding = have_nn = have_snn = false;
+    }
+};
+
+namespace {
+template <class T>
+void scale_host(int64_t m, int64_t n, const DevBuf<int64_t> &cp, int64_t nnz, const T *nzval,
+                T scale_max, T *out, T *mu, hipStream_t st) {
+    DevBuf<T> x, b, dmu, dsd, dclip;
+    upload(x, nzval, (size_t)nnz, st);
+    b.ensure((size_t)nnz);
+    dmu.ensure((size_t)n);
+    dsd.ensure((size_t)n);
+    dclip.ensure((size_t)n);
+    scale_stats<T>(m, n, cp.get(), x.get(), scale_max, dsd.get(), dmu.get(), dclip.get(), st);
+    scale_map<T>(n, cp.get(), x.get(), dsd.get(), dclip.get(), b.get(), st);
+    download(out, b.get(), (size_t)nnz, st);
+    download(mu, dmu.get(), (size_t)n, st);
+    SEV_HIP(hipStreamSynchronize(st));
+}
+}  // namespace
+
+namespace {
+template <class T>
+void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+              int32_t *nn_index, T *distances) {
+    SEV_REQUIRE(n >= 1 && d >= 1 && ld >= n, "bad X shape");
+    SEV_REQUIRE(k >= 1 && k <= (include_self ? n : n - 1), "k larger than the number of points");
+    Stream s;
+    DevBuf<T> dx;
+    dx.ensure((size_t)(n * d));
+    SEV_HIP(hipMemcpy2DAsync(dx.get(), sizeof(T) * n, X, sizeof(T) * ld, sizeof(T) * n, d,
+                             hipMemcpyHostToDevice, s.st));
+    DevBuf<int32_t> idx;
+    DevBuf<T> dist;
+    idx.ensure((size_t)(n * k));
+    dist.ensure((size_t)(n * k));
+    knn_exact<T>(dx.get(), n, d, n, k, include_self, nullptr, n, idx.get(), dist.get(), s.st);
+    std::vector<int32_t> hi((size_t)(n * k));
+    std::vector<T> hd((size_t)(n * k));
+    download(hi.data(), idx.get(), hi.size(), s.st);
+    download(hd.data(), dist.get(), hd.size(), s.st);
+    SEV_HIP(hipStreamSynchronize(s.st));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t t = 0; t < k; ++t) {
+            nn_index[i + t * n] = hi[(size_t)(i * k + t)] + 1;
+            distances[i + t * n] = hd[(size_t)(i * k + t)];
+        }
+}
+}  // namespace
+
+extern "C" {
+
+const char *sev_last_error(void) { return g_err.c_str(); }
+
+int sev_version(void) { return 1; }
+
+int sev_device_init(int device) {
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            fail(kErrNoDevice, "no HIP device available");
+        SEV_REQUIRE(device >= 0 && device < count, "device index out of range");
+        SEV_HIP(hipSetDevice(device));
+        ensure_device();
+    });
+}
+
+int sev_pipeline_create(sev_pipeline **out) {
+    return guarded([&] {
+        SEV_REQUIRE(out != nullptr, "out is NULL");
+        ensure_device();
+        *out = new sev_pipeline();
+    });
+}
+
+void sev_pipeline_destroy(sev_pipeline *p) { delete p; }
+
+int sev_pipeline_synchronize(sev_pipeline *p) {
+    return guarded([&] { SEV_HIP(hipStreamSynchronize(p->st())); });
+}
+
+int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_t *colptr,
+                             const int64_t *rowval, const int32_t *nzval) {
+    return guarded([&] {
+        hipStream_t st = p->st();
+        csc_to_device(m, n, colptr, rowval, p->cp, p->row, p->nnz, st);
+        upload(p->cnt, nzval, (size_t)p->nnz, st);
+        SEV_HIP(hipStreamSynchronize(st));
+        p->m = m;
+        p->g = n;
+        p->have_counts = true;
+        p->invalidate_from_counts();
+    });
+}
+
+int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t seed, double r,
+                              const double *p0, const double *q, double boost_frac) {
+    return guarded([&] {
+        SEV_REQUIRE(m > 0 && n > 0 && m < (int64_t)0x7FFFFFFF && n < (int64_t)0x7FFFFFFF,
+                    "bad synthetic shape");
+        hipStream_t st = p->st();
+        upload(p->p0, p0, (size_t)(2 * n), st);
+        upload(p->q, q, (size_t)(2 * n), st);
+        p->cls.ensure((size_t)m);
+        SynthPlan plan;
+        const int64_t nchunk = ceil_div(m, 16384);
+        p->chunk_cnt.ensure((size_t)(n * nchunk));
+        p->chunk_off.ensure((size_t)(n * nchunk + 1));
+        p->scan_scratch.ensure(scan_scratch_elems(n * nchunk));
+        p->cp.ensure((size_t)n + 1);
+        const int64_t nnz = synth_count(m, n, seed, r, p->p0.get(), p->q.get(), boost_frac,
+                                        p->cls.get(), p->chunk_cnt.get(), p->chunk_off.get(),
+                                        p->cp.get(), p->scan_scratch.get(), st, &plan);
+        p->row.ensure((size_t)nnz);
+        p->cnt.ensure((size_t)nnz);
+        synth_fill(plan, seed, r, p->p0.get(), p->q.get(), p->cls.get(), p->chunk_off.get(),
+                   p->row.get(), p->cnt.get(), st);
+        SEV_HIP(hipStreamSynchronize(st));
+        p->m = m;
+        p->g = n;
+        p->nnz = nnz;
+        p->have_counts = true;
+        p->invalidate_from_counts();
+    });
+}
+
+int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *nnz) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        if (m) *m = p->m;
+        if (n) *n = p->g;
+        if (nnz) *nnz = p->nnz;
+    });
+}
+
+int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, int32_t *nzval) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        hipStream_t st = p->st();
+        std::vector<int32_t> rv((size_t)p->nnz);
+        if (colptr) download(colptr, p->cp.get(), (size_t)p->g + 1, st);
+        if (rowval) download(rv.data(), p->row.get(), (size_t)p->nnz, st);
+        if (nzval) download(nzval, p->cnt.get(), (size_t)p->nnz, st);
+        SEV_HIP(hipStreamSynchronize(st));
+        if (colptr)
+            for (int64_t j = 0; j <= p->g; ++j) colptr[j] += 1;
+        if (rowval)
+            for (int64_t i = 0; i < p->nnz; ++i) rowval[i] = (int64_t)rv[(size_t)i] + 1;
+    });
+}
+
+int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        SEV_REQUIRE(method == SEV_LOGNORMALIZE || method == SEV_RELATIVECOUNTS,
+                    "unknown normalization method");
+        hipStream_t st = p->st();
+        p->tot.ensure((size_t)p->m);
+        p->norm.ensure((size_t)p->nnz);
+        row_totals(p->row.get(), p->cnt.get(), p->nnz, p->m, p->tot.get(), st);
+        lognorm_f32(p->row.get(), p->cnt.get(), p->tot.get(), p->nnz, (float)scale_factor,
+                    method == SEV_LOGNORMALIZE, p->norm.get(), st);
+        p->have_norm = true;
+        p->have_scaled = p->have_pca = p->have_embedding = p->have_nn = p->have_snn = false;
+    });
+}
+
+int sev_pipeline_get_normalized(sev_pipeline *p, float *nzval) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_norm, "normalize has not run");
+        download(nzval, p->norm.get(), (size_t)p->nnz, p->st());
+        SEV_HIP(hipStreamSynchronize(p->st()));
+    });
+}
+
+int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *features,
+                              double scale_max) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_norm, "normalize has not run");
+        SEV_REQUIRE(nfeat >= 1, "need at least one feature");
+        hipStream_t st = p->st();
+        std::vector<int64_t> f((size_t)nfeat);
+        for (int64_t j = 0; j < nfeat; ++j) {
+            SEV_REQUIRE(features[j] >= 1 && features[j] <= p->g, "feature index out of range");
+            f[(size_t)j] = features[j] - 1;
+        }
+        upload(p->feat, f.data(), (size_t)nfeat, st);
+        p->flen.ensure((size_t)nfeat + 1);
+        p->fcp.ensure((size_t)nfeat + 1);
+        p->scan_scratch.ensure(scan_scratch_elems(nfeat));
+        gather_lengths(p->feat.get(), p->cp.get(), nfeat, p->flen.get(), st);
+        scan_exclusive_i64(p->flen.get(), p->fcp.get(), nfeat, p->scan_scratch.get(), st);
+        int64_t fnnz = 0;
+        SEV_HIP(hipMemcpyAsync(&fnnz, p->fcp.get() + nfeat, sizeof fnnz, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        p->frow.ensure((size_t)fnnz);
+        p->fval.ensure((size_t)fnnz);
+        p->bval.ensure((size_t)fnnz);
+        p->mu.ensure((size_t)nfeat);
+        p->sd.ensure((size_t)nfeat);
+        p->clip.ensure((size_t)nfeat);
+        gather_copy<float>(p->feat.get(), p->cp.get(), p->row.get(), p->norm.get(), nfeat,
+                           p->fcp.get(), p->frow.get(), p->fval.get(), st);
+        scale_stats<float>(p->m, nfeat, p->fcp.get(), p->fval.get(), (float)scale_max, p->sd.get(),
+                           p->mu.get(), p->clip.get(), st);
+        scale_map<float>(nfeat, p->fcp.get(), p->fval.get(), p->sd.get(), p->clip.get(),
+                         p->bval.get(), st);
+        p->F = nfeat;
+        p->fnnz = fnnz;
+        p->have_scaled = true;
+        p->have_pca = p->have_embedding = p->have_nn = p->have_snn = false;
+    });
+}
+
+int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_scaled, "scale has not run");
+        *nnz = p->fnnz;
+    });
+}
+
+int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, float *nzval,
+                            float *mu) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_scaled, "scale has not run");
+        hipStream_t st = p->st();
+        std::vector<int32_t> rv((size_t)p->fnnz);
+        if (colptr) download(colptr, p->fcp.get(), (size_t)p->F + 1, st);
+        if (rowval) download(rv.data(), p->frow.get(), (size_t)p->fnnz, st);
+        if (nzval) download(nzval, p->bval.get(), (size_t)p->fnnz, st);
+        if (mu) download(mu, p->mu.get(), (size_t)p->F, st);
+        SEV_HIP(hipStreamSynchronize(st));
+        if (colptr)
+            for (int64_t j = 0; j <= p->F; ++j) colptr[j] += 1;
+        if (rowval)
+            for (int64_t i = 0; i < p->fnnz; ++i) rowval[i] = (int64_t)rv[(size_t)i] + 1;
+    });
+}
+
+int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, const double *init,
+                     int precision, int64_t *iters, int64_t *nmult) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_scaled, "scale has not run");
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        hipStream_t st = p->st();
+        const int64_t m = p->m, n = p->F;
+        const int64_t nu = std::min(std::min(m, n), npcs);
+        SEV_REQUIRE(nu >= 1, "npcs must be >= 1");
+        SEV_REQUIRE(init != nullptr, "init is NULL");
+        build_slab_csc(p->op, m, n, p->fcp.get(), p->frow.get(), p->bval.get(), false,
+                       p->mu.get(), precision == SEV_PREC_BF16, st);
+        IrlbaParams prm;
+        prm.nu = nu;
+        prm.work = nu + 7;
+        prm.maxit = maxit;
+        prm.tol = tol;
+        prm.svtol = tol;
+        p->s.assign((size_t)nu, 0.0);
+        p->U.ensure((size_t)(m * nu));
+        p->V.ensure((size_t)(n * nu));
+        IrlbaOut out;
+        irlba_device(p->op, prm, init, p->iw, p->s.data(), p->U.get(), p->V.get(), st, &out);
+        if (iters) *iters = out.iters;
+        if (nmult) *nmult = out.nmult;
+        p->npcs = nu;
+        upload(p->s_d, p->s.data(), (size_t)nu, st);
+        p->Z.ensure((size_t)(m * nu));
+        embedding_f32(p->U.get(), p->s_d.get(), m, nu, p->Z.get(), 1, m, st);
+        p->zdim = nu;
+        p->have_pca = p->have_embedding = true;
+        p->have_nn = p->have_snn = false;
+    });
+}
+
+int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float *Z) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_pca, "pca has not run");
+        hipStream_t st = p->st();
+        if (s) std::memcpy(s, p->s.data(), sizeof(double) * (size_t)p->npcs);
+        if (U) download(U, p->U.get(), (size_t)(p->m * p->npcs), st);
+        if (V) download(V, p->V.get(), (size_t)(p->F * p->npcs), st);
+        if (Z) download(Z, p->Z.get(), (size_t)(p->m * p->npcs), st);
+        SEV_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const float *Z, int64_t ld) {
+    return guarded([&] {
+        SEV_REQUIRE(m >= 1 && d >= 1 && ld >= m, "bad embedding shape");
+        hipStream_t st = p->st();
+        p->Z.ensure((size_t)(m * d));
+        SEV_HIP(hipMemcpy2DAsync(p->Z.get(), sizeof(float) * m, Z, sizeof(float) * ld,
+                                 sizeof(float) * m, d, hipMemcpyHostToDevice, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        p->m = m;
+        p->zdim = d;
+        p->have_embedding = true;
+        p->have_nn = p->have_snn = false;
+    });
+}
+
+int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                            int precision, int64_t *n_fallback) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_embedding, "no embedding");
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        const int64_t m = p->m;
+        const int64_t d = (ndims <= 0 || ndims > p->zdim) ? p->zdim : ndims;
+        SEV_REQUIRE(k >= 1 && k <= (include_self ? m : m - 1), "k larger than the number of cells");
+        hipStream_t st = p->st();
+        p->nn.ensure((size_t)(m * k));
+        p->dist.ensure((size_t)(m * k));
+        knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
+                         p->dist.get(), st);
+        if (n_fallback) *n_fallback = m;
+        p->k = k;
+        p->have_nn = true;
+        p->have_snn = false;
+    });
+}
+
+int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_nn, "neighbours have not run");
+        hipStream_t st = p->st();
+        const int64_t m = p->m, k = p->k;
+        std::vector<int32_t> idx((size_t)(m * k));
+        std::vector<float> dd((size_t)(m * k));
+        download(idx.data(), p->nn.get(), idx.size(), st);
+        download(dd.data(), p->dist.get(), dd.size(), st);
+        SEV_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t t = 0; t < k; ++t) {
+                if (nn_index) nn_index[i + t * m] = idx[(size_t)(i * k + t)] + 1;
+                if (distances) distances[i + t * m] = dd[(size_t)(i * k + t)];
+            }
+    });
+}
+
+int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_nn, "neighbours have not run");
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        hipStream_t st = p->st();
+        const int64_t m = p->m, k = p->k;
+        p->scp.ensure((size_t)m + 1);
+        int64_t total;
+        if (dtype == SEV_FLOAT32) {
+            std::vector<float> w;
+            const int xmin = snn_table<float>(k, prune, w);
+            upload(p->wtab32, w.data(), w.size(), st);
+            total = snn_count<float>(p->nn.get(), m, k, xmin, p->sw, p->scp.get(), st);
+            p->srow.ensure((size_t)total);
+            p->sval32.ensure((size_t)total);
+            snn_fill<float>(p->nn.get(), m, k, xmin, p->wtab32.get(), p->sw, p->scp.get(),
+                            p->srow.get(), p->sval32.get(), st);
+        } else {
+            std::vector<double> w;
+            const int xmin = snn_table<double>(k, prune, w);
+            upload(p->wtab64, w.data(), w.size(), st);
+            total = snn_count<double>(p->nn.get(), m, k, xmin, p->sw, p->scp.get(), st);
+            p->srow.ensure((size_t)total);
+            p->sval64.ensure((size_t)total);
+            snn_fill<double>(p->nn.get(), m, k, xmin, p->wtab64.get(), p->sw, p->scp.get(),
+                             p->srow.get(), p->sval64.get(), st);
+        }
+        p->snn_nnz = total;
+        p->snn_dtype = dtype;
+        p->have_snn = true;
+        if (nnz) *nnz = total;
+    });
+}
+
+int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void *nzval) {
+    return guarded([&] {
+        SEV_REQUIRE(p->have_snn, "snn has not run");
+        hipStream_t st = p->st();
+        const int64_t m = p->m, nnz = p->snn_nnz;
+        std::vector<int32_t> rv((size_t)nnz);
+        if (colptr) download(colptr, p->scp.get(), (size_t)m + 1, st);
+        if (rowval) download(rv.data(), p->srow.get(), (size_t)nnz, st);
+        if (nzval) {
+            if (p->snn_dtype == SEV_FLOAT32)
+                download((float *)nzval, p->sval32.get(), (size_t)nnz, st);
+            else
+                download((double *)nzval, p->sval64.get(), (size_t)nnz, st);
+        }
+        SEV_HIP(hipStreamSynchronize(st));
+        if (colptr)
+            for (int64_t j = 0; j <= m; ++j) colptr[j] += 1;
+        if (rowval)
+            for (int64_t i = 0; i < nnz; ++i) rowval[i] = (int64_t)rv[(size_t)i] + 1;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Drop-ins over host buffers
+// ---------------------------------------------------------------------------
+
+int sev_normalize_cells(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int count_type, int method, double scale_factor,
+                        int dtype, void *out_nzval) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(method == SEV_LOGNORMALIZE || method == SEV_RELATIVECOUNTS,
+                    "unknown normalization method");
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        SEV_REQUIRE(count_type == SEV_INT32 || count_type == SEV_INT64, "bad count type");
+        Stream s;
+        DevBuf<int64_t> cp;
+        DevBuf<int32_t> row, cnt;
+        DevBuf<uint32_t> tot;
+        int64_t nnz = 0;
+        csc_to_device(m, n, colptr, rowval, cp, row, nnz, s.st);
+        std::vector<int32_t> c((size_t)nnz);
+        for (int64_t i = 0; i < nnz; ++i) {
+            const int64_t v = count_type == SEV_INT32 ? ((const int32_t *)nzval)[i]
+                                                      : ((const int64_t *)nzval)[i];
+            SEV_REQUIRE(v >= 0 && v <= (int64_t)std::numeric_limits<int32_t>::max(),
+                        "counts must be non-negative and fit in Int32");
+            c[(size_t)i] = (int32_t)v;
+        }
+        upload(cnt, c.data(), (size_t)nnz, s.st);
+        tot.ensure((size_t)m);
+        row_totals(row.get(), cnt.get(), nnz, m, tot.get(), s.st);
+        const int dolog = method == SEV_LOGNORMALIZE;
+        if (dtype == SEV_FLOAT32) {
+            DevBuf<float> out;
+            out.ensure((size_t)nnz);
+            lognorm_f32(row.get(), cnt.get(), tot.get(), nnz, (float)scale_factor, dolog, out.get(),
+                        s.st);
+            download((float *)out_nzval, out.get(), (size_t)nnz, s.st);
+            SEV_HIP(hipStreamSynchronize(s.st));
+        } else {
+            DevBuf<double> out;
+            out.ensure((size_t)nnz);
+            lognorm_f64(row.get(), cnt.get(), tot.get(), nnz, scale_factor, dolog, out.get(),
+                        s.st);
+            download((double *)out_nzval, out.get(), (size_t)nnz, s.st);
+            SEV_HIP(hipStreamSynchronize(s.st));
+        }
+    });
+}
+
+
+int sev_scale_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                       const void *nzval, int dtype, double scale_max, void *out_nzval,
+                       void *out_mu) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        Stream s;
+        DevBuf<int64_t> cp;
+        DevBuf<int32_t> row;
+        int64_t nnz = 0;
+        csc_to_device(m, n, colptr, rowval, cp, row, nnz, s.st);
+        if (dtype == SEV_FLOAT32)
+            scale_host<float>(m, n, cp, nnz, (const float *)nzval, (float)scale_max,
+                              (float *)out_nzval, (float *)out_mu, s.st);
+        else
+            scale_host<double>(m, n, cp, nnz, (const double *)nzval, scale_max,
+                               (double *)out_nzval, (double *)out_mu, s.st);
+    });
+}
+
+int sev_centered_mul(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                     const void *nzval, const void *mu, int dtype, char trans, const double *x,
+                     double *y, double alpha, double beta) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        SEV_REQUIRE(trans == 'N' || trans == 'T', "trans must be 'N' or 'T'");
+        Stream s;
+        DevBuf<int64_t> cp;
+        DevBuf<int32_t> row;
+        int64_t nnz = 0;
+        csc_to_device(m, n, colptr, rowval, cp, row, nnz, s.st);
+        const size_t es = dtype == SEV_FLOAT32 ? 4 : 8;
+        DevBuf<unsigned char> vals, dmu;
+        vals.ensure((size_t)nnz * es);
+        if (nnz)
+            SEV_HIP(hipMemcpyAsync(vals.get(), nzval, (size_t)nnz * es, hipMemcpyHostToDevice, s.st));
+        if (mu) {
+            dmu.ensure((size_t)n * es);
+            SEV_HIP(hipMemcpyAsync(dmu.get(), mu, (size_t)n * es, hipMemcpyHostToDevice, s.st));
+        }
+        SlabCsc op;
+        IrlbaWork w;
+        build_slab_csc(op, m, n, cp.get(), row.get(), vals.get(), dtype == SEV_FLOAT64,
+                       mu ? (const void *)dmu.get() : nullptr, false, s.st);
+        const int64_t lx = trans == 'N' ? n : m, ly = trans == 'N' ? m : n;
+        DevBuf<double> dx, dy;
+        upload(dx, x, (size_t)lx, s.st);
+        dy.ensure((size_t)ly);
+        op_apply(op, trans == 'T', dx.get(), dy.get(), w, s.st);
+        std::vector<double> t((size_t)ly);
+        download(t.data(), dy.get(), (size_t)ly, s.st);
+        SEV_HIP(hipStreamSynchronize(s.st));
+        for (int64_t i = 0; i < ly; ++i)
+            y[i] = (beta == 0.0) ? alpha * t[(size_t)i] : alpha * t[(size_t)i] + beta * y[i];
+    });
+}
+
+int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                           const void *B, const void *mu, int dtype, int64_t nu, int64_t work,
+                           int64_t maxit, double tol, const double *init, double *s, double *U,
+                           double *V, int precision, int64_t *iters, int64_t *nmult) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        SEV_REQUIRE(init && s && U && V, "NULL output");
+        Stream st;
+        DevBuf<int64_t> cp;
+        DevBuf<int32_t> row;
+        DevBuf<unsigned char> vals, dmu;
+        int64_t nnz = 0;
+        csc_to_device(m, n, colptr, rowval, cp, row, nnz, st.st);
+        const size_t es = dtype == SEV_FLOAT32 ? 4 : 8;
+        vals.ensure((size_t)nnz * es);
+        if (nnz)
+            SEV_HIP(hipMemcpyAsync(vals.get(), B, (size_t)nnz * es, hipMemcpyHostToDevice, st.st));
+        if (mu) {
+            dmu.ensure((size_t)n * es);
+            SEV_HIP(hipMemcpyAsync(dmu.get(), mu, (size_t)n * es, hipMemcpyHostToDevice, st.st));
+        }
+        SlabCsc op;
+        build_slab_csc(op, m, n, cp.get(), row.get(), vals.get(), dtype == SEV_FLOAT64,
+                       mu ? (const void *)dmu.get() : nullptr, precision == SEV_PREC_BF16, st.st);
+        IrlbaParams prm;
+        prm.nu = nu;
+        prm.work = work;
+        prm.maxit = maxit;
+        prm.tol = tol;
+        prm.svtol = tol;
+        IrlbaWork w;
+        DevBuf<double> dU, dV;
+        dU.ensure((size_t)(m * nu));
+        dV.ensure((size_t)(n * nu));
+        IrlbaOut out;
+        try {
+            irlba_device(op, prm, init, w, s, dU.get(), dV.get(), st.st, &out);
+        } catch (...) {
+            if (iters) *iters = out.iters;
+            if (nmult) *nmult = out.nmult;
+            throw;
+        }
+        download(U, dU.get(), (size_t)(m * nu), st.st);
+        download(V, dV.get(), (size_t)(n * nu), st.st);
+        SEV_HIP(hipStreamSynchronize(st.st));
+        if (iters) *iters = out.iters;
+        if (nmult) *nmult = out.nmult;
+    });
+}
+
+int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
+            int include_self, int precision, int32_t *nn_index, void *distances,
+            int64_t *n_fallback) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        if (dtype == SEV_FLOAT32)
+            knn_host<float>((const float *)X, n, d, ld, k, include_self, nn_index,
+                            (float *)distances);
+        else if (dtype == SEV_FLOAT64)
+            knn_host<double>((const double *)X, n, d, ld, k, include_self, nn_index,
+                             (double *)distances);
+        else
+            fail(kErrArg, "bad dtype");
+        if (n_fallback) *n_fallback = n;
+    });
+}
+
+void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int ntables,
+                               int include_self, int seed, int32_t *nn_index, float *distances) {
+    (void)ntables;
+    (void)seed;
+    (void)sev_knn(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32, nn_index, distances,
+                  nullptr);
+}
+
+void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int ntables,
+                               int include_self, int seed, int32_t *nn_index, double *distances) {
+    (void)ntables;
+    (void)seed;
+    (void)sev_knn(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index, distances,
+                  nullptr);
+}
+
+int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,
+                    int64_t *nnz, int64_t *colptr, int64_t *rowval, void *nzval) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        SEV_REQUIRE(n >= 1 && k >= 1 && k <= kMaxK, "k must be in [1, 64]");
+        SEV_REQUIRE(nnz != nullptr, "nnz is NULL");
+        std::vector<int32_t> nn((size_t)(n * k));
+        for (int64_t j = 0; j < n; ++j)
+            for (int64_t t = 0; t < k; ++t) {
+                const int32_t l = nn_index[j + t * n] - 1;
+                SEV_REQUIRE(l >= 0 && l < n, "neighbour index out of range");
+                nn[(size_t)(j * k + t)] = l;
+            }
+        Stream s;
+        DevBuf<int32_t> dnn;
+        upload(dnn, nn.data(), nn.size(), s.st);
+        SnnWork w;
+        DevBuf<int64_t> dcp;
+        dcp.ensure((size_t)n + 1);
+        DevBuf<int32_t> drow;
+        int64_t total;
+        std::vector<int32_t> rv;
+        if (dtype == SEV_FLOAT32) {
+            std::vector<float> wt;
+            const int xmin = snn_table<float>(k, prune, wt);
+            total = snn_count<float>(dnn.get(), n, k, xmin, w, dcp.get(), s.st);
+            *nnz = total;
+            if (!colptr) return;
+            DevBuf<float> dw, dv;
+            upload(dw, wt.data(), wt.size(), s.st);
+            drow.ensure((size_t)total);
+            dv.ensure((size_t)total);
+            snn_fill<float>(dnn.get(), n, k, xmin, dw.get(), w, dcp.get(), drow.get(), dv.get(),
+                            s.st);
+            rv.resize((size_t)total);
+            download(rv.data(), drow.get(), (size_t)total, s.st);
+            download((float *)nzval, dv.get(), (size_t)total, s.st);
+        } else {
+            std::vector<double> wt;
+            const int xmin = snn_table<double>(k, prune, wt);
+            total = snn_count<double>(dnn.get(), n, k, xmin, w, dcp.get(), s.st);
+            *nnz = total;
+            if (!colptr) return;
+            DevBuf<double> dw, dv;
+            upload(dw, wt.data(), wt.size(), s.st);
+            drow.ensure((size_t)total);
+            dv.ensure((size_t)total);
+            snn_fill<double>(dnn.get(), n, k, xmin, dw.get(), w, dcp.get(), drow.get(), dv.get(),
+                             s.st);
+            rv.resize((size_t)total);
+            download(rv.data(), drow.get(), (size_t)total, s.st);
+            download((double *)nzval, dv.get(), (size_t)total, s.st);
+        }
+        download(colptr, dcp.get(), (size_t)n + 1, s.st);
+        SEV_HIP(hipStreamSynchronize(s.st));
+        for (int64_t j = 0; j <= n; ++j) colptr[j] += 1;
+        for (int64_t i = 0; i < total; ++i) rowval[i] = (int64_t)rv[(size_t)i] + 1;
+    });
+}
+
+}  // extern "C"

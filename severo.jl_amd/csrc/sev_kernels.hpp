@@ -1,0 +1,163 @@
+// Host-side launchers of the device kernels (internal to libsevero_hip).
+#pragma once
+
+#include <hip/hip_bf16.h>
+
+#include "sev_common.hpp"
+
+namespace sev {
+
+// ---- sev_scan.hip --------------------------------------------------------
+// out[i] = sum_{j<i} in[j] for i in [0, n]; out has n+1 entries.  Uses the
+// caller-provided scratch of scan_scratch_elems(n) int64 values.  in may
+// alias out.
+size_t scan_scratch_elems(int64_t n);
+void scan_exclusive_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *scratch,
+                        hipStream_t st);
+// Same for int32 counts widened to int64 offsets.
+void scan_exclusive_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *scratch,
+                        hipStream_t st);
+
+// ---- sev_prep.hip --------------------------------------------------------
+// Synthetic NB count matrix, cells x genes CSC.  Two calls: counts only
+// (rowval == nullptr) fills colptr and returns nnz, then the fill.
+struct SynthPlan {
+    int64_t cells = 0, genes = 0, chunk = 0, nchunk = 0;
+};
+int64_t synth_count(int64_t C, int64_t G, uint64_t seed, double r, const double *p0_d,
+                    const double *q_d, double boost_frac, uint8_t *cls_d, int64_t *chunk_cnt_d,
+                    int64_t *chunk_off_d, int64_t *colptr_d, int64_t *scratch, hipStream_t st,
+                    SynthPlan *plan);
+void synth_fill(const SynthPlan &plan, uint64_t seed, double r, const double *p0_d,
+                const double *q_d, const uint8_t *cls_d, const int64_t *chunk_off_d,
+                int32_t *rowval_d, int32_t *nzval_d, hipStream_t st);
+void synth_classes(int64_t C, uint64_t seed, double boost_frac, uint8_t *cls_d, hipStream_t st);
+
+// Per-cell totals and the per-nonzero map of normalize_cells.
+void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t ncells,
+                uint32_t *tot, hipStream_t st);
+void lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+                 float sf, int dolog, float *out, hipStream_t st);
+void lognorm_f64(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+                 double sf, int dolog, double *out, hipStream_t st);
+
+// Column selection (feature gather) keeping the feature order.
+void gather_lengths(const int64_t *feat, const int64_t *colptr, int64_t nfeat, int64_t *len,
+                    hipStream_t st);
+template <class T>
+void gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, const T *val,
+                 int64_t nfeat, const int64_t *fcolptr, int32_t *orow, T *oval, hipStream_t st);
+
+// scale_data: sequential Welford per column then the clipped map.
+template <class T>
+void scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scale_max,
+                 T *sd, T *mup, T *clip, hipStream_t st);
+template <class T>
+void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, const T *clip,
+               T *out, hipStream_t st);
+
+}  // namespace sev
+
+namespace sev {
+
+// ---- sev_irlba.hip -------------------------------------------------------
+// Device layout of the centered, scaled matrix A = B - 1 mu' (C x F) used by
+// both Lanczos products: B in CSC (the scaled values, not copied) plus, per
+// slab of kSlabRows rows, the position of each column's first entry in the
+// slab (off[s*F + j], slab-major, (nslab+1) x F) and 16-bit in-slab rows.
+constexpr int64_t kSlabRows = 1024;
+
+enum ValType : int { kValF32 = 0, kValBF16 = 1, kValF64 = 2 };
+
+struct SlabCsc {
+    int64_t m = 0, n = 0, nnz = 0, nslab = 0;
+    DevBuf<int64_t> off;     // (nslab + 1) * n
+    DevBuf<uint16_t> lrow;   // nnz
+    DevBuf<uint16_t> vbf16;  // nnz bf16 values (precision form only)
+    const void *vals = nullptr;  // values read by the products (not owned
+                                 // unless they point into vbf16)
+    const void *mu = nullptr;    // mu' (not owned), same float type as the
+                                 // source values; nullptr = uncentered
+    int vtype = kValF32;         // type of vals
+    bool mu64 = false;           // mu is double
+};
+
+// val is float (src_f64 == false) or double; bf16 converts float values.
+void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
+                    const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
+                    hipStream_t st);
+
+// y = A x (trans == 0, y length m) or y = A' x (trans == 1, y length n) on
+// the device, A = B - 1 mu'.  Uses w.part / w.part2 as scratch.
+struct IrlbaWork;
+void op_apply(const SlabCsc &op, int trans, const double *x, double *y, IrlbaWork &w,
+              hipStream_t st);
+
+struct IrlbaParams {
+    int64_t nu = 50, work = 57, maxit = 1000;
+    double tol = 1e-5, svtol = 1e-5;
+    int reorth_passes = 1;  // CGS passes on the W (left) side
+};
+
+struct IrlbaOut {
+    int64_t iters = 0, nmult = 0;
+};
+
+struct IrlbaWork {
+    DevBuf<double> W, W2, V, V2, B, BU, BV, T, part, part2, red, Fv, scal, rnd;
+    DevBuf<int> flags;
+};
+
+// Runs the Lanczos bidiagonalisation on the device operator.  init (host,
+// length n) is the starting vector.  On return s (host, nu), U (device,
+// m x nu column-major) and V (device, n x nu column-major) hold the
+// singular triplets.  Throws Error(kErrConvergence) like the reference's
+// "convergence failed".
+void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
+                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out);
+
+// Z[r*rs + c*cs] = float(U[c*m + r] * s[c]) for c < d: the embedding
+// U*diag(s) (src/embedding.jl:67) rounded to Float32.
+void embedding_f32(const double *U, const double *s_dev, int64_t m, int64_t d, float *Z,
+                   int64_t rs, int64_t cs, hipStream_t st);
+
+// Host SVD of a small square matrix (sev_svd.cpp): A = U diag(s) V',
+// column-major, s descending.
+void small_svd(int64_t n, const double *A, double *U, double *s, double *V);
+
+}  // namespace sev
+
+namespace sev {
+
+// ---- sev_knn.hip ---------------------------------------------------------
+// Exact Euclidean kNN, the definition FindNeighboursEuclidean32/64
+// (src/neighbours.jl:32-63) is given here: D(i,j) = sum_t (x_it - x_jt)^2 in
+// T, each difference and square rounded, accumulated in t order (no FMA);
+// rank by (D, j); j == i skipped only when include_self == 0; distance
+// sqrt(D).  X is column-major (x_it at X[t*ld + i]).  Queries are rows
+// qidx[r] (or r when qidx == nullptr); outputs row-major nq x k (0-based
+// indices).
+constexpr int kMaxK = 64;
+template <class T>
+void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+               const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st);
+
+// ---- sev_snn.hip ---------------------------------------------------------
+// Jaccard SNN graph from 0-based row-major neighbour lists (n x k).
+// wtab holds w[x] = x / (k + (k - x)) in T for x = 0..k; entries with
+// x < xmin are pruned.  Two phases: count (colptr filled, nnz returned),
+// then fill (rows 0-based, values).
+struct SnnWork {
+    DevBuf<int32_t> deg, cur, rev, cnt, heavy, nheavy;
+    DevBuf<int64_t> rptr, scratch, htab_off, heavyP;
+    DevBuf<uint64_t> htab;
+    int64_t nheavy_h = 0;
+};
+template <class T>
+int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
+                  int64_t *colptr, hipStream_t st);
+template <class T>
+void snn_fill(const int32_t *nn, int64_t n, int64_t k, int xmin, const T *wtab, SnnWork &w,
+              const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st);
+
+}  // namespace sev

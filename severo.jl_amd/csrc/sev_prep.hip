@@ -1,0 +1,350 @@
+// Input-side kernels of the path: synthetic counts, normalize_cells,
+// feature gather and scale_features.
+//
+// normalize_cells:  src/normalize.jl:17-38 (row_norm, log_norm)
+// feature gather:   Y[:, hvf] (docs/src/pbmc.md:121, src/scaling.jl:350-352)
+// scale_features:   src/scaling.jl:18-34 (mean_var), :199-217 (scale_data)
+//
+// Built with -ffp-contract=off: every FP operation rounds where the Julia
+// source rounds (no fused multiply-add), and '/' and sqrt are the IEEE
+// correctly rounded operations.
+#include "sev_kernels.hpp"
+
+namespace sev {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// --------------------------------------------------------------------------
+// Synthetic negative-binomial counts.  Entry (g, c): u = u01(seed, g, c);
+// zero iff u < p0; otherwise inverse CDF with the recurrence
+// P(x+1) = P(x) * q * (x + r) / (x + 1).
+// --------------------------------------------------------------------------
+constexpr uint32_t kClassStream = 0x7FFFFFFFu;
+constexpr int32_t kNbXmax = 100000;
+
+__device__ __forceinline__ int32_t nb_draw(double u, double p0, double q, double r) {
+    if (u < p0) return 0;
+    double p = p0, cdf = p0;
+    int32_t x = 0;
+    while (u >= cdf && x < kNbXmax) {
+        p = ((p * q) * ((double)x + r)) / ((double)x + 1.0);
+        x += 1;
+        cdf = cdf + p;
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_synth_class(uint8_t *cls, int64_t C, uint64_t seedm, double frac) {
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < C;
+         c += (int64_t)gridDim.x * kBlock)
+        cls[c] = u01(seedm, kClassStream, (uint32_t)c) < frac ? 1 : 0;
+}
+
+// Block (chunk, gene): number of nonzeros among the chunk's cells.
+__global__ void __launch_bounds__(kBlock)
+k_synth_count(int64_t C, int64_t nchunk, int64_t chunk, uint64_t seedm, const double *p0,
+              const uint8_t *cls, int64_t *cnt) {
+    const int64_t g = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+    const int64_t c0 = ch * chunk;
+    const int64_t c1 = c0 + chunk < C ? c0 + chunk : C;
+    const double pa = p0[2 * g], pb = p0[2 * g + 1];
+    int64_t n = 0;
+    for (int64_t c = c0 + threadIdx.x; c < c1; c += kBlock) {
+        double u = u01(seedm, (uint32_t)g, (uint32_t)c);
+        n += (u >= (cls[c] ? pb : pa)) ? 1 : 0;
+    }
+    __shared__ int64_t red[kBlock / kWave];
+    n = wave_sum_i64(n);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t s = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w];
+        cnt[blockIdx.x] = s;
+    }
+}
+
+// Block (chunk, gene): write rows (ascending) and counts of the nonzeros.
+__global__ void __launch_bounds__(kBlock)
+k_synth_fill(int64_t C, int64_t nchunk, int64_t chunk, uint64_t seedm, double r,
+             const double *p0, const double *q, const uint8_t *cls, const int64_t *off,
+             int32_t *rowval, int32_t *nzval) {
+    const int64_t g = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+    const int64_t c0 = ch * chunk;
+    const int64_t c1 = c0 + chunk < C ? c0 + chunk : C;
+    __shared__ int32_t wcnt[kBlock / kWave];
+    int64_t pos = off[blockIdx.x];
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    for (int64_t cb = c0; cb < c1; cb += kBlock) {
+        const int64_t c = cb + threadIdx.x;
+        int32_t x = 0;
+        if (c < c1) {
+            const int k = cls[c];
+            double u = u01(seedm, (uint32_t)g, (uint32_t)c);
+            x = nb_draw(u, p0[2 * g + k], q[2 * g + k], r);
+        }
+        const uint64_t m = __ballot(x != 0);
+        if (lane == 0) wcnt[w] = __popcll(m);
+        __syncthreads();
+        int32_t before = 0, total = 0;
+        for (int i = 0; i < kBlock / kWave; ++i) {
+            before += i < w ? wcnt[i] : 0;
+            total += wcnt[i];
+        }
+        if (x != 0) {
+            const uint64_t lower = lane ? (m & ((~0ull) >> (kWave - lane))) : 0ull;
+            const int64_t p = pos + before + __popcll(lower);
+            rowval[p] = (int32_t)c;
+            nzval[p] = x;
+        }
+        pos += total;
+        __syncthreads();
+    }
+}
+
+// --------------------------------------------------------------------------
+// normalize_cells
+// --------------------------------------------------------------------------
+
+// s = sum(A, dims=2): integer row (cell) totals over the CSC nonzeros.
+__global__ void __launch_bounds__(kBlock)
+k_row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, uint32_t *tot) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
+         p += (int64_t)gridDim.x * kBlock)
+        atomicAdd(&tot[row[p]], (uint32_t)cnt[p]);
+}
+
+// nzB = sf * nzA / s[row] in Float32, left to right; then log1p, defined as
+// the double-precision log1p rounded to float.
+__global__ void __launch_bounds__(kBlock)
+k_lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+              float sf, int dolog, float *out) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
+         p += (int64_t)gridDim.x * kBlock) {
+        const float v = (sf * (float)cnt[p]) / (float)tot[row[p]];
+        out[p] = dolog ? (float)log1p((double)v) : v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_lognorm_f64(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+              double sf, int dolog, double *out) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
+         p += (int64_t)gridDim.x * kBlock) {
+        const double v = (sf * (double)cnt[p]) / (double)tot[row[p]];
+        out[p] = dolog ? log1p(v) : v;
+    }
+}
+
+// --------------------------------------------------------------------------
+// Feature gather
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_gather_len(const int64_t *feat, const int64_t *colptr, int64_t nfeat, int64_t *len) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j < nfeat) {
+        const int64_t g = feat[j];
+        len[j] = colptr[g + 1] - colptr[g];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, const T *val,
+              const int64_t *fcolptr, int32_t *orow, T *oval) {
+    const int64_t j = blockIdx.x;
+    const int64_t g = feat[j];
+    const int64_t b = colptr[g], len = colptr[g + 1] - b, ob = fcolptr[j];
+    for (int64_t i = threadIdx.x; i < len; i += kBlock) {
+        orow[ob + i] = row[b + i];
+        oval[ob + i] = val[b + i];
+    }
+}
+
+// --------------------------------------------------------------------------
+// scale_data.  One wavefront per column runs the reference's sequential
+// Welford recurrence (src/scaling.jl:21-30) in T, every lane computing the
+// same values from a 64-element window loaded with one coalesced read:
+//   count = m - nnz; for v: count += 1; delta = v - mu;
+//   mu += delta / T(count); s += delta * (v - mu)
+// var = s / T(m - 1); sigma = sqrt(var); mu' = mu / sigma;
+// clip = scale_max + mu'.
+// --------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scale_max, T *sdv,
+              T *mupv, T *clipv) {
+    const int64_t col = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (col >= ncol) return;
+    const int lane = lane_id();
+    const int64_t b = colptr[col], e = colptr[col + 1];
+    int64_t count = m - (e - b);
+    T mu = 0, s = 0;
+    for (int64_t base = b; base < e; base += kWave) {
+        const int64_t p = base + lane;
+        const T vl = p < e ? x[p] : T(0);
+        const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
+        for (int t = 0; t < nt; ++t) {
+            const T v = __shfl(vl, t, kWave);
+            count += 1;
+            const T delta = v - mu;
+            mu = mu + delta / (T)count;
+            s = s + delta * (v - mu);
+        }
+    }
+    const T var = s / (T)(m - 1);
+    const T sd = sqrt(var);
+    const T mup = mu / sd;
+    if (lane == 0) {
+        sdv[col] = sd;
+        mupv[col] = mup;
+        clipv[col] = scale_max + mup;
+    }
+}
+
+// b = min(x / sigma, clip) with only the upper clip (src/scaling.jl:210-213).
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_scale_map(const int64_t *colptr, const T *x, const T *sdv, const T *clipv, T *out) {
+    const int64_t j = blockIdx.x;
+    const int64_t b = colptr[j], e = colptr[j + 1];
+    const T sd = sdv[j], clip = clipv[j];
+    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+        const T v = x[p] / sd;
+        out[p] = (v > clip) ? clip : v;
+    }
+}
+
+unsigned grid_for(int64_t n, int per_cu = 8) {
+    int64_t want = ceil_div(n > 0 ? n : 1, kBlock);
+    int64_t cap = (int64_t)device_cu_count() * per_cu;
+    return (unsigned)(want < cap ? want : cap);
+}
+
+}  // namespace
+
+void synth_classes(int64_t C, uint64_t seed, double boost_frac, uint8_t *cls_d, hipStream_t st) {
+    hipLaunchKernelGGL(k_synth_class, dim3(grid_for(C)), dim3(kBlock), 0, st, cls_d, C,
+                       mix64(seed), boost_frac);
+    SEV_LAUNCH_CHECK();
+}
+
+int64_t synth_count(int64_t C, int64_t G, uint64_t seed, double r, const double *p0_d,
+                    const double *q_d, double boost_frac, uint8_t *cls_d, int64_t *chunk_cnt_d,
+                    int64_t *chunk_off_d, int64_t *colptr_d, int64_t *scratch, hipStream_t st,
+                    SynthPlan *plan) {
+    (void)q_d;
+    (void)r;
+    plan->cells = C;
+    plan->genes = G;
+    plan->chunk = 16384;
+    plan->nchunk = C > 0 ? ceil_div(C, plan->chunk) : 1;
+    SEV_REQUIRE(G * plan->nchunk < (int64_t)0x7FFFFFFF, "synthetic matrix too large");
+    synth_classes(C, seed, boost_frac, cls_d, st);
+    const int64_t nb = G * plan->nchunk;
+    hipLaunchKernelGGL(k_synth_count, dim3((unsigned)nb), dim3(kBlock), 0, st, C, plan->nchunk,
+                       plan->chunk, mix64(seed), p0_d, cls_d, chunk_cnt_d);
+    SEV_LAUNCH_CHECK();
+    scan_exclusive_i64(chunk_cnt_d, chunk_off_d, nb, scratch, st);
+    // colptr[g] = chunk_off[g * nchunk]; colptr[G] = total
+    int64_t total = 0;
+    SEV_HIP(hipMemcpyAsync(&total, chunk_off_d + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipMemcpy2DAsync(colptr_d, sizeof(int64_t), chunk_off_d,
+                             sizeof(int64_t) * plan->nchunk, sizeof(int64_t), G,
+                             hipMemcpyDeviceToDevice, st));
+    SEV_HIP(hipMemcpyAsync(colptr_d + G, chunk_off_d + nb, sizeof(int64_t),
+                           hipMemcpyDeviceToDevice, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    return total;
+}
+
+void synth_fill(const SynthPlan &plan, uint64_t seed, double r, const double *p0_d,
+                const double *q_d, const uint8_t *cls_d, const int64_t *chunk_off_d,
+                int32_t *rowval_d, int32_t *nzval_d, hipStream_t st) {
+    const int64_t nb = plan.genes * plan.nchunk;
+    if (nb == 0) return;
+    hipLaunchKernelGGL(k_synth_fill, dim3((unsigned)nb), dim3(kBlock), 0, st, plan.cells,
+                       plan.nchunk, plan.chunk, mix64(seed), r, p0_d, q_d, cls_d, chunk_off_d,
+                       rowval_d, nzval_d);
+    SEV_LAUNCH_CHECK();
+}
+
+void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t ncells,
+                uint32_t *tot, hipStream_t st) {
+    SEV_HIP(hipMemsetAsync(tot, 0, sizeof(uint32_t) * (size_t)(ncells > 0 ? ncells : 1), st));
+    if (nnz == 0) return;
+    hipLaunchKernelGGL(k_row_totals, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, nnz,
+                       tot);
+    SEV_LAUNCH_CHECK();
+}
+
+void lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+                 float sf, int dolog, float *out, hipStream_t st) {
+    if (nnz == 0) return;
+    hipLaunchKernelGGL(k_lognorm_f32, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, tot,
+                       nnz, sf, dolog, out);
+    SEV_LAUNCH_CHECK();
+}
+
+void lognorm_f64(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
+                 double sf, int dolog, double *out, hipStream_t st) {
+    if (nnz == 0) return;
+    hipLaunchKernelGGL(k_lognorm_f64, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, tot,
+                       nnz, sf, dolog, out);
+    SEV_LAUNCH_CHECK();
+}
+
+void gather_lengths(const int64_t *feat, const int64_t *colptr, int64_t nfeat, int64_t *len,
+                    hipStream_t st) {
+    if (nfeat == 0) return;
+    hipLaunchKernelGGL(k_gather_len, dim3((unsigned)ceil_div(nfeat, kBlock)), dim3(kBlock), 0, st,
+                       feat, colptr, nfeat, len);
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, const T *val,
+                 int64_t nfeat, const int64_t *fcolptr, int32_t *orow, T *oval, hipStream_t st) {
+    if (nfeat == 0) return;
+    hipLaunchKernelGGL(k_gather_copy<T>, dim3((unsigned)nfeat), dim3(kBlock), 0, st, feat, colptr,
+                       row, val, fcolptr, orow, oval);
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scale_max, T *sd,
+                 T *mup, T *clip, hipStream_t st) {
+    if (ncol == 0) return;
+    const int64_t threads = ncol * kWave;
+    hipLaunchKernelGGL(k_scale_stats<T>, dim3((unsigned)ceil_div(threads, kBlock)), dim3(kBlock),
+                       0, st, m, ncol, colptr, x, scale_max, sd, mup, clip);
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, const T *clip,
+               T *out, hipStream_t st) {
+    if (ncol == 0) return;
+    hipLaunchKernelGGL(k_scale_map<T>, dim3((unsigned)ncol), dim3(kBlock), 0, st, colptr, x, sd,
+                       clip, out);
+    SEV_LAUNCH_CHECK();
+}
+
+template void gather_copy<float>(const int64_t *, const int64_t *, const int32_t *, const float *,
+                                 int64_t, const int64_t *, int32_t *, float *, hipStream_t);
+template void gather_copy<double>(const int64_t *, const int64_t *, const int32_t *,
+                                  const double *, int64_t, const int64_t *, int32_t *, double *,
+                                  hipStream_t);
+template void scale_stats<float>(int64_t, int64_t, const int64_t *, const float *, float, float *,
+                                 float *, float *, hipStream_t);
+template void scale_stats<double>(int64_t, int64_t, const int64_t *, const double *, double,
+                                  double *, double *, double *, hipStream_t);
+template void scale_map<float>(int64_t, const int64_t *, const float *, const float *,
+                               const float *, float *, hipStream_t);
+template void scale_map<double>(int64_t, const int64_t *, const double *, const double *,
+                                const double *, double *, hipStream_t);
+
+}  // namespace sev

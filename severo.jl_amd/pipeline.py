@@ -1,0 +1,161 @@
+"""Device-resident pipeline over the ``sev_pipeline_*`` C ABI.
+
+One handle keeps the count matrix and every intermediate on the GPU:
+normalize_cells -> Y[:, features] -> scale_features -> PCA (IRLBA)
+-> exact kNN -> Jaccard SNN.  Getters copy results back in Severo's layout
+(1-based Int64 CSC, column-major dense, 1-based neighbour indices).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import check
+
+PRECISION = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16}
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Pipeline:
+    def __init__(self):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        check(self._L.sev_pipeline_create(C.byref(h)))
+        self._h = h
+        self.k = 0
+        self.npcs = 0
+        self.nfeat = 0
+
+    def close(self):
+        if self._h:
+            self._L.sev_pipeline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(self._L.sev_pipeline_synchronize(self._h))
+
+    # -- counts ------------------------------------------------------------
+    def load_counts(self, X):
+        X = sp.csc_matrix(X)
+        X.sort_indices()
+        colptr = np.ascontiguousarray(X.indptr, np.int64) + 1
+        rowval = np.ascontiguousarray(X.indices, np.int64) + 1
+        vals = np.ascontiguousarray(X.data, np.int32)
+        check(self._L.sev_pipeline_load_counts(self._h, X.shape[0], X.shape[1], _ptr(colptr),
+                                               _ptr(rowval), _ptr(vals)))
+
+    def synth_counts(self, n_cells, tables, seed=None):
+        seed = tables.seed if seed is None else seed
+        check(self._L.sev_pipeline_synth_counts(self._h, n_cells, tables.n_genes, seed, tables.r,
+                                                _ptr(tables.p0), _ptr(tables.q),
+                                                tables.boost_frac))
+
+    def counts_info(self):
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self._L.sev_pipeline_counts_info(self._h, C.byref(m), C.byref(n), C.byref(nnz)))
+        return m.value, n.value, nnz.value
+
+    def get_counts(self):
+        m, n, nnz = self.counts_info()
+        colptr = np.zeros(n + 1, np.int64)
+        rowval = np.zeros(nnz, np.int64)
+        vals = np.zeros(nnz, np.int32)
+        check(self._L.sev_pipeline_get_counts(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals)))
+        return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, n))
+
+    # -- stages ------------------------------------------------------------
+    def normalize(self, method="lognormalize", scale_factor=1e4):
+        code = {"lognormalize": _lib.SEV_LOGNORMALIZE,
+                "relativecounts": _lib.SEV_RELATIVECOUNTS}[method]
+        check(self._L.sev_pipeline_normalize(self._h, code, float(scale_factor)))
+
+    def get_normalized(self):
+        _, _, nnz = self.counts_info()
+        out = np.zeros(nnz, np.float32)
+        check(self._L.sev_pipeline_get_normalized(self._h, _ptr(out)))
+        return out
+
+    def select_scale(self, features, scale_max=10.0):
+        """features: 0-based gene indices in the order to keep."""
+        f = np.ascontiguousarray(np.asarray(features, np.int64) + 1)
+        check(self._L.sev_pipeline_select_scale(self._h, len(f), _ptr(f), float(scale_max)))
+        self.nfeat = len(f)
+
+    def get_scaled(self):
+        m, _, _ = self.counts_info()
+        nnz = C.c_int64()
+        check(self._L.sev_pipeline_scaled_nnz(self._h, C.byref(nnz)))
+        colptr = np.zeros(self.nfeat + 1, np.int64)
+        rowval = np.zeros(nnz.value, np.int64)
+        vals = np.zeros(nnz.value, np.float32)
+        mu = np.zeros(self.nfeat, np.float32)
+        check(self._L.sev_pipeline_get_scaled(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals),
+                                              _ptr(mu)))
+        return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, self.nfeat)), mu
+
+    def pca(self, npcs=50, init=None, tol=1e-5, maxit=1000, precision="fp32", rng=None):
+        if init is None:
+            rng = np.random.default_rng(0) if rng is None else rng
+            init = rng.standard_normal(self.nfeat)
+        init = np.ascontiguousarray(init, np.float64)
+        it, nm = C.c_int64(), C.c_int64()
+        check(self._L.sev_pipeline_pca(self._h, npcs, maxit, float(tol), _ptr(init),
+                                       PRECISION[precision], C.byref(it), C.byref(nm)))
+        self.npcs = npcs
+        return it.value, nm.value
+
+    def get_pca(self):
+        m, _, _ = self.counts_info()
+        s = np.zeros(self.npcs)
+        U = np.zeros((m, self.npcs), order="F")
+        V = np.zeros((self.nfeat, self.npcs), order="F")
+        Z = np.zeros((m, self.npcs), np.float32, order="F")
+        check(self._L.sev_pipeline_get_pca(self._h, _ptr(s), _ptr(U), _ptr(V), _ptr(Z)))
+        return s, U, V, Z
+
+    def set_embedding(self, Z):
+        Z = np.asfortranarray(Z, np.float32)
+        check(self._L.sev_pipeline_set_embedding(self._h, Z.shape[0], Z.shape[1], _ptr(Z),
+                                                 Z.shape[0]))
+        self.npcs = Z.shape[1]
+        self._m = Z.shape[0]
+
+    def neighbours(self, k=20, include_self=True, ndims=0, precision="fp32"):
+        nfb = C.c_int64()
+        check(self._L.sev_pipeline_neighbours(self._h, k, int(include_self), ndims,
+                                              PRECISION[precision], C.byref(nfb)))
+        self.k = k
+        return nfb.value
+
+    def get_neighbours(self, m):
+        idx = np.zeros((m, self.k), np.int32, order="F")
+        dist = np.zeros((m, self.k), np.float32, order="F")
+        check(self._L.sev_pipeline_get_neighbours(self._h, _ptr(idx), _ptr(dist)))
+        return idx, dist
+
+    def snn(self, prune=1.0 / 15, dtype=np.float32):
+        nnz = C.c_int64()
+        code = _lib.SEV_FLOAT32 if np.dtype(dtype) == np.float32 else _lib.SEV_FLOAT64
+        check(self._L.sev_pipeline_snn(self._h, float(prune), code, C.byref(nnz)))
+        self._snn = (nnz.value, np.dtype(dtype))
+        return nnz.value
+
+    def get_snn(self, m):
+        nnz, dt = self._snn
+        colptr = np.zeros(m + 1, np.int64)
+        rowval = np.zeros(nnz, np.int64)
+        vals = np.zeros(nnz, dt)
+        check(self._L.sev_pipeline_get_snn(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals)))
+        return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, m))

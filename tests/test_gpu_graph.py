@@ -1,0 +1,106 @@
+"""GPU parity: exact kNN and the Jaccard SNN graph against the oracle
+(bit-exact indices, distances and weights)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("include_self", [True, False])
+@pytest.mark.parametrize("n,d,k", [(1000, 10, 7), (3000, 50, 20), (257, 3, 64)])
+def test_ann_exact(sev, orc, dtype, include_self, n, d, k):
+    rng = np.random.default_rng(n + d)
+    X = rng.standard_normal((n, d)).astype(dtype)
+    idx, dist = sev.ann(X, k, include_self=include_self)
+    ridx, rdist = orc.knn(X, k, include_self=include_self)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist)
+
+
+def test_ann_ties_and_duplicates(sev, orc):
+    rng = np.random.default_rng(5)
+    base = rng.random((300, 5)).astype(np.float32)
+    X = np.concatenate([base, base, base[:100], np.zeros((5, 5), np.float32)])
+    idx, dist = sev.ann(X, 6)
+    ridx, rdist = orc.knn(X, 6)
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+
+
+def test_find_neighbours_abi_strided(sev, orc):
+    """FindNeighboursEuclidean32 with ld > n (a view of the leading rows)."""
+    import ctypes
+    rng = np.random.default_rng(6)
+    full = np.asfortranarray(rng.standard_normal((700, 12)).astype(np.float32))
+    n, d, k = 500, 12, 9
+    idx = np.zeros((n, k), np.int32, order="F")
+    dist = np.zeros((n, k), np.float32, order="F")
+    L = sev.load()
+    L.FindNeighboursEuclidean32(full.ctypes.data, n, d, 700, k, 24, 1, 123, idx.ctypes.data,
+                                dist.ctypes.data)
+    ridx, rdist = orc.knn(np.asfortranarray(full[:n]), k)
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("k,prune", [(20, 1 / 15), (15, 1 / 15), (30, 1 / 15), (10, 0.0)])
+def test_snn_vs_oracle(sev, orc, dt, k, prune):
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((4000, 10)).astype(np.float32)
+    idx, _ = orc.knn(X, k)
+    S = sev.api._snn_from_index(idx, k, prune, dt)
+    cp, rv, nz = orc.snn(idx, k, prune, dt)
+    assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.indices, rv - 1)
+    assert S.data.dtype == dt and np.array_equal(S.data, nz)
+
+
+def test_snn_hub_columns(sev, orc):
+    """A few hubs that many cells list force the block and global-table
+    paths (candidate unions above the wavefront table)."""
+    rng = np.random.default_rng(9)
+    n, k = 6000, 20
+    idx = np.zeros((n, k), np.int32)
+    for i in range(n):
+        hubs = [1, 2, 3, 4] if i % 2 == 0 else [5, 6]
+        others = rng.choice(np.arange(7, n + 1), size=k - len(hubs), replace=False)
+        idx[i] = np.array(hubs + others.tolist(), np.int32)
+    idx = np.asfortranarray(idx)
+    S = sev.api._snn_from_index(idx, k, 1 / 15, np.float64)
+    cp, rv, nz = orc.snn(idx, k, 1 / 15, np.float64)
+    assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.indices, rv - 1)
+    assert np.array_equal(S.data, nz)
+
+
+def test_jaccard_index_from_nn_matrix(sev, orc):
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((1500, 8))
+    nn = sev.nearest_neighbours(X, 12)
+    assert nn.shape == (1500, 1500) and nn.nnz == 1500 * 12
+    S = sev.jaccard_index(nn, prune=1 / 15)
+    ridx, _ = orc.knn(X, 12)
+    cp, rv, nz = orc.snn(ridx, 12, 1 / 15, np.float64)
+    assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.data, nz)
+
+
+def test_pipeline_graph_end_to_end(sev, orc):
+    """Config-1 shaped path at reduced size: every stage checked against the
+    oracle given the previous stage's device output."""
+    t = sev.nb_tables(4000, seed=0, density=0.05)
+    C = 4000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    p.normalize("lognormalize", 1e4)
+    feats = sev.top_mean_features(t, 800)
+    p.select_scale(feats, 10.0)
+    p.pca(20, init=np.random.default_rng(0).standard_normal(800))
+    s, U, V, Z = p.get_pca()
+    p.neighbours(20)
+    idx, dist = p.get_neighbours(C)
+    ridx, rdist = orc.knn(Z, 20)
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+    nnz = p.snn(1 / 15, np.float32)
+    S = p.get_snn(C)
+    cp, rv, nz = orc.snn(ridx, 20, 1 / 15, np.float32)
+    assert nnz == len(rv)
+    assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.indices, rv - 1)
+    assert np.array_equal(S.data, nz)

@@ -1,0 +1,152 @@
+"""GPU parity: CenteredMatrix products and IRLBA against the oracle and
+LAPACK (the cases of test/test_irlba.jl), plus the pipeline PCA against an
+FP64 oracle of the centered Gram matrix.
+
+Tolerances: products rtol 1e-12 (FP64 vectors, different summation order);
+singular values within 1e-5 relative of LAPACK in FP32 form (1e-4 in BF16
+form); residual ||A'U - V S||_F/||A||_F below the requested tol; singular
+subspaces within 1e-3 rad (largest principal angle, sign-free)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def test_centered_mul_known_answers(sev, golden):
+    A = sp.csc_matrix(golden["cm_A"])
+    Cm = sev.CenteredMatrix(A, golden["cm_mu"])
+    r, y0 = golden["cm_r"], golden["cm_y0"]
+    assert np.allclose(Cm @ r, golden["cm_Qr"], rtol=1e-14)
+    y1 = Cm.mul(y0.copy(), r, 2.0, 1.0)
+    assert np.allclose(y1, golden["cm_y1"], rtol=1e-14)
+    assert np.allclose(Cm.rmatvec(y1), golden["cm_Qty1"], rtol=1e-14)
+    r1 = Cm.mul(r.copy(), y1, 2.0, 1.0, trans=True)
+    assert np.allclose(r1, golden["cm_r1"], rtol=1e-14)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_centered_mul_vs_oracle(sev, orc, dt):
+    rng = np.random.default_rng(1)
+    m, n = 5000, 300  # several row slabs
+    A = sp.random(m, n, 0.1, format="csc", random_state=1, dtype=np.float64).astype(dt)
+    mu = rng.random(n).astype(dt)
+    Cm = sev.CenteredMatrix(A, mu)
+    x = rng.standard_normal(n)
+    w = rng.standard_normal(m)
+    y = Cm @ x
+    z = Cm.rmatvec(w)
+    assert np.allclose(y, orc.centered_mul(m, A.indptr, A.indices, A.data, mu, x), rtol=1e-12,
+                       atol=1e-12)
+    assert np.allclose(z, orc.centered_mul(m, A.indptr, A.indices, A.data, mu, w, trans=True),
+                       rtol=1e-12, atol=1e-11)
+
+
+def _angle(U1, U2):
+    q1, _ = np.linalg.qr(U1)
+    q2, _ = np.linalg.qr(U2)
+    s = np.linalg.svd(q1.T @ q2, compute_uv=False)
+    return float(np.arccos(np.clip(s.min(), -1, 1)))
+
+
+@pytest.mark.parametrize("m,n,nu", [(50, 50, 20), (100, 50, 20)])
+def test_irlba_dense(sev, m, n, nu):
+    rng = np.random.default_rng(m)
+    X = rng.standard_normal((m, n))
+    S = sev.irlba(sp.csc_matrix(X), nu, tol=1e-5, init=rng.standard_normal(n))
+    ss = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(S.S, ss[:nu], rtol=1e-8)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
+
+
+def test_irlba_sparse_two(sev):
+    X = sp.random(2000, 400, 0.1, format="csc", random_state=7, dtype=np.float64)
+    S = sev.irlba(X, 2, tol=1e-9, init=np.random.default_rng(1).standard_normal(400))
+    ss = np.linalg.svd(X.toarray(), compute_uv=False)
+    assert np.allclose(S.S, ss[:2], rtol=1e-10)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X.toarray()) < 1e-9
+
+
+def test_irlba_centered_sparse(sev):
+    X = sp.random(2000, 400, 0.1, format="csc", random_state=3, dtype=np.float64)
+    mu = np.asarray(X.mean(axis=0)).ravel()
+    Q = X.toarray() - mu
+    S = sev.irlba(sev.CenteredMatrix(X, mu), 2, tol=1e-9,
+                  init=np.random.default_rng(0).standard_normal(400))
+    ss = np.linalg.svd(Q, compute_uv=False)
+    assert np.allclose(S.S, ss[:2], rtol=1e-10)
+    assert np.linalg.norm(Q.T @ S.U - S.V * S.S) / np.linalg.norm(Q) < 1e-9
+
+
+def test_irlba_centered_dense(sev):
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((20, 10))
+    mu = X.mean(axis=0)
+    Q = X - mu
+    S = sev.irlba(sev.CenteredMatrix(sp.csc_matrix(X), mu), 3, tol=1e-5,
+                  init=rng.standard_normal(10))
+    ss = np.linalg.svd(Q, compute_uv=False)
+    assert np.allclose(S.S, ss[:3], rtol=1e-8)
+    assert np.linalg.norm(Q.T @ S.U - S.V * S.S) / np.linalg.norm(Q) < 1e-9
+
+
+def test_irlba_count_matrix(sev):
+    rng = np.random.default_rng(11)
+    X = sp.random(3000, 2000, 0.01, format="csc", random_state=11,
+                  data_rvs=lambda k: rng.poisson(10, k).astype(float))
+    mu = np.asarray(X.mean(axis=1)).ravel()
+    At = sp.csc_matrix(X.T)
+    S = sev.irlba(sev.CenteredMatrix(At, mu), 10, init=rng.standard_normal(3000))
+    ss = np.linalg.svd(At.toarray() - mu[None, :], compute_uv=False)
+    assert np.allclose(S.S, ss[:10], rtol=1e-6)
+
+
+def test_irlba_maxit_raises(sev):
+    rng = np.random.default_rng(2)
+    X = sp.csc_matrix(rng.standard_normal((300, 100)))
+    with pytest.raises(sev.ConvergenceError):
+        sev.irlba(X, 20, tol=1e-14, maxit=1, init=rng.standard_normal(100))
+
+
+def test_irlba_deterministic(sev):
+    X = sp.random(5000, 300, 0.1, format="csc", random_state=5, dtype=np.float64)
+    init = np.random.default_rng(3).standard_normal(300)
+    a = sev.irlba(X, 10, init=init)
+    b = sev.irlba(X, 10, init=init)
+    assert np.array_equal(a.S, b.S) and np.array_equal(a.U, b.U)
+
+
+def _gram_oracle(B, mu):
+    """FP64 singular values / right vectors of the centered matrix from its
+    Gram matrix (src/scaling.jl:274-296 form), no Lanczos involved."""
+    Bd = B.astype(np.float64)
+    m = B.shape[0]
+    mu = mu.astype(np.float64)
+    q = np.asarray(Bd.sum(axis=0)).ravel()
+    G = (Bd.T @ Bd).toarray() - np.outer(mu, q) - np.outer(q, mu) + m * np.outer(mu, mu)
+    w, V = np.linalg.eigh(G)
+    order = np.argsort(w)[::-1]
+    return np.sqrt(np.maximum(w[order], 0)), V[:, order]
+
+
+@pytest.mark.parametrize("precision,tol_s", [("fp32", 1e-5), ("bf16", 1e-4)])
+def test_pipeline_pca_vs_gram_oracle(sev, precision, tol_s):
+    t = sev.nb_tables(5000, seed=0, density=0.05)
+    C = 5000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    p.normalize("lognormalize", 1e4)
+    feats = sev.top_mean_features(t, 1000)
+    p.select_scale(feats, 10.0)
+    B, mu = p.get_scaled()
+    it, nm = p.pca(30, init=np.random.default_rng(0).standard_normal(1000), precision=precision)
+    s, U, V, Z = p.get_pca()
+    sg, Vg = _gram_oracle(B, mu)
+    assert np.max(np.abs(s - sg[:30]) / sg[:30]) < tol_s
+    if precision == "fp32":
+        assert _angle(V[:, :20], Vg[:, :20]) < 1e-3
+    assert np.allclose(Z, (U * s).astype(np.float32), rtol=1e-6, atol=1e-6)
+    # residual invariant of test/test_irlba.jl on the FP64 centered matrix
+    Q = B.toarray().astype(np.float64) - mu.astype(np.float64)
+    res = np.linalg.norm(Q.T @ U - V * s) / np.linalg.norm(Q)
+    assert res < (1e-5 if precision == "fp32" else 1e-3)

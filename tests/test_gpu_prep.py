@@ -1,0 +1,110 @@
+"""GPU parity: synthetic counts, normalize_cells, feature gather and
+scale_features against the oracle (bit-exact; log1p within 1 ulp)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def ulp_diff(a, b):
+    a = np.ascontiguousarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.ascontiguousarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+@pytest.mark.parametrize("C,G,dens,boost", [(3000, 500, 0.05, 0.0), (4100, 300, 0.04, 0.05),
+                                            (1, 7, 0.5, 0.0)])
+def test_synth_bit_exact(sev, orc, C, G, dens, boost):
+    t = sev.nb_tables(G, seed=11, density=dens, boost_frac=boost)
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    X = p.get_counts()
+    cp, rv, cnt = orc.synth_csc(C, t)
+    assert np.array_equal(X.indptr, cp) and np.array_equal(X.indices, rv)
+    assert np.array_equal(X.data, cnt)
+
+
+@pytest.mark.parametrize("log", [True, False])
+def test_pipeline_normalize_vs_oracle(sev, orc, log):
+    t = sev.nb_tables(2000, seed=2, density=0.05)
+    C = 5000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    cp, rv, cnt = orc.synth_csc(C, t)
+    p.normalize("lognormalize" if log else "relativecounts", 1e4)
+    got = p.get_normalized()
+    ref = orc.normalize(C, cp, rv, cnt, 1e4, log, np.float32)
+    d = ulp_diff(got, ref)
+    if log:
+        assert d.max() <= 1 and (d > 0).mean() < 1e-5
+    else:
+        assert d.max() == 0
+    assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref))
+
+
+@pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
+@pytest.mark.parametrize("sf", [1.0, 10.0, 1e4])
+@pytest.mark.parametrize("log", [False, True])
+def test_normalize_cells_golden(sev, golden, dt, tag, sf, log):
+    X = sp.csc_matrix(golden["counts"])
+    Y = sev.normalize_cells(X, method="lognormalize" if log else "relativecounts",
+                            scale_factor=sf, dtype=dt)
+    exp = golden[f"norm_{tag}_sf{sf:g}_{'log' if log else 'rel'}"]
+    assert Y.dtype == dt
+    if dt == np.float32 and log:
+        assert ulp_diff(Y.data, exp).max() <= 1
+    else:
+        assert np.array_equal(Y.data, exp)
+
+
+def test_normalize_cells_errors(sev):
+    X = sp.csc_matrix(np.array([[1, 0], [0, 2]]))
+    with pytest.raises(ValueError):
+        sev.normalize_cells(X, method="bogus")
+
+
+def test_normalize_f64_vs_oracle(sev, orc):
+    t = sev.nb_tables(800, seed=5, density=0.08)
+    cp, rv, cnt = orc.synth_csc(1500, t)
+    X = sp.csc_matrix((cnt, rv, cp), shape=(1500, 800))
+    Y = sev.normalize_cells(X, scale_factor=1e4, dtype=np.float64)
+    ref = orc.normalize(1500, cp, rv, cnt, 1e4, True, np.float64)
+    assert np.allclose(Y.data, ref, rtol=2e-16, atol=0)
+
+
+@pytest.mark.parametrize("scale_max", [10.0, np.inf, 0.5])
+def test_pipeline_scale_bit_exact(sev, orc, scale_max):
+    t = sev.nb_tables(3000, seed=8, density=0.05)
+    C = 5000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    cp, rv, cnt = orc.synth_csc(C, t)
+    p.normalize("lognormalize", 1e4)
+    norm = p.get_normalized()
+    feats = sev.top_mean_features(t, 500)[::-1].copy()  # non-monotone order
+    p.select_scale(feats, scale_max)
+    B, mu = p.get_scaled()
+    fcp, frow, fval = orc.gather(feats, cp, rv, norm)
+    assert np.array_equal(B.indptr, fcp) and np.array_equal(B.indices, frow)
+    bref, muref, _ = orc.scale(C, fcp, fval, np.float32(scale_max))
+    assert np.array_equal(B.data, bref)
+    assert np.array_equal(mu, muref)
+
+
+def test_scale_features_golden(sev, golden):
+    X = sp.csc_matrix(golden["counts"].astype(np.float64))
+    S = sev.scale_features(X)
+    assert np.allclose(S.mu, golden["scale_mu_over_sd"], rtol=1e-13)
+    assert np.allclose(S.to_dense(), golden["scale_dense"], rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_scale_features_vs_oracle(sev, orc, dt):
+    rng = np.random.default_rng(3)
+    m, n = 20000, 40
+    D = (rng.random((m, n)) < 0.15) * rng.gamma(2.0, 1.0, (m, n))
+    X = sp.csc_matrix(D.astype(dt))
+    S = sev.scale_features(X, scale_max=4.0)
+    out, mu, _ = orc.scale(m, X.indptr, X.data, dt(4.0))
+    assert np.array_equal(S.A.data, out) and np.array_equal(S.mu, mu)

@@ -1,0 +1,352 @@
+"""Pins the CPU oracle against the reference's fixtures, closed forms, library
+routines (numpy/LAPACK) and brute force, so that the oracle can serve as the
+checker of the GPU path.  CPU only."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+
+# ---------------------------------------------------------------------------
+# synthetic generator
+# ---------------------------------------------------------------------------
+def test_u01_moments(orc):
+    u = np.array([orc.u01(5, 3, i) for i in range(20000)])
+    assert 0.0 <= u.min() and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.003
+
+
+def test_synth_matches_negative_binomial(sev, orc):
+    from scipy import stats
+    t = sev.nb_tables(40, seed=3, density=0.3)
+    C = 20000
+    colptr, rowval, counts = orc.synth_csc(C, t)
+    D = sp.csc_matrix((counts, rowval, colptr), shape=(C, 40)).toarray()
+    for g in range(40):
+        mu = t.size_factor * t.gene_mean[g]
+        p = t.r / (t.r + mu)
+        # zero fraction and mean against the NB(r, p) law
+        p0 = stats.nbinom.pmf(0, t.r, p)
+        assert abs((D[:, g] == 0).mean() - p0) < 5 * np.sqrt(p0 * (1 - p0) / C) + 1e-3
+        sd = np.sqrt(stats.nbinom.var(t.r, p) / C)
+        assert abs(D[:, g].mean() - mu) < 6 * sd + 1e-3
+    # achieved density close to the calibrated expectation
+    assert abs(len(counts) / (C * 40) - t.expected_density()) < 0.01
+
+
+def test_synth_boost_class(sev, orc):
+    t = sev.nb_tables(200, seed=1, density=0.04, boost_frac=0.05)
+    colptr, rowval, counts = orc.synth_csc(4000, t)
+    X = sp.csc_matrix((counts, rowval, colptr), shape=(4000, 200))
+    lib = np.asarray(X.sum(axis=1)).ravel()
+    cls = np.array([orc.u01(t.seed, 0x7FFFFFFF, c) < 0.05 for c in range(4000)])
+    assert 0.03 < cls.mean() < 0.07
+    ratio = lib[cls].mean() / lib[~cls].mean()
+    assert 2.5 < ratio < 3.5
+
+
+# ---------------------------------------------------------------------------
+# normalize_cells  (src/normalize.jl)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
+@pytest.mark.parametrize("sf", [1.0, 10.0, 1e4])
+@pytest.mark.parametrize("log", [False, True])
+def test_normalize_golden(orc, golden, dt, tag, sf, log):
+    out = orc.normalize(10, golden["colptr"], golden["rowval"], golden["nzval"], sf, log, dt)
+    exp = golden[f"norm_{tag}_sf{sf:g}_{'log' if log else 'rel'}"]
+    assert out.dtype == dt and np.array_equal(out, exp)
+
+
+def test_normalize_row_sum_invariants(orc, golden):
+    # test/test_input.jl:64,74: relative counts sum to the scale factor per cell
+    for sf in (1.0, 10.0):
+        out = orc.normalize(10, golden["colptr"], golden["rowval"], golden["nzval"], sf, False,
+                            np.float64)
+        X = sp.csc_matrix((out, golden["rowval"], golden["colptr"]), shape=(10, 5))
+        rs = np.asarray(X.sum(axis=1)).ravel()
+        nonempty = np.asarray((golden["counts"] != 0).any(axis=1))
+        assert np.allclose(rs[nonempty], sf, rtol=1e-14)
+    # log path is log1p of the relative counts (test/test_input.jl:70)
+    rel = orc.normalize(10, golden["colptr"], golden["rowval"], golden["nzval"], 1.0, False,
+                        np.float64)
+    lg = orc.normalize(10, golden["colptr"], golden["rowval"], golden["nzval"], 1.0, True,
+                       np.float64)
+    assert np.array_equal(lg, np.log1p(rel))
+
+
+def test_normalize_operation_order(orc):
+    """sf*x/s (left to right) differs from x/s*sf in Float32 on some inputs;
+    the oracle must follow the reference's order (src/normalize.jl:27)."""
+    rng = np.random.default_rng(0)
+    C, G = 400, 50
+    D = rng.integers(0, 40, size=(C, G)) * (rng.random((C, G)) < 0.3)
+    D[:, 0] += 1
+    X = sp.csc_matrix(D)
+    out = orc.normalize(C, X.indptr, X.indices, X.data, 1e4, False, np.float32)
+    tot = D.sum(axis=1).astype(np.float32)
+    rows = X.indices
+    x = X.data.astype(np.float32)
+    ref = (np.float32(1e4) * x) / tot[rows]
+    alt = (x / tot[rows]) * np.float32(1e4)
+    assert np.array_equal(out, ref)
+    assert not np.array_equal(out, alt)
+
+
+# ---------------------------------------------------------------------------
+# scale_features  (src/scaling.jl)
+# ---------------------------------------------------------------------------
+def test_scale_golden_closed_form(orc, golden):
+    # test/test_scaling.jl:41-44 on the 10x5 fixture (Float64)
+    vals = golden["nzval"].astype(np.float64)
+    out, mu, sd = orc.scale(10, golden["colptr"], vals)
+    assert np.allclose(mu, golden["scale_mu_over_sd"], rtol=1e-13)
+    B = sp.csc_matrix((out, golden["rowval"], golden["colptr"]), shape=(10, 5)).toarray()
+    assert np.allclose(B - mu[None, :], golden["scale_dense"], rtol=1e-12, atol=1e-13)
+
+
+def _welford_py(col_vals, m):
+    """Pure-Python Float32 restatement of src/scaling.jl:18-34 (small cases)."""
+    f = np.float32
+    count = m - len(col_vals)
+    mu = f(0.0)
+    s = f(0.0)
+    for v in col_vals:
+        v = f(v)
+        count += 1
+        delta = f(v - mu)
+        mu = f(mu + f(delta / f(count)))
+        s = f(s + f(delta * f(v - mu)))
+    var = f(s / f(m - 1))
+    return mu, f(np.sqrt(var))
+
+
+def test_scale_welford_independent_loop(orc):
+    rng = np.random.default_rng(4)
+    m, n = 3000, 6
+    D = (rng.random((m, n)) < 0.2) * rng.gamma(2.0, 1.5, (m, n))
+    X = sp.csc_matrix(D.astype(np.float32))
+    out, mu, sd = orc.scale(m, X.indptr, X.data, np.float32(10.0))
+    for j in range(n):
+        v = X.data[X.indptr[j]:X.indptr[j + 1]]
+        mu_j, sd_j = _welford_py(v, m)
+        assert sd[j] == sd_j
+        assert mu[j] == np.float32(mu_j / sd_j)
+        # close to the exact statistics (catches a wrong variance formula)
+        full = D[:, j].astype(np.float64)
+        assert abs(sd_j - full.std(ddof=1)) < 1e-5 * full.std(ddof=1)
+
+
+def test_scale_clip_upper_only(orc):
+    rng = np.random.default_rng(5)
+    m = 500
+    d = np.zeros(m, np.float32)
+    d[:40] = rng.gamma(1.0, 1.0, 40)
+    d[0] = 500.0  # an outlier to clip
+    X = sp.csc_matrix(d[:, None])
+    out, mu, sd = orc.scale(m, X.indptr, X.data, np.float32(3.0))
+    v = X.data / sd[0]
+    clip = np.float32(3.0) + mu[0]
+    assert np.array_equal(out, np.where(v > clip, clip, v))
+    assert out.max() == clip and (v > clip).sum() >= 1
+    out_inf, _, _ = orc.scale(m, X.indptr, X.data, np.float32(np.inf))
+    assert np.array_equal(out_inf, v)
+
+
+# ---------------------------------------------------------------------------
+# CenteredMatrix mul!  (src/scaling.jl:245-257, known answers of
+# test/test_scaling.jl:70-113)
+# ---------------------------------------------------------------------------
+def test_centered_mul_known_answers(orc, golden):
+    A = sp.csc_matrix(golden["cm_A"])
+    mu, r, y0 = golden["cm_mu"], golden["cm_r"], golden["cm_y0"]
+    Qr = orc.centered_mul(4, A.indptr, A.indices, A.data, mu, r)
+    assert np.allclose(Qr, golden["cm_Qr"], rtol=1e-14)
+    y1 = orc.centered_mul(4, A.indptr, A.indices, A.data, mu, r, y=y0, alpha=2.0, beta=1.0)
+    assert np.allclose(y1, golden["cm_y1"], rtol=1e-14)
+    Qty = orc.centered_mul(4, A.indptr, A.indices, A.data, mu, y1, trans=True)
+    assert np.allclose(Qty, golden["cm_Qty1"], rtol=1e-14)
+    r1 = orc.centered_mul(4, A.indptr, A.indices, A.data, mu, y1, trans=True, y=r, alpha=2.0,
+                          beta=1.0)
+    assert np.allclose(r1, golden["cm_r1"], rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# small SVD and IRLBA  (test/test_irlba.jl cases against LAPACK)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,rank", [(7, 7), (30, 30), (20, 12)])
+def test_svd_jacobi_vs_lapack(orc, n, rank):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, rank)) @ rng.standard_normal((rank, n))
+    U, s, V = orc.svd_jacobi(A)
+    ref = np.linalg.svd(A, compute_uv=False)
+    assert np.allclose(s, ref, rtol=1e-12, atol=1e-12 * ref[0])
+    assert np.allclose(U @ np.diag(s) @ V.T, A, atol=1e-11 * ref[0])
+    assert np.allclose(U.T @ U, np.eye(n), atol=1e-12)
+
+
+def _rel_err(X, U, s, V):
+    return np.linalg.norm(X - (U * s) @ V.T)
+
+
+@pytest.mark.parametrize("m,n,nu,tol", [(50, 50, 20, 1e-5), (100, 50, 20, 1e-5)])
+def test_irlba_dense(orc, m, n, nu, tol):
+    rng = np.random.default_rng(m + n)
+    X = rng.standard_normal((m, n))
+    rc, s, U, V, it, nm = orc.irlba_dense(X, nu, rng.standard_normal(n), tol=tol)
+    assert rc == 0
+    ss = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(s, ss[:nu], rtol=1e-8)
+    assert np.linalg.norm(X.T @ U - V * s) / np.linalg.norm(X) < tol
+    Us, sv, Vt = np.linalg.svd(X)
+    assert np.isclose(_rel_err(X, U, s, V), _rel_err(X, Us[:, :nu], sv[:nu], Vt[:nu].T),
+                      rtol=1e-6)
+
+
+def test_irlba_centered_sparse(orc):
+    A = sp.random(2000, 400, 0.1, format="csc", random_state=3, dtype=np.float64)
+    mu = np.asarray(A.mean(axis=0)).ravel()
+    Q = A.toarray() - mu
+    rng = np.random.default_rng(0)
+    rc, s, U, V, it, nm = orc.irlba_csc(2000, A.indptr, A.indices, A.data, mu, 2,
+                                        rng.standard_normal(400), tol=1e-9)
+    assert rc == 0
+    ss = np.linalg.svd(Q, compute_uv=False)
+    assert np.allclose(s, ss[:2], rtol=1e-10)
+    assert np.linalg.norm(Q.T @ U - V * s) / np.linalg.norm(Q) < 1e-9
+
+
+def test_irlba_tall_skinny_transpose(orc):
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((10000, 10))
+    rc1, s1, U1, V1, _, _ = orc.irlba_dense(X, 2, rng.standard_normal(10), tol=1e-9)
+    rc2, s2, U2, V2, _, _ = orc.irlba_dense(X.T.copy(), 2, rng.standard_normal(10000), tol=1e-9)
+    assert rc1 == 0 and rc2 == 0
+    assert np.allclose(s1, s2, rtol=1e-10)
+    # svd_flip! (src/utils.jl:215-228): align signs, then compare vectors
+    for j in range(2):
+        a = U1[:, j] * np.sign(U1[np.argmax(np.abs(U1[:, j])), j])
+        b = V2[:, j] * np.sign(V2[np.argmax(np.abs(V2[:, j])), j])
+        assert np.allclose(a, b, atol=1e-7)
+
+
+def test_irlba_count_matrix(orc):
+    # test/test_irlba.jl:107-115: Poisson(10) counts, centered by row means,
+    # transposed operator
+    rng = np.random.default_rng(11)
+    X = sp.random(3000, 2000, 0.01, format="csc", random_state=11,
+                  data_rvs=lambda k: rng.poisson(10, k).astype(float))
+    mu = np.asarray(X.mean(axis=1)).ravel()
+    At = sp.csc_matrix(X.T)
+    rc, s, U, V, it, nm = orc.irlba_csc(2000, At.indptr, At.indices, At.data, mu, 10,
+                                        rng.standard_normal(3000))
+    assert rc == 0
+    ss = np.linalg.svd(At.toarray() - mu[None, :], compute_uv=False)
+    assert np.allclose(s, ss[:10], rtol=1e-6)
+
+
+def test_irlba_mutation_is_caught(orc):
+    """A dropped centering term (planted in the oracle's input, on the CPU)
+    is detected by the LAPACK comparison above."""
+    A = sp.random(500, 80, 0.2, format="csc", random_state=2, dtype=np.float64)
+    mu = np.asarray(A.mean(axis=0)).ravel()
+    Q = A.toarray() - mu
+    rc, s, U, V, _, _ = orc.irlba_csc(500, A.indptr, A.indices, A.data, None, 3,
+                                      np.ones(80))
+    ss = np.linalg.svd(Q, compute_uv=False)[:3]
+    assert not np.allclose(s, ss, rtol=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# exact kNN
+# ---------------------------------------------------------------------------
+def _knn_numpy(Z, k, include_self=True):
+    """Independent restatement: float32 sequential-t distances, stable argsort."""
+    n, d = Z.shape
+    D = np.zeros((n, n), Z.dtype)
+    for t in range(d):
+        diff = Z[:, None, t] - Z[None, :, t]
+        D = D + diff * diff
+    if not include_self:
+        D[np.arange(n), np.arange(n)] = np.inf
+    idx = np.argsort(D, axis=1, kind="stable")[:, :k]
+    return idx + 1, np.sqrt(np.take_along_axis(D, idx, 1))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("include_self", [True, False])
+def test_knn_bruteforce(orc, dtype, include_self):
+    rng = np.random.default_rng(12)
+    Z = rng.random((300, 10)).astype(dtype)
+    idx, dist = orc.knn(Z, 7, include_self=include_self)
+    ridx, rdist = _knn_numpy(Z, 7, include_self)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist.astype(dtype))
+
+
+def test_knn_ties_lower_index_wins(orc):
+    rng = np.random.default_rng(13)
+    base = rng.random((40, 5)).astype(np.float32)
+    Z = np.concatenate([base, base, base[:10]])  # exact duplicate points
+    idx, dist = orc.knn(Z, 4)
+    ridx, _ = _knn_numpy(Z, 4)
+    assert np.array_equal(idx, ridx)
+    # duplicates at distance 0: the lowest index comes first
+    assert idx[40, 0] == 1 and idx[40, 1] == 41 and idx[80, 0] == 1
+
+
+def test_knn_reference_recall_property(orc):
+    # test/test_nn.jl:31-38: 30th percentile Jaccard recall vs brute force == 1
+    rng = np.random.default_rng(14)
+    X = rng.random((100, 10))
+    idx, _ = orc.knn(X, 4)
+    D = ((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)
+    j = []
+    for i in range(100):
+        nn = set(np.argsort(D[:, i], kind="stable")[:4].tolist())
+        x = len(nn & set((idx[i] - 1).tolist()))
+        j.append(x / (8 - x))
+    assert np.quantile(j, 0.3) == 1.0
+
+
+# ---------------------------------------------------------------------------
+# Jaccard SNN
+# ---------------------------------------------------------------------------
+def _snn_sets(idx, k, prune, dt):
+    n = idx.shape[0]
+    N = [set(r) for r in idx.tolist()]
+    cols = []
+    for j in range(n):
+        rows, vals = [], []
+        for i in range(n):
+            x = len(N[i] & N[j])
+            if x == 0:
+                continue
+            xf = dt(x)
+            w = xf / (dt(k) + (dt(k) - xf))
+            if abs(w) > dt(prune):
+                rows.append(i + 1)
+                vals.append(w)
+        cols.append((rows, vals))
+    return cols
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("k,prune", [(5, 1 / 15), (4, 0.0), (6, 0.3)])
+def test_snn_sets(orc, dt, k, prune):
+    rng = np.random.default_rng(k)
+    Z = rng.random((150, 6)).astype(np.float32)
+    idx, _ = orc.knn(Z, k)
+    cp, rv, nz = orc.snn(idx, k, prune, dt)
+    cols = _snn_sets(idx, k, prune, dt)
+    for j, (rows, vals) in enumerate(cols):
+        assert list(rv[cp[j] - 1:cp[j + 1] - 1]) == rows
+        assert np.array_equal(nz[cp[j] - 1:cp[j + 1] - 1], np.array(vals, dt))
+
+
+def test_snn_symmetric_with_unit_diagonal(orc):
+    rng = np.random.default_rng(3)
+    Z = rng.random((400, 8)).astype(np.float32)
+    idx, _ = orc.knn(Z, 10)
+    cp, rv, nz = orc.snn(idx, 10, 1 / 15)
+    S = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(400, 400))
+    assert abs(S - S.T).max() == 0
+    assert np.all(S.diagonal() == 1.0)
