@@ -1,0 +1,182 @@
+"""Throughput of Severo's embedding-and-graph path on one MI355X.
+
+A step is one pass of normalize_cells -> Y[:, features] -> scale_features
+(clip 10) -> PCA (IRLBA, 50 components, tol 1e-5) -> exact kNN (k = 20) ->
+Jaccard SNN (prune 1/15) over a synthetic negative-binomial count matrix that
+is generated on the device before timing starts (inputs resident in HBM).
+
+Workloads (cells x genes, density; features = the 2000 genes with the
+largest generator mean):
+  small : 5,000 x 20,000, 5%
+  100k  : 100,000 x 33,538, 4%, 5% of cells with 3x library size
+  1m    : 1,000,000 x 27,998, 3%   (default: the quoted metric's workload)
+
+The headline value is the precision form (BF16 operator values for the
+Lanczos products, BF16 candidate search for kNN with exact FP32 re-rank);
+the plain FP32 form is timed in the same run.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__ as entry  # noqa: E402
+
+WORKLOADS = {
+    "small": dict(cells=5000, genes=20000, density=0.05, boost=0.0),
+    "100k": dict(cells=100_000, genes=33538, density=0.04, boost=0.05),
+    "1m": dict(cells=1_000_000, genes=27998, density=0.03, boost=0.0),
+}
+NFEAT, NPCS, K, PRUNE, SCALE_MAX, SF = 2000, 50, 20, 1.0 / 15, 10.0, 1e4
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0}
+
+
+def run_step(p, feats, init, form):
+    p.normalize("lognormalize", SF)
+    p.select_scale(feats, SCALE_MAX)
+    it, nm = p.pca(NPCS, init=init, tol=1e-5, precision=form)
+    nfb = p.neighbours(K, include_self=True, precision=form)
+    nnz = p.snn(PRUNE, np.float32)
+    p.synchronize()
+    return it, nm, nfb, nnz
+
+
+def time_form(p, feats, init, form, steps, warmup, profile):
+    for _ in range(warmup):
+        run_step(p, feats, init, form)
+    if profile:
+        p.profile(True)
+    p.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, nm, nfb, nnz = run_step(p, feats, init, form)
+    p.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    prof = p.profile_read() if profile else {}
+    if profile:
+        p.profile(False)
+    return dt, dict(irlba_iters=it, irlba_nmult=nm, knn_fallback_rows=nfb, snn_nnz=nnz), prof
+
+
+def roofline(prof, form, steps):
+    if not prof:
+        return None
+    name, (launches, ms, by, fl) = max(prof.items(), key=lambda kv: kv[1][1])
+    if fl > 0:
+        achieved = fl / (ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS[form if form in PEAK_TFLOPS else "fp32"]
+        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "launches_per_step": launches // steps,
+                "avg_launch_ms": round(ms / launches, 4)}
+    achieved = by / (ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "launches_per_step": launches // steps, "avg_launch_ms": round(ms / launches, 4)}
+
+
+def cpu_baseline(sev, orc, w, sample_cells):
+    """The oracle's scalar C restatement of the same path on a bounded
+    sample of the workload (same generator tables, first `sample_cells`
+    cells' worth of draws), single thread."""
+    t = sev.nb_tables(w["genes"], seed=0, density=w["density"], boost_frac=w["boost"])
+    cp, rv, cnt = orc.synth_csc(sample_cells, t)
+    feats = sev.top_mean_features(t, NFEAT)
+    init = np.random.default_rng(0).standard_normal(NFEAT)
+    t0 = time.perf_counter()
+    norm = orc.normalize(sample_cells, cp, rv, cnt, SF, True, np.float32)
+    fcp, frow, fval = orc.gather(feats, cp, rv, norm)
+    b, mu, _ = orc.scale(sample_cells, fcp, fval, np.float32(SCALE_MAX))
+    rc, s, U, V, it, nm = orc.irlba_csc(sample_cells, fcp, frow, b, mu, NPCS, init, tol=1e-5)
+    Z = np.asfortranarray((U * s).astype(np.float32))
+    idx, _ = orc.knn(Z, K, nthreads=1)
+    orc.snn(idx, K, PRUNE, np.float32)
+    dt = time.perf_counter() - t0
+    return {"value": round(sample_cells / dt, 2), "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"{sample_cells} cells of the same generator and genes, whole path, "
+                      f"{dt:.1f} s (kNN is O(cells^2), so cells/s falls further at 1M cells)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--forms", default="bf16,fp32")
+    ap.add_argument("--cpu-sample", type=int, default=8000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the path "
+                          "is described in DESIGN.md but not implemented", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    sev = entry.load_package()
+    sev.load()
+    w = WORKLOADS[args.workload]
+    t = sev.nb_tables(w["genes"], seed=0, density=w["density"], boost_frac=w["boost"])
+    p = sev.Pipeline()
+    t0 = time.perf_counter()
+    p.synth_counts(w["cells"], t)
+    m, g, nnz = p.counts_info()
+    gen_s = time.perf_counter() - t0
+    feats = sev.top_mean_features(t, NFEAT)
+    init = np.random.default_rng(0).standard_normal(NFEAT)
+    print(f"# generated {m} x {g} counts, nnz {nnz} ({nnz / (m * g):.4f}) in {gen_s:.1f} s",
+          file=sys.stderr, flush=True)
+
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    for form in forms:
+        dt, info, prof = time_form(p, feats, init, form, args.steps, args.warmup, True)
+        res[form] = (dt, info, prof)
+        print(f"# {form}: {dt * 1e3:.1f} ms/step, {m / dt:.0f} cells/s, {info}", file=sys.stderr,
+              flush=True)
+        for name, (l, ms, by, fl) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            print(f"#   {name:14s} {l // args.steps:6d} launches/step {ms / args.steps:9.2f} ms/step"
+                  f"  {by / (ms * 1e-3) / 1e9 if ms > 0 and by > 0 else 0:8.1f} GB/s"
+                  f"  {fl / (ms * 1e-3) / 1e12 if ms > 0 and fl > 0 else 0:7.2f} TF/s",
+                  file=sys.stderr, flush=True)
+    head = forms[0]
+    dt, info, prof = res[head]
+    out = {
+        "metric": "cells/s normalise->PCA50->kNN20->SNN, BF16 vs FP32 form, 1M cells",
+        "value": round(m / dt, 2),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic negative-binomial counts generated on the device",
+        "config": {"workload": f"{args.workload}: {m} cells x {g} genes, nnz {nnz}, "
+                               f"{NFEAT} features, {NPCS} PCs, k={K}, prune 1/15, form {head}",
+                   "cells": m, "genes": g, "nnz": nnz, "features": NFEAT, "npcs": NPCS, "k": K,
+                   "form": head},
+        "roofline": roofline(prof, head, args.steps),
+        "forms": {f: {"cells_per_s": round(m / res[f][0], 2), "ms_per_step": round(res[f][0] * 1e3, 3),
+                      **res[f][1]} for f in forms},
+    }
+    if "fp32" in res and "bf16" in res:
+        out["bf16_speedup_vs_fp32"] = round(res["fp32"][0] / res["bf16"][0], 3)
+    if not args.no_cpu:
+        orc = entry.load_oracle()
+        out["cpu_baseline"] = cpu_baseline(sev, orc, w, args.cpu_sample)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

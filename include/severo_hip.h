@@ -180,6 +180,16 @@ int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *dista
 int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz);
 int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void *nzval);
 
+/* Kernel timing (no counterpart in the reference, which has no tracing):
+ * with enable != 0 every kernel launched by this handle is bracketed by HIP
+ * events on the handle's stream; enabling again clears the records.
+ * sev_pipeline_profile_read synchronises the stream and returns up to max
+ * aggregated entries: name (32 bytes each, NUL-terminated), launches, total
+ * milliseconds, and the algorithmic bytes and flops of those launches. */
+int sev_pipeline_profile(sev_pipeline *p, int enable);
+int sev_pipeline_profile_read(sev_pipeline *p, int max, char *names, int64_t *launches,
+                              double *ms, double *bytes, double *flops, int *count);
+
 #ifdef __cplusplus
 }
 #endif

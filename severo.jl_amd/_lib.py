@@ -53,6 +53,8 @@ SIGNATURES = {
     "sev_pipeline_get_neighbours": (_int, [_p, _p, _p]),
     "sev_pipeline_snn": (_int, [_p, _dbl, _int, _pi64]),
     "sev_pipeline_get_snn": (_int, [_p, _p, _p, _p]),
+    "sev_pipeline_profile": (_int, [_p, _int]),
+    "sev_pipeline_profile_read": (_int, [_p, _int, _p, _p, _p, _p, _p, C.POINTER(_int)]),
 }
 
 _LIB = None

@@ -19,9 +19,18 @@
 
 using namespace sev;
 
+thread_local Prof *sev::t_prof = nullptr;
+
 namespace {
 
 thread_local std::string g_err;
+
+// Installs the handle's profiler for the duration of an entry point.
+struct ProfInstall {
+    Prof *prev;
+    explicit ProfInstall(Prof *p) : prev(t_prof) { t_prof = p; }
+    ~ProfInstall() { t_prof = prev; }
+};
 
 template <class F>
 int guarded(F &&f) {
@@ -134,6 +143,7 @@ int snn_table(int64_t k, double prune, std::vector<T> &w) {
 // ---------------------------------------------------------------------------
 struct sev_pipeline {
     Stream stream;
+    Prof prof;
     // counts (cells x genes CSC)
     int64_t m = 0, g = 0, nnz = 0;
     DevBuf<int64_t> cp;
@@ -267,6 +277,7 @@ int sev_pipeline_synchronize(sev_pipeline *p) {
 int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_t *colptr,
                              const int64_t *rowval, const int32_t *nzval) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         hipStream_t st = p->st();
         csc_to_device(m, n, colptr, rowval, p->cp, p->row, p->nnz, st);
         upload(p->cnt, nzval, (size_t)p->nnz, st);
@@ -281,6 +292,7 @@ int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_
 int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t seed, double r,
                               const double *p0, const double *q, double boost_frac) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(m > 0 && n > 0 && m < (int64_t)0x7FFFFFFF && n < (int64_t)0x7FFFFFFF,
                     "bad synthetic shape");
         hipStream_t st = p->st();
@@ -311,6 +323,7 @@ int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t se
 
 int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *nnz) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_counts, "no counts loaded");
         if (m) *m = p->m;
         if (n) *n = p->g;
@@ -320,6 +333,7 @@ int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *n
 
 int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, int32_t *nzval) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_counts, "no counts loaded");
         hipStream_t st = p->st();
         std::vector<int32_t> rv((size_t)p->nnz);
@@ -336,6 +350,7 @@ int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, i
 
 int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_counts, "no counts loaded");
         SEV_REQUIRE(method == SEV_LOGNORMALIZE || method == SEV_RELATIVECOUNTS,
                     "unknown normalization method");
@@ -352,6 +367,7 @@ int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor) {
 
 int sev_pipeline_get_normalized(sev_pipeline *p, float *nzval) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_norm, "normalize has not run");
         download(nzval, p->norm.get(), (size_t)p->nnz, p->st());
         SEV_HIP(hipStreamSynchronize(p->st()));
@@ -361,6 +377,7 @@ int sev_pipeline_get_normalized(sev_pipeline *p, float *nzval) {
 int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *features,
                               double scale_max) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_norm, "normalize has not run");
         SEV_REQUIRE(nfeat >= 1, "need at least one feature");
         hipStream_t st = p->st();
@@ -399,6 +416,7 @@ int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *fea
 
 int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_scaled, "scale has not run");
         *nnz = p->fnnz;
     });
@@ -407,6 +425,7 @@ int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz) {
 int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, float *nzval,
                             float *mu) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_scaled, "scale has not run");
         hipStream_t st = p->st();
         std::vector<int32_t> rv((size_t)p->fnnz);
@@ -425,6 +444,7 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
 int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, const double *init,
                      int precision, int64_t *iters, int64_t *nmult) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_scaled, "scale has not run");
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
         hipStream_t st = p->st();
@@ -459,6 +479,7 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, c
 
 int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float *Z) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_pca, "pca has not run");
         hipStream_t st = p->st();
         if (s) std::memcpy(s, p->s.data(), sizeof(double) * (size_t)p->npcs);
@@ -471,6 +492,7 @@ int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float
 
 int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const float *Z, int64_t ld) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(m >= 1 && d >= 1 && ld >= m, "bad embedding shape");
         hipStream_t st = p->st();
         p->Z.ensure((size_t)(m * d));
@@ -487,6 +509,7 @@ int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const floa
 int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
                             int precision, int64_t *n_fallback) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_embedding, "no embedding");
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
         const int64_t m = p->m;
@@ -506,6 +529,7 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
 
 int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_nn, "neighbours have not run");
         hipStream_t st = p->st();
         const int64_t m = p->m, k = p->k;
@@ -524,6 +548,7 @@ int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *dista
 
 int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_nn, "neighbours have not run");
         SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
         hipStream_t st = p->st();
@@ -556,8 +581,37 @@ int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz) {
     });
 }
 
+int sev_pipeline_profile(sev_pipeline *p, int enable) {
+    return guarded([&] {
+        SEV_HIP(hipStreamSynchronize(p->st()));
+        p->prof.reset();
+        p->prof.on = enable != 0;
+    });
+}
+
+int sev_pipeline_profile_read(sev_pipeline *p, int max, char *names, int64_t *launches,
+                              double *ms, double *bytes, double *flops, int *count) {
+    return guarded([&] {
+        SEV_HIP(hipStreamSynchronize(p->st()));
+        auto agg = p->prof.aggregate();
+        const int n = (int)std::min<size_t>(agg.size(), (size_t)(max > 0 ? max : 0));
+        for (int i = 0; i < n; ++i) {
+            if (names) {
+                std::memset(names + 32 * i, 0, 32);
+                std::strncpy(names + 32 * i, agg[(size_t)i].name.c_str(), 31);
+            }
+            if (launches) launches[i] = agg[(size_t)i].launches;
+            if (ms) ms[i] = agg[(size_t)i].ms;
+            if (bytes) bytes[i] = agg[(size_t)i].bytes;
+            if (flops) flops[i] = agg[(size_t)i].flops;
+        }
+        if (count) *count = (int)agg.size();
+    });
+}
+
 int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void *nzval) {
     return guarded([&] {
+        ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_snn, "snn has not run");
         hipStream_t st = p->st();
         const int64_t m = p->m, nnz = p->snn_nnz;

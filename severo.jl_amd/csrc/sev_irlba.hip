@@ -433,6 +433,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     op.mu64 = src_f64;
     op.vals = val;
     op.vtype = src_f64 ? kValF64 : kValF32;
+    ProfScope ps("slab_build", st, 6.0 * op.nnz + 8.0 * (op.nslab + 1) * n);
     hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, op.nslab,
                        colptr, row, op.off.get(), op.lrow.get());
     SEV_LAUNCH_CHECK();
@@ -451,9 +452,15 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
 namespace {
 
 
+int val_bytes(const SlabCsc &op) {
+    return op.vtype == kValBF16 ? 2 : (op.vtype == kValF64 ? 8 : 4);
+}
+
 template <class VT, class MT>
 void launch_av_t(const SlabCsc &op, size_t smem, int use_lds, const double *v, const double *rf,
                  const double *wprev, double *y, hipStream_t st) {
+    ProfScope ps("spmv_av", st,
+                 (double)op.nnz * (val_bytes(op) + 2) + 8.0 * op.m * (wprev ? 2 : 1) + 8.0 * op.n);
     hipLaunchKernelGGL((k_av<VT, MT>), dim3((unsigned)op.nslab), dim3(kBlock), smem, st, op.m, op.n,
                        op.off.get(), op.lrow.get(), (const VT *)op.vals, (const MT *)op.mu, v, rf,
                        wprev, y, use_lds);
@@ -472,6 +479,9 @@ void launch_av(const SlabCsc &op, size_t smem, int use_lds, const double *v, con
 
 void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T, double *y,
                     double *part, hipStream_t st) {
+    ProfScope ps("spmv_atw", st,
+                 (double)op.nnz * (val_bytes(op) + 2) + 8.0 * op.m * (jc > 0 ? jc + 2 : 1) +
+                     8.0 * op.n);
     if (op.vtype == kValBF16)
         hipLaunchKernelGGL(k_upd_atw<uint16_t>, dim3((unsigned)op.nslab), dim3(kBlock), 0, st,
                            op.m, op.n, op.off.get(), op.lrow.get(), (const uint16_t *)op.vals, W,
@@ -490,6 +500,7 @@ void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T,
 void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, double *V,
                        double *Fv, double *B, double *scal, int *flags, uint64_t seed,
                        hipStream_t st) {
+    ProfScope ps("right_step", st, 16.0 * op.n * (j + 1));
     if (op.mu64)
         hipLaunchKernelGGL(k_right_step<double>, dim3(1), dim3(1024), 0, st, op.n, j, work, part2,
                            (const double *)op.mu, V, Fv, B, scal, flags, seed);
@@ -568,6 +579,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             // reorthogonalise against W[:,0:j]
             const int jc = j;
             if (jc > 0) {
+                ProfScope ps("reorth_dot", st, 8.0 * m * (jc + 1));
                 hipLaunchKernelGGL(k_dot_w, dim3(dot_blocks), dim3(kBlock), 0, st, m, Wd, jc, y,
                                    w.red.get());
                 SEV_LAUNCH_CHECK();
@@ -577,11 +589,16 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             }
             launch_upd_atw(op, Wd, jc, w.T.get(), y, w.part.get(), st);
             nmult++;
-            hipLaunchKernelGGL(k_red_slabs, dim3((unsigned)ceil_div(n + 2, kBlock), kRedChunks),
-                               dim3(kBlock), 0, st, op.nslab, n + 2, w.part.get(), w.part2.get());
-            SEV_LAUNCH_CHECK();
+            {
+                ProfScope ps("reduce_slabs", st, 8.0 * op.nslab * (n + 2));
+                hipLaunchKernelGGL(k_red_slabs, dim3((unsigned)ceil_div(n + 2, kBlock), kRedChunks),
+                                   dim3(kBlock), 0, st, op.nslab, n + 2, w.part.get(),
+                                   w.part2.get());
+                SEV_LAUNCH_CHECK();
+            }
             launch_right_step(op, j, work, w.part2.get(), Vd, w.Fv.get(), w.B.get(),
                               w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u, st);
+            ProfScope ps("scale_col", st, 16.0 * m);
             hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st, y, m,
                                w.scal.get());
             SEV_LAUNCH_CHECK();
@@ -625,6 +642,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                            w.V.get(), w.BV.get(), (int)k, w.V2.get(), w.Fv.get(), w.scal.get());
         SEV_LAUNCH_CHECK();
         w.V.swap(w.V2);
+        ProfScope ps("restart_w", st, 8.0 * m * (work + k), 2.0 * m * work * k);
         hipLaunchKernelGGL(k_gemm_small, dim3(grid_rows(m)), dim3(kBlock), 0, st, m, work,
                            w.W.get(), w.BU.get(), (int)k, w.W2.get(), (const double *)nullptr,
                            w.scal.get());

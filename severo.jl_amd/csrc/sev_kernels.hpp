@@ -4,6 +4,7 @@
 #include <hip/hip_bf16.h>
 
 #include "sev_common.hpp"
+#include "sev_prof.hpp"
 
 namespace sev {
 

@@ -156,6 +156,7 @@ void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int incl
     const size_t smem = sizeof(T) * (size_t)QB * (size_t)d;
     SEV_REQUIRE(smem <= 16384, "knn: too many dimensions for the exact kernel");
     if (nq == 0) return;
+    ProfScope ps("knn_exact", st, 0.0, 2.0 * (double)nq * (double)n * (double)d);
     hipLaunchKernelGGL((k_knn_exact<T, QB>), dim3((unsigned)ceil_div(nq, QB)), dim3(kBlock), smem,
                        st, X, n, d, ld, (int)k, include_self, qidx, nq, idx, dist);
     SEV_LAUNCH_CHECK();

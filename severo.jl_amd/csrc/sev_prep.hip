@@ -276,6 +276,7 @@ void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t nce
                 uint32_t *tot, hipStream_t st) {
     SEV_HIP(hipMemsetAsync(tot, 0, sizeof(uint32_t) * (size_t)(ncells > 0 ? ncells : 1), st));
     if (nnz == 0) return;
+    ProfScope ps("row_totals", st, 8.0 * nnz + 4.0 * ncells);
     hipLaunchKernelGGL(k_row_totals, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, nnz,
                        tot);
     SEV_LAUNCH_CHECK();
@@ -284,6 +285,7 @@ void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t nce
 void lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
                  float sf, int dolog, float *out, hipStream_t st) {
     if (nnz == 0) return;
+    ProfScope ps("lognorm", st, 12.0 * nnz);
     hipLaunchKernelGGL(k_lognorm_f32, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, tot,
                        nnz, sf, dolog, out);
     SEV_LAUNCH_CHECK();
@@ -292,6 +294,7 @@ void lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, in
 void lognorm_f64(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
                  double sf, int dolog, double *out, hipStream_t st) {
     if (nnz == 0) return;
+    ProfScope ps("lognorm", st, 16.0 * nnz);
     hipLaunchKernelGGL(k_lognorm_f64, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, tot,
                        nnz, sf, dolog, out);
     SEV_LAUNCH_CHECK();
@@ -309,6 +312,7 @@ template <class T>
 void gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, const T *val,
                  int64_t nfeat, const int64_t *fcolptr, int32_t *orow, T *oval, hipStream_t st) {
     if (nfeat == 0) return;
+    ProfScope ps("gather", st);
     hipLaunchKernelGGL(k_gather_copy<T>, dim3((unsigned)nfeat), dim3(kBlock), 0, st, feat, colptr,
                        row, val, fcolptr, orow, oval);
     SEV_LAUNCH_CHECK();
@@ -319,6 +323,7 @@ void scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T s
                  T *mup, T *clip, hipStream_t st) {
     if (ncol == 0) return;
     const int64_t threads = ncol * kWave;
+    ProfScope ps("scale_stats", st);
     hipLaunchKernelGGL(k_scale_stats<T>, dim3((unsigned)ceil_div(threads, kBlock)), dim3(kBlock),
                        0, st, m, ncol, colptr, x, scale_max, sd, mup, clip);
     SEV_LAUNCH_CHECK();
@@ -328,6 +333,7 @@ template <class T>
 void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, const T *clip,
                T *out, hipStream_t st) {
     if (ncol == 0) return;
+    ProfScope ps("scale_map", st);
     hipLaunchKernelGGL(k_scale_map<T>, dim3((unsigned)ncol), dim3(kBlock), 0, st, colptr, x, sd,
                        clip, out);
     SEV_LAUNCH_CHECK();

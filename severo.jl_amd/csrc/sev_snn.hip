@@ -348,6 +348,7 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
                        w.rptr.get(), w.cur.get(), w.rev.get());
     SEV_LAUNCH_CHECK();
     const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
+    ProfScope ps("snn_count", st);
     hipLaunchKernelGGL(k_snn_wave<T>, dim3(wg), dim3(kBlock), 0, st, nn, n, (int)k, xmin,
                        w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(), w.heavyP.get(),
                        w.nheavy.get(), (const T *)nullptr, (const int64_t *)nullptr, (int32_t *)nullptr,
@@ -392,6 +393,7 @@ template <class T>
 void snn_fill(const int32_t *nn, int64_t n, int64_t k, int xmin, const T *wtab, SnnWork &w,
               const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
     const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
+    ProfScope ps("snn_fill", st);
     hipLaunchKernelGGL(k_snn_wave<T>, dim3(wg), dim3(kBlock), 0, st, nn, n, (int)k, xmin,
                        w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(), w.heavyP.get(),
                        w.nheavy.get(), wtab, colptr, rows, vals);

@@ -46,6 +46,28 @@ class Pipeline:
     def synchronize(self):
         check(self._L.sev_pipeline_synchronize(self._h))
 
+    def profile(self, enable=True):
+        """Bracket every kernel of this handle with HIP events (clears the
+        previous records)."""
+        check(self._L.sev_pipeline_profile(self._h, int(enable)))
+
+    def profile_read(self, max_entries=64):
+        """{name: (launches, total_ms, algorithmic_bytes, algorithmic_flops)}"""
+        names = C.create_string_buffer(32 * max_entries)
+        launches = np.zeros(max_entries, np.int64)
+        ms = np.zeros(max_entries)
+        by = np.zeros(max_entries)
+        fl = np.zeros(max_entries)
+        cnt = C.c_int()
+        check(self._L.sev_pipeline_profile_read(self._h, max_entries, names, _ptr(launches),
+                                                _ptr(ms), _ptr(by), _ptr(fl), C.byref(cnt)))
+        raw = names.raw
+        out = {}
+        for i in range(min(cnt.value, max_entries)):
+            name = raw[32 * i:32 * i + 32].split(b"\0", 1)[0].decode()
+            out[name] = (int(launches[i]), float(ms[i]), float(by[i]), float(fl[i]))
+        return out
+
     # -- counts ------------------------------------------------------------
     def load_counts(self, X):
         X = sp.csc_matrix(X)
