@@ -48,10 +48,16 @@ extern "C" {
 /* precision forms */
 #define SEV_PREC_FP32 0 /* plain form */
 #define SEV_PREC_BF16 1 /* BF16 values / BF16 MFMA candidates, exact re-rank */
+#define SEV_PREC_EXACT 2 /* kNN only: exhaustive exact search, no MFMA stage */
 
 /* ---- library ----------------------------------------------------------- */
 const char *sev_last_error(void);
 int sev_version(void);
+/* Device self-checks (no counterpart in the reference): what == 0 / 1 / 2
+ * checks the FP32 / BF16 / FP64 matrix-core lane layouts the kNN and IRLBA
+ * kernels rely on; *result receives the largest deviation (0 when they
+ * hold). */
+int sev_selftest(int what, double *result);
 /* Selects the device for the calling thread and checks it is usable. */
 int sev_device_init(int device);
 
@@ -109,10 +115,12 @@ void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int 
 void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int ntables,
                                int include_self, int seed, int32_t *nn_index, double *distances);
 
-/* Same search with a status and a precision form.  precision SEV_PREC_FP32
- * or SEV_PREC_BF16 (candidates on MFMA, then exact FP32 re-rank; the result
- * equals the FP32 exact search).  *n_fallback (may be NULL) receives the
- * number of query rows that needed the exhaustive exact search. */
+/* Same search with a status and a precision form: SEV_PREC_FP32 (FP32
+ * candidates on MFMA), SEV_PREC_BF16 (BF16 candidates on MFMA) -- both
+ * followed by an exact FP32 re-rank and a certificate, so the result equals
+ * the exact search -- or SEV_PREC_EXACT (exhaustive).  Float64 input always
+ * uses the exhaustive search.  *n_fallback (may be NULL) receives the number
+ * of query rows that were recomputed exhaustively. */
 int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
             int include_self, int precision, int32_t *nn_index, void *distances,
             int64_t *n_fallback);

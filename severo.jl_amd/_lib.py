@@ -11,7 +11,7 @@ SEV_OK = 0
 SEV_FLOAT32, SEV_FLOAT64 = 0, 1
 SEV_INT32, SEV_INT64 = 0, 1
 SEV_LOGNORMALIZE, SEV_RELATIVECOUNTS = 0, 1
-SEV_PREC_FP32, SEV_PREC_BF16 = 0, 1
+SEV_PREC_FP32, SEV_PREC_BF16, SEV_PREC_EXACT = 0, 1, 2
 
 _i64 = C.c_int64
 _int = C.c_int
@@ -24,6 +24,7 @@ _pi64 = C.POINTER(C.c_int64)
 SIGNATURES = {
     "sev_last_error": (C.c_char_p, []),
     "sev_version": (_int, []),
+    "sev_selftest": (_int, [_int, C.POINTER(_dbl)]),
     "sev_device_init": (_int, [_int]),
     "sev_normalize_cells": (_int, [_i64, _i64, _p, _p, _p, _int, _int, _dbl, _int, _p]),
     "sev_scale_features": (_int, [_i64, _i64, _p, _p, _p, _int, _dbl, _p, _p]),

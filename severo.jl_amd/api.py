@@ -228,10 +228,12 @@ def _coords(X, dims):
     return X
 
 
-def ann(X, k: int, include_self=True, ntables=None, precision="fp32"):
+def ann(X, k: int, include_self=True, ntables=None, precision="fp32", return_fallback=False):
     """Exact Euclidean kNN (the search behind ann!/FindNeighboursEuclidean,
     src/neighbours.jl:19-63).  Returns (nn_index n x k 1-based Int32,
-    distances n x k), column-major like the reference."""
+    distances n x k), column-major like the reference.  ``precision``
+    selects how candidates are proposed ("fp32" / "bf16" matrix cores with an
+    exact re-rank, or "exact" exhaustive); the result is the same."""
     X = np.asfortranarray(X)
     if X.dtype not in (np.float32, np.float64):
         X = X.astype(np.float64, order="F")
@@ -239,9 +241,12 @@ def ann(X, k: int, include_self=True, ntables=None, precision="fp32"):
     idx = np.zeros((n, k), np.int32, order="F")
     dist = np.zeros((n, k), X.dtype, order="F")
     nfb = C.c_int64(0)
-    prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16}[precision]
+    prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16,
+            "exact": _lib.SEV_PREC_EXACT}[precision]
     check(_lib.load().sev_knn(_ptr(X), n, d, n, _dtype_code(X.dtype), k, int(include_self), prec,
                               _ptr(idx), _ptr(dist), C.byref(nfb)))
+    if return_fallback:
+        return idx, dist, nfb.value
     return idx, dist
 
 

@@ -176,6 +176,7 @@ struct sev_pipeline {
     DevBuf<float> dist;
     bool have_nn = false;
     // SNN
+    KnnWork kw;
     SnnWork sw;
     int64_t snn_nnz = 0;
     int snn_dtype = SEV_FLOAT32;
@@ -217,7 +218,7 @@ void scale_host(int64_t m, int64_t n, const DevBuf<int64_t> &cp, int64_t nnz, co
 namespace {
 template <class T>
 void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
-              int32_t *nn_index, T *distances) {
+              int precision, int32_t *nn_index, T *distances, int64_t *n_fallback) {
     SEV_REQUIRE(n >= 1 && d >= 1 && ld >= n, "bad X shape");
     SEV_REQUIRE(k >= 1 && k <= (include_self ? n : n - 1), "k larger than the number of points");
     Stream s;
@@ -229,7 +230,15 @@ void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int inclu
     DevBuf<T> dist;
     idx.ensure((size_t)(n * k));
     dist.ensure((size_t)(n * k));
-    knn_exact<T>(dx.get(), n, d, n, k, include_self, nullptr, n, idx.get(), dist.get(), s.st);
+    int64_t nfb = n;
+    if (sizeof(T) == 4 && precision != SEV_PREC_EXACT) {
+        KnnWork w;
+        knn_mfma((const float *)dx.get(), n, d, n, k, include_self, precision == SEV_PREC_BF16, w,
+                 idx.get(), (float *)dist.get(), s.st, &nfb);
+    } else {
+        knn_exact<T>(dx.get(), n, d, n, k, include_self, nullptr, n, idx.get(), dist.get(), s.st);
+    }
+    if (n_fallback) *n_fallback = nfb;
     std::vector<int32_t> hi((size_t)(n * k));
     std::vector<T> hd((size_t)(n * k));
     download(hi.data(), idx.get(), hi.size(), s.st);
@@ -248,6 +257,15 @@ extern "C" {
 const char *sev_last_error(void) { return g_err.c_str(); }
 
 int sev_version(void) { return 1; }
+
+int sev_selftest(int what, double *result) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(what >= 0 && what <= 2, "unknown self-test");
+        SEV_REQUIRE(result != nullptr, "result is NULL");
+        *result = mfma_selftest(what);
+    });
+}
 
 int sev_device_init(int device) {
     return guarded([&] {
@@ -511,16 +529,23 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
     return guarded([&] {
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_embedding, "no embedding");
-        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
+                        precision == SEV_PREC_EXACT,
+                    "bad precision");
         const int64_t m = p->m;
         const int64_t d = (ndims <= 0 || ndims > p->zdim) ? p->zdim : ndims;
         SEV_REQUIRE(k >= 1 && k <= (include_self ? m : m - 1), "k larger than the number of cells");
         hipStream_t st = p->st();
         p->nn.ensure((size_t)(m * k));
         p->dist.ensure((size_t)(m * k));
-        knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
-                         p->dist.get(), st);
-        if (n_fallback) *n_fallback = m;
+        if (precision == SEV_PREC_EXACT) {
+            knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
+                             p->dist.get(), st);
+            if (n_fallback) *n_fallback = m;
+        } else {
+            knn_mfma(p->Z.get(), m, d, m, k, include_self, precision == SEV_PREC_BF16, p->kw,
+                     p->nn.get(), p->dist.get(), st, n_fallback);
+        }
         p->k = k;
         p->have_nn = true;
         p->have_snn = false;
@@ -797,16 +822,17 @@ int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t 
             int64_t *n_fallback) {
     return guarded([&] {
         ensure_device();
-        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
+                        precision == SEV_PREC_EXACT,
+                    "bad precision");
         if (dtype == SEV_FLOAT32)
-            knn_host<float>((const float *)X, n, d, ld, k, include_self, nn_index,
-                            (float *)distances);
+            knn_host<float>((const float *)X, n, d, ld, k, include_self, precision, nn_index,
+                            (float *)distances, n_fallback);
         else if (dtype == SEV_FLOAT64)
-            knn_host<double>((const double *)X, n, d, ld, k, include_self, nn_index,
-                             (double *)distances);
+            knn_host<double>((const double *)X, n, d, ld, k, include_self, precision, nn_index,
+                             (double *)distances, n_fallback);
         else
             fail(kErrArg, "bad dtype");
-        if (n_fallback) *n_fallback = n;
     });
 }
 

@@ -12,11 +12,11 @@
 // Gram-Schmidt pass of full reorthogonalisation on each side, restart size
 // k = nu + #converged capped at work - 3, convergence when nu Ritz values
 // have |residual| < tol * max(sigma) and relative change < svtol).  All
-// vector arithmetic is FP64; B is read as FP32 (plain form) or BF16
-// (precision form) values with FP64 accumulation.
+// vector arithmetic is FP64; B is read as FP32 (plain form), BF16 (precision
+// form) or FP64 values with FP64 accumulation.
 //
-// Every reduction has a fixed order (wave butterflies read back from lane 0,
-// per-slab partials summed in slab order), so results are bit-for-bit
+// Every reduction has a fixed order (wave butterflies read back from one
+// lane, per-slab partials summed in slab order), so results are bit-for-bit
 // reproducible run to run.
 #include "sev_kernels.hpp"
 
@@ -28,20 +28,55 @@ namespace sev {
 
 namespace {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxWork = 64;
-constexpr int64_t kMaxFL = 3584;  // largest feature count kept in LDS
-constexpr int kRedChunks = 16;    // first-level chunks of the slab partials
-constexpr int kDotBlocks = 512;   // blocks of the W' y partial products
+constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v)
+constexpr int kRedChunks = 16;       // first-level chunks of the slab partials
+constexpr int kDotBlocks = 2048;     // blocks of the W' y partial products
+constexpr int64_t kBuildRows = 256;  // row slabs of the CSR build
 
 __device__ __forceinline__ double ldval(const float *v, int64_t p) { return (double)v[p]; }
 __device__ __forceinline__ double ldval(const double *v, int64_t p) { return v[p]; }
-__device__ __forceinline__ double ldmu(const float *mu, int64_t i) { return (double)mu[i]; }
-__device__ __forceinline__ double ldmu(const double *mu, int64_t i) { return mu[i]; }
 __device__ __forceinline__ double ldval(const uint16_t *v, int64_t p) {
     return (double)__uint_as_float((uint32_t)v[p] << 16);
 }
+__device__ __forceinline__ double ldmu(const float *mu, int64_t i) { return (double)mu[i]; }
+__device__ __forceinline__ double ldmu(const double *mu, int64_t i) { return mu[i]; }
+
+// four consecutive values starting at a multiple of 4
+__device__ __forceinline__ void ld4(const float *v, int64_t p, double o[4]) {
+    const float4 x = *(const float4 *)(v + p);
+    o[0] = x.x;
+    o[1] = x.y;
+    o[2] = x.z;
+    o[3] = x.w;
+}
+__device__ __forceinline__ void ld4(const uint16_t *v, int64_t p, double o[4]) {
+    const ushort4 x = *(const ushort4 *)(v + p);
+    o[0] = (double)__uint_as_float((uint32_t)x.x << 16);
+    o[1] = (double)__uint_as_float((uint32_t)x.y << 16);
+    o[2] = (double)__uint_as_float((uint32_t)x.z << 16);
+    o[3] = (double)__uint_as_float((uint32_t)x.w << 16);
+}
+__device__ __forceinline__ void ld4(const double *v, int64_t p, double o[4]) {
+    const double2 a = *(const double2 *)(v + p);
+    const double2 b = *(const double2 *)(v + p + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+}
+
+__device__ __forceinline__ float cvt_store(float x, float *) { return x; }
+__device__ __forceinline__ uint16_t cvt_store(float x, uint16_t *) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ double cvt_store(double x, double *) { return x; }
 
 __device__ __forceinline__ int wave_index() {
     return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
@@ -60,10 +95,13 @@ __device__ double block_sum(double v, double *scratch) {
 }
 
 // --------------------------------------------------------------------------
-// Layout build: slab offsets and 16-bit in-slab rows.
+// Layout build
 // --------------------------------------------------------------------------
+
+// off[s*n + j] = first position of column j with row >= s*R (s = 0..nslab);
+// lrow[p] = row - slab base (when lrow != nullptr).
 __global__ void __launch_bounds__(kBlock)
-k_slab_offsets(int64_t n, int64_t nslab, const int64_t *colptr, const int32_t *row,
+k_slab_offsets(int64_t n, int64_t nslab, int64_t R, const int64_t *colptr, const int32_t *row,
                int64_t *off, uint16_t *lrow) {
     const int64_t j = blockIdx.x;
     const int64_t b = colptr[j], e = colptr[j + 1];
@@ -73,77 +111,124 @@ k_slab_offsets(int64_t n, int64_t nslab, const int64_t *colptr, const int32_t *r
     }
     for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
         const int64_t r = row[p];
-        const int64_t sc = r / kSlabRows;
-        const int64_t sp = (p == b) ? -1 : (int64_t)row[p - 1] / kSlabRows;
+        const int64_t sc = r / R;
+        const int64_t sp = (p == b) ? -1 : (int64_t)row[p - 1] / R;
         for (int64_t s = sp + 1; s <= sc; ++s) off[s * n + j] = p;
         if (p == e - 1)
             for (int64_t s = sc + 1; s <= nslab; ++s) off[s * n + j] = e;
-        lrow[p] = (uint16_t)(r - sc * kSlabRows);
+        if (lrow) lrow[p] = (uint16_t)(r - sc * R);
     }
 }
 
 __global__ void __launch_bounds__(kBlock) k_to_bf16(const float *in, uint16_t *out, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock) {
-        // round to nearest even
-        uint32_t u = __float_as_uint(in[i]);
-        uint32_t lsb = (u >> 16) & 1u;
-        u += 0x7FFFu + lsb;
-        out[i] = (uint16_t)(u >> 16);
+         i += (int64_t)gridDim.x * kBlock)
+        out[i] = cvt_store(in[i], (uint16_t *)nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_row_count(const int32_t *row, int64_t nnz, int32_t *cnt) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
+         p += (int64_t)gridDim.x * kBlock)
+        atomicAdd(&cnt[row[p]], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_round4(int32_t *cnt, int64_t m) {
+    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * kBlock)
+        cnt[r] = (cnt[r] + 3) & ~3;
+}
+
+// Padded CSR fill: one wavefront per slab of kBuildRows rows walks the
+// columns in order and appends each entry at rowptr[row] + cursor[row]; the
+// rows of one column are distinct, so the LDS cursors need no atomics and
+// every row comes out sorted by column.  Padding slots were zeroed before.
+template <class ST, class DT>
+__global__ void __launch_bounds__(kBlock)
+k_csr_fill(int64_t m, int64_t n, int64_t nbs, const int64_t *boff, const int32_t *row,
+           const ST *val, const int64_t *rowptr, uint16_t *ccol, DT *cval) {
+    __shared__ int32_t cur[kWaves][kBuildRows];
+    __shared__ int64_t rp[kWaves][kBuildRows];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t s = (int64_t)blockIdx.x * kWaves + w;
+    if (s >= nbs) return;
+    const int64_t r0 = s * kBuildRows;
+    for (int i = lane; i < kBuildRows; i += kWave) {
+        cur[w][i] = 0;
+        rp[w][i] = r0 + i < m ? rowptr[r0 + i] : 0;
+    }
+    for (int64_t j0 = 0; j0 < n; j0 += kWave) {
+        const int64_t jl = j0 + lane;
+        const int64_t a_l = jl < n ? boff[s * n + jl] : 0;
+        const int64_t b_l = jl < n ? boff[(s + 1) * n + jl] : 0;
+        const int nj = (int)((n - j0) < kWave ? (n - j0) : kWave);
+        for (int jj = 0; jj < nj; ++jj) {
+            const int64_t a = __shfl(a_l, jj, kWave), b = __shfl(b_l, jj, kWave);
+            for (int64_t p = a + lane; p < b; p += kWave) {
+                const int lr = (int)(row[p] - r0);
+                const int c = cur[w][lr];
+                cur[w][lr] = c + 1;
+                const int64_t pos = rp[w][lr] + c;
+                ccol[pos] = (uint16_t)(j0 + jj);
+                cval[pos] = cvt_store(val[p], (DT *)nullptr);
+            }
+        }
     }
 }
 
 // --------------------------------------------------------------------------
-// y = B v - (mu'.v) 1 - RF * wprev          (one block per slab)
-// Each wave owns a private FP64 accumulator over the slab's rows and walks
-// the columns j = wave, wave + kWaves, ...; within one column the rows are
-// distinct, so the read-modify-write needs no atomics.  The four wave
-// accumulators are summed in wave order.
+// y = B v - (mu'.v) 1 - RF * wprev   (padded CSR; G lanes per row, every
+// lane reads 4 entries per step; the G partial sums are combined by a fixed
+// butterfly and the group leader's value is used)
 // --------------------------------------------------------------------------
-template <class VT, class MT>
+template <class VT, class MT, int G>
 __global__ void __launch_bounds__(kBlock)
-k_av(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *val,
-     const MT *mu, const double *v, const double *rf, const double *wprev, double *y,
-     int use_lds) {
-    extern __shared__ double smem[];
-    double *acc = smem;                       // kWaves * kSlabRows
-    double *vl = smem + kWaves * kSlabRows;   // n (if use_lds)
+k_av_csr(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const VT *val,
+         const MT *mu, const double *v, const double *rf, const double *wprev, double *y,
+         int use_lds) {
+    extern __shared__ double vl[];
     __shared__ double dmu_s;
-    const int64_t s = blockIdx.x;
-    const int64_t r0 = s * kSlabRows;
-    const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
     const int w = wave_index(), lane = lane_id();
-
     if (use_lds)
         for (int64_t i = threadIdx.x; i < n; i += kBlock) vl[i] = v[i];
-    for (int64_t i = threadIdx.x; i < kWaves * kSlabRows; i += kBlock) acc[i] = 0.0;
     __syncthreads();
     const double *vs = use_lds ? vl : v;
     if (w == 0) {
         double d = 0.0;
         if (mu)
-            for (int64_t i = lane; i < n; i += kWave) d += ldmu(mu, i) * vs[i];
+            for (int64_t i = lane; i < n; i += kWave) d = fma(ldmu(mu, i), vs[i], d);
         d = wave_sum(d);
         if (lane == 0) dmu_s = d;
-    }
-    double *aw = acc + w * kSlabRows;
-    for (int64_t j = w; j < n; j += kWaves) {
-        const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
-        const double vj = vs[j];
-        for (int64_t p = a + lane; p < b; p += kWave) {
-            const int lr = lrow[p];
-            aw[lr] = fma(ldval(val, p), vj, aw[lr]);
-        }
     }
     __syncthreads();
     const double dmu = dmu_s;
     const double rfv = rf ? *rf : 0.0;
-    for (int64_t lr = threadIdx.x; lr < nrows; lr += kBlock) {
-        double t = acc[lr];
-        for (int q = 1; q < kWaves; ++q) t += acc[q * kSlabRows + lr];
-        t -= dmu;
-        if (wprev) t -= rfv * wprev[r0 + lr];
-        y[r0 + lr] = t;
+    constexpr int RPW = kWave / G;  // rows per wave
+    const int gl = lane % G, gi = lane / G;
+    const int64_t stride = (int64_t)gridDim.x * kWaves * RPW;
+    for (int64_t r0 = ((int64_t)blockIdx.x * kWaves + w) * RPW; r0 < m; r0 += stride) {
+        const int64_t r = r0 + gi;
+        double acc = 0.0;
+        if (r < m) {
+            const int64_t b = rowptr[r], e = rowptr[r + 1];
+            for (int64_t p = b + 4 * gl; p < e; p += 4 * G) {
+                const ushort4 c4 = *(const ushort4 *)(col + p);
+                double x[4];
+                ld4(val, p, x);
+                acc = fma(x[0], vs[c4.x], acc);
+                acc = fma(x[1], vs[c4.y], acc);
+                acc = fma(x[2], vs[c4.z], acc);
+                acc = fma(x[3], vs[c4.w], acc);
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+        acc = __shfl(acc, gi * G, kWave);
+        if (gl == 0 && r < m) {
+            double t = acc - dmu;
+            if (wprev) t -= rfv * wprev[r];
+            y[r] = t;
+        }
     }
 }
 
@@ -183,14 +268,23 @@ k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
     }
 }
 
-// T[i] = sum_b part[b*kMaxWork + i] in block order (single block, 1024 threads:
-// 16 threads per column each summing a strided subset, combined in order).
+// T[i] = sum_b part[b*kMaxWork + i] (single block, 1024 threads: 16 threads
+// per column each summing a strided subset, combined in a fixed order).
 __global__ void __launch_bounds__(1024) k_red_t(int nblk, int jc, const double *part, double *T) {
     const int i = threadIdx.x / 16, q = threadIdx.x % 16;
     double t = 0.0;
-    if (i < jc)
-        for (int b = q; b < nblk; b += 16) t += part[(int64_t)b * kMaxWork + i];
-    // fixed-order combine of the 16 partials (lanes q = 0..15 of a group)
+    if (i < jc) {
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        int b = q;
+        for (; b + 48 < nblk; b += 64) {
+            t += part[(int64_t)b * kMaxWork + i];
+            t1 += part[(int64_t)(b + 16) * kMaxWork + i];
+            t2 += part[(int64_t)(b + 32) * kMaxWork + i];
+            t3 += part[(int64_t)(b + 48) * kMaxWork + i];
+        }
+        for (; b < nblk; b += 16) t += part[(int64_t)b * kMaxWork + i];
+        t = (t + t1) + (t2 + t3);
+    }
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) {
         double u = __shfl_down(t, o, 16);
@@ -200,52 +294,70 @@ __global__ void __launch_bounds__(1024) k_red_t(int nblk, int jc, const double *
 }
 
 // --------------------------------------------------------------------------
-// Fused: y -= W[:,0:jc] T (reorthogonalisation apply), slab partials of
-// ||y||^2 and sum(y), and the slab partials of B' y (one block per slab).
-// part row s holds n column dots followed by ||y||^2 and sum(y).
+// y' = y - W[:,0:jc] T (reorthogonalisation apply, one pass over W) into
+// yout, with the slab partials of ||y'||^2 and sum(y') (one block per slab;
+// part row s holds n column dots followed by these two).
 // --------------------------------------------------------------------------
-template <class VT>
 __global__ void __launch_bounds__(kBlock)
-k_upd_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *val,
-          const double *W, int jc, const double *T, double *y, double *part) {
-    __shared__ double wl[kSlabRows];
+k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const double *y,
+        double *yout, double *part) {
     __shared__ double tl[kMaxWork];
     __shared__ double scratch[kWaves];
     const int64_t s = blockIdx.x;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
-    const int w = wave_index(), lane = lane_id();
-    if (threadIdx.x < jc) tl[threadIdx.x] = T[threadIdx.x];
+    if ((int)threadIdx.x < jc) tl[threadIdx.x] = T[threadIdx.x];
     __syncthreads();
     double nrm = 0.0, sm = 0.0;
-    for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock) {
-        double yv = 0.0;
-        if (lr < nrows) {
-            const int64_t r = r0 + lr;
-            yv = y[r];
-            if (jc > 0) {
-                double d = 0.0;
-                for (int i = 0; i < jc; ++i) d = fma(W[(int64_t)i * m + r], tl[i], d);
-                yv -= d;
-                y[r] = yv;
-            }
-            nrm = fma(yv, yv, nrm);
-            sm += yv;
+    for (int64_t lr = threadIdx.x; lr < nrows; lr += kBlock) {
+        const int64_t r = r0 + lr;
+        double yv = y[r];
+        if (jc > 0) {
+            double d = 0.0;
+            for (int i = 0; i < jc; ++i) d = fma(W[(int64_t)i * m + r], tl[i], d);
+            yv -= d;
         }
-        wl[lr] = yv;
+        yout[r] = yv;
+        nrm = fma(yv, yv, nrm);
+        sm += yv;
     }
     nrm = block_sum(nrm, scratch);
     sm = block_sum(sm, scratch);
-    double *prow = part + s * (n + 2);
     if (threadIdx.x == 0) {
-        prow[n] = nrm;
-        prow[n + 1] = sm;
+        part[s * (n + 2) + n] = nrm;
+        part[s * (n + 2) + n + 1] = sm;
     }
-    for (int64_t j = w; j < n; j += kWaves) {
+}
+
+// --------------------------------------------------------------------------
+// Slab partials of B' y: block (slab s, column group g) stages the slab of y
+// in LDS and computes the dots of the columns of its group, one wavefront per
+// column segment (two accumulators, combined in order).
+// --------------------------------------------------------------------------
+template <class VT>
+__global__ void __launch_bounds__(kBlock)
+k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *val,
+      const double *y, double *part) {
+    __shared__ double wl[kSlabRows];
+    const int64_t s = blockIdx.x;
+    const int g = blockIdx.y, ng = gridDim.y;
+    const int64_t r0 = s * kSlabRows;
+    const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
+    const int w = wave_index(), lane = lane_id();
+    for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock)
+        wl[lr] = lr < nrows ? y[r0 + lr] : 0.0;
+    __syncthreads();
+    double *prow = part + s * (n + 2);
+    for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
         const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
-        double acc = 0.0;
-        for (int64_t p = a + lane; p < b; p += kWave) acc = fma(ldval(val, p), wl[lrow[p]], acc);
-        acc = wave_sum(acc);
+        double acc0 = 0.0, acc1 = 0.0;
+        int64_t p = a + lane;
+        for (; p + kWave < b; p += 2 * kWave) {
+            acc0 = fma(ldval(val, p), wl[lrow[p]], acc0);
+            acc1 = fma(ldval(val, p + kWave), wl[lrow[p + kWave]], acc1);
+        }
+        if (p < b) acc0 = fma(ldval(val, p), wl[lrow[p]], acc0);
+        const double acc = wave_sum(acc0 + acc1);
         if (lane == 0) prow[j] = acc;
     }
 }
@@ -264,7 +376,7 @@ k_red_slabs(int64_t nslab, int64_t ncol, const double *part, double *part2) {
     part2[q * ncol + c] = t;
 }
 
-// y = sum_q part2[q][i] - (sum x) mu'_i  (finishes A' x after k_upd_atw)
+// y = sum_q part2[q][i] - (sum x) mu'_i  (finishes A' x after k_atw)
 template <class MT>
 __global__ void __launch_bounds__(kBlock)
 k_finish_at(int64_t n, const double *part2, const MT *mu, double *y) {
@@ -370,34 +482,64 @@ k_right_step(int64_t n, int j, int work, const double *part2, const MT *mu, doub
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_scale_col(double *y, int64_t m, const double *scal) {
+// dst = src / S (the normalised Lanczos vector w_j = y / ||y||)
+__global__ void __launch_bounds__(kBlock)
+k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
     const double S = scal[0];
     for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * kBlock)
-        y[r] = y[r] / S;
+        dst[r] = src[r] / S;
 }
 
-// Y[:, c] = X[:, 0:work] Cf[:, c] for c < kout (X, Y column-major, ld = m);
-// optionally Y[:, kout] = extra / scal[2] (the normalised residual vector).
+// Y[:, c] = X[:, 0:work] Cf[:, c] for c < kout (X, Y column-major, ld = m),
+// computed as the 16x16 tiles of D = Cf' X' on v_mfma_f64_16x16x4_f64: one
+// wavefront per 16 rows, lanes (n = row, k-group) read 16 consecutive rows
+// of one column of X, and each lane writes 16 consecutive rows of an output
+// column.  In this instruction lane l holds D[l/16 + 4i][l%16], i = 0..3.
+// Optionally Y[:, kout] = extra / scal[2] (the normalised residual).
 __global__ void __launch_bounds__(kBlock)
-k_gemm_small(int64_t m, int work, const double *X, const double *Cf, int kout, double *Y,
-             const double *extra, const double *scal) {
-    __shared__ double cf[kMaxWork * kMaxWork];
-    for (int i = threadIdx.x; i < work * kout; i += kBlock) cf[i] = Cf[(i / work) * work + i % work];
+k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, double *Y,
+            const double *extra, const double *scal) {
+    constexpr int LD = kMaxWork + 1;
+    __shared__ double cfl[kMaxWork * LD];  // cfl[c*LD + l] = Cf[c*work + l], zero padded
+    for (int i = threadIdx.x; i < kMaxWork * kMaxWork; i += kBlock) {
+        const int c = i / kMaxWork, l = i % kMaxWork;
+        cfl[c * LD + l] = (c < kout && l < work) ? Cf[c * work + l] : 0.0;
+    }
     __syncthreads();
-    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
-         r += (int64_t)gridDim.x * kBlock) {
-        double x[kMaxWork];
+    const int w = wave_index(), lane = lane_id();
+    const int nr = lane & 15, kq = lane >> 4;
+    const int ncb = (kout + 15) / 16;
+    const int nks = (work + 3) / 4;
+    const int64_t nrb = (m + 15) / 16;
+    for (int64_t rb = (int64_t)blockIdx.x * kWaves + w; rb < nrb;
+         rb += (int64_t)gridDim.x * kWaves) {
+        const int64_t row = rb * 16 + nr;
+        v4d acc[4];
 #pragma unroll
-        for (int l = 0; l < kMaxWork; ++l) x[l] = l < work ? X[(int64_t)l * m + r] : 0.0;
-        for (int c = 0; c < kout; ++c) {
-            double a = 0.0;
+        for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < nks; ++s) {
+            const int k = 4 * s + kq;
+            const double bv = (k < work && row < m) ? X[(int64_t)k * m + row] : 0.0;
 #pragma unroll
-            for (int l = 0; l < kMaxWork; ++l)
-                if (l < work) a = fma(x[l], cf[c * work + l], a);
-            Y[(int64_t)c * m + r] = a;
+            for (int cb = 0; cb < 4; ++cb) {
+                if (cb < ncb) {
+                    const double av = cfl[(cb * 16 + nr) * LD + k];
+                    acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+                }
+            }
         }
-        if (extra) Y[(int64_t)kout * m + r] = extra[r] / scal[2];
+        if (row < m) {
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cb * 16 + kq + 4 * i;
+                    if (cb < ncb && c < kout) Y[(int64_t)c * m + row] = acc[cb][i];
+                }
+            }
+            if (extra && kq == 0) Y[(int64_t)kout * m + row] = extra[row] / scal[2];
+        }
     }
 }
 
@@ -415,12 +557,24 @@ unsigned grid_rows(int64_t m, int per_cu = 4) {
     return (unsigned)(want < cap ? want : cap);
 }
 
+int val_bytes(int vtype) { return vtype == kValBF16 ? 2 : (vtype == kValF64 ? 8 : 4); }
+
+template <class ST, class DT>
+void launch_csr_fill(int64_t m, int64_t n, int64_t nbs, const int64_t *boff, const int32_t *row,
+                     const void *val, const int64_t *rowptr, uint16_t *ccol, void *cval,
+                     hipStream_t st) {
+    hipLaunchKernelGGL((k_csr_fill<ST, DT>), dim3((unsigned)ceil_div(nbs, kWaves)), dim3(kBlock),
+                       0, st, m, n, nbs, boff, row, (const ST *)val, rowptr, ccol, (DT *)cval);
+    SEV_LAUNCH_CHECK();
+}
+
 }  // namespace
 
 void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                     const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
                     hipStream_t st) {
     SEV_REQUIRE(m > 0 && n > 0, "empty operator");
+    SEV_REQUIRE(n <= 65535, "more than 65535 features");
     SEV_REQUIRE(!(bf16 && src_f64), "bf16 operator needs Float32 source values");
     op.m = m;
     op.n = n;
@@ -433,10 +587,12 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     op.mu64 = src_f64;
     op.vals = val;
     op.vtype = src_f64 ? kValF64 : kValF32;
-    ProfScope ps("slab_build", st, 6.0 * op.nnz + 8.0 * (op.nslab + 1) * n);
-    hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, op.nslab,
-                       colptr, row, op.off.get(), op.lrow.get());
-    SEV_LAUNCH_CHECK();
+    {
+        ProfScope ps("slab_build", st, 6.0 * op.nnz + 8.0 * (op.nslab + 1) * n);
+        hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, op.nslab,
+                           kSlabRows, colptr, row, op.off.get(), op.lrow.get());
+        SEV_LAUNCH_CHECK();
+    }
     if (bf16) {
         op.vbf16.ensure((size_t)op.nnz);
         if (op.nnz > 0) {
@@ -447,53 +603,117 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         op.vals = op.vbf16.get();
         op.vtype = kValBF16;
     }
+    // padded CSR for A v
+    ProfScope ps("csr_build", st, 2.0 * 10.0 * op.nnz);
+    BuildScratch b;
+    const int64_t nbs = ceil_div(m, kBuildRows);
+    b.boff.ensure((size_t)((nbs + 1) * n));
+    b.cnt.ensure((size_t)m);
+    b.scan.ensure(scan_scratch_elems(m));
+    op.rowptr.ensure((size_t)m + 1);
+    hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, nbs, kBuildRows,
+                       colptr, row, b.boff.get(), (uint16_t *)nullptr);
+    SEV_LAUNCH_CHECK();
+    SEV_HIP(hipMemsetAsync(b.cnt.get(), 0, sizeof(int32_t) * m, st));
+    if (op.nnz > 0) {
+        hipLaunchKernelGGL(k_row_count, dim3(grid_rows(op.nnz, 16)), dim3(kBlock), 0, st, row,
+                           op.nnz, b.cnt.get());
+        SEV_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_round4, dim3(grid_rows(m)), dim3(kBlock), 0, st, b.cnt.get(), m);
+    SEV_LAUNCH_CHECK();
+    scan_exclusive_i32(b.cnt.get(), op.rowptr.get(), m, b.scan.get(), st);
+    SEV_HIP(hipMemcpyAsync(&op.csr_nnz, op.rowptr.get() + m, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    const size_t vb = (size_t)val_bytes(op.vtype);
+    op.ccol.ensure((size_t)op.csr_nnz + 4);
+    op.cval.ensure(((size_t)op.csr_nnz + 4) * vb);
+    SEV_HIP(hipMemsetAsync(op.ccol.get(), 0, sizeof(uint16_t) * (size_t)op.csr_nnz, st));
+    SEV_HIP(hipMemsetAsync(op.cval.get(), 0, vb * (size_t)op.csr_nnz, st));
+    if (op.vtype == kValBF16)
+        launch_csr_fill<float, uint16_t>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
+                                         op.ccol.get(), op.cval.get(), st);
+    else if (op.vtype == kValF64)
+        launch_csr_fill<double, double>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
+                                        op.ccol.get(), op.cval.get(), st);
+    else
+        launch_csr_fill<float, float>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
+                                      op.ccol.get(), op.cval.get(), st);
+    const double avg = (double)op.csr_nnz / (double)m;
+    op.lanes_per_row = avg > 192 ? 64 : (avg > 96 ? 32 : (avg > 48 ? 16 : 8));
+    // the build scratch is released when b goes out of scope: wait for the fill
+    SEV_HIP(hipStreamSynchronize(st));
 }
 
 namespace {
 
-
-int val_bytes(const SlabCsc &op) {
-    return op.vtype == kValBF16 ? 2 : (op.vtype == kValF64 ? 8 : 4);
-}
-
-template <class VT, class MT>
-void launch_av_t(const SlabCsc &op, size_t smem, int use_lds, const double *v, const double *rf,
+template <class VT, class MT, int G>
+void launch_av_g(const SlabCsc &op, int use_lds, const double *v, const double *rf,
                  const double *wprev, double *y, hipStream_t st) {
-    ProfScope ps("spmv_av", st,
-                 (double)op.nnz * (val_bytes(op) + 2) + 8.0 * op.m * (wprev ? 2 : 1) + 8.0 * op.n);
-    hipLaunchKernelGGL((k_av<VT, MT>), dim3((unsigned)op.nslab), dim3(kBlock), smem, st, op.m, op.n,
-                       op.off.get(), op.lrow.get(), (const VT *)op.vals, (const MT *)op.mu, v, rf,
-                       wprev, y, use_lds);
+    const size_t smem = use_lds ? sizeof(double) * (size_t)op.n : 0;
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        ceil_div(op.m, (int64_t)kWaves * (kWave / G)), (int64_t)device_cu_count() * 8);
+    hipLaunchKernelGGL((k_av_csr<VT, MT, G>), dim3(grid), dim3(kBlock), smem, st, op.m, op.n,
+                       op.rowptr.get(), op.ccol.get(), (const VT *)op.cval.get(),
+                       (const MT *)op.mu, v, rf, wprev, y, use_lds);
     SEV_LAUNCH_CHECK();
 }
 
-void launch_av(const SlabCsc &op, size_t smem, int use_lds, const double *v, const double *rf,
-               const double *wprev, double *y, hipStream_t st) {
-    if (op.vtype == kValBF16)
-        launch_av_t<uint16_t, float>(op, smem, use_lds, v, rf, wprev, y, st);
-    else if (op.vtype == kValF64)
-        launch_av_t<double, double>(op, smem, use_lds, v, rf, wprev, y, st);
-    else
-        launch_av_t<float, float>(op, smem, use_lds, v, rf, wprev, y, st);
+template <class VT, class MT>
+void launch_av_t(const SlabCsc &op, int use_lds, const double *v, const double *rf,
+                 const double *wprev, double *y, hipStream_t st) {
+    switch (op.lanes_per_row) {
+        case 8: launch_av_g<VT, MT, 8>(op, use_lds, v, rf, wprev, y, st); break;
+        case 16: launch_av_g<VT, MT, 16>(op, use_lds, v, rf, wprev, y, st); break;
+        case 32: launch_av_g<VT, MT, 32>(op, use_lds, v, rf, wprev, y, st); break;
+        default: launch_av_g<VT, MT, 64>(op, use_lds, v, rf, wprev, y, st); break;
+    }
 }
 
-void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T, double *y,
-                    double *part, hipStream_t st) {
-    ProfScope ps("spmv_atw", st,
-                 (double)op.nnz * (val_bytes(op) + 2) + 8.0 * op.m * (jc > 0 ? jc + 2 : 1) +
-                     8.0 * op.n);
+void launch_av(const SlabCsc &op, const double *v, const double *rf, const double *wprev,
+               double *y, hipStream_t st) {
+    const int use_lds = op.n <= kMaxFL ? 1 : 0;
+    ProfScope ps("spmv_av", st,
+                 (double)op.csr_nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m * (wprev ? 2 : 1) +
+                     8.0 * (op.m + 1) + 8.0 * op.n);
     if (op.vtype == kValBF16)
-        hipLaunchKernelGGL(k_upd_atw<uint16_t>, dim3((unsigned)op.nslab), dim3(kBlock), 0, st,
-                           op.m, op.n, op.off.get(), op.lrow.get(), (const uint16_t *)op.vals, W,
-                           jc, T, y, part);
+        launch_av_t<uint16_t, float>(op, use_lds, v, rf, wprev, y, st);
     else if (op.vtype == kValF64)
-        hipLaunchKernelGGL(k_upd_atw<double>, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, op.m,
-                           op.n, op.off.get(), op.lrow.get(), (const double *)op.vals, W, jc, T,
-                           y, part);
+        launch_av_t<double, double>(op, use_lds, v, rf, wprev, y, st);
     else
-        hipLaunchKernelGGL(k_upd_atw<float>, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, op.m,
-                           op.n, op.off.get(), op.lrow.get(), (const float *)op.vals, W, jc, T, y,
-                           part);
+        launch_av_t<float, float>(op, use_lds, v, rf, wprev, y, st);
+}
+
+int atw_groups(const SlabCsc &op) {
+    const int64_t want = ceil_div((int64_t)device_cu_count() * 4, op.nslab);
+    const int64_t cap = std::max<int64_t>(1, op.n / (4 * kWaves));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, 16), cap));
+}
+
+// y' = y - W[:,0:jc] T into yout (plus norm/sum partials), then the slab
+// partials of B' y'.
+void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T,
+                    const double *y, double *yout, double *part, hipStream_t st) {
+    {
+        ProfScope ps("reorth_apply", st, 8.0 * op.m * (jc + 2));
+        hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, op.m, op.n, W,
+                           jc, T, y, yout, part);
+        SEV_LAUNCH_CHECK();
+    }
+    ProfScope ps("spmv_atw", st,
+                 (double)op.nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m +
+                     8.0 * (op.nslab + 1) * op.n);
+    const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
+    if (op.vtype == kValBF16)
+        hipLaunchKernelGGL(k_atw<uint16_t>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const uint16_t *)op.vals, yout, part);
+    else if (op.vtype == kValF64)
+        hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const double *)op.vals, yout, part);
+    else
+        hipLaunchKernelGGL(k_atw<float>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const float *)op.vals, yout, part);
     SEV_LAUNCH_CHECK();
 }
 
@@ -510,8 +730,14 @@ void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, 
     SEV_LAUNCH_CHECK();
 }
 
-size_t av_smem_bytes(int64_t n, int use_lds) {
-    return sizeof(double) * (size_t)(kWaves * kSlabRows + (use_lds ? n : 0));
+void launch_gemm(const char *name, int64_t m, int work, const double *X, const double *Cf,
+                 int kout, double *Y, const double *extra, const double *scal, hipStream_t st) {
+    ProfScope ps(name, st, 8.0 * m * (work + kout), 2.0 * m * work * kout);
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(m, 16), kWaves),
+                                                      (int64_t)device_cu_count() * 8);
+    hipLaunchKernelGGL(k_gemm_mfma, dim3(grid), dim3(kBlock), 0, st, m, work, X, Cf, kout, Y,
+                       extra, scal);
+    SEV_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -540,7 +766,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     w.part.ensure((size_t)(op.nslab * (n + 2)));
     w.part2.ensure((size_t)(kRedChunks * (n + 2)));
     w.red.ensure((size_t)kDotBlocks * kMaxWork);
-    w.Fv.ensure((size_t)n);
+    w.Fv.ensure((size_t)(n + m));  // F (n) followed by the updated y (m)
     w.scal.ensure(4);
     w.flags.ensure(4);
 
@@ -556,8 +782,6 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     SEV_HIP(hipMemsetAsync(w.flags.get(), 0, sizeof(int) * 4, st));
     SEV_HIP(hipMemsetAsync(w.scal.get(), 0, sizeof(double) * 4, st));
 
-    const int use_lds = n <= kMaxFL ? 1 : 0;
-    const size_t av_smem = av_smem_bytes(n, use_lds);
     const int dot_blocks = (int)std::min<int64_t>(kDotBlocks, ceil_div(m, kBlock));
 
     std::vector<double> Bh((size_t)work * work), BUh((size_t)work * work),
@@ -573,7 +797,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             double *Wd = w.W.get(), *Vd = w.V.get();
             double *y = Wd + (int64_t)j * m;
             // W[:,j] = A v_j - RF w_{j-1}
-            launch_av(op, av_smem, use_lds, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr,
+            launch_av(op, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr,
                       j > j0 ? Wd + (int64_t)(j - 1) * m : nullptr, y, st);
             nmult++;
             // reorthogonalise against W[:,0:j]
@@ -587,7 +811,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                                    w.red.get(), w.T.get());
                 SEV_LAUNCH_CHECK();
             }
-            launch_upd_atw(op, Wd, jc, w.T.get(), y, w.part.get(), st);
+            launch_upd_atw(op, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get(), st);
             nmult++;
             {
                 ProfScope ps("reduce_slabs", st, 8.0 * op.nslab * (n + 2));
@@ -598,17 +822,20 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             }
             launch_right_step(op, j, work, w.part2.get(), Vd, w.Fv.get(), w.B.get(),
                               w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u, st);
-            ProfScope ps("scale_col", st, 16.0 * m);
-            hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st, y, m,
-                               w.scal.get());
-            SEV_LAUNCH_CHECK();
+            {
+                ProfScope ps("scale_col", st, 16.0 * m);
+                hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st,
+                                   w.Fv.get() + n, y, m, w.scal.get());
+                SEV_LAUNCH_CHECK();
+            }
         }
         SEV_HIP(hipMemcpyAsync(Bh.data(), w.B.get(), sizeof(double) * work * work,
                                hipMemcpyDeviceToHost, st));
         SEV_HIP(hipMemcpyAsync(scal_h, w.scal.get(), sizeof scal_h, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipMemcpyAsync(flags_h, w.flags.get(), sizeof flags_h, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipStreamSynchronize(st));
-        if (flags_h[0]) fail(kErrConvergence, "irlba: invariant subspace encountered (rank-deficient operator)");
+        if (flags_h[0])
+            fail(kErrConvergence, "irlba: invariant subspace encountered (rank-deficient operator)");
         small_svd(work, Bh.data(), BUh.data(), BSh.data(), BVh.data());
         double RF = scal_h[2];
         const double RFn = RF;
@@ -638,15 +865,11 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         SEV_HIP(hipMemcpyAsync(w.BV.get(), BVh.data(), sizeof(double) * work * work,
                                hipMemcpyHostToDevice, st));
         if (RFn <= 0.0) fail(kErrConvergence, "irlba: zero residual vector at restart");
-        hipLaunchKernelGGL(k_gemm_small, dim3(grid_rows(n)), dim3(kBlock), 0, st, n, work,
-                           w.V.get(), w.BV.get(), (int)k, w.V2.get(), w.Fv.get(), w.scal.get());
-        SEV_LAUNCH_CHECK();
+        launch_gemm("restart_v", n, work, w.V.get(), w.BV.get(), (int)k, w.V2.get(), w.Fv.get(),
+                    w.scal.get(), st);
         w.V.swap(w.V2);
-        ProfScope ps("restart_w", st, 8.0 * m * (work + k), 2.0 * m * work * k);
-        hipLaunchKernelGGL(k_gemm_small, dim3(grid_rows(m)), dim3(kBlock), 0, st, m, work,
-                           w.W.get(), w.BU.get(), (int)k, w.W2.get(), (const double *)nullptr,
-                           w.scal.get());
-        SEV_LAUNCH_CHECK();
+        launch_gemm("restart_w", m, work, w.W.get(), w.BU.get(), (int)k, w.W2.get(), nullptr,
+                    w.scal.get(), st);
         w.W.swap(w.W2);
         std::fill(Bh.begin(), Bh.end(), 0.0);
         for (int64_t cc = 0; cc < k; ++cc) {
@@ -666,27 +889,22 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                            hipMemcpyHostToDevice, st));
     SEV_HIP(hipMemcpyAsync(w.BV.get(), BVh.data(), sizeof(double) * work * work,
                            hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_gemm_small, dim3(grid_rows(m)), dim3(kBlock), 0, st, m, work, w.W.get(),
-                       w.BU.get(), (int)nu, U, (const double *)nullptr, w.scal.get());
-    SEV_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_gemm_small, dim3(grid_rows(n)), dim3(kBlock), 0, st, n, work, w.V.get(),
-                       w.BV.get(), (int)nu, V, (const double *)nullptr, w.scal.get());
-    SEV_LAUNCH_CHECK();
+    launch_gemm("final_u", m, work, w.W.get(), w.BU.get(), (int)nu, U, nullptr, w.scal.get(), st);
+    launch_gemm("final_v", n, work, w.V.get(), w.BV.get(), (int)nu, V, nullptr, w.scal.get(), st);
     SEV_HIP(hipStreamSynchronize(st));
 }
 
 void op_apply(const SlabCsc &op, int trans, const double *x, double *y, IrlbaWork &w,
               hipStream_t st) {
-    const int64_t m = op.m, n = op.n;
+    const int64_t n = op.n;
     if (!trans) {
-        const int use_lds = n <= kMaxFL ? 1 : 0;
-        launch_av(op, av_smem_bytes(n, use_lds), use_lds, x, nullptr, nullptr, y, st);
+        launch_av(op, x, nullptr, nullptr, y, st);
         return;
     }
     w.part.ensure((size_t)(op.nslab * (n + 2)));
     w.part2.ensure((size_t)(kRedChunks * (n + 2)));
-    // k_upd_atw with jc == 0 leaves its y argument unchanged
-    launch_upd_atw(op, nullptr, 0, nullptr, const_cast<double *>(x), w.part.get(), st);
+    w.Fv.ensure((size_t)(n + op.m));
+    launch_upd_atw(op, nullptr, 0, nullptr, x, w.Fv.get() + n, w.part.get(), st);
     hipLaunchKernelGGL(k_red_slabs, dim3((unsigned)ceil_div(n + 2, kBlock), kRedChunks),
                        dim3(kBlock), 0, st, op.nslab, n + 2, w.part.get(), w.part2.get());
     SEV_LAUNCH_CHECK();
@@ -697,12 +915,12 @@ void op_apply(const SlabCsc &op, int trans, const double *x, double *y, IrlbaWor
         hipLaunchKernelGGL(k_finish_at<float>, dim3((unsigned)ceil_div(n, kBlock)), dim3(kBlock),
                            0, st, n, w.part2.get(), (const float *)op.mu, y);
     SEV_LAUNCH_CHECK();
-    (void)m;
 }
 
 void embedding_f32(const double *U, const double *s_dev, int64_t m, int64_t d, float *Z,
                    int64_t rs, int64_t cs, hipStream_t st) {
-    hipLaunchKernelGGL(k_embed, dim3(grid_rows(m)), dim3(kBlock), 0, st, U, s_dev, m, d, Z, rs, cs);
+    hipLaunchKernelGGL(k_embed, dim3(grid_rows(m)), dim3(kBlock), 0, st, U, s_dev, m, d, Z, rs,
+                       cs);
     SEV_LAUNCH_CHECK();
 }
 

@@ -62,28 +62,40 @@ void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, con
 namespace sev {
 
 // ---- sev_irlba.hip -------------------------------------------------------
-// Device layout of the centered, scaled matrix A = B - 1 mu' (C x F) used by
-// both Lanczos products: B in CSC (the scaled values, not copied) plus, per
-// slab of kSlabRows rows, the position of each column's first entry in the
-// slab (off[s*F + j], slab-major, (nslab+1) x F) and 16-bit in-slab rows.
-constexpr int64_t kSlabRows = 1024;
+// Device layouts of the centered, scaled matrix A = B - 1 mu' (C x F) used by
+// the two Lanczos products:
+//  * A v  : CSR with every row padded to a multiple of 4 entries (16-bit
+//           column indices), read one 4-entry vector per lane;
+//  * A' w : the CSC itself, cut into slabs of kSlabRows rows: off[s*F + j] is
+//           the position of column j's first entry in slab s ((nslab+1) x F,
+//           slab-major) and lrow holds 16-bit in-slab rows.
+constexpr int64_t kSlabRows = 4096;
 
 enum ValType : int { kValF32 = 0, kValBF16 = 1, kValF64 = 2 };
 
 struct SlabCsc {
-    int64_t m = 0, n = 0, nnz = 0, nslab = 0;
-    DevBuf<int64_t> off;     // (nslab + 1) * n
-    DevBuf<uint16_t> lrow;   // nnz
-    DevBuf<uint16_t> vbf16;  // nnz bf16 values (precision form only)
-    const void *vals = nullptr;  // values read by the products (not owned
-                                 // unless they point into vbf16)
+    int64_t m = 0, n = 0, nnz = 0, nslab = 0, csr_nnz = 0;
+    DevBuf<int64_t> off;      // (nslab + 1) * n
+    DevBuf<uint16_t> lrow;    // nnz
+    DevBuf<uint16_t> vbf16;   // nnz bf16 values (precision form only)
+    DevBuf<int64_t> rowptr;   // m + 1 (padded CSR)
+    DevBuf<uint16_t> ccol;    // csr_nnz
+    DevBuf<unsigned char> cval;  // csr_nnz values of vtype
+    int lanes_per_row = 64;
+    const void *vals = nullptr;  // CSC values read by A' w (not owned unless
+                                 // they point into vbf16)
     const void *mu = nullptr;    // mu' (not owned), same float type as the
                                  // source values; nullptr = uncentered
-    int vtype = kValF32;         // type of vals
+    int vtype = kValF32;         // type of vals / cval
     bool mu64 = false;           // mu is double
 };
 
 // val is float (src_f64 == false) or double; bf16 converts float values.
+// scratch holds temporaries of the build (grown as needed).
+struct BuildScratch {
+    DevBuf<int64_t> boff, scan;
+    DevBuf<int32_t> cnt;
+};
 void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                     const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
                     hipStream_t st);
@@ -142,6 +154,29 @@ constexpr int kMaxK = 64;
 template <class T>
 void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st);
+// Same, but result row r is written at row qidx[r] of idx/dist (n x k).
+template <class T>
+void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                    const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st);
+
+// ---- sev_knn_mfma.hip ----------------------------------------------------
+// The same exact kNN (FP32 definition) by MFMA candidate search (FP32 or
+// BF16 operands), exact FP32 re-rank and a certificate; uncertified rows are
+// recomputed exhaustively.  Z is n x d column-major (ld).  Outputs row-major
+// n x k, 0-based.  *n_fallback = rows recomputed exhaustively.
+struct KnnWork {
+    DevBuf<unsigned char> zq, zc;
+    DevBuf<float> zr, nhat, eps, cthr;
+    DevBuf<unsigned> gmax;
+    DevBuf<int32_t> cidx, ccnt, nfail;
+    DevBuf<int64_t> fail;
+};
+void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+              int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,
+              int64_t *n_fallback);
+// Largest deviation of the MFMA lane-layout model from scalar arithmetic
+// (mode 0: 16x16x4 FP32, 1: 16x16x32 BF16); 0 when the model holds.
+double mfma_selftest(int mode);  // mode 2: 16x16x4 FP64
 
 // ---- sev_snn.hip ---------------------------------------------------------
 // Jaccard SNN graph from 0-based row-major neighbour lists (n x k).

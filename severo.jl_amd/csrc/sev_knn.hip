@@ -43,7 +43,7 @@ __device__ __forceinline__ double inf_val<double>() {
 template <class T, int kQB>
 __global__ void __launch_bounds__(kBlock)
 k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_self,
-            const int64_t *qidx, int64_t nq, int32_t *oidx, T *odist) {
+            const int64_t *qidx, int64_t nq, int scatter, int32_t *oidx, T *odist) {
     extern __shared__ unsigned char smem_raw[];
     T *qx = (T *)smem_raw;  // kQB * d
     __shared__ T pendD[kQB][kBlock];
@@ -137,7 +137,7 @@ k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_sel
     for (int q = w; q < nqb; q += kWaves) {
         const int cnt = listN[q];
         if (lane < k) {
-            const int64_t r = q0 + q;
+            const int64_t r = scatter ? qrow[q] : q0 + q;
             oidx[r * k + lane] = lane < cnt ? listJ[q][lane] : -1;
             odist[r * k + lane] = lane < cnt ? sqrt(listD[q][lane]) : inf_val<T>();
         }
@@ -147,8 +147,9 @@ k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_sel
 }  // namespace
 
 template <class T>
-void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
-               const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st) {
+void knn_exact_impl(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                    const int64_t *qidx, int64_t nq, int scatter, int32_t *idx, T *dist,
+                    hipStream_t st) {
     SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
     SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
     // 16 queries per block in FP32, 8 in FP64 (keeps LDS under 64 KB)
@@ -158,13 +159,30 @@ void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int incl
     if (nq == 0) return;
     ProfScope ps("knn_exact", st, 0.0, 2.0 * (double)nq * (double)n * (double)d);
     hipLaunchKernelGGL((k_knn_exact<T, QB>), dim3((unsigned)ceil_div(nq, QB)), dim3(kBlock), smem,
-                       st, X, n, d, ld, (int)k, include_self, qidx, nq, idx, dist);
+                       st, X, n, d, ld, (int)k, include_self, qidx, nq, scatter, idx, dist);
     SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+               const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st) {
+    knn_exact_impl<T>(X, n, d, ld, k, include_self, qidx, nq, 0, idx, dist, st);
+}
+
+template <class T>
+void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                    const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st) {
+    SEV_REQUIRE(qidx != nullptr, "knn_exact_rows needs row indices");
+    knn_exact_impl<T>(X, n, d, ld, k, include_self, qidx, nq, 1, idx, dist, st);
 }
 
 template void knn_exact<float>(const float *, int64_t, int64_t, int64_t, int64_t, int,
                                const int64_t *, int64_t, int32_t *, float *, hipStream_t);
 template void knn_exact<double>(const double *, int64_t, int64_t, int64_t, int64_t, int,
                                 const int64_t *, int64_t, int32_t *, double *, hipStream_t);
+template void knn_exact_rows<float>(const float *, int64_t, int64_t, int64_t, int64_t, int,
+                                    const int64_t *, int64_t, int32_t *, float *, hipStream_t);
+template void knn_exact_rows<double>(const double *, int64_t, int64_t, int64_t, int64_t, int,
+                                     const int64_t *, int64_t, int32_t *, double *, hipStream_t);
 
 }  // namespace sev

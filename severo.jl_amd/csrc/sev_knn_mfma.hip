@@ -1,0 +1,490 @@
+// Exact kNN by MFMA candidate search + exact re-rank + certificate.
+//
+// The result is identical to the exact definition of knn_exact (sev_knn.hip):
+// candidates are proposed by a matrix-core distance GEMM, then every
+// candidate is re-ranked with the exact FP32 distance, and a rigorous error
+// bound certifies that no candidate outside the list could enter the top k.
+// Rows that fail the certificate are recomputed by the exhaustive exact
+// kernel.
+//
+// Distance GEMM.  With q' = [q, 1, (1)] and c' = [c, -|c|^2/2 split over one
+// (FP32) or two (BF16 hi + lo) extra columns], dot(q', c') = q.c - |c|^2/2,
+// so |q - c|^2 = |q|^2 - 2 dot(q', c'): per query the ranking is by dot'
+// descending and the epilogue is one compare per pair.
+//   FP32 form: v_mfma_f32_16x16x4_f32, operands the FP32 embedding.
+//   BF16 form: v_mfma_f32_16x16x32_bf16, operands rounded to BF16 (per point
+//              rounding error eps = |bf16(z) - z|_2 is kept), FP32 accumulate.
+//
+// Certificate.  Every candidate outside a query's final list has
+// dot' <= thr (the list's K'-th value), i.e. d~ >= d~_K' = |q|^2 - 2 thr.
+// The computed d~ differs from |q^ - c^|^2 by at most
+// eta (|q^|^2 + max|c^|^2) + 2^-23 d~ (FP32 accumulation over KP terms and
+// the split of |c|^2/2), |q - c| >= |q^ - c^| - eps_q - max eps, and the
+// exact sequential FP32 distance is >= |q - c|^2 (1 - 2^-24)^(d+3).  If that
+// lower bound exceeds the k-th exact distance among the list, the top k is
+// exact.
+#include "sev_kernels.hpp"
+
+#include <climits>
+#include <cmath>
+
+namespace sev {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int kBlock = 256;
+constexpr int kNW = kBlock / kWave;  // waves per block
+constexpr int kCT = 64;              // candidates per tile
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float ninf() { return -__builtin_huge_valf(); }
+
+// ---------------------------------------------------------------------------
+// Operand preparation (thread per point).  Z column-major (x_it at Z[t*ld+i]).
+// FP32: Zq[i] = [z, 1, 0..], Zc[i] = [z, -n/2, 0..], n = |z|^2.
+// BF16: Zq[i] = [zb, 1, 1, 0..], Zc[i] = [zb, hi, lo, 0..], hi + lo = -|zb|^2/2.
+// Also the row-major FP32 copy Zr used by the exact re-rank, n^ (FP32,
+// rounded up) and eps (FP32, rounded up), and the global maxima of both.
+// ---------------------------------------------------------------------------
+template <int MODE, int KP>
+__global__ void __launch_bounds__(kBlock)
+k_knn_prep(const float *Z, int64_t n, int64_t d, int64_t ld, void *Zq, void *Zc, float *Zr,
+           float *nhat, float *eps, unsigned *gmax) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        double nn = 0.0, ee = 0.0;
+        if (MODE == 0) {
+            float *q = (float *)Zq + i * KP;
+            float *c = (float *)Zc + i * KP;
+            for (int64_t t = 0; t < d; ++t) {
+                const float z = Z[t * ld + i];
+                q[t] = z;
+                c[t] = z;
+                Zr[i * d + t] = z;
+                nn += (double)z * (double)z;
+            }
+            q[d] = 1.0f;
+            c[d] = (float)(-0.5 * nn);
+            for (int64_t t = d + 1; t < KP; ++t) {
+                q[t] = 0.0f;
+                c[t] = 0.0f;
+            }
+        } else {
+            uint16_t *q = (uint16_t *)Zq + i * KP;
+            uint16_t *c = (uint16_t *)Zc + i * KP;
+            for (int64_t t = 0; t < d; ++t) {
+                const float z = Z[t * ld + i];
+                const uint16_t b = f2bf(z);
+                const float zb = bf2f(b);
+                q[t] = b;
+                c[t] = b;
+                Zr[i * d + t] = z;
+                nn += (double)zb * (double)zb;
+                const double e = (double)zb - (double)z;
+                ee += e * e;
+            }
+            const float half = (float)(-0.5 * nn);
+            const uint16_t hi = f2bf(half);
+            const uint16_t lo = f2bf((float)((double)half - (double)bf2f(hi)));
+            q[d] = 0x3F80;  // 1.0
+            q[d + 1] = 0x3F80;
+            c[d] = hi;
+            c[d + 1] = lo;
+            for (int64_t t = d + 2; t < KP; ++t) {
+                q[t] = 0;
+                c[t] = 0;
+            }
+        }
+        const float nf = (float)(nn * (1.0 + 1e-6)) + 1e-30f;
+        const float ef = (float)(sqrt(ee) * (1.0 + 1e-6));
+        nhat[i] = nf;
+        eps[i] = ef;
+        atomicMax(&gmax[0], __float_as_uint(nf));
+        atomicMax(&gmax[1], __float_as_uint(ef));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate search.  A block owns QB = kNW*QW*16 queries (wave w owns
+// QW row-blocks of 16); candidate tiles of kCT are staged in LDS and
+// multiplied with MFMA; each query keeps its KL largest dot' values (ties by
+// lower index) in LDS, updated by one wavefront-wide insertion per passing
+// pair.
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct Elem;
+template <>
+struct Elem<0> {
+    typedef float T;
+};
+template <>
+struct Elem<1> {
+    typedef uint16_t T;
+};
+
+template <int MODE, int KP, int QW, int KL>
+__global__ void __launch_bounds__(kBlock)
+k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_t *cidx,
+           float *cthr, int32_t *ccnt) {
+    typedef typename Elem<MODE>::T E;
+    constexpr int QB = kNW * QW * 16;
+    constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
+    constexpr int KS = MODE == 0 ? KP / 4 : KP / 32;  // MFMA steps per block
+    __shared__ __attribute__((aligned(16))) E tile[kCT * ROW];
+    __shared__ float listV[QB][KL];
+    __shared__ int32_t listI[QB][KL];
+    __shared__ int listN[QB];
+    __shared__ float thrL[QB];
+
+    const E *Zq = (const E *)Zq_;
+    const E *Zc = (const E *)Zc_;
+    const int lane = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int64_t q0 = (int64_t)blockIdx.x * QB;
+    const int col = lane & 15, grp = lane >> 4;
+
+    for (int i = threadIdx.x; i < QB; i += kBlock) {
+        listN[i] = 0;
+        thrL[i] = ninf();
+    }
+    // query operands in registers
+    float af[QW][MODE == 0 ? KS : 1];
+    v8bf ab[QW][MODE == 1 ? KS : 1];
+#pragma unroll
+    for (int rb = 0; rb < QW; ++rb) {
+        const int64_t qrow = q0 + w * QW * 16 + rb * 16 + col;
+        const bool ok = qrow < n;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            if (MODE == 0) {
+                af[rb][s] = ok ? ((const float *)Zq)[qrow * KP + 4 * s + grp] : 0.0f;
+            } else {
+                v8bf v;
+                if (ok) {
+                    v = *(const v8bf *)((const uint16_t *)Zq + qrow * KP + 32 * s + 8 * grp);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+                }
+                ab[rb][s] = v;
+            }
+        }
+    }
+
+    for (int64_t t0 = 0; t0 < n; t0 += kCT) {
+        __syncthreads();
+        if (MODE == 0) {
+            for (int e = threadIdx.x; e < kCT * KP; e += kBlock) {
+                const int r = e / KP, k = e % KP;
+                const int64_t c = t0 + r;
+                tile[r * ROW + k] = c < n ? Zc[c * KP + k] : E(0);
+            }
+        } else {
+            constexpr int CH = KP / 8;  // 16-byte chunks per row
+            for (int e = threadIdx.x; e < kCT * CH; e += kBlock) {
+                const int r = e / CH, part = e % CH;
+                const int64_t c = t0 + r;
+                uint4 v = make_uint4(0, 0, 0, 0);
+                if (c < n) v = *(const uint4 *)((const uint16_t *)Zc + c * KP + part * 8);
+                *(uint4 *)(&tile[r * ROW + part * 8]) = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int cb = 0; cb < kCT / 16; ++cb) {
+            float bfv[MODE == 0 ? KS : 1];
+            v8bf bbv[MODE == 1 ? KS : 1];
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                if (MODE == 0)
+                    bfv[s] = ((const float *)tile)[(cb * 16 + col) * ROW + 4 * s + grp];
+                else
+                    bbv[s] = *(const v8bf *)((const uint16_t *)tile + (cb * 16 + col) * ROW +
+                                             32 * s + 8 * grp);
+            }
+            const int64_t c = t0 + cb * 16 + col;
+#pragma unroll
+            for (int rb = 0; rb < QW; ++rb) {
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < KS; ++s) {
+                    if (MODE == 0)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rb][s], bfv[s], acc, 0, 0, 0);
+                    else
+                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[rb][s], bbv[s], acc, 0, 0,
+                                                                      0);
+                }
+                const int rbase = w * QW * 16 + rb * 16 + 4 * grp;  // local query row of acc[0]
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ql = rbase + i;
+                    const int64_t q = q0 + ql;
+                    const bool pass = c < n && q < n && (include_self || c != q) &&
+                                      acc[i] > thrL[ql];
+                    uint64_t mask = __ballot(pass);
+                    while (mask) {
+                        const int src = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const float v = __shfl(acc[i], src, kWave);
+                        const int qlo = w * QW * 16 + rb * 16 + 4 * (src >> 4) + i;
+                        const int32_t cand = (int32_t)(t0 + cb * 16 + (src & 15));
+                        if (!(v > thrL[qlo])) continue;
+                        // insert (v, cand) into the descending list of qlo
+                        const int cnt = listN[qlo];
+                        const float lv = lane < cnt && lane < KL ? listV[qlo][lane] : ninf();
+                        const int32_t li = lane < cnt && lane < KL ? listI[qlo][lane] : INT_MAX;
+                        const bool better = lane < cnt && (lv > v || (lv == v && li < cand));
+                        const int pos = __popcll(__ballot(better));
+                        const float uv = __shfl_up(lv, 1, kWave);
+                        const int32_t ui = __shfl_up(li, 1, kWave);
+                        const int ncnt = cnt + 1 < KL ? cnt + 1 : KL;
+                        if (lane < KL && lane >= pos && lane < ncnt) {
+                            listV[qlo][lane] = lane == pos ? v : uv;
+                            listI[qlo][lane] = lane == pos ? cand : ui;
+                        }
+                        const float last = __shfl(lane == pos ? v : (lane > pos ? uv : lv), KL - 1,
+                                                  kWave);
+                        if (lane == 0) {
+                            listN[qlo] = ncnt;
+                            thrL[qlo] = ncnt == KL ? last : ninf();
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int ql = w; ql < QB; ql += kNW) {
+        const int64_t q = q0 + ql;
+        if (q >= n) continue;
+        if (lane < KL) cidx[q * KL + lane] = lane < listN[ql] ? listI[ql][lane] : -1;
+        if (lane == 0) {
+            cthr[q] = thrL[ql];
+            ccnt[q] = listN[ql];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact re-rank and certificate (one wavefront per query).
+// ---------------------------------------------------------------------------
+template <int KL>
+__global__ void __launch_bounds__(kBlock)
+k_knn_rerank(const float *Zr, int64_t n, int d, int k, const int32_t *cidx, const float *cthr,
+             const int32_t *ccnt, const float *nhat, const float *eps, const unsigned *gmax,
+             double eta, double spec_factor, int32_t *oidx, float *odist, int32_t *nfail,
+             int64_t *fail) {
+    const int lane = lane_id();
+    const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (q >= n) return;
+    const int cnt = ccnt[q];
+    const int32_t c = lane < cnt && lane < KL ? cidx[q * KL + lane] : -1;
+    float D = __builtin_huge_valf();
+    if (c >= 0) {
+        const float *zq = Zr + q * d;
+        const float *zc = Zr + (int64_t)c * d;
+        float acc = 0.0f;
+        for (int t = 0; t < d; ++t) {
+            const float diff = zq[t] - zc[t];
+            acc = acc + diff * diff;
+        }
+        D = acc;
+    }
+    int32_t ci = c >= 0 ? c : INT_MAX;
+    // bitonic sort of (D, ci) ascending across the 64 lanes
+    for (int size = 2; size <= kWave; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const float oD = __shfl_xor(D, stride, kWave);
+            const int32_t oc = __shfl_xor(ci, stride, kWave);
+            const bool up = (lane & size) == 0;
+            const bool lower = (lane & stride) == 0;
+            const bool o_less = oD < D || (oD == D && oc < ci);
+            const bool o_greater = oD > D || (oD == D && oc > ci);
+            const bool take = (lower == up) ? o_less : o_greater;
+            if (take) {
+                D = oD;
+                ci = oc;
+            }
+        }
+    }
+    const float Dk = __shfl(D, k - 1, kWave);
+    bool ok;
+    if (cnt < KL) {
+        ok = cnt >= k;  // every admissible candidate was seen and re-ranked
+    } else {
+        const double ncmax = (double)__uint_as_float(gmax[0]);
+        const double emax = (double)__uint_as_float(gmax[1]);
+        const double nq = (double)nhat[q];
+        const double dt = nq - 2.0 * (double)cthr[q];
+        const double lb2 = dt * (1.0 - 0x1p-22) - eta * (nq + ncmax) - 1e-30;
+        const double lb = lb2 > 0.0 ? sqrt(lb2) - (double)eps[q] - emax : -1.0;
+        ok = lb > 0.0 && lb * lb * spec_factor > (double)Dk;
+    }
+    ok = ok && !(Dk != Dk);
+    if (ok) {
+        if (lane < k) {
+            oidx[q * k + lane] = ci;
+            odist[q * k + lane] = sqrtf(D);
+        }
+    } else if (lane == 0) {
+        fail[atomicAdd(nfail, 1)] = q;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Self-test of the MFMA operand layouts assumed above: computes a 16x16 tile
+// with MFMA and with scalar code, returns the largest difference.
+// ---------------------------------------------------------------------------
+__global__ void k_selftest_mfma(int mode, float *out) {
+    const int lane = lane_id();
+    const int col = lane & 15, grp = lane >> 4;
+    auto A = [](int m, int kk) { return (float)((m * 7 + kk * 3) % 11) - 5.0f; };
+    auto B = [](int kk, int nn) { return (float)((kk * 5 + nn * 9) % 13) - 6.0f; };
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    int K;
+    if (mode == 2) {
+        typedef double v4d __attribute__((ext_vector_type(4)));
+        v4d a64 = {0.0, 0.0, 0.0, 0.0};
+        a64 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)A(col, grp), (double)B(grp, col), a64,
+                                                    0, 0, 0);
+        for (int i = 0; i < 4; ++i) acc[i] = (float)a64[i];
+        K = 4;
+    } else if (mode == 0) {
+        K = 4;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A(col, grp), B(grp, col), acc, 0, 0, 0);
+    } else {
+        K = 32;
+        v8bf a, b;
+        for (int e = 0; e < 8; ++e) {
+            a[e] = (__bf16)A(col, 8 * grp + e);
+            b[e] = (__bf16)B(8 * grp + e, col);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    float err = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        // FP32/BF16 16x16: lane holds rows 4*(l/16) + i; FP64 16x16x4: (l/16) + 4i
+        const int m = mode == 2 ? grp + 4 * i : 4 * grp + i, nn = col;
+        float ref = 0.0f;
+        for (int kk = 0; kk < K; ++kk) ref += A(m, kk) * B(kk, nn);
+        err = fmaxf(err, fabsf(ref - acc[i]));
+    }
+    for (int o = 32; o > 0; o >>= 1) err = fmaxf(err, __shfl_xor(err, o, kWave));
+    if (lane == 0) out[0] = err;
+}
+
+}  // namespace
+
+double mfma_selftest(int mode) {
+    DevBuf<float> o;
+    o.ensure(1);
+    hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(kWave), 0, 0, mode, o.get());
+    SEV_LAUNCH_CHECK();
+    float h = -1.0f;
+    SEV_HIP(hipMemcpy(&h, o.get(), sizeof h, hipMemcpyDeviceToHost));
+    return h;
+}
+
+namespace {
+
+template <int MODE, int KP, int QW, int KL>
+void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+              KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback) {
+    const size_t es = MODE == 0 ? 4 : 2;
+    w.zq.ensure((size_t)n * KP * es);
+    w.zc.ensure((size_t)n * KP * es);
+    w.zr.ensure((size_t)(n * d));
+    w.nhat.ensure((size_t)n);
+    w.eps.ensure((size_t)n);
+    w.gmax.ensure(2);
+    w.cidx.ensure((size_t)n * KL);
+    w.cthr.ensure((size_t)n);
+    w.ccnt.ensure((size_t)n);
+    w.nfail.ensure(1);
+    w.fail.ensure((size_t)n);
+    SEV_HIP(hipMemsetAsync(w.gmax.get(), 0, 2 * sizeof(unsigned), st));
+    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, sizeof(int32_t), st));
+    {
+        ProfScope ps("knn_prep", st, 4.0 * n * d * 2 + 2.0 * n * KP * es);
+        const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
+        hipLaunchKernelGGL((k_knn_prep<MODE, KP>), dim3(g), dim3(kBlock), 0, st, Z, n, d, ld,
+                           (void *)w.zq.get(), (void *)w.zc.get(), w.zr.get(), w.nhat.get(),
+                           w.eps.get(), w.gmax.get());
+        SEV_LAUNCH_CHECK();
+    }
+    {
+        constexpr int QB = kNW * QW * 16;
+        ProfScope ps(MODE == 0 ? "knn_cand_fp32" : "knn_cand_bf16", st, 0.0,
+                     2.0 * (double)n * (double)n * (double)d);
+        hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)ceil_div(n, QB)),
+                           dim3(kBlock), 0, st, (const void *)w.zq.get(), (const void *)w.zc.get(),
+                           n, include_self, w.cidx.get(), w.cthr.get(), w.ccnt.get());
+        SEV_LAUNCH_CHECK();
+    }
+    {
+        const double eta = 4.0 * (KP + 2) * 0x1p-24 + (MODE == 0 ? 0x1p-22 : 0x1p-14);
+        const double spec = pow(1.0 - 0x1p-24, (double)(d + 3));
+        ProfScope ps("knn_rerank", st, (double)n * KL * (4.0 + 4.0 * d));
+        hipLaunchKernelGGL((k_knn_rerank<KL>), dim3((unsigned)ceil_div(n * kWave, kBlock)),
+                           dim3(kBlock), 0, st, w.zr.get(), n, (int)d, (int)k, w.cidx.get(),
+                           w.cthr.get(), w.ccnt.get(), w.nhat.get(), w.eps.get(), w.gmax.get(),
+                           eta, spec, idx, dist, w.nfail.get(), w.fail.get());
+        SEV_LAUNCH_CHECK();
+    }
+    int32_t nf = 0;
+    SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    if (nf > 0)
+        knn_exact_rows<float>(Z, n, d, ld, k, include_self, w.fail.get(), nf, idx, dist, st);
+    if (n_fallback) *n_fallback = nf;
+}
+
+}  // namespace
+
+void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+              int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,
+              int64_t *n_fallback) {
+    SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
+    SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
+    if (bf16) {
+        SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
+        if (k <= 24) {
+            if (d + 2 <= 64)
+                run_mfma<1, 64, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+            else
+                run_mfma<1, 128, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        } else {
+            if (d + 2 <= 64)
+                run_mfma<1, 64, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+            else
+                run_mfma<1, 128, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        }
+    } else {
+        SEV_REQUIRE(d + 1 <= 128, "knn: FP32 MFMA form supports d <= 127");
+        if (d + 1 <= 52) {
+            if (k <= 24)
+                run_mfma<0, 52, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+            else
+                run_mfma<0, 52, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        } else {
+            if (k <= 24) {
+                run_mfma<0, 128, 1, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+            } else {
+                // wide rows with long lists exceed the LDS budget: exhaustive
+                knn_exact<float>(Z, n, d, ld, k, include_self, nullptr, n, idx, dist, st);
+                if (n_fallback) *n_fallback = n;
+            }
+        }
+    }
+}
+
+}  // namespace sev

@@ -30,6 +30,7 @@ constexpr int kWaveSlots = 2048;   // per-wave table
 constexpr int kWaveMaxP = 1024;    // candidates handled by the wave path
 constexpr int kBlockSlots = 4096;  // per-block LDS table
 constexpr int kBlockMaxP = 2048;
+constexpr int kBlockKept = 2048;   // LDS sort buffer of kept entries
 
 // Entries outside [0, n) (an incomplete neighbour list) are ignored.
 __global__ void __launch_bounds__(kBlock)
@@ -241,8 +242,10 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
             const int64_t *gtab_off, const T *wtab, const int64_t *colptr, int32_t *rows,
             T *vals) {
     __shared__ uint64_t ltab[kBlockSlots];
+    __shared__ uint64_t lkept[kBlockKept];
     __shared__ int64_t sbeg[kMaxK], spre[kMaxK + 1];
     __shared__ int wk[kWaves];
+    __shared__ int nkept;
     const int64_t h = blockIdx.x;
     if (h >= nheavy) return;
     const int64_t j = heavy[h];
@@ -258,14 +261,17 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
     }
     __syncthreads();
     const int64_t P = spre[kMaxK];
-    uint64_t *tab;
+    uint64_t *tab, *kept;
     uint32_t nslots;
     if (P <= kBlockMaxP) {
         tab = ltab;
+        kept = lkept;
         nslots = kBlockSlots;
     } else {
+        // global region: nslots table slots followed by nslots/2 sort slots
         tab = gtab + gtab_off[h];
-        nslots = (uint32_t)(gtab_off[h + 1] - gtab_off[h]);
+        nslots = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
+        kept = tab + nslots;
     }
     for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) tab[i] = 0;
     __syncthreads();
@@ -298,18 +304,23 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
         if (threadIdx.x == 0) cnt[j] = L;
         return;
     }
-    // clear pruned slots, then sort the whole table ascending: empty slots
-    // (0) come first and the L kept entries end up last, ordered by row
+    // gather the L kept entries (any order), pad to a power of two with the
+    // largest key and sort ascending: rows come out in order
+    if (threadIdx.x == 0) nkept = 0;
+    __syncthreads();
     for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) {
         const uint64_t v = tab[i];
-        if (v != 0 && (int)(v & 0xFFFFFFFFu) < xmin) tab[i] = 0;
+        if (v != 0 && (int)(v & 0xFFFFFFFFu) >= xmin) kept[atomicAdd(&nkept, 1)] = v;
     }
     __syncthreads();
-    block_bitonic(tab, (int)nslots);
+    int Lp = 1;
+    while (Lp < L) Lp <<= 1;
+    for (int i = L + threadIdx.x; i < Lp; i += kBlock) kept[i] = ~0ull;
+    __syncthreads();
+    block_bitonic(kept, Lp);
     const int64_t ob = colptr[j];
-    const int64_t first = (int64_t)nslots - L;
     for (int64_t i = threadIdx.x; i < L; i += kBlock) {
-        const uint64_t v = tab[first + i];
+        const uint64_t v = kept[i];
         rows[ob + i] = (int32_t)((v >> 32) - 1);
         vals[ob + i] = wtab[v & 0xFFFFFFFFu];
     }
@@ -369,6 +380,7 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
             if (hp[h] > kBlockMaxP) {
                 slots = 1;
                 while (slots < 2 * hp[h]) slots <<= 1;
+                slots += slots / 2;  // table + sort buffer (>= next_pow2(P))
             }
             offs[h + 1] = offs[h] + slots;
         }

@@ -1,0 +1,185 @@
+# SeveroHIP.jl -- Julia binding of libsevero_hip.so (MI355X / gfx950) for
+# Severo.jl's embedding-and-graph path.
+#
+# Two ways to use it:
+#   1. Drop-in at the native boundary: point Severo's `libcell` at this
+#      library (it exports FindNeighboursEuclidean32/64 with libcell's
+#      signature, src/neighbours.jl:32-63); see INTEGRATION.md.
+#   2. The functions below, which take and return the same types as
+#      Severo's API (normalize_cells, scale_features, embedding/pca,
+#      nearest_neighbours, jaccard_index, shared_nearest_neighbours) and run
+#      every stage on the GPU.  `Pipeline` keeps a dataset resident on the
+#      device across stages.
+module SeveroHIP
+
+using SparseArrays, LinearAlgebra
+import NamedArrays: NamedArray, NamedMatrix, names, dimnames
+import Severo
+
+const libsev = get(ENV, "SEVERO_HIP_LIB", joinpath(@__DIR__, "..", "libsevero_hip.so"))
+
+const SEV_FLOAT32, SEV_FLOAT64 = Cint(0), Cint(1)
+const SEV_INT32, SEV_INT64 = Cint(0), Cint(1)
+const SEV_PREC = Dict(:fp32 => Cint(0), :bf16 => Cint(1), :exact => Cint(2))
+
+function check(rc::Integer)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:sev_last_error, libsev), Cstring, ()))
+    rc == -3 && error("convergence failed: $msg")
+    error("libsevero_hip: $msg (status $rc)")
+end
+
+dtcode(::Type{Float32}) = SEV_FLOAT32
+dtcode(::Type{Float64}) = SEV_FLOAT64
+
+# ---------------------------------------------------------------- normalize
+# src/normalize.jl:40-55
+function normalize_cells(X::SparseMatrixCSC{<:Integer,Int64}; method=:lognormalize,
+                         scale_factor::Real=1.0, dtype::Type{T}=Float64) where {T<:AbstractFloat}
+    method = Symbol(method)
+    meth = method == :lognormalize ? Cint(0) :
+           method == :relativecounts ? Cint(1) : error("unknown normalization method: $method")
+    m, n = size(X)
+    nz = eltype(X) == Int32 ? nonzeros(X) : convert(Vector{Int64}, nonzeros(X))
+    ct = eltype(nz) == Int32 ? SEV_INT32 : SEV_INT64
+    out = Vector{T}(undef, nnz(X))
+    check(ccall((:sev_normalize_cells, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Cint, Cint, Cdouble, Cint,
+                 Ptr{Cvoid}),
+                m, n, X.colptr, X.rowval, nz, ct, meth, scale_factor, dtcode(T), out))
+    SparseMatrixCSC(m, n, copy(X.colptr), copy(X.rowval), out)
+end
+
+normalize_cells(X::NamedArray; kw...) = NamedArray(normalize_cells(X.array; kw...), X.dicts,
+                                                   X.dimnames)
+
+# ----------------------------------------------------------------- scaling
+# src/scaling.jl:199-217, :335-357
+function scale_features(X::NamedArray{T,2,SparseMatrixCSC{T,Int64}}; scale_max::Real=Inf,
+                        dtype::Type{R}=T, features=nothing) where {T<:AbstractFloat,R<:AbstractFloat}
+    if features !== nothing
+        X = X[:, features]
+    end
+    A = X.array
+    m, n = size(A)
+    vals = convert(Vector{R}, nonzeros(A))
+    out = similar(vals)
+    mu = Vector{R}(undef, n)
+    check(ccall((:sev_scale_features, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Cint, Cdouble, Ptr{Cvoid},
+                 Ptr{Cvoid}),
+                m, n, A.colptr, A.rowval, vals, dtcode(R), scale_max, out, mu))
+    B = SparseMatrixCSC(m, n, copy(A.colptr), copy(A.rowval), out)
+    Severo.CenteredMatrix(NamedArray(B, X.dicts, X.dimnames),
+                          NamedArray(mu, (X.dicts[2],), (X.dimnames[2],)))
+end
+
+# -------------------------------------------------------------------- PCA
+# src/irlba.jl:47-85 and src/embedding.jl:46-94 (algorithm = :irlba)
+function irlba(C::Severo.CenteredMatrix, nu::Integer; tol=1e-5, maxit=1000, init=nothing,
+               precision=:fp32)
+    A, mu = Severo._A_mu(C)
+    m, n = size(A)
+    init = init === nothing ? randn(n) : convert(Vector{Float64}, init)
+    T = eltype(nonzeros(A))
+    vals = nonzeros(A)
+    muv = convert(Vector{T}, mu)
+    s = zeros(nu)
+    U = zeros(m, nu)
+    V = zeros(n, nu)
+    iters = Ref{Int64}(0)
+    nmult = Ref{Int64}(0)
+    check(ccall((:sev_irlba_centered_csc, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Int64,
+                 Int64, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint,
+                 Ref{Int64}, Ref{Int64}),
+                m, n, A.colptr, A.rowval, vals, muv, dtcode(T), nu, nu + 7, maxit, tol, init, s, U,
+                V, SEV_PREC[precision], iters, nmult))
+    SVD(U, s, V')
+end
+
+function pca(X::Severo.CenteredMatrix, npcs::Integer; kw...)
+    m, n = size(X)
+    npcs = min(min(m, n), npcs)
+    S = irlba(X, npcs; kw...)
+    Z = S.U[:, 1:npcs] * Diagonal(S.S[1:npcs])
+    stdev = S.S[1:npcs] ./ sqrt(max(1, m - 1))
+    latent = map(x -> string("PC-", x), 1:npcs)
+    rown, coln = names(X.A)
+    rowdim, _ = dimnames(X.A)
+    Severo.LinearEmbedding(X, NamedArray(Z, (rown, latent), (rowdim, :latent)),
+                           NamedArray(stdev, (latent,), (:latent,)),
+                           NamedArray(Matrix(S.V[:, 1:npcs]), (coln, latent), (rowdim, :latent)))
+end
+
+embedding(X, n::Integer=50; method=:pca, kw...) =
+    Symbol(method) == :pca ? pca(X, n; kw...) : error("unknown reduction method: $method")
+
+# ------------------------------------------------------------- neighbours
+# exact kNN behind ann! (src/neighbours.jl:19-63); indices 1-based, n x k
+function ann(X::StridedMatrix{T}, k::Integer; include_self::Bool=true,
+             precision=:fp32) where {T<:Union{Float32,Float64}}
+    n, d = size(X)
+    @assert stride(X, 1) == 1
+    idx = Matrix{Int32}(undef, n, k)
+    dist = Matrix{T}(undef, n, k)
+    nfb = Ref{Int64}(0)
+    check(ccall((:sev_knn, libsev), Cint,
+                (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Int64, Cint, Cint, Ptr{Int32}, Ptr{Cvoid},
+                 Ref{Int64}),
+                X, n, d, stride(X, 2), dtcode(T), k, include_self, SEV_PREC[precision], idx, dist,
+                nfb))
+    idx, dist
+end
+
+# src/neighbours.jl:76-86
+function nearest_neighbours(X::NamedMatrix, k::Integer; dims=:, include_self::Bool=true, kw...)
+    Xd = dims === (:) ? X.array : view(X.array, :, dims)
+    idx, _ = ann(Xd, k; include_self=include_self, kw...)
+    n = size(idx, 1)
+    nn = sparse(vec(idx'), repeat(1:n, inner=k), trues(length(idx)))
+    NamedArray(nn, (X.dicts[1], X.dicts[1]), (X.dimnames[1], X.dimnames[1]))
+end
+nearest_neighbours(em::Severo.LinearEmbedding, k::Integer; kw...) =
+    nearest_neighbours(em.coordinates, k; kw...)
+
+function _snn(idx::Matrix{Int32}, k::Integer, prune::Real, ::Type{T}) where {T<:AbstractFloat}
+    n = size(idx, 1)
+    nz = Ref{Int64}(0)
+    check(ccall((:sev_snn_jaccard, libsev), Cint,
+                (Int64, Int64, Ptr{Int32}, Cdouble, Cint, Ref{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Cvoid}),
+                n, k, idx, prune, dtcode(T), nz, C_NULL, C_NULL, C_NULL))
+    colptr = Vector{Int64}(undef, n + 1)
+    rowval = Vector{Int64}(undef, nz[])
+    nzval = Vector{T}(undef, nz[])
+    check(ccall((:sev_snn_jaccard, libsev), Cint,
+                (Int64, Int64, Ptr{Int32}, Cdouble, Cint, Ref{Int64}, Ptr{Int64}, Ptr{Int64},
+                 Ptr{Cvoid}),
+                n, k, idx, prune, dtcode(T), nz, colptr, rowval, nzval))
+    SparseMatrixCSC(n, n, colptr, rowval, nzval)
+end
+
+# src/neighbours.jl:127-131 (k taken from the column counts)
+function jaccard_index(nn::NamedArray; prune::Real=1/15, dtype::Type{T}=Float64) where {T}
+    A = nn.array
+    counts = diff(A.colptr)
+    k = counts[1]
+    all(==(k), counts) || error("every column of nn must hold k neighbours")
+    idx = Matrix{Int32}(reshape(rowvals(A), k, :)')
+    S = _snn(idx, k, prune, T)
+    NamedArray(S, (nn.dicts[1], nn.dicts[1]), (nn.dimnames[1], nn.dimnames[1]))
+end
+
+# src/neighbours.jl:272-278, :302
+function shared_nearest_neighbours(X::NamedMatrix{T}, k::Integer; dims=:, include_self::Bool=true,
+                                   prune::Real=1/15, kw...) where {T}
+    Xd = dims === (:) ? X.array : view(X.array, :, dims)
+    idx, _ = ann(Xd, k; include_self=include_self, kw...)
+    S = _snn(idx, k, prune, T)
+    NamedArray(S, (X.dicts[1], X.dicts[1]), (X.dimnames[1], X.dimnames[1]))
+end
+shared_nearest_neighbours(em::Severo.LinearEmbedding, k::Integer; kw...) =
+    shared_nearest_neighbours(em.coordinates, k; kw...)
+
+end # module

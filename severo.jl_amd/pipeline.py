@@ -15,7 +15,8 @@ import scipy.sparse as sp
 from . import _lib
 from ._lib import check
 
-PRECISION = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16}
+PRECISION = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16,
+             "exact": _lib.SEV_PREC_EXACT}
 
 
 def _ptr(a):

@@ -104,3 +104,54 @@ def test_pipeline_graph_end_to_end(sev, orc):
     assert nnz == len(rv)
     assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.indices, rv - 1)
     assert np.array_equal(S.data, nz)
+
+
+@pytest.mark.parametrize("what", [0, 1, 2])
+def test_mfma_lane_layout_selftest(sev, what):
+    import ctypes
+    r = ctypes.c_double(-1.0)
+    sev._lib.check(sev.load().sev_selftest(what, ctypes.byref(r)))
+    assert r.value == 0.0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "exact"])
+@pytest.mark.parametrize("include_self", [True, False])
+@pytest.mark.parametrize("n,d,k", [(5000, 50, 20), (3000, 30, 30), (700, 10, 5), (130, 3, 64)])
+def test_ann_precision_forms_bit_exact(sev, orc, precision, include_self, n, d, k):
+    rng = np.random.default_rng(n * 7 + d)
+    # clustered data: neighbours much closer than the norms
+    centers = rng.standard_normal((max(n // 200, 2), d)) * 5.0
+    X = (centers[rng.integers(0, len(centers), n)] + rng.standard_normal((n, d))).astype(np.float32)
+    idx, dist, nfb = sev.ann(X, k, include_self=include_self, precision=precision,
+                             return_fallback=True)
+    ridx, rdist = orc.knn(X, k, include_self=include_self)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist)
+    print(f"{precision} n={n} d={d} k={k}: {nfb} fallback rows")
+
+
+def test_ann_mfma_duplicates(sev, orc):
+    rng = np.random.default_rng(8)
+    base = rng.standard_normal((1000, 20)).astype(np.float32)
+    X = np.concatenate([base, base, base[:300]])
+    for prec in ("fp32", "bf16"):
+        idx, dist = sev.ann(X, 10, precision=prec)
+        ridx, rdist = orc.knn(X, 10)
+        assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_pipeline_neighbours_forms(sev, orc, precision):
+    t = sev.nb_tables(3000, seed=1, density=0.05)
+    C = 6000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    p.normalize("lognormalize", 1e4)
+    p.select_scale(sev.top_mean_features(t, 600), 10.0)
+    p.pca(30, init=np.random.default_rng(0).standard_normal(600))
+    _, _, _, Z = p.get_pca()
+    nfb = p.neighbours(20, precision=precision)
+    idx, dist = p.get_neighbours(C)
+    ridx, rdist = orc.knn(Z, 20)
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+    print(f"pipeline {precision}: {nfb} of {C} rows fell back")

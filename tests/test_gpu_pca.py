@@ -150,3 +150,20 @@ def test_pipeline_pca_vs_gram_oracle(sev, precision, tol_s):
     Q = B.toarray().astype(np.float64) - mu.astype(np.float64)
     res = np.linalg.norm(Q.T @ U - V * s) / np.linalg.norm(Q)
     assert res < (1e-5 if precision == "fp32" else 1e-3)
+
+
+@pytest.mark.parametrize("m,n,dens", [(3000, 600, 0.5), (9000, 400, 0.3), (4000, 900, 0.08),
+                                      (2500, 8000, 0.02), (70000, 50, 0.2)])
+def test_centered_mul_layouts(sev, orc, m, n, dens):
+    """Row lengths that select every CSR lane-group width, a feature count
+    above the LDS limit of A v, and more slabs than column groups."""
+    rng = np.random.default_rng(m + n)
+    A = sp.random(m, n, dens, format="csc", random_state=m, dtype=np.float64).astype(np.float32)
+    mu = rng.random(n).astype(np.float32)
+    Cm = sev.CenteredMatrix(A, mu)
+    x = rng.standard_normal(n)
+    w = rng.standard_normal(m)
+    assert np.allclose(Cm @ x, orc.centered_mul(m, A.indptr, A.indices, A.data, mu, x),
+                       rtol=1e-12, atol=1e-11)
+    assert np.allclose(Cm.rmatvec(w), orc.centered_mul(m, A.indptr, A.indices, A.data, mu, w,
+                                                       trans=True), rtol=1e-12, atol=1e-10)
