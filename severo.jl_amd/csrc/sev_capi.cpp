@@ -415,7 +415,7 @@ int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *fea
         SEV_HIP(hipStreamSynchronize(st));
         p->frow.ensure((size_t)fnnz);
         p->fval.ensure((size_t)fnnz);
-        p->bval.ensure((size_t)fnnz);
+        p->bval.ensure((size_t)fnnz + kValuePad);
         p->mu.ensure((size_t)nfeat);
         p->sd.ensure((size_t)nfeat);
         p->clip.ensure((size_t)nfeat);
@@ -472,6 +472,7 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, c
         SEV_REQUIRE(init != nullptr, "init is NULL");
         build_slab_csc(p->op, m, n, p->fcp.get(), p->frow.get(), p->bval.get(), false,
                        p->mu.get(), precision == SEV_PREC_BF16, st);
+        autotune_operator(p->op, st);
         IrlbaParams prm;
         prm.nu = nu;
         prm.work = nu + 7;
@@ -741,7 +742,8 @@ int sev_centered_mul(int64_t m, int64_t n, const int64_t *colptr, const int64_t 
         csc_to_device(m, n, colptr, rowval, cp, row, nnz, s.st);
         const size_t es = dtype == SEV_FLOAT32 ? 4 : 8;
         DevBuf<unsigned char> vals, dmu;
-        vals.ensure((size_t)nnz * es);
+        vals.ensure((size_t)(nnz + kValuePad) * es);
+        SEV_HIP(hipMemsetAsync(vals.get(), 0, (size_t)(nnz + kValuePad) * es, s.st));
         if (nnz)
             SEV_HIP(hipMemcpyAsync(vals.get(), nzval, (size_t)nnz * es, hipMemcpyHostToDevice, s.st));
         if (mu) {
@@ -781,7 +783,8 @@ int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const in
         int64_t nnz = 0;
         csc_to_device(m, n, colptr, rowval, cp, row, nnz, st.st);
         const size_t es = dtype == SEV_FLOAT32 ? 4 : 8;
-        vals.ensure((size_t)nnz * es);
+        vals.ensure((size_t)(nnz + kValuePad) * es);
+        SEV_HIP(hipMemsetAsync(vals.get(), 0, (size_t)(nnz + kValuePad) * es, st.st));
         if (nnz)
             SEV_HIP(hipMemcpyAsync(vals.get(), B, (size_t)nnz * es, hipMemcpyHostToDevice, st.st));
         if (mu) {

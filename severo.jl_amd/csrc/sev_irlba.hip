@@ -22,6 +22,9 @@
 
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace sev {
@@ -332,7 +335,10 @@ k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const do
 // --------------------------------------------------------------------------
 // Slab partials of B' y: block (slab s, column group g) stages the slab of y
 // in LDS and computes the dots of the columns of its group, one wavefront per
-// column segment (two accumulators, combined in order).
+// column segment.  Each lane reads 4 consecutive entries per step from the
+// 4-aligned cover of the segment (entries outside it are masked; the value
+// and row arrays are padded so the cover stays in bounds); two accumulators
+// alternate and are combined in order.
 // --------------------------------------------------------------------------
 template <class VT>
 __global__ void __launch_bounds__(kBlock)
@@ -351,12 +357,17 @@ k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *
     for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
         const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
         double acc0 = 0.0, acc1 = 0.0;
-        int64_t p = a + lane;
-        for (; p + kWave < b; p += 2 * kWave) {
-            acc0 = fma(ldval(val, p), wl[lrow[p]], acc0);
-            acc1 = fma(ldval(val, p + kWave), wl[lrow[p + kWave]], acc1);
+        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
+            const ushort4 l4 = *(const ushort4 *)(lrow + p);
+            double x[4];
+            ld4(val, p, x);
+            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
+                       v3 = p + 3 < b;
+            acc0 = fma(v0 ? x[0] : 0.0, wl[v0 ? l4.x : 0], acc0);
+            acc1 = fma(v1 ? x[1] : 0.0, wl[v1 ? l4.y : 0], acc1);
+            acc0 = fma(v2 ? x[2] : 0.0, wl[v2 ? l4.z : 0], acc0);
+            acc1 = fma(v3 ? x[3] : 0.0, wl[v3 ? l4.w : 0], acc1);
         }
-        if (p < b) acc0 = fma(ldval(val, p), wl[lrow[p]], acc0);
         const double acc = wave_sum(acc0 + acc1);
         if (lane == 0) prow[j] = acc;
     }
@@ -400,68 +411,117 @@ __device__ double hash_normal(uint64_t seed, uint32_t i) {
 }
 
 // --------------------------------------------------------------------------
-// Right-side step (single block): finishes A' w_j from the partials, forms
-// F = A' w_j - S v_j, reorthogonalises it against V[:,0:j+1], and writes
-// v_{j+1} = F/||F|| plus the bidiagonal entries B(j,j) = S, B(j,j+1) = RF.
-// scal: [0] S, [1] RF (next step), [2] ||F|| after the last step.
-// flags[0] is set when ||w_j|| underflows (invariant subspace).
+// Right-side step, three launches over the n-vectors (n = features):
+//  A: every block forms F = (B'y - sum(y) mu')/S - S v_j for its rows
+//     (S = ||y|| from the slab partials) and the block partials of
+//     V[:,0:j+1]' F;
+//  B: every block sums those partials in block order (T), applies
+//     F -= V[:,0:j+1] T to its rows and writes the block partial of ||F||^2;
+//  C: one block: RF = ||F||, v_{j+1} = F / RF, B(j,j) = S, B(j,j+1) = RF
+//     (a deterministic pseudo-random direction replaces F when it vanishes).
+// scal: [0] S, [1] RF (next step), [2] ||F|| after the last step; flags[0]
+// is set when ||y|| underflows (invariant subspace).
 // --------------------------------------------------------------------------
+constexpr int kRsBlock = 256;
+
 template <class MT>
-__global__ void __launch_bounds__(1024)
-k_right_step(int64_t n, int j, int work, const double *part2, const MT *mu, double *V,
-             double *Fv, double *B, double *scal, int *flags, uint64_t seed) {
-    __shared__ double scratch[16];
-    __shared__ double tsh[kMaxWork];
-    __shared__ double sc[2];
+__global__ void __launch_bounds__(kRsBlock)
+k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, double *Fv,
+          double *rpart) {
+    __shared__ double red[kRsBlock / kWave][kMaxWork];
     const int64_t ncol = n + 2;
-    const double eps2 = 2.0 * 2.220446049250313e-16;
-    if (threadIdx.x < 2) {
+    double nrm = 0.0, sumy = 0.0;
+    for (int q = 0; q < kRedChunks; ++q) {
+        nrm += part2[q * ncol + n];
+        sumy += part2[q * ncol + n + 1];
+    }
+    double S = sqrt(nrm);
+    if (!(S >= 2.0 * 2.220446049250313e-16)) S = 1.0;
+    const int64_t i = (int64_t)blockIdx.x * kRsBlock + threadIdx.x;
+    double f = 0.0;
+    if (i < n) {
         double t = 0.0;
-        for (int q = 0; q < kRedChunks; ++q) t += part2[q * ncol + n + threadIdx.x];
-        sc[threadIdx.x] = t;
+        for (int q = 0; q < kRedChunks; ++q) t += part2[q * ncol + i];
+        if (mu) t -= sumy * ldmu(mu, i);
+        f = t / S - S * V[(int64_t)j * n + i];
+        Fv[i] = f;
+    }
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    for (int c = 0; c <= j; ++c) {
+        const double d = wave_sum(i < n ? V[(int64_t)c * n + i] * f : 0.0);
+        if (lane == 0) red[w][c] = d;
     }
     __syncthreads();
-    double S = sqrt(sc[0]);
-    const double sumy = sc[1];
+    for (int c = threadIdx.x; c <= j; c += kRsBlock) {
+        double t = 0.0;
+        for (int q = 0; q < kRsBlock / kWave; ++q) t += red[q][c];
+        rpart[(int64_t)blockIdx.x * kMaxWork + c] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kRsBlock)
+k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, double *Fv,
+          double *npart) {
+    __shared__ double T[kMaxWork];
+    __shared__ double scratch[kRsBlock / kWave];
+    for (int c = threadIdx.x; c <= j; c += kRsBlock) {
+        double t = 0.0;
+        for (int b = 0; b < nblk; ++b) t += rpart[(int64_t)b * kMaxWork + c];
+        T[c] = t;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kRsBlock + threadIdx.x;
+    double f = 0.0;
+    if (i < n) {
+        double d = 0.0;
+        for (int c = 0; c <= j; ++c) d = fma(V[(int64_t)c * n + i], T[c], d);
+        f = Fv[i] - d;
+        Fv[i] = f;
+    }
+    const double s2 = block_sum(f * f, scratch);
+    if (threadIdx.x == 0) npart[blockIdx.x] = s2;
+}
+
+template <class MT>
+__global__ void __launch_bounds__(1024)
+k_right_c(int64_t n, int j, int work, int nblk, const double *part2, const double *npart,
+          double *V, double *Fv, double *B, double *scal, int *flags, uint64_t seed) {
+    __shared__ double scratch[16];
+    __shared__ double tsh[kMaxWork];
+    const int64_t ncol = n + 2;
+    const double eps2 = 2.0 * 2.220446049250313e-16;
+    double nrm = 0.0;
+    for (int q = 0; q < kRedChunks; ++q) nrm += part2[q * ncol + n];
+    double S = sqrt(nrm);
     if (!(S >= eps2)) {
         if (threadIdx.x == 0) flags[0] = 1;
         S = 1.0;
     }
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        double t = 0.0;
-        for (int q = 0; q < kRedChunks; ++q) t += part2[q * ncol + i];
-        if (mu) t -= sumy * ldmu(mu, i);
-        Fv[i] = t / S - S * V[(int64_t)j * n + i];
-    }
-    __syncthreads();
-    const int w = threadIdx.x / kWave, lane = lane_id();
-    const int nw = blockDim.x / kWave;
-    auto orthog = [&]() {
-        for (int c = w; c <= j; c += nw) {
-            double d = 0.0;
-            for (int64_t i = lane; i < n; i += kWave) d = fma(V[(int64_t)c * n + i], Fv[i], d);
-            d = wave_sum(d);
-            if (lane == 0) tsh[c] = d;
-        }
-        __syncthreads();
-        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-            double d = 0.0;
-            for (int c = 0; c <= j; ++c) d = fma(V[(int64_t)c * n + i], tsh[c], d);
-            Fv[i] -= d;
-        }
-        __syncthreads();
-    };
-    orthog();
     double nf = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) nf = fma(Fv[i], Fv[i], nf);
-    const double RF = sqrt(block_sum(nf, scratch));
+    for (int b = 0; b < nblk; ++b) nf += npart[b];
+    const double RF = sqrt(nf);
     if (j + 1 < work) {
         double RFs = RF, div = RF;
         if (RF < eps2) {
+            // replace F by a pseudo-random direction orthogonal to V[:,0:j+1]
+            const int w = threadIdx.x / kWave, lane = lane_id();
+            const int nw = blockDim.x / kWave;
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
                 Fv[i] = hash_normal(seed, (uint32_t)i);
             __syncthreads();
-            orthog();
+            for (int c = w; c <= j; c += nw) {
+                double d = 0.0;
+                for (int64_t i = lane; i < n; i += kWave) d = fma(V[(int64_t)c * n + i], Fv[i], d);
+                d = wave_sum(d);
+                if (lane == 0) tsh[c] = d;
+            }
+            __syncthreads();
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                double d = 0.0;
+                for (int c = 0; c <= j; ++c) d = fma(V[(int64_t)c * n + i], tsh[c], d);
+                Fv[i] -= d;
+            }
+            __syncthreads();
             double nn = 0.0;
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x) nn = fma(Fv[i], Fv[i], nn);
             div = sqrt(block_sum(nn, scratch));
@@ -582,7 +642,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     SEV_HIP(hipMemcpyAsync(&op.nnz, colptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
     op.off.ensure((size_t)((op.nslab + 1) * n));
-    op.lrow.ensure((size_t)op.nnz);
+    op.lrow.ensure((size_t)op.nnz + kValuePad);
     op.mu = mu;
     op.mu64 = src_f64;
     op.vals = val;
@@ -594,7 +654,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         SEV_LAUNCH_CHECK();
     }
     if (bf16) {
-        op.vbf16.ensure((size_t)op.nnz);
+        op.vbf16.ensure((size_t)op.nnz + kValuePad);
         if (op.nnz > 0) {
             hipLaunchKernelGGL(k_to_bf16, dim3(grid_rows(op.nnz, 16)), dim3(kBlock), 0, st,
                                (const float *)val, op.vbf16.get(), op.nnz);
@@ -685,10 +745,29 @@ void launch_av(const SlabCsc &op, const double *v, const double *rf, const doubl
         launch_av_t<float, float>(op, use_lds, v, rf, wprev, y, st);
 }
 
-int atw_groups(const SlabCsc &op) {
+int atw_groups_default(const SlabCsc &op) {
     const int64_t want = ceil_div((int64_t)device_cu_count() * 4, op.nslab);
     const int64_t cap = std::max<int64_t>(1, op.n / (4 * kWaves));
     return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(want, 16), cap));
+}
+
+int atw_groups(const SlabCsc &op) { return op.atw_groups > 0 ? op.atw_groups : atw_groups_default(op); }
+
+void launch_atw_only(const SlabCsc &op, const double *y, double *part, hipStream_t st) {
+    ProfScope ps("spmv_atw", st,
+                 (double)op.nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m +
+                     8.0 * (op.nslab + 1) * op.n);
+    const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
+    if (op.vtype == kValBF16)
+        hipLaunchKernelGGL(k_atw<uint16_t>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const uint16_t *)op.vals, y, part);
+    else if (op.vtype == kValF64)
+        hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const double *)op.vals, y, part);
+    else
+        hipLaunchKernelGGL(k_atw<float>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow.get(), (const float *)op.vals, y, part);
+    SEV_LAUNCH_CHECK();
 }
 
 // y' = y - W[:,0:jc] T into yout (plus norm/sum partials), then the slab
@@ -701,32 +780,28 @@ void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T,
                            jc, T, y, yout, part);
         SEV_LAUNCH_CHECK();
     }
-    ProfScope ps("spmv_atw", st,
-                 (double)op.nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m +
-                     8.0 * (op.nslab + 1) * op.n);
-    const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
-    if (op.vtype == kValBF16)
-        hipLaunchKernelGGL(k_atw<uint16_t>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const uint16_t *)op.vals, yout, part);
-    else if (op.vtype == kValF64)
-        hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const double *)op.vals, yout, part);
-    else
-        hipLaunchKernelGGL(k_atw<float>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const float *)op.vals, yout, part);
-    SEV_LAUNCH_CHECK();
+    launch_atw_only(op, yout, part, st);
 }
 
 void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, double *V,
                        double *Fv, double *B, double *scal, int *flags, uint64_t seed,
-                       hipStream_t st) {
+                       double *rpart, hipStream_t st) {
     ProfScope ps("right_step", st, 16.0 * op.n * (j + 1));
+    const int64_t n = op.n;
+    const int nblk = (int)ceil_div(n, kRsBlock);
+    double *npart = rpart + (size_t)nblk * kMaxWork;
     if (op.mu64)
-        hipLaunchKernelGGL(k_right_step<double>, dim3(1), dim3(1024), 0, st, op.n, j, work, part2,
-                           (const double *)op.mu, V, Fv, B, scal, flags, seed);
+        hipLaunchKernelGGL(k_right_a<double>, dim3(nblk), dim3(kRsBlock), 0, st, n, j, part2,
+                           (const double *)op.mu, V, Fv, rpart);
     else
-        hipLaunchKernelGGL(k_right_step<float>, dim3(1), dim3(1024), 0, st, op.n, j, work, part2,
-                           (const float *)op.mu, V, Fv, B, scal, flags, seed);
+        hipLaunchKernelGGL(k_right_a<float>, dim3(nblk), dim3(kRsBlock), 0, st, n, j, part2,
+                           (const float *)op.mu, V, Fv, rpart);
+    SEV_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_right_b, dim3(nblk), dim3(kRsBlock), 0, st, n, j, nblk, rpart, V, Fv,
+                       npart);
+    SEV_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_right_c<double>, dim3(1), dim3(1024), 0, st, n, j, work, nblk, part2,
+                       npart, V, Fv, B, scal, flags, seed);
     SEV_LAUNCH_CHECK();
 }
 
@@ -739,6 +814,88 @@ void launch_gemm(const char *name, int64_t m, int work, const double *X, const d
                        extra, scal);
     SEV_LAUNCH_CHECK();
 }
+
+// ---------------------------------------------------------------------------
+// Autotuner: for an operator shape not seen before (and large enough for the
+// choice to matter) time every A v lane-group width and a ladder of A' w
+// column-group counts on the operator itself, keep the fastest, and cache it
+// per (rows, features, stored entries, value type).
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+    int lanes = 0, groups = 0;
+};
+
+std::map<std::tuple<int64_t, int64_t, int64_t, int>, TuneEntry> &tune_cache() {
+    static std::map<std::tuple<int64_t, int64_t, int64_t, int>, TuneEntry> c;
+    return c;
+}
+
+float time_launches(hipStream_t st, const std::function<void()> &f, int reps) {
+    hipEvent_t a, b;
+    SEV_HIP(hipEventCreate(&a));
+    SEV_HIP(hipEventCreate(&b));
+    f();  // warm-up
+    SEV_HIP(hipEventRecord(a, st));
+    for (int r = 0; r < reps; ++r) f();
+    SEV_HIP(hipEventRecord(b, st));
+    SEV_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SEV_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+}  // namespace
+
+void autotune_operator(SlabCsc &op, hipStream_t st) {
+    const auto key = std::make_tuple(op.m, op.n, op.csr_nnz, op.vtype);
+    auto &cache = tune_cache();
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        op.lanes_per_row = it->second.lanes;
+        op.atw_groups = it->second.groups;
+        return;
+    }
+    if (op.m < 50000) return;  // heuristics are as good as timing here
+    ProfScope ps("autotune", st);
+    DevBuf<double> v, y, part;
+    v.ensure((size_t)op.n);
+    y.ensure((size_t)op.m);
+    part.ensure((size_t)(op.nslab * (op.n + 2)));
+    SEV_HIP(hipMemsetAsync(v.get(), 0, sizeof(double) * op.n, st));
+    SEV_HIP(hipMemsetAsync(y.get(), 0, sizeof(double) * op.m, st));
+    Prof *saved = t_prof;
+    t_prof = nullptr;  // keep the trial launches out of the kernel profile
+    TuneEntry best;
+    float best_ms = 1e30f;
+    for (int g : {8, 16, 32, 64}) {
+        op.lanes_per_row = g;
+        const float ms = time_launches(st, [&] { launch_av(op, v.get(), nullptr, nullptr, y.get(), st); }, 3);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.lanes = g;
+        }
+    }
+    op.lanes_per_row = best.lanes;
+    best_ms = 1e30f;
+    const int cap = (int)std::max<int64_t>(1, op.n / (4 * kWaves));
+    for (int g : {1, 2, 3, 4, 6, 8, 12, 16}) {
+        if (g > cap) break;
+        op.atw_groups = g;
+        const float ms = time_launches(st, [&] { launch_atw_only(op, y.get(), part.get(), st); }, 3);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.groups = g;
+        }
+    }
+    op.atw_groups = best.groups;
+    t_prof = saved;
+    cache[key] = best;
+    SEV_HIP(hipStreamSynchronize(st));
+}
+
+namespace {
 
 }  // namespace
 
@@ -767,6 +924,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     w.part2.ensure((size_t)(kRedChunks * (n + 2)));
     w.red.ensure((size_t)kDotBlocks * kMaxWork);
     w.Fv.ensure((size_t)(n + m));  // F (n) followed by the updated y (m)
+    w.rpart.ensure((size_t)ceil_div(n, kRsBlock) * (kMaxWork + 1));
     w.scal.ensure(4);
     w.flags.ensure(4);
 
@@ -821,7 +979,8 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                 SEV_LAUNCH_CHECK();
             }
             launch_right_step(op, j, work, w.part2.get(), Vd, w.Fv.get(), w.B.get(),
-                              w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u, st);
+                              w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u,
+                              w.rpart.get(), st);
             {
                 ProfScope ps("scale_col", st, 16.0 * m);
                 hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st,

@@ -70,6 +70,9 @@ namespace sev {
 //           the position of column j's first entry in slab s ((nslab+1) x F,
 //           slab-major) and lrow holds 16-bit in-slab rows.
 constexpr int64_t kSlabRows = 4096;
+// CSC value arrays handed to build_slab_csc must hold kValuePad readable
+// elements past nnz (A' w reads 4-aligned covers of each column segment).
+constexpr int64_t kValuePad = 8;
 
 enum ValType : int { kValF32 = 0, kValBF16 = 1, kValF64 = 2 };
 
@@ -81,7 +84,8 @@ struct SlabCsc {
     DevBuf<int64_t> rowptr;   // m + 1 (padded CSR)
     DevBuf<uint16_t> ccol;    // csr_nnz
     DevBuf<unsigned char> cval;  // csr_nnz values of vtype
-    int lanes_per_row = 64;
+    int lanes_per_row = 64;       // A v: lanes per CSR row (8/16/32/64)
+    int atw_groups = 0;           // A' w: column groups per slab (0: heuristic)
     const void *vals = nullptr;  // CSC values read by A' w (not owned unless
                                  // they point into vbf16)
     const void *mu = nullptr;    // mu' (not owned), same float type as the
@@ -100,6 +104,11 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                     const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
                     hipStream_t st);
 
+// Times the A v lane widths and A' w column-group counts on this operator
+// and keeps the fastest (cached per shape; small operators keep the
+// heuristics).
+void autotune_operator(SlabCsc &op, hipStream_t st);
+
 // y = A x (trans == 0, y length m) or y = A' x (trans == 1, y length n) on
 // the device, A = B - 1 mu'.  Uses w.part / w.part2 as scratch.
 struct IrlbaWork;
@@ -117,7 +126,7 @@ struct IrlbaOut {
 };
 
 struct IrlbaWork {
-    DevBuf<double> W, W2, V, V2, B, BU, BV, T, part, part2, red, Fv, scal, rnd;
+    DevBuf<double> W, W2, V, V2, B, BU, BV, T, part, part2, red, Fv, scal, rnd, rpart;
     DevBuf<int> flags;
 };
 
