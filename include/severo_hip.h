@@ -104,6 +104,23 @@ int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const in
                            int64_t maxit, double tol, const double *init, double *s, double *U,
                            double *V, int precision, int64_t *iters, int64_t *nmult);
 
+/* libcell's irlba entry point, bound unchanged by src/irlba.jl:66-71
+ * (BlasInt = int64 arguments, Cint status): the truncated SVD of any operator
+ * the caller applies through `matmul` (src/irlba.jl:22-38): y (n) = A' x (m)
+ * when trans == 'T', else y (m) = A x (n); x and y are host arrays and temp
+ * holds max(m, n) doubles.  `randv` (src/irlba.jl:40-45) fills a host
+ * vector with normal deviates; it is called only when init is NULL.  restart
+ * = d > 0 warm-starts from the d singular triplets already in the first d
+ * entries / columns of s, U, V (src/irlba.jl:87-99).  The Lanczos vectors,
+ * reorthogonalisation and restarts run on the GPU; every product copies one
+ * vector to the host and back.  Returns 0, or a nonzero status where the
+ * reference raises "convergence failed". */
+typedef void (*sev_irlba_randv_fn)(double *y, int32_t n, void *data);
+typedef void (*sev_irlba_matmul_fn)(double *y, char trans, double *x, double *temp, void *data);
+int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t restart,
+          double tol, double *init, double *s, double *U, double *V, sev_irlba_randv_fn randv,
+          sev_irlba_matmul_fn matmul, void *data);
+
 /* Exact Euclidean kNN with the native libcell signature
  * (src/neighbours.jl:32-39 and :54-61).  X is n x d column-major with
  * leading dimension ld; nn_index (1-based) and distances are n x k

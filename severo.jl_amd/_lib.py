@@ -19,6 +19,11 @@ _dbl = C.c_double
 _p = C.c_void_p
 _pp = C.POINTER(C.c_void_p)
 _pi64 = C.POINTER(C.c_int64)
+_pdbl = C.POINTER(C.c_double)
+
+# callback types of the irlba entry point (src/irlba.jl:22-45)
+MATMUL_FN = C.CFUNCTYPE(None, _pdbl, C.c_char, _pdbl, _pdbl, C.c_void_p)
+RANDV_FN = C.CFUNCTYPE(None, _pdbl, C.c_int32, C.c_void_p)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -31,6 +36,7 @@ SIGNATURES = {
     "sev_centered_mul": (_int, [_i64, _i64, _p, _p, _p, _p, _int, C.c_char, _p, _p, _dbl, _dbl]),
     "sev_irlba_centered_csc": (_int, [_i64, _i64, _p, _p, _p, _p, _int, _i64, _i64, _i64, _dbl, _p,
                                       _p, _p, _p, _int, _pi64, _pi64]),
+    "irlba": (_int, [_i64, _i64, _i64, _i64, _i64, _i64, _dbl, _p, _p, _p, _p, _p, _p, _p]),
     "FindNeighboursEuclidean32": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "FindNeighboursEuclidean64": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "sev_knn": (_int, [_p, _i64, _i64, _i64, _int, _i64, _int, _int, _p, _p, _pi64]),

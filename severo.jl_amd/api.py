@@ -151,13 +151,69 @@ class SVD:
     nmult: int = 0
 
 
+def _host_operator(A):
+    """((m, n), A x, A' y) of an operator applied on the host."""
+    if isinstance(A, CenteredMatrix):
+        B = sp.csc_matrix(A.A, dtype=np.float64)
+        mu = np.asarray(A.mu, np.float64)
+        return B.shape, (lambda x: B @ x - mu @ x), (lambda y: B.T @ y - mu * y.sum())
+    if hasattr(A, "matvec") and hasattr(A, "rmatvec"):
+        return A.shape, A.matvec, A.rmatvec
+    return A.shape, (lambda x: A @ x), (lambda y: A.T @ y)
+
+
+def irlba_operator(A, nu: int, S: SVD | None = None, tol=1e-5, maxit=1000, init=None,
+                   rng=None, work=None) -> SVD:
+    """irlba through libcell's callback interface (src/irlba.jl:22-99): the
+    library runs the Lanczos process on the GPU and calls back for every
+    product, which the operator computes wherever it lives (a LinearOperator,
+    numpy or scipy matrix, or a CenteredMatrix applied as B x - (mu'x) 1 and
+    B'y - mu sum(y)).  ``S`` (a previous SVD with d <= nu triplets) warm-starts
+    the bases, as ``irlba(A, nu, S)`` does (src/irlba.jl:87-99)."""
+    (m, n), matvec, rmatvec = _host_operator(A)
+    if init is None:
+        rng = np.random.default_rng() if rng is None else rng
+        init = rng.standard_normal(n)
+    init = np.ascontiguousarray(init, np.float64)
+    s = np.zeros(nu)
+    U = np.zeros((m, nu), order="F")
+    V = np.zeros((n, nu), order="F")
+    restart = 0
+    if S is not None:
+        restart = len(S.S)
+        s[:restart] = S.S
+        U[:, :restart] = S.U[:, :restart]
+        V[:, :restart] = S.V[:, :restart]
+    work = nu + 7 if work is None else work
+    work = min(work, m, n)
+    errors = []
+
+    def _mm(yp, trans, xp, tp, data):
+        try:
+            if trans == b"T":
+                np.ctypeslib.as_array(yp, (n,))[:] = rmatvec(np.ctypeslib.as_array(xp, (m,)))
+            else:
+                np.ctypeslib.as_array(yp, (m,))[:] = matvec(np.ctypeslib.as_array(xp, (n,)))
+        except Exception as e:  # surfaced after the call returns
+            errors.append(e)
+
+    cb = _lib.MATMUL_FN(_mm)
+    rc = _lib.load().irlba(m, n, nu, work, maxit, restart, float(tol), _ptr(init), _ptr(s),
+                           _ptr(U), _ptr(V), None, C.cast(cb, C.c_void_p), None)
+    if errors:
+        raise errors[0]
+    check(rc)
+    return SVD(U, s, V)
+
+
 def irlba(A, nu: int, tol=1e-5, maxit=1000, init=None, rng=None, precision="fp32",
           work=None) -> SVD:
     """Truncated SVD by augmented restarted Lanczos bidiagonalisation on the
     GPU (src/irlba.jl:47-85).  ``A`` is a CenteredMatrix or any sparse/dense
-    matrix; work defaults to nu + 7 (src/irlba.jl:50); ``init`` defaults to a
-    standard-normal vector (src/irlba.jl:63).  Raises ConvergenceError like
-    the reference's ``error("convergence failed")``."""
+    matrix, held on the device for the whole run; work defaults to nu + 7
+    (src/irlba.jl:50); ``init`` defaults to a standard-normal vector
+    (src/irlba.jl:63).  Raises ConvergenceError like the reference's
+    ``error("convergence failed")``."""
     mu = None
     if isinstance(A, CenteredMatrix):
         mu, A = A.mu, A.A

@@ -820,6 +820,57 @@ int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const in
     });
 }
 
+int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t restart,
+          double tol, double *init, double *s, double *U, double *V, sev_irlba_randv_fn randv,
+          sev_irlba_matmul_fn matmul, void *data) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(m >= 1 && n >= 1 && nu >= 1, "irlba: empty operator or nu < 1");
+        SEV_REQUIRE(matmul && s && U && V, "irlba: NULL matmul or output");
+        SEV_REQUIRE(init || randv, "irlba: need init or randv");
+        Stream st;
+        SlabCsc op;  // shape only: the products are the caller's
+        op.m = m;
+        op.n = n;
+        op.nslab = ceil_div(m, kSlabRows);
+        std::vector<double> v0((size_t)n);
+        if (init)
+            std::copy(init, init + n, v0.begin());
+        else
+            randv(v0.data(), (int32_t)n, data);
+        HostOperator hop;
+        hop.matmul = matmul;
+        hop.randv = randv;
+        hop.data = data;
+        IrlbaParams prm;
+        prm.nu = nu;
+        prm.work = work;
+        prm.maxit = maxit;
+        prm.tol = tol;
+        prm.svtol = tol;
+        // previous triplets are read before the outputs are written
+        std::vector<double> rU, rs, rV;
+        if (restart > 0) {
+            SEV_REQUIRE(restart <= nu, "irlba: restart exceeds nu");
+            rU.assign(U, U + m * restart);
+            rs.assign(s, s + restart);
+            rV.assign(V, V + n * restart);
+            prm.restart = restart;
+            prm.rU = rU.data();
+            prm.rs = rs.data();
+            prm.rV = rV.data();
+        }
+        IrlbaWork w;
+        DevBuf<double> dU, dV;
+        dU.ensure((size_t)(m * nu));
+        dV.ensure((size_t)(n * nu));
+        irlba_device(op, prm, v0.data(), w, s, dU.get(), dV.get(), st.st, nullptr, &hop);
+        download(U, dU.get(), (size_t)(m * nu), st.st);
+        download(V, dV.get(), (size_t)(n * nu), st.st);
+        SEV_HIP(hipStreamSynchronize(st.st));
+    });
+}
+
 int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
             int include_self, int precision, int32_t *nn_index, void *distances,
             int64_t *n_fallback) {

@@ -542,6 +542,16 @@ k_right_c(int64_t n, int j, int work, int nblk, const double *part2, const doubl
     }
 }
 
+// y -= RF w_prev (the recurrence term of A v_j when the product came from a
+// host operator; same rounding as the fused form in k_av_csr)
+__global__ void __launch_bounds__(kBlock)
+k_sub_rf(double *y, int64_t m, const double *rf, const double *wprev) {
+    const double r = *rf;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * kBlock)
+        y[i] -= r * wprev[i];
+}
+
 // dst = src / S (the normalised Lanczos vector w_j = y / ||y||)
 __global__ void __launch_bounds__(kBlock)
 k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
@@ -900,7 +910,8 @@ namespace {
 }  // namespace
 
 void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
-                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out) {
+                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
+                  const HostOperator *hop) {
     const int64_t m = op.m, n = op.n;
     const int64_t nu = p.nu;
     int64_t workl = p.work;
@@ -928,17 +939,64 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     w.scal.ensure(4);
     w.flags.ensure(4);
 
-    // starting vector v_0 = init / ||init||
+    SEV_REQUIRE(!hop || (hop->matmul && !op.mu), "irlba: host operator needs matmul, no mu");
+    const int64_t d0 = p.restart;
+    SEV_REQUIRE(d0 >= 0 && d0 < work, "irlba: restart must be below the work size");
+    SEV_REQUIRE(d0 == 0 || (p.rU && p.rs && p.rV), "irlba: restart needs the previous U, s, V");
+
+    // starting vector v_d0 = init, orthogonalised against the d0 previous
+    // right vectors (two classical Gram-Schmidt passes), normalised
     std::vector<double> v0(init, init + n);
+    for (int pass = 0; pass < (d0 > 0 ? 2 : 0); ++pass) {
+        std::vector<double> t((size_t)d0, 0.0);
+        for (int64_t c = 0; c < d0; ++c)
+            for (int64_t i = 0; i < n; ++i) t[c] += p.rV[c * n + i] * v0[i];
+        for (int64_t c = 0; c < d0; ++c)
+            for (int64_t i = 0; i < n; ++i) v0[i] -= t[c] * p.rV[c * n + i];
+    }
     double nr = 0.0;
     for (int64_t i = 0; i < n; ++i) nr += v0[i] * v0[i];
     nr = std::sqrt(nr);
     if (nr < 2 * eps) fail(kErrConvergence, "irlba: starting vector is zero");
     for (int64_t i = 0; i < n; ++i) v0[i] /= nr;
-    SEV_HIP(hipMemcpyAsync(w.V.get(), v0.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    SEV_HIP(hipMemcpyAsync(w.V.get() + d0 * n, v0.data(), sizeof(double) * n,
+                           hipMemcpyHostToDevice, st));
     SEV_HIP(hipMemsetAsync(w.B.get(), 0, sizeof(double) * work * work, st));
     SEV_HIP(hipMemsetAsync(w.flags.get(), 0, sizeof(int) * 4, st));
     SEV_HIP(hipMemsetAsync(w.scal.get(), 0, sizeof(double) * 4, st));
+    if (d0 > 0) {
+        // V[:,0:d0] = V_prev, W[:,0:d0] = U_prev, B = diag(s_prev); the
+        // coupling column B[0:d0, d0] is taken from the first
+        // reorthogonalisation (W[:,0:d0]' A v_d0)
+        SEV_HIP(hipMemcpyAsync(w.V.get(), p.rV, sizeof(double) * n * d0, hipMemcpyHostToDevice,
+                               st));
+        SEV_HIP(hipMemcpyAsync(w.W.get(), p.rU, sizeof(double) * m * d0, hipMemcpyHostToDevice,
+                               st));
+        std::vector<double> Bd((size_t)work * work, 0.0);
+        for (int64_t c = 0; c < d0; ++c) Bd[(size_t)c * work + c] = p.rs[c];
+        SEV_HIP(hipMemcpyAsync(w.B.get(), Bd.data(), sizeof(double) * work * work,
+                               hipMemcpyHostToDevice, st));
+        SEV_HIP(hipStreamSynchronize(st));
+    }
+
+    // host operator staging: every product copies its input vector to the
+    // host, calls matmul, and copies the result back
+    std::vector<double> hx, hy, htemp;
+    if (hop) {
+        hx.resize((size_t)std::max(m, n));
+        hy.resize((size_t)std::max(m, n));
+        htemp.resize((size_t)std::max(m, n));
+        // k_atw never runs: the column entries of the slab partials stay zero
+        SEV_HIP(hipMemsetAsync(w.part.get(), 0, sizeof(double) * op.nslab * (n + 2), st));
+    }
+    auto host_mul = [&](char trans, const double *xd, int64_t nx, double *yd, int64_t ny) {
+        ProfScope ps("host_matmul", st, 8.0 * (nx + ny));
+        SEV_HIP(hipMemcpyAsync(hx.data(), xd, sizeof(double) * nx, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        hop->matmul(hy.data(), trans, hx.data(), htemp.data(), hop->data);
+        SEV_HIP(hipMemcpyAsync(yd, hy.data(), sizeof(double) * ny, hipMemcpyHostToDevice, st));
+        SEV_HIP(hipStreamSynchronize(st));
+    };
 
     const int dot_blocks = (int)std::min<int64_t>(kDotBlocks, ceil_div(m, kBlock));
 
@@ -950,13 +1008,22 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     bool converged = false;
 
     while (iter < p.maxit) {
-        const int j0 = (iter == 0) ? 0 : (int)k;
+        const int j0 = (iter == 0) ? (int)d0 : (int)k;
         for (int j = j0; j < work; ++j) {
             double *Wd = w.W.get(), *Vd = w.V.get();
             double *y = Wd + (int64_t)j * m;
             // W[:,j] = A v_j - RF w_{j-1}
-            launch_av(op, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr,
-                      j > j0 ? Wd + (int64_t)(j - 1) * m : nullptr, y, st);
+            if (hop) {
+                host_mul('N', Vd + (int64_t)j * n, n, y, m);
+                if (j > j0) {
+                    hipLaunchKernelGGL(k_sub_rf, dim3(grid_rows(m)), dim3(kBlock), 0, st, y, m,
+                                       w.scal.get() + 1, Wd + (int64_t)(j - 1) * m);
+                    SEV_LAUNCH_CHECK();
+                }
+            } else {
+                launch_av(op, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr,
+                          j > j0 ? Wd + (int64_t)(j - 1) * m : nullptr, y, st);
+            }
             nmult++;
             // reorthogonalise against W[:,0:j]
             const int jc = j;
@@ -969,7 +1036,17 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                                    w.red.get(), w.T.get());
                 SEV_LAUNCH_CHECK();
             }
-            launch_upd_atw(op, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get(), st);
+            if (iter == 0 && j == j0 && d0 > 0)
+                SEV_HIP(hipMemcpyAsync(w.B.get() + (int64_t)d0 * work, w.T.get(),
+                                       sizeof(double) * d0, hipMemcpyDeviceToDevice, st));
+            if (hop) {
+                ProfScope ps("reorth_apply", st, 8.0 * m * (jc + 2));
+                hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, m, n,
+                                   Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get());
+                SEV_LAUNCH_CHECK();
+            } else {
+                launch_upd_atw(op, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get(), st);
+            }
             nmult++;
             {
                 ProfScope ps("reduce_slabs", st, 8.0 * op.nslab * (n + 2));
@@ -978,6 +1055,9 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                                    w.part2.get());
                 SEV_LAUNCH_CHECK();
             }
+            // host operator: B' y' lands in the first reduction row (the
+            // other rows hold zeros for the columns)
+            if (hop) host_mul('T', w.Fv.get() + n, m, w.part2.get(), n);
             launch_right_step(op, j, work, w.part2.get(), Vd, w.Fv.get(), w.B.get(),
                               w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u,
                               w.rpart.get(), st);

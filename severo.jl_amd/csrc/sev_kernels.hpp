@@ -119,6 +119,21 @@ struct IrlbaParams {
     int64_t nu = 50, work = 57, maxit = 1000;
     double tol = 1e-5, svtol = 1e-5;
     int reorth_passes = 1;  // CGS passes on the W (left) side
+    // Warm start from `restart` previous singular triplets (host arrays,
+    // column-major, leading dimensions m and n): their vectors seed the first
+    // columns of the Krylov bases and B starts as diag(rs).
+    int64_t restart = 0;
+    const double *rU = nullptr, *rs = nullptr, *rV = nullptr;
+};
+
+// An operator applied by the host: matmul(y, 'N', x, temp, data) sets
+// y (m) = A x (n); matmul(y, 'T', x, temp, data) sets y (n) = A' x (m).  x and
+// y are host arrays; temp holds max(m, n) doubles of scratch.  This is the
+// callback pair of the reference's libcell irlba (src/irlba.jl:22-45).
+struct HostOperator {
+    void (*matmul)(double *y, char trans, double *x, double *temp, void *data) = nullptr;
+    void (*randv)(double *y, int32_t n, void *data) = nullptr;
+    void *data = nullptr;
 };
 
 struct IrlbaOut {
@@ -135,8 +150,12 @@ struct IrlbaWork {
 // m x nu column-major) and V (device, n x nu column-major) hold the
 // singular triplets.  Throws Error(kErrConvergence) like the reference's
 // "convergence failed".
+//
+// With `hop` set, the products are the host operator's and `op` only gives
+// the shape (m, n, nslab; mu must be null).
 void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
-                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out);
+                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
+                  const HostOperator *hop = nullptr);
 
 // Z[r*rs + c*cs] = float(U[c*m + r] * s[c]) for c < d: the embedding
 // U*diag(s) (src/embedding.jl:67) rounded to Float32.

@@ -167,3 +167,90 @@ def test_centered_mul_layouts(sev, orc, m, n, dens):
                        rtol=1e-12, atol=1e-11)
     assert np.allclose(Cm.rmatvec(w), orc.centered_mul(m, A.indptr, A.indices, A.data, mu, w,
                                                        trans=True), rtol=1e-12, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# libcell's callback irlba entry point (src/irlba.jl:22-99)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("m,n,nu", [(50, 50, 20), (100, 50, 20)])
+def test_irlba_callback_dense(sev, m, n, nu):
+    """The dense cases of test/test_irlba.jl through host matmul callbacks."""
+    from scipy.sparse.linalg import aslinearoperator
+    rng = np.random.default_rng(m + 1)
+    X = rng.standard_normal((m, n))
+    S = sev.irlba_operator(aslinearoperator(X), nu, tol=1e-5, init=rng.standard_normal(n))
+    ss = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(S.S, ss[:nu], rtol=1e-8)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
+
+
+def test_irlba_callback_matches_device_operator(sev):
+    """Same centered matrix and start: the callback path and the device
+    operator agree to the FP64 product rounding."""
+    X = sp.random(3000, 400, 0.1, format="csc", random_state=21, dtype=np.float64)
+    mu = np.asarray(X.mean(axis=0)).ravel()
+    Cm = sev.CenteredMatrix(X, mu)
+    init = np.random.default_rng(5).standard_normal(400)
+    a = sev.irlba(Cm, 8, tol=1e-9, init=init)
+    b = sev.irlba_operator(Cm, 8, tol=1e-9, init=init)
+    assert np.allclose(a.S, b.S, rtol=1e-11)
+    Q = X.toarray() - mu
+    assert np.linalg.norm(Q.T @ b.U - b.V * b.S) / np.linalg.norm(Q) < 1e-9
+    assert _angle(a.U, b.U) < 1e-6
+
+
+def test_irlba_callback_restart(sev):
+    """irlba(A, nu, S) warm start (test/test_irlba.jl "restart"; the
+    reference marks the third value @test_broken, here it must hold)."""
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((20, 10))
+    ss = np.linalg.svd(X, compute_uv=False)
+    S2 = sev.irlba_operator(X, 2, tol=1e-5, init=rng.standard_normal(10))
+    assert np.allclose(S2.S, ss[:2], rtol=1e-8)
+    S3 = sev.irlba_operator(X, 3, S2, tol=1e-5, init=rng.standard_normal(10))
+    assert np.allclose(S3.S, ss[:3], rtol=1e-8)
+    Y = sp.random(3000, 300, 0.05, format="csc", random_state=4, dtype=np.float64)
+    sy = np.linalg.svd(Y.toarray(), compute_uv=False)
+    S5 = sev.irlba_operator(Y, 5, tol=1e-9, init=rng.standard_normal(300))
+    S12 = sev.irlba_operator(Y, 12, S5, tol=1e-9, init=rng.standard_normal(300))
+    assert np.allclose(S12.S, sy[:12], rtol=1e-8)
+    Yd = Y.toarray()
+    assert np.linalg.norm(Yd.T @ S12.U - S12.V * S12.S) / np.linalg.norm(Yd) < 1e-8
+
+
+def test_irlba_callback_randv_and_maxit(sev):
+    """init = NULL draws the start from randv once; maxit exhaustion returns
+    the convergence status."""
+    import ctypes as C
+    lib = sev._lib
+    rng = np.random.default_rng(3)
+    X = np.asfortranarray(rng.standard_normal((60, 30)))
+    m, n, nu = 60, 30, 4
+    calls = []
+
+    def mm(yp, trans, xp, tp, data):
+        if trans == b"T":
+            np.ctypeslib.as_array(yp, (n,))[:] = X.T @ np.ctypeslib.as_array(xp, (m,))
+        else:
+            np.ctypeslib.as_array(yp, (m,))[:] = X @ np.ctypeslib.as_array(xp, (n,))
+
+    def rv(yp, k, data):
+        calls.append(k)
+        np.ctypeslib.as_array(yp, (k,))[:] = rng.standard_normal(k)
+
+    mcb, rcb = lib.MATMUL_FN(mm), lib.RANDV_FN(rv)
+    s = np.zeros(nu)
+    U = np.zeros((m, nu), order="F")
+    V = np.zeros((n, nu), order="F")
+    L = sev.load()
+    rc = L.irlba(m, n, nu, nu + 7, 1000, 0, 1e-8, None, s.ctypes.data, U.ctypes.data,
+                 V.ctypes.data, C.cast(rcb, C.c_void_p), C.cast(mcb, C.c_void_p), None)
+    assert rc == 0 and calls == [n]
+    assert np.allclose(s, np.linalg.svd(X, compute_uv=False)[:nu], rtol=1e-8)
+    init = rng.standard_normal(n)
+    s2 = np.zeros(20)
+    U2 = np.zeros((m, 20), order="F")
+    V2 = np.zeros((n, 20), order="F")
+    rc = L.irlba(m, n, 20, 27, 1, 0, 1e-14, init.ctypes.data, s2.ctypes.data, U2.ctypes.data,
+                 V2.ctypes.data, None, C.cast(mcb, C.c_void_p), None)
+    assert rc == -3
