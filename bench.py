@@ -67,20 +67,39 @@ def time_form(p, feats, init, form, steps, warmup, profile):
     return dt, dict(irlba_iters=it, irlba_nmult=nm, knn_fallback_rows=nfb, snn_nnz=nnz), prof
 
 
-def roofline(prof, form, steps):
+def measured_traffic():
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (scaled by the copy probe), from the `TRAFFIC <kernel> <form> <workload>
+    <bytes>` lines of the committed profile summaries."""
+    out = {}
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return out
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            f = line.split()
+            if len(f) == 5 and f[0] == "TRAFFIC":
+                out[(f[1], f[2], f[3])] = float(f[4])
+    return out
+
+
+def roofline(prof, form, steps, workload):
     if not prof:
         return None
     name, (launches, ms, by, fl) = max(prof.items(), key=lambda kv: kv[1][1])
+    traffic = measured_traffic().get((name, form, workload))
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
         peak = PEAK_TFLOPS[form if form in PEAK_TFLOPS else "fp32"]
         return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                 "launches_per_step": launches // steps,
                 "avg_launch_ms": round(ms / launches, 4)}
     achieved = by / (ms * 1e-3) / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
             "launches_per_step": launches // steps, "avg_launch_ms": round(ms / launches, 4)}
 
 
@@ -123,6 +142,10 @@ def main():
 
     sev = entry.load_package()
     sev.load()
+    # known-size copy kernel: the device's achievable HBM rate, and the
+    # calibration point of the byte counters in the profiles
+    copy_gbs = sev.copy_probe()
+    print(f"# HBM copy probe: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     w = WORKLOADS[args.workload]
     t = sev.nb_tables(w["genes"], seed=0, density=w["density"], boost_frac=w["boost"])
     p = sev.Pipeline()
@@ -150,7 +173,7 @@ def main():
     head = forms[0]
     dt, info, prof = res[head]
     out = {
-        "metric": "cells/s normalise->PCA50->kNN20->SNN, BF16 vs FP32 form, 1M cells",
+        "metric": "cells/s normalise\u2192PCA50\u2192kNN20\u2192SNN, BF16 vs FP32 form, 1M cells",
         "value": round(m / dt, 2),
         "unit": "cells/s",
         "n_gpus": 1,
@@ -166,7 +189,7 @@ def main():
                                f"{NFEAT} features, {NPCS} PCs, k={K}, prune 1/15, form {head}",
                    "cells": m, "genes": g, "nnz": nnz, "features": NFEAT, "npcs": NPCS, "k": K,
                    "form": head},
-        "roofline": roofline(prof, head, args.steps),
+        "roofline": dict(roofline(prof, head, args.steps, args.workload) or {}, copy_probe_gbs=round(copy_gbs, 1)),
         "forms": {f: {"cells_per_s": round(m / res[f][0], 2), "ms_per_step": round(res[f][0] * 1e3, 3),
                       **res[f][1]} for f in forms},
     }

@@ -56,7 +56,9 @@ int sev_version(void);
 /* Device self-checks (no counterpart in the reference): what == 0 / 1 / 2
  * checks the FP32 / BF16 / FP64 matrix-core lane layouts the kNN and IRLBA
  * kernels rely on; *result receives the largest deviation (0 when they
- * hold). */
+ * hold).  what == 3 runs the HBM copy probe (k_copy_probe: 1 GiB read +
+ * 1 GiB written per launch, best of 5) and returns its rate in GB/s; it is
+ * the known-size kernel against which HBM byte counters are calibrated. */
 int sev_selftest(int what, double *result);
 /* Selects the device for the calling thread and checks it is usable. */
 int sev_device_init(int device);

@@ -11,7 +11,8 @@ The directory name is not a Python identifier; load it with
 module name ``severo_jl_amd``.
 """
 from ._lib import ConvergenceError, SeveroError, load  # noqa: F401
-from .api import (CenteredMatrix, LinearEmbedding, SVD, ann, embedding, irlba, irlba_operator,  # noqa: F401,E501
+from .api import (CenteredMatrix, LinearEmbedding, SVD, ann, copy_probe, embedding, irlba,  # noqa: F401
+                  irlba_operator,
                   jaccard_index, nearest_neighbours, normalize_cells, pca, scale_features,
                   shared_nearest_neighbours)
 from .pipeline import Pipeline  # noqa: F401

@@ -359,3 +359,11 @@ def shared_nearest_neighbours(X, k: int, dims=None, include_self=True, ntables=N
     Xc = _coords(X, dims)
     idx, _ = ann(Xc, k, include_self=include_self)
     return _snn_from_index(idx, k, prune, Xc.dtype)
+
+
+def copy_probe() -> float:
+    """Achievable HBM copy rate of the device in GB/s (1 GiB read + 1 GiB
+    written per launch, best of 5)."""
+    r = C.c_double(0.0)
+    check(_lib.load().sev_selftest(3, C.byref(r)))
+    return r.value

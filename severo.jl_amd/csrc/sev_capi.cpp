@@ -261,9 +261,9 @@ int sev_version(void) { return 1; }
 int sev_selftest(int what, double *result) {
     return guarded([&] {
         ensure_device();
-        SEV_REQUIRE(what >= 0 && what <= 2, "unknown self-test");
+        SEV_REQUIRE(what >= 0 && what <= 3, "unknown self-test");
         SEV_REQUIRE(result != nullptr, "result is NULL");
-        *result = mfma_selftest(what);
+        *result = what == 3 ? hbm_copy_probe() : mfma_selftest(what);
     });
 }
 

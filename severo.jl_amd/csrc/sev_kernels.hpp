@@ -206,6 +206,11 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
 // (mode 0: 16x16x4 FP32, 1: 16x16x32 BF16); 0 when the model holds.
 double mfma_selftest(int mode);  // mode 2: 16x16x4 FP64
 
+// Copies 1 GiB between two device buffers with 16-byte loads and stores
+// (k_copy_probe) and returns the best rate of 5 launches in GB/s (2 GiB of
+// traffic per launch).
+double hbm_copy_probe();
+
 // ---- sev_snn.hip ---------------------------------------------------------
 // Jaccard SNN graph from 0-based row-major neighbour lists (n x k).
 // wtab holds w[x] = x / (k + (k - x)) in T for x = 0..k; entries with

@@ -353,4 +353,46 @@ template void scale_map<float>(int64_t, const int64_t *, const float *, const fl
 template void scale_map<double>(int64_t, const int64_t *, const double *, const double *,
                                 const double *, double *, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// HBM copy probe: the calibration kernel for byte counters and the achievable
+// copy rate of the device.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_copy_probe(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+double hbm_copy_probe() {
+    const int64_t bytes = (int64_t)1 << 30;
+    const int64_t n = bytes / (int64_t)sizeof(double2);
+    DevBuf<double2> a, b;
+    a.ensure((size_t)n);
+    b.ensure((size_t)n);
+    hipStream_t st;
+    SEV_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    SEV_HIP(hipMemsetAsync(a.get(), 0, (size_t)bytes, st));
+    hipEvent_t e0, e1;
+    SEV_HIP(hipEventCreate(&e0));
+    SEV_HIP(hipEventCreate(&e1));
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), 256 * 32);
+    float best = 1e30f;
+    for (int r = 0; r < 6; ++r) {
+        SEV_HIP(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a.get(), b.get(), n);
+        SEV_LAUNCH_CHECK();
+        SEV_HIP(hipEventRecord(e1, st));
+        SEV_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        SEV_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0 && ms < best) best = ms;  // launch 0 warms up
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SEV_HIP(hipStreamSynchronize(st));
+    (void)hipStreamDestroy(st);
+    return 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+}
+
 }  // namespace sev
