@@ -100,6 +100,17 @@ inline int device_cu_count() {
     return cus;
 }
 
+inline int device_lds_per_block() {
+    static int lds = 0;
+    if (lds == 0) {
+        int dev = 0;
+        SEV_HIP(hipGetDevice(&dev));
+        SEV_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        if (lds <= 0) lds = 65536;
+    }
+    return lds;
+}
+
 // ---------------------------------------------------------------------------
 // Device helpers
 // ---------------------------------------------------------------------------

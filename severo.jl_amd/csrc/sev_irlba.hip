@@ -20,6 +20,9 @@
 // reproducible run to run.
 #include "sev_kernels.hpp"
 
+#include <cstdio>
+#include <cstdlib>
+
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -95,6 +98,32 @@ __device__ double block_sum(double v, double *scratch) {
     for (int i = 0; i < (int)(blockDim.x / kWave); ++i) s += scratch[i];
     __syncthreads();
     return s;
+}
+
+// Reduce-scatter of 32 per-lane values over the 64 lanes of a wavefront by a
+// butterfly: each step halves the values a lane keeps and adds its partner's
+// copy of them.  On return v[0] of lane l holds the sum over all lanes of
+// the input v[l >> 1] (lanes 2t and 2t+1 hold the same sum; the order of the
+// additions is fixed).
+template <int HALF>
+__device__ __forceinline__ void butterfly_step(double (&v)[32], int lane) {
+    const bool hi = (lane & (2 * HALF)) != 0;
+#pragma unroll
+    for (int i = 0; i < HALF; ++i) {
+        const double send = hi ? v[i] : v[HALF + i];
+        const double keep = hi ? v[HALF + i] : v[i];
+        v[i] = keep + __shfl_xor(send, 2 * HALF, kWave);
+    }
+}
+
+__device__ __forceinline__ double butterfly32(double (&v)[32]) {
+    const int lane = lane_id();
+    butterfly_step<16>(v, lane);
+    butterfly_step<8>(v, lane);
+    butterfly_step<4>(v, lane);
+    butterfly_step<2>(v, lane);
+    butterfly_step<1>(v, lane);
+    return v[0] + __shfl_xor(v[0], 1, kWave);
 }
 
 // --------------------------------------------------------------------------
@@ -236,15 +265,18 @@ k_av_csr(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const
 }
 
 // --------------------------------------------------------------------------
-// Reorthogonalisation partial products: part[b*kMaxWork + i] = W_i' y over
-// the rows of block b, for i < jc.
+// Reorthogonalisation partial products: part[i*nblk + b] = W_i' y over the
+// rows of block b, for i < jc (columns in chunks of 32, each chunk reduced
+// across the wavefront by the butterfly and across waves in wave order).
 // --------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
     __shared__ double red[kWaves][32];
-    const int64_t per = (m + gridDim.x - 1) / gridDim.x;
+    const int64_t nblk = gridDim.x;
+    const int64_t per = (m + nblk - 1) / nblk;
     const int64_t rb = (int64_t)blockIdx.x * per;
     const int64_t re = rb + per < m ? rb + per : m;
+    const int w = threadIdx.x / kWave, lane = lane_id();
     for (int c0 = 0; c0 < jc; c0 += 32) {
         double acc[32];
 #pragma unroll
@@ -255,45 +287,32 @@ k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
             for (int i = 0; i < 32; ++i)
                 if (c0 + i < jc) acc[i] = fma(W[(int64_t)(c0 + i) * m + r], yr, acc[i]);
         }
-        const int w = threadIdx.x / kWave;
-#pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            double t = wave_sum(acc[i]);
-            if (lane_id() == 0) red[w][i] = t;
-        }
+        const double t = butterfly32(acc);
+        if ((lane & 1) == 0) red[w][lane >> 1] = t;
         __syncthreads();
         if (threadIdx.x < 32 && c0 + (int)threadIdx.x < jc) {
-            double t = 0.0;
-            for (int q = 0; q < kWaves; ++q) t += red[q][threadIdx.x];
-            part[(int64_t)blockIdx.x * kMaxWork + c0 + threadIdx.x] = t;
+            double u = 0.0;
+            for (int q = 0; q < kWaves; ++q) u += red[q][threadIdx.x];
+            part[(int64_t)(c0 + threadIdx.x) * nblk + blockIdx.x] = u;
         }
         __syncthreads();
     }
 }
 
-// T[i] = sum_b part[b*kMaxWork + i] (single block, 1024 threads: 16 threads
-// per column each summing a strided subset, combined in a fixed order).
-__global__ void __launch_bounds__(1024) k_red_t(int nblk, int jc, const double *part, double *T) {
-    const int i = threadIdx.x / 16, q = threadIdx.x % 16;
-    double t = 0.0;
-    if (i < jc) {
-        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        int b = q;
-        for (; b + 48 < nblk; b += 64) {
-            t += part[(int64_t)b * kMaxWork + i];
-            t1 += part[(int64_t)(b + 16) * kMaxWork + i];
-            t2 += part[(int64_t)(b + 32) * kMaxWork + i];
-            t3 += part[(int64_t)(b + 48) * kMaxWork + i];
-        }
-        for (; b < nblk; b += 16) t += part[(int64_t)b * kMaxWork + i];
-        t = (t + t1) + (t2 + t3);
+// T[i] = sum_b part[i*nblk + b]: block i sums column i (each thread a
+// strided subset with independent loads, then the fixed-order block sum).
+__global__ void __launch_bounds__(kBlock) k_red_t(int nblk, const double *part, double *T) {
+    __shared__ double scratch[kWaves];
+    const double *pc = part + (int64_t)blockIdx.x * nblk;
+    double t0 = 0.0, t1 = 0.0;
+    int b = threadIdx.x;
+    for (; b + kBlock < nblk; b += 2 * kBlock) {
+        t0 += pc[b];
+        t1 += pc[b + kBlock];
     }
-#pragma unroll
-    for (int o = 1; o < 16; o <<= 1) {
-        double u = __shfl_down(t, o, 16);
-        if ((q & (2 * o - 1)) == 0) t += u;
-    }
-    if (q == 0 && i < jc) T[i] = t;
+    if (b < nblk) t0 += pc[b];
+    const double t = block_sum(t0 + t1, scratch);
+    if (threadIdx.x == 0) T[blockIdx.x] = t;
 }
 
 // --------------------------------------------------------------------------
@@ -301,22 +320,25 @@ __global__ void __launch_bounds__(1024) k_red_t(int nblk, int jc, const double *
 // yout, with the slab partials of ||y'||^2 and sum(y') (one block per slab;
 // part row s holds n column dots followed by these two).
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
+constexpr int kApplyThreads = 1024;
+
+__global__ void __launch_bounds__(kApplyThreads)
 k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const double *y,
         double *yout, double *part) {
     __shared__ double tl[kMaxWork];
-    __shared__ double scratch[kWaves];
+    __shared__ double scratch[kApplyThreads / kWave];
     const int64_t s = blockIdx.x;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
     if ((int)threadIdx.x < jc) tl[threadIdx.x] = T[threadIdx.x];
     __syncthreads();
     double nrm = 0.0, sm = 0.0;
-    for (int64_t lr = threadIdx.x; lr < nrows; lr += kBlock) {
+    for (int64_t lr = threadIdx.x; lr < nrows; lr += kApplyThreads) {
         const int64_t r = r0 + lr;
         double yv = y[r];
         if (jc > 0) {
             double d = 0.0;
+#pragma unroll 8
             for (int i = 0; i < jc; ++i) d = fma(W[(int64_t)i * m + r], tl[i], d);
             yv -= d;
         }
@@ -447,9 +469,13 @@ k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, 
         Fv[i] = f;
     }
     const int w = threadIdx.x / kWave, lane = lane_id();
-    for (int c = 0; c <= j; ++c) {
-        const double d = wave_sum(i < n ? V[(int64_t)c * n + i] * f : 0.0);
-        if (lane == 0) red[w][c] = d;
+    for (int c0 = 0; c0 <= j; c0 += 32) {
+        double v[32];
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            v[c] = (i < n && c0 + c <= j) ? V[(int64_t)(c0 + c) * n + i] * f : 0.0;
+        const double d = butterfly32(v);
+        if ((lane & 1) == 0 && c0 + (lane >> 1) <= j) red[w][c0 + (lane >> 1)] = d;
     }
     __syncthreads();
     for (int c = threadIdx.x; c <= j; c += kRsBlock) {
@@ -611,6 +637,191 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
             if (extra && kq == 0) Y[(int64_t)kout * m + row] = extra[row] / scal[2];
         }
     }
+}
+
+// --------------------------------------------------------------------------
+// SVD of the projected matrix B (work x work, column-major) on the device:
+// one-sided Jacobi on G = B' with the round-robin (circle) pair schedule, so
+// that the ne/2 pairs of a round touch disjoint columns and run in parallel,
+// one 32-lane group per pair.  Rotations accumulate in J; at convergence
+// B' J = G, the singular values are the column norms of G, U = J and
+// V = G diag(1/s) (ordered by descending s, ties by column index).
+// status[0] = 1 if a singular value vanished (V then needs completing; the
+// caller redoes that restart on the host).
+// --------------------------------------------------------------------------
+constexpr int kSvdThreads = 512;  // 32 groups of 16 lanes: one pair each
+
+// DPP lane exchange of a double within rows of 16 lanes.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Sum over a row of 16 lanes (quad xor 1, quad xor 2, half-row mirror, row
+// mirror).  Every step adds two equal-order partial sums, so all 16 lanes
+// end with bit-identical totals.
+__device__ __forceinline__ double sum16(double v) {
+    v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);  // row_half_mirror
+    v += dpp_d<0x140>(v);  // row_mirror
+    return v;
+}
+
+__global__ void __launch_bounds__(kSvdThreads)
+k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status) {
+    extern __shared__ double sm[];
+    double *G = sm;
+    double *J = sm + n * n;
+    __shared__ int rotated;
+    __shared__ double sv[kMaxWork];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < n * n; e += kSvdThreads) {
+        const int i = e % n, j = e / n;
+        G[e] = B[(int64_t)i * n + j];
+        J[e] = i == j ? 1.0 : 0.0;
+    }
+    const int ne = n + (n & 1);
+    const int pr = tid >> 4, gl = tid & 15;
+    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    __syncthreads();
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        for (int r = 0; r < ne - 1; ++r) {
+            int p = r, q = ne - 1;
+            if (pr > 0) {
+                p = (r + pr) % (ne - 1);
+                q = (r - pr + 2 * (ne - 1)) % (ne - 1);
+            }
+            if (p > q) {
+                const int t = p;
+                p = q;
+                q = t;
+            }
+            const bool valid = pr < ne / 2 && q < n;
+            double *gp = G + p * n, *gq = G + q * n;
+            double x[4], y[4];
+            double app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = gl + 16 * e;
+                const bool in = valid && i < n;
+                x[e] = in ? gp[i] : 0.0;
+                y[e] = in ? gq[i] : 0.0;
+                app = fma(x[e], x[e], app);
+                aqq = fma(y[e], y[e], aqq);
+                apq = fma(x[e], y[e], apq);
+            }
+            app = sum16(app);
+            aqq = sum16(aqq);
+            apq = sum16(apq);
+            if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
+                if (gl == 0) rotated = 1;
+                const double zeta = (aqq - app) / (2.0 * apq);
+                const double az = fabs(zeta);
+                const double t = az > 1e150 ? 0.5 / zeta
+                                            : copysign(1.0, zeta) / (az + sqrt(fma(az, az, 1.0)));
+                const double c = 1.0 / sqrt(fma(t, t, 1.0));
+                const double sn = c * t;
+                double *jp = J + p * n, *jq = J + q * n;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = gl + 16 * e;
+                    if (i < n) {
+                        gp[i] = c * x[e] - sn * y[e];
+                        gq[i] = sn * x[e] + c * y[e];
+                        const double u = jp[i], v = jq[i];
+                        jp[i] = c * u - sn * v;
+                        jq[i] = sn * u + c * v;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        const int any = rotated;
+        __syncthreads();
+        if (tid == 0) status[1] = sweep + 1;
+        if (!any) break;
+    }
+    if (tid < n) {
+        double t = 0.0;
+        for (int i = 0; i < n; ++i) t += G[tid * n + i] * G[tid * n + i];
+        sv[tid] = sqrt(t);
+    }
+    __syncthreads();
+    if (tid == 0) status[0] = 0;
+    __syncthreads();
+    for (int j = tid >> 6; j < n; j += kSvdThreads / 64) {
+        const int l = tid & 63;
+        const double sj = sv[j];
+        int rk = 0;
+        for (int i = 0; i < n; ++i) rk += (sv[i] > sj || (sv[i] == sj && i < j)) ? 1 : 0;
+        if (l == 0) {
+            s[rk] = sj;
+            if (!(sj > 0.0)) status[0] = 1;
+        }
+        for (int i = l; i < n; i += 64) {
+            U[(int64_t)rk * n + i] = J[j * n + i];
+            V[(int64_t)rk * n + i] = sj > 0.0 ? G[j * n + i] / sj : 0.0;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Restart decision after the SVD of B (one thread; the rules of the
+// augmented restart): residuals res_i = ||F|| BU(work-1, i), Ritz-value
+// change svr_i = |s_i^prev - s_i| / s_i, converged when nu pairs have
+// |res_i| < tol max(s) and svr_i < svtol; otherwise the restart size
+// k = max(k, nu + nconv) capped to [1, work - 3] and the new B =
+// [diag(s_0..s_k-1) res_0..k-1 in column k].  info: [0] converged,
+// [1] k, [2] ||F|| <= 0 (cannot restart), [3] nconv, [4] SVD needs the host.
+// --------------------------------------------------------------------------
+__global__ void k_restart_check(int work, int nu, double tol, double svtol, const double *BU,
+                                const double *BS, const double *scal, const int *status,
+                                double *svr, double *B, int *info, int last) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    info[5] += status[1];
+    if (status[0]) {
+        info[4] = 1;
+        return;
+    }
+    info[4] = 0;
+    const double eps = 2.220446049250313e-16;
+    double RF = scal[2];
+    const double RFn = RF;
+    if (RF < 2 * eps) RF = 0.0;
+    double smax = 0.0;
+    for (int i = 0; i < work; ++i)
+        if (BS[i] > smax) smax = BS[i];
+    int nconv = 0;
+    for (int i = 0; i < work; ++i) {
+        const double r = RF * BU[(int64_t)i * work + (work - 1)];
+        const double ratio = fabs(svr[i] - BS[i]) / BS[i];
+        if (fabs(r) < tol * smax && ratio < svtol) nconv++;
+    }
+    info[3] = nconv;
+    info[2] = RFn <= 0.0 ? 1 : 0;
+    if (nconv >= nu) {
+        info[0] = 1;
+        return;
+    }
+    info[0] = 0;
+    if (last) return;
+    int k = info[1];
+    if (k < nu + nconv) k = nu + nconv;
+    if (k > work - 3) k = work - 3;
+    if (k < 1) k = 1;
+    info[1] = k;
+    for (int e = 0; e < work * work; ++e) B[e] = 0.0;
+    for (int c = 0; c < k; ++c) {
+        B[(int64_t)c * work + c] = BS[c];
+        B[(int64_t)k * work + c] = RF * BU[(int64_t)c * work + (work - 1)];
+    }
+    for (int i = 0; i < work; ++i) svr[i] = BS[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -786,8 +997,8 @@ void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T,
                     const double *y, double *yout, double *part, hipStream_t st) {
     {
         ProfScope ps("reorth_apply", st, 8.0 * op.m * (jc + 2));
-        hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, op.m, op.n, W,
-                           jc, T, y, yout, part);
+        hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0, st, op.m,
+                           op.n, W, jc, T, y, yout, part);
         SEV_LAUNCH_CHECK();
     }
     launch_atw_only(op, yout, part, st);
@@ -938,6 +1149,9 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     w.rpart.ensure((size_t)ceil_div(n, kRsBlock) * (kMaxWork + 1));
     w.scal.ensure(4);
     w.flags.ensure(4);
+    w.BS.ensure(kMaxWork);
+    w.svr.ensure(kMaxWork);
+    w.info.ensure(16);
 
     SEV_REQUIRE(!hop || (hop->matmul && !op.mu), "irlba: host operator needs matmul, no mu");
     const int64_t d0 = p.restart;
@@ -1001,9 +1215,14 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     const int dot_blocks = (int)std::min<int64_t>(kDotBlocks, ceil_div(m, kBlock));
 
     std::vector<double> Bh((size_t)work * work), BUh((size_t)work * work),
-        BVh((size_t)work * work), BSh(work), svr(work, 0.0), res(work);
-    double scal_h[4];
+        BVh((size_t)work * work), BSh(work);
     int flags_h[4];
+    int info_h[8] = {0, (int)nu, 0, 0, 0, 0, 0, 0};
+    SEV_HIP(hipMemcpyAsync(w.info.get(), info_h, sizeof info_h, hipMemcpyHostToDevice, st));
+    SEV_HIP(hipMemsetAsync(w.svr.get(), 0, sizeof(double) * kMaxWork, st));
+    const size_t svd_lds = sizeof(double) * 2 * (size_t)work * (size_t)work;
+    SEV_REQUIRE(svd_lds + 1024 <= (size_t)device_lds_per_block(),
+                "irlba: projected matrix too large for the device SVD");
     int64_t iter = 0, nmult = 0, k = nu;
     bool converged = false;
 
@@ -1032,7 +1251,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                 hipLaunchKernelGGL(k_dot_w, dim3(dot_blocks), dim3(kBlock), 0, st, m, Wd, jc, y,
                                    w.red.get());
                 SEV_LAUNCH_CHECK();
-                hipLaunchKernelGGL(k_red_t, dim3(1), dim3(1024), 0, st, dot_blocks, jc,
+                hipLaunchKernelGGL(k_red_t, dim3(jc), dim3(kBlock), 0, st, dot_blocks,
                                    w.red.get(), w.T.get());
                 SEV_LAUNCH_CHECK();
             }
@@ -1041,8 +1260,8 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                                        sizeof(double) * d0, hipMemcpyDeviceToDevice, st));
             if (hop) {
                 ProfScope ps("reorth_apply", st, 8.0 * m * (jc + 2));
-                hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kBlock), 0, st, m, n,
-                                   Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get());
+                hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0, st,
+                                   m, n, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get());
                 SEV_LAUNCH_CHECK();
             } else {
                 launch_upd_atw(op, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get(), st);
@@ -1068,66 +1287,77 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                 SEV_LAUNCH_CHECK();
             }
         }
-        SEV_HIP(hipMemcpyAsync(Bh.data(), w.B.get(), sizeof(double) * work * work,
-                               hipMemcpyDeviceToHost, st));
-        SEV_HIP(hipMemcpyAsync(scal_h, w.scal.get(), sizeof scal_h, hipMemcpyDeviceToHost, st));
+        // SVD of B and the restart decision on the device; the host only
+        // reads the decision (converged, k) back
+        {
+            ProfScope ps("small_svd", st);
+            hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kSvdThreads), svd_lds, st, work,
+                               w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get() + 8);
+            SEV_LAUNCH_CHECK();
+        }
+        const int last = iter + 1 >= p.maxit ? 1 : 0;
+        hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(64), 0, st, work, (int)nu, p.tol,
+                           p.svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.info.get() + 8,
+                           w.svr.get(), w.B.get(), w.info.get(), last);
+        SEV_LAUNCH_CHECK();
+        SEV_HIP(hipMemcpyAsync(info_h, w.info.get(), sizeof(int) * 8, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipMemcpyAsync(flags_h, w.flags.get(), sizeof flags_h, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipStreamSynchronize(st));
         if (flags_h[0])
             fail(kErrConvergence, "irlba: invariant subspace encountered (rank-deficient operator)");
-        small_svd(work, Bh.data(), BUh.data(), BSh.data(), BVh.data());
-        double RF = scal_h[2];
-        const double RFn = RF;
-        if (RF < 2 * eps) RF = 0.0;
-        double smax = 0.0;
-        for (int i = 0; i < work; ++i) {
-            if (BSh[i] > smax) smax = BSh[i];
-            svr[i] = std::fabs(svr[i] - BSh[i]) / BSh[i];
+        if (info_h[4]) {
+            // a singular value of B vanished: complete the basis on the host
+            SEV_HIP(hipMemcpyAsync(Bh.data(), w.B.get(), sizeof(double) * work * work,
+                                   hipMemcpyDeviceToHost, st));
+            SEV_HIP(hipStreamSynchronize(st));
+            small_svd(work, Bh.data(), BUh.data(), BSh.data(), BVh.data());
+            SEV_HIP(hipMemcpyAsync(w.BU.get(), BUh.data(), sizeof(double) * work * work,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipMemcpyAsync(w.BV.get(), BVh.data(), sizeof(double) * work * work,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipMemcpyAsync(w.BS.get(), BSh.data(), sizeof(double) * work,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipMemsetAsync(w.info.get() + 8, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(64), 0, st, work, (int)nu, p.tol,
+                               p.svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.info.get() + 8,
+                               w.svr.get(), w.B.get(), w.info.get(), last);
+            SEV_LAUNCH_CHECK();
+            SEV_HIP(hipMemcpyAsync(info_h, w.info.get(), sizeof(int) * 8, hipMemcpyDeviceToHost,
+                                   st));
+            SEV_HIP(hipStreamSynchronize(st));
         }
-        for (int i = 0; i < work; ++i) res[i] = RF * BUh[(size_t)i * work + (work - 1)];
-        int64_t nconv = 0;
-        for (int i = 0; i < work; ++i)
-            if (std::fabs(res[i]) < p.tol * smax && svr[i] < p.svtol) nconv++;
         iter++;
-        if (nconv >= nu) {
+        if (info_h[0]) {
             converged = true;
             break;
         }
         if (iter >= p.maxit) break;
-        if (k < nu + nconv) k = nu + nconv;
-        if (k > work - 3) k = work - 3;
-        if (k < 1) k = 1;
-        for (int i = 0; i < work; ++i) svr[i] = BSh[i];
-        // restart: V <- [V BV[:,0:k], F/||F||], W <- W BU[:,0:k]
-        SEV_HIP(hipMemcpyAsync(w.BU.get(), BUh.data(), sizeof(double) * work * work,
-                               hipMemcpyHostToDevice, st));
-        SEV_HIP(hipMemcpyAsync(w.BV.get(), BVh.data(), sizeof(double) * work * work,
-                               hipMemcpyHostToDevice, st));
-        if (RFn <= 0.0) fail(kErrConvergence, "irlba: zero residual vector at restart");
+        k = info_h[1];
+        if (info_h[2]) fail(kErrConvergence, "irlba: zero residual vector at restart");
+        // restart: V <- [V BV[:,0:k], F/||F||], W <- W BU[:,0:k]; B was
+        // rebuilt by the restart decision
         launch_gemm("restart_v", n, work, w.V.get(), w.BV.get(), (int)k, w.V2.get(), w.Fv.get(),
                     w.scal.get(), st);
         w.V.swap(w.V2);
         launch_gemm("restart_w", m, work, w.W.get(), w.BU.get(), (int)k, w.W2.get(), nullptr,
                     w.scal.get(), st);
         w.W.swap(w.W2);
-        std::fill(Bh.begin(), Bh.end(), 0.0);
-        for (int64_t cc = 0; cc < k; ++cc) {
-            Bh[(size_t)cc * work + cc] = BSh[cc];
-            Bh[(size_t)k * work + cc] = res[cc];
-        }
-        SEV_HIP(hipMemcpyAsync(w.B.get(), Bh.data(), sizeof(double) * work * work,
-                               hipMemcpyHostToDevice, st));
     }
     if (out) {
         out->iters = iter;
         out->nmult = nmult;
+        out->svd_sweeps = info_h[5];
     }
+    if (std::getenv("SEV_VERBOSE"))
+        std::fprintf(stderr, "[severo_hip] irlba %lld x %lld nu=%lld work=%d: %lld restarts, %lld "
+                     "products, %d Jacobi sweeps in the projected SVDs\n", (long long)m,
+                     (long long)n, (long long)nu, work, (long long)iter, (long long)nmult,
+                     info_h[5]);
     if (!converged) fail(kErrConvergence, "irlba: convergence failed (maxit reached)");
+    SEV_HIP(hipMemcpyAsync(BSh.data(), w.BS.get(), sizeof(double) * work, hipMemcpyDeviceToHost,
+                           st));
+    SEV_HIP(hipStreamSynchronize(st));
     for (int64_t i = 0; i < nu; ++i) s[i] = BSh[i];
-    SEV_HIP(hipMemcpyAsync(w.BU.get(), BUh.data(), sizeof(double) * work * work,
-                           hipMemcpyHostToDevice, st));
-    SEV_HIP(hipMemcpyAsync(w.BV.get(), BVh.data(), sizeof(double) * work * work,
-                           hipMemcpyHostToDevice, st));
     launch_gemm("final_u", m, work, w.W.get(), w.BU.get(), (int)nu, U, nullptr, w.scal.get(), st);
     launch_gemm("final_v", n, work, w.V.get(), w.BV.get(), (int)nu, V, nullptr, w.scal.get(), st);
     SEV_HIP(hipStreamSynchronize(st));

@@ -137,12 +137,12 @@ struct HostOperator {
 };
 
 struct IrlbaOut {
-    int64_t iters = 0, nmult = 0;
+    int64_t iters = 0, nmult = 0, svd_sweeps = 0;
 };
 
 struct IrlbaWork {
-    DevBuf<double> W, W2, V, V2, B, BU, BV, T, part, part2, red, Fv, scal, rnd, rpart;
-    DevBuf<int> flags;
+    DevBuf<double> W, W2, V, V2, B, BU, BV, BS, svr, T, part, part2, red, Fv, scal, rnd, rpart;
+    DevBuf<int> flags, info;
 };
 
 // Runs the Lanczos bidiagonalisation on the device operator.  init (host,
