@@ -26,6 +26,7 @@
 #include "sev_kernels.hpp"
 
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 
 namespace sev {
@@ -156,13 +157,20 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         listN[i] = 0;
         thrL[i] = ninf();
     }
-    // query operands in registers
+    // query operands and the running thresholds of this lane's output rows
+    // (rows 4*grp + i of each row block) in registers; rows past n never
+    // accept a candidate
     float af[QW][MODE == 0 ? KS : 1];
     v8bf ab[QW][MODE == 1 ? KS : 1];
+    float thr[QW][4];
 #pragma unroll
     for (int rb = 0; rb < QW; ++rb) {
         const int64_t qrow = q0 + w * QW * 16 + rb * 16 + col;
         const bool ok = qrow < n;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            thr[rb][i] = q0 + w * QW * 16 + rb * 16 + 4 * grp + i < n ? ninf()
+                                                                       : __builtin_huge_valf();
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             if (MODE == 0) {
@@ -199,6 +207,10 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
             }
         }
         __syncthreads();
+        // the tile holds padding columns, or this block's own queries when
+        // self pairs are excluded: those pairs take the checked path
+        const bool edge = t0 + kCT > n ||
+                          (!include_self && t0 < q0 + QB && q0 < t0 + kCT);
 #pragma unroll
         for (int cb = 0; cb < kCT / 16; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
@@ -224,13 +236,17 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                                                                       0);
                 }
                 const int rbase = w * QW * 16 + rb * 16 + 4 * grp;  // local query row of acc[0]
+                if (edge) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (c >= n || (!include_self && c == q0 + rbase + i)) acc[i] = ninf();
+                }
+                const bool any = acc[0] > thr[rb][0] || acc[1] > thr[rb][1] ||
+                                 acc[2] > thr[rb][2] || acc[3] > thr[rb][3];
+                if (!__ballot(any)) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int ql = rbase + i;
-                    const int64_t q = q0 + ql;
-                    const bool pass = c < n && q < n && (include_self || c != q) &&
-                                      acc[i] > thrL[ql];
-                    uint64_t mask = __ballot(pass);
+                    uint64_t mask = __ballot(acc[i] > thr[rb][i]);
                     while (mask) {
                         const int src = __builtin_ctzll(mask);
                         mask &= mask - 1;
@@ -253,10 +269,12 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                         }
                         const float last = __shfl(lane == pos ? v : (lane > pos ? uv : lv), KL - 1,
                                                   kWave);
+                        const float nthr = ncnt == KL ? last : ninf();
                         if (lane == 0) {
                             listN[qlo] = ncnt;
-                            thrL[qlo] = ncnt == KL ? last : ninf();
+                            thrL[qlo] = nthr;
                         }
+                        if (grp == (src >> 4)) thr[rb][i] = nthr;
                     }
                 }
             }
@@ -457,8 +475,11 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
     if (bf16) {
         SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
+        static const int qw = std::getenv("SEV_KNN_QW") ? std::atoi(std::getenv("SEV_KNN_QW")) : 2;
         if (k <= 24) {
-            if (d + 2 <= 64)
+            if (d + 2 <= 64 && qw == 4)
+                run_mfma<1, 64, 4, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+            else if (d + 2 <= 64)
                 run_mfma<1, 64, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
             else
                 run_mfma<1, 128, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
