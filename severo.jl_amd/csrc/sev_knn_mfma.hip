@@ -26,6 +26,7 @@
 #include "sev_kernels.hpp"
 
 #include <climits>
+#include <type_traits>
 #include <cstdlib>
 #include <cmath>
 
@@ -140,7 +141,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
     constexpr int QB = kNW * QW * 16;
     constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
     constexpr int KS = MODE == 0 ? KP / 4 : KP / 32;  // MFMA steps per block
-    __shared__ __attribute__((aligned(16))) E tile[kCT * ROW];
+    __shared__ __attribute__((aligned(16))) E tiles[2][kCT * ROW];
     __shared__ float listV[QB][KL];
     __shared__ int32_t listI[QB][KL];
     __shared__ int listN[QB];
@@ -188,29 +189,59 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         }
     }
 
-    for (int64_t t0 = 0; t0 < n; t0 += kCT) {
-        __syncthreads();
-        if (MODE == 0) {
-            for (int e = threadIdx.x; e < kCT * KP; e += kBlock) {
-                const int r = e / KP, k = e % KP;
-                const int64_t c = t0 + r;
-                tile[r * ROW + k] = c < n ? Zc[c * KP + k] : E(0);
-            }
-        } else {
-            constexpr int CH = KP / 8;  // 16-byte chunks per row
-            for (int e = threadIdx.x; e < kCT * CH; e += kBlock) {
-                const int r = e / CH, part = e % CH;
-                const int64_t c = t0 + r;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (c < n) v = *(const uint4 *)((const uint16_t *)Zc + c * KP + part * 8);
-                *(uint4 *)(&tile[r * ROW + part * 8]) = v;
+    // Candidate tiles are double-buffered: the loads of tile t+1 are issued
+    // into registers before tile t is multiplied and land in the other LDS
+    // buffer afterwards, so one barrier per tile suffices.
+    // 16-byte loads: 4 floats (FP32) or 8 bf16 values per load
+    constexpr int CH = MODE == 0 ? KP / 4 : KP / 8;       // loads per candidate row
+    constexpr int NLD = (kCT * CH + kBlock - 1) / kBlock;  // loads per thread
+    static_assert(KP % (MODE == 0 ? 4 : 8) == 0, "row length must be a multiple of 16 bytes");
+    uint4 stage[NLD];
+    auto fetch = [&](int64_t t0) {
+#pragma unroll
+        for (int u = 0; u < NLD; ++u) {
+            const int e = threadIdx.x + u * kBlock;
+            const int r = e / CH, part = e % CH;
+            const int64_t c = t0 + r;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (e < kCT * CH && c < n)
+                v = *(const uint4 *)((const E *)Zc + c * KP + part * (MODE == 0 ? 4 : 8));
+            stage[u] = v;
+        }
+    };
+    auto deposit = [&](E *tile) {
+#pragma unroll
+        for (int u = 0; u < NLD; ++u) {
+            const int e = threadIdx.x + u * kBlock;
+            if (e >= kCT * CH) break;
+            const int r = e / CH, part = e % CH;
+            if (MODE == 0) {
+                float *d = (float *)tile + r * ROW + part * 4;  // odd row stride: scalar stores
+                d[0] = __uint_as_float(stage[u].x);
+                d[1] = __uint_as_float(stage[u].y);
+                d[2] = __uint_as_float(stage[u].z);
+                d[3] = __uint_as_float(stage[u].w);
+            } else {
+                *(uint4 *)(&tile[r * ROW + part * 8]) = stage[u];
             }
         }
-        __syncthreads();
+    };
+    fetch(0);
+    deposit(tiles[0]);
+    // every global load issued so far (query operands, first tile) is
+    // complete here, so the loop only waits for its own prefetches
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += kCT, buf ^= 1) {
+        const E *tile = tiles[buf];
+        const bool more = t0 + kCT < n;
+        if (more) fetch(t0 + kCT);
         // the tile holds padding columns, or this block's own queries when
         // self pairs are excluded: those pairs take the checked path
         const bool edge = t0 + kCT > n ||
                           (!include_self && t0 < q0 + QB && q0 < t0 + kCT);
+        const int t32 = (int)t0, n32 = (int)n, q32 = (int)q0;
 #pragma unroll
         for (int cb = 0; cb < kCT / 16; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
@@ -223,7 +254,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                     bbv[s] = *(const v8bf *)((const uint16_t *)tile + (cb * 16 + col) * ROW +
                                              32 * s + 8 * grp);
             }
-            const int64_t c = t0 + cb * 16 + col;
+            const int c = t32 + cb * 16 + col;
 #pragma unroll
             for (int rb = 0; rb < QW; ++rb) {
                 v4f acc = {0.f, 0.f, 0.f, 0.f};
@@ -239,10 +270,10 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                 if (edge) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        if (c >= n || (!include_self && c == q0 + rbase + i)) acc[i] = ninf();
+                        if (c >= n32 || (!include_self && c == q32 + rbase + i)) acc[i] = ninf();
                 }
-                const bool any = acc[0] > thr[rb][0] || acc[1] > thr[rb][1] ||
-                                 acc[2] > thr[rb][2] || acc[3] > thr[rb][3];
+                const bool any = (acc[0] > thr[rb][0]) | (acc[1] > thr[rb][1]) |
+                                 (acc[2] > thr[rb][2]) | (acc[3] > thr[rb][3]);
                 if (!__ballot(any)) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -252,7 +283,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                         mask &= mask - 1;
                         const float v = __shfl(acc[i], src, kWave);
                         const int qlo = w * QW * 16 + rb * 16 + 4 * (src >> 4) + i;
-                        const int32_t cand = (int32_t)(t0 + cb * 16 + (src & 15));
+                        const int32_t cand = t32 + cb * 16 + (src & 15);
                         if (!(v > thrL[qlo])) continue;
                         // insert (v, cand) into the descending list of qlo
                         const int cnt = listN[qlo];
@@ -279,8 +310,9 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                 }
             }
         }
+        if (more) deposit(tiles[buf ^ 1]);
+        __syncthreads();
     }
-    __syncthreads();
     for (int ql = w; ql < QB; ql += kNW) {
         const int64_t q = q0 + ql;
         if (q >= n) continue;
@@ -473,6 +505,7 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
               int64_t *n_fallback) {
     SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
     SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
+    SEV_REQUIRE(n < ((int64_t)1 << 31) - 256, "knn: at most 2^31 - 256 points");
     if (bf16) {
         SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
         static const int qw = std::getenv("SEV_KNN_QW") ? std::atoi(std::getenv("SEV_KNN_QW")) : 2;
