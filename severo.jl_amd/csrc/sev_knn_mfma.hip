@@ -238,7 +238,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         const bool more = t0 + kCT < n;
         if (more) fetch(t0 + kCT);
         // the tile holds padding columns, or this block's own queries when
-        // self pairs are excluded: those pairs take the checked path
+        // self pairs are excluded: such pairs are rejected in the insertion
+        // path (their products may pass the threshold test)
         const bool edge = t0 + kCT > n ||
                           (!include_self && t0 < q0 + QB && q0 < t0 + kCT);
         const int t32 = (int)t0, n32 = (int)n, q32 = (int)q0;
@@ -254,7 +255,6 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                     bbv[s] = *(const v8bf *)((const uint16_t *)tile + (cb * 16 + col) * ROW +
                                              32 * s + 8 * grp);
             }
-            const int c = t32 + cb * 16 + col;
 #pragma unroll
             for (int rb = 0; rb < QW; ++rb) {
                 v4f acc = {0.f, 0.f, 0.f, 0.f};
@@ -265,12 +265,6 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                     else
                         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[rb][s], bbv[s], acc, 0, 0,
                                                                       0);
-                }
-                const int rbase = w * QW * 16 + rb * 16 + 4 * grp;  // local query row of acc[0]
-                if (edge) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        if (c >= n32 || (!include_self && c == q32 + rbase + i)) acc[i] = ninf();
                 }
                 const bool any = (acc[0] > thr[rb][0]) | (acc[1] > thr[rb][1]) |
                                  (acc[2] > thr[rb][2]) | (acc[3] > thr[rb][3]);
@@ -284,6 +278,9 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                         const float v = __shfl(acc[i], src, kWave);
                         const int qlo = w * QW * 16 + rb * 16 + 4 * (src >> 4) + i;
                         const int32_t cand = t32 + cb * 16 + (src & 15);
+                        // padding columns and excluded self pairs reach this
+                        // path only in the tiles that hold them
+                        if (edge && (cand >= n32 || (!include_self && cand == q32 + qlo))) continue;
                         if (!(v > thrL[qlo])) continue;
                         // insert (v, cand) into the descending list of qlo
                         const int cnt = listN[qlo];
