@@ -141,11 +141,11 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
     constexpr int QB = kNW * QW * 16;
     constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
     constexpr int KS = MODE == 0 ? KP / 4 : KP / 32;  // MFMA steps per block
+    constexpr int NCB = kCT / 16;                     // candidate blocks per tile
     __shared__ __attribute__((aligned(16))) E tiles[2][kCT * ROW];
     __shared__ float listV[QB][KL];
     __shared__ int32_t listI[QB][KL];
     __shared__ int listN[QB];
-    __shared__ float thrL[QB];
 
     const E *Zq = (const E *)Zq_;
     const E *Zc = (const E *)Zc_;
@@ -154,10 +154,11 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
     const int64_t q0 = (int64_t)blockIdx.x * QB;
     const int col = lane & 15, grp = lane >> 4;
 
-    for (int i = threadIdx.x; i < QB; i += kBlock) {
-        listN[i] = 0;
-        thrL[i] = ninf();
+    for (int e = threadIdx.x; e < QB * KL; e += kBlock) {
+        (&listV[0][0])[e] = ninf();
+        (&listI[0][0])[e] = INT_MAX;
     }
+    for (int i = threadIdx.x; i < QB; i += kBlock) listN[i] = 0;
     // query operands and the running thresholds of this lane's output rows
     // (rows 4*grp + i of each row block) in registers; rows past n never
     // accept a candidate
@@ -243,8 +244,10 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         const bool edge = t0 + kCT > n ||
                           (!include_self && t0 < q0 + QB && q0 < t0 + kCT);
         const int t32 = (int)t0, n32 = (int)n, q32 = (int)q0;
+        // all products of the tile: acc[cb][rb] = 16 queries x 16 candidates
+        v4f acc[NCB][QW];
 #pragma unroll
-        for (int cb = 0; cb < kCT / 16; ++cb) {
+        for (int cb = 0; cb < NCB; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
             v8bf bbv[MODE == 1 ? KS : 1];
 #pragma unroll
@@ -257,52 +260,75 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
             }
 #pragma unroll
             for (int rb = 0; rb < QW; ++rb) {
-                v4f acc = {0.f, 0.f, 0.f, 0.f};
+                v4f a = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
                     if (MODE == 0)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rb][s], bfv[s], acc, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rb][s], bfv[s], a, 0, 0, 0);
                     else
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[rb][s], bbv[s], acc, 0, 0,
-                                                                      0);
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[rb][s], bbv[s], a, 0, 0, 0);
                 }
-                const bool any = (acc[0] > thr[rb][0]) | (acc[1] > thr[rb][1]) |
-                                 (acc[2] > thr[rb][2]) | (acc[3] > thr[rb][3]);
-                if (!__ballot(any)) continue;
+                acc[cb][rb] = a;
+            }
+        }
+#pragma unroll
+        for (int rb = 0; rb < QW; ++rb) {
+            // one test per output row for the whole tile: the largest of its
+            // 4 x 16 products against the row's threshold
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float m = fmaxf(fmaxf(acc[0][rb][i], acc[1][rb][i]),
+                                      fmaxf(acc[2][rb][i], acc[3][rb][i]));
+                any |= m > thr[rb][i];
+            }
+            if (!__ballot(any)) continue;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                // products as integers for readlane (on vector elements it was
+                // seen to read element 0 for every element)
+                int ai[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) ai[i] = __float_as_int(acc[cb][rb][i]);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    uint64_t mask = __ballot(acc[i] > thr[rb][i]);
+                    uint64_t mask = __ballot(acc[cb][rb][i] > thr[rb][i]);
                     while (mask) {
                         const int src = __builtin_ctzll(mask);
                         mask &= mask - 1;
-                        const float v = __shfl(acc[i], src, kWave);
-                        const int qlo = w * QW * 16 + rb * 16 + 4 * (src >> 4) + i;
+                        const int g = src >> 4;
+                        const float v = __int_as_float(__builtin_amdgcn_readlane(ai[i], src));
+                        // the row's threshold lives in the lanes of group g
+                        const float ts =
+                            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[rb][i]),
+                                                                     16 * g));
+                        if (!(v > ts)) continue;
+                        const int qlo = w * QW * 16 + rb * 16 + 4 * g + i;
                         const int32_t cand = t32 + cb * 16 + (src & 15);
-                        // padding columns and excluded self pairs reach this
-                        // path only in the tiles that hold them
                         if (edge && (cand >= n32 || (!include_self && cand == q32 + qlo))) continue;
-                        if (!(v > thrL[qlo])) continue;
-                        // insert (v, cand) into the descending list of qlo
+                        // insert (v, cand) into the descending list of qlo:
+                        // entries from pos on move one slot up by LDS stores
                         const int cnt = listN[qlo];
-                        const float lv = lane < cnt && lane < KL ? listV[qlo][lane] : ninf();
-                        const int32_t li = lane < cnt && lane < KL ? listI[qlo][lane] : INT_MAX;
+                        const float lv = lane < KL ? listV[qlo][lane] : ninf();
+                        const int32_t li = lane < KL ? listI[qlo][lane] : INT_MAX;
                         const bool better = lane < cnt && (lv > v || (lv == v && li < cand));
                         const int pos = __popcll(__ballot(better));
-                        const float uv = __shfl_up(lv, 1, kWave);
-                        const int32_t ui = __shfl_up(li, 1, kWave);
                         const int ncnt = cnt + 1 < KL ? cnt + 1 : KL;
-                        if (lane < KL && lane >= pos && lane < ncnt) {
-                            listV[qlo][lane] = lane == pos ? v : uv;
-                            listI[qlo][lane] = lane == pos ? cand : ui;
+                        if (lane >= pos && lane + 1 < ncnt) {
+                            listV[qlo][lane + 1] = lv;
+                            listI[qlo][lane + 1] = li;
                         }
-                        const float last = __shfl(lane == pos ? v : (lane > pos ? uv : lv), KL - 1,
-                                                  kWave);
+                        if (lane == pos) {
+                            listV[qlo][pos] = v;
+                            listI[qlo][pos] = cand;
+                        }
+                        const float last =
+                            pos == KL - 1 ? v
+                                          : __int_as_float(__builtin_amdgcn_readlane(
+                                                __float_as_int(lv), KL - 2));
                         const float nthr = ncnt == KL ? last : ninf();
-                        if (lane == 0) {
-                            listN[qlo] = ncnt;
-                            thrL[qlo] = nthr;
-                        }
-                        if (grp == (src >> 4)) thr[rb][i] = nthr;
+                        if (lane == 0) listN[qlo] = ncnt;
+                        if (grp == g) thr[rb][i] = nthr;
                     }
                 }
             }
@@ -313,10 +339,11 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
     for (int ql = w; ql < QB; ql += kNW) {
         const int64_t q = q0 + ql;
         if (q >= n) continue;
-        if (lane < KL) cidx[q * KL + lane] = lane < listN[ql] ? listI[ql][lane] : -1;
+        const int cnt = listN[ql];
+        if (lane < KL) cidx[q * KL + lane] = lane < cnt ? listI[ql][lane] : -1;
         if (lane == 0) {
-            cthr[q] = thrL[ql];
-            ccnt[q] = listN[ql];
+            cthr[q] = cnt == KL ? listV[ql][KL - 1] : ninf();
+            ccnt[q] = cnt;
         }
     }
 }
