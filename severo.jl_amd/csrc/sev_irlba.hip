@@ -39,7 +39,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
 constexpr int kMaxWork = 64;
-constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v)
+constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v, FP64)
+constexpr int64_t kMaxFL32 = 14336;  // the same for the FP32 copy of the BF16 form
 constexpr int kRedChunks = 16;       // first-level chunks of the slab partials
 constexpr int kDotBlocks = 2048;     // blocks of the W' y partial products
 constexpr int64_t kBuildRows = 256;  // row slabs of the CSR build
@@ -165,10 +166,11 @@ k_row_count(const int32_t *row, int64_t nnz, int32_t *cnt) {
         atomicAdd(&cnt[row[p]], 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_round4(int32_t *cnt, int64_t m) {
+// row lengths rounded up to a multiple of R (a power of two)
+__global__ void __launch_bounds__(kBlock) k_round_up(int32_t *cnt, int64_t m, int R) {
     for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
          r += (int64_t)gridDim.x * kBlock)
-        cnt[r] = (cnt[r] + 3) & ~3;
+        cnt[r] = (cnt[r] + R - 1) & ~(R - 1);
 }
 
 // Padded CSR fill: one wavefront per slab of kBuildRows rows walks the
@@ -258,6 +260,67 @@ k_av_csr(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const
         acc = __shfl(acc, gi * G, kWave);
         if (gl == 0 && r < m) {
             double t = acc - dmu;
+            if (wprev) t -= rfv * wprev[r];
+            y[r] = t;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// BF16 form of A v: the same product with BF16 matrix values and FP32
+// arithmetic (v staged in LDS as FP32, FP32 accumulation); the centering and
+// recurrence terms are applied in FP64.  Rows are padded to multiples of 8, so
+// each lane reads 8 columns and 8 values with two 16-byte loads; two
+// accumulators alternate and are combined in order.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
+
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const uint16_t *val,
+          const float *mu, const double *v, const double *rf, const double *wprev, double *y) {
+    extern __shared__ float vf[];
+    __shared__ double dmu_s;
+    const int w = wave_index(), lane = lane_id();
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) vf[i] = (float)v[i];
+    if (w == 0) {
+        double d = 0.0;
+        if (mu)
+            for (int64_t i = lane; i < n; i += kWave) d = fma((double)mu[i], v[i], d);
+        d = wave_sum(d);
+        if (lane == 0) dmu_s = d;
+    }
+    __syncthreads();
+    const double dmu = dmu_s;
+    const double rfv = rf ? *rf : 0.0;
+    constexpr int RPW = kWave / G;  // rows per wave
+    const int gl = lane % G, gi = lane / G;
+    const int64_t stride = (int64_t)gridDim.x * kWaves * RPW;
+    for (int64_t r0 = ((int64_t)blockIdx.x * kWaves + w) * RPW; r0 < m; r0 += stride) {
+        const int64_t r = r0 + gi;
+        float a0 = 0.0f, a1 = 0.0f;
+        if (r < m) {
+            const int64_t b = rowptr[r], e = rowptr[r + 1];
+            for (int64_t p = b + 8 * gl; p < e; p += 8 * G) {
+                const uint4 c8 = *(const uint4 *)(col + p);
+                const uint4 v8 = *(const uint4 *)(val + p);
+                a0 = fmaf(bf_lo(v8.x), vf[c8.x & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.x), vf[c8.x >> 16], a1);
+                a0 = fmaf(bf_lo(v8.y), vf[c8.y & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.y), vf[c8.y >> 16], a1);
+                a0 = fmaf(bf_lo(v8.z), vf[c8.z & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.z), vf[c8.z >> 16], a1);
+                a0 = fmaf(bf_lo(v8.w), vf[c8.w & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.w), vf[c8.w >> 16], a1);
+            }
+        }
+        float acc = a0 + a1;
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+        acc = __shfl(acc, gi * G, kWave);
+        if (gl == 0 && r < m) {
+            double t = (double)acc - dmu;
             if (wprev) t -= rfv * wprev[r];
             y[r] = t;
         }
@@ -392,6 +455,39 @@ k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *
         }
         const double acc = wave_sum(acc0 + acc1);
         if (lane == 0) prow[j] = acc;
+    }
+}
+
+// BF16 form of the slab partials of B' y: y staged in LDS as FP32, FP32
+// accumulation per column segment (the slab sums are reduced in FP64).
+__global__ void __launch_bounds__(kBlock)
+k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const uint16_t *val,
+           const double *y, double *part) {
+    __shared__ float wl[kSlabRows];
+    const int64_t s = blockIdx.x;
+    const int g = blockIdx.y, ng = gridDim.y;
+    const int64_t r0 = s * kSlabRows;
+    const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
+    const int w = wave_index(), lane = lane_id();
+    for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock)
+        wl[lr] = lr < nrows ? (float)y[r0 + lr] : 0.0f;
+    __syncthreads();
+    double *prow = part + s * (n + 2);
+    for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
+        const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
+        float acc0 = 0.0f, acc1 = 0.0f;
+        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
+            const ushort4 l4 = *(const ushort4 *)(lrow + p);
+            const uint2 v4 = *(const uint2 *)(val + p);
+            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
+                       v3 = p + 3 < b;
+            acc0 = fmaf(v0 ? bf_lo(v4.x) : 0.0f, wl[v0 ? l4.x : 0], acc0);
+            acc1 = fmaf(v1 ? bf_hi(v4.x) : 0.0f, wl[v1 ? l4.y : 0], acc1);
+            acc0 = fmaf(v2 ? bf_lo(v4.y) : 0.0f, wl[v2 ? l4.z : 0], acc0);
+            acc1 = fmaf(v3 ? bf_hi(v4.y) : 0.0f, wl[v3 ? l4.w : 0], acc1);
+        }
+        const float acc = wave_sum(acc0 + acc1);
+        if (lane == 0) prow[j] = (double)acc;
     }
 }
 
@@ -901,15 +997,18 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                            op.nnz, b.cnt.get());
         SEV_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_round4, dim3(grid_rows(m)), dim3(kBlock), 0, st, b.cnt.get(), m);
+    // BF16 rows are padded to 8 entries (16-byte loads of values and columns)
+    const int ralign = bf16 ? 8 : 4;
+    hipLaunchKernelGGL(k_round_up, dim3(grid_rows(m)), dim3(kBlock), 0, st, b.cnt.get(), m,
+                       ralign);
     SEV_LAUNCH_CHECK();
     scan_exclusive_i32(b.cnt.get(), op.rowptr.get(), m, b.scan.get(), st);
     SEV_HIP(hipMemcpyAsync(&op.csr_nnz, op.rowptr.get() + m, sizeof(int64_t),
                            hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
     const size_t vb = (size_t)val_bytes(op.vtype);
-    op.ccol.ensure((size_t)op.csr_nnz + 4);
-    op.cval.ensure(((size_t)op.csr_nnz + 4) * vb);
+    op.ccol.ensure((size_t)op.csr_nnz + 8);
+    op.cval.ensure(((size_t)op.csr_nnz + 8) * vb);
     SEV_HIP(hipMemsetAsync(op.ccol.get(), 0, sizeof(uint16_t) * (size_t)op.csr_nnz, st));
     SEV_HIP(hipMemsetAsync(op.cval.get(), 0, vb * (size_t)op.csr_nnz, st));
     if (op.vtype == kValBF16)
@@ -922,7 +1021,10 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         launch_csr_fill<float, float>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
                                       op.ccol.get(), op.cval.get(), st);
     const double avg = (double)op.csr_nnz / (double)m;
-    op.lanes_per_row = avg > 192 ? 64 : (avg > 96 ? 32 : (avg > 48 ? 16 : 8));
+    if (op.vtype == kValBF16)
+        op.lanes_per_row = avg > 256 ? 16 : (avg > 64 ? 8 : 4);
+    else
+        op.lanes_per_row = avg > 192 ? 64 : (avg > 96 ? 32 : (avg > 48 ? 16 : 8));
     // the build scratch is released when b goes out of scope: wait for the fill
     SEV_HIP(hipStreamSynchronize(st));
 }
@@ -952,13 +1054,36 @@ void launch_av_t(const SlabCsc &op, int use_lds, const double *v, const double *
     }
 }
 
+template <int G>
+void launch_av_bf16_g(const SlabCsc &op, const double *v, const double *rf, const double *wprev,
+                      double *y, hipStream_t st) {
+    const unsigned grid = (unsigned)std::min<int64_t>(
+        ceil_div(op.m, (int64_t)kWaves * (kWave / G)), (int64_t)device_cu_count() * 8);
+    hipLaunchKernelGGL((k_av_bf16<G>), dim3(grid), dim3(kBlock), sizeof(float) * (size_t)op.n, st,
+                       op.m, op.n, op.rowptr.get(), op.ccol.get(), (const uint16_t *)op.cval.get(),
+                       (const float *)op.mu, v, rf, wprev, y);
+    SEV_LAUNCH_CHECK();
+}
+
+void launch_av_bf16(const SlabCsc &op, const double *v, const double *rf, const double *wprev,
+                    double *y, hipStream_t st) {
+    switch (op.lanes_per_row) {
+        case 4: launch_av_bf16_g<4>(op, v, rf, wprev, y, st); break;
+        case 8: launch_av_bf16_g<8>(op, v, rf, wprev, y, st); break;
+        case 16: launch_av_bf16_g<16>(op, v, rf, wprev, y, st); break;
+        default: launch_av_bf16_g<32>(op, v, rf, wprev, y, st); break;
+    }
+}
+
 void launch_av(const SlabCsc &op, const double *v, const double *rf, const double *wprev,
                double *y, hipStream_t st) {
     const int use_lds = op.n <= kMaxFL ? 1 : 0;
     ProfScope ps("spmv_av", st,
                  (double)op.csr_nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m * (wprev ? 2 : 1) +
                      8.0 * (op.m + 1) + 8.0 * op.n);
-    if (op.vtype == kValBF16)
+    if (op.vtype == kValBF16 && op.n <= kMaxFL32)
+        launch_av_bf16(op, v, rf, wprev, y, st);
+    else if (op.vtype == kValBF16)
         launch_av_t<uint16_t, float>(op, use_lds, v, rf, wprev, y, st);
     else if (op.vtype == kValF64)
         launch_av_t<double, double>(op, use_lds, v, rf, wprev, y, st);
@@ -980,7 +1105,7 @@ void launch_atw_only(const SlabCsc &op, const double *y, double *part, hipStream
                      8.0 * (op.nslab + 1) * op.n);
     const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
     if (op.vtype == kValBF16)
-        hipLaunchKernelGGL(k_atw<uint16_t>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+        hipLaunchKernelGGL(k_atw_bf16, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
                            op.lrow.get(), (const uint16_t *)op.vals, y, part);
     else if (op.vtype == kValF64)
         hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
@@ -1090,7 +1215,9 @@ void autotune_operator(SlabCsc &op, hipStream_t st) {
     t_prof = nullptr;  // keep the trial launches out of the kernel profile
     TuneEntry best;
     float best_ms = 1e30f;
-    for (int g : {8, 16, 32, 64}) {
+    const std::vector<int> lanes_opts = op.vtype == kValBF16 ? std::vector<int>{4, 8, 16, 32}
+                                                              : std::vector<int>{8, 16, 32, 64};
+    for (int g : lanes_opts) {
         op.lanes_per_row = g;
         const float ms = time_launches(st, [&] { launch_av(op, v.get(), nullptr, nullptr, y.get(), st); }, 3);
         if (ms < best_ms) {

@@ -254,3 +254,31 @@ def test_irlba_callback_randv_and_maxit(sev):
     rc = L.irlba(m, n, 20, 27, 1, 0, 1e-14, init.ctypes.data, s2.ctypes.data, U2.ctypes.data,
                  V2.ctypes.data, None, C.cast(mcb, C.c_void_p), None)
     assert rc == -3
+
+
+def _bf16_round(a):
+    """Round float32 values to bfloat16 (nearest even), returned as float64."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("m,n,dens", [(3000, 600, 0.5), (9000, 400, 0.3), (4000, 900, 0.08),
+                                      (2500, 8000, 0.02), (2000, 15000, 0.01)])
+def test_irlba_bf16_form_layouts(sev, m, n, dens):
+    """The BF16 form decomposes the BF16-rounded centered matrix with FP32
+    products: singular values within 1e-5 relative of LAPACK on that matrix,
+    for row lengths that select every lane width and feature counts above the
+    LDS copy of v."""
+    rng = np.random.default_rng(m * 3 + n)
+    A = sp.random(m, n, dens, format="csc", random_state=m + 1, dtype=np.float64)
+    A = A.astype(np.float32)
+    mu = np.asarray(A.mean(axis=0), np.float32).ravel()
+    nu = 6
+    S = sev.irlba(sev.CenteredMatrix(A, mu), nu, tol=1e-7, init=rng.standard_normal(n),
+                  precision="bf16")
+    Ab = A.copy().astype(np.float64)
+    Ab.data = _bf16_round(A.data)
+    Q = Ab.toarray() - mu.astype(np.float64)
+    ss = np.linalg.svd(Q, compute_uv=False)
+    assert np.max(np.abs(S.S - ss[:nu]) / ss[:nu]) < 1e-5
