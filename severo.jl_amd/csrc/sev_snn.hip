@@ -18,6 +18,8 @@
 #include "sev_kernels.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace sev {
@@ -26,11 +28,10 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
-constexpr int kWaveSlots = 2048;   // per-wave table
-constexpr int kWaveMaxP = 1024;    // candidates handled by the wave path
-constexpr int kBlockSlots = 4096;  // per-block LDS table
-constexpr int kBlockMaxP = 2048;
-constexpr int kBlockKept = 2048;   // LDS sort buffer of kept entries
+constexpr int kWaveKept = 256;    // kept entries a wave sorts in LDS
+constexpr int kBlockKept = 2048;  // kept entries a block sorts in its buffer
+constexpr int kTableBytes = 16384;    // per-wave hash table
+constexpr int kBlockTableBytes = 65536;  // per-block hash table
 
 // Entries outside [0, n) (an incomplete neighbour list) are ignored.
 __global__ void __launch_bounds__(kBlock)
@@ -54,29 +55,79 @@ k_rev_fill(const int32_t *nn, int64_t total, int64_t n, int64_t k, const int64_t
     }
 }
 
+// Hash-table slots: key + 1 in the high bits, the count in the low bits, so
+// that sorting slots sorts by row.  32-bit slots (rows < 2^25, counts < 128)
+// halve the table size; 64-bit slots serve larger matrices.
+template <class S>
+struct Slot;
+template <>
+struct Slot<uint32_t> {
+    static constexpr int kCountBits = 7;
+    static constexpr uint32_t kCountMask = 127u;
+};
+template <>
+struct Slot<uint64_t> {
+    static constexpr int kCountBits = 32;
+    static constexpr uint64_t kCountMask = 0xFFFFFFFFull;
+};
+
 __device__ __forceinline__ uint32_t slot_hash(uint32_t key) {
     uint32_t h = key * 0x9E3779B1u;
     return h ^ (h >> 15);
 }
 
-// Insert key into an open-addressing table of nslots (power of two).
-__device__ __forceinline__ void tab_insert(uint64_t *tab, uint32_t nslots, uint32_t key) {
-    const uint64_t tag = (uint64_t)(key + 1u) << 32;
+template <class S>
+__device__ __forceinline__ S slot_cas(S *a, S cmp, S val) {
+    return atomicCAS(a, cmp, val);
+}
+template <>
+__device__ __forceinline__ uint64_t slot_cas<uint64_t>(uint64_t *a, uint64_t cmp, uint64_t val) {
+    return (uint64_t)atomicCAS((unsigned long long *)a, (unsigned long long)cmp,
+                               (unsigned long long)val);
+}
+__device__ __forceinline__ void slot_inc(uint32_t *a) { atomicAdd(a, 1u); }
+__device__ __forceinline__ void slot_inc(uint64_t *a) {
+    atomicAdd((unsigned long long *)a, 1ull);
+}
+
+// Insert key into an open-addressing table of nslots (a power of two).
+template <class S>
+__device__ __forceinline__ void tab_insert(S *tab, uint32_t nslots, uint32_t key) {
+    const S tag = (S)(key + 1u) << Slot<S>::kCountBits;
     uint32_t h = slot_hash(key) & (nslots - 1);
     while (true) {
-        uint64_t cur = tab[h];
+        S cur = tab[h];
         if (cur == 0) {
-            const uint64_t old = atomicCAS((unsigned long long *)&tab[h], 0ull,
-                                           (unsigned long long)(tag | 1u));
+            const S old = slot_cas<S>(&tab[h], (S)0, (S)(tag | 1u));
             if (old == 0) return;
             cur = old;
         }
-        if ((cur >> 32) == (uint64_t)(key + 1u)) {
-            atomicAdd((unsigned long long *)&tab[h], 1ull);
+        if ((cur >> Slot<S>::kCountBits) == (S)(key + 1u)) {
+            slot_inc(&tab[h]);
             return;
         }
         h = (h + 1) & (nslots - 1);
     }
+}
+
+template <class S>
+__device__ __forceinline__ bool slot_kept(S v, int xmin) {
+    return v != 0 && (int)(v & Slot<S>::kCountMask) >= xmin;
+}
+template <class S>
+__device__ __forceinline__ int32_t slot_row(S v) {
+    return (int32_t)((v >> Slot<S>::kCountBits) - 1);
+}
+template <class S>
+__device__ __forceinline__ int slot_count(S v) {
+    return (int)(v & Slot<S>::kCountMask);
+}
+
+// Table size for P candidates: the next power of two >= 2P (at least 64).
+__device__ __forceinline__ uint32_t table_size(int64_t P) {
+    uint32_t t = 64;
+    while ((int64_t)t < 2 * P) t <<= 1;
+    return t;
 }
 
 // Per-column neighbour setup held in lanes 0..k-1: deduplicated l, the length
@@ -122,9 +173,9 @@ __device__ __forceinline__ int owner_lane(int64_t e, int64_t mypre) {
     return lo;
 }
 
-// Bitonic sort of n (power of two) 64-bit values in ascending order by one
-// wavefront (n <= slots of the table region).
-__device__ void wave_bitonic(uint64_t *a, int n) {
+// Bitonic sort of n (power of two) values ascending by one wavefront.
+template <class S>
+__device__ void wave_bitonic(S *a, int n) {
     const int lane = lane_id();
     for (int size = 2; size <= n; size <<= 1)
         for (int stride = size >> 1; stride > 0; stride >>= 1)
@@ -132,7 +183,7 @@ __device__ void wave_bitonic(uint64_t *a, int n) {
                 const int lo = 2 * i - (i & (stride - 1));
                 const int hi = lo + stride;
                 const bool up = ((lo & size) == 0);
-                const uint64_t x = a[lo], y = a[hi];
+                const S x = a[lo], y = a[hi];
                 if ((x > y) == up) {
                     a[lo] = y;
                     a[hi] = x;
@@ -140,14 +191,15 @@ __device__ void wave_bitonic(uint64_t *a, int n) {
             }
 }
 
-__device__ void block_bitonic(uint64_t *a, int n) {
-    for (int size = 2; size <= n; size <<= 1)
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = threadIdx.x; i < n / 2; i += blockDim.x) {
-                const int lo = 2 * i - (i & (stride - 1));
-                const int hi = lo + stride;
+template <class S>
+__device__ void block_bitonic(S *a, int64_t n) {
+    for (int64_t size = 2; size <= n; size <<= 1)
+        for (int64_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int64_t i = threadIdx.x; i < n / 2; i += blockDim.x) {
+                const int64_t lo = 2 * i - (i & (stride - 1));
+                const int64_t hi = lo + stride;
                 const bool up = ((lo & size) == 0);
-                const uint64_t x = a[lo], y = a[hi];
+                const S x = a[lo], y = a[hi];
                 if ((x > y) == up) {
                     a[lo] = y;
                     a[hi] = x;
@@ -157,48 +209,46 @@ __device__ void block_bitonic(uint64_t *a, int n) {
         }
 }
 
-// Wave path.  count mode (rows == nullptr): cnt[j] = kept entries, or -1 and
-// j appended to the heavy list when P exceeds the wave table.  fill mode:
-// writes the sorted kept entries of the light columns at colptr[j].
-template <class T>
+// Wave path: one wavefront per column with a private LDS table, for columns
+// with P <= table/2 candidates and at most kWaveKept kept entries; other
+// columns are listed as heavy (count mode) or skipped (fill mode) by the
+// same rule.  count mode (rows == nullptr): cnt[j] = kept entries.  fill
+// mode: the kept entries sorted by row at colptr[j].
+template <class T, class S>
 __global__ void __launch_bounds__(kBlock)
 k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
            const int32_t *rev, int32_t *cnt, int32_t *heavy, int64_t *heavyP, int32_t *nheavy,
            const T *wtab, const int64_t *colptr, int32_t *rows, T *vals) {
-    __shared__ uint64_t tabs[kWaves][kWaveSlots];
+    constexpr int NS = kTableBytes / (int)sizeof(S);
+    __shared__ S tabs[kWaves][NS];
+    __shared__ S kbufs[kWaves][kWaveKept];
     const int lane = lane_id(), w = threadIdx.x / kWave;
-    uint64_t *tab = tabs[w];
-    for (int i = lane; i < kWaveSlots; i += kWave) tab[i] = 0;
+    S *tab = tabs[w];
+    S *kbuf = kbufs[w];
+    for (int i = lane; i < NS; i += kWave) tab[i] = 0;
     const int64_t stride = (int64_t)gridDim.x * kWaves;
     for (int64_t j = (int64_t)blockIdx.x * kWaves + w; j < n; j += stride) {
         int64_t mybeg, mypre, mylen;
         const int64_t P = column_setup(nn, n, j, k, rptr, mybeg, mypre, mylen);
-        if (P > kWaveMaxP) {
+        if (P > NS / 2) {
             if (!rows && lane == 0) {
-                cnt[j] = -1;
                 const int h = atomicAdd(nheavy, 1);
                 heavy[h] = (int32_t)j;
                 heavyP[h] = P;
             }
             continue;
         }
+        const uint32_t T_ = table_size(P);
         // every lane runs the shuffles each round; lanes past P insert nothing
         for (int64_t base = 0; base < P; base += kWave) {
             const int64_t e = base + lane;
             const int o = owner_lane(e < P ? e : P - 1, mypre);
             const int64_t ob = __shfl(mybeg, o, kWave);
             const int64_t op = __shfl(mypre, o, kWave);
-            if (e < P) tab_insert(tab, kWaveSlots, (uint32_t)rev[ob + (e - op)]);
+            if (e < P) tab_insert<S>(tab, T_, (uint32_t)rev[ob + (e - op)]);
         }
-        // compact kept entries to the front of the table
-        uint64_t kept[kWaveSlots / kWave];
         int nk = 0;
-#pragma unroll
-        for (int i = 0; i < kWaveSlots / kWave; ++i) {
-            const uint64_t v = tab[lane + i * kWave];
-            kept[i] = v;
-            if (v != 0 && (int)(v & 0xFFFFFFFFu) >= xmin) nk++;
-        }
+        for (uint32_t i = lane; i < T_; i += kWave) nk += slot_kept<S>(tab[i], xmin) ? 1 : 0;
         int pre = nk;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
@@ -206,43 +256,118 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
             if (lane >= o) pre += y;
         }
         const int L = __shfl(pre, kWave - 1, kWave);
-        if (!rows) {
-            if (lane == 0) cnt[j] = L;
-            for (int i = lane; i < kWaveSlots; i += kWave) tab[i] = 0;
+        if (L > kWaveKept || !rows) {
+            if (!rows && lane == 0) {
+                if (L > kWaveKept) {
+                    const int h = atomicAdd(nheavy, 1);
+                    heavy[h] = (int32_t)j;
+                    heavyP[h] = P;
+                } else {
+                    cnt[j] = L;
+                }
+            }
+            for (uint32_t i = lane; i < T_; i += kWave) tab[i] = 0;
             continue;
         }
         int pos = pre - nk;
-        for (int i = lane; i < kWaveSlots; i += kWave) tab[i] = 0;
-#pragma unroll
-        for (int i = 0; i < kWaveSlots / kWave; ++i) {
-            const uint64_t v = kept[i];
-            if (v != 0 && (int)(v & 0xFFFFFFFFu) >= xmin) tab[pos++] = v;
+        for (uint32_t i = lane; i < T_; i += kWave) {
+            const S v = tab[i];
+            if (slot_kept<S>(v, xmin)) kbuf[pos++] = v;
+            tab[i] = 0;
         }
         int Lp = 1;
         while (Lp < L) Lp <<= 1;
-        for (int i = L + lane; i < Lp; i += kWave) tab[i] = ~0ull;
-        wave_bitonic(tab, Lp);
+        for (int i = L + lane; i < Lp; i += kWave) kbuf[i] = ~(S)0;
+        wave_bitonic<S>(kbuf, Lp);
         const int64_t ob = colptr[j];
         for (int i = lane; i < L; i += kWave) {
-            const uint64_t v = tab[i];
-            rows[ob + i] = (int32_t)((v >> 32) - 1);
-            vals[ob + i] = wtab[v & 0xFFFFFFFFu];
+            const S v = kbuf[i];
+            rows[ob + i] = slot_row<S>(v);
+            vals[ob + i] = wtab[slot_count<S>(v)];
         }
-        for (int i = lane; i < Lp; i += kWave) tab[i] = 0;
     }
 }
 
-// Heavy columns: one block per column, table in LDS (P <= kBlockMaxP) or in
-// global memory at gtab + gtab_off[h] with gtab_size[h] slots.
-template <class T>
+// Count the candidates of one column into tab (nslots slots, zeroed here)
+// with the whole block, and return the number of kept entries.
+template <class S>
+__device__ int block_count(S *tab, uint32_t nslots, int64_t P, int k, const int32_t *rev,
+                           const int64_t *sbeg, const int64_t *spre, int xmin, int *wk) {
+    const int lane = lane_id(), w = threadIdx.x / kWave;
+    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) tab[i] = 0;
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < P; e += kBlock) {
+        // owner = last neighbour slot whose prefix is <= e (deduplicated
+        // slots have zero length and share the next slot's prefix)
+        int lo = 0;
+        for (int t = 1; t < k; ++t)
+            if (spre[t] <= e) lo = t;
+        tab_insert<S>(tab, nslots, (uint32_t)rev[sbeg[lo] + (e - spre[lo])]);
+    }
+    __syncthreads();
+    int nk = 0;
+    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) nk += slot_kept<S>(tab[i], xmin);
+    int wsum = nk;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(wsum, o, kWave);
+        if (lane >= o) wsum += y;
+    }
+    if (lane == kWave - 1) wk[w] = wsum;
+    __syncthreads();
+    int L = 0;
+    for (int q = 0; q < kWaves; ++q) L += wk[q];
+    __syncthreads();
+    return L;
+}
+
+// Write the kept entries of tab sorted by row: gathered into kept (when
+// given and large enough) and sorted there, or sorted in place in the table
+// after the other slots are set to the largest key.
+template <class S, class T>
+__device__ void block_emit(S *tab, uint32_t nslots, S *kept, int64_t kcap, int L, int xmin,
+                           int *nkept, const T *wtab, int32_t *rows, T *vals) {
+    S *src;
+    int64_t Lp = 1;
+    while (Lp < L) Lp <<= 1;
+    if (kept && L <= kcap) {
+        if (threadIdx.x == 0) *nkept = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) {
+            const S v = tab[i];
+            if (slot_kept<S>(v, xmin)) kept[atomicAdd(nkept, 1)] = v;
+        }
+        __syncthreads();
+        for (int64_t i = L + threadIdx.x; i < Lp; i += kBlock) kept[i] = ~(S)0;
+        __syncthreads();
+        block_bitonic<S>(kept, Lp);
+        src = kept;
+    } else {
+        for (uint32_t i = threadIdx.x; i < nslots; i += kBlock)
+            if (!slot_kept<S>(tab[i], xmin)) tab[i] = ~(S)0;
+        __syncthreads();
+        block_bitonic<S>(tab, nslots);
+        src = tab;
+    }
+    for (int64_t i = threadIdx.x; i < L; i += kBlock) {
+        const S v = src[i];
+        rows[i] = slot_row<S>(v);
+        vals[i] = wtab[slot_count<S>(v)];
+    }
+}
+
+// Heavy columns: one block per column, table in LDS (P <= half the LDS table)
+// or in global memory at gtab + gtab_off[h] (64-bit slots: 2/3 of the region
+// is the table, 1/3 the sort buffer).
+template <class T, class S>
 __global__ void __launch_bounds__(kBlock)
 k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
-            const int32_t *rev,
-            const int32_t *heavy, int64_t nheavy, int32_t *cnt, uint64_t *gtab,
-            const int64_t *gtab_off, const T *wtab, const int64_t *colptr, int32_t *rows,
-            T *vals) {
-    __shared__ uint64_t ltab[kBlockSlots];
-    __shared__ uint64_t lkept[kBlockKept];
+            const int32_t *rev, const int32_t *heavy, int64_t nheavy, int32_t *cnt,
+            uint64_t *gtab, const int64_t *gtab_off, const T *wtab, const int64_t *colptr,
+            int32_t *rows, T *vals) {
+    constexpr int NS = kBlockTableBytes / (int)sizeof(S);
+    __shared__ S ltab[NS];
+    __shared__ S lkept[kBlockKept];
     __shared__ int64_t sbeg[kMaxK], spre[kMaxK + 1];
     __shared__ int wk[kWaves];
     __shared__ int nkept;
@@ -261,68 +386,25 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
     }
     __syncthreads();
     const int64_t P = spre[kMaxK];
-    uint64_t *tab, *kept;
-    uint32_t nslots;
-    if (P <= kBlockMaxP) {
-        tab = ltab;
-        kept = lkept;
-        nslots = kBlockSlots;
+    if (P <= NS / 2) {
+        const uint32_t T_ = table_size(P);
+        const int L = block_count<S>(ltab, T_, P, k, rev, sbeg, spre, xmin, wk);
+        if (!rows) {
+            if (threadIdx.x == 0) cnt[j] = L;
+            return;
+        }
+        block_emit<S, T>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab, rows + colptr[j],
+                         vals + colptr[j]);
     } else {
-        // global region: nslots table slots followed by nslots/2 sort slots
-        tab = gtab + gtab_off[h];
-        nslots = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
-        kept = tab + nslots;
-    }
-    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) tab[i] = 0;
-    __syncthreads();
-    for (int64_t e = threadIdx.x; e < P; e += kBlock) {
-        // owner = last neighbour slot whose prefix is <= e (deduplicated
-        // slots have zero length and share the next slot's prefix)
-        int lo = 0;
-        for (int t = 1; t < k; ++t)
-            if (spre[t] <= e) lo = t;
-        tab_insert(tab, nslots, (uint32_t)rev[sbeg[lo] + (e - spre[lo])]);
-    }
-    __syncthreads();
-    // count kept entries
-    int nk = 0;
-    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) {
-        const uint64_t v = tab[i];
-        if (v != 0 && (int)(v & 0xFFFFFFFFu) >= xmin) nk++;
-    }
-    int wsum = nk;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const int y = __shfl_up(wsum, o, kWave);
-        if (lane >= o) wsum += y;
-    }
-    if (lane == kWave - 1) wk[w] = wsum;
-    __syncthreads();
-    int L = 0;
-    for (int q = 0; q < kWaves; ++q) L += wk[q];
-    if (!rows) {
-        if (threadIdx.x == 0) cnt[j] = L;
-        return;
-    }
-    // gather the L kept entries (any order), pad to a power of two with the
-    // largest key and sort ascending: rows come out in order
-    if (threadIdx.x == 0) nkept = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) {
-        const uint64_t v = tab[i];
-        if (v != 0 && (int)(v & 0xFFFFFFFFu) >= xmin) kept[atomicAdd(&nkept, 1)] = v;
-    }
-    __syncthreads();
-    int Lp = 1;
-    while (Lp < L) Lp <<= 1;
-    for (int i = L + threadIdx.x; i < Lp; i += kBlock) kept[i] = ~0ull;
-    __syncthreads();
-    block_bitonic(kept, Lp);
-    const int64_t ob = colptr[j];
-    for (int64_t i = threadIdx.x; i < L; i += kBlock) {
-        const uint64_t v = kept[i];
-        rows[ob + i] = (int32_t)((v >> 32) - 1);
-        vals[ob + i] = wtab[v & 0xFFFFFFFFu];
+        uint64_t *tab = gtab + gtab_off[h];
+        const uint32_t T_ = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
+        const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
+        if (!rows) {
+            if (threadIdx.x == 0) cnt[j] = L;
+            return;
+        }
+        block_emit<uint64_t, T>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
+                                rows + colptr[j], vals + colptr[j]);
     }
 }
 
@@ -330,6 +412,47 @@ unsigned grid_for(int64_t n, int per_cu = 8) {
     int64_t want = ceil_div(n > 0 ? n : 1, kBlock);
     int64_t cap = (int64_t)device_cu_count() * per_cu;
     return (unsigned)(want < cap ? want : cap);
+}
+
+// rows below 2^25 (and counts below 128) fit 32-bit slots
+bool small_slots(int64_t n) { return n < ((int64_t)1 << 25) - 1; }
+
+int64_t block_lds_maxP(int64_t n) {
+    return small_slots(n) ? kBlockTableBytes / 4 / 2 : kBlockTableBytes / 8 / 2;
+}
+
+template <class T>
+void launch_wave(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
+                 const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
+    const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
+    if (small_slots(n))
+        hipLaunchKernelGGL((k_snn_wave<T, uint32_t>), dim3(wg), dim3(kBlock), 0, st, nn, n, k,
+                           xmin, w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(),
+                           w.heavyP.get(), w.nheavy.get(), wtab, colptr, count ? nullptr : rows,
+                           vals);
+    else
+        hipLaunchKernelGGL((k_snn_wave<T, uint64_t>), dim3(wg), dim3(kBlock), 0, st, nn, n, k,
+                           xmin, w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(),
+                           w.heavyP.get(), w.nheavy.get(), wtab, colptr, count ? nullptr : rows,
+                           vals);
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_block(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
+                  const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
+    if (w.nheavy_h <= 0) return;
+    if (small_slots(n))
+        hipLaunchKernelGGL((k_snn_block<T, uint32_t>), dim3((unsigned)w.nheavy_h), dim3(kBlock),
+                           0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
+                           w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
+                           count ? nullptr : rows, vals);
+    else
+        hipLaunchKernelGGL((k_snn_block<T, uint64_t>), dim3((unsigned)w.nheavy_h), dim3(kBlock),
+                           0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
+                           w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
+                           count ? nullptr : rows, vals);
+    SEV_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -358,41 +481,40 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
     hipLaunchKernelGGL(k_rev_fill, dim3(grid_for(total)), dim3(kBlock), 0, st, nn, total, n, k,
                        w.rptr.get(), w.cur.get(), w.rev.get());
     SEV_LAUNCH_CHECK();
-    const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
     ProfScope ps("snn_count", st);
-    hipLaunchKernelGGL(k_snn_wave<T>, dim3(wg), dim3(kBlock), 0, st, nn, n, (int)k, xmin,
-                       w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(), w.heavyP.get(),
-                       w.nheavy.get(), (const T *)nullptr, (const int64_t *)nullptr, (int32_t *)nullptr,
-                       (T *)nullptr);
-    SEV_LAUNCH_CHECK();
+    launch_wave<T>(true, nn, n, (int)k, xmin, w, nullptr, nullptr, nullptr, nullptr, st);
     int32_t nh = 0;
     SEV_HIP(hipMemcpyAsync(&nh, w.nheavy.get(), sizeof nh, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
     w.nheavy_h = nh;
     if (nh > 0) {
         // columns above the LDS capacity get a global table of
-        // 2 * next_pow2(P) slots
+        // 2 * next_pow2(P) slots plus a sort buffer of half that
         std::vector<int64_t> hp(nh);
         SEV_HIP(hipMemcpy(hp.data(), w.heavyP.get(), sizeof(int64_t) * nh, hipMemcpyDeviceToHost));
+        const int64_t lds_maxP = block_lds_maxP(n);
         std::vector<int64_t> offs(nh + 1, 0);
+        int64_t big = 0, mx = 0;
         for (int32_t h = 0; h < nh; ++h) {
             int64_t slots = 0;
-            if (hp[h] > kBlockMaxP) {
+            if (hp[h] > lds_maxP) {
                 slots = 1;
                 while (slots < 2 * hp[h]) slots <<= 1;
-                slots += slots / 2;  // table + sort buffer (>= next_pow2(P))
+                slots += slots / 2;
+                big++;
             }
+            mx = std::max(mx, hp[h]);
             offs[h + 1] = offs[h] + slots;
         }
+        if (std::getenv("SEV_VERBOSE"))
+            std::fprintf(stderr, "[severo_hip] snn n=%lld k=%lld: %d block columns (max P %lld, "
+                         "%lld with a global table)\n", (long long)n, (long long)k, nh,
+                         (long long)mx, (long long)big);
         w.htab_off.ensure((size_t)nh + 1);
         SEV_HIP(hipMemcpy(w.htab_off.get(), offs.data(), sizeof(int64_t) * (nh + 1),
                           hipMemcpyHostToDevice));
         w.htab.ensure((size_t)(offs[nh] > 0 ? offs[nh] : 1));
-        hipLaunchKernelGGL(k_snn_block<T>, dim3((unsigned)nh), dim3(kBlock), 0, st, nn, n,
-                           (int)k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(), (int64_t)nh,
-                           w.cnt.get(), w.htab.get(), w.htab_off.get(), (const T *)nullptr,
-                           (const int64_t *)nullptr, (int32_t *)nullptr, (T *)nullptr);
-        SEV_LAUNCH_CHECK();
+        launch_block<T>(true, nn, n, (int)k, xmin, w, nullptr, nullptr, nullptr, nullptr, st);
     }
     scan_exclusive_i32(w.cnt.get(), colptr, n, w.scratch.get(), st);
     int64_t nnz = 0;
@@ -404,18 +526,9 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
 template <class T>
 void snn_fill(const int32_t *nn, int64_t n, int64_t k, int xmin, const T *wtab, SnnWork &w,
               const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
-    const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
     ProfScope ps("snn_fill", st);
-    hipLaunchKernelGGL(k_snn_wave<T>, dim3(wg), dim3(kBlock), 0, st, nn, n, (int)k, xmin,
-                       w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(), w.heavyP.get(),
-                       w.nheavy.get(), wtab, colptr, rows, vals);
-    SEV_LAUNCH_CHECK();
-    if (w.nheavy_h > 0) {
-        hipLaunchKernelGGL(k_snn_block<T>, dim3((unsigned)w.nheavy_h), dim3(kBlock), 0, st, nn,
-                           n, (int)k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(), w.nheavy_h,
-                           w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr, rows, vals);
-        SEV_LAUNCH_CHECK();
-    }
+    launch_wave<T>(false, nn, n, (int)k, xmin, w, wtab, colptr, rows, vals, st);
+    launch_block<T>(false, nn, n, (int)k, xmin, w, wtab, colptr, rows, vals, st);
 }
 
 template int64_t snn_count<float>(const int32_t *, int64_t, int64_t, int, SnnWork &, int64_t *,
