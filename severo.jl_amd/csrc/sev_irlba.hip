@@ -42,7 +42,6 @@ constexpr int kMaxWork = 64;
 constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v, FP64)
 constexpr int64_t kMaxFL32 = 14336;  // the same for the FP32 copy of the BF16 form
 constexpr int kRedChunks = 16;       // first-level chunks of the slab partials
-constexpr int kDotBlocks = 2048;     // blocks of the W' y partial products
 constexpr int64_t kBuildRows = 256;  // row slabs of the CSR build
 
 __device__ __forceinline__ double ldval(const float *v, int64_t p) { return (double)v[p]; }
@@ -329,22 +328,25 @@ k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, cons
 
 // --------------------------------------------------------------------------
 // Reorthogonalisation partial products: part[i*nblk + b] = W_i' y over the
-// rows of block b, for i < jc (columns in chunks of 32, each chunk reduced
-// across the wavefront by the butterfly and across waves in wave order).
+// rows of slab b (kSlabRows rows, one block of kDotThreads), for i < jc:
+// columns in chunks of 32, each chunk reduced across the wavefront by the
+// butterfly and across the block's waves in wave order.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
+constexpr int kDotThreads = 1024;
+
+__global__ void __launch_bounds__(kDotThreads)
 k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
-    __shared__ double red[kWaves][32];
+    constexpr int NW = kDotThreads / kWave;
+    __shared__ double red[NW][32];
     const int64_t nblk = gridDim.x;
-    const int64_t per = (m + nblk - 1) / nblk;
-    const int64_t rb = (int64_t)blockIdx.x * per;
-    const int64_t re = rb + per < m ? rb + per : m;
+    const int64_t rb = (int64_t)blockIdx.x * kSlabRows;
+    const int64_t re = rb + kSlabRows < m ? rb + kSlabRows : m;
     const int w = threadIdx.x / kWave, lane = lane_id();
     for (int c0 = 0; c0 < jc; c0 += 32) {
         double acc[32];
 #pragma unroll
         for (int i = 0; i < 32; ++i) acc[i] = 0.0;
-        for (int64_t r = rb + threadIdx.x; r < re; r += kBlock) {
+        for (int64_t r = rb + threadIdx.x; r < re; r += kDotThreads) {
             const double yr = y[r];
 #pragma unroll
             for (int i = 0; i < 32; ++i)
@@ -355,7 +357,7 @@ k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
         __syncthreads();
         if (threadIdx.x < 32 && c0 + (int)threadIdx.x < jc) {
             double u = 0.0;
-            for (int q = 0; q < kWaves; ++q) u += red[q][threadIdx.x];
+            for (int q = 0; q < NW; ++q) u += red[q][threadIdx.x];
             part[(int64_t)(c0 + threadIdx.x) * nblk + blockIdx.x] = u;
         }
         __syncthreads();
@@ -1271,7 +1273,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     w.T.ensure(kMaxWork);
     w.part.ensure((size_t)(op.nslab * (n + 2)));
     w.part2.ensure((size_t)(kRedChunks * (n + 2)));
-    w.red.ensure((size_t)kDotBlocks * kMaxWork);
+    w.red.ensure((size_t)op.nslab * kMaxWork);
     w.Fv.ensure((size_t)(n + m));  // F (n) followed by the updated y (m)
     w.rpart.ensure((size_t)ceil_div(n, kRsBlock) * (kMaxWork + 1));
     w.scal.ensure(4);
@@ -1339,7 +1341,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         SEV_HIP(hipStreamSynchronize(st));
     };
 
-    const int dot_blocks = (int)std::min<int64_t>(kDotBlocks, ceil_div(m, kBlock));
+    const int dot_blocks = (int)op.nslab;
 
     std::vector<double> Bh((size_t)work * work), BUh((size_t)work * work),
         BVh((size_t)work * work), BSh(work);
@@ -1375,8 +1377,8 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             const int jc = j;
             if (jc > 0) {
                 ProfScope ps("reorth_dot", st, 8.0 * m * (jc + 1));
-                hipLaunchKernelGGL(k_dot_w, dim3(dot_blocks), dim3(kBlock), 0, st, m, Wd, jc, y,
-                                   w.red.get());
+                hipLaunchKernelGGL(k_dot_w, dim3(dot_blocks), dim3(kDotThreads), 0, st, m, Wd, jc,
+                                   y, w.red.get());
                 SEV_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_red_t, dim3(jc), dim3(kBlock), 0, st, dot_blocks,
                                    w.red.get(), w.T.get());
