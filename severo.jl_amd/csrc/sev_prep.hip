@@ -167,12 +167,40 @@ k_gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, co
 // --------------------------------------------------------------------------
 // scale_data.  One wavefront per column runs the reference's sequential
 // Welford recurrence (src/scaling.jl:21-30) in T, every lane computing the
-// same values from a 64-element window loaded with one coalesced read:
+// same values:
 //   count = m - nnz; for v: count += 1; delta = v - mu;
 //   mu += delta / T(count); s += delta * (v - mu)
 // var = s / T(m - 1); sigma = sqrt(var); mu' = mu / sigma;
 // clip = scale_max + mu'.
+// Each chunk of 64 values is loaded with one coalesced read (one per lane)
+// and consumed in order through readlane; the 64 reciprocals of the chunk's
+// counts are computed in parallel by the lanes.
 // --------------------------------------------------------------------------
+__device__ __forceinline__ float lane_value(float x, int t) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), t));
+}
+__device__ __forceinline__ double lane_value(double x, int t) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, t);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), t);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// delta / T(count) rounded to T (the reference divides by count converted
+// to T).  For Float32 the quotient is delta * (1/F), F = Float32(count),
+// evaluated in FP64 and rounded once to FP32: the FP64 value is within 2^-52
+// (relative) of delta/F, while delta/F is at least 2^-49 (relative) from
+// every FP32 rounding boundary -- it cannot be a tie, since an odd 25-bit
+// significand times F's significand is no 24-bit float -- so the rounding is
+// the IEEE FP32 division's.  Subnormal deltas divide directly.
+__device__ __forceinline__ float welford_quot(float delta, double rc, float countf, bool fast) {
+    if (fast && fabsf(delta) >= 0x1p-100f) return (float)((double)delta * rc);
+    return delta / countf;
+}
+__device__ __forceinline__ double welford_quot(double delta, double, double countf, bool) {
+    return delta / countf;
+}
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scale_max, T *sdv,
@@ -182,18 +210,24 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
     const int lane = lane_id();
     const int64_t b = colptr[col], e = colptr[col + 1];
     int64_t count = m - (e - b);
+    const bool fast = sizeof(T) == 4;
     T mu = 0, s = 0;
     for (int64_t base = b; base < e; base += kWave) {
         const int64_t p = base + lane;
         const T vl = p < e ? x[p] : T(0);
+        const double rcl = fast ? 1.0 / (double)(float)(count + lane + 1) : 0.0;
         const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
-        for (int t = 0; t < nt; ++t) {
-            const T v = __shfl(vl, t, kWave);
-            count += 1;
-            const T delta = v - mu;
-            mu = mu + delta / (T)count;
-            s = s + delta * (v - mu);
+#pragma unroll
+        for (int t = 0; t < kWave; ++t) {
+            if (t < nt) {
+                const T v = lane_value(vl, t);
+                const T delta = v - mu;
+                mu = mu + welford_quot(delta, fast ? lane_value(rcl, t) : 0.0,
+                                       (T)(count + t + 1), fast);
+                s = s + delta * (v - mu);
+            }
         }
+        count += nt;
     }
     const T var = s / (T)(m - 1);
     const T sd = sqrt(var);

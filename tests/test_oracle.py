@@ -350,3 +350,22 @@ def test_snn_symmetric_with_unit_diagonal(orc):
     S = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(400, 400))
     assert abs(S - S.T).max() == 0
     assert np.all(S.diagonal() == 1.0)
+
+
+def test_welford_quotient_identity():
+    """The device Welford step computes delta / Float32(count) as
+    Float32(Float64(delta) * (1 / Float64(Float32(count)))); for normal
+    deltas this equals the IEEE Float32 division bit for bit (checked here on
+    random deltas over a wide exponent range and counts up to 2^26, where the
+    Float32 count itself is rounded)."""
+    rng = np.random.default_rng(7)
+    for hi in (2**6, 2**12, 2**20, 2**26 - 1):
+        N = 400_000
+        e = rng.integers(-100, 100, N)
+        d = (np.sign(rng.random(N) - 0.5) * (rng.random(N) + 1.0) *
+             np.exp2(e.astype(np.float64))).astype(np.float32)
+        d[: N // 8] = rng.integers(-2**24, 2**24, N // 8).astype(np.float32)
+        cf = rng.integers(1, hi, N).astype(np.float32)
+        ref = d / cf
+        fast = (d.astype(np.float64) * (1.0 / cf.astype(np.float64))).astype(np.float32)
+        assert np.array_equal(fast.view(np.uint32), ref.view(np.uint32))
