@@ -194,10 +194,10 @@ void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int
 // n x k, 0-based.  *n_fallback = rows recomputed exhaustively.
 struct KnnWork {
     DevBuf<unsigned char> zq, zc;
-    DevBuf<float> zr, nhat, eps, cthr;
+    DevBuf<float> zr, nhat, eps, cthr, pval, pthr;
     DevBuf<unsigned> gmax;
-    DevBuf<int32_t> cidx, ccnt, nfail;
-    DevBuf<int64_t> fail;
+    DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt;
+    DevBuf<int64_t> fail, fail2;
 };
 void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
               int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,

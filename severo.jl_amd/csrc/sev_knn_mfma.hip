@@ -133,10 +133,14 @@ struct Elem<1> {
     typedef uint16_t T;
 };
 
+// Queries: rows q0..q0+QB-1 of the query list (qidx[i], or i itself when
+// qidx is null; nq entries).  Candidates: chunk blockIdx.y of gridDim.y equal
+// chunks of [0, n) (multiples of the tile).  Outputs at [chunk][list index]:
+// cidx (KL ids), cthr, ccnt, and the list values in cval when it is given.
 template <int MODE, int KP, int QW, int KL>
 __global__ void __launch_bounds__(kBlock)
-k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_t *cidx,
-           float *cthr, int32_t *ccnt) {
+k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const int64_t *qidx,
+           int64_t nq, int32_t *cidx, float *cthr, int32_t *ccnt, float *cval) {
     typedef typename Elem<MODE>::T E;
     constexpr int QB = kNW * QW * 16;
     constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
@@ -146,6 +150,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
     __shared__ float listV[QB][KL];
     __shared__ int32_t listI[QB][KL];
     __shared__ int listN[QB];
+    __shared__ int32_t qg[QB];  // query point of each list row (-1: none)
 
     const E *Zq = (const E *)Zq_;
     const E *Zc = (const E *)Zc_;
@@ -158,20 +163,30 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         (&listV[0][0])[e] = ninf();
         (&listI[0][0])[e] = INT_MAX;
     }
-    for (int i = threadIdx.x; i < QB; i += kBlock) listN[i] = 0;
+    for (int i = threadIdx.x; i < QB; i += kBlock) {
+        listN[i] = 0;
+        const int64_t qi = q0 + i;
+        qg[i] = qi < nq ? (int32_t)(qidx ? qidx[qi] : qi) : -1;
+    }
+    // candidate chunk
+    const int64_t csz = ((n + gridDim.y - 1) / gridDim.y + kCT - 1) / kCT * kCT;
+    const int64_t cbeg = (int64_t)blockIdx.y * csz;
+    const int64_t cend = cbeg + csz < n ? cbeg + csz : n;
     // query operands and the running thresholds of this lane's output rows
     // (rows 4*grp + i of each row block) in registers; rows past n never
     // accept a candidate
     float af[QW][MODE == 0 ? KS : 1];
     v8bf ab[QW][MODE == 1 ? KS : 1];
     float thr[QW][4];
+    __syncthreads();
 #pragma unroll
     for (int rb = 0; rb < QW; ++rb) {
-        const int64_t qrow = q0 + w * QW * 16 + rb * 16 + col;
-        const bool ok = qrow < n;
+        const int32_t qpt = qg[w * QW * 16 + rb * 16 + col];
+        const bool ok = qpt >= 0;
+        const int64_t qrow = ok ? qpt : 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            thr[rb][i] = q0 + w * QW * 16 + rb * 16 + 4 * grp + i < n ? ninf()
+            thr[rb][i] = qg[w * QW * 16 + rb * 16 + 4 * grp + i] >= 0 ? ninf()
                                                                        : __builtin_huge_valf();
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
@@ -205,7 +220,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
             const int r = e / CH, part = e % CH;
             const int64_t c = t0 + r;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (e < kCT * CH && c < n)
+            if (e < kCT * CH && c < cend)
                 v = *(const uint4 *)((const E *)Zc + c * KP + part * (MODE == 0 ? 4 : 8));
             stage[u] = v;
         }
@@ -227,23 +242,23 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
             }
         }
     };
-    fetch(0);
+    fetch(cbeg);
     deposit(tiles[0]);
     // every global load issued so far (query operands, first tile) is
     // complete here, so the loop only waits for its own prefetches
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int buf = 0;
-    for (int64_t t0 = 0; t0 < n; t0 += kCT, buf ^= 1) {
+    for (int64_t t0 = cbeg; t0 < cend; t0 += kCT, buf ^= 1) {
         const E *tile = tiles[buf];
-        const bool more = t0 + kCT < n;
+        const bool more = t0 + kCT < cend;
         if (more) fetch(t0 + kCT);
-        // the tile holds padding columns, or this block's own queries when
-        // self pairs are excluded: such pairs are rejected in the insertion
+        // the tile holds padding columns, or (self pairs excluded) may hold
+        // this block's own queries: such pairs are rejected in the insertion
         // path (their products may pass the threshold test)
-        const bool edge = t0 + kCT > n ||
-                          (!include_self && t0 < q0 + QB && q0 < t0 + kCT);
-        const int t32 = (int)t0, n32 = (int)n, q32 = (int)q0;
+        const bool edge = t0 + kCT > cend ||
+                          (!include_self && (qidx || (t0 < q0 + QB && q0 < t0 + kCT)));
+        const int t32 = (int)t0, n32 = (int)cend;
         // all products of the tile: acc[cb][rb] = 16 queries x 16 candidates
         v4f acc[NCB][QW];
 #pragma unroll
@@ -305,7 +320,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
                         if (!(v > ts)) continue;
                         const int qlo = w * QW * 16 + rb * 16 + 4 * g + i;
                         const int32_t cand = t32 + cb * 16 + (src & 15);
-                        if (edge && (cand >= n32 || (!include_self && cand == q32 + qlo))) continue;
+                        if (edge && (cand >= n32 || (!include_self && cand == qg[qlo]))) continue;
                         // insert (v, cand) into the descending list of qlo:
                         // entries from pos on move one slot up by LDS stores
                         const int cnt = listN[qlo];
@@ -337,13 +352,17 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
         __syncthreads();
     }
     for (int ql = w; ql < QB; ql += kNW) {
-        const int64_t q = q0 + ql;
-        if (q >= n) continue;
+        const int64_t qi = q0 + ql;
+        if (qi >= nq) continue;
+        const int64_t o = (int64_t)blockIdx.y * nq + qi;
         const int cnt = listN[ql];
-        if (lane < KL) cidx[q * KL + lane] = lane < cnt ? listI[ql][lane] : -1;
+        if (lane < KL) {
+            cidx[o * KL + lane] = lane < cnt ? listI[ql][lane] : -1;
+            if (cval) cval[o * KL + lane] = listV[ql][lane];
+        }
         if (lane == 0) {
-            cthr[q] = cnt == KL ? listV[ql][KL - 1] : ninf();
-            ccnt[q] = cnt;
+            cthr[o] = cnt == KL ? listV[ql][KL - 1] : ninf();
+            ccnt[o] = cnt;
         }
     }
 }
@@ -351,17 +370,74 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, int32_
 // ---------------------------------------------------------------------------
 // Exact re-rank and certificate (one wavefront per query).
 // ---------------------------------------------------------------------------
+// Merge the per-chunk candidate lists of each query (one wavefront per list
+// index): the running top KL (lanes 0..KL-1) and the next chunk's list
+// (lanes KL..2KL-1) are sorted together by (value desc, id asc) and the first
+// KL kept.  The merged threshold is the KL-th value when KL candidates were
+// kept over all chunks.  Every candidate outside the merged list lies below
+// its chunk's threshold, which is at most the merged one.
 template <int KL>
 __global__ void __launch_bounds__(kBlock)
-k_knn_rerank(const float *Zr, int64_t n, int d, int k, const int32_t *cidx, const float *cthr,
-             const int32_t *ccnt, const float *nhat, const float *eps, const unsigned *gmax,
-             double eta, double spec_factor, int32_t *oidx, float *odist, int32_t *nfail,
-             int64_t *fail) {
+k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, const int32_t *pcnt,
+            int32_t *cidx, float *cthr, int32_t *ccnt) {
+    static_assert(2 * KL <= kWave, "two lists must fit a wavefront");
     const int lane = lane_id();
-    const int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
-    if (q >= n) return;
-    const int cnt = ccnt[q];
-    const int32_t c = lane < cnt && lane < KL ? cidx[q * KL + lane] : -1;
+    const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (qi >= nq) return;
+    float v = ninf();
+    int32_t id = INT_MAX;
+    int tot = 0;
+    for (int c = 0; c < nchunk; ++c) {
+        const int64_t o = (int64_t)c * nq + qi;
+        const int cnt = pcnt[o];
+        tot += cnt;
+        if (lane >= KL && lane < 2 * KL) {
+            const int l = lane - KL;
+            v = l < cnt ? pval[o * KL + l] : ninf();
+            id = l < cnt ? pidx[o * KL + l] : INT_MAX;
+        } else if (lane >= 2 * KL) {
+            v = ninf();
+            id = INT_MAX;
+        }
+        // bitonic sort of the 64 lanes, descending by value, ascending by id
+        for (int size = 2; size <= kWave; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const float ov = __shfl_xor(v, stride, kWave);
+                const int32_t oi = __shfl_xor(id, stride, kWave);
+                const bool other_first = ov > v || (ov == v && oi < id);
+                const bool mine_first = v > ov || (v == ov && id < oi);
+                const bool up = (lane & size) == 0;
+                const bool lower = (lane & stride) == 0;
+                if ((lower == up) ? other_first : mine_first) {
+                    v = ov;
+                    id = oi;
+                }
+            }
+        }
+    }
+    const int cnt = tot < KL ? tot : KL;
+    if (lane < KL) cidx[qi * KL + lane] = lane < cnt ? id : -1;
+    const float last = __shfl(v, KL - 1, kWave);
+    if (lane == 0) {
+        ccnt[qi] = cnt;
+        cthr[qi] = tot >= KL ? last : ninf();
+    }
+}
+
+// Queries: list index i maps to the point qidx[i] (or i when qidx is null);
+// the candidate list of i is row i of cidx/cthr/ccnt.
+template <int KL>
+__global__ void __launch_bounds__(kBlock)
+k_knn_rerank(const float *Zr, int64_t nq, const int64_t *qidx, int d, int k, const int32_t *cidx,
+             const float *cthr, const int32_t *ccnt, const float *nhat, const float *eps,
+             const unsigned *gmax, double eta, double spec_factor, int32_t *oidx, float *odist,
+             int32_t *nfail, int64_t *fail) {
+    const int lane = lane_id();
+    const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (qi >= nq) return;
+    const int64_t q = qidx ? qidx[qi] : qi;
+    const int cnt = ccnt[qi];
+    const int32_t c = lane < cnt && lane < KL ? cidx[qi * KL + lane] : -1;
     float D = __builtin_huge_valf();
     if (c >= 0) {
         const float *zq = Zr + q * d;
@@ -397,9 +473,9 @@ k_knn_rerank(const float *Zr, int64_t n, int d, int k, const int32_t *cidx, cons
     } else {
         const double ncmax = (double)__uint_as_float(gmax[0]);
         const double emax = (double)__uint_as_float(gmax[1]);
-        const double nq = (double)nhat[q];
-        const double dt = nq - 2.0 * (double)cthr[q];
-        const double lb2 = dt * (1.0 - 0x1p-22) - eta * (nq + ncmax) - 1e-30;
+        const double nq_ = (double)nhat[q];
+        const double dt = nq_ - 2.0 * (double)cthr[qi];
+        const double lb2 = dt * (1.0 - 0x1p-22) - eta * (nq_ + ncmax) - 1e-30;
         const double lb = lb2 > 0.0 ? sqrt(lb2) - (double)eps[q] - emax : -1.0;
         ok = lb > 0.0 && lb * lb * spec_factor > (double)Dk;
     }
@@ -470,9 +546,12 @@ double mfma_selftest(int mode) {
 
 namespace {
 
+// Candidate search + merge (chunked) + re-rank for a query list.  Rows that
+// fail the certificate are appended to w.fail (count in w.nfail).
 template <int MODE, int KP, int QW, int KL>
-void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
-              KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback) {
+void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                 const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
+                 int32_t *nfail, int64_t *fail, hipStream_t st) {
     const size_t es = MODE == 0 ? 4 : 2;
     w.zq.ensure((size_t)n * KP * es);
     w.zc.ensure((size_t)n * KP * es);
@@ -480,13 +559,7 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     w.nhat.ensure((size_t)n);
     w.eps.ensure((size_t)n);
     w.gmax.ensure(2);
-    w.cidx.ensure((size_t)n * KL);
-    w.cthr.ensure((size_t)n);
-    w.ccnt.ensure((size_t)n);
-    w.nfail.ensure(1);
-    w.fail.ensure((size_t)n);
     SEV_HIP(hipMemsetAsync(w.gmax.get(), 0, 2 * sizeof(unsigned), st));
-    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, sizeof(int32_t), st));
     {
         ProfScope ps("knn_prep", st, 4.0 * n * d * 2 + 2.0 * n * KP * es);
         const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
@@ -495,30 +568,80 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
                            w.eps.get(), w.gmax.get());
         SEV_LAUNCH_CHECK();
     }
+    constexpr int QB = kNW * QW * 16;
+    const int64_t qblocks = ceil_div(nq, QB);
+    // split the candidates when the query blocks alone cannot fill the device
+    int nchunk = 1;
+    const int64_t want = (int64_t)device_cu_count() * 8;
+    if (2 * KL <= kWave && qblocks < want)
+        nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, want / qblocks));
+    nchunk = (int)std::min<int64_t>(nchunk, std::max<int64_t>(1, n / (4 * kCT)));
+    w.cidx.ensure((size_t)nq * KL);
+    w.cthr.ensure((size_t)nq);
+    w.ccnt.ensure((size_t)nq);
     {
-        constexpr int QB = kNW * QW * 16;
         ProfScope ps(MODE == 0 ? "knn_cand_fp32" : "knn_cand_bf16", st, 0.0,
-                     2.0 * (double)n * (double)n * (double)d);
-        hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)ceil_div(n, QB)),
-                           dim3(kBlock), 0, st, (const void *)w.zq.get(), (const void *)w.zc.get(),
-                           n, include_self, w.cidx.get(), w.cthr.get(), w.ccnt.get());
-        SEV_LAUNCH_CHECK();
+                     2.0 * (double)nq * (double)n * (double)d);
+        if (nchunk == 1) {
+            hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks),
+                               dim3(kBlock), 0, st, (const void *)w.zq.get(),
+                               (const void *)w.zc.get(), n, include_self, qidx, nq, w.cidx.get(),
+                               w.cthr.get(), w.ccnt.get(), (float *)nullptr);
+            SEV_LAUNCH_CHECK();
+        } else if constexpr (2 * KL <= kWave) {
+            w.pidx.ensure((size_t)nchunk * nq * KL);
+            w.pval.ensure((size_t)nchunk * nq * KL);
+            w.pthr.ensure((size_t)nchunk * nq);
+            w.pcnt.ensure((size_t)nchunk * nq);
+            hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks, nchunk),
+                               dim3(kBlock), 0, st, (const void *)w.zq.get(),
+                               (const void *)w.zc.get(), n, include_self, qidx, nq, w.pidx.get(),
+                               w.pthr.get(), w.pcnt.get(), w.pval.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL((k_knn_merge<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
+                               dim3(kBlock), 0, st, nq, nchunk, w.pidx.get(), w.pval.get(),
+                               w.pcnt.get(), w.cidx.get(), w.cthr.get(), w.ccnt.get());
+            SEV_LAUNCH_CHECK();
+        }
     }
     {
         const double eta = 4.0 * (KP + 2) * 0x1p-24 + (MODE == 0 ? 0x1p-22 : 0x1p-14);
         const double spec = pow(1.0 - 0x1p-24, (double)(d + 3));
-        ProfScope ps("knn_rerank", st, (double)n * KL * (4.0 + 4.0 * d));
-        hipLaunchKernelGGL((k_knn_rerank<KL>), dim3((unsigned)ceil_div(n * kWave, kBlock)),
-                           dim3(kBlock), 0, st, w.zr.get(), n, (int)d, (int)k, w.cidx.get(),
-                           w.cthr.get(), w.ccnt.get(), w.nhat.get(), w.eps.get(), w.gmax.get(),
-                           eta, spec, idx, dist, w.nfail.get(), w.fail.get());
+        ProfScope ps("knn_rerank", st, (double)nq * KL * (4.0 + 4.0 * d));
+        hipLaunchKernelGGL((k_knn_rerank<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
+                           dim3(kBlock), 0, st, w.zr.get(), nq, qidx, (int)d, (int)k,
+                           w.cidx.get(), w.cthr.get(), w.ccnt.get(), w.nhat.get(), w.eps.get(),
+                           w.gmax.get(), eta, spec, idx, dist, nfail, fail);
         SEV_LAUNCH_CHECK();
     }
+}
+
+template <int MODE, int KP, int QW, int KL, int KP32>
+void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+              KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback) {
+    w.nfail.ensure(2);
+    w.fail.ensure((size_t)n);
+    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 2 * sizeof(int32_t), st));
+    cand_rerank<MODE, KP, QW, KL>(Z, n, d, ld, k, include_self, nullptr, n, w, idx, dist,
+                                  w.nfail.get(), w.fail.get(), st);
     int32_t nf = 0;
     SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
+    const int64_t *left = w.fail.get();
+    if constexpr (MODE == 1 && KP32 > 0) {
+        if (nf > 0) {
+            // BF16 rows without a certificate: FP32 candidates for those rows
+            w.fail2.ensure((size_t)nf);
+            SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
+                                   hipMemcpyDeviceToDevice, st));
+            cand_rerank<0, KP32, 2, KL>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
+                                        dist, w.nfail.get() + 1, w.fail.get(), st);
+            SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 1, sizeof nf, hipMemcpyDeviceToHost, st));
+            SEV_HIP(hipStreamSynchronize(st));
+        }
+    }
     if (nf > 0)
-        knn_exact_rows<float>(Z, n, d, ld, k, include_self, w.fail.get(), nf, idx, dist, st);
+        knn_exact_rows<float>(Z, n, d, ld, k, include_self, left, nf, idx, dist, st);
     if (n_fallback) *n_fallback = nf;
 }
 
@@ -532,30 +655,31 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     SEV_REQUIRE(n < ((int64_t)1 << 31) - 256, "knn: at most 2^31 - 256 points");
     if (bf16) {
         SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
-        static const int qw = std::getenv("SEV_KNN_QW") ? std::atoi(std::getenv("SEV_KNN_QW")) : 2;
-        if (k <= 24) {
-            if (d + 2 <= 64 && qw == 4)
-                run_mfma<1, 64, 4, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
-            else if (d + 2 <= 64)
-                run_mfma<1, 64, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
-            else
-                run_mfma<1, 128, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
-        } else {
-            if (d + 2 <= 64)
-                run_mfma<1, 64, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
-            else
-                run_mfma<1, 128, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
-        }
+        if (k <= 24 && d + 2 <= 64 && d + 1 <= 52)
+            run_mfma<1, 64, 2, 32, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        else if (k <= 24 && d + 2 <= 64)
+            run_mfma<1, 64, 2, 32, 128>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                        n_fallback);
+        else if (k <= 24)
+            run_mfma<1, 128, 2, 32, 128>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                         n_fallback);
+        else if (d + 2 <= 64)
+            run_mfma<1, 64, 1, 64, 0>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        else
+            run_mfma<1, 128, 1, 64, 0>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
     } else {
         SEV_REQUIRE(d + 1 <= 128, "knn: FP32 MFMA form supports d <= 127");
         if (d + 1 <= 52) {
             if (k <= 24)
-                run_mfma<0, 52, 2, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+                run_mfma<0, 52, 2, 32, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                           n_fallback);
             else
-                run_mfma<0, 52, 1, 64>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+                run_mfma<0, 52, 1, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                           n_fallback);
         } else {
             if (k <= 24) {
-                run_mfma<0, 128, 1, 32>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+                run_mfma<0, 128, 1, 32, 128>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                             n_fallback);
             } else {
                 // wide rows with long lists exceed the LDS budget: exhaustive
                 knn_exact<float>(Z, n, d, ld, k, include_self, nullptr, n, idx, dist, st);
