@@ -103,6 +103,57 @@ def roofline(prof, form, steps, workload):
             "launches_per_step": launches // steps, "avg_launch_ms": round(ms / launches, 4)}
 
 
+def end_to_end(sev, p, feats, init, form, m):
+    """One step with the counts handed over from host memory in Julia's
+    layout (1-based Int64 colptr/rowval, Int32 values, as a ccall passes
+    them) and the SNN graph copied back: the PCIe-inclusive rate."""
+    mm, nn, colptr, rowval, vals = p.get_counts_julia()
+    p.synchronize()
+    t0 = time.perf_counter()
+    p.load_counts_julia(mm, nn, colptr, rowval, vals)
+    p.synchronize()
+    t1 = time.perf_counter()
+    run_step(p, feats, init, form)
+    S = p.get_snn(m)
+    t2 = time.perf_counter()
+    del colptr, rowval, vals
+    return {"cells_per_s": round(m / (t2 - t0), 2), "ms": round((t2 - t0) * 1e3, 1),
+            "h2d_counts_ms": round((t1 - t0) * 1e3, 1), "snn_nnz": int(S.nnz), "form": form}
+
+
+def fp64_errors(sev, p, npcs):
+    """Singular values and right singular vectors of the pipeline's centered
+    scaled matrix against an FP64 reference: the centered Gram matrix
+    B'B - mu q' - q mu' + m mu mu' (q = column sums) accumulated in FP64 on
+    the host from densified row blocks (BLAS) and eigendecomposed in FP64 --
+    no Lanczos involved.  Returns a function of (s, V) giving the errors."""
+    B, mu = p.get_scaled()
+    m, n = B.shape
+    Bc = B.tocsr()
+    del B
+    G = np.zeros((n, n))
+    q = np.zeros(n)
+    step = 65536
+    for r0 in range(0, m, step):
+        blk = Bc[r0:r0 + step].astype(np.float64).toarray()
+        G += blk.T @ blk
+        q += blk.sum(axis=0)
+        del blk
+    mu = mu.astype(np.float64)
+    G = G - np.outer(mu, q) - np.outer(q, mu) + m * np.outer(mu, mu)
+    w, V = np.linalg.eigh(G)
+    sg = np.sqrt(np.maximum(w[::-1][:npcs], 0))
+    Vg = V[:, ::-1][:, :npcs]
+
+    def errors(s, Vf):
+        rel = np.abs(s[:npcs] - sg) / sg
+        qa, _ = np.linalg.qr(Vf[:, :npcs])
+        cos = np.linalg.svd(qa.T @ Vg, compute_uv=False)
+        return {"sv_max_rel": float(rel.max()), "sv_mean_rel": float(rel.mean()),
+                "subspace_angle_rad": float(np.arccos(np.clip(cos.min(), -1, 1)))}
+    return errors
+
+
 def cpu_baseline(sev, orc, w, sample_cells):
     """The oracle's scalar C restatement of the same path on a bounded
     sample of the workload (same generator tables, first `sample_cells`
@@ -134,6 +185,10 @@ def main():
     ap.add_argument("--forms", default="bf16,fp32")
     ap.add_argument("--cpu-sample", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive step")
+    ap.add_argument("--errors", action="store_true",
+                    help="report singular-value and subspace errors against an FP64 Gram "
+                         "eigendecomposition (host BLAS; adds about a minute at 1M cells)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the path "
@@ -160,9 +215,19 @@ def main():
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
+    errs = {}
+    err_fn = None
     for form in forms:
         dt, info, prof = time_form(p, feats, init, form, args.steps, args.warmup, True)
         res[form] = (dt, info, prof)
+        if args.errors:
+            try:
+                if err_fn is None:
+                    err_fn = fp64_errors(sev, p, NPCS)
+                s_, _, V_, _ = p.get_pca()
+                errs[form] = err_fn(s_, V_)
+            except Exception as e:  # the report must not hide the timing
+                errs[form] = {"error": str(e)[:200]}
         print(f"# {form}: {dt * 1e3:.1f} ms/step, {m / dt:.0f} cells/s, {info}", file=sys.stderr,
               flush=True)
         for name, (l, ms, by, fl) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
@@ -195,6 +260,11 @@ def main():
     }
     if "fp32" in res and "bf16" in res:
         out["bf16_speedup_vs_fp32"] = round(res["fp32"][0] / res["bf16"][0], 3)
+    if errs:
+        out["errors_vs_fp64"] = errs
+    if not args.no_e2e:
+        out["end_to_end"] = end_to_end(sev, p, feats, init, head, m)
+        print(f"# end to end ({head}): {out['end_to_end']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         orc = entry.load_oracle()
         out["cpu_baseline"] = cpu_baseline(sev, orc, w, args.cpu_sample)

@@ -84,19 +84,33 @@ void csc_to_device(int64_t m, int64_t n, const int64_t *colptr, const int64_t *r
         if (j > 0) SEV_REQUIRE(cp[(size_t)j] >= cp[(size_t)j - 1], "colptr not nondecreasing");
     }
     nnz = cp[(size_t)n];
-    std::vector<int32_t> rv((size_t)nnz);
-    for (int64_t p = 0; p < nnz; ++p) {
-        const int64_t r = rowval[p] - 1;
-        SEV_REQUIRE(r >= 0 && r < m, "row index out of range");
-        rv[(size_t)p] = (int32_t)r;
-    }
+    SEV_REQUIRE(nnz == 0 || rowval != nullptr, "rowval is NULL");
     dcp.ensure((size_t)n + 1);
     drow.ensure((size_t)nnz);
     SEV_HIP(hipMemcpyAsync(dcp.get(), cp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
                            st));
-    if (nnz)
-        SEV_HIP(hipMemcpyAsync(drow.get(), rv.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice,
-                               st));
+    // the 1-based Int64 rows go up in chunks through two device staging
+    // buffers and are converted (and range-checked) on the device
+    if (nnz) {
+        const int64_t chunk = std::min<int64_t>(nnz, (int64_t)1 << 25);  // 256 MB of Int64
+        DevBuf<int64_t> stage[2];
+        DevBuf<int32_t> bad;
+        stage[0].ensure((size_t)chunk);
+        stage[1].ensure((size_t)std::min<int64_t>(chunk, std::max<int64_t>(nnz - chunk, 1)));
+        bad.ensure(1);
+        SEV_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+        int u = 0;
+        for (int64_t p0 = 0; p0 < nnz; p0 += chunk, u ^= 1) {
+            const int64_t len = std::min<int64_t>(chunk, nnz - p0);
+            SEV_HIP(hipMemcpyAsync(stage[u].get(), rowval + p0, sizeof(int64_t) * len,
+                                   hipMemcpyHostToDevice, st));
+            rows_to_device(stage[u].get(), len, m, drow.get() + p0, bad.get(), st);
+        }
+        int32_t bad_h = 0;
+        SEV_HIP(hipMemcpyAsync(&bad_h, bad.get(), sizeof bad_h, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        SEV_REQUIRE(bad_h == 0, "row index out of range");
+    }
     SEV_HIP(hipStreamSynchronize(st));
 }
 

@@ -19,6 +19,11 @@ void scan_exclusive_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *scr
 void scan_exclusive_i32(const int32_t *in, int64_t *out, int64_t n, int64_t *scratch,
                         hipStream_t st);
 
+// out[p] = in[p] - 1 for 1-based Int64 rows; *bad is set when a row falls
+// outside [1, m].
+void rows_to_device(const int64_t *in, int64_t len, int64_t m, int32_t *out, int32_t *bad,
+                    hipStream_t st);
+
 // ---- sev_prep.hip --------------------------------------------------------
 // Synthetic NB count matrix, cells x genes CSC.  Two calls: counts only
 // (rowval == nullptr) fills colptr and returns nnz, then the fill.

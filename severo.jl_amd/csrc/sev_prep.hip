@@ -429,4 +429,27 @@ double hbm_copy_probe() {
     return 2.0 * (double)bytes / (best * 1e-3) / 1e9;
 }
 
+namespace {
+__global__ void __launch_bounds__(kBlock)
+k_rows_to_device(const int64_t *in, int64_t len, int64_t m, int32_t *out, int32_t *bad) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < len;
+         p += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = in[p] - 1;
+        if (r < 0 || r >= m) {
+            atomicOr(bad, 1);
+            out[p] = 0;
+        } else {
+            out[p] = (int32_t)r;
+        }
+    }
+}
+}  // namespace
+
+void rows_to_device(const int64_t *in, int64_t len, int64_t m, int32_t *out, int32_t *bad,
+                    hipStream_t st) {
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(len, kBlock), device_cu_count() * 16);
+    hipLaunchKernelGGL(k_rows_to_device, dim3(g), dim3(kBlock), 0, st, in, len, m, out, bad);
+    SEV_LAUNCH_CHECK();
+}
+
 }  // namespace sev

@@ -79,6 +79,25 @@ class Pipeline:
         check(self._L.sev_pipeline_load_counts(self._h, X.shape[0], X.shape[1], _ptr(colptr),
                                                _ptr(rowval), _ptr(vals)))
 
+    def load_counts_julia(self, m, n, colptr, rowval, nzval):
+        """Upload counts given in Julia's layout as they are: 1-based Int64
+        colptr (n + 1) and rowval (nnz), Int32 nzval (nnz)."""
+        for a, dt in ((colptr, np.int64), (rowval, np.int64), (nzval, np.int32)):
+            if a.dtype != dt or not a.flags.c_contiguous:
+                raise TypeError("load_counts_julia needs contiguous int64/int64/int32 arrays")
+        check(self._L.sev_pipeline_load_counts(self._h, m, n, _ptr(colptr), _ptr(rowval),
+                                               _ptr(nzval)))
+
+    def get_counts_julia(self):
+        """The device counts in Julia's layout: (m, n, colptr, rowval, nzval),
+        1-based Int64 structure and Int32 values."""
+        m, n, nnz = self.counts_info()
+        colptr = np.zeros(n + 1, np.int64)
+        rowval = np.zeros(nnz, np.int64)
+        vals = np.zeros(nnz, np.int32)
+        check(self._L.sev_pipeline_get_counts(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals)))
+        return m, n, colptr, rowval, vals
+
     def synth_counts(self, n_cells, tables, seed=None):
         seed = tables.seed if seed is None else seed
         check(self._L.sev_pipeline_synth_counts(self._h, n_cells, tables.n_genes, seed, tables.r,
