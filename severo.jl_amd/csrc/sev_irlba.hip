@@ -878,48 +878,65 @@ k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status
 // [diag(s_0..s_k-1) res_0..k-1 in column k].  info: [0] converged,
 // [1] k, [2] ||F|| <= 0 (cannot restart), [3] nconv, [4] SVD needs the host.
 // --------------------------------------------------------------------------
-__global__ void k_restart_check(int work, int nu, double tol, double svtol, const double *BU,
-                                const double *BS, const double *scal, const int *status,
-                                double *svr, double *B, int *info, int last) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    info[5] += status[1];
-    if (status[0]) {
-        info[4] = 1;
-        return;
+__global__ void __launch_bounds__(kBlock)
+k_restart_check(int work, int nu, double tol, double svtol, const double *BU, const double *BS,
+                const double *scal, const int *status, double *svr, double *B, int *info,
+                int last) {
+    __shared__ int s_go;  // rebuild B for a restart
+    __shared__ int s_k;
+    __shared__ double s_rf;
+    const int tid = threadIdx.x, lane = lane_id();
+    if (tid < kWave) {
+        const int st = status[0];
+        const double eps = 2.220446049250313e-16;
+        double RF = scal[2];
+        const double RFn = RF;
+        if (RF < 2 * eps) RF = 0.0;
+        double bs = lane < work ? BS[lane] : 0.0;
+        double smax = bs;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) smax = fmax(smax, __shfl_xor(smax, o, kWave));
+        bool conv = false;
+        if (lane < work) {
+            const double r = RF * BU[(int64_t)lane * work + (work - 1)];
+            const double ratio = fabs(svr[lane] - bs) / bs;
+            conv = fabs(r) < tol * smax && ratio < svtol;
+        }
+        const int nconv = __popcll(__ballot(conv));
+        int go = 0, k = info[1];
+        if (!st && nconv < nu && !last) {
+            go = 1;
+            if (k < nu + nconv) k = nu + nconv;
+            if (k > work - 3) k = work - 3;
+            if (k < 1) k = 1;
+        }
+        if (lane == 0) {
+            info[5] += status[1];
+            info[4] = st ? 1 : 0;
+            if (!st) {
+                info[3] = nconv;
+                info[2] = RFn <= 0.0 ? 1 : 0;
+                info[0] = nconv >= nu ? 1 : 0;
+                info[1] = k;
+            }
+            s_go = go;
+            s_k = k;
+            s_rf = RF;
+        }
     }
-    info[4] = 0;
-    const double eps = 2.220446049250313e-16;
-    double RF = scal[2];
-    const double RFn = RF;
-    if (RF < 2 * eps) RF = 0.0;
-    double smax = 0.0;
-    for (int i = 0; i < work; ++i)
-        if (BS[i] > smax) smax = BS[i];
-    int nconv = 0;
-    for (int i = 0; i < work; ++i) {
-        const double r = RF * BU[(int64_t)i * work + (work - 1)];
-        const double ratio = fabs(svr[i] - BS[i]) / BS[i];
-        if (fabs(r) < tol * smax && ratio < svtol) nconv++;
-    }
-    info[3] = nconv;
-    info[2] = RFn <= 0.0 ? 1 : 0;
-    if (nconv >= nu) {
-        info[0] = 1;
-        return;
-    }
-    info[0] = 0;
-    if (last) return;
-    int k = info[1];
-    if (k < nu + nconv) k = nu + nconv;
-    if (k > work - 3) k = work - 3;
-    if (k < 1) k = 1;
-    info[1] = k;
-    for (int e = 0; e < work * work; ++e) B[e] = 0.0;
-    for (int c = 0; c < k; ++c) {
+    __syncthreads();
+    if (!s_go) return;
+    const int k = s_k;
+    const double RF = s_rf;
+    // B = [diag(s_0..s_k-1), residuals in column k]; Ritz values kept for
+    // the next convergence test
+    for (int e = tid; e < work * work; e += kBlock) B[e] = 0.0;
+    __syncthreads();
+    for (int c = tid; c < k; c += kBlock) {
         B[(int64_t)c * work + c] = BS[c];
         B[(int64_t)k * work + c] = RF * BU[(int64_t)c * work + (work - 1)];
     }
-    for (int i = 0; i < work; ++i) svr[i] = BS[i];
+    for (int i = tid; i < work; i += kBlock) svr[i] = BS[i];
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1425,7 +1442,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             SEV_LAUNCH_CHECK();
         }
         const int last = iter + 1 >= p.maxit ? 1 : 0;
-        hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(64), 0, st, work, (int)nu, p.tol,
+        hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(kBlock), 0, st, work, (int)nu, p.tol,
                            p.svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.info.get() + 8,
                            w.svr.get(), w.B.get(), w.info.get(), last);
         SEV_LAUNCH_CHECK();
@@ -1447,7 +1464,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
             SEV_HIP(hipMemcpyAsync(w.BS.get(), BSh.data(), sizeof(double) * work,
                                    hipMemcpyHostToDevice, st));
             SEV_HIP(hipMemsetAsync(w.info.get() + 8, 0, sizeof(int), st));
-            hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(64), 0, st, work, (int)nu, p.tol,
+            hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(kBlock), 0, st, work, (int)nu, p.tol,
                                p.svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.info.get() + 8,
                                w.svr.get(), w.B.get(), w.info.get(), last);
             SEV_LAUNCH_CHECK();
