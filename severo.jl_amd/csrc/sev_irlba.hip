@@ -749,6 +749,22 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
 // --------------------------------------------------------------------------
 constexpr int kSvdThreads = 512;  // 32 groups of 16 lanes: one pair each
 
+// 1/x and 1/sqrt(x) from the hardware approximations with two Newton steps
+// (relative error at the FP64 rounding level).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
 // DPP lane exchange of a double within rows of 16 lanes.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
@@ -819,11 +835,19 @@ k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status
             apq = sum16(apq);
             if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
                 if (gl == 0) rotated = 1;
-                const double zeta = (aqq - app) / (2.0 * apq);
+                // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2),
+                // with Newton-refined hardware reciprocals (the rotation
+                // needs c^2 + s^2 = 1 to working precision, not rounding)
+                const double zeta = (aqq - app) * rcp_nr(2.0 * apq);
                 const double az = fabs(zeta);
-                const double t = az > 1e150 ? 0.5 / zeta
-                                            : copysign(1.0, zeta) / (az + sqrt(fma(az, az, 1.0)));
-                const double c = 1.0 / sqrt(fma(t, t, 1.0));
+                double t;
+                if (az > 1e150) {
+                    t = 0.5 * rcp_nr(zeta);
+                } else {
+                    const double x = fma(az, az, 1.0);
+                    t = copysign(rcp_nr(az + x * rsq_nr(x)), zeta);
+                }
+                const double c = rsq_nr(fma(t, t, 1.0));
                 const double sn = c * t;
                 double *jp = J + p * n, *jq = J + q * n;
 #pragma unroll
