@@ -167,6 +167,20 @@ int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_
  * q[2g+class(c)] and size r; class(c) = 1 for a boost_frac share of cells. */
 int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t seed, double r,
                               const double *p0, const double *q, double boost_frac);
+/* Batches (the reference runs its path once per sample, e.g. per dataset of
+ * an integration, src/embedding.jl:46-94 and src/neighbours.jl:272-278 per
+ * call): S samples share the gene columns and are stacked by rows (cells of
+ * sample s are rows row0[s] .. row0[s+1]-1, row0 = prefix sums of cells).
+ * sev_pipeline_synth_batch generates sample s like sev_pipeline_synth_counts
+ * (seeds[s], r[s], p0/q rows s of S x 2n, boost_frac[s]) and stacks them;
+ * sev_pipeline_set_batch declares loaded counts to be such a stack (S = 0
+ * clears it).  Normalisation is per cell and needs nothing more; the stages
+ * below have batch forms that keep every sample separate, and the SNN of a
+ * batch's neighbours is the block-diagonal union of the per-sample graphs. */
+int sev_pipeline_synth_batch(sev_pipeline *p, int64_t S, const int64_t *cells, int64_t n,
+                             const uint64_t *seeds, const double *r, const double *p0,
+                             const double *q, const double *boost_frac);
+int sev_pipeline_set_batch(sev_pipeline *p, int64_t S, const int64_t *cells);
 int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *nnz);
 int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, int32_t *nzval);
 
@@ -179,8 +193,14 @@ int sev_pipeline_get_normalized(sev_pipeline *p, float *nzval);
  * features are 1-based gene indices in the order to keep. */
 int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *features,
                               double scale_max);
+/* Batch form: features is S x nfeat (row s = sample s's 1-based genes);
+ * column j of sample s is scaled with sample s's own statistics
+ * (src/scaling.jl:199-217 over its cells).  The result is one stacked
+ * cells x nfeat CSC. */
+int sev_pipeline_select_scale_batch(sev_pipeline *p, int64_t nfeat, const int64_t *features,
+                                    double scale_max);
 /* nnz of the selected matrix, then its 1-based structure, scaled values
- * and mu' (length nfeat). */
+ * and mu' (length nfeat; S x nfeat, row per sample, for a batch). */
 int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz);
 int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, float *nzval,
                             float *mu);
@@ -190,15 +210,25 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
  * Z = U diag(s) kept on the device in Float32 for the neighbour search. */
 int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
                      const double *init, int precision, int64_t *iters, int64_t *nmult);
+/* Batch form: one IRLBA per sample (src/irlba.jl:47-85 on the sample's
+ * centered matrix), all run in lockstep over the Lanczos steps with BF16
+ * operator values and FP64 vectors; init is S x nfeat (row per sample);
+ * iters[S].  use_graph != 0 records each restart round once as a HIP graph
+ * and replays it.  npcs is capped by the smallest sample. */
+int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+                           const double *init, int use_graph, int64_t *iters, int64_t *nmult);
 /* s[npcs]; U (m x npcs) and V (nfeat x npcs) column-major FP64; Z (m x npcs
- * column-major Float32).  Any pointer may be NULL. */
+ * column-major Float32).  Any pointer may be NULL.  For a batch, s is
+ * S x npcs (row per sample), V holds S blocks of nfeat x npcs, and U, Z
+ * keep the stacked rows. */
 int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float *Z);
 /* Replace the embedding with a caller-supplied m x d Float32 matrix
  * (column-major, ld >= m), e.g. to search neighbours of given coordinates. */
 int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const float *Z, int64_t ld);
 
 /* Stage 4: exact kNN on the first ndims embedding dimensions (ndims <= 0:
- * all). */
+ * all).  For a batch, the neighbours of a cell are cells of its own sample
+ * (indices stay those of the stacked rows). */
 int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
                             int precision, int64_t *n_fallback);
 int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances);

@@ -46,14 +46,18 @@ SIGNATURES = {
     "sev_pipeline_synchronize": (_int, [_p]),
     "sev_pipeline_load_counts": (_int, [_p, _i64, _i64, _p, _p, _p]),
     "sev_pipeline_synth_counts": (_int, [_p, _i64, _i64, C.c_uint64, _dbl, _p, _p, _dbl]),
+    "sev_pipeline_synth_batch": (_int, [_p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
+    "sev_pipeline_set_batch": (_int, [_p, _i64, _p]),
     "sev_pipeline_counts_info": (_int, [_p, _pi64, _pi64, _pi64]),
     "sev_pipeline_get_counts": (_int, [_p, _p, _p, _p]),
     "sev_pipeline_normalize": (_int, [_p, _int, _dbl]),
     "sev_pipeline_get_normalized": (_int, [_p, _p]),
     "sev_pipeline_select_scale": (_int, [_p, _i64, _p, _dbl]),
+    "sev_pipeline_select_scale_batch": (_int, [_p, _i64, _p, _dbl]),
     "sev_pipeline_scaled_nnz": (_int, [_p, _pi64]),
     "sev_pipeline_get_scaled": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_pca": (_int, [_p, _i64, _i64, _dbl, _p, _int, _pi64, _pi64]),
+    "sev_pipeline_pca_batch": (_int, [_p, _i64, _i64, _dbl, _p, _int, _p, _pi64]),
     "sev_pipeline_get_pca": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_set_embedding": (_int, [_p, _i64, _i64, _p, _i64]),
     "sev_pipeline_neighbours": (_int, [_p, _i64, _int, _i64, _int, _pi64]),

@@ -203,8 +203,16 @@ struct sev_pipeline {
     DevBuf<int64_t> scan_scratch, chunk_cnt, chunk_off;
     DevBuf<uint8_t> cls;
     DevBuf<double> p0, q;
+    // batch: S samples stacked by rows (cells), every per-sample stage runs
+    // on its own rows
+    BatchLayout bl;
+    bool batched = false;
+    DevBuf<int64_t> soff, vsrc, vcp;
+    DevBuf<float> mu_all;
+    BatchIrlbaWork biw;
 
     hipStream_t st() const { return stream.st; }
+    int64_t nsamples() const { return batched ? bl.S : 1; }
 
     void invalidate_from_counts() {
         have_norm = have_scaled = have_pca = have_embedding = have_nn = have_snn = false;
@@ -317,6 +325,7 @@ int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_
         p->m = m;
         p->g = n;
         p->have_counts = true;
+        p->batched = false;
         p->invalidate_from_counts();
     });
 }
@@ -349,7 +358,98 @@ int sev_pipeline_synth_counts(sev_pipeline *p, int64_t m, int64_t n, uint64_t se
         p->g = n;
         p->nnz = nnz;
         p->have_counts = true;
+        p->batched = false;
         p->invalidate_from_counts();
+    });
+}
+
+int sev_pipeline_synth_batch(sev_pipeline *p, int64_t S, const int64_t *cells, int64_t n,
+                             const uint64_t *seeds, const double *r, const double *p0,
+                             const double *q, const double *boost_frac) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(S >= 1 && n > 0 && n < (int64_t)0x7FFFFFFF, "bad batch shape");
+        hipStream_t st = p->st();
+        std::vector<int64_t> cv(cells, cells + S);
+        int64_t M = 0;
+        for (int64_t s = 0; s < S; ++s) {
+            SEV_REQUIRE(cv[(size_t)s] >= 2, "every sample needs at least two cells");
+            M += cv[(size_t)s];
+        }
+        SEV_REQUIRE(M < (int64_t)0x7FFFFFFF, "batch too large");
+        // each sample into its own CSC, then stacked by rows
+        std::vector<DevBuf<int64_t>> cps((size_t)S);
+        std::vector<DevBuf<int32_t>> rows((size_t)S), cnts((size_t)S);
+        std::vector<const int64_t *> cph((size_t)S);
+        for (int64_t s = 0; s < S; ++s) {
+            const int64_t m = cv[(size_t)s];
+            upload(p->p0, p0 + (size_t)(2 * n * s), (size_t)(2 * n), st);
+            upload(p->q, q + (size_t)(2 * n * s), (size_t)(2 * n), st);
+            p->cls.ensure((size_t)m);
+            SynthPlan plan;
+            const int64_t nchunk = ceil_div(m, 16384);
+            p->chunk_cnt.ensure((size_t)(n * nchunk));
+            p->chunk_off.ensure((size_t)(n * nchunk + 1));
+            p->scan_scratch.ensure(scan_scratch_elems(n * nchunk));
+            cps[(size_t)s].ensure((size_t)n + 1);
+            const int64_t nnz = synth_count(m, n, seeds[s], r[s], p->p0.get(), p->q.get(),
+                                            boost_frac[s], p->cls.get(), p->chunk_cnt.get(),
+                                            p->chunk_off.get(), cps[(size_t)s].get(),
+                                            p->scan_scratch.get(), st, &plan);
+            rows[(size_t)s].ensure((size_t)nnz + 1);
+            cnts[(size_t)s].ensure((size_t)nnz + 1);
+            synth_fill(plan, seeds[s], r[s], p->p0.get(), p->q.get(), p->cls.get(),
+                       p->chunk_off.get(), rows[(size_t)s].get(), cnts[(size_t)s].get(), st);
+            SEV_HIP(hipStreamSynchronize(st));
+            cph[(size_t)s] = cps[(size_t)s].get();
+        }
+        DevBuf<const int64_t *> cps_d;
+        upload(cps_d, cph.data(), (size_t)S, st);
+        DevBuf<int64_t> starts;
+        starts.ensure((size_t)(n * S + 1));
+        p->scan_scratch.ensure(scan_scratch_elems(n * S));
+        stack_lengths(n, S, cps_d.get(), starts.get(), st);
+        scan_exclusive_i64(starts.get(), starts.get(), n * S, p->scan_scratch.get(), st);
+        int64_t nnz = 0;
+        SEV_HIP(hipMemcpyAsync(&nnz, starts.get() + n * S, sizeof nnz, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        p->cp.ensure((size_t)n + 1);
+        p->row.ensure((size_t)nnz);
+        p->cnt.ensure((size_t)nnz);
+        stride_ptr(starts.get(), n, S, p->cp.get(), st);
+        int64_t roff = 0;
+        for (int64_t s = 0; s < S; ++s) {
+            stack_copy(n, S, s, cps[(size_t)s].get(), rows[(size_t)s].get(), cnts[(size_t)s].get(),
+                       roff, starts.get(), p->row.get(), p->cnt.get(), st);
+            roff += cv[(size_t)s];
+        }
+        SEV_HIP(hipStreamSynchronize(st));
+        p->m = M;
+        p->g = n;
+        p->nnz = nnz;
+        p->have_counts = true;
+        p->invalidate_from_counts();
+        batch_layout(p->bl, cv, st);
+        p->batched = true;
+    });
+}
+
+int sev_pipeline_set_batch(sev_pipeline *p, int64_t S, const int64_t *cells) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        if (S == 0) {
+            p->batched = false;
+        } else {
+            SEV_REQUIRE(S >= 1 && cells, "bad batch");
+            std::vector<int64_t> cv(cells, cells + S);
+            int64_t M = 0;
+            for (int64_t c : cv) M += c;
+            SEV_REQUIRE(M == p->m, "sample cells do not add up to the loaded cells");
+            batch_layout(p->bl, cv, p->st());
+            p->batched = true;
+        }
+        p->have_scaled = p->have_pca = p->have_embedding = p->have_nn = p->have_snn = false;
     });
 }
 
@@ -446,6 +546,57 @@ int sev_pipeline_select_scale(sev_pipeline *p, int64_t nfeat, const int64_t *fea
     });
 }
 
+int sev_pipeline_select_scale_batch(sev_pipeline *p, int64_t nfeat, const int64_t *features,
+                                    double scale_max) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_norm, "normalize has not run");
+        SEV_REQUIRE(p->batched, "no batch declared");
+        SEV_REQUIRE(nfeat >= 1, "need at least one feature");
+        hipStream_t st = p->st();
+        const int64_t S = p->bl.S, G = p->g, F = nfeat, NV = F * S;
+        std::vector<int64_t> f((size_t)NV);
+        for (int64_t i = 0; i < NV; ++i) {
+            SEV_REQUIRE(features[i] >= 1 && features[i] <= G, "feature index out of range");
+            f[(size_t)i] = features[i] - 1;
+        }
+        upload(p->feat, f.data(), (size_t)NV, st);
+        // virtual columns v = j*S + s: feature j of sample s on its own rows
+        p->soff.ensure((size_t)((S + 1) * G));
+        sample_offsets(G, S, p->bl.row_sample_d.get(), p->cp.get(), p->row.get(), p->soff.get(),
+                       st);
+        p->vsrc.ensure((size_t)NV);
+        p->vcp.ensure((size_t)NV + 1);
+        p->scan_scratch.ensure(scan_scratch_elems(NV));
+        batch_seg_lengths(F, S, G, p->feat.get(), p->soff.get(), p->vsrc.get(), p->vcp.get(), st);
+        scan_exclusive_i64(p->vcp.get(), p->vcp.get(), NV, p->scan_scratch.get(), st);
+        int64_t fnnz = 0;
+        SEV_HIP(hipMemcpyAsync(&fnnz, p->vcp.get() + NV, sizeof fnnz, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        p->frow.ensure((size_t)fnnz + 1);
+        p->fval.ensure((size_t)fnnz + 1);
+        p->bval.ensure((size_t)fnnz + kValuePad);
+        p->mu.ensure((size_t)NV);
+        p->sd.ensure((size_t)NV);
+        p->clip.ensure((size_t)NV);
+        p->mu_all.ensure((size_t)NV);
+        p->fcp.ensure((size_t)F + 1);
+        batch_gather(NV, p->vsrc.get(), p->vcp.get(), p->row.get(), p->norm.get(), p->frow.get(),
+                     p->fval.get(), st);
+        batch_scale_stats(NV, p->vcp.get(), p->fval.get(), (float)scale_max, p->sd.get(),
+                          p->mu.get(), p->clip.get(), p->bl.row0_d.get(), S, st);
+        scale_map<float>(NV, p->vcp.get(), p->fval.get(), p->sd.get(), p->clip.get(),
+                         p->bval.get(), st);
+        stride_ptr(p->vcp.get(), F, S, p->fcp.get(), st);
+        mu_by_sample(p->mu.get(), F, S, p->mu_all.get(), st);
+        SEV_HIP(hipStreamSynchronize(st));
+        p->F = F;
+        p->fnnz = fnnz;
+        p->have_scaled = true;
+        p->have_pca = p->have_embedding = p->have_nn = p->have_snn = false;
+    });
+}
+
 int sev_pipeline_scaled_nnz(sev_pipeline *p, int64_t *nnz) {
     return guarded([&] {
         ProfInstall pi_(&p->prof);
@@ -464,7 +615,8 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
         if (colptr) download(colptr, p->fcp.get(), (size_t)p->F + 1, st);
         if (rowval) download(rv.data(), p->frow.get(), (size_t)p->fnnz, st);
         if (nzval) download(nzval, p->bval.get(), (size_t)p->fnnz, st);
-        if (mu) download(mu, p->mu.get(), (size_t)p->F, st);
+        if (mu) download(mu, p->batched ? p->mu_all.get() : p->mu.get(),
+                         (size_t)(p->F * p->nsamples()), st);
         SEV_HIP(hipStreamSynchronize(st));
         if (colptr)
             for (int64_t j = 0; j <= p->F; ++j) colptr[j] += 1;
@@ -478,6 +630,7 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, c
     return guarded([&] {
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_scaled, "scale has not run");
+        SEV_REQUIRE(!p->batched, "a batch runs sev_pipeline_pca_batch");
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16, "bad precision");
         hipStream_t st = p->st();
         const int64_t m = p->m, n = p->F;
@@ -510,14 +663,59 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, c
     });
 }
 
+int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+                           const double *init, int use_graph, int64_t *iters, int64_t *nmult) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_scaled && p->batched, "no scaled batch");
+        SEV_REQUIRE(init != nullptr, "init is NULL");
+        hipStream_t st = p->st();
+        const int64_t M = p->m, n = p->F, S = p->bl.S;
+        int64_t minm = M;
+        for (int64_t s = 0; s < S; ++s)
+            minm = std::min<int64_t>(minm, p->bl.row0[(size_t)s + 1] - p->bl.row0[(size_t)s]);
+        const int64_t nu = std::min(std::min(minm, n), npcs);
+        SEV_REQUIRE(nu >= 1, "npcs must be >= 1");
+        SEV_REQUIRE(n <= 14336, "batches need at most 14336 features");
+        // the stacked BF16 operator; lane widths tuned on it, then slabs cut
+        // at the sample boundaries
+        build_slab_csc(p->op, M, n, p->fcp.get(), p->frow.get(), p->bval.get(), false, nullptr,
+                       true, st);
+        autotune_operator(p->op, st);
+        batch_slabs(p->op, p->bl, p->fcp.get(), p->frow.get(), st);
+        IrlbaParams prm;
+        prm.nu = nu;
+        prm.work = nu + 7;
+        prm.maxit = maxit;
+        prm.tol = tol;
+        prm.svtol = tol;
+        p->s.assign((size_t)(S * nu), 0.0);
+        p->U.ensure((size_t)(M * nu));
+        p->V.ensure((size_t)(S * n * nu));
+        std::vector<int64_t> it((size_t)S, 0);
+        irlba_batch(p->op, p->bl, p->mu_all.get(), prm, init, p->biw, p->s.data(), p->U.get(),
+                    p->V.get(), it.data(), nmult, use_graph != 0, st);
+        if (iters) std::memcpy(iters, it.data(), sizeof(int64_t) * (size_t)S);
+        p->npcs = nu;
+        upload(p->s_d, p->s.data(), (size_t)(S * nu), st);
+        p->Z.ensure((size_t)(M * nu));
+        batch_embedding(p->U.get(), p->s_d.get(), p->bl, nu, p->Z.get(), 1, M, st);
+        SEV_HIP(hipStreamSynchronize(st));
+        p->zdim = nu;
+        p->have_pca = p->have_embedding = true;
+        p->have_nn = p->have_snn = false;
+    });
+}
+
 int sev_pipeline_get_pca(sev_pipeline *p, double *s, double *U, double *V, float *Z) {
     return guarded([&] {
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_pca, "pca has not run");
         hipStream_t st = p->st();
-        if (s) std::memcpy(s, p->s.data(), sizeof(double) * (size_t)p->npcs);
+        const int64_t S = p->nsamples();
+        if (s) std::memcpy(s, p->s.data(), sizeof(double) * (size_t)(S * p->npcs));
         if (U) download(U, p->U.get(), (size_t)(p->m * p->npcs), st);
-        if (V) download(V, p->V.get(), (size_t)(p->F * p->npcs), st);
+        if (V) download(V, p->V.get(), (size_t)(S * p->F * p->npcs), st);
         if (Z) download(Z, p->Z.get(), (size_t)(p->m * p->npcs), st);
         SEV_HIP(hipStreamSynchronize(st));
     });
@@ -532,6 +730,7 @@ int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const floa
         SEV_HIP(hipMemcpy2DAsync(p->Z.get(), sizeof(float) * m, Z, sizeof(float) * ld,
                                  sizeof(float) * m, d, hipMemcpyHostToDevice, st));
         SEV_HIP(hipStreamSynchronize(st));
+        if (p->batched && m != p->bl.M) p->batched = false;
         p->m = m;
         p->zdim = d;
         p->have_embedding = true;
@@ -553,7 +752,31 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
         hipStream_t st = p->st();
         p->nn.ensure((size_t)(m * k));
         p->dist.ensure((size_t)(m * k));
-        if (precision == SEV_PREC_EXACT) {
+        if (p->batched) {
+            for (int64_t s = 0; s < p->bl.S; ++s)
+                SEV_REQUIRE(p->bl.row0[(size_t)s + 1] - p->bl.row0[(size_t)s] >=
+                                k + (include_self ? 0 : 1),
+                            "k larger than the cells of a sample");
+            if (precision == SEV_PREC_EXACT) {
+                std::vector<int64_t> q;
+                q.reserve((size_t)(m + 16 * p->bl.S));
+                for (int64_t s = 0; s < p->bl.S; ++s) {
+                    for (int64_t r = p->bl.row0[(size_t)s]; r < p->bl.row0[(size_t)s + 1]; ++r)
+                        q.push_back(r);
+                    while (q.size() % 16) q.push_back(p->bl.row0[(size_t)s + 1] - 1);
+                }
+                DevBuf<int64_t> qd;
+                upload(qd, q.data(), q.size(), st);
+                knn_exact_rows_batch(p->Z.get(), m, d, m, k, include_self, qd.get(),
+                                     (int64_t)q.size(), p->bl.row0_d.get(),
+                                     p->bl.row_sample_d.get(), p->nn.get(), p->dist.get(), st);
+                SEV_HIP(hipStreamSynchronize(st));
+                if (n_fallback) *n_fallback = m;
+            } else {
+                knn_mfma_batch(p->Z.get(), m, d, m, k, include_self, precision == SEV_PREC_BF16,
+                               p->bl, p->kw, p->nn.get(), p->dist.get(), st, n_fallback);
+            }
+        } else if (precision == SEV_PREC_EXACT) {
             knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
                              p->dist.get(), st);
             if (n_fallback) *n_fallback = m;

@@ -545,9 +545,8 @@ __device__ double hash_normal(uint64_t seed, uint32_t i) {
 constexpr int kRsBlock = 256;
 
 template <class MT>
-__global__ void __launch_bounds__(kRsBlock)
-k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, double *Fv,
-          double *rpart) {
+__device__ void right_a_body(int64_t n, int j, const double *part2, const MT *mu, const double *V,
+                             double *Fv, double *rpart, unsigned bid) {
     __shared__ double red[kRsBlock / kWave][kMaxWork];
     const int64_t ncol = n + 2;
     double nrm = 0.0, sumy = 0.0;
@@ -557,7 +556,7 @@ k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, 
     }
     double S = sqrt(nrm);
     if (!(S >= 2.0 * 2.220446049250313e-16)) S = 1.0;
-    const int64_t i = (int64_t)blockIdx.x * kRsBlock + threadIdx.x;
+    const int64_t i = (int64_t)bid * kRsBlock + threadIdx.x;
     double f = 0.0;
     if (i < n) {
         double t = 0.0;
@@ -579,13 +578,19 @@ k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, 
     for (int c = threadIdx.x; c <= j; c += kRsBlock) {
         double t = 0.0;
         for (int q = 0; q < kRsBlock / kWave; ++q) t += red[q][c];
-        rpart[(int64_t)blockIdx.x * kMaxWork + c] = t;
+        rpart[(int64_t)bid * kMaxWork + c] = t;
     }
 }
 
+template <class MT>
 __global__ void __launch_bounds__(kRsBlock)
-k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, double *Fv,
-          double *npart) {
+k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, double *Fv,
+          double *rpart) {
+    right_a_body<MT>(n, j, part2, mu, V, Fv, rpart, blockIdx.x);
+}
+
+__device__ void right_b_body(int64_t n, int j, int nblk, const double *rpart, const double *V,
+                             double *Fv, double *npart, unsigned bid) {
     __shared__ double T[kMaxWork];
     __shared__ double scratch[kRsBlock / kWave];
     for (int c = threadIdx.x; c <= j; c += kRsBlock) {
@@ -594,7 +599,7 @@ k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, doub
         T[c] = t;
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * kRsBlock + threadIdx.x;
+    const int64_t i = (int64_t)bid * kRsBlock + threadIdx.x;
     double f = 0.0;
     if (i < n) {
         double d = 0.0;
@@ -603,13 +608,18 @@ k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, doub
         Fv[i] = f;
     }
     const double s2 = block_sum(f * f, scratch);
-    if (threadIdx.x == 0) npart[blockIdx.x] = s2;
+    if (threadIdx.x == 0) npart[bid] = s2;
 }
 
-template <class MT>
-__global__ void __launch_bounds__(1024)
-k_right_c(int64_t n, int j, int work, int nblk, const double *part2, const double *npart,
-          double *V, double *Fv, double *B, double *scal, int *flags, uint64_t seed) {
+__global__ void __launch_bounds__(kRsBlock)
+k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, double *Fv,
+          double *npart) {
+    right_b_body(n, j, nblk, rpart, V, Fv, npart, blockIdx.x);
+}
+
+__device__ void right_c_body(int64_t n, int j, int work, int nblk, const double *part2,
+                             const double *npart, double *V, double *Fv, double *B, double *scal,
+                             int *flags, uint64_t seed) {
     __shared__ double scratch[16];
     __shared__ double tsh[kMaxWork];
     const int64_t ncol = n + 2;
@@ -664,6 +674,13 @@ k_right_c(int64_t n, int j, int work, int nblk, const double *part2, const doubl
         scal[0] = S;
         scal[2] = RF;
     }
+}
+
+template <class MT>
+__global__ void __launch_bounds__(1024)
+k_right_c(int64_t n, int j, int work, int nblk, const double *part2, const double *npart,
+          double *V, double *Fv, double *B, double *scal, int *flags, uint64_t seed) {
+    right_c_body(n, j, work, nblk, part2, npart, V, Fv, B, scal, flags, seed);
 }
 
 // y -= RF w_prev (the recurrence term of A v_j when the product came from a
@@ -785,8 +802,8 @@ __device__ __forceinline__ double sum16(double v) {
     return v;
 }
 
-__global__ void __launch_bounds__(kSvdThreads)
-k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status) {
+__device__ void small_svd_body(int n, const double *B, double *U, double *s, double *V,
+                               int *status) {
     extern __shared__ double sm[];
     double *G = sm;
     double *J = sm + n * n;
@@ -893,6 +910,11 @@ k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status
     }
 }
 
+__global__ void __launch_bounds__(kSvdThreads)
+k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status) {
+    small_svd_body(n, B, U, s, V, status);
+}
+
 // --------------------------------------------------------------------------
 // Restart decision after the SVD of B (one thread; the rules of the
 // augmented restart): residuals res_i = ||F|| BU(work-1, i), Ritz-value
@@ -902,10 +924,9 @@ k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status
 // [diag(s_0..s_k-1) res_0..k-1 in column k].  info: [0] converged,
 // [1] k, [2] ||F|| <= 0 (cannot restart), [3] nconv, [4] SVD needs the host.
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
-k_restart_check(int work, int nu, double tol, double svtol, const double *BU, const double *BS,
-                const double *scal, const int *status, double *svr, double *B, int *info,
-                int last) {
+__device__ void restart_check_body(int work, int nu, double tol, double svtol, const double *BU,
+                                   const double *BS, const double *scal, const int *status,
+                                   double *svr, double *B, int *info, int last) {
     __shared__ int s_go;  // rebuild B for a restart
     __shared__ int s_k;
     __shared__ double s_rf;
@@ -961,6 +982,13 @@ k_restart_check(int work, int nu, double tol, double svtol, const double *BU, co
         B[(int64_t)k * work + c] = RF * BU[(int64_t)c * work + (work - 1)];
     }
     for (int i = tid; i < work; i += kBlock) svr[i] = BS[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_restart_check(int work, int nu, double tol, double svtol, const double *BU, const double *BS,
+                const double *scal, const int *status, double *svr, double *B, int *info,
+                int last) {
+    restart_check_body(work, nu, tol, svtol, BU, BS, scal, status, svr, B, info, last);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1560,6 +1588,804 @@ void embedding_f32(const double *U, const double *s_dev, int64_t m, int64_t d, f
                    int64_t rs, int64_t cs, hipStream_t st) {
     hipLaunchKernelGGL(k_embed, dim3(grid_rows(m)), dim3(kBlock), 0, st, U, s_dev, m, d, Z, rs,
                        cs);
+    SEV_LAUNCH_CHECK();
+}
+
+// ===========================================================================
+// Batched IRLBA: S independent centered problems on the stacked rows of one
+// operator, run in lockstep over the Lanczos steps.  Each sample keeps its
+// own restart size k, convergence test and restart; at step j a sample takes
+// part when it is not done and j >= its first step j0 (0 before the first
+// restart, k after).  Per-sample state lives on the device, so a restart
+// round is a fixed launch sequence for a given first step and buffer parity.
+// ===========================================================================
+namespace {
+
+struct BDev {
+    int64_t M, n;
+    int work, S;
+    const int32_t *row_sample;    // M
+    const int64_t *slab_row0;     // nslab + 1
+    const int32_t *slab_sample;   // nslab
+    const int64_t *sample_slab0;  // S + 1
+    const int32_t *blk_sample;    // nblk
+    const int64_t *blk_r0;        // nblk + 1
+    int64_t nslab;
+    int *j0, *done, *iters, *fail;  // S each (state)
+};
+
+__device__ __forceinline__ bool b_active(const BDev &d, int s, int j) {
+    return !d.done[s] && j >= d.j0[s];
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock)
+k_b_av_bf16(BDev d, const int64_t *rowptr, const uint16_t *col, const uint16_t *val,
+            const float *mu_all, const double *Vall, int j, const double *scal, const double *W,
+            double *y) {
+    extern __shared__ float vf[];
+    __shared__ double dmu_s;
+    const int s = d.blk_sample[blockIdx.x];
+    if (!b_active(d, s, j)) return;
+    const bool first = j == d.j0[s];
+    const int64_t n = d.n;
+    const double *v = Vall + ((int64_t)s * d.work + j) * n;
+    const float *mu = mu_all + (int64_t)s * n;
+    const int w = wave_index(), lane = lane_id();
+    for (int64_t i = threadIdx.x; i < n; i += kBlock) vf[i] = (float)v[i];
+    if (w == 0) {
+        double dd = 0.0;
+        for (int64_t i = lane; i < n; i += kWave) dd = fma((double)mu[i], v[i], dd);
+        dd = wave_sum(dd);
+        if (lane == 0) dmu_s = dd;
+    }
+    __syncthreads();
+    const double dmu = dmu_s;
+    const double rfv = first ? 0.0 : scal[4 * s + 1];
+    const double *wprev = first ? nullptr : W + (int64_t)(j - 1) * d.M;
+    constexpr int RPW = kWave / G;
+    const int gl = lane % G, gi = lane / G;
+    const int64_t rb = d.blk_r0[blockIdx.x], re = d.blk_r0[blockIdx.x + 1];
+    for (int64_t r0 = rb + (int64_t)w * RPW; r0 < re; r0 += (int64_t)kWaves * RPW) {
+        const int64_t r = r0 + gi;
+        float a0 = 0.0f, a1 = 0.0f;
+        if (r < re) {
+            const int64_t b = rowptr[r], e = rowptr[r + 1];
+            for (int64_t p = b + 8 * gl; p < e; p += 8 * G) {
+                const uint4 c8 = *(const uint4 *)(col + p);
+                const uint4 v8 = *(const uint4 *)(val + p);
+                a0 = fmaf(bf_lo(v8.x), vf[c8.x & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.x), vf[c8.x >> 16], a1);
+                a0 = fmaf(bf_lo(v8.y), vf[c8.y & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.y), vf[c8.y >> 16], a1);
+                a0 = fmaf(bf_lo(v8.z), vf[c8.z & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.z), vf[c8.z >> 16], a1);
+                a0 = fmaf(bf_lo(v8.w), vf[c8.w & 0xFFFFu], a0);
+                a1 = fmaf(bf_hi(v8.w), vf[c8.w >> 16], a1);
+            }
+        }
+        float acc = a0 + a1;
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+        acc = __shfl(acc, gi * G, kWave);
+        if (gl == 0 && r < re) {
+            double t = (double)acc - dmu;
+            if (wprev) t -= rfv * wprev[r];
+            y[r] = t;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kDotThreads)
+k_b_dot_w(BDev d, const double *W, int j, const double *y, double *part) {
+    constexpr int NW = kDotThreads / kWave;
+    __shared__ double red[NW][32];
+    const int slab = blockIdx.x;
+    const int s = d.slab_sample[slab];
+    if (j == 0 || !b_active(d, s, j)) return;
+    const int jc = j;
+    const int64_t m = d.M;
+    const int64_t rb = d.slab_row0[slab], re = d.slab_row0[slab + 1];
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    for (int c0 = 0; c0 < jc; c0 += 32) {
+        double acc[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) acc[i] = 0.0;
+        for (int64_t r = rb + threadIdx.x; r < re; r += kDotThreads) {
+            const double yr = y[r];
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (c0 + i < jc) acc[i] = fma(W[(int64_t)(c0 + i) * m + r], yr, acc[i]);
+        }
+        const double t = butterfly32(acc);
+        if ((lane & 1) == 0) red[w][lane >> 1] = t;
+        __syncthreads();
+        if (threadIdx.x < 32 && c0 + (int)threadIdx.x < jc) {
+            double u = 0.0;
+            for (int q = 0; q < NW; ++q) u += red[q][threadIdx.x];
+            part[(int64_t)(c0 + threadIdx.x) * d.nslab + slab] = u;
+        }
+        __syncthreads();
+    }
+}
+
+// T_s[c] = sum over the sample's slabs (same order as the single problem)
+__global__ void __launch_bounds__(kBlock)
+k_b_red_t(BDev d, int j, const double *part, double *T) {
+    __shared__ double scratch[kWaves];
+    const int c = blockIdx.x, s = blockIdx.y;
+    if (j == 0 || !b_active(d, s, j)) return;
+    const int64_t sb = d.sample_slab0[s];
+    const int nblk = (int)(d.sample_slab0[s + 1] - sb);
+    const double *pc = part + (int64_t)c * d.nslab + sb;
+    double t0 = 0.0, t1 = 0.0;
+    int b = threadIdx.x;
+    for (; b + kBlock < nblk; b += 2 * kBlock) {
+        t0 += pc[b];
+        t1 += pc[b + kBlock];
+    }
+    if (b < nblk) t0 += pc[b];
+    const double t = block_sum(t0 + t1, scratch);
+    if (threadIdx.x == 0) T[(int64_t)s * kMaxWork + c] = t;
+}
+
+__global__ void __launch_bounds__(kApplyThreads)
+k_b_apply(BDev d, const double *W, int j, const double *T, const double *y, double *yout,
+          double *part) {
+    __shared__ double tl[kMaxWork];
+    __shared__ double scratch[kApplyThreads / kWave];
+    const int slab = blockIdx.x;
+    const int s = d.slab_sample[slab];
+    if (!b_active(d, s, j)) return;
+    const int jc = j;
+    const int64_t m = d.M, n = d.n;
+    const int64_t r0 = d.slab_row0[slab];
+    const int64_t nrows = d.slab_row0[slab + 1] - r0;
+    if ((int)threadIdx.x < jc) tl[threadIdx.x] = T[(int64_t)s * kMaxWork + threadIdx.x];
+    __syncthreads();
+    double nrm = 0.0, sm = 0.0;
+    for (int64_t lr = threadIdx.x; lr < nrows; lr += kApplyThreads) {
+        const int64_t r = r0 + lr;
+        double yv = y[r];
+        if (jc > 0) {
+            double dd = 0.0;
+#pragma unroll 8
+            for (int i = 0; i < jc; ++i) dd = fma(W[(int64_t)i * m + r], tl[i], dd);
+            yv -= dd;
+        }
+        yout[r] = yv;
+        nrm = fma(yv, yv, nrm);
+        sm += yv;
+    }
+    nrm = block_sum(nrm, scratch);
+    sm = block_sum(sm, scratch);
+    if (threadIdx.x == 0) {
+        part[(int64_t)slab * (n + 2) + n] = nrm;
+        part[(int64_t)slab * (n + 2) + n + 1] = sm;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_atw_bf16(BDev d, int j, const int64_t *off, const uint16_t *lrow, const uint16_t *val,
+             const double *y, double *part) {
+    __shared__ float wl[kSlabRows];
+    const int64_t slab = blockIdx.x;
+    const int s = d.slab_sample[slab];
+    if (!b_active(d, s, j)) return;
+    const int64_t n = d.n;
+    const int g = blockIdx.y, ng = gridDim.y;
+    const int64_t r0 = d.slab_row0[slab];
+    const int64_t nrows = d.slab_row0[slab + 1] - r0;
+    const int w = wave_index(), lane = lane_id();
+    for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock)
+        wl[lr] = lr < nrows ? (float)y[r0 + lr] : 0.0f;
+    __syncthreads();
+    double *prow = part + slab * (n + 2);
+    for (int64_t jj = (int64_t)g * kWaves + w; jj < n; jj += (int64_t)ng * kWaves) {
+        const int64_t a = off[slab * n + jj], b = off[(slab + 1) * n + jj];
+        float acc0 = 0.0f, acc1 = 0.0f;
+        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
+            const ushort4 l4 = *(const ushort4 *)(lrow + p);
+            const uint2 v4 = *(const uint2 *)(val + p);
+            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
+                       v3 = p + 3 < b;
+            acc0 = fmaf(v0 ? bf_lo(v4.x) : 0.0f, wl[v0 ? l4.x : 0], acc0);
+            acc1 = fmaf(v1 ? bf_hi(v4.x) : 0.0f, wl[v1 ? l4.y : 0], acc1);
+            acc0 = fmaf(v2 ? bf_lo(v4.y) : 0.0f, wl[v2 ? l4.z : 0], acc0);
+            acc1 = fmaf(v3 ? bf_hi(v4.y) : 0.0f, wl[v3 ? l4.w : 0], acc1);
+        }
+        const float acc = wave_sum(acc0 + acc1);
+        if (lane == 0) prow[jj] = (double)acc;
+    }
+}
+
+// part2_s[q][c] = sum over the slabs of chunk q of sample s
+__global__ void __launch_bounds__(kBlock)
+k_b_red_slabs(BDev d, int j, const double *part, double *part2) {
+    const int s = blockIdx.z;
+    if (!b_active(d, s, j)) return;
+    const int64_t ncol = d.n + 2;
+    const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t q = blockIdx.y;
+    if (c >= ncol) return;
+    const int64_t sb = d.sample_slab0[s], ns = d.sample_slab0[s + 1] - sb;
+    const int64_t per = (ns + kRedChunks - 1) / kRedChunks;
+    const int64_t b0 = q * per, b1 = b0 + per < ns ? b0 + per : ns;
+    double t = 0.0;
+    for (int64_t b = b0; b < b1; ++b) t += part[(sb + b) * ncol + c];
+    part2[((int64_t)s * kRedChunks + q) * ncol + c] = t;
+}
+
+__global__ void __launch_bounds__(kRsBlock)
+k_b_right_a(BDev d, int j, const double *part2, const float *mu_all, const double *Vall,
+            double *Fall, double *rpart, int nblk) {
+    const int s = blockIdx.y;
+    if (!b_active(d, s, j)) return;
+    const int64_t n = d.n;
+    right_a_body<float>(n, j, part2 + (int64_t)s * kRedChunks * (n + 2), mu_all + (int64_t)s * n,
+                        Vall + (int64_t)s * d.work * n, Fall + (int64_t)s * n,
+                        rpart + (int64_t)s * nblk * (kMaxWork + 1), blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kRsBlock)
+k_b_right_b(BDev d, int j, int nblk, const double *rpart, const double *Vall, double *Fall) {
+    const int s = blockIdx.y;
+    if (!b_active(d, s, j)) return;
+    const int64_t n = d.n;
+    const double *rp = rpart + (int64_t)s * nblk * (kMaxWork + 1);
+    right_b_body(n, j, nblk, rp, Vall + (int64_t)s * d.work * n, Fall + (int64_t)s * n,
+                 (double *)rp + (size_t)nblk * kMaxWork, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(1024)
+k_b_right_c(BDev d, int j, int nblk, const double *part2, const double *rpart, double *Vall,
+            double *Fall, double *Ball, double *scal, int *flags) {
+    const int s = blockIdx.x;
+    if (!b_active(d, s, j)) return;
+    const int64_t n = d.n;
+    const int work = d.work;
+    const double *rp = rpart + (int64_t)s * nblk * (kMaxWork + 1);
+    right_c_body(n, j, work, nblk, part2 + (int64_t)s * kRedChunks * (n + 2),
+                 rp + (size_t)nblk * kMaxWork, Vall + (int64_t)s * work * n,
+                 Fall + (int64_t)s * n, Ball + (int64_t)s * work * work, scal + 4 * s,
+                 flags + 4 * s, (uint64_t)(d.iters[s] * work + j) + 17u);
+}
+
+// W[:, j] = yout / S for the rows of active samples
+__global__ void __launch_bounds__(kBlock)
+k_b_scale_col(BDev d, int j, const double *yout, double *W, const double *scal) {
+    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < d.M;
+         r += (int64_t)gridDim.x * kBlock) {
+        const int s = d.row_sample[r];
+        if (b_active(d, s, j)) W[(int64_t)j * d.M + r] = yout[r] / scal[4 * s];
+    }
+}
+
+__global__ void __launch_bounds__(kSvdThreads)
+k_b_small_svd(BDev d, const double *Ball, double *BUall, double *BSall, double *BVall,
+              int *info) {
+    const int s = blockIdx.x;
+    if (d.done[s]) return;
+    const int work = d.work;
+    const int64_t o = (int64_t)s * work * work;
+    small_svd_body(work, Ball + o, BUall + o, BSall + (int64_t)s * kMaxWork, BVall + o,
+                   info + 16 * s + 8);
+}
+
+// The restart decision of each sample; then its state for the next round:
+// j0 = k, iters + 1, done on convergence, failure or the last iteration.
+__global__ void __launch_bounds__(kBlock)
+k_b_restart_check(BDev d, int nu, int maxit, double tol, double svtol, const double *BUall,
+                  const double *BSall, const double *scal, double *svr, double *Ball, int *info,
+                  const int *flags) {
+    const int s = blockIdx.x;
+    if (d.done[s]) return;
+    const int work = d.work;
+    const int64_t o = (int64_t)s * work * work;
+    int *inf = info + 16 * s;
+    const int last = d.iters[s] + 1 >= maxit ? 1 : 0;
+    const int inv = flags[4 * s];
+    if (!inv)
+        restart_check_body(work, nu, tol, svtol, BUall + o, BSall + (int64_t)s * kMaxWork,
+                           scal + 4 * s, inf + 8, svr + (int64_t)s * kMaxWork, Ball + o, inf,
+                           last);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d.iters[s] += 1;
+        if (inv) {
+            d.fail[s] = 1;  // invariant subspace
+            d.done[s] = 1;
+        } else if (inf[4]) {
+            d.fail[s] = 4;  // needs the host SVD: reported as a failure here
+            d.done[s] = 1;
+        } else if (inf[0]) {
+            d.done[s] = 1;  // converged
+        } else if (last) {
+            d.fail[s] = 2;  // maxit
+            d.done[s] = 1;
+        } else if (inf[2]) {
+            d.fail[s] = 3;  // zero residual at restart
+            d.done[s] = 1;
+        } else {
+            d.j0[s] = inf[1];
+        }
+    }
+}
+
+// Y_s[:, c] = X_s[:, 0:work] C_s[:, c] for c < kout_s: rows of the stacked
+// W (ld M, sample from row_sample) or of the per-sample V blocks (n rows,
+// ld n, sample = block).  The extra column kout_s = F_s / ||F_s|| for V.
+// Samples that are done are skipped (restart) or selected by sel (final).
+__global__ void __launch_bounds__(kBlock)
+k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
+         int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
+         bool restart) {
+    const int64_t rows = vmode ? (int64_t)d.S * d.n : d.M;
+    const int work = d.work;
+    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < rows;
+         g += (int64_t)gridDim.x * kBlock) {
+        int s;
+        int64_t r, ld;
+        if (vmode) {
+            s = (int)(g / d.n);
+            r = g - (int64_t)s * d.n;
+            ld = d.n;
+        } else {
+            s = d.row_sample[g];
+            r = g;
+            ld = d.M;
+        }
+        if (restart && d.done[s]) continue;
+        const int kout = kfix > 0 ? kfix : d.j0[s];
+        const double *X = (xsel && xsel[s]) ? X1 : X0;
+        const double *Xs = vmode ? X + (int64_t)s * work * d.n : X;
+        double *Ys = vmode ? Y + (int64_t)s * ldy * d.n : Y;
+        const double *Cs = Call + (int64_t)s * work * work;
+        for (int c0 = 0; c0 < kout; c0 += 16) {
+            double acc[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) acc[c] = 0.0;
+            for (int l = 0; l < work; ++l) {
+                const double x = Xs[(int64_t)l * ld + r];
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c0 + c < kout) acc[c] = fma(x, Cs[(int64_t)(c0 + c) * work + l], acc[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c0 + c < kout) Ys[(int64_t)(c0 + c) * ld + r] = acc[c];
+        }
+        if (vmode && restart) Ys[(int64_t)kout * ld + r] = Fall[(int64_t)s * d.n + r] / scal[4 * s + 2];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_b_embed(const double *U, const double *s_all, const int32_t *row_sample, int64_t M, int64_t dd,
+          float *Z, int64_t rs, int64_t cs) {
+    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < M;
+         r += (int64_t)gridDim.x * kBlock) {
+        const int s = row_sample[r];
+        for (int64_t c = 0; c < dd; ++c)
+            Z[r * rs + c * cs] = (float)(U[c * M + r] * s_all[(int64_t)s * dd + c]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_var_slab_offsets(int64_t n, int64_t nslab, const int32_t *slab_of_row, const int64_t *slab_row0,
+                   const int64_t *colptr, const int32_t *row, int64_t *off, uint16_t *lrow) {
+    const int64_t j = blockIdx.x;
+    const int64_t b = colptr[j], e = colptr[j + 1];
+    if (b == e) {
+        for (int64_t s = threadIdx.x; s <= nslab; s += kBlock) off[s * n + j] = b;
+        return;
+    }
+    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+        const int64_t r = row[p];
+        const int64_t sc = slab_of_row[r];
+        const int64_t sp = (p == b) ? -1 : (int64_t)slab_of_row[row[p - 1]];
+        for (int64_t s = sp + 1; s <= sc; ++s) off[s * n + j] = p;
+        if (p == e - 1)
+            for (int64_t s = sc + 1; s <= nslab; ++s) off[s * n + j] = e;
+        lrow[p] = (uint16_t)(r - slab_row0[sc]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fill_slab_of_row(int64_t M, const int64_t *slab_row0, int64_t nslab, int32_t *out) {
+    for (int64_t s = blockIdx.x; s < nslab; s += gridDim.x)
+        for (int64_t r = slab_row0[s] + threadIdx.x; r < slab_row0[s + 1]; r += kBlock)
+            out[r] = (int32_t)s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fill_i32(int32_t *a, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fill_f64(double *a, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        a[i] = v;
+}
+
+}  // namespace
+
+void batch_layout(BatchLayout &L, const std::vector<int64_t> &cells, hipStream_t st) {
+    L.S = (int64_t)cells.size();
+    SEV_REQUIRE(L.S >= 1, "batch: no samples");
+    L.row0.assign((size_t)L.S + 1, 0);
+    for (int64_t s = 0; s < L.S; ++s) {
+        SEV_REQUIRE(cells[(size_t)s] >= 2, "batch: every sample needs at least two cells");
+        L.row0[(size_t)s + 1] = L.row0[(size_t)s] + cells[(size_t)s];
+    }
+    L.M = L.row0[(size_t)L.S];
+    L.slab_row0.clear();
+    L.slab_sample.clear();
+    L.sample_slab0.assign((size_t)L.S + 1, 0);
+    L.blk_sample.clear();
+    L.blk_r0.clear();
+    constexpr int64_t kBlkRows = 1024;
+    for (int64_t s = 0; s < L.S; ++s) {
+        L.sample_slab0[(size_t)s] = (int64_t)L.slab_sample.size();
+        for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; r += kSlabRows) {
+            L.slab_row0.push_back(r);
+            L.slab_sample.push_back((int32_t)s);
+        }
+        for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; r += kBlkRows) {
+            L.blk_r0.push_back(r);
+            L.blk_sample.push_back((int32_t)s);
+        }
+    }
+    L.sample_slab0[(size_t)L.S] = (int64_t)L.slab_sample.size();
+    L.slab_row0.push_back(L.M);
+    L.blk_r0.push_back(L.M);
+    std::vector<int32_t> rs((size_t)L.M);
+    for (int64_t s = 0; s < L.S; ++s)
+        for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; ++r) rs[(size_t)r] = (int32_t)s;
+    auto up64 = [&](DevBuf<int64_t> &b, const std::vector<int64_t> &h) {
+        b.ensure(h.size());
+        SEV_HIP(hipMemcpyAsync(b.get(), h.data(), sizeof(int64_t) * h.size(), hipMemcpyHostToDevice,
+                               st));
+    };
+    auto up32 = [&](DevBuf<int32_t> &b, const std::vector<int32_t> &h) {
+        b.ensure(h.size());
+        SEV_HIP(hipMemcpyAsync(b.get(), h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice,
+                               st));
+    };
+    up64(L.row0_d, L.row0);
+    up64(L.slab_row0_d, L.slab_row0);
+    up64(L.sample_slab0_d, L.sample_slab0);
+    up64(L.blk_r0_d, L.blk_r0);
+    up32(L.row_sample_d, rs);
+    up32(L.slab_sample_d, L.slab_sample);
+    up32(L.blk_sample_d, L.blk_sample);
+    SEV_HIP(hipStreamSynchronize(st));
+}
+
+void batch_slabs(SlabCsc &op, const BatchLayout &L, const int64_t *colptr, const int32_t *row,
+                 hipStream_t st) {
+    SEV_REQUIRE(op.m == L.M, "batch: operator rows differ from the layout");
+    const int64_t nslab = (int64_t)L.slab_sample.size();
+    DevBuf<int32_t> slab_of_row;
+    slab_of_row.ensure((size_t)L.M);
+    hipLaunchKernelGGL(k_fill_slab_of_row, dim3((unsigned)std::min<int64_t>(nslab, 65535)),
+                       dim3(kBlock), 0, st, L.M, L.slab_row0_d.get(), nslab, slab_of_row.get());
+    SEV_LAUNCH_CHECK();
+    op.nslab = nslab;
+    op.off.ensure((size_t)((nslab + 1) * op.n));
+    hipLaunchKernelGGL(k_var_slab_offsets, dim3((unsigned)op.n), dim3(kBlock), 0, st, op.n, nslab,
+                       slab_of_row.get(), L.slab_row0_d.get(), colptr, row, op.off.get(),
+                       op.lrow.get());
+    SEV_LAUNCH_CHECK();
+    SEV_HIP(hipStreamSynchronize(st));
+}
+
+BatchIrlbaWork::~BatchIrlbaWork() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+}
+
+void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
+                 const IrlbaParams &p, const double *init, BatchIrlbaWork &w, double *s_out,
+                 double *U, double *V, int64_t *iters_out, int64_t *nmult_out, bool graph,
+                 hipStream_t st) {
+    SEV_REQUIRE(op.vtype == kValBF16 && op.n <= kMaxFL32,
+                "batched irlba: BF16 operator with at most 14336 features");
+    const int64_t S = L.S, M = L.M, n = op.n;
+    const int nu = (int)p.nu;
+    int64_t workl = p.work;
+    int64_t minm = M;
+    for (int64_t s = 0; s < S; ++s) minm = std::min<int64_t>(minm, L.row0[s + 1] - L.row0[s]);
+    if (workl > minm) workl = minm;
+    if (workl > n) workl = n;
+    const int work = (int)workl;
+    SEV_REQUIRE(nu >= 1 && nu <= work, "batched irlba: need 1 <= nu <= min(cells, features)");
+    SEV_REQUIRE(work <= kMaxWork, "batched irlba: work exceeds 64");
+    const int64_t nslab = (int64_t)L.slab_sample.size();
+    const int nblk_rs = (int)ceil_div(n, kRsBlock);
+    w.W[0].ensure((size_t)(M * work));
+    w.W[1].ensure((size_t)(M * work));
+    w.V[0].ensure((size_t)(S * n * work));
+    w.V[1].ensure((size_t)(S * n * work));
+    w.B.ensure((size_t)(S * work * work));
+    w.BU.ensure((size_t)(S * work * work));
+    w.BV.ensure((size_t)(S * work * work));
+    w.BS.ensure((size_t)(S * kMaxWork));
+    w.svr.ensure((size_t)(S * kMaxWork));
+    w.T.ensure((size_t)(S * kMaxWork));
+    w.part.ensure((size_t)(nslab * (n + 2)));
+    w.part2.ensure((size_t)(S * kRedChunks * (n + 2)));
+    w.red.ensure((size_t)(nslab * kMaxWork));
+    w.Fy.ensure((size_t)M);
+    w.F.ensure((size_t)(S * n));
+    w.scal.ensure((size_t)(4 * S));
+    w.rpart.ensure((size_t)(S * nblk_rs * (kMaxWork + 1)));
+    w.flags.ensure((size_t)(4 * S));
+    w.info.ensure((size_t)(16 * S));
+    w.state.ensure((size_t)(4 * S));
+    const size_t svd_lds = sizeof(double) * 2 * (size_t)work * (size_t)work;
+    SEV_REQUIRE(svd_lds + 1024 <= (size_t)device_lds_per_block(),
+                "batched irlba: projected matrix too large for the device SVD");
+
+    // starting vectors and state
+    std::vector<double> v0((size_t)(S * n * work), 0.0);
+    for (int64_t s = 0; s < S; ++s) {
+        const double *in = init + s * n;
+        double nr = 0.0;
+        for (int64_t i = 0; i < n; ++i) nr += in[i] * in[i];
+        nr = std::sqrt(nr);
+        if (nr < 4.5e-16) fail(kErrConvergence, "batched irlba: a starting vector is zero");
+        for (int64_t i = 0; i < n; ++i) v0[(size_t)(s * work * n + i)] = in[i] / nr;
+    }
+    SEV_HIP(hipMemcpyAsync(w.V[0].get(), v0.data(), sizeof(double) * v0.size(),
+                           hipMemcpyHostToDevice, st));
+    std::vector<int> st_h((size_t)(4 * S), 0);  // j0 | done | iters | fail
+    std::vector<int> info_h((size_t)(16 * S), 0);
+    for (int64_t s = 0; s < S; ++s) info_h[(size_t)(16 * s + 1)] = nu;
+    SEV_HIP(hipMemcpyAsync(w.state.get(), st_h.data(), sizeof(int) * st_h.size(),
+                           hipMemcpyHostToDevice, st));
+    SEV_HIP(hipMemcpyAsync(w.info.get(), info_h.data(), sizeof(int) * info_h.size(),
+                           hipMemcpyHostToDevice, st));
+    auto zero_f64 = [&](DevBuf<double> &b, int64_t cnt) {
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_rows(cnt)), dim3(kBlock), 0, st, b.get(), cnt,
+                           0.0);
+        SEV_LAUNCH_CHECK();
+    };
+    zero_f64(w.B, S * work * work);
+    zero_f64(w.scal, 4 * S);
+    zero_f64(w.svr, S * kMaxWork);
+    zero_f64(w.part, nslab * (n + 2));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(kBlock), 0, st, w.flags.get(), 4 * S, 0);
+    SEV_LAUNCH_CHECK();
+
+    int *state = w.state.get();
+    BDev d;
+    d.M = M;
+    d.n = n;
+    d.work = work;
+    d.S = (int)S;
+    d.row_sample = L.row_sample_d.get();
+    d.slab_row0 = L.slab_row0_d.get();
+    d.slab_sample = L.slab_sample_d.get();
+    d.sample_slab0 = L.sample_slab0_d.get();
+    d.blk_sample = L.blk_sample_d.get();
+    d.blk_r0 = L.blk_r0_d.get();
+    d.nslab = nslab;
+    d.j0 = state;
+    d.done = state + S;
+    d.iters = state + 2 * S;
+    d.fail = state + 3 * S;
+    const int64_t nblk_av = (int64_t)L.blk_sample.size();
+    const int G = op.lanes_per_row;
+    const int ng = atw_groups(op);
+    const double tol = p.tol, svtol = p.svtol;
+    const int maxit = (int)std::min<int64_t>(p.maxit, 1 << 30);
+    // recorded graphs hold every pointer and size below: they are replayed
+    // only while all of them are unchanged
+    {
+        const double *bufs[] = {w.W[0].get(), w.W[1].get(), w.V[0].get(), w.V[1].get(),
+                                w.B.get(), w.BU.get(), w.BV.get(), w.BS.get(), w.svr.get(),
+                                w.T.get(), w.part.get(), w.part2.get(), w.red.get(), w.Fy.get(),
+                                w.F.get(), w.scal.get(), w.rpart.get()};
+        std::vector<uint64_t> sig;
+        for (const double *b : bufs) sig.push_back((uint64_t)(uintptr_t)b);
+        const void *ptrs[] = {w.flags.get(), w.info.get(), w.state.get(), mu_all,
+                              op.rowptr.get(), op.ccol.get(), op.cval.get(), op.off.get(),
+                              op.lrow.get(), op.vals, L.row_sample_d.get(), L.slab_row0_d.get(),
+                              L.slab_sample_d.get(), L.sample_slab0_d.get(),
+                              L.blk_sample_d.get(), L.blk_r0_d.get()};
+        for (const void *q : ptrs) sig.push_back((uint64_t)(uintptr_t)q);
+        uint64_t tolb, svb;
+        std::memcpy(&tolb, &tol, 8);
+        std::memcpy(&svb, &svtol, 8);
+        const uint64_t sc[] = {(uint64_t)M, (uint64_t)n, (uint64_t)S, (uint64_t)work,
+                               (uint64_t)nu, (uint64_t)maxit, tolb, svb, (uint64_t)G,
+                               (uint64_t)ng, (uint64_t)nslab, (uint64_t)nblk_av};
+        sig.insert(sig.end(), std::begin(sc), std::end(sc));
+        if (sig != w.sig) {
+            for (auto &kv : w.graphs) (void)hipGraphExecDestroy(kv.second);
+            w.graphs.clear();
+            w.sig = sig;
+        }
+    }
+
+    auto round = [&](int jmin, int cur) {
+        double *Wc = w.W[cur].get(), *Wn = w.W[cur ^ 1].get();
+        double *Vc = w.V[cur].get(), *Vn = w.V[cur ^ 1].get();
+        for (int j = jmin; j < work; ++j) {
+            double *y = Wc + (int64_t)j * M;
+            switch (G) {
+                case 4:
+                    hipLaunchKernelGGL(k_b_av_bf16<4>, dim3((unsigned)nblk_av), dim3(kBlock),
+                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
+                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
+                                       w.scal.get(), Wc, y);
+                    break;
+                case 8:
+                    hipLaunchKernelGGL(k_b_av_bf16<8>, dim3((unsigned)nblk_av), dim3(kBlock),
+                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
+                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
+                                       w.scal.get(), Wc, y);
+                    break;
+                case 16:
+                    hipLaunchKernelGGL(k_b_av_bf16<16>, dim3((unsigned)nblk_av), dim3(kBlock),
+                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
+                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
+                                       w.scal.get(), Wc, y);
+                    break;
+                default:
+                    hipLaunchKernelGGL(k_b_av_bf16<32>, dim3((unsigned)nblk_av), dim3(kBlock),
+                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
+                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
+                                       w.scal.get(), Wc, y);
+                    break;
+            }
+            SEV_LAUNCH_CHECK();
+            if (j > 0) {
+                hipLaunchKernelGGL(k_b_dot_w, dim3((unsigned)nslab), dim3(kDotThreads), 0, st, d, Wc,
+                                   j, y, w.red.get());
+                SEV_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_b_red_t, dim3(j, (unsigned)S), dim3(kBlock), 0, st, d, j,
+                                   w.red.get(), w.T.get());
+                SEV_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(k_b_apply, dim3((unsigned)nslab), dim3(kApplyThreads), 0, st, d, Wc,
+                               j, w.T.get(), y, w.Fy.get(), w.part.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_atw_bf16, dim3((unsigned)nslab, ng), dim3(kBlock), 0, st, d, j,
+                               op.off.get(), op.lrow.get(), (const uint16_t *)op.vals, w.Fy.get(),
+                               w.part.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_red_slabs,
+                               dim3((unsigned)ceil_div(n + 2, kBlock), kRedChunks, (unsigned)S),
+                               dim3(kBlock), 0, st, d, j, w.part.get(), w.part2.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_right_a, dim3(nblk_rs, (unsigned)S), dim3(kRsBlock), 0, st, d,
+                               j, w.part2.get(), mu_all, Vc, w.F.get(), w.rpart.get(), nblk_rs);
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_right_b, dim3(nblk_rs, (unsigned)S), dim3(kRsBlock), 0, st, d,
+                               j, nblk_rs, w.rpart.get(), Vc, w.F.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_right_c, dim3((unsigned)S), dim3(1024), 0, st, d, j, nblk_rs,
+                               w.part2.get(), w.rpart.get(), Vc, w.F.get(), w.B.get(),
+                               w.scal.get(), w.flags.get());
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_scale_col, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, j,
+                               w.Fy.get(), Wc, w.scal.get());
+            SEV_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_b_small_svd, dim3((unsigned)S), dim3(kSvdThreads), svd_lds, st, d,
+                           w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get());
+        SEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_b_restart_check, dim3((unsigned)S), dim3(kBlock), 0, st, d, nu, maxit,
+                           tol, svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.svr.get(),
+                           w.B.get(), w.info.get(), w.flags.get());
+        SEV_LAUNCH_CHECK();
+        // restart of the continuing samples into the other buffers
+        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(S * n)), dim3(kBlock), 0, st, d, 1, Vc, Vc,
+                           (const int *)nullptr, Vn, (int64_t)work, w.BV.get(), 0, w.F.get(),
+                           w.scal.get(), true);
+        SEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, 0, Wc, Wc,
+                           (const int *)nullptr, Wn, M, w.BU.get(), 0, (const double *)nullptr,
+                           w.scal.get(), true);
+        SEV_LAUNCH_CHECK();
+    };
+
+    std::vector<int> fin_buf((size_t)S, -1);
+    int cur = 0, jmin = 0, nround = 0;
+    int64_t nmult = 0;
+    std::vector<int> j0_prev((size_t)S, 0);
+    Prof *saved = t_prof;
+    while (true) {
+        {
+            ProfScope ps("batch_round", st);
+            if (graph) {
+                const auto key = std::make_pair(jmin, cur);
+                auto it = w.graphs.find(key);
+                if (it == w.graphs.end()) {
+                    t_prof = nullptr;  // no profiler events inside the capture
+                    hipGraph_t g;
+                    SEV_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    round(jmin, cur);
+                    SEV_HIP(hipStreamEndCapture(st, &g));
+                    hipGraphExec_t ge;
+                    SEV_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                    (void)hipGraphDestroy(g);
+                    it = w.graphs.emplace(key, ge).first;
+                    t_prof = saved;
+                }
+                SEV_HIP(hipGraphLaunch(it->second, st));
+            } else {
+                round(jmin, cur);
+            }
+        }
+        nround++;
+        SEV_HIP(hipMemcpyAsync(st_h.data(), state, sizeof(int) * st_h.size(),
+                               hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        bool any = false;
+        jmin = work;
+        for (int64_t s = 0; s < S; ++s) {
+            const int done = st_h[(size_t)(S + s)];
+            if (fin_buf[(size_t)s] < 0) nmult += 2 * (work - j0_prev[(size_t)s]);
+            if (done && fin_buf[(size_t)s] < 0) fin_buf[(size_t)s] = cur;
+            if (!done) {
+                any = true;
+                jmin = std::min(jmin, st_h[(size_t)s]);
+                j0_prev[(size_t)s] = st_h[(size_t)s];
+            }
+        }
+        if (!any) break;
+        cur ^= 1;
+    }
+    t_prof = saved;
+    for (int64_t s = 0; s < S; ++s) {
+        const int f = st_h[(size_t)(3 * S + s)];
+        if (iters_out) iters_out[s] = st_h[(size_t)(2 * S + s)];
+        if (f) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "batched irlba: sample %lld %s", (long long)s,
+                          f == 2 ? "did not converge (maxit reached)"
+                                 : f == 1 ? "hit an invariant subspace (rank-deficient operator)"
+                                          : f == 3 ? "had a zero residual at restart"
+                                                   : "needs a singular-vector completion");
+            fail(kErrConvergence, msg);
+        }
+    }
+    if (nmult_out) *nmult_out = nmult;
+    // final vectors: U = W BU[:, 0:nu] and V = V BV[:, 0:nu] from the buffer
+    // that held each sample when it finished
+    DevBuf<int> sel;
+    sel.ensure((size_t)S);
+    SEV_HIP(hipMemcpyAsync(sel.get(), fin_buf.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
+    {
+        ProfScope ps("batch_final", st);
+        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, 0,
+                           (const double *)w.W[0].get(), (const double *)w.W[1].get(),
+                           (const int *)sel.get(), U, M, w.BU.get(), nu, (const double *)nullptr,
+                           w.scal.get(), false);
+        SEV_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(S * n)), dim3(kBlock), 0, st, d, 1,
+                           (const double *)w.V[0].get(), (const double *)w.V[1].get(),
+                           (const int *)sel.get(), V, (int64_t)nu, w.BV.get(), nu,
+                           (const double *)nullptr, w.scal.get(), false);
+        SEV_LAUNCH_CHECK();
+    }
+    std::vector<double> bs((size_t)(S * kMaxWork));
+    SEV_HIP(hipMemcpyAsync(bs.data(), w.BS.get(), sizeof(double) * bs.size(),
+                           hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    for (int64_t s = 0; s < S; ++s)
+        for (int i = 0; i < nu; ++i) s_out[s * nu + i] = bs[(size_t)(s * kMaxWork + i)];
+}
+
+void batch_embedding(const double *U, const double *s_dev, const BatchLayout &L, int64_t dd,
+                     float *Z, int64_t rs, int64_t cs, hipStream_t st) {
+    hipLaunchKernelGGL(k_b_embed, dim3(grid_rows(L.M)), dim3(kBlock), 0, st, U, s_dev,
+                       L.row_sample_d.get(), L.M, dd, Z, rs, cs);
     SEV_LAUNCH_CHECK();
 }
 

@@ -6,6 +6,10 @@
 #include "sev_common.hpp"
 #include "sev_prof.hpp"
 
+#include <map>
+#include <utility>
+#include <vector>
+
 namespace sev {
 
 // ---- sev_scan.hip --------------------------------------------------------
@@ -61,6 +65,33 @@ void scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T s
 template <class T>
 void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, const T *clip,
                T *out, hipStream_t st);
+
+// Batches of S samples stacked by rows of one cells x genes CSC.
+// soff[s*G + g] (s = 0..S): first entry of column g in sample >= s.
+void sample_offsets(int64_t G, int64_t S, const int32_t *row_sample, const int64_t *colptr,
+                    const int32_t *row, int64_t *soff, hipStream_t st);
+// Virtual column v = j*S + s is gene feat[s*F + j] restricted to sample s:
+// src_b[v] = its first entry, len[v] = its length.
+void batch_seg_lengths(int64_t F, int64_t S, int64_t G, const int64_t *feat, const int64_t *soff,
+                       int64_t *src_b, int64_t *len, hipStream_t st);
+void batch_gather(int64_t nv, const int64_t *src_b, const int64_t *vcp, const int32_t *row,
+                  const float *val, int32_t *orow, float *oval, hipStream_t st);
+// scale_stats over virtual columns, m of column v = cells of sample v % S.
+void batch_scale_stats(int64_t ncol, const int64_t *colptr, const float *x, float scale_max,
+                       float *sd, float *mup, float *clip, const int64_t *row0, int64_t S,
+                       hipStream_t st);
+// out[j] = in[j*S], j = 0..n
+void stride_ptr(const int64_t *in, int64_t n, int64_t S, int64_t *out, hipStream_t st);
+// mu_all[s*F + j] = mup[j*S + s]
+void mu_by_sample(const float *mup, int64_t F, int64_t S, float *mu_all, hipStream_t st);
+// Stacking S sample CSCs (G columns each, device column pointers cps_d[s]):
+// len[g*S + s] = entries of column g in sample s; after an exclusive scan
+// into starts, stack_copy places sample s (rows offset by roff).
+void stack_lengths(int64_t G, int64_t S, const int64_t *const *cps_d, int64_t *len,
+                   hipStream_t st);
+void stack_copy(int64_t G, int64_t S, int64_t s, const int64_t *cp, const int32_t *row,
+                const int32_t *cnt, int64_t roff, const int64_t *starts, int32_t *orow,
+                int32_t *ocnt, hipStream_t st);
 
 }  // namespace sev
 
@@ -162,6 +193,49 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                   double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
                   const HostOperator *hop = nullptr);
 
+// ---- batched IRLBA ---------------------------------------------------------
+// S independent problems whose rows are stacked in one matrix (sample s owns
+// rows row0[s] .. row0[s+1]-1; all share the n feature columns, each with its
+// own centering).  The operator is a SlabCsc built on the stacked matrix with
+// slabs that never straddle a sample (BatchLayout::slab_row0).
+struct BatchLayout {
+    int64_t S = 0, M = 0;
+    std::vector<int64_t> row0;              // S+1 (host)
+    std::vector<int64_t> slab_row0;          // nslab+1 (host)
+    std::vector<int32_t> slab_sample;        // nslab
+    std::vector<int64_t> sample_slab0;       // S+1
+    std::vector<int32_t> blk_sample;         // A v row blocks
+    std::vector<int64_t> blk_r0;             // nblk+1
+    DevBuf<int64_t> row0_d, slab_row0_d, sample_slab0_d, blk_r0_d;
+    DevBuf<int32_t> row_sample_d, slab_sample_d, blk_sample_d;
+};
+// Partition rows (cells per sample) into slabs of at most kSlabRows rows and
+// A v blocks of at most 1024 rows inside each sample; uploads the tables.
+void batch_layout(BatchLayout &L, const std::vector<int64_t> &cells, hipStream_t st);
+// Rebuild op.off / op.lrow / op.nslab of a built operator for the layout's
+// slabs (A' w partials per slab never mix samples).
+void batch_slabs(SlabCsc &op, const BatchLayout &L, const int64_t *colptr, const int32_t *row,
+                 hipStream_t st);
+
+struct BatchIrlbaWork {
+    DevBuf<double> W[2], V[2], B, BU, BV, BS, svr, T, part, part2, red, Fy, F, scal, rpart;
+    DevBuf<int> flags, info, state;  // state: j0, done, iters, fail per sample
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (first step, buffer parity)
+    std::vector<uint64_t> sig;  // pointers and sizes the graphs were recorded with
+    ~BatchIrlbaWork();
+};
+// Batched IRLBA (mu_all: S x n centering).  init (host, S x n) are the
+// starting vectors.  Outputs: s (host, S x nu), U (device, M x nu column-major
+// over stacked rows), V (device, S x n x nu), iters (host, S).  graph: record
+// each restart round once as a hipGraph and replay it.
+void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
+                 const IrlbaParams &p, const double *init, BatchIrlbaWork &w, double *s,
+                 double *U, double *V, int64_t *iters, int64_t *nmult, bool graph,
+                 hipStream_t st);
+// Z[r*rs + c*cs] = float(U[c*M + r] * s[sample(r)*d + c]) for c < d.
+void batch_embedding(const double *U, const double *s_dev, const BatchLayout &L, int64_t d,
+                     float *Z, int64_t rs, int64_t cs, hipStream_t st);
+
 // Z[r*rs + c*cs] = float(U[c*m + r] * s[c]) for c < d: the embedding
 // U*diag(s) (src/embedding.jl:67) rounded to Float32.
 void embedding_f32(const double *U, const double *s_dev, int64_t m, int64_t d, float *Z,
@@ -192,6 +266,13 @@ template <class T>
 void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                     const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st);
 
+// Batches: queries qidx (scattered into idx/dist rows) search only the rows
+// of their own sample (seg = row0 of the samples, row_sample = sample of each
+// row); qidx lists every query of a 16-row block from one sample.
+void knn_exact_rows_batch(const float *X, int64_t n, int64_t d, int64_t ld, int64_t k,
+                          int include_self, const int64_t *qidx, int64_t nq, const int64_t *seg,
+                          const int32_t *row_sample, int32_t *idx, float *dist, hipStream_t st);
+
 // ---- sev_knn_mfma.hip ----------------------------------------------------
 // The same exact kNN (FP32 definition) by MFMA candidate search (FP32 or
 // BF16 operands), exact FP32 re-rank and a certificate; uncertified rows are
@@ -202,11 +283,16 @@ struct KnnWork {
     DevBuf<float> zr, nhat, eps, cthr, pval, pthr;
     DevBuf<unsigned> gmax;
     DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt;
-    DevBuf<int64_t> fail, fail2;
+    DevBuf<int64_t> fail, fail2, qpad;
 };
 void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
               int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,
               int64_t *n_fallback);
+// The same per sample of a batch (every row's neighbours are rows of its own
+// sample; indices stay global, 0-based).
+void knn_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
+                    int include_self, int bf16, const BatchLayout &L, KnnWork &w, int32_t *idx,
+                    float *dist, hipStream_t st, int64_t *n_fallback);
 // Largest deviation of the MFMA lane-layout model from scalar arithmetic
 // (mode 0: 16x16x4 FP32, 1: 16x16x32 BF16); 0 when the model holds.
 double mfma_selftest(int mode);  // mode 2: 16x16x4 FP64

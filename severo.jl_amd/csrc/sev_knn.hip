@@ -43,7 +43,8 @@ __device__ __forceinline__ double inf_val<double>() {
 template <class T, int kQB>
 __global__ void __launch_bounds__(kBlock)
 k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_self,
-            const int64_t *qidx, int64_t nq, int scatter, int32_t *oidx, T *odist) {
+            const int64_t *qidx, int64_t nq, int scatter, int32_t *oidx, T *odist,
+            const int64_t *seg, const int32_t *row_sample) {
     extern __shared__ unsigned char smem_raw[];
     T *qx = (T *)smem_raw;  // kQB * d
     __shared__ T pendD[kQB][kBlock];
@@ -72,10 +73,18 @@ k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_sel
     }
     __syncthreads();
 
+    // candidates: all n points, or (batches) the rows of the sample of the
+    // block's first query -- every query of a block is from that sample
+    int64_t cb = 0, ce = n;
+    if (seg) {
+        const int32_t s = row_sample[qrow[0]];
+        cb = seg[s];
+        ce = seg[s + 1];
+    }
     const int lane = lane_id(), w = threadIdx.x / kWave;
-    for (int64_t tile = 0; tile < n; tile += kBlock) {
+    for (int64_t tile = cb; tile < ce; tile += kBlock) {
         const int64_t j = tile + threadIdx.x;
-        if (j < n) {
+        if (j < ce) {
             T D[kQB];
 #pragma unroll
             for (int q = 0; q < kQB; ++q) D[q] = T(0);
@@ -149,7 +158,8 @@ k_knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int k, int include_sel
 template <class T>
 void knn_exact_impl(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                     const int64_t *qidx, int64_t nq, int scatter, int32_t *idx, T *dist,
-                    hipStream_t st) {
+                    hipStream_t st, const int64_t *seg = nullptr,
+                    const int32_t *row_sample = nullptr) {
     SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
     SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
     // 16 queries per block in FP32, 8 in FP64 (keeps LDS under 64 KB)
@@ -159,7 +169,8 @@ void knn_exact_impl(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int
     if (nq == 0) return;
     ProfScope ps("knn_exact", st, 0.0, 2.0 * (double)nq * (double)n * (double)d);
     hipLaunchKernelGGL((k_knn_exact<T, QB>), dim3((unsigned)ceil_div(nq, QB)), dim3(kBlock), smem,
-                       st, X, n, d, ld, (int)k, include_self, qidx, nq, scatter, idx, dist);
+                       st, X, n, d, ld, (int)k, include_self, qidx, nq, scatter, idx, dist, seg,
+                       row_sample);
     SEV_LAUNCH_CHECK();
 }
 
@@ -174,6 +185,14 @@ void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int
                     const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st) {
     SEV_REQUIRE(qidx != nullptr, "knn_exact_rows needs row indices");
     knn_exact_impl<T>(X, n, d, ld, k, include_self, qidx, nq, 1, idx, dist, st);
+}
+
+void knn_exact_rows_batch(const float *X, int64_t n, int64_t d, int64_t ld, int64_t k,
+                          int include_self, const int64_t *qidx, int64_t nq, const int64_t *seg,
+                          const int32_t *row_sample, int32_t *idx, float *dist, hipStream_t st) {
+    SEV_REQUIRE(qidx != nullptr && seg != nullptr, "knn_exact_rows_batch needs rows and samples");
+    knn_exact_impl<float>(X, n, d, ld, k, include_self, qidx, nq, 1, idx, dist, st, seg,
+                          row_sample);
 }
 
 template void knn_exact<float>(const float *, int64_t, int64_t, int64_t, int64_t, int,

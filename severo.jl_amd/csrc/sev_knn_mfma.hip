@@ -29,6 +29,8 @@
 #include <type_traits>
 #include <cstdlib>
 #include <cmath>
+#include <algorithm>
+#include <vector>
 
 namespace sev {
 
@@ -134,13 +136,17 @@ struct Elem<1> {
 };
 
 // Queries: rows q0..q0+QB-1 of the query list (qidx[i], or i itself when
-// qidx is null; nq entries).  Candidates: chunk blockIdx.y of gridDim.y equal
-// chunks of [0, n) (multiples of the tile).  Outputs at [chunk][list index]:
-// cidx (KL ids), cthr, ccnt, and the list values in cval when it is given.
+// qidx is null; nq entries; qidx[i] < 0 marks an unused slot).  Candidates:
+// chunk blockIdx.y of gridDim.y equal chunks of [0, n) (multiples of the
+// tile), or, with seg, of the rows seg[s] .. seg[s+1]-1 of the sample s of
+// the block's first query (batches: every query of a block is from s).
+// Outputs at [chunk][list index]: cidx (KL ids), cthr, ccnt, and the list
+// values in cval when it is given.
 template <int MODE, int KP, int QW, int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const int64_t *qidx,
-           int64_t nq, int32_t *cidx, float *cthr, int32_t *ccnt, float *cval) {
+           int64_t nq, int32_t *cidx, float *cthr, int32_t *ccnt, float *cval,
+           const int64_t *seg, const int32_t *row_sample) {
     typedef typename Elem<MODE>::T E;
     constexpr int QB = kNW * QW * 16;
     constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
@@ -168,10 +174,6 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
         const int64_t qi = q0 + i;
         qg[i] = qi < nq ? (int32_t)(qidx ? qidx[qi] : qi) : -1;
     }
-    // candidate chunk
-    const int64_t csz = ((n + gridDim.y - 1) / gridDim.y + kCT - 1) / kCT * kCT;
-    const int64_t cbeg = (int64_t)blockIdx.y * csz;
-    const int64_t cend = cbeg + csz < n ? cbeg + csz : n;
     // query operands and the running thresholds of this lane's output rows
     // (rows 4*grp + i of each row block) in registers; rows past n never
     // accept a candidate
@@ -179,6 +181,17 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     v8bf ab[QW][MODE == 1 ? KS : 1];
     float thr[QW][4];
     __syncthreads();
+    // candidate chunk
+    int64_t rb = 0, re = n;
+    if (seg) {
+        const int32_t q = qg[0];
+        const int32_t smp = q >= 0 ? row_sample[q] : 0;
+        rb = q >= 0 ? seg[smp] : 0;
+        re = q >= 0 ? seg[smp + 1] : 0;
+    }
+    const int64_t csz = ((re - rb + gridDim.y - 1) / gridDim.y + kCT - 1) / kCT * kCT;
+    const int64_t cbeg = rb + (int64_t)blockIdx.y * csz;
+    const int64_t cend = cbeg + csz < re ? cbeg + csz : re;
 #pragma unroll
     for (int rb = 0; rb < QW; ++rb) {
         const int32_t qpt = qg[w * QW * 16 + rb * 16 + col];
@@ -436,6 +449,7 @@ k_knn_rerank(const float *Zr, int64_t nq, const int64_t *qidx, int d, int k, con
     const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     if (qi >= nq) return;
     const int64_t q = qidx ? qidx[qi] : qi;
+    if (q < 0) return;  // unused slot of a padded query list
     const int cnt = ccnt[qi];
     const int32_t c = lane < cnt && lane < KL ? cidx[qi * KL + lane] : -1;
     float D = __builtin_huge_valf();
@@ -551,7 +565,8 @@ namespace {
 template <int MODE, int KP, int QW, int KL>
 void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                  const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
-                 int32_t *nfail, int64_t *fail, hipStream_t st) {
+                 int32_t *nfail, int64_t *fail, hipStream_t st, const int64_t *seg = nullptr,
+                 const int32_t *row_sample = nullptr, bool prep = true) {
     const size_t es = MODE == 0 ? 4 : 2;
     w.zq.ensure((size_t)n * KP * es);
     w.zc.ensure((size_t)n * KP * es);
@@ -559,8 +574,8 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     w.nhat.ensure((size_t)n);
     w.eps.ensure((size_t)n);
     w.gmax.ensure(2);
-    SEV_HIP(hipMemsetAsync(w.gmax.get(), 0, 2 * sizeof(unsigned), st));
-    {
+    if (prep) {
+        SEV_HIP(hipMemsetAsync(w.gmax.get(), 0, 2 * sizeof(unsigned), st));
         ProfScope ps("knn_prep", st, 4.0 * n * d * 2 + 2.0 * n * KP * es);
         const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
         hipLaunchKernelGGL((k_knn_prep<MODE, KP>), dim3(g), dim3(kBlock), 0, st, Z, n, d, ld,
@@ -586,7 +601,7 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
             hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks),
                                dim3(kBlock), 0, st, (const void *)w.zq.get(),
                                (const void *)w.zc.get(), n, include_self, qidx, nq, w.cidx.get(),
-                               w.cthr.get(), w.ccnt.get(), (float *)nullptr);
+                               w.cthr.get(), w.ccnt.get(), (float *)nullptr, seg, row_sample);
             SEV_LAUNCH_CHECK();
         } else if constexpr (2 * KL <= kWave) {
             w.pidx.ensure((size_t)nchunk * nq * KL);
@@ -596,7 +611,7 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
             hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks, nchunk),
                                dim3(kBlock), 0, st, (const void *)w.zq.get(),
                                (const void *)w.zc.get(), n, include_self, qidx, nq, w.pidx.get(),
-                               w.pthr.get(), w.pcnt.get(), w.pval.get());
+                               w.pthr.get(), w.pcnt.get(), w.pval.get(), seg, row_sample);
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL((k_knn_merge<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
                                dim3(kBlock), 0, st, nq, nchunk, w.pidx.get(), w.pval.get(),
@@ -685,6 +700,141 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
                 knn_exact<float>(Z, n, d, ld, k, include_self, nullptr, n, idx, dist, st);
                 if (n_fallback) *n_fallback = n;
             }
+        }
+    }
+}
+
+namespace {
+
+// Query list of a batch: the given rows (all rows when rows is null; else
+// sorted ascending) grouped by sample, each sample's group padded to a
+// multiple of QB -- with -1 (unused slots) or with copies of its last row.
+std::vector<int64_t> batch_queries(const BatchLayout &L, const int64_t *rows, int64_t nrows,
+                                   int QB, bool pad_copy) {
+    std::vector<int64_t> q;
+    auto pad = [&](int64_t last) {
+        while (q.size() % (size_t)QB) q.push_back(pad_copy ? last : -1);
+    };
+    if (!rows) {
+        q.reserve((size_t)(L.M + L.S * QB));
+        for (int64_t s = 0; s < L.S; ++s) {
+            for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; ++r) q.push_back(r);
+            pad(L.row0[(size_t)s + 1] - 1);
+        }
+        return q;
+    }
+    int64_t s = 0;
+    for (int64_t i = 0; i < nrows; ++i) {
+        const int64_t r = rows[i];
+        if (r >= L.row0[(size_t)s + 1]) {
+            if (!q.empty()) pad(q.back());
+            while (r >= L.row0[(size_t)s + 1]) ++s;
+        }
+        q.push_back(r);
+    }
+    if (!q.empty()) pad(q.back());
+    return q;
+}
+
+void upload_sync(DevBuf<int64_t> &b, const std::vector<int64_t> &h, hipStream_t st) {
+    b.ensure(h.size() ? h.size() : 1);
+    if (!h.empty())
+        SEV_HIP(hipMemcpyAsync(b.get(), h.data(), sizeof(int64_t) * h.size(),
+                               hipMemcpyHostToDevice, st));
+    SEV_HIP(hipStreamSynchronize(st));
+}
+
+std::vector<int64_t> read_fails(const int64_t *fail, int32_t nf, hipStream_t st) {
+    std::vector<int64_t> f((size_t)nf);
+    if (nf > 0)
+        SEV_HIP(hipMemcpyAsync(f.data(), fail, sizeof(int64_t) * (size_t)nf,
+                               hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    std::sort(f.begin(), f.end());
+    return f;
+}
+
+template <int MODE, int KP, int QW, int KL, int KP32>
+void run_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                    const BatchLayout &L, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,
+                    int64_t *n_fallback) {
+    const int64_t *seg = L.row0_d.get();
+    const int32_t *rs = L.row_sample_d.get();
+    w.nfail.ensure(2);
+    w.fail.ensure((size_t)n);
+    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 2 * sizeof(int32_t), st));
+    const std::vector<int64_t> q = batch_queries(L, nullptr, 0, kNW * QW * 16, false);
+    upload_sync(w.qpad, q, st);
+    cand_rerank<MODE, KP, QW, KL>(Z, n, d, ld, k, include_self, w.qpad.get(), (int64_t)q.size(),
+                                  w, idx, dist, w.nfail.get(), w.fail.get(), st, seg, rs);
+    int32_t nf = 0;
+    SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> f = read_fails(w.fail.get(), nf, st);
+    if constexpr (MODE == 1 && KP32 > 0) {
+        if (nf > 0) {
+            // BF16 rows without a certificate: FP32 candidates for those rows
+            const std::vector<int64_t> q2 = batch_queries(L, f.data(), nf, kNW * 2 * 16, false);
+            upload_sync(w.fail2, q2, st);
+            cand_rerank<0, KP32, 2, KL>(Z, n, d, ld, k, include_self, w.fail2.get(),
+                                        (int64_t)q2.size(), w, idx, dist, w.nfail.get() + 1,
+                                        w.fail.get(), st, seg, rs);
+            SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 1, sizeof nf, hipMemcpyDeviceToHost, st));
+            SEV_HIP(hipStreamSynchronize(st));
+            f = read_fails(w.fail.get(), nf, st);
+        }
+    }
+    if (nf > 0) {
+        const std::vector<int64_t> q3 = batch_queries(L, f.data(), nf, 16, true);
+        upload_sync(w.qpad, q3, st);
+        knn_exact_rows_batch(Z, n, d, ld, k, include_self, w.qpad.get(), (int64_t)q3.size(), seg,
+                             rs, idx, dist, st);
+    }
+    if (n_fallback) *n_fallback = nf;
+}
+
+}  // namespace
+
+void knn_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
+                    int include_self, int bf16, const BatchLayout &L, KnnWork &w, int32_t *idx,
+                    float *dist, hipStream_t st, int64_t *n_fallback) {
+    SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
+    SEV_REQUIRE(d >= 1, "knn: d must be >= 1");
+    SEV_REQUIRE(n == L.M, "knn: embedding rows differ from the batch");
+    SEV_REQUIRE(n < ((int64_t)1 << 31) - 256, "knn: at most 2^31 - 256 points");
+    for (int64_t s = 0; s < L.S; ++s)
+        SEV_REQUIRE(L.row0[(size_t)s + 1] - L.row0[(size_t)s] >= k + (include_self ? 0 : 1),
+                    "knn: k larger than the cells of a sample");
+    if (bf16) {
+        SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
+        if (k <= 24 && d + 2 <= 64 && d + 1 <= 52)
+            run_mfma_batch<1, 64, 2, 32, 52>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                             n_fallback);
+        else if (k <= 24 && d + 2 <= 64)
+            run_mfma_batch<1, 64, 2, 32, 128>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                              n_fallback);
+        else if (k <= 24)
+            run_mfma_batch<1, 128, 2, 32, 128>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                               n_fallback);
+        else if (d + 2 <= 64)
+            run_mfma_batch<1, 64, 1, 64, 0>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                            n_fallback);
+        else
+            run_mfma_batch<1, 128, 1, 64, 0>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                             n_fallback);
+    } else {
+        SEV_REQUIRE(d + 1 <= 128, "knn: FP32 MFMA form supports d <= 127");
+        if (d + 1 <= 52) {
+            if (k <= 24)
+                run_mfma_batch<0, 52, 2, 32, 52>(Z, n, d, ld, k, include_self, L, w, idx, dist,
+                                                 st, n_fallback);
+            else
+                run_mfma_batch<0, 52, 1, 64, 52>(Z, n, d, ld, k, include_self, L, w, idx, dist,
+                                                 st, n_fallback);
+        } else {
+            SEV_REQUIRE(k <= 24, "knn: batches with d > 51 need k <= 24 in the FP32 form");
+            run_mfma_batch<0, 128, 1, 32, 128>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                               n_fallback);
         }
     }
 }

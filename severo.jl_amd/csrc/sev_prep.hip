@@ -204,10 +204,15 @@ __device__ __forceinline__ double welford_quot(double delta, double, double coun
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scale_max, T *sdv,
-              T *mupv, T *clipv) {
+              T *mupv, T *clipv, const int64_t *row0, int64_t S) {
     const int64_t col = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     if (col >= ncol) return;
     const int lane = lane_id();
+    if (row0) {
+        // column (j, s) of a stacked batch: the rows of sample s = col % S
+        const int64_t s = col % S;
+        m = row0[s + 1] - row0[s];
+    }
     const int64_t b = colptr[col], e = colptr[col + 1];
     int64_t count = m - (e - b);
     const bool fast = sizeof(T) == 4;
@@ -359,7 +364,8 @@ void scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T s
     const int64_t threads = ncol * kWave;
     ProfScope ps("scale_stats", st);
     hipLaunchKernelGGL(k_scale_stats<T>, dim3((unsigned)ceil_div(threads, kBlock)), dim3(kBlock),
-                       0, st, m, ncol, colptr, x, scale_max, sd, mup, clip);
+                       0, st, m, ncol, colptr, x, scale_max, sd, mup, clip,
+                       (const int64_t *)nullptr, (int64_t)1);
     SEV_LAUNCH_CHECK();
 }
 
@@ -449,6 +455,157 @@ void rows_to_device(const int64_t *in, int64_t len, int64_t m, int32_t *out, int
                     hipStream_t st) {
     const unsigned g = (unsigned)std::min<int64_t>(ceil_div(len, kBlock), device_cu_count() * 16);
     hipLaunchKernelGGL(k_rows_to_device, dim3(g), dim3(kBlock), 0, st, in, len, m, out, bad);
+    SEV_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Batches: S samples stacked by rows (sample s owns rows row0[s] ..
+// row0[s+1]-1 of one cells x genes CSC).  Per-sample stages read the
+// segment of a column that belongs to one sample.
+// ---------------------------------------------------------------------------
+namespace {
+
+// soff[s*G + g] = first position of column g whose row belongs to sample
+// >= s, for s = 0..S (rows are sorted within each column)
+__global__ void __launch_bounds__(kBlock)
+k_sample_offsets(int64_t G, int64_t S, const int32_t *row_sample, const int64_t *colptr,
+                 const int32_t *row, int64_t *soff) {
+    const int64_t g = blockIdx.x;
+    const int64_t b = colptr[g], e = colptr[g + 1];
+    if (b == e) {
+        for (int64_t s = threadIdx.x; s <= S; s += kBlock) soff[s * G + g] = b;
+        return;
+    }
+    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+        const int64_t sc = row_sample[row[p]];
+        const int64_t sp = p == b ? -1 : (int64_t)row_sample[row[p - 1]];
+        for (int64_t s = sp + 1; s <= sc; ++s) soff[s * G + g] = p;
+        if (p == e - 1)
+            for (int64_t s = sc + 1; s <= S; ++s) soff[s * G + g] = e;
+    }
+}
+
+// virtual column v = j*S + s: the segment of gene feat[s*F + j] in sample s
+__global__ void __launch_bounds__(kBlock)
+k_seg_len(int64_t F, int64_t S, int64_t G, const int64_t *feat, const int64_t *soff,
+          int64_t *src_b, int64_t *len) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= F * S) return;
+    const int64_t j = v / S, s = v % S;
+    const int64_t g = feat[s * F + j];
+    const int64_t b = soff[s * G + g], e = soff[(s + 1) * G + g];
+    src_b[v] = b;
+    len[v] = e - b;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_gather_seg(const int64_t *src_b, const int64_t *vcp, const int32_t *row, const T *val,
+             int32_t *orow, T *oval) {
+    const int64_t v = blockIdx.x;
+    const int64_t b = src_b[v], ob = vcp[v], len = vcp[v + 1] - ob;
+    for (int64_t i = threadIdx.x; i < len; i += kBlock) {
+        orow[ob + i] = row[b + i];
+        oval[ob + i] = val[b + i];
+    }
+}
+
+// out[j] = in[j*S] for j = 0..n (column pointers of the n real columns from
+// those of the n*S virtual ones)
+__global__ void __launch_bounds__(kBlock)
+k_stride_ptr(const int64_t *in, int64_t n, int64_t S, int64_t *out) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j <= n) out[j] = in[j * S];
+}
+
+// mu_all[s*F + j] = mup[j*S + s]
+__global__ void __launch_bounds__(kBlock)
+k_mu_by_sample(const float *mup, int64_t F, int64_t S, float *mu_all) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= F * S) return;
+    const int64_t j = v / S, s = v % S;
+    mu_all[s * F + j] = mup[v];
+}
+
+// len[g*S + s] = entries of column g in sample s's own CSC
+__global__ void __launch_bounds__(kBlock)
+k_stack_len(int64_t G, int64_t S, const int64_t *const *cps, int64_t *len) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= G * S) return;
+    const int64_t g = i / S, s = i % S;
+    len[i] = cps[s][g + 1] - cps[s][g];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_stack_copy(int64_t S, int64_t s, const int64_t *cp, const int32_t *row, const int32_t *cnt,
+             int64_t roff, const int64_t *starts, int32_t *orow, int32_t *ocnt) {
+    const int64_t g = blockIdx.x;
+    const int64_t b = cp[g], len = cp[g + 1] - b, ob = starts[g * S + s];
+    for (int64_t i = threadIdx.x; i < len; i += kBlock) {
+        orow[ob + i] = row[b + i] + (int32_t)roff;
+        ocnt[ob + i] = cnt[b + i];
+    }
+}
+
+}  // namespace
+
+void sample_offsets(int64_t G, int64_t S, const int32_t *row_sample, const int64_t *colptr,
+                    const int32_t *row, int64_t *soff, hipStream_t st) {
+    hipLaunchKernelGGL(k_sample_offsets, dim3((unsigned)G), dim3(kBlock), 0, st, G, S, row_sample,
+                       colptr, row, soff);
+    SEV_LAUNCH_CHECK();
+}
+
+void batch_seg_lengths(int64_t F, int64_t S, int64_t G, const int64_t *feat, const int64_t *soff,
+                       int64_t *src_b, int64_t *len, hipStream_t st) {
+    hipLaunchKernelGGL(k_seg_len, dim3((unsigned)ceil_div(F * S, kBlock)), dim3(kBlock), 0, st, F,
+                       S, G, feat, soff, src_b, len);
+    SEV_LAUNCH_CHECK();
+}
+
+void batch_gather(int64_t nv, const int64_t *src_b, const int64_t *vcp, const int32_t *row,
+                  const float *val, int32_t *orow, float *oval, hipStream_t st) {
+    ProfScope ps("gather", st);
+    hipLaunchKernelGGL(k_gather_seg<float>, dim3((unsigned)nv), dim3(kBlock), 0, st, src_b, vcp,
+                       row, val, orow, oval);
+    SEV_LAUNCH_CHECK();
+}
+
+void batch_scale_stats(int64_t ncol, const int64_t *colptr, const float *x, float scale_max,
+                       float *sd, float *mup, float *clip, const int64_t *row0, int64_t S,
+                       hipStream_t st) {
+    const int64_t threads = ncol * kWave;
+    ProfScope ps("scale_stats", st);
+    hipLaunchKernelGGL(k_scale_stats<float>, dim3((unsigned)ceil_div(threads, kBlock)),
+                       dim3(kBlock), 0, st, (int64_t)0, ncol, colptr, x, scale_max, sd, mup, clip,
+                       row0, S);
+    SEV_LAUNCH_CHECK();
+}
+
+void stride_ptr(const int64_t *in, int64_t n, int64_t S, int64_t *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_stride_ptr, dim3((unsigned)ceil_div(n + 1, kBlock)), dim3(kBlock), 0, st,
+                       in, n, S, out);
+    SEV_LAUNCH_CHECK();
+}
+
+void mu_by_sample(const float *mup, int64_t F, int64_t S, float *mu_all, hipStream_t st) {
+    hipLaunchKernelGGL(k_mu_by_sample, dim3((unsigned)ceil_div(F * S, kBlock)), dim3(kBlock), 0,
+                       st, mup, F, S, mu_all);
+    SEV_LAUNCH_CHECK();
+}
+
+void stack_lengths(int64_t G, int64_t S, const int64_t *const *cps_d, int64_t *len,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_stack_len, dim3((unsigned)ceil_div(G * S, kBlock)), dim3(kBlock), 0, st,
+                       G, S, cps_d, len);
+    SEV_LAUNCH_CHECK();
+}
+
+void stack_copy(int64_t G, int64_t S, int64_t s, const int64_t *cp, const int32_t *row,
+                const int32_t *cnt, int64_t roff, const int64_t *starts, int32_t *orow,
+                int32_t *ocnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_stack_copy, dim3((unsigned)G), dim3(kBlock), 0, st, S, s, cp, row, cnt,
+                       roff, starts, orow, ocnt);
     SEV_LAUNCH_CHECK();
 }
 

@@ -32,6 +32,7 @@ class Pipeline:
         self.k = 0
         self.npcs = 0
         self.nfeat = 0
+        self.cells = None  # cells per sample of a batch
 
     def close(self):
         if self._h:
@@ -78,6 +79,7 @@ class Pipeline:
         vals = np.ascontiguousarray(X.data, np.int32)
         check(self._L.sev_pipeline_load_counts(self._h, X.shape[0], X.shape[1], _ptr(colptr),
                                                _ptr(rowval), _ptr(vals)))
+        self.cells = None
 
     def load_counts_julia(self, m, n, colptr, rowval, nzval):
         """Upload counts given in Julia's layout as they are: 1-based Int64
@@ -87,6 +89,7 @@ class Pipeline:
                 raise TypeError("load_counts_julia needs contiguous int64/int64/int32 arrays")
         check(self._L.sev_pipeline_load_counts(self._h, m, n, _ptr(colptr), _ptr(rowval),
                                                _ptr(nzval)))
+        self.cells = None
 
     def get_counts_julia(self):
         """The device counts in Julia's layout: (m, n, colptr, rowval, nzval),
@@ -103,6 +106,77 @@ class Pipeline:
         check(self._L.sev_pipeline_synth_counts(self._h, n_cells, tables.n_genes, seed, tables.r,
                                                 _ptr(tables.p0), _ptr(tables.q),
                                                 tables.boost_frac))
+        self.cells = None
+
+    # -- batches (samples stacked by rows) -----------------------------------
+    def synth_batch(self, cells, tables, seeds=None):
+        """Generate one synthetic sample per entry of ``cells`` (with the
+        matching ``tables[s]`` and seed) and stack them by rows."""
+        cells = np.ascontiguousarray(cells, np.int64)
+        S = len(cells)
+        if len(tables) != S:
+            raise ValueError("one table per sample")
+        n = tables[0].n_genes
+        if any(t.n_genes != n for t in tables):
+            raise ValueError("samples must share the genes")
+        seeds = np.ascontiguousarray([t.seed for t in tables] if seeds is None else seeds,
+                                     np.uint64)
+        r = np.ascontiguousarray([t.r for t in tables], np.float64)
+        p0 = np.ascontiguousarray(np.stack([t.p0 for t in tables]), np.float64)
+        q = np.ascontiguousarray(np.stack([t.q for t in tables]), np.float64)
+        bf = np.ascontiguousarray([t.boost_frac for t in tables], np.float64)
+        check(self._L.sev_pipeline_synth_batch(self._h, S, _ptr(cells), n, _ptr(seeds), _ptr(r),
+                                               _ptr(p0), _ptr(q), _ptr(bf)))
+        self.cells = cells
+
+    def set_batch(self, cells):
+        """Declare the loaded counts to be samples of ``cells`` rows each,
+        stacked in order (None clears it)."""
+        if cells is None:
+            check(self._L.sev_pipeline_set_batch(self._h, 0, None))
+            self.cells = None
+            return
+        cells = np.ascontiguousarray(cells, np.int64)
+        check(self._L.sev_pipeline_set_batch(self._h, len(cells), _ptr(cells)))
+        self.cells = cells
+
+    def select_scale_batch(self, features, scale_max=10.0):
+        """features: S x F 0-based gene indices (row s for sample s)."""
+        f = np.ascontiguousarray(np.asarray(features, np.int64) + 1)
+        if f.ndim != 2 or f.shape[0] != len(self.cells):
+            raise ValueError("features must be S x F")
+        check(self._L.sev_pipeline_select_scale_batch(self._h, f.shape[1], _ptr(f),
+                                                      float(scale_max)))
+        self.nfeat = f.shape[1]
+
+    def pca_batch(self, npcs=30, init=None, tol=1e-5, maxit=1000, graph=True, rng=None):
+        """One IRLBA per sample (BF16 operator values, FP64 vectors), in
+        lockstep; returns (iterations per sample, total products)."""
+        S = len(self.cells)
+        if init is None:
+            rng = np.random.default_rng(0) if rng is None else rng
+            init = rng.standard_normal((S, self.nfeat))
+        init = np.ascontiguousarray(init, np.float64)
+        if init.shape != (S, self.nfeat):
+            raise ValueError("init must be S x F")
+        it = np.zeros(S, np.int64)
+        nm = C.c_int64()
+        check(self._L.sev_pipeline_pca_batch(self._h, npcs, maxit, float(tol), _ptr(init),
+                                             int(bool(graph)), _ptr(it), C.byref(nm)))
+        self.npcs = min(npcs, int(self.cells.min()), self.nfeat)
+        return it, nm.value
+
+    def get_pca_batch(self):
+        """s (S x d), U (M x d), V (S x F x d), Z (M x d) for the batch."""
+        S = len(self.cells)
+        M = int(self.cells.sum())
+        d = self.npcs
+        s = np.zeros((S, d))
+        U = np.zeros((M, d), order="F")
+        V = np.zeros((S, d, self.nfeat))
+        Z = np.zeros((M, d), np.float32, order="F")
+        check(self._L.sev_pipeline_get_pca(self._h, _ptr(s), _ptr(U), _ptr(V), _ptr(Z)))
+        return s, U, np.ascontiguousarray(V.transpose(0, 2, 1)), Z
 
     def counts_info(self):
         m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
@@ -142,7 +216,8 @@ class Pipeline:
         colptr = np.zeros(self.nfeat + 1, np.int64)
         rowval = np.zeros(nnz.value, np.int64)
         vals = np.zeros(nnz.value, np.float32)
-        mu = np.zeros(self.nfeat, np.float32)
+        S = len(self.cells) if self.cells is not None else 0
+        mu = np.zeros((S, self.nfeat) if S else self.nfeat, np.float32)
         check(self._L.sev_pipeline_get_scaled(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals),
                                               _ptr(mu)))
         return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, self.nfeat)), mu
