@@ -13,7 +13,9 @@ largest generator mean):
 
 The headline value is the precision form (BF16 operator values for the
 Lanczos products, BF16 candidate search for kNN with exact FP32 re-rank);
-the plain FP32 form is timed in the same run.  Prints one JSON line.
+the plain FP32 form is timed in the same run.  The batching form (64 samples
+of uneven size, 20,000 genes, 30 PCs, k = 15) is timed after it, against the
+single-sample pipeline looped over the same samples.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -154,10 +156,113 @@ def fp64_errors(sev, p, npcs):
     return errors
 
 
-def cpu_baseline(sev, orc, w, sample_cells):
+BATCH = dict(samples=64, genes=20000, cells=(500, 50000), density=(0.02, 0.08), features=2000,
+             npcs=30, k=15, seed=1000)
+
+
+def batch_leg(sev, steps, warmup, graph=True):
+    """The batching form: 64 samples over one 20,000-gene set, cells
+    log-uniform in [500, 50000], densities uniform in [2%, 8%], seed per
+    sample = 1000 + s; per-sample normalise, 2000 features (each sample's
+    top generator means), IRLBA 30 components (BF16 operator values, FP64
+    vectors), kNN k = 15, SNN prune 1/15.  The batch runs all samples in one
+    stacked launch sequence (restart rounds replayed from HIP graphs); the
+    comparison loops the single-sample pipeline over the samples."""
+    b = BATCH
+    rng = np.random.default_rng(b["seed"])
+    S = b["samples"]
+    lo, hi = b["cells"]
+    cells = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), S))).astype(np.int64)
+    dens = rng.uniform(b["density"][0], b["density"][1], S)
+    tabs = [sev.nb_tables(b["genes"], seed=b["seed"] + s, density=float(dens[s]))
+            for s in range(S)]
+    feats = np.stack([sev.top_mean_features(t, b["features"]) for t in tabs])
+    init = rng.standard_normal((S, b["features"]))
+    M = int(cells.sum())
+
+    pb = sev.Pipeline()
+    pb.synth_batch(cells, tabs)
+    _, _, nnz = pb.counts_info()
+
+    def batched():
+        pb.normalize("lognormalize", SF)
+        pb.select_scale_batch(feats, SCALE_MAX)
+        it, nm = pb.pca_batch(b["npcs"], init=init, tol=1e-5, graph=graph)
+        nfb = pb.neighbours(b["k"], include_self=True, precision="bf16")
+        nz = pb.snn(PRUNE, np.float32)
+        pb.synchronize()
+        return it, nm, nfb, nz
+
+    for _ in range(warmup):
+        batched()
+    pb.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, nm, nfb, nz = batched()
+    dt_b = (time.perf_counter() - t0) / steps
+    prof = pb.profile_read()
+    pb.profile(False)
+    for name, (l, ms, by, fl) in sorted(prof.items(), key=lambda kv: -kv[1][1])[:12]:
+        print(f"#   batch {name:14s} {l // steps:6d} launches/step {ms / steps:9.2f} ms/step",
+              file=sys.stderr, flush=True)
+    pb.close()
+
+    # the same samples through the single-sample pipeline, one after another
+    ps = []
+    for s in range(S):
+        q = sev.Pipeline()
+        q.synth_counts(int(cells[s]), tabs[s])
+        ps.append(q)
+
+    def looped(form):
+        its = []
+        for s, q in enumerate(ps):
+            q.normalize("lognormalize", SF)
+            q.select_scale(feats[s], SCALE_MAX)
+            its.append(q.pca(b["npcs"], init=init[s], tol=1e-5, precision=form)[0])
+            q.neighbours(b["k"], include_self=True, precision=form)
+            q.snn(PRUNE, np.float32)
+        for q in ps:
+            q.synchronize()
+        return its
+
+    # one warm-up pass and one timed pass each (a pass is 64 pipelines)
+    loops = {}
+    for form in ("fp32", "bf16"):
+        looped(form)
+        t0 = time.perf_counter()
+        looped(form)
+        loops[form] = time.perf_counter() - t0
+    for q in ps:
+        q.close()
+    out = {
+        "workload": f"{S} samples x {b['genes']} genes, cells log-uniform in [{lo}, {hi}] "
+                    f"(total {M}), density U({b['density'][0]}, {b['density'][1]}), "
+                    f"{b['features']} features, {b['npcs']} PCs, k={b['k']}, prune 1/15",
+        "samples": S, "cells": M, "nnz": int(nnz),
+        "ms_per_step": round(dt_b * 1e3, 3), "cells_per_s": round(M / dt_b, 2),
+        "form": "bf16 operator values, bf16 kNN candidates, one stacked launch sequence, "
+                + ("restart rounds replayed from HIP graphs" if graph else
+                   "restart rounds launched directly"),
+        "irlba_iters": [int(it.min()), int(np.median(it)), int(it.max())],
+        "irlba_products": int(nm), "knn_fallback_rows": int(nfb), "snn_nnz": int(nz),
+        "loop_plain_fp32": {"ms_per_step": round(loops["fp32"] * 1e3, 3),
+                            "cells_per_s": round(M / loops["fp32"], 2)},
+        "loop_bf16": {"ms_per_step": round(loops["bf16"] * 1e3, 3),
+                      "cells_per_s": round(M / loops["bf16"], 2)},
+        "speedup_vs_loop_plain_fp32": round(loops["fp32"] / dt_b, 3),
+        "speedup_vs_loop_bf16": round(loops["bf16"] / dt_b, 3),
+    }
+    print(f"# batch: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_baseline(sev, orc, w, sample_cells, threads=1):
     """The oracle's scalar C restatement of the same path on a bounded
     sample of the workload (same generator tables, first `sample_cells`
-    cells' worth of draws), single thread."""
+    cells' worth of draws).  threads > 1 runs the kNN stage (the only
+    threaded one) on that many threads; the other stages stay scalar, as
+    the reference's Julia loops are."""
     t = sev.nb_tables(w["genes"], seed=0, density=w["density"], boost_frac=w["boost"])
     cp, rv, cnt = orc.synth_csc(sample_cells, t)
     feats = sev.top_mean_features(t, NFEAT)
@@ -168,12 +273,15 @@ def cpu_baseline(sev, orc, w, sample_cells):
     b, mu, _ = orc.scale(sample_cells, fcp, fval, np.float32(SCALE_MAX))
     rc, s, U, V, it, nm = orc.irlba_csc(sample_cells, fcp, frow, b, mu, NPCS, init, tol=1e-5)
     Z = np.asfortranarray((U * s).astype(np.float32))
-    idx, _ = orc.knn(Z, K, nthreads=1)
+    idx, _ = orc.knn(Z, K, nthreads=threads)
     orc.snn(idx, K, PRUNE, np.float32)
     dt = time.perf_counter() - t0
-    return {"value": round(sample_cells / dt, 2), "unit": "cells/s", "cores": 1, "kind": "port",
+    return {"value": round(sample_cells / dt, 2), "unit": "cells/s", "cores": threads,
+            "kind": "port",
             "sample": f"{sample_cells} cells of the same generator and genes, whole path, "
-                      f"{dt:.1f} s (kNN is O(cells^2), so cells/s falls further at 1M cells)"}
+                      f"{dt:.1f} s (kNN is O(cells^2), so cells/s falls further at 1M cells)"
+                      + (f"; kNN on {threads} threads, other stages scalar" if threads > 1 else
+                         "")}
 
 
 def main():
@@ -186,6 +294,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive step")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
+    ap.add_argument("--batch-direct", action="store_true",
+                    help="launch the batch's restart rounds directly instead of from HIP graphs")
     ap.add_argument("--errors", action="store_true",
                     help="report singular-value and subspace errors against an FP64 Gram "
                          "eigendecomposition (host BLAS; adds about a minute at 1M cells)")
@@ -265,9 +376,16 @@ def main():
     if not args.no_e2e:
         out["end_to_end"] = end_to_end(sev, p, feats, init, head, m)
         print(f"# end to end ({head}): {out['end_to_end']}", file=sys.stderr, flush=True)
+    p.close()
+    if not args.no_batch:
+        out["batch"] = batch_leg(sev, max(1, min(args.steps, 2)), max(1, args.warmup),
+                                 graph=not args.batch_direct)
     if not args.no_cpu:
         orc = entry.load_oracle()
         out["cpu_baseline"] = cpu_baseline(sev, orc, w, args.cpu_sample)
+        nthr = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        if nthr > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline(sev, orc, w, args.cpu_sample, nthr)
     print(json.dumps(out))
 
 
