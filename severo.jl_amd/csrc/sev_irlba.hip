@@ -1959,6 +1959,82 @@ k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel,
     }
 }
 
+// The same on MFMA (the tiles of k_gemm_mfma): W side (vmode 0) one block
+// per A v row block (rows of one sample, ld M); V side (vmode 1) block
+// (x, s) takes rows 256x .. of sample s's n x work block (ld n).
+__global__ void __launch_bounds__(kBlock)
+k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
+              int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
+              bool restart) {
+    constexpr int LD = kMaxWork + 1;
+    __shared__ double cfl[kMaxWork * LD];
+    const int work = d.work;
+    int s;
+    int64_t rb0, re0, ld;
+    const double *X;
+    double *Ys;
+    if (vmode) {
+        s = blockIdx.y;
+        rb0 = (int64_t)blockIdx.x * 256;
+        re0 = rb0 + 256 < d.n ? rb0 + 256 : d.n;
+        ld = d.n;
+    } else {
+        s = d.blk_sample[blockIdx.x];
+        rb0 = d.blk_r0[blockIdx.x];
+        re0 = d.blk_r0[blockIdx.x + 1];
+        ld = d.M;
+    }
+    if (restart && d.done[s]) return;
+    X = (xsel && xsel[s]) ? X1 : X0;
+    if (vmode) {
+        X += (int64_t)s * work * d.n;
+        Ys = Y + (int64_t)s * ldy * d.n;
+    } else {
+        Ys = Y;
+    }
+    const int kout = kfix > 0 ? kfix : d.j0[s];
+    const double *Cf = Call + (int64_t)s * work * work;
+    for (int i = threadIdx.x; i < kMaxWork * kMaxWork; i += kBlock) {
+        const int c = i / kMaxWork, l = i % kMaxWork;
+        cfl[c * LD + l] = (c < kout && l < work) ? Cf[c * work + l] : 0.0;
+    }
+    __syncthreads();
+    const int w = wave_index(), lane = lane_id();
+    const int nr = lane & 15, kq = lane >> 4;
+    const int ncb = (kout + 15) / 16;
+    const int nks = (work + 3) / 4;
+    for (int64_t t0 = rb0 + (int64_t)w * 16; t0 < re0; t0 += (int64_t)kWaves * 16) {
+        const int64_t row = t0 + nr;
+        const bool ok = row < re0;
+        v4d acc[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int st = 0; st < nks; ++st) {
+            const int k = 4 * st + kq;
+            const double bv = (k < work && ok) ? X[(int64_t)k * ld + row] : 0.0;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                if (cb < ncb) {
+                    const double av = cfl[(cb * 16 + nr) * LD + k];
+                    acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+                }
+            }
+        }
+        if (ok) {
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cb * 16 + kq + 4 * i;
+                    if (cb < ncb && c < kout) Ys[(int64_t)c * ld + row] = acc[cb][i];
+                }
+            }
+            if (vmode && restart && kq == 0)
+                Ys[(int64_t)kout * ld + row] = Fall[(int64_t)s * d.n + row] / scal[4 * s + 2];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_b_embed(const double *U, const double *s_all, const int32_t *row_sample, int64_t M, int64_t dd,
           float *Z, int64_t rs, int64_t cs) {
@@ -2027,7 +2103,7 @@ void batch_layout(BatchLayout &L, const std::vector<int64_t> &cells, hipStream_t
     L.sample_slab0.assign((size_t)L.S + 1, 0);
     L.blk_sample.clear();
     L.blk_r0.clear();
-    constexpr int64_t kBlkRows = 1024;
+    constexpr int64_t kBlkRows = 256;  // A v and restart row blocks
     for (int64_t s = 0; s < L.S; ++s) {
         L.sample_slab0[(size_t)s] = (int64_t)L.slab_sample.size();
         for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; r += kSlabRows) {
@@ -2179,7 +2255,9 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     d.fail = state + 3 * S;
     const int64_t nblk_av = (int64_t)L.blk_sample.size();
     const int G = op.lanes_per_row;
-    const int ng = atw_groups(op);
+    // A' w column groups: enough blocks per slab for the late rounds, when
+    // only the slabs of a few samples are active
+    const int ng = (int)std::max<int64_t>(1, std::min<int64_t>(16, n / (4 * kWaves)));
     const double tol = p.tol, svtol = p.svtol;
     const int maxit = (int)std::min<int64_t>(p.maxit, 1 << 30);
     // recorded graphs hold every pointer and size below: they are replayed
@@ -2284,13 +2362,13 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                            w.B.get(), w.info.get(), w.flags.get());
         SEV_LAUNCH_CHECK();
         // restart of the continuing samples into the other buffers
-        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(S * n)), dim3(kBlock), 0, st, d, 1, Vc, Vc,
-                           (const int *)nullptr, Vn, (int64_t)work, w.BV.get(), 0, w.F.get(),
-                           w.scal.get(), true);
+        hipLaunchKernelGGL(k_b_gemm_mfma, dim3((unsigned)ceil_div(n, 256), (unsigned)S),
+                           dim3(kBlock), 0, st, d, 1, Vc, Vc, (const int *)nullptr, Vn,
+                           (int64_t)work, w.BV.get(), 0, w.F.get(), w.scal.get(), true);
         SEV_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, 0, Wc, Wc,
-                           (const int *)nullptr, Wn, M, w.BU.get(), 0, (const double *)nullptr,
-                           w.scal.get(), true);
+        hipLaunchKernelGGL(k_b_gemm_mfma, dim3((unsigned)nblk_av), dim3(kBlock), 0, st, d, 0, Wc,
+                           Wc, (const int *)nullptr, Wn, M, w.BU.get(), 0,
+                           (const double *)nullptr, w.scal.get(), true);
         SEV_LAUNCH_CHECK();
     };
 
@@ -2342,6 +2420,11 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
         cur ^= 1;
     }
     t_prof = saved;
+    if (std::getenv("SEV_VERBOSE"))
+        std::fprintf(stderr, "[severo_hip] batched irlba: %lld samples, %lld rows x %lld, nu=%d "
+                     "work=%d: %d rounds, %lld products, %zu graphs\n", (long long)S,
+                     (long long)M, (long long)n, nu, work, nround, (long long)nmult,
+                     w.graphs.size());
     for (int64_t s = 0; s < S; ++s) {
         const int f = st_h[(size_t)(3 * S + s)];
         if (iters_out) iters_out[s] = st_h[(size_t)(2 * S + s)];

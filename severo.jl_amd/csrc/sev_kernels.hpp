@@ -210,7 +210,7 @@ struct BatchLayout {
     DevBuf<int32_t> row_sample_d, slab_sample_d, blk_sample_d;
 };
 // Partition rows (cells per sample) into slabs of at most kSlabRows rows and
-// A v blocks of at most 1024 rows inside each sample; uploads the tables.
+// A v blocks of at most 256 rows inside each sample; uploads the tables.
 void batch_layout(BatchLayout &L, const std::vector<int64_t> &cells, hipStream_t st);
 // Rebuild op.off / op.lrow / op.nslab of a built operator for the layout's
 // slabs (A' w partials per slab never mix samples).
