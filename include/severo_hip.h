@@ -49,14 +49,16 @@ extern "C" {
 #define SEV_PREC_FP32 0 /* plain form */
 #define SEV_PREC_BF16 1 /* BF16 values / BF16 MFMA candidates, exact re-rank */
 #define SEV_PREC_EXACT 2 /* kNN only: exhaustive exact search, no MFMA stage */
+#define SEV_PREC_FP8 3   /* kNN only: FP8 e4m3 candidates (per-128-row scaling), then
+                          * BF16 / FP32 candidates for rows without a certificate */
 
 /* ---- library ----------------------------------------------------------- */
 const char *sev_last_error(void);
 int sev_version(void);
 /* Device self-checks (no counterpart in the reference): what == 0 / 1 / 2
- * checks the FP32 / BF16 / FP64 matrix-core lane layouts the kNN and IRLBA
- * kernels rely on; *result receives the largest deviation (0 when they
- * hold).  what == 3 runs the HBM copy probe (k_copy_probe: 1 GiB read +
+ * / 4 checks the FP32 / BF16 / FP64 / FP8 e4m3 matrix-core lane layouts the
+ * kNN and IRLBA kernels rely on; *result receives the largest deviation (0
+ * when they hold).  what == 3 runs the HBM copy probe (k_copy_probe: 1 GiB read +
  * 1 GiB written per launch, best of 5) and returns its rate in GB/s; it is
  * the known-size kernel against which HBM byte counters are calibrated. */
 int sev_selftest(int what, double *result);
@@ -232,6 +234,9 @@ int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const floa
 int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
                             int precision, int64_t *n_fallback);
 int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances);
+/* FP8 form only: rows left without a certificate after the FP8, BF16 and
+ * FP32 candidate passes of the last search (passes[3]). */
+int sev_pipeline_knn_passes(sev_pipeline *p, int64_t *passes);
 
 /* Stage 5: Jaccard SNN of the current neighbours (dtype of the weights). */
 int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz);

@@ -11,7 +11,7 @@ SEV_OK = 0
 SEV_FLOAT32, SEV_FLOAT64 = 0, 1
 SEV_INT32, SEV_INT64 = 0, 1
 SEV_LOGNORMALIZE, SEV_RELATIVECOUNTS = 0, 1
-SEV_PREC_FP32, SEV_PREC_BF16, SEV_PREC_EXACT = 0, 1, 2
+SEV_PREC_FP32, SEV_PREC_BF16, SEV_PREC_EXACT, SEV_PREC_FP8 = 0, 1, 2, 3
 
 _i64 = C.c_int64
 _int = C.c_int
@@ -62,6 +62,7 @@ SIGNATURES = {
     "sev_pipeline_set_embedding": (_int, [_p, _i64, _i64, _p, _i64]),
     "sev_pipeline_neighbours": (_int, [_p, _i64, _int, _i64, _int, _pi64]),
     "sev_pipeline_get_neighbours": (_int, [_p, _p, _p]),
+    "sev_pipeline_knn_passes": (_int, [_p, _p]),
     "sev_pipeline_snn": (_int, [_p, _dbl, _int, _pi64]),
     "sev_pipeline_get_snn": (_int, [_p, _p, _p, _p]),
     "sev_pipeline_profile": (_int, [_p, _int]),

@@ -298,7 +298,7 @@ def ann(X, k: int, include_self=True, ntables=None, precision="fp32", return_fal
     dist = np.zeros((n, k), X.dtype, order="F")
     nfb = C.c_int64(0)
     prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16,
-            "exact": _lib.SEV_PREC_EXACT}[precision]
+            "exact": _lib.SEV_PREC_EXACT, "fp8": _lib.SEV_PREC_FP8}[precision]
     check(_lib.load().sev_knn(_ptr(X), n, d, n, _dtype_code(X.dtype), k, int(include_self), prec,
                               _ptr(idx), _ptr(dist), C.byref(nfb)))
     if return_fallback:

@@ -191,6 +191,7 @@ struct sev_pipeline {
     bool have_nn = false;
     // SNN
     KnnWork kw;
+    int64_t knn_passes[3] = {0, 0, 0};  // FP8 form: rows left after each candidate pass
     SnnWork sw;
     int64_t snn_nnz = 0;
     int snn_dtype = SEV_FLOAT32;
@@ -253,7 +254,11 @@ void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int inclu
     idx.ensure((size_t)(n * k));
     dist.ensure((size_t)(n * k));
     int64_t nfb = n;
-    if (sizeof(T) == 4 && precision != SEV_PREC_EXACT) {
+    if (sizeof(T) == 4 && precision == SEV_PREC_FP8) {
+        KnnWork w;
+        knn_mfma_fp8((const float *)dx.get(), n, d, n, k, include_self, w, idx.get(),
+                     (float *)dist.get(), s.st, &nfb, nullptr);
+    } else if (sizeof(T) == 4 && precision != SEV_PREC_EXACT) {
         KnnWork w;
         knn_mfma((const float *)dx.get(), n, d, n, k, include_self, precision == SEV_PREC_BF16, w,
                  idx.get(), (float *)dist.get(), s.st, &nfb);
@@ -283,9 +288,9 @@ int sev_version(void) { return 1; }
 int sev_selftest(int what, double *result) {
     return guarded([&] {
         ensure_device();
-        SEV_REQUIRE(what >= 0 && what <= 3, "unknown self-test");
+        SEV_REQUIRE(what >= 0 && what <= 4, "unknown self-test");
         SEV_REQUIRE(result != nullptr, "result is NULL");
-        *result = what == 3 ? hbm_copy_probe() : mfma_selftest(what);
+        *result = what == 3 ? hbm_copy_probe() : mfma_selftest(what == 4 ? 3 : what);
     });
 }
 
@@ -744,8 +749,10 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_embedding, "no embedding");
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
-                        precision == SEV_PREC_EXACT,
+                        precision == SEV_PREC_EXACT || precision == SEV_PREC_FP8,
                     "bad precision");
+        SEV_REQUIRE(!(p->batched && precision == SEV_PREC_FP8),
+                    "the FP8 candidate form is not provided for batches");
         const int64_t m = p->m;
         const int64_t d = (ndims <= 0 || ndims > p->zdim) ? p->zdim : ndims;
         SEV_REQUIRE(k >= 1 && k <= (include_self ? m : m - 1), "k larger than the number of cells");
@@ -780,6 +787,9 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
             knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
                              p->dist.get(), st);
             if (n_fallback) *n_fallback = m;
+        } else if (precision == SEV_PREC_FP8) {
+            knn_mfma_fp8(p->Z.get(), m, d, m, k, include_self, p->kw, p->nn.get(), p->dist.get(),
+                         st, n_fallback, p->knn_passes);
         } else {
             knn_mfma(p->Z.get(), m, d, m, k, include_self, precision == SEV_PREC_BF16, p->kw,
                      p->nn.get(), p->dist.get(), st, n_fallback);
@@ -787,6 +797,13 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
         p->k = k;
         p->have_nn = true;
         p->have_snn = false;
+    });
+}
+
+int sev_pipeline_knn_passes(sev_pipeline *p, int64_t *passes) {
+    return guarded([&] {
+        SEV_REQUIRE(passes != nullptr, "passes is NULL");
+        for (int i = 0; i < 3; ++i) passes[i] = p->knn_passes[i];
     });
 }
 
@@ -1114,7 +1131,7 @@ int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t 
     return guarded([&] {
         ensure_device();
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
-                        precision == SEV_PREC_EXACT,
+                        precision == SEV_PREC_EXACT || precision == SEV_PREC_FP8,
                     "bad precision");
         if (dtype == SEV_FLOAT32)
             knn_host<float>((const float *)X, n, d, ld, k, include_self, precision, nn_index,

@@ -280,14 +280,20 @@ void knn_exact_rows_batch(const float *X, int64_t n, int64_t d, int64_t ld, int6
 // n x k, 0-based.  *n_fallback = rows recomputed exhaustively.
 struct KnnWork {
     DevBuf<unsigned char> zq, zc;
-    DevBuf<float> zr, nhat, eps, cthr, pval, pthr;
-    DevBuf<unsigned> gmax;
+    DevBuf<float> zr, nhat, eps, cthr, pval, pthr, scf, hn;
+    DevBuf<unsigned> gmax, bmax;
     DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt;
     DevBuf<int64_t> fail, fail2, qpad;
 };
 void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
               int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,
               int64_t *n_fallback);
+// FP8 (e4m3, per-128-row power-of-two scaling) candidates; rows without a
+// certificate go through BF16, then FP32 candidates, then the exhaustive
+// kernel.  passes[0..2] (optional) = rows left after each candidate pass.
+void knn_mfma_fp8(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                  KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback,
+                  int64_t *passes);
 // The same per sample of a batch (every row's neighbours are rows of its own
 // sample; indices stay global, 0-based).
 void knn_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
@@ -295,7 +301,7 @@ void knn_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
                     float *dist, hipStream_t st, int64_t *n_fallback);
 // Largest deviation of the MFMA lane-layout model from scalar arithmetic
 // (mode 0: 16x16x4 FP32, 1: 16x16x32 BF16); 0 when the model holds.
-double mfma_selftest(int mode);  // mode 2: 16x16x4 FP64
+double mfma_selftest(int mode);  // mode 2: 16x16x4 FP64, mode 3: 16x16x32 FP8 e4m3
 
 // Copies 1 GiB between two device buffers with 16-byte loads and stores
 // (k_copy_probe) and returns the best rate of 5 launches in GB/s (2 GiB of

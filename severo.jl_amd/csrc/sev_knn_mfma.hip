@@ -15,6 +15,11 @@
 //   BF16 form: v_mfma_f32_16x16x32_bf16, operands rounded to BF16 (per point
 //              rounding error eps = |bf16(z) - z|_2 is kept), FP32 accumulate.
 //
+//   FP8 form:  v_mfma_f32_16x16x32_fp8_fp8 on e4m3 operands, each point
+//              scaled by the power of two sigma of its 128-row block (block
+//              max |z| / sigma <= 240); the score is sigma_q sigma_c acc -
+//              |c^|^2/2 with the norm term kept in FP32 (no norm columns).
+//
 // Certificate.  Every candidate outside a query's final list has
 // dot' <= thr (the list's K'-th value), i.e. d~ >= d~_K' = |q|^2 - 2 thr.
 // The computed d~ differs from |q^ - c^|^2 by at most
@@ -59,14 +64,61 @@ __device__ __forceinline__ float ninf() { return -__builtin_huge_valf(); }
 // Also the row-major FP32 copy Zr used by the exact re-rank, n^ (FP32,
 // rounded up) and eps (FP32, rounded up), and the global maxima of both.
 // ---------------------------------------------------------------------------
+// Block maxima for the FP8 form: bmax[i >> 7] = max_t |z_it| (float bits).
+__global__ void __launch_bounds__(kBlock)
+k_knn_blockmax(const float *Z, int64_t n, int64_t d, int64_t ld, unsigned *bmax) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        float mx = 0.0f;
+        for (int64_t t = 0; t < d; ++t) mx = fmaxf(mx, fabsf(Z[t * ld + i]));
+        atomicMax(&bmax[i >> 7], __float_as_uint(mx));
+    }
+}
+
+// FP8 (e4m3) of x (|x| <= 240) and back, by the hardware conversions.
+__device__ __forceinline__ uint32_t f2fp8x2(float a, float b) {
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xFFFFu;
+}
+
 template <int MODE, int KP>
 __global__ void __launch_bounds__(kBlock)
 k_knn_prep(const float *Z, int64_t n, int64_t d, int64_t ld, void *Zq, void *Zc, float *Zr,
-           float *nhat, float *eps, unsigned *gmax) {
+           float *nhat, float *eps, unsigned *gmax, const unsigned *bmax, float *scf,
+           float *hn) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock) {
         double nn = 0.0, ee = 0.0;
-        if (MODE == 0) {
+        if (MODE == 2) {
+            // sigma = the power of two with block max / sigma in (120, 240]
+            const float bm = __uint_as_float(bmax[i >> 7]);
+            float sig = 1.0f;
+            if (bm > 0.0f) {
+                int e;
+                (void)frexpf(bm / 240.0f, &e);
+                sig = ldexpf(1.0f, e);
+            }
+            uint8_t *q = (uint8_t *)Zq + i * KP;
+            for (int64_t t = 0; t < d; t += 2) {
+                const float z0 = Z[t * ld + i];
+                const float z1 = t + 1 < d ? Z[(t + 1) * ld + i] : 0.0f;
+                const int pk = (int)f2fp8x2(z0 / sig, z1 / sig);
+                const float c0 = __builtin_amdgcn_cvt_f32_fp8(pk, 0) * sig;
+                const float c1 = __builtin_amdgcn_cvt_f32_fp8(pk, 1) * sig;
+                q[t] = (uint8_t)(pk & 0xFF);
+                Zr[i * d + t] = z0;
+                nn += (double)c0 * (double)c0;
+                ee += ((double)c0 - (double)z0) * ((double)c0 - (double)z0);
+                if (t + 1 < d) {
+                    q[t + 1] = (uint8_t)(pk >> 8);
+                    Zr[i * d + t + 1] = z1;
+                    nn += (double)c1 * (double)c1;
+                    ee += ((double)c1 - (double)z1) * ((double)c1 - (double)z1);
+                }
+            }
+            for (int64_t t = d; t < KP; ++t) q[t] = 0;
+            scf[i] = sig;
+            hn[i] = (float)(-0.5 * nn);
+        } else if (MODE == 0) {
             float *q = (float *)Zq + i * KP;
             float *c = (float *)Zc + i * KP;
             for (int64_t t = 0; t < d; ++t) {
@@ -134,6 +186,10 @@ template <>
 struct Elem<1> {
     typedef uint16_t T;
 };
+template <>
+struct Elem<2> {
+    typedef uint8_t T;
+};
 
 // Queries: rows q0..q0+QB-1 of the query list (qidx[i], or i itself when
 // qidx is null; nq entries; qidx[i] < 0 marks an unused slot).  Candidates:
@@ -146,13 +202,16 @@ template <int MODE, int KP, int QW, int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const int64_t *qidx,
            int64_t nq, int32_t *cidx, float *cthr, int32_t *ccnt, float *cval,
-           const int64_t *seg, const int32_t *row_sample) {
+           const int64_t *seg, const int32_t *row_sample, const float *scf, const float *hn) {
     typedef typename Elem<MODE>::T E;
     constexpr int QB = kNW * QW * 16;
-    constexpr int ROW = MODE == 0 ? KP + 1 : KP + 8;  // LDS row stride (elements)
+    // LDS row stride (elements): odd for FP32, 16 bytes of padding otherwise
+    constexpr int ROW = MODE == 0 ? KP + 1 : (MODE == 1 ? KP + 8 : KP + 16);
     constexpr int KS = MODE == 0 ? KP / 4 : KP / 32;  // MFMA steps per block
     constexpr int NCB = kCT / 16;                     // candidate blocks per tile
     __shared__ __attribute__((aligned(16))) E tiles[2][kCT * ROW];
+    // FP8: per-candidate scale and -|c^|^2/2 of each tile
+    __shared__ float tilef[2][MODE == 2 ? kCT : 1], tileh[2][MODE == 2 ? kCT : 1];
     __shared__ float listV[QB][KL];
     __shared__ int32_t listI[QB][KL];
     __shared__ int listN[QB];
@@ -179,6 +238,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     // accept a candidate
     float af[QW][MODE == 0 ? KS : 1];
     v8bf ab[QW][MODE == 1 ? KS : 1];
+    long a8[QW][MODE == 2 ? KS : 1];
+    float sq[QW][MODE == 2 ? 4 : 1];  // FP8: scale of the lane's output rows
     float thr[QW][4];
     __syncthreads();
     // candidate chunk
@@ -198,12 +259,17 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
         const bool ok = qpt >= 0;
         const int64_t qrow = ok ? qpt : 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            thr[rb][i] = qg[w * QW * 16 + rb * 16 + 4 * grp + i] >= 0 ? ninf()
-                                                                       : __builtin_huge_valf();
+        for (int i = 0; i < 4; ++i) {
+            const int32_t qo = qg[w * QW * 16 + rb * 16 + 4 * grp + i];
+            thr[rb][i] = qo >= 0 ? ninf() : __builtin_huge_valf();
+            if constexpr (MODE == 2) sq[rb][i] = qo >= 0 ? scf[qo] : 1.0f;
+        }
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            if (MODE == 0) {
+            if constexpr (MODE == 2) {
+                a8[rb][s] = ok ? *(const long *)((const uint8_t *)Zq + qrow * KP + 32 * s + 8 * grp)
+                               : 0L;
+            } else if (MODE == 0) {
                 af[rb][s] = ok ? ((const float *)Zq)[qrow * KP + 4 * s + grp] : 0.0f;
             } else {
                 v8bf v;
@@ -221,11 +287,13 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     // Candidate tiles are double-buffered: the loads of tile t+1 are issued
     // into registers before tile t is multiplied and land in the other LDS
     // buffer afterwards, so one barrier per tile suffices.
-    // 16-byte loads: 4 floats (FP32) or 8 bf16 values per load
-    constexpr int CH = MODE == 0 ? KP / 4 : KP / 8;       // loads per candidate row
-    constexpr int NLD = (kCT * CH + kBlock - 1) / kBlock;  // loads per thread
-    static_assert(KP % (MODE == 0 ? 4 : 8) == 0, "row length must be a multiple of 16 bytes");
+    // 16-byte loads: 4 floats (FP32), 8 bf16 or 16 fp8 values per load
+    constexpr int EPL = MODE == 0 ? 4 : (MODE == 1 ? 8 : 16);  // elements per load
+    constexpr int CH = KP / EPL;                               // loads per candidate row
+    constexpr int NLD = (kCT * CH + kBlock - 1) / kBlock;      // loads per thread
+    static_assert(KP % EPL == 0, "row length must be a multiple of 16 bytes");
     uint4 stage[NLD];
+    float fstage = 1.0f, hstage = 0.0f;
     auto fetch = [&](int64_t t0) {
 #pragma unroll
         for (int u = 0; u < NLD; ++u) {
@@ -234,11 +302,24 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             const int64_t c = t0 + r;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (e < kCT * CH && c < cend)
-                v = *(const uint4 *)((const E *)Zc + c * KP + part * (MODE == 0 ? 4 : 8));
+                v = *(const uint4 *)((const E *)Zc + c * KP + part * EPL);
             stage[u] = v;
         }
+        if constexpr (MODE == 2) {
+            const int64_t c = t0 + threadIdx.x;
+            if (threadIdx.x < kCT) {
+                fstage = c < cend ? scf[c] : 1.0f;
+                hstage = c < cend ? hn[c] : 0.0f;
+            }
+        }
     };
-    auto deposit = [&](E *tile) {
+    auto deposit = [&](E *tile, int b) {
+        if constexpr (MODE == 2) {
+            if (threadIdx.x < kCT) {
+                tilef[b][threadIdx.x] = fstage;
+                tileh[b][threadIdx.x] = hstage;
+            }
+        }
 #pragma unroll
         for (int u = 0; u < NLD; ++u) {
             const int e = threadIdx.x + u * kBlock;
@@ -251,12 +332,12 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 d[2] = __uint_as_float(stage[u].z);
                 d[3] = __uint_as_float(stage[u].w);
             } else {
-                *(uint4 *)(&tile[r * ROW + part * 8]) = stage[u];
+                *(uint4 *)(&tile[r * ROW + part * EPL]) = stage[u];
             }
         }
     };
     fetch(cbeg);
-    deposit(tiles[0]);
+    deposit(tiles[0], 0);
     // every global load issued so far (query operands, first tile) is
     // complete here, so the loop only waits for its own prefetches
     __builtin_amdgcn_s_waitcnt(0);
@@ -278,9 +359,13 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
         for (int cb = 0; cb < NCB; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
             v8bf bbv[MODE == 1 ? KS : 1];
+            long b8[MODE == 2 ? KS : 1];
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                if (MODE == 0)
+                if constexpr (MODE == 2)
+                    b8[s] = *(const long *)((const uint8_t *)tile + (cb * 16 + col) * ROW +
+                                            32 * s + 8 * grp);
+                else if (MODE == 0)
                     bfv[s] = ((const float *)tile)[(cb * 16 + col) * ROW + 4 * s + grp];
                 else
                     bbv[s] = *(const v8bf *)((const uint16_t *)tile + (cb * 16 + col) * ROW +
@@ -291,10 +376,20 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 v4f a = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int s = 0; s < KS; ++s) {
-                    if (MODE == 0)
+                    if constexpr (MODE == 2)
+                        a = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a8[rb][s], b8[s], a, 0, 0,
+                                                                       0);
+                    else if (MODE == 0)
                         a = __builtin_amdgcn_mfma_f32_16x16x4f32(af[rb][s], bfv[s], a, 0, 0, 0);
                     else
                         a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab[rb][s], bbv[s], a, 0, 0, 0);
+                }
+                if constexpr (MODE == 2) {
+                    // score = sigma_q sigma_c (q^'.c^') - |c^|^2/2 (the scales are
+                    // powers of two: their product is exact)
+                    const float fc = tilef[buf][cb * 16 + col], hc = tileh[buf][cb * 16 + col];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) a[i] = fmaf(a[i], sq[rb][i] * fc, hc);
                 }
                 acc[cb][rb] = a;
             }
@@ -361,7 +456,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 }
             }
         }
-        if (more) deposit(tiles[buf ^ 1]);
+        if (more) deposit(tiles[buf ^ 1], buf ^ 1);
         __syncthreads();
     }
     for (int ql = w; ql < QB; ql += kNW) {
@@ -515,7 +610,25 @@ __global__ void k_selftest_mfma(int mode, float *out) {
     auto B = [](int kk, int nn) { return (float)((kk * 5 + nn * 9) % 13) - 6.0f; };
     v4f acc = {0.f, 0.f, 0.f, 0.f};
     int K;
-    if (mode == 2) {
+    if (mode == 3) {
+        // FP8 e4m3: small integers are exact; lanes hold 8 consecutive k
+        K = 32;
+        int lo = 0, hi = 0, blo = 0, bhi = 0;
+        for (int e = 0; e < 8; e += 2) {
+            const int pa = (int)f2fp8x2(A(col, 8 * grp + e), A(col, 8 * grp + e + 1));
+            const int pb = (int)f2fp8x2(B(8 * grp + e, col), B(8 * grp + e + 1, col));
+            if (e < 4) {
+                lo |= pa << (8 * e);
+                blo |= pb << (8 * e);
+            } else {
+                hi |= pa << (8 * (e - 4));
+                bhi |= pb << (8 * (e - 4));
+            }
+        }
+        const long a8 = (long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+        const long b8 = (long)(((uint64_t)(uint32_t)bhi << 32) | (uint32_t)blo);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a8, b8, acc, 0, 0, 0);
+    } else if (mode == 2) {
         typedef double v4d __attribute__((ext_vector_type(4)));
         v4d a64 = {0.0, 0.0, 0.0, 0.0};
         a64 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)A(col, grp), (double)B(grp, col), a64,
@@ -567,22 +680,35 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
                  const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
                  int32_t *nfail, int64_t *fail, hipStream_t st, const int64_t *seg = nullptr,
                  const int32_t *row_sample = nullptr, bool prep = true) {
-    const size_t es = MODE == 0 ? 4 : 2;
+    const size_t es = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
     w.zq.ensure((size_t)n * KP * es);
-    w.zc.ensure((size_t)n * KP * es);
+    if (MODE != 2) w.zc.ensure((size_t)n * KP * es);  // FP8: one operand array for both
     w.zr.ensure((size_t)(n * d));
     w.nhat.ensure((size_t)n);
     w.eps.ensure((size_t)n);
     w.gmax.ensure(2);
+    if (MODE == 2) {
+        w.bmax.ensure((size_t)ceil_div(n, 128));
+        w.scf.ensure((size_t)n);
+        w.hn.ensure((size_t)n);
+    }
     if (prep) {
         SEV_HIP(hipMemsetAsync(w.gmax.get(), 0, 2 * sizeof(unsigned), st));
         ProfScope ps("knn_prep", st, 4.0 * n * d * 2 + 2.0 * n * KP * es);
         const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
+        if (MODE == 2) {
+            SEV_HIP(hipMemsetAsync(w.bmax.get(), 0, sizeof(unsigned) * ceil_div(n, 128), st));
+            hipLaunchKernelGGL(k_knn_blockmax, dim3(g), dim3(kBlock), 0, st, Z, n, d, ld,
+                               w.bmax.get());
+            SEV_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL((k_knn_prep<MODE, KP>), dim3(g), dim3(kBlock), 0, st, Z, n, d, ld,
-                           (void *)w.zq.get(), (void *)w.zc.get(), w.zr.get(), w.nhat.get(),
-                           w.eps.get(), w.gmax.get());
+                           (void *)w.zq.get(), (void *)(MODE == 2 ? w.zq.get() : w.zc.get()),
+                           w.zr.get(), w.nhat.get(), w.eps.get(), w.gmax.get(),
+                           (const unsigned *)w.bmax.get(), w.scf.get(), w.hn.get());
         SEV_LAUNCH_CHECK();
     }
+    const void *zc_op = MODE == 2 ? (const void *)w.zq.get() : (const void *)w.zc.get();
     constexpr int QB = kNW * QW * 16;
     const int64_t qblocks = ceil_div(nq, QB);
     // split the candidates when the query blocks alone cannot fill the device
@@ -595,13 +721,15 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     w.cthr.ensure((size_t)nq);
     w.ccnt.ensure((size_t)nq);
     {
-        ProfScope ps(MODE == 0 ? "knn_cand_fp32" : "knn_cand_bf16", st, 0.0,
+        ProfScope ps(MODE == 0 ? "knn_cand_fp32" : (MODE == 1 ? "knn_cand_bf16" : "knn_cand_fp8"),
+                     st, 0.0,
                      2.0 * (double)nq * (double)n * (double)d);
         if (nchunk == 1) {
             hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks),
-                               dim3(kBlock), 0, st, (const void *)w.zq.get(),
-                               (const void *)w.zc.get(), n, include_self, qidx, nq, w.cidx.get(),
-                               w.cthr.get(), w.ccnt.get(), (float *)nullptr, seg, row_sample);
+                               dim3(kBlock), 0, st, (const void *)w.zq.get(), zc_op, n,
+                               include_self, qidx, nq, w.cidx.get(),
+                               w.cthr.get(), w.ccnt.get(), (float *)nullptr, seg, row_sample,
+                               w.scf.get(), w.hn.get());
             SEV_LAUNCH_CHECK();
         } else if constexpr (2 * KL <= kWave) {
             w.pidx.ensure((size_t)nchunk * nq * KL);
@@ -609,9 +737,10 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
             w.pthr.ensure((size_t)nchunk * nq);
             w.pcnt.ensure((size_t)nchunk * nq);
             hipLaunchKernelGGL((k_knn_cand<MODE, KP, QW, KL>), dim3((unsigned)qblocks, nchunk),
-                               dim3(kBlock), 0, st, (const void *)w.zq.get(),
-                               (const void *)w.zc.get(), n, include_self, qidx, nq, w.pidx.get(),
-                               w.pthr.get(), w.pcnt.get(), w.pval.get(), seg, row_sample);
+                               dim3(kBlock), 0, st, (const void *)w.zq.get(), zc_op, n,
+                               include_self, qidx, nq, w.pidx.get(),
+                               w.pthr.get(), w.pcnt.get(), w.pval.get(), seg, row_sample,
+                               w.scf.get(), w.hn.get());
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL((k_knn_merge<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
                                dim3(kBlock), 0, st, nq, nchunk, w.pidx.get(), w.pval.get(),
@@ -620,7 +749,10 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
         }
     }
     {
-        const double eta = 4.0 * (KP + 2) * 0x1p-24 + (MODE == 0 ? 0x1p-22 : 0x1p-14);
+        // FP32: accumulation + the FP32 norm column; BF16: + the hi/lo split of
+        // the norm; FP8: accumulation of exact products, the exact power-of-two
+        // descale, the FMA and the FP32 norm term
+        const double eta = 4.0 * (KP + 2) * 0x1p-24 + (MODE == 1 ? 0x1p-14 : 0x1p-22);
         const double spec = pow(1.0 - 0x1p-24, (double)(d + 3));
         ProfScope ps("knn_rerank", st, (double)nq * KL * (4.0 + 4.0 * d));
         hipLaunchKernelGGL((k_knn_rerank<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
@@ -660,7 +792,79 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     if (n_fallback) *n_fallback = nf;
 }
 
+// FP8 candidates, then for the rows without a certificate BF16 candidates,
+// then FP32 candidates, then the exhaustive kernel.
+template <int KP8, int QW, int KL, int KP16, int KP32>
+void run_mfma_fp8(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                  KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback,
+                  int64_t *passes) {
+    w.nfail.ensure(4);
+    w.fail.ensure((size_t)n);
+    w.fail2.ensure((size_t)n);
+    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 4 * sizeof(int32_t), st));
+    cand_rerank<2, KP8, QW, KL>(Z, n, d, ld, k, include_self, nullptr, n, w, idx, dist,
+                                w.nfail.get(), w.fail.get(), st);
+    auto count = [&](int c) {
+        int32_t nf = 0;
+        SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + c, sizeof nf, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        return nf;
+    };
+    int32_t nf = count(0);
+    if (passes) passes[0] = nf;
+    if (nf > 0) {
+        SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
+                               hipMemcpyDeviceToDevice, st));
+        cand_rerank<1, KP16, QW, KL>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
+                                     dist, w.nfail.get() + 1, w.fail.get(), st);
+        nf = count(1);
+    }
+    if (passes) passes[1] = nf;
+    if constexpr (KP32 > 0) {
+        if (nf > 0) {
+            SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
+                                   hipMemcpyDeviceToDevice, st));
+            cand_rerank<0, KP32, QW, KL>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
+                                         dist, w.nfail.get() + 2, w.fail.get(), st);
+            nf = count(2);
+        }
+    }
+    if (passes) passes[2] = nf;
+    if (nf > 0)
+        knn_exact_rows<float>(Z, n, d, ld, k, include_self, w.fail.get(), nf, idx, dist, st);
+    if (n_fallback) *n_fallback = nf;
+}
+
 }  // namespace
+
+void knn_mfma_fp8(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
+                  KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback,
+                  int64_t *passes) {
+    SEV_REQUIRE(k >= 1 && k <= kMaxK, "knn: k must be in [1, 64]");
+    SEV_REQUIRE(d >= 1 && d + 2 <= 64, "knn: the FP8 form supports d <= 62");
+    SEV_REQUIRE(n < ((int64_t)1 << 31) - 256, "knn: at most 2^31 - 256 points");
+    if (k <= 24) {
+        if (d <= 32 && d + 1 <= 52)
+            run_mfma_fp8<32, 2, 32, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                            n_fallback, passes);
+        else if (d + 1 <= 52)
+            run_mfma_fp8<64, 2, 32, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                            n_fallback, passes);
+        else
+            run_mfma_fp8<64, 2, 32, 64, 0>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                           n_fallback, passes);
+    } else {
+        if (d <= 32 && d + 1 <= 52)
+            run_mfma_fp8<32, 1, 64, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                            n_fallback, passes);
+        else if (d + 1 <= 52)
+            run_mfma_fp8<64, 1, 64, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                            n_fallback, passes);
+        else
+            run_mfma_fp8<64, 1, 64, 64, 0>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                           n_fallback, passes);
+    }
+}
 
 void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
               int bf16, KnnWork &w, int32_t *idx, float *dist, hipStream_t st,

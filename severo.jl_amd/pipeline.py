@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import check
 
 PRECISION = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16,
-             "exact": _lib.SEV_PREC_EXACT}
+             "exact": _lib.SEV_PREC_EXACT, "fp8": _lib.SEV_PREC_FP8}
 
 
 def _ptr(a):
@@ -255,6 +255,13 @@ class Pipeline:
                                               PRECISION[precision], C.byref(nfb)))
         self.k = k
         return nfb.value
+
+    def knn_passes(self):
+        """FP8 form: rows left without a certificate after the FP8, BF16 and
+        FP32 candidate passes of the last search."""
+        out = np.zeros(3, np.int64)
+        check(self._L.sev_pipeline_knn_passes(self._h, _ptr(out)))
+        return out
 
     def get_neighbours(self, m):
         idx = np.zeros((m, self.k), np.int32, order="F")

@@ -106,7 +106,7 @@ def test_pipeline_graph_end_to_end(sev, orc):
     assert np.array_equal(S.data, nz)
 
 
-@pytest.mark.parametrize("what", [0, 1, 2])
+@pytest.mark.parametrize("what", [0, 1, 2, 4])
 def test_mfma_lane_layout_selftest(sev, what):
     import ctypes
     r = ctypes.c_double(-1.0)
@@ -114,7 +114,7 @@ def test_mfma_lane_layout_selftest(sev, what):
     assert r.value == 0.0
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16", "exact"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8", "exact"])
 @pytest.mark.parametrize("include_self", [True, False])
 @pytest.mark.parametrize("n,d,k", [(5000, 50, 20), (3000, 30, 30), (700, 10, 5), (130, 3, 64)])
 def test_ann_precision_forms_bit_exact(sev, orc, precision, include_self, n, d, k):
@@ -134,13 +134,13 @@ def test_ann_mfma_duplicates(sev, orc):
     rng = np.random.default_rng(8)
     base = rng.standard_normal((1000, 20)).astype(np.float32)
     X = np.concatenate([base, base, base[:300]])
-    for prec in ("fp32", "bf16"):
+    for prec in ("fp32", "bf16", "fp8"):
         idx, dist = sev.ann(X, 10, precision=prec)
         ridx, rdist = orc.knn(X, 10)
         assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
 def test_pipeline_neighbours_forms(sev, orc, precision):
     t = sev.nb_tables(3000, seed=1, density=0.05)
     C = 6000
@@ -155,3 +155,7 @@ def test_pipeline_neighbours_forms(sev, orc, precision):
     ridx, rdist = orc.knn(Z, 20)
     assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
     print(f"pipeline {precision}: {nfb} of {C} rows fell back")
+    if precision == "fp8":
+        left = p.knn_passes()
+        assert left[0] >= left[1] >= left[2] == nfb
+        print(f"fp8 passes: rows left after FP8 / BF16 / FP32 candidates: {left.tolist()}")
