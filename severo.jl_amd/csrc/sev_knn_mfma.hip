@@ -781,7 +781,8 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
             w.fail2.ensure((size_t)nf);
             SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
                                    hipMemcpyDeviceToDevice, st));
-            cand_rerank<0, KP32, 2, KL>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
+            cand_rerank<0, KP32, (KL > 32 ? 1 : 2), KL>(Z, n, d, ld, k, include_self,
+                                                        w.fail2.get(), nf, w, idx,
                                         dist, w.nfail.get() + 1, w.fail.get(), st);
             SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 1, sizeof nf, hipMemcpyDeviceToHost, st));
             SEV_HIP(hipStreamSynchronize(st));
@@ -882,6 +883,9 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
         else if (k <= 24)
             run_mfma<1, 128, 2, 32, 128>(Z, n, d, ld, k, include_self, w, idx, dist, st,
                                          n_fallback);
+        else if (d + 2 <= 64 && d + 1 <= 52)
+            run_mfma<1, 64, 1, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                       n_fallback);
         else if (d + 2 <= 64)
             run_mfma<1, 64, 1, 64, 0>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
         else
@@ -978,9 +982,10 @@ void run_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
     if constexpr (MODE == 1 && KP32 > 0) {
         if (nf > 0) {
             // BF16 rows without a certificate: FP32 candidates for those rows
-            const std::vector<int64_t> q2 = batch_queries(L, f.data(), nf, kNW * 2 * 16, false);
+            constexpr int QW32 = KL > 32 ? 1 : 2;
+            const std::vector<int64_t> q2 = batch_queries(L, f.data(), nf, kNW * QW32 * 16, false);
             upload_sync(w.fail2, q2, st);
-            cand_rerank<0, KP32, 2, KL>(Z, n, d, ld, k, include_self, w.fail2.get(),
+            cand_rerank<0, KP32, QW32, KL>(Z, n, d, ld, k, include_self, w.fail2.get(),
                                         (int64_t)q2.size(), w, idx, dist, w.nfail.get() + 1,
                                         w.fail.get(), st, seg, rs);
             SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 1, sizeof nf, hipMemcpyDeviceToHost, st));
@@ -1020,6 +1025,9 @@ void knn_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
         else if (k <= 24)
             run_mfma_batch<1, 128, 2, 32, 128>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
                                                n_fallback);
+        else if (d + 2 <= 64 && d + 1 <= 52)
+            run_mfma_batch<1, 64, 1, 64, 52>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
+                                             n_fallback);
         else if (d + 2 <= 64)
             run_mfma_batch<1, 64, 1, 64, 0>(Z, n, d, ld, k, include_self, L, w, idx, dist, st,
                                             n_fallback);
