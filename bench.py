@@ -344,6 +344,17 @@ def main():
     print(f"# generated {m} x {g} counts, nnz {nnz} ({nnz / (m * g):.4f}) in {gen_s:.1f} s",
           file=sys.stderr, flush=True)
 
+    # find_variable_features(:vst) on the same counts, outside the timed
+    # step (the feature list of the step is fixed in advance)
+    p.find_variable_features(NFEAT)
+    p.synchronize()
+    t0 = time.perf_counter()
+    hvf = p.find_variable_features(NFEAT)
+    hvg_ms = (time.perf_counter() - t0) * 1e3
+    print(f"# find_variable_features(:vst, {NFEAT}): {hvg_ms:.1f} ms, "
+          f"{len(set(hvf.tolist()) & set(feats.tolist()))} of its genes among the top-mean ones",
+          file=sys.stderr, flush=True)
+
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
     errs = {}
@@ -393,6 +404,7 @@ def main():
         out["bf16_speedup_vs_fp32"] = round(res["fp32"][0] / res["bf16"][0], 3)
     if "fp32" in res and "fp8" in res:
         out["fp8_speedup_vs_fp32"] = round(res["fp32"][0] / res["fp8"][0], 3)
+    out["find_variable_features_ms"] = round(hvg_ms, 2)
     if errs:
         out["errors_vs_fp64"] = errs
     if not args.no_e2e:

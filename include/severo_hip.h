@@ -154,6 +154,19 @@ int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t 
 int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,
                     int64_t *nnz, int64_t *colptr, int64_t *rowval, void *nzval);
 
+/* find_variable_features(counts, nfeatures; method=:vst, loess_span)
+ * (src/variablefeatures.jl:34-50,128-161): per-gene mean and sd of the
+ * counts (mean_std(Float64, A), src/scaling.jl:119-130), a degree-2 loess of
+ * log10 sd on log10 mean over the non-constant genes (evaluated exactly at
+ * every gene; Loess.jl interpolates between kd-tree vertices), the clipped
+ * standardized variance per gene (:21-28), and the nfeatures genes with the
+ * largest one.  selected[nfeatures] receives 1-based gene indices in
+ * descending order (ties to the lower index); metric[n] (optional) the
+ * variances.  rowval is accepted for the CSC's shape and not read. */
+int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                               const int32_t *nzval, int64_t nfeatures, double loess_span,
+                               int64_t *selected, double *metric);
+
 /* ---- device-resident pipeline ----------------------------------------- */
 typedef struct sev_pipeline sev_pipeline;
 
@@ -185,6 +198,11 @@ int sev_pipeline_synth_batch(sev_pipeline *p, int64_t S, const int64_t *cells, i
 int sev_pipeline_set_batch(sev_pipeline *p, int64_t S, const int64_t *cells);
 int sev_pipeline_counts_info(sev_pipeline *p, int64_t *m, int64_t *n, int64_t *nnz);
 int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, int32_t *nzval);
+
+/* find_variable_features(:vst) on the loaded counts (as
+ * sev_find_variable_features). */
+int sev_pipeline_find_variable_features(sev_pipeline *p, int64_t nfeatures, double loess_span,
+                                        int64_t *selected, double *metric);
 
 /* Stage 1: normalize_cells in Float32 (method SEV_LOGNORMALIZE or
  * SEV_RELATIVECOUNTS). */

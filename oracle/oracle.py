@@ -58,6 +58,10 @@ def lib():
             f.argtypes = [_p, _i64, _i64, _i64, _i64, C.c_int, _p, _i64, _p, _p, C.c_int]
         L.so_snn_jaccard.restype = _i64
         L.so_snn_jaccard.argtypes = [_i64, _i64, _p, _dbl, C.c_int, _p, _p, _p]
+        L.so_count_mean_std.argtypes = [_i64, _i64, _p, _p, _p, _p]
+        L.so_loess_fit.restype = C.c_int
+        L.so_loess_fit.argtypes = [_i64, _p, _p, _dbl, _p]
+        L.so_standardized_var.argtypes = [_i64, _i64, _p, _p, _p, _p, _p]
         _LIB = L
     return _LIB
 
@@ -236,3 +240,53 @@ def snn(nn_index, k, prune=1.0 / 15, dtype=np.float64):
     nzval = np.zeros(nnz, dtype)
     L.so_snn_jaccard(n, k, _ptr(nn_index), prune, f64, _ptr(colptr), _ptr(rowval), _ptr(nzval))
     return colptr, rowval, nzval
+
+
+def count_mean_std(m, colptr, counts):
+    """mean_std(Float64, A) of an Int32 count CSC (src/scaling.jl:119-130)."""
+    n = len(colptr) - 1
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    counts = np.ascontiguousarray(counts, np.int32)
+    mu = np.empty(n)
+    sd = np.empty(n)
+    lib().so_count_mean_std(m, n, _ptr(colptr), _ptr(counts), _ptr(mu), _ptr(sd))
+    return mu, sd
+
+
+def loess_fit(x, y, span):
+    """Exact degree-2 loess at every point of x (ascending)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    fit = np.empty(len(x))
+    if lib().so_loess_fit(len(x), _ptr(x), _ptr(y), span, _ptr(fit)) != 0:
+        raise ValueError("loess needs at least one point")
+    return fit
+
+
+def standardized_var(m, colptr, counts, mu, sd):
+    n = len(colptr) - 1
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    counts = np.ascontiguousarray(counts, np.int32)
+    out = np.empty(n)
+    lib().so_standardized_var(m, n, _ptr(colptr), _ptr(counts), _ptr(np.ascontiguousarray(mu)),
+                              _ptr(np.ascontiguousarray(sd)), _ptr(out))
+    return out
+
+
+def vst(m, colptr, counts, nfeatures=2000, loess_span=0.5):
+    """find_variable_features(:vst) (src/variablefeatures.jl:34-50,128-161):
+    returns (selected 0-based genes, metric, mu, sd, expected sd).  The
+    non-constant genes are ordered by (log10 mu, gene) for the fit; ties in
+    the metric go to the lower gene index."""
+    mu, sd = count_mean_std(m, colptr, counts)
+    nc = np.nonzero(sd > 0)[0]
+    xs, ys = np.log10(mu[nc]), np.log10(sd[nc])
+    order = np.lexsort((nc, xs))
+    fit = np.empty(len(nc))
+    fit[order] = loess_fit(xs[order], ys[order], loess_span)
+    esd = sd.copy()
+    esd[nc] = 10.0 ** fit
+    esd[np.isnan(esd)] = 0.0
+    metric = standardized_var(m, colptr, counts, mu, esd)
+    sel = np.lexsort((np.arange(len(metric)), -metric))[:nfeatures]
+    return sel, metric, mu, sd, esd

@@ -24,6 +24,10 @@
  *   so_knn_*            numpy float32 brute force with stable argsort,
  *                       crafted duplicate-point ties.
  *   so_snn_jaccard      Python set intersections on small graphs.
+ *   so_count_mean_std   numpy mean / var(ddof=1) of the densified columns.
+ *   so_loess_fit        exact reproduction of quadratics, and a numpy
+ *                       weighted least-squares fit (lstsq) per point.
+ *   so_standardized_var numpy closed form of the clipped standardized values.
  */
 #include <math.h>
 #include <stdint.h>
@@ -821,4 +825,152 @@ SO_EXPORT int64_t so_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, 
     free(cnt);
     free(touched);
     return nnz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* find_variable_features(:vst)  (src/variablefeatures.jl:34-50,128-161)     */
+/* ------------------------------------------------------------------------ */
+
+/* mean_std(Float64, A) (src/scaling.jl:18-38,119-130): per column, Welford
+ * over the Int32 counts in FP64, count starting at m - nnz; sd = sqrt(s/(m-1)). */
+SO_EXPORT void so_count_mean_std(int64_t m, int64_t n, const int64_t *colptr, const int32_t *x,
+                                 double *mu_out, double *sd_out)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t count = m - (colptr[j + 1] - colptr[j]);
+        double mu = 0.0, s = 0.0;
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+            const double v = (double)x[p];
+            count += 1;
+            const double delta = v - mu;
+            mu = mu + delta / (double)count;
+            s = s + delta * (v - mu);
+        }
+        mu_out[j] = mu;
+        sd_out[j] = sqrt(s / (double)(m - 1));
+    }
+}
+
+/* Loess (Cleveland's local regression, degree 2, tricube weights) evaluated
+ * at every point of x (ascending, N points), the model of
+ * `loess(xs, ys, span=...)` + `predict(model, xs)` at :41-45; Loess.jl
+ * interpolates between kd-tree vertices, this is the exact local fit.
+ *  q = floor(N span), at least min(N, 3); the neighbourhood of x_i is the
+ *  window of q consecutive points containing i0 (the first point equal to
+ *  x_i) with the smallest dmax = max(x_i - x_s, x_{s+q-1} - x_i) (the
+ *  smaller s on a tie);
+ *  w_j = (1 - (|x_j - x_i| / dmax)^3)^3 (0 at dmax; 1 when dmax = 0);
+ *  u_j = (x_j - x_i) / dmax; weighted least squares y ~ a + b u + c u^2 by
+ *  elimination on the normal equations (degree lowered when a pivot
+ *  vanishes); fit_i = a. */
+static double so_loess_point(int64_t N, const double *x, const double *y, int64_t q, int64_t i)
+{
+    /* the fit depends on x_i only: tied points use the window of the first
+     * of them (its nonzero-weight terms are summed in one order) */
+    {
+        int64_t a = 0, b = i;
+        while (a < b) {
+            const int64_t mid = a + (b - a) / 2;
+            if (x[mid] < x[i]) a = mid + 1;
+            else b = mid;
+        }
+        i = a;
+    }
+    int64_t lo = i - q + 1 < 0 ? 0 : i - q + 1;
+    int64_t hi = i < N - q ? i : N - q;
+    /* smallest s in [lo, hi] with x[s+q-1] - x[i] >= x[i] - x[s] */
+    int64_t a = lo, b = hi + 1;
+    while (a < b) {
+        const int64_t mid = a + (b - a) / 2;
+        if (x[mid + q - 1] - x[i] >= x[i] - x[mid]) b = mid;
+        else a = mid + 1;
+    }
+    int64_t s;
+    double dmax;
+    if (a > hi) {
+        s = hi;
+        dmax = x[i] - x[hi];
+    } else {
+        const double dr = x[a + q - 1] - x[i];
+        if (a > lo && x[i] - x[a - 1] <= dr) {
+            s = a - 1;
+            dmax = x[i] - x[a - 1];
+        } else {
+            s = a;
+            dmax = dr;
+        }
+    }
+    double S0 = 0, S1 = 0, S2 = 0, S3 = 0, S4 = 0, T0 = 0, T1 = 0, T2 = 0;
+    for (int64_t j = s; j < s + q; ++j) {
+        double w, u;
+        if (dmax > 0.0) {
+            u = (x[j] - x[i]) / dmax;
+            const double r = fabs(u);
+            const double t = 1.0 - r * r * r;
+            w = r < 1.0 ? t * t * t : 0.0;
+        } else {
+            u = 0.0;
+            w = 1.0;
+        }
+        const double wu = w * u, wu2 = wu * u;
+        S0 += w;
+        S1 += wu;
+        S2 += wu2;
+        S3 += wu2 * u;
+        S4 += wu2 * u * u;
+        T0 += w * y[j];
+        T1 += wu * y[j];
+        T2 += wu2 * y[j];
+    }
+    if (!(S0 > 0.0)) return y[i];
+    const double l21 = S1 / S0, l31 = S2 / S0;
+    const double a22 = S2 - l21 * S1, a23 = S3 - l21 * S2, a33 = S4 - l31 * S2;
+    const double b2 = T1 - l21 * T0, b3 = T2 - l31 * T0;
+    if (!(a22 > 1e-12 * S2)) return T0 / S0;
+    const double l32 = a23 / a22;
+    const double a33b = a33 - l32 * a23, b3b = b3 - l32 * b2;
+    if (!(a33b > 1e-12 * S4)) {
+        const double bl = b2 / a22;
+        return (T0 - S1 * bl) / S0;
+    }
+    const double c = b3b / a33b;
+    const double bb = (b2 - a23 * c) / a22;
+    return (T0 - S1 * bb - S2 * c) / S0;
+}
+
+SO_EXPORT int so_loess_fit(int64_t N, const double *x, const double *y, double span, double *fit)
+{
+    if (N <= 0) return -1;
+    int64_t q = (int64_t)floor((double)N * span);
+    const int64_t qmin = N < 3 ? N : 3;
+    if (q < qmin) q = qmin;
+    if (q > N) q = N;
+    for (int64_t i = 0; i < N; ++i) fit[i] = so_loess_point(N, x, y, q, i);
+    return 0;
+}
+
+/* standardized_var_clipped (src/variablefeatures.jl:21-28): for sd_j > 0,
+ * (sum over nonzeros of min((x - mu)/sd, vmax)^2
+ *  + (m - nnz) min(-mu/sd, vmax)^2) / (m - 1), vmax = sqrt(m); else 0.
+ * The sum runs in row order. */
+SO_EXPORT void so_standardized_var(int64_t m, int64_t n, const int64_t *colptr, const int32_t *x,
+                                   const double *mu, const double *sd, double *out)
+{
+    const double vmax = sqrt((double)m);
+    for (int64_t j = 0; j < n; ++j) {
+        if (sd[j] == 0.0) {
+            out[j] = 0.0;
+            continue;
+        }
+        double t = 0.0;
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+            double v = ((double)x[p] - mu[j]) / sd[j];
+            if (v > vmax) v = vmax;
+            t += v * v;
+        }
+        double z = (0.0 - mu[j]) / sd[j];
+        if (z > vmax) z = vmax;
+        const int64_t nz = colptr[j + 1] - colptr[j];
+        out[j] = (t + (double)(m - nz) * (z * z)) / (double)(m - 1);
+    }
 }

@@ -12,7 +12,7 @@ module name ``severo_jl_amd``.
 """
 from ._lib import ConvergenceError, SeveroError, load  # noqa: F401
 from .api import (CenteredMatrix, LinearEmbedding, SVD, ann, copy_probe, embedding, irlba,  # noqa: F401
-                  irlba_operator,
+                  find_variable_features, irlba_operator,
                   jaccard_index, nearest_neighbours, normalize_cells, pca, scale_features,
                   shared_nearest_neighbours)
 from .pipeline import Pipeline  # noqa: F401

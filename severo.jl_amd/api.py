@@ -68,6 +68,34 @@ def normalize_cells(X, method="lognormalize", scale_factor=1.0, dtype=np.float64
 
 
 # ---------------------------------------------------------------------------
+# find_variable_features  (src/variablefeatures.jl:128-161)
+# ---------------------------------------------------------------------------
+def find_variable_features(X, nfeatures=2000, method="vst", loess_span=0.5,
+                           return_metric=False):
+    """The ``nfeatures`` genes (0-based, best first) with the largest
+    clipped standardized variance after a loess fit of log10 sd on log10
+    mean (``:vst``, the reference's default method).  Ties go to the lower
+    gene index.  The loess is the exact local fit at every gene (Loess.jl
+    interpolates between kd-tree vertices)."""
+    if isinstance(method, str):
+        method = method.lstrip(":")
+    if method != "vst":
+        raise ValueError(f"unknown selection method: {method} (only vst is provided)")
+    X, colptr, rowval = _csc_julia(X)
+    if not np.issubdtype(X.dtype, np.integer):
+        raise TypeError("find_variable_features expects an integer count matrix")
+    counts = np.ascontiguousarray(X.data, np.int32)
+    m, n = X.shape
+    sel = np.zeros(nfeatures, np.int64)
+    metric = np.zeros(n)
+    L = _lib.load()
+    check(L.sev_find_variable_features(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts), nfeatures,
+                                       float(loess_span), _ptr(sel), _ptr(metric)))
+    sel -= 1
+    return (sel, metric) if return_metric else sel
+
+
+# ---------------------------------------------------------------------------
 # CenteredMatrix / scale_features  (src/scaling.jl:219-357)
 # ---------------------------------------------------------------------------
 class CenteredMatrix:

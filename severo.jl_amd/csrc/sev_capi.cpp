@@ -800,6 +800,18 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
     });
 }
 
+int sev_pipeline_find_variable_features(sev_pipeline *p, int64_t nfeatures, double loess_span,
+                                        int64_t *selected, double *metric) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        SEV_REQUIRE(selected != nullptr, "selected is NULL");
+        find_variable_features(p->m, p->g, p->cp.get(), p->cnt.get(), nfeatures, loess_span,
+                               selected, metric, p->st());
+        for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;
+    });
+}
+
 int sev_pipeline_knn_passes(sev_pipeline *p, int64_t *passes) {
     return guarded([&] {
         SEV_REQUIRE(passes != nullptr, "passes is NULL");
@@ -1122,6 +1134,31 @@ int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t
         download(U, dU.get(), (size_t)(m * nu), st.st);
         download(V, dV.get(), (size_t)(n * nu), st.st);
         SEV_HIP(hipStreamSynchronize(st.st));
+    });
+}
+
+int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                               const int32_t *nzval, int64_t nfeatures, double loess_span,
+                               int64_t *selected, double *metric) {
+    (void)rowval;  // the statistics only need each column's values in row order
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(m >= 2 && n >= 1 && colptr && nzval && selected, "bad arguments");
+        SEV_REQUIRE(colptr[0] == 1, "colptr must be 1-based");
+        std::vector<int64_t> cp((size_t)n + 1);
+        for (int64_t j = 0; j <= n; ++j) {
+            cp[(size_t)j] = colptr[j] - 1;
+            SEV_REQUIRE(j == 0 || cp[(size_t)j] >= cp[(size_t)j - 1], "colptr not monotone");
+        }
+        const int64_t nnz = cp[(size_t)n];
+        Stream s;
+        DevBuf<int64_t> dcp;
+        DevBuf<int32_t> dcnt;
+        upload(dcp, cp.data(), cp.size(), s.st);
+        upload(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+        find_variable_features(m, n, dcp.get(), dcnt.get(), nfeatures, loess_span, selected,
+                               metric, s.st);
+        for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;
     });
 }
 

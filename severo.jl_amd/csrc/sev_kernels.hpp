@@ -66,6 +66,19 @@ template <class T>
 void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, const T *clip,
                T *out, hipStream_t st);
 
+// mean_std(Float64, A) of Int32 counts per column (FP64 Welford in row order).
+void count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *x, double *mu,
+                    double *sd, hipStream_t st);
+
+// ---- sev_hvg.hip ---------------------------------------------------------
+// find_variable_features(:vst) on device-resident counts (cells x genes CSC,
+// rows sorted within columns): selected[0..nfeat) = 0-based genes by
+// descending clipped standardized variance (ties to the lower gene);
+// metric_out (host, G entries) optional.
+void find_variable_features(int64_t m, int64_t G, const int64_t *colptr_d, const int32_t *cnt_d,
+                            int64_t nfeat, double span, int64_t *selected, double *metric_out,
+                            hipStream_t st);
+
 // Batches of S samples stacked by rows of one cells x genes CSC.
 // soff[s*G + g] (s = 0..S): first entry of column g in sample >= s.
 void sample_offsets(int64_t G, int64_t S, const int32_t *row_sample, const int64_t *colptr,

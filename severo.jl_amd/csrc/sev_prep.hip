@@ -244,6 +244,36 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
     }
 }
 
+// mean_std(Float64, A) of the Int32 counts (src/scaling.jl:18-38,119-130):
+// the same one-wavefront-per-column Welford recurrence in FP64 (IEEE
+// division by the count), sd = sqrt(s / (m - 1)).
+__global__ void __launch_bounds__(kBlock)
+k_count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *x, double *muv,
+                 double *sdv) {
+    const int64_t col = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (col >= ncol) return;
+    const int lane = lane_id();
+    const int64_t b = colptr[col], e = colptr[col + 1];
+    int64_t count = m - (e - b);
+    double mu = 0.0, s = 0.0;
+    for (int64_t base = b; base < e; base += kWave) {
+        const int64_t p = base + lane;
+        const double vl = p < e ? (double)x[p] : 0.0;
+        const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
+        for (int t = 0; t < nt; ++t) {
+            const double v = lane_value(vl, t);
+            const double delta = v - mu;
+            mu = mu + delta / (double)(count + t + 1);
+            s = s + delta * (v - mu);
+        }
+        count += nt;
+    }
+    if (lane == 0) {
+        muv[col] = mu;
+        sdv[col] = sqrt(s / (double)(m - 1));
+    }
+}
+
 // b = min(x / sigma, clip) with only the upper clip (src/scaling.jl:210-213).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -384,6 +414,15 @@ template void gather_copy<float>(const int64_t *, const int64_t *, const int32_t
 template void gather_copy<double>(const int64_t *, const int64_t *, const int32_t *,
                                   const double *, int64_t, const int64_t *, int32_t *, double *,
                                   hipStream_t);
+void count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *x, double *mu,
+                    double *sd, hipStream_t st) {
+    if (ncol == 0) return;
+    ProfScope ps("count_stats", st);
+    hipLaunchKernelGGL(k_count_mean_std, dim3((unsigned)ceil_div(ncol * kWave, kBlock)),
+                       dim3(kBlock), 0, st, m, ncol, colptr, x, mu, sd);
+    SEV_LAUNCH_CHECK();
+}
+
 template void scale_stats<float>(int64_t, int64_t, const int64_t *, const float *, float, float *,
                                  float *, float *, hipStream_t);
 template void scale_stats<double>(int64_t, int64_t, const int64_t *, const double *, double,

@@ -191,6 +191,17 @@ class Pipeline:
         check(self._L.sev_pipeline_get_counts(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals)))
         return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, n))
 
+    def find_variable_features(self, nfeatures=2000, loess_span=0.5, return_metric=False):
+        """find_variable_features(:vst) on the loaded counts: 0-based genes,
+        best first."""
+        _, n, _ = self.counts_info()
+        sel = np.zeros(nfeatures, np.int64)
+        metric = np.zeros(n)
+        check(self._L.sev_pipeline_find_variable_features(self._h, nfeatures, float(loess_span),
+                                                          _ptr(sel), _ptr(metric)))
+        sel -= 1
+        return (sel, metric) if return_metric else sel
+
     # -- stages ------------------------------------------------------------
     def normalize(self, method="lognormalize", scale_factor=1e4):
         code = {"lognormalize": _lib.SEV_LOGNORMALIZE,

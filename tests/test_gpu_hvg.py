@@ -1,0 +1,56 @@
+"""GPU parity of find_variable_features(:vst) against the oracle: the
+per-gene variances after the loess fit (FP64, within 1e-9 relative: the
+device sums run in a different order) and the selected genes."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _counts(sev, orc, cells, genes, density, seed):
+    t = sev.nb_tables(genes, seed=seed, density=density)
+    cp, rv, cnt = orc.synth_csc(cells, t)
+    return t, sp.csc_matrix((cnt, rv, cp), shape=(cells, genes))
+
+
+@pytest.mark.parametrize("cells,genes,density,nfeat,span",
+                         [(5000, 3000, 0.05, 300, 0.5), (800, 1200, 0.1, 1200, 0.3),
+                          (3000, 500, 0.02, 50, 0.75)])
+def test_vst_matches_oracle(sev, orc, cells, genes, density, nfeat, span):
+    _, X = _counts(sev, orc, cells, genes, density, seed=genes)
+    sel, metric = sev.find_variable_features(X, nfeat, loess_span=span, return_metric=True)
+    rsel, rmetric, mu, sd, esd = orc.vst(cells, X.indptr, X.data, nfeatures=nfeat,
+                                         loess_span=span)
+    assert np.allclose(metric, rmetric, rtol=1e-9, atol=1e-300)
+    assert np.array_equal(metric == 0, rmetric == 0)
+    # the selection is the oracle's up to reordering among near ties (genes
+    # whose counts differ only in row order have variances 1 ulp apart in
+    # either implementation, and the two sum in different orders)
+    assert np.allclose(rmetric[sel], rmetric[rsel], rtol=1e-9, atol=0)
+    order = np.lexsort((np.arange(genes), -rmetric))
+    if nfeat < genes:
+        gap = rmetric[order[nfeat - 1]] - rmetric[order[nfeat]]
+        assert gap > 1e-8 * abs(rmetric[order[nfeat - 1]])
+        assert set(sel.tolist()) == set(rsel.tolist())
+    exact = np.array_equal(sel, rsel)
+    print(f"selection identical in order: {exact}")
+
+
+def test_vst_pipeline_equals_host_entry(sev, orc):
+    t, X = _counts(sev, orc, 4000, 2000, 0.05, seed=3)
+    p = sev.Pipeline()
+    p.synth_counts(4000, t)
+    sel_p, m_p = p.find_variable_features(200, return_metric=True)
+    sel_h, m_h = sev.find_variable_features(p.get_counts(), 200, return_metric=True)
+    assert np.array_equal(sel_p, sel_h) and np.array_equal(m_p, m_h)
+
+
+def test_vst_constant_genes_rank_last(sev):
+    rng = np.random.default_rng(9)
+    X = rng.poisson(0.5, (600, 40)).astype(np.int32)
+    X[:, :5] = 0
+    X[:, 5:8] = 2
+    sel, metric = sev.find_variable_features(sp.csc_matrix(X), 40, return_metric=True)
+    assert np.all(metric[:8] == 0)
+    assert set(sel[-8:].tolist()) == set(range(8))

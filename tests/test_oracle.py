@@ -369,3 +369,84 @@ def test_welford_quotient_identity():
         ref = d / cf
         fast = (d.astype(np.float64) * (1.0 / cf.astype(np.float64))).astype(np.float32)
         assert np.array_equal(fast.view(np.uint32), ref.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# find_variable_features(:vst) pieces (src/variablefeatures.jl:21-50)
+# ---------------------------------------------------------------------------
+def _count_csc(seed, m=400, n=60):
+    rng = np.random.default_rng(seed)
+    lam = np.exp(rng.normal(-1.0, 1.5, n))
+    X = rng.poisson(lam[None, :] * rng.uniform(0.3, 2.0, (m, 1))).astype(np.int32)
+    X[:, 5] = 0          # an all-zero (constant) gene
+    X[:, 7] = 3          # a constant nonzero gene
+    return sp.csc_matrix(X), X
+
+
+def test_count_mean_std_vs_numpy(orc):
+    S, X = _count_csc(1)
+    mu, sd = orc.count_mean_std(X.shape[0], S.indptr, S.data)
+    assert np.allclose(mu, X.mean(axis=0), rtol=1e-13, atol=0)
+    assert np.allclose(sd, X.std(axis=0, ddof=1), rtol=1e-12, atol=1e-15)
+    assert sd[5] == 0.0 and sd[7] == 0.0
+
+
+def test_loess_reproduces_quadratics(orc):
+    rng = np.random.default_rng(2)
+    x = np.sort(np.concatenate([rng.uniform(-2, 3, 300), np.full(20, 0.5)]))  # with ties
+    y = 1.5 - 0.7 * x + 0.3 * x * x
+    for span in (0.1, 0.5, 0.9):
+        assert np.abs(orc.loess_fit(x, y, span) - y).max() < 1e-10
+
+
+def _loess_bruteforce(x, y, span):
+    """The definition, written independently: the q-th smallest distance,
+    tricube weights, numpy weighted least squares per point."""
+    N = len(x)
+    q = max(min(N, 3), min(N, int(np.floor(N * span))))
+    out = np.empty(N)
+    for i in range(N):
+        d = np.abs(x - x[i])
+        dmax = np.sort(d)[q - 1]
+        r = d / dmax
+        w = np.where(r < 1, (1 - r ** 3) ** 3, 0.0)
+        u = (x - x[i]) / dmax
+        A = np.stack([np.ones(N), u, u * u], axis=1) * np.sqrt(w)[:, None]
+        coef = np.linalg.lstsq(A, y * np.sqrt(w), rcond=None)[0]
+        out[i] = coef[0]
+    return out
+
+
+def test_loess_vs_weighted_least_squares(orc):
+    rng = np.random.default_rng(3)
+    x = np.sort(np.round(rng.normal(0, 1, 250), 2))  # rounded: many tied x
+    y = np.sin(2 * x) + 0.1 * rng.standard_normal(len(x))
+    for span in (0.2, 0.5):
+        f = orc.loess_fit(x, y, span)
+        assert np.allclose(f, _loess_bruteforce(x, y, span), rtol=1e-8, atol=1e-10)
+    # a wrong neighbourhood (one point fewer) is detected
+    f_bad = _loess_bruteforce(x, y, 0.5 - 1.0 / len(x))
+    assert not np.allclose(orc.loess_fit(x, y, 0.5), f_bad, rtol=1e-8, atol=1e-10)
+
+
+def test_standardized_var_closed_form(orc):
+    S, X = _count_csc(4)
+    m = X.shape[0]
+    mu = X.mean(axis=0)
+    esd = np.abs(np.random.default_rng(5).normal(1.0, 0.3, X.shape[1])) + 0.05
+    esd[5] = 0.0
+    v = orc.standardized_var(m, S.indptr, S.data, mu, esd)
+    Z = np.minimum((X - mu[None, :]) / np.where(esd > 0, esd, 1)[None, :], np.sqrt(m))
+    ref = (Z ** 2).sum(axis=0) / (m - 1)
+    ref[5] = 0.0
+    assert np.allclose(v, ref, rtol=1e-12, atol=0)
+    # the clip at sqrt(m) matters for a rare, large count
+    assert (((X - mu) / np.where(esd > 0, esd, 1)) > np.sqrt(m)).any()
+
+
+def test_vst_selection_order(orc):
+    S, X = _count_csc(6, m=500, n=80)
+    sel, metric, mu, sd, esd = orc.vst(X.shape[0], S.indptr, S.data, nfeatures=20)
+    assert np.array_equal(sel, np.lexsort((np.arange(80), -metric))[:20])
+    assert metric[5] == 0.0 and metric[7] == 0.0
+    assert np.all(np.diff(metric[sel]) <= 0)
