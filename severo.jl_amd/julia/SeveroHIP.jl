@@ -20,7 +20,7 @@ const libsev = get(ENV, "SEVERO_HIP_LIB", joinpath(@__DIR__, "..", "libsevero_hi
 
 const SEV_FLOAT32, SEV_FLOAT64 = Cint(0), Cint(1)
 const SEV_INT32, SEV_INT64 = Cint(0), Cint(1)
-const SEV_PREC = Dict(:fp32 => Cint(0), :bf16 => Cint(1), :exact => Cint(2))
+const SEV_PREC = Dict(:fp32 => Cint(0), :bf16 => Cint(1), :exact => Cint(2), :fp8 => Cint(3))
 
 function check(rc::Integer)
     rc == 0 && return nothing
@@ -52,6 +52,23 @@ end
 
 normalize_cells(X::NamedArray; kw...) = NamedArray(normalize_cells(X.array; kw...), X.dicts,
                                                    X.dimnames)
+
+# ------------------------------------------------------ variable features
+# src/variablefeatures.jl:128-161 (method :vst; the loess is the exact local
+# fit at every gene)
+function find_variable_features(counts::NamedArray, nfeatures::Integer=2000; method=:vst,
+                                loess_span::Real=0.5)
+    Symbol(method) == :vst || error("unknown selection method: $method")
+    X = counts.array
+    m, n = size(X)
+    nz = convert(Vector{Int32}, nonzeros(X))
+    selected = Vector{Int64}(undef, nfeatures)
+    check(ccall((:sev_find_variable_features, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}, Int64, Cdouble, Ptr{Int64},
+                 Ptr{Cdouble}),
+                m, n, X.colptr, X.rowval, nz, nfeatures, loess_span, selected, C_NULL))
+    NamedArray(selected, (names(counts, 2)[selected],), (dimnames(counts, 2),))
+end
 
 # ----------------------------------------------------------------- scaling
 # src/scaling.jl:199-217, :335-357
