@@ -193,10 +193,6 @@ __device__ __forceinline__ double lane_value(double x, int t) {
 // every FP32 rounding boundary -- it cannot be a tie, since an odd 25-bit
 // significand times F's significand is no 24-bit float -- so the rounding is
 // the IEEE FP32 division's.  Subnormal deltas divide directly.
-__device__ __forceinline__ float welford_quot(float delta, double rc, float countf, bool fast) {
-    if (fast && fabsf(delta) >= 0x1p-100f) return (float)((double)delta * rc);
-    return delta / countf;
-}
 __device__ __forceinline__ double welford_quot(double delta, double, double countf, bool) {
     return delta / countf;
 }
@@ -222,14 +218,36 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
         const T vl = p < e ? x[p] : T(0);
         const double rcl = fast ? 1.0 / (double)(float)(count + lane + 1) : 0.0;
         const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
-#pragma unroll
-        for (int t = 0; t < kWave; ++t) {
-            if (t < nt) {
-                const T v = lane_value(vl, t);
-                const T delta = v - mu;
-                mu = mu + welford_quot(delta, fast ? lane_value(rcl, t) : 0.0,
-                                       (T)(count + t + 1), fast);
+        if constexpr (sizeof(T) == 4) {
+            // every lane holds the same mu and delta, so the rare exact
+            // division (tiny nonzero deltas) is a wave-uniform scalar branch,
+            // and full chunks run without per-element bounds checks
+            auto step = [&](int t) {
+                const float v = lane_value(vl, t);
+                const float delta = v - mu;
+                float q = (float)((double)delta * lane_value(rcl, t));
+                if (__builtin_expect(__builtin_amdgcn_readfirstlane(
+                                         (int)(fabsf(delta) < 0x1p-100f && delta != 0.0f)),
+                                     0))
+                    q = delta / (float)(count + t + 1);
+                mu = mu + q;
                 s = s + delta * (v - mu);
+            };
+            if (nt == kWave) {
+#pragma unroll
+                for (int t = 0; t < kWave; ++t) step(t);
+            } else {
+                for (int t = 0; t < nt; ++t) step(t);
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < kWave; ++t) {
+                if (t < nt) {
+                    const T v = lane_value(vl, t);
+                    const T delta = v - mu;
+                    mu = mu + welford_quot(delta, 0.0, (T)(count + t + 1), false);
+                    s = s + delta * (v - mu);
+                }
             }
         }
         count += nt;
