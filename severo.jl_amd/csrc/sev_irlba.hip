@@ -806,93 +806,136 @@ __device__ void small_svd_body(int n, const double *B, double *U, double *s, dou
                                int *status) {
     extern __shared__ double sm[];
     double *G = sm;
+    // second array: B itself (for U = B V diag(1/s)), or the accumulated
+    // rotations when B is too ill-conditioned for that product
     double *J = sm + n * n;
-    __shared__ int rotated;
+    __shared__ int rotated, need_j, nsweeps;
     __shared__ double sv[kMaxWork];
     const int tid = threadIdx.x;
-    for (int e = tid; e < n * n; e += kSvdThreads) {
-        const int i = e % n, j = e / n;
-        G[e] = B[(int64_t)i * n + j];
-        J[e] = i == j ? 1.0 : 0.0;
-    }
     const int ne = n + (n & 1);
     const int pr = tid >> 4, gl = tid & 15;
     const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
-    __syncthreads();
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        if (tid == 0) rotated = 0;
+    if (tid == 0) {
+        need_j = 0;
+        nsweeps = 0;
+    }
+    auto load = [&](bool withj) {
+        for (int e = tid; e < n * n; e += kSvdThreads) {
+            const int i = e % n, j = e / n;
+            G[e] = B[(int64_t)i * n + j];
+            J[e] = withj ? (i == j ? 1.0 : 0.0) : B[e];
+        }
         __syncthreads();
-        for (int r = 0; r < ne - 1; ++r) {
-            int p = r, q = ne - 1;
-            if (pr > 0) {
-                p = (r + pr) % (ne - 1);
-                q = (r - pr + 2 * (ne - 1)) % (ne - 1);
-            }
-            if (p > q) {
-                const int t = p;
-                p = q;
-                q = t;
-            }
-            const bool valid = pr < ne / 2 && q < n;
-            double *gp = G + p * n, *gq = G + q * n;
-            double x[4], y[4];
-            double app = 0.0, aqq = 0.0, apq = 0.0;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = gl + 16 * e;
-                const bool in = valid && i < n;
-                x[e] = in ? gp[i] : 0.0;
-                y[e] = in ? gq[i] : 0.0;
-                app = fma(x[e], x[e], app);
-                aqq = fma(y[e], y[e], aqq);
-                apq = fma(x[e], y[e], apq);
-            }
-            app = sum16(app);
-            aqq = sum16(aqq);
-            apq = sum16(apq);
-            if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
-                if (gl == 0) rotated = 1;
-                // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2),
-                // with Newton-refined hardware reciprocals (the rotation
-                // needs c^2 + s^2 = 1 to working precision, not rounding)
-                const double zeta = (aqq - app) * rcp_nr(2.0 * apq);
-                const double az = fabs(zeta);
-                double t;
-                if (az > 1e150) {
-                    t = 0.5 * rcp_nr(zeta);
-                } else {
-                    const double x = fma(az, az, 1.0);
-                    t = copysign(rcp_nr(az + x * rsq_nr(x)), zeta);
+    };
+    auto sweeps = [&](bool withj) {
+        for (int sweep = 0; sweep < 100; ++sweep) {
+            if (tid == 0) rotated = 0;
+            __syncthreads();
+            for (int r = 0; r < ne - 1; ++r) {
+                int p = r, q = ne - 1;
+                if (pr > 0) {
+                    p = (r + pr) % (ne - 1);
+                    q = (r - pr + 2 * (ne - 1)) % (ne - 1);
                 }
-                const double c = rsq_nr(fma(t, t, 1.0));
-                const double sn = c * t;
-                double *jp = J + p * n, *jq = J + q * n;
+                if (p > q) {
+                    const int t = p;
+                    p = q;
+                    q = t;
+                }
+                const bool valid = pr < ne / 2 && q < n;
+                double *gp = G + p * n, *gq = G + q * n;
+                double x[4], y[4];
+                double app = 0.0, aqq = 0.0, apq = 0.0;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int i = gl + 16 * e;
-                    if (i < n) {
-                        gp[i] = c * x[e] - sn * y[e];
-                        gq[i] = sn * x[e] + c * y[e];
-                        const double u = jp[i], v = jq[i];
-                        jp[i] = c * u - sn * v;
-                        jq[i] = sn * u + c * v;
+                    const bool in = valid && i < n;
+                    x[e] = in ? gp[i] : 0.0;
+                    y[e] = in ? gq[i] : 0.0;
+                    app = fma(x[e], x[e], app);
+                    aqq = fma(y[e], y[e], aqq);
+                    apq = fma(x[e], y[e], apq);
+                }
+                app = sum16(app);
+                aqq = sum16(aqq);
+                apq = sum16(apq);
+                if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
+                    if (gl == 0) rotated = 1;
+                    // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2),
+                    // with Newton-refined hardware reciprocals (the rotation
+                    // needs c^2 + s^2 = 1 to working precision, not rounding)
+                    const double zeta = (aqq - app) * rcp_nr(2.0 * apq);
+                    const double az = fabs(zeta);
+                    double t;
+                    if (az > 1e150) {
+                        t = 0.5 * rcp_nr(zeta);
+                    } else {
+                        const double xx = fma(az, az, 1.0);
+                        t = copysign(rcp_nr(az + xx * rsq_nr(xx)), zeta);
+                    }
+                    const double c = rsq_nr(fma(t, t, 1.0));
+                    const double sn = c * t;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int i = gl + 16 * e;
+                        if (i < n) {
+                            gp[i] = c * x[e] - sn * y[e];
+                            gq[i] = sn * x[e] + c * y[e];
+                        }
+                    }
+                    if (withj) {
+                        double *jp = J + p * n, *jq = J + q * n;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int i = gl + 16 * e;
+                            if (i < n) {
+                                const double u = jp[i], v = jq[i];
+                                jp[i] = c * u - sn * v;
+                                jq[i] = sn * u + c * v;
+                            }
+                        }
                     }
                 }
+                __syncthreads();
             }
+            const int any = rotated;
             __syncthreads();
+            if (tid == 0) nsweeps = nsweeps + 1;
+            if (!any) break;
         }
-        const int any = rotated;
+    };
+    auto norms = [&]() {
+        if (tid < n) {
+            double t = 0.0;
+            for (int i = 0; i < n; ++i) t += G[tid * n + i] * G[tid * n + i];
+            sv[tid] = sqrt(t);
+        }
         __syncthreads();
-        if (tid == 0) status[1] = sweep + 1;
-        if (!any) break;
-    }
-    if (tid < n) {
-        double t = 0.0;
-        for (int i = 0; i < n; ++i) t += G[tid * n + i] * G[tid * n + i];
-        sv[tid] = sqrt(t);
+    };
+    load(false);
+    sweeps(false);
+    norms();
+    // U = B V diag(1/s) is accurate to eps s_max / s_j: without the
+    // accumulated rotations only while s_min >= 1e-5 s_max
+    if (tid == 0) {
+        double mx = 0.0, mn = __builtin_huge_val();
+        for (int i = 0; i < n; ++i) {
+            mx = fmax(mx, sv[i]);
+            mn = fmin(mn, sv[i]);
+        }
+        need_j = !(mn >= 1e-5 * mx);
     }
     __syncthreads();
-    if (tid == 0) status[0] = 0;
+    const bool withj = need_j != 0;
+    if (withj) {
+        load(true);
+        sweeps(true);
+        norms();
+    }
+    if (tid == 0) {
+        status[0] = 0;
+        status[1] = nsweeps;
+    }
     __syncthreads();
     for (int j = tid >> 6; j < n; j += kSvdThreads / 64) {
         const int l = tid & 63;
@@ -904,7 +947,14 @@ __device__ void small_svd_body(int n, const double *B, double *U, double *s, dou
             if (!(sj > 0.0)) status[0] = 1;
         }
         for (int i = l; i < n; i += 64) {
-            U[(int64_t)rk * n + i] = J[j * n + i];
+            if (withj) {
+                U[(int64_t)rk * n + i] = J[j * n + i];
+            } else {
+                // (B g_j)_i / s_j^2, g_j = s_j v_j the column of G
+                double t = 0.0;
+                for (int c = 0; c < n; ++c) t = fma(J[i + c * n], G[j * n + c], t);
+                U[(int64_t)rk * n + i] = t / (sj * sj);
+            }
             V[(int64_t)rk * n + i] = sj > 0.0 ? G[j * n + i] / sj : 0.0;
         }
     }

@@ -59,6 +59,22 @@ def test_irlba_dense(sev, m, n, nu):
     assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
 
 
+def test_irlba_graded_spectrum(sev):
+    """Singular values spread over eight decades: the projected matrices are
+    too ill-conditioned for U = B V / s, and the device SVD takes its
+    accumulated-rotation path."""
+    rng = np.random.default_rng(12)
+    m, n, nu = 300, 120, 10
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sig = 10.0 ** (-0.5 * np.arange(n))
+    X = (Q1 * sig) @ Q2.T
+    S = sev.irlba(sp.csc_matrix(X), nu, tol=1e-8, init=rng.standard_normal(n))
+    assert np.allclose(S.S, sig[:nu], rtol=1e-7)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-7
+    assert _angle(S.U, Q1[:, :nu]) < 1e-6 and _angle(S.V, Q2[:, :nu]) < 1e-6
+
+
 def test_irlba_sparse_two(sev):
     X = sp.random(2000, 400, 0.1, format="csc", random_state=7, dtype=np.float64)
     S = sev.irlba(X, 2, tol=1e-9, init=np.random.default_rng(1).standard_normal(400))
