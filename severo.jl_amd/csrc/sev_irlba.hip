@@ -275,9 +275,54 @@ k_av_csr(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const
 __device__ __forceinline__ float bf_lo(uint32_t u) { return __uint_as_float(u << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t u) { return __uint_as_float(u & 0xFFFF0000u); }
 
-template <int G>
+// 12-bit packed indices (columns of A v when n <= 4096, in-slab rows of A' w):
+// entries 8g .. 8g+7 share three 32-bit words, entry e at bits 12e .. 12e+11
+// of the little-endian 96-bit group.
+__device__ __forceinline__ void unpack12(const uint3 w, uint32_t c[8]) {
+    c[0] = w.x & 0xFFFu;
+    c[1] = (w.x >> 12) & 0xFFFu;
+    c[2] = (w.x >> 24) | ((w.y & 0xFu) << 8);
+    c[3] = (w.y >> 4) & 0xFFFu;
+    c[4] = (w.y >> 16) & 0xFFFu;
+    c[5] = (w.y >> 28) | ((w.z & 0xFFu) << 4);
+    c[6] = (w.z >> 8) & 0xFFFu;
+    c[7] = w.z >> 20;
+}
+
+// the 8 indices of entries p .. p+7 (p a multiple of 8)
+template <bool C12>
+__device__ __forceinline__ void load_idx8(const void *idx, int64_t p, uint32_t c[8]) {
+    if constexpr (C12) {
+        unpack12(((const uint3 *)idx)[p >> 3], c);
+    } else {
+        const uint4 c8 = *(const uint4 *)((const uint16_t *)idx + p);
+        c[0] = c8.x & 0xFFFFu;
+        c[1] = c8.x >> 16;
+        c[2] = c8.y & 0xFFFFu;
+        c[3] = c8.y >> 16;
+        c[4] = c8.z & 0xFFFFu;
+        c[5] = c8.z >> 16;
+        c[6] = c8.w & 0xFFFFu;
+        c[7] = c8.w >> 16;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, const uint16_t *val,
+k_pack12(const uint16_t *in, int64_t nent, int64_t ngroups, uint32_t *out) {
+    for (int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x; g < ngroups;
+         g += (int64_t)gridDim.x * kBlock) {
+        uint32_t c[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) c[e] = 8 * g + e < nent ? (in[8 * g + e] & 0xFFFu) : 0u;
+        out[3 * g] = c[0] | (c[1] << 12) | (c[2] << 24);
+        out[3 * g + 1] = (c[2] >> 8) | (c[3] << 4) | (c[4] << 16) | (c[5] << 28);
+        out[3 * g + 2] = (c[5] >> 4) | (c[6] << 8) | (c[7] << 20);
+    }
+}
+
+template <int G, bool C12>
+__global__ void __launch_bounds__(kBlock)
+k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const void *col, const uint16_t *val,
           const float *mu, const double *v, const double *rf, const double *wprev, double *y) {
     extern __shared__ float vf[];
     __shared__ double dmu_s;
@@ -302,16 +347,17 @@ k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const uint16_t *col, cons
         if (r < m) {
             const int64_t b = rowptr[r], e = rowptr[r + 1];
             for (int64_t p = b + 8 * gl; p < e; p += 8 * G) {
-                const uint4 c8 = *(const uint4 *)(col + p);
+                uint32_t c[8];
+                load_idx8<C12>(col, p, c);
                 const uint4 v8 = *(const uint4 *)(val + p);
-                a0 = fmaf(bf_lo(v8.x), vf[c8.x & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.x), vf[c8.x >> 16], a1);
-                a0 = fmaf(bf_lo(v8.y), vf[c8.y & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.y), vf[c8.y >> 16], a1);
-                a0 = fmaf(bf_lo(v8.z), vf[c8.z & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.z), vf[c8.z >> 16], a1);
-                a0 = fmaf(bf_lo(v8.w), vf[c8.w & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.w), vf[c8.w >> 16], a1);
+                a0 = fmaf(bf_lo(v8.x), vf[c[0]], a0);
+                a1 = fmaf(bf_hi(v8.x), vf[c[1]], a1);
+                a0 = fmaf(bf_lo(v8.y), vf[c[2]], a0);
+                a1 = fmaf(bf_hi(v8.y), vf[c[3]], a1);
+                a0 = fmaf(bf_lo(v8.z), vf[c[4]], a0);
+                a1 = fmaf(bf_hi(v8.z), vf[c[5]], a1);
+                a0 = fmaf(bf_lo(v8.w), vf[c[6]], a0);
+                a1 = fmaf(bf_hi(v8.w), vf[c[7]], a1);
             }
         }
         float acc = a0 + a1;
@@ -461,9 +507,34 @@ k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *
 }
 
 // BF16 form of the slab partials of B' y: y staged in LDS as FP32, FP32
-// accumulation per column segment (the slab sums are reduced in FP64).
+// accumulation per column segment (the slab sums are reduced in FP64).  The
+// in-slab rows are 12-bit packed; each lane reads the 8-aligned groups of 8
+// entries covering the segment [a, b) (entries outside it are masked; the
+// value and index arrays are padded so the cover stays in bounds).
+__device__ __forceinline__ float atw_seg_bf16(int64_t a, int64_t b, const uint32_t *lrow12,
+                                              const uint16_t *val, const float *wl, int lane) {
+    float acc0 = 0.0f, acc1 = 0.0f;
+    for (int64_t p = (a & ~(int64_t)7) + 8 * lane; p < b; p += 8 * kWave) {
+        uint32_t l[8];
+        load_idx8<true>(lrow12, p, l);
+        const uint4 v8 = *(const uint4 *)(val + p);
+        const uint32_t vw[4] = {v8.x, v8.y, v8.z, v8.w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const bool ok = p + e >= a && p + e < b;
+            const float v = (e & 1) ? bf_hi(vw[e >> 1]) : bf_lo(vw[e >> 1]);
+            const float t = ok ? v : 0.0f;
+            if (e & 1)
+                acc1 = fmaf(t, wl[ok ? l[e] : 0], acc1);
+            else
+                acc0 = fmaf(t, wl[ok ? l[e] : 0], acc0);
+        }
+    }
+    return acc0 + acc1;
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const uint16_t *val,
+k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint32_t *lrow12, const uint16_t *val,
            const double *y, double *part) {
     __shared__ float wl[kSlabRows];
     const int64_t s = blockIdx.x;
@@ -477,18 +548,7 @@ k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const
     double *prow = part + s * (n + 2);
     for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
         const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
-        float acc0 = 0.0f, acc1 = 0.0f;
-        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
-            const ushort4 l4 = *(const ushort4 *)(lrow + p);
-            const uint2 v4 = *(const uint2 *)(val + p);
-            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
-                       v3 = p + 3 < b;
-            acc0 = fmaf(v0 ? bf_lo(v4.x) : 0.0f, wl[v0 ? l4.x : 0], acc0);
-            acc1 = fmaf(v1 ? bf_hi(v4.x) : 0.0f, wl[v1 ? l4.y : 0], acc1);
-            acc0 = fmaf(v2 ? bf_lo(v4.y) : 0.0f, wl[v2 ? l4.z : 0], acc0);
-            acc1 = fmaf(v3 ? bf_hi(v4.y) : 0.0f, wl[v3 ? l4.w : 0], acc1);
-        }
-        const float acc = wave_sum(acc0 + acc1);
+        const float acc = wave_sum(atw_seg_bf16(a, b, lrow12, val, wl, lane));
         if (lane == 0) prow[j] = (double)acc;
     }
 }
@@ -1068,6 +1128,16 @@ void launch_csr_fill(int64_t m, int64_t n, int64_t nbs, const int64_t *boff, con
 
 }  // namespace
 
+// 12-bit copy of a 16-bit index array (nent entries, readable through the
+// 8-entry group covering nent + kValuePad).
+static void pack12(const uint16_t *in, int64_t nent, DevBuf<uint32_t> &out, hipStream_t st) {
+    const int64_t ngroups = ceil_div(nent + kValuePad, (int64_t)8);
+    out.ensure((size_t)(3 * ngroups));
+    hipLaunchKernelGGL(k_pack12, dim3(grid_rows(ngroups)), dim3(kBlock), 0, st, in, nent,
+                       ngroups, out.get());
+    SEV_LAUNCH_CHECK();
+}
+
 void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                     const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
                     hipStream_t st) {
@@ -1100,6 +1170,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         }
         op.vals = op.vbf16.get();
         op.vtype = kValBF16;
+        pack12(op.lrow.get(), op.nnz, op.lrow12, st);
     }
     // padded CSR for A v
     ProfScope ps("csr_build", st, 2.0 * 10.0 * op.nnz);
@@ -1141,6 +1212,8 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     else
         launch_csr_fill<float, float>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
                                       op.ccol.get(), op.cval.get(), st);
+    op.col12 = op.vtype == kValBF16 && n <= 4096;
+    if (op.col12) pack12(op.ccol.get(), op.csr_nnz, op.ccol12, st);
     const double avg = (double)op.csr_nnz / (double)m;
     if (op.vtype == kValBF16)
         op.lanes_per_row = avg > 256 ? 16 : (avg > 64 ? 8 : 4);
@@ -1180,9 +1253,16 @@ void launch_av_bf16_g(const SlabCsc &op, const double *v, const double *rf, cons
                       double *y, hipStream_t st) {
     const unsigned grid = (unsigned)std::min<int64_t>(
         ceil_div(op.m, (int64_t)kWaves * (kWave / G)), (int64_t)device_cu_count() * 8);
-    hipLaunchKernelGGL((k_av_bf16<G>), dim3(grid), dim3(kBlock), sizeof(float) * (size_t)op.n, st,
-                       op.m, op.n, op.rowptr.get(), op.ccol.get(), (const uint16_t *)op.cval.get(),
-                       (const float *)op.mu, v, rf, wprev, y);
+    if (op.col12)
+        hipLaunchKernelGGL((k_av_bf16<G, true>), dim3(grid), dim3(kBlock),
+                           sizeof(float) * (size_t)op.n, st, op.m, op.n, op.rowptr.get(),
+                           (const void *)op.ccol12.get(), (const uint16_t *)op.cval.get(),
+                           (const float *)op.mu, v, rf, wprev, y);
+    else
+        hipLaunchKernelGGL((k_av_bf16<G, false>), dim3(grid), dim3(kBlock),
+                           sizeof(float) * (size_t)op.n, st, op.m, op.n, op.rowptr.get(),
+                           (const void *)op.ccol.get(), (const uint16_t *)op.cval.get(),
+                           (const float *)op.mu, v, rf, wprev, y);
     SEV_LAUNCH_CHECK();
 }
 
@@ -1199,9 +1279,10 @@ void launch_av_bf16(const SlabCsc &op, const double *v, const double *rf, const 
 void launch_av(const SlabCsc &op, const double *v, const double *rf, const double *wprev,
                double *y, hipStream_t st) {
     const int use_lds = op.n <= kMaxFL ? 1 : 0;
+    const bool bf = op.vtype == kValBF16 && op.n <= kMaxFL32;
     ProfScope ps("spmv_av", st,
-                 (double)op.csr_nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m * (wprev ? 2 : 1) +
-                     8.0 * (op.m + 1) + 8.0 * op.n);
+                 (double)op.csr_nnz * (val_bytes(op.vtype) + (bf && op.col12 ? 1.5 : 2.0)) +
+                     8.0 * op.m * (wprev ? 2 : 1) + 8.0 * (op.m + 1) + 8.0 * op.n);
     if (op.vtype == kValBF16 && op.n <= kMaxFL32)
         launch_av_bf16(op, v, rf, wprev, y, st);
     else if (op.vtype == kValBF16)
@@ -1222,12 +1303,12 @@ int atw_groups(const SlabCsc &op) { return op.atw_groups > 0 ? op.atw_groups : a
 
 void launch_atw_only(const SlabCsc &op, const double *y, double *part, hipStream_t st) {
     ProfScope ps("spmv_atw", st,
-                 (double)op.nnz * (val_bytes(op.vtype) + 2) + 8.0 * op.m +
-                     8.0 * (op.nslab + 1) * op.n);
+                 (double)op.nnz * (val_bytes(op.vtype) + (op.vtype == kValBF16 ? 1.5 : 2.0)) +
+                     8.0 * op.m + 8.0 * (op.nslab + 1) * op.n);
     const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
     if (op.vtype == kValBF16)
         hipLaunchKernelGGL(k_atw_bf16, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const uint16_t *)op.vals, y, part);
+                           op.lrow12.get(), (const uint16_t *)op.vals, y, part);
     else if (op.vtype == kValF64)
         hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
                            op.lrow.get(), (const double *)op.vals, y, part);
@@ -1668,9 +1749,9 @@ __device__ __forceinline__ bool b_active(const BDev &d, int s, int j) {
     return !d.done[s] && j >= d.j0[s];
 }
 
-template <int G>
+template <int G, bool C12>
 __global__ void __launch_bounds__(kBlock)
-k_b_av_bf16(BDev d, const int64_t *rowptr, const uint16_t *col, const uint16_t *val,
+k_b_av_bf16(BDev d, const int64_t *rowptr, const void *col, const uint16_t *val,
             const float *mu_all, const double *Vall, int j, const double *scal, const double *W,
             double *y) {
     extern __shared__ float vf[];
@@ -1702,16 +1783,17 @@ k_b_av_bf16(BDev d, const int64_t *rowptr, const uint16_t *col, const uint16_t *
         if (r < re) {
             const int64_t b = rowptr[r], e = rowptr[r + 1];
             for (int64_t p = b + 8 * gl; p < e; p += 8 * G) {
-                const uint4 c8 = *(const uint4 *)(col + p);
+                uint32_t c[8];
+                load_idx8<C12>(col, p, c);
                 const uint4 v8 = *(const uint4 *)(val + p);
-                a0 = fmaf(bf_lo(v8.x), vf[c8.x & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.x), vf[c8.x >> 16], a1);
-                a0 = fmaf(bf_lo(v8.y), vf[c8.y & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.y), vf[c8.y >> 16], a1);
-                a0 = fmaf(bf_lo(v8.z), vf[c8.z & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.z), vf[c8.z >> 16], a1);
-                a0 = fmaf(bf_lo(v8.w), vf[c8.w & 0xFFFFu], a0);
-                a1 = fmaf(bf_hi(v8.w), vf[c8.w >> 16], a1);
+                a0 = fmaf(bf_lo(v8.x), vf[c[0]], a0);
+                a1 = fmaf(bf_hi(v8.x), vf[c[1]], a1);
+                a0 = fmaf(bf_lo(v8.y), vf[c[2]], a0);
+                a1 = fmaf(bf_hi(v8.y), vf[c[3]], a1);
+                a0 = fmaf(bf_lo(v8.z), vf[c[4]], a0);
+                a1 = fmaf(bf_hi(v8.z), vf[c[5]], a1);
+                a0 = fmaf(bf_lo(v8.w), vf[c[6]], a0);
+                a1 = fmaf(bf_hi(v8.w), vf[c[7]], a1);
             }
         }
         float acc = a0 + a1;
@@ -1816,7 +1898,7 @@ k_b_apply(BDev d, const double *W, int j, const double *T, const double *y, doub
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_b_atw_bf16(BDev d, int j, const int64_t *off, const uint16_t *lrow, const uint16_t *val,
+k_b_atw_bf16(BDev d, int j, const int64_t *off, const uint32_t *lrow12, const uint16_t *val,
              const double *y, double *part) {
     __shared__ float wl[kSlabRows];
     const int64_t slab = blockIdx.x;
@@ -1833,18 +1915,7 @@ k_b_atw_bf16(BDev d, int j, const int64_t *off, const uint16_t *lrow, const uint
     double *prow = part + slab * (n + 2);
     for (int64_t jj = (int64_t)g * kWaves + w; jj < n; jj += (int64_t)ng * kWaves) {
         const int64_t a = off[slab * n + jj], b = off[(slab + 1) * n + jj];
-        float acc0 = 0.0f, acc1 = 0.0f;
-        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
-            const ushort4 l4 = *(const ushort4 *)(lrow + p);
-            const uint2 v4 = *(const uint2 *)(val + p);
-            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
-                       v3 = p + 3 < b;
-            acc0 = fmaf(v0 ? bf_lo(v4.x) : 0.0f, wl[v0 ? l4.x : 0], acc0);
-            acc1 = fmaf(v1 ? bf_hi(v4.x) : 0.0f, wl[v1 ? l4.y : 0], acc1);
-            acc0 = fmaf(v2 ? bf_lo(v4.y) : 0.0f, wl[v2 ? l4.z : 0], acc0);
-            acc1 = fmaf(v3 ? bf_hi(v4.y) : 0.0f, wl[v3 ? l4.w : 0], acc1);
-        }
-        const float acc = wave_sum(acc0 + acc1);
+        const float acc = wave_sum(atw_seg_bf16(a, b, lrow12, val, wl, lane));
         if (lane == 0) prow[jj] = (double)acc;
     }
 }
@@ -2206,6 +2277,7 @@ void batch_slabs(SlabCsc &op, const BatchLayout &L, const int64_t *colptr, const
                        slab_of_row.get(), L.slab_row0_d.get(), colptr, row, op.off.get(),
                        op.lrow.get());
     SEV_LAUNCH_CHECK();
+    if (op.vtype == kValBF16) pack12(op.lrow.get(), op.nnz, op.lrow12, st);
     SEV_HIP(hipStreamSynchronize(st));
 }
 
@@ -2344,31 +2416,29 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
         double *Vc = w.V[cur].get(), *Vn = w.V[cur ^ 1].get();
         for (int j = jmin; j < work; ++j) {
             double *y = Wc + (int64_t)j * M;
-            switch (G) {
-                case 4:
-                    hipLaunchKernelGGL(k_b_av_bf16<4>, dim3((unsigned)nblk_av), dim3(kBlock),
-                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
-                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
-                                       w.scal.get(), Wc, y);
-                    break;
-                case 8:
-                    hipLaunchKernelGGL(k_b_av_bf16<8>, dim3((unsigned)nblk_av), dim3(kBlock),
-                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
-                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
-                                       w.scal.get(), Wc, y);
-                    break;
-                case 16:
-                    hipLaunchKernelGGL(k_b_av_bf16<16>, dim3((unsigned)nblk_av), dim3(kBlock),
-                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
-                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
-                                       w.scal.get(), Wc, y);
-                    break;
-                default:
-                    hipLaunchKernelGGL(k_b_av_bf16<32>, dim3((unsigned)nblk_av), dim3(kBlock),
-                                       sizeof(float) * n, st, d, op.rowptr.get(), op.ccol.get(),
-                                       (const uint16_t *)op.cval.get(), mu_all, Vc, j,
-                                       w.scal.get(), Wc, y);
-                    break;
+            {
+                const void *colp = op.col12 ? (const void *)op.ccol12.get()
+                                            : (const void *)op.ccol.get();
+                const size_t lds = sizeof(float) * n;
+                const dim3 gb((unsigned)nblk_av);
+#define SEV_B_AV(GG)                                                                       \
+    do {                                                                                   \
+        if (op.col12)                                                                      \
+            hipLaunchKernelGGL((k_b_av_bf16<GG, true>), gb, dim3(kBlock), lds, st, d,      \
+                               op.rowptr.get(), colp, (const uint16_t *)op.cval.get(),     \
+                               mu_all, Vc, j, w.scal.get(), Wc, y);                        \
+        else                                                                               \
+            hipLaunchKernelGGL((k_b_av_bf16<GG, false>), gb, dim3(kBlock), lds, st, d,     \
+                               op.rowptr.get(), colp, (const uint16_t *)op.cval.get(),     \
+                               mu_all, Vc, j, w.scal.get(), Wc, y);                        \
+    } while (0)
+                switch (G) {
+                    case 4: SEV_B_AV(4); break;
+                    case 8: SEV_B_AV(8); break;
+                    case 16: SEV_B_AV(16); break;
+                    default: SEV_B_AV(32); break;
+                }
+#undef SEV_B_AV
             }
             SEV_LAUNCH_CHECK();
             if (j > 0) {
@@ -2383,7 +2453,7 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                                j, w.T.get(), y, w.Fy.get(), w.part.get());
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_b_atw_bf16, dim3((unsigned)nslab, ng), dim3(kBlock), 0, st, d, j,
-                               op.off.get(), op.lrow.get(), (const uint16_t *)op.vals, w.Fy.get(),
+                               op.off.get(), op.lrow12.get(), (const uint16_t *)op.vals, w.Fy.get(),
                                w.part.get());
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_b_red_slabs,

@@ -130,8 +130,11 @@ struct SlabCsc {
     DevBuf<int64_t> off;      // (nslab + 1) * n
     DevBuf<uint16_t> lrow;    // nnz
     DevBuf<uint16_t> vbf16;   // nnz bf16 values (precision form only)
+    DevBuf<uint32_t> lrow12;  // precision form: lrow 12-bit packed (8 entries / 12 bytes)
     DevBuf<int64_t> rowptr;   // m + 1 (padded CSR)
     DevBuf<uint16_t> ccol;    // csr_nnz
+    DevBuf<uint32_t> ccol12;  // precision form, n <= 4096: ccol 12-bit packed
+    bool col12 = false;       // A v reads ccol12
     DevBuf<unsigned char> cval;  // csr_nnz values of vtype
     int lanes_per_row = 64;       // A v: lanes per CSR row (8/16/32/64)
     int atw_groups = 0;           // A' w: column groups per slab (0: heuristic)
