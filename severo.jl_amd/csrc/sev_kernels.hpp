@@ -330,7 +330,8 @@ double hbm_copy_probe();
 // x < xmin are pruned.  Two phases: count (colptr filled, nnz returned),
 // then fill (rows 0-based, values).
 struct SnnWork {
-    DevBuf<int32_t> deg, cur, rev, cnt, heavy, nheavy;
+    DevBuf<int32_t> deg, cur, rev, cnt, heavy, nheavy, staged;
+    DevBuf<unsigned char> stage;  // sorted kept slots of the wave-path columns
     DevBuf<int64_t> rptr, scratch, htab_off, heavyP;
     DevBuf<uint64_t> htab;
     int64_t nheavy_h = 0;

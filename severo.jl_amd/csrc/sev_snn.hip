@@ -212,13 +212,16 @@ __device__ void block_bitonic(S *a, int64_t n) {
 // Wave path: one wavefront per column with a private LDS table, for columns
 // with P <= table/2 candidates and at most kWaveKept kept entries; other
 // columns are listed as heavy (count mode) or skipped (fill mode) by the
-// same rule.  count mode (rows == nullptr): cnt[j] = kept entries.  fill
-// mode: the kept entries sorted by row at colptr[j].
+// same rule.  count mode (rows == nullptr): cnt[j] = kept entries, and with
+// stage given the kept slots sorted by row at stage[j * kWaveKept] (staged[j]
+// = 1), so that the fill only copies them.  fill mode: the kept entries
+// sorted by row at colptr[j].
 template <class T, class S>
 __global__ void __launch_bounds__(kBlock)
 k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
            const int32_t *rev, int32_t *cnt, int32_t *heavy, int64_t *heavyP, int32_t *nheavy,
-           const T *wtab, const int64_t *colptr, int32_t *rows, T *vals) {
+           const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, S *stage,
+           int32_t *staged) {
     constexpr int NS = kTableBytes / (int)sizeof(S);
     __shared__ S tabs[kWaves][NS];
     __shared__ S kbufs[kWaves][kWaveKept];
@@ -256,7 +259,7 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
             if (lane >= o) pre += y;
         }
         const int L = __shfl(pre, kWave - 1, kWave);
-        if (L > kWaveKept || !rows) {
+        if (L > kWaveKept || (!rows && !stage)) {
             if (!rows && lane == 0) {
                 if (L > kWaveKept) {
                     const int h = atomicAdd(nheavy, 1);
@@ -279,9 +282,39 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
         while (Lp < L) Lp <<= 1;
         for (int i = L + lane; i < Lp; i += kWave) kbuf[i] = ~(S)0;
         wave_bitonic<S>(kbuf, Lp);
+        if (!rows) {
+            // count mode with staging
+            S *dst = stage + j * kWaveKept;
+            for (int i = lane; i < L; i += kWave) dst[i] = kbuf[i];
+            if (lane == 0) {
+                cnt[j] = L;
+                staged[j] = 1;
+            }
+            continue;
+        }
         const int64_t ob = colptr[j];
         for (int i = lane; i < L; i += kWave) {
             const S v = kbuf[i];
+            rows[ob + i] = slot_row<S>(v);
+            vals[ob + i] = wtab[slot_count<S>(v)];
+        }
+    }
+}
+
+// Fill of the staged columns: copy the sorted kept slots to colptr[j].
+template <class T, class S>
+__global__ void __launch_bounds__(kBlock)
+k_snn_stage_copy(int64_t n, const S *stage, const int32_t *staged, const T *wtab,
+                 const int64_t *colptr, int32_t *rows, T *vals) {
+    const int lane = lane_id();
+    const int64_t stride = (int64_t)gridDim.x * kWaves;
+    for (int64_t j = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave; j < n; j += stride) {
+        if (!staged[j]) continue;
+        const int64_t ob = colptr[j];
+        const int L = (int)(colptr[j + 1] - ob);
+        const S *src = stage + j * kWaveKept;
+        for (int i = lane; i < L; i += kWave) {
+            const S v = src[i];
             rows[ob + i] = slot_row<S>(v);
             vals[ob + i] = wtab[slot_count<S>(v)];
         }
@@ -358,7 +391,8 @@ __device__ void block_emit(S *tab, uint32_t nslots, S *kept, int64_t kcap, int L
 
 // Heavy columns: one block per column, table in LDS (P <= half the LDS table)
 // or in global memory at gtab + gtab_off[h] (64-bit slots: 2/3 of the region
-// is the table, 1/3 the sort buffer).
+// is the table, 1/3 the sort buffer).  The fill pass recounts LDS-table
+// columns and reuses the global tables filled by the count pass.
 template <class T, class S>
 __global__ void __launch_bounds__(kBlock)
 k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
@@ -396,13 +430,16 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
         block_emit<S, T>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab, rows + colptr[j],
                          vals + colptr[j]);
     } else {
+        // the global table of this column keeps its counts from the count
+        // pass (its region is private to the column): the fill only emits
         uint64_t *tab = gtab + gtab_off[h];
         const uint32_t T_ = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
-        const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
         if (!rows) {
+            const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
             if (threadIdx.x == 0) cnt[j] = L;
             return;
         }
+        const int L = (int)(colptr[j + 1] - colptr[j]);
         block_emit<uint64_t, T>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
                                 rows + colptr[j], vals + colptr[j]);
     }
@@ -421,19 +458,43 @@ int64_t block_lds_maxP(int64_t n) {
     return small_slots(n) ? kBlockTableBytes / 4 / 2 : kBlockTableBytes / 8 / 2;
 }
 
+// Staging of the wave columns' kept entries (n x kWaveKept slots) when it
+// fits this budget; otherwise the fill recounts those columns.
+constexpr int64_t kStageBudget = (int64_t)8 << 30;
+
+bool use_stage(int64_t n) {
+    return n * kWaveKept * (small_slots(n) ? 4 : 8) <= kStageBudget;
+}
+
 template <class T>
 void launch_wave(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
                  const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
     const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
+    const bool stage = count && use_stage(n);
     if (small_slots(n))
         hipLaunchKernelGGL((k_snn_wave<T, uint32_t>), dim3(wg), dim3(kBlock), 0, st, nn, n, k,
                            xmin, w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(),
                            w.heavyP.get(), w.nheavy.get(), wtab, colptr, count ? nullptr : rows,
-                           vals);
+                           vals, stage ? (uint32_t *)w.stage.get() : nullptr, w.staged.get());
     else
         hipLaunchKernelGGL((k_snn_wave<T, uint64_t>), dim3(wg), dim3(kBlock), 0, st, nn, n, k,
                            xmin, w.rptr.get(), w.rev.get(), w.cnt.get(), w.heavy.get(),
                            w.heavyP.get(), w.nheavy.get(), wtab, colptr, count ? nullptr : rows,
+                           vals, stage ? (uint64_t *)w.stage.get() : nullptr, w.staged.get());
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_stage_copy(int64_t n, SnnWork &w, const T *wtab, const int64_t *colptr,
+                       int32_t *rows, T *vals, hipStream_t st) {
+    const unsigned wg = (unsigned)std::min<int64_t>(ceil_div(n, kWaves), device_cu_count() * 8);
+    if (small_slots(n))
+        hipLaunchKernelGGL((k_snn_stage_copy<T, uint32_t>), dim3(wg), dim3(kBlock), 0, st, n,
+                           (const uint32_t *)w.stage.get(), w.staged.get(), wtab, colptr, rows,
+                           vals);
+    else
+        hipLaunchKernelGGL((k_snn_stage_copy<T, uint64_t>), dim3(wg), dim3(kBlock), 0, st, n,
+                           (const uint64_t *)w.stage.get(), w.staged.get(), wtab, colptr, rows,
                            vals);
     SEV_LAUNCH_CHECK();
 }
@@ -471,6 +532,9 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
     w.nheavy.ensure(1);
     w.rptr.ensure((size_t)n + 1);
     w.scratch.ensure(scan_scratch_elems(n + 1));
+    w.staged.ensure((size_t)n);
+    SEV_HIP(hipMemsetAsync(w.staged.get(), 0, sizeof(int32_t) * n, st));
+    if (use_stage(n)) w.stage.ensure((size_t)(n * kWaveKept * (small_slots(n) ? 4 : 8)));
     SEV_HIP(hipMemsetAsync(w.deg.get(), 0, sizeof(int32_t) * n, st));
     SEV_HIP(hipMemsetAsync(w.cur.get(), 0, sizeof(int32_t) * n, st));
     SEV_HIP(hipMemsetAsync(w.nheavy.get(), 0, sizeof(int32_t), st));
@@ -527,7 +591,10 @@ template <class T>
 void snn_fill(const int32_t *nn, int64_t n, int64_t k, int xmin, const T *wtab, SnnWork &w,
               const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
     ProfScope ps("snn_fill", st);
-    launch_wave<T>(false, nn, n, (int)k, xmin, w, wtab, colptr, rows, vals, st);
+    if (use_stage(n))
+        launch_stage_copy<T>(n, w, wtab, colptr, rows, vals, st);
+    else
+        launch_wave<T>(false, nn, n, (int)k, xmin, w, wtab, colptr, rows, vals, st);
     launch_block<T>(false, nn, n, (int)k, xmin, w, wtab, colptr, rows, vals, st);
 }
 
