@@ -782,34 +782,44 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
     const int nr = lane & 15, kq = lane >> 4;
     const int ncb = (kout + 15) / 16;
     const int nks = (work + 3) / 4;
-    const int64_t nrb = (m + 15) / 16;
+    // two 16-row tiles per wavefront step share every A operand
+    const int64_t nrb = (m + 31) / 32;
     for (int64_t rb = (int64_t)blockIdx.x * kWaves + w; rb < nrb;
          rb += (int64_t)gridDim.x * kWaves) {
-        const int64_t row = rb * 16 + nr;
-        v4d acc[4];
+        const int64_t row0 = rb * 32 + nr, row1 = row0 + 16;
+        v4d acc0[4], acc1[4];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        for (int cb = 0; cb < 4; ++cb) {
+            acc0[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc1[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
         for (int s = 0; s < nks; ++s) {
             const int k = 4 * s + kq;
-            const double bv = (k < work && row < m) ? X[(int64_t)k * m + row] : 0.0;
+            const double bv0 = (k < work && row0 < m) ? X[(int64_t)k * m + row0] : 0.0;
+            const double bv1 = (k < work && row1 < m) ? X[(int64_t)k * m + row1] : 0.0;
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb) {
                 if (cb < ncb) {
                     const double av = cfl[(cb * 16 + nr) * LD + k];
-                    acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+                    acc0[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv0, acc0[cb], 0, 0, 0);
+                    acc1[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv1, acc1[cb], 0, 0, 0);
                 }
             }
         }
-        if (row < m) {
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
+        for (int cb = 0; cb < 4; ++cb) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int c = cb * 16 + kq + 4 * i;
-                    if (cb < ncb && c < kout) Y[(int64_t)c * m + row] = acc[cb][i];
+            for (int i = 0; i < 4; ++i) {
+                const int c = cb * 16 + kq + 4 * i;
+                if (cb < ncb && c < kout) {
+                    if (row0 < m) Y[(int64_t)c * m + row0] = acc0[cb][i];
+                    if (row1 < m) Y[(int64_t)c * m + row1] = acc1[cb][i];
                 }
             }
-            if (extra && kq == 0) Y[(int64_t)kout * m + row] = extra[row] / scal[2];
+        }
+        if (extra && kq == 0) {
+            if (row0 < m) Y[(int64_t)kout * m + row0] = extra[row0] / scal[2];
+            if (row1 < m) Y[(int64_t)kout * m + row1] = extra[row1] / scal[2];
         }
     }
 }
@@ -1356,7 +1366,7 @@ void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, 
 void launch_gemm(const char *name, int64_t m, int work, const double *X, const double *Cf,
                  int kout, double *Y, const double *extra, const double *scal, hipStream_t st) {
     ProfScope ps(name, st, 8.0 * m * (work + kout), 2.0 * m * work * kout);
-    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(m, 16), kWaves),
+    const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(m, 32), kWaves),
                                                       (int64_t)device_cu_count() * 8);
     hipLaunchKernelGGL(k_gemm_mfma, dim3(grid), dim3(kBlock), 0, st, m, work, X, Cf, kout, Y,
                        extra, scal);
