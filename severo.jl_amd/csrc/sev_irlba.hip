@@ -465,6 +465,17 @@ k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const do
     }
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (linear id mod 8).  A' w maps
+// block x to the slab that puts consecutive slabs of a column group on one
+// XCD and 8 dispatches apart: the cache lines at the boundary of a column's
+// consecutive slab segments, which both slabs read, are then found in that
+// XCD's L2.  A bijection of [0, nslab).
+__device__ __forceinline__ int64_t xcd_slab(int64_t x, int64_t nslab) {
+    const int64_t xcd = x & 7, i = x >> 3;
+    const int64_t q = nslab >> 3, r = nslab & 7;
+    return xcd * q + (xcd < r ? xcd : r) + i;
+}
+
 // --------------------------------------------------------------------------
 // Slab partials of B' y: block (slab s, column group g) stages the slab of y
 // in LDS and computes the dots of the columns of its group, one wavefront per
@@ -478,7 +489,7 @@ __global__ void __launch_bounds__(kBlock)
 k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *val,
       const double *y, double *part) {
     __shared__ double wl[kSlabRows];
-    const int64_t s = blockIdx.x;
+    const int64_t s = xcd_slab(blockIdx.x, gridDim.x);
     const int g = blockIdx.y, ng = gridDim.y;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
@@ -537,7 +548,7 @@ __global__ void __launch_bounds__(kBlock)
 k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint32_t *lrow12, const uint16_t *val,
            const double *y, double *part) {
     __shared__ float wl[kSlabRows];
-    const int64_t s = blockIdx.x;
+    const int64_t s = xcd_slab(blockIdx.x, gridDim.x);
     const int g = blockIdx.y, ng = gridDim.y;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
