@@ -27,7 +27,9 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <stdexcept>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace sev {
@@ -378,7 +380,7 @@ k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const void *col, const ui
 // columns in chunks of 32, each chunk reduced across the wavefront by the
 // butterfly and across the block's waves in wave order.
 // --------------------------------------------------------------------------
-constexpr int kDotThreads = 1024;
+constexpr int kDotThreads = 512;
 
 __global__ void __launch_bounds__(kDotThreads)
 k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
@@ -394,9 +396,15 @@ k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
         for (int i = 0; i < 32; ++i) acc[i] = 0.0;
         for (int64_t r = rb + threadIdx.x; r < re; r += kDotThreads) {
             const double yr = y[r];
+            // every load is issued (columns past jc read column jc - 1; their
+            // sums are never stored), so the 32 loads of a row are in flight
+            // together
+            double wv[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i)
-                if (c0 + i < jc) acc[i] = fma(W[(int64_t)(c0 + i) * m + r], yr, acc[i]);
+            for (int i = 0; i < 32; ++i) wv[i] = W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the FMAs
+#pragma unroll
+            for (int i = 0; i < 32; ++i) acc[i] = fma(wv[i], yr, acc[i]);
         }
         const double t = butterfly32(acc);
         if ((lane & 1) == 0) red[w][lane >> 1] = t;
@@ -616,7 +624,7 @@ __device__ double hash_normal(uint64_t seed, uint32_t i) {
 constexpr int kRsBlock = 256;
 
 template <class MT>
-__device__ void right_a_body(int64_t n, int j, const double *part2, const MT *mu, const double *V,
+__device__ __forceinline__ void right_a_body(int64_t n, int j, const double *part2, const MT *mu, const double *V,
                              double *Fv, double *rpart, unsigned bid) {
     __shared__ double red[kRsBlock / kWave][kMaxWork];
     const int64_t ncol = n + 2;
@@ -637,11 +645,16 @@ __device__ void right_a_body(int64_t n, int j, const double *part2, const MT *mu
         Fv[i] = f;
     }
     const int w = threadIdx.x / kWave, lane = lane_id();
+    const int64_t ic = i < n ? i : n - 1;
     for (int c0 = 0; c0 <= j; c0 += 32) {
         double v[32];
+        // unconditional (clamped) loads: all 32 in flight together
 #pragma unroll
-        for (int c = 0; c < 32; ++c)
-            v[c] = (i < n && c0 + c <= j) ? V[(int64_t)(c0 + c) * n + i] * f : 0.0;
+        for (int c = 0; c < 32; ++c) {
+            const bool in = i < n && c0 + c <= j;
+            const double vv = V[(int64_t)(c0 + c <= j ? c0 + c : j) * n + ic];
+            v[c] = in ? vv * f : 0.0;
+        }
         const double d = butterfly32(v);
         if ((lane & 1) == 0 && c0 + (lane >> 1) <= j) red[w][c0 + (lane >> 1)] = d;
     }
@@ -660,7 +673,7 @@ k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, 
     right_a_body<MT>(n, j, part2, mu, V, Fv, rpart, blockIdx.x);
 }
 
-__device__ void right_b_body(int64_t n, int j, int nblk, const double *rpart, const double *V,
+__device__ __forceinline__ void right_b_body(int64_t n, int j, int nblk, const double *rpart, const double *V,
                              double *Fv, double *npart, unsigned bid) {
     __shared__ double T[kMaxWork];
     __shared__ double scratch[kRsBlock / kWave];
@@ -688,7 +701,7 @@ k_right_b(int64_t n, int j, int nblk, const double *rpart, const double *V, doub
     right_b_body(n, j, nblk, rpart, V, Fv, npart, blockIdx.x);
 }
 
-__device__ void right_c_body(int64_t n, int j, int work, int nblk, const double *part2,
+__device__ __forceinline__ void right_c_body(int64_t n, int j, int work, int nblk, const double *part2,
                              const double *npart, double *V, double *Fv, double *B, double *scal,
                              int *flags, uint64_t seed) {
     __shared__ double scratch[16];
@@ -775,10 +788,15 @@ k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
 
 // Y[:, c] = X[:, 0:work] Cf[:, c] for c < kout (X, Y column-major, ld = m),
 // computed as the 16x16 tiles of D = Cf' X' on v_mfma_f64_16x16x4_f64: one
-// wavefront per 16 rows, lanes (n = row, k-group) read 16 consecutive rows
-// of one column of X, and each lane writes 16 consecutive rows of an output
-// column.  In this instruction lane l holds D[l/16 + 4i][l%16], i = 0..3.
-// Optionally Y[:, kout] = extra / scal[2] (the normalised residual).
+// wavefront per 32 rows (two 16-row tiles sharing every A operand), lanes
+// (n = row, k-group) read 16 consecutive rows of one column of X, and each
+// lane writes 16 consecutive rows of an output column.  In this instruction
+// lane l holds D[l/16 + 4i][l%16], i = 0..3.  NS = ceil(work / 4) k-steps
+// (the launch picks the instantiation): the whole 32 x work slice of X is
+// loaded before the MFMA chain, so its 2 NS loads per lane are in flight
+// together.  Optionally Y[:, kout] = extra / scal[2] (the normalised
+// residual).
+template <int NS>
 __global__ void __launch_bounds__(kBlock)
 k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, double *Y,
             const double *extra, const double *scal) {
@@ -792,39 +810,37 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
     const int w = wave_index(), lane = lane_id();
     const int nr = lane & 15, kq = lane >> 4;
     const int ncb = (kout + 15) / 16;
-    const int nks = (work + 3) / 4;
-    // two 16-row tiles per wavefront step share every A operand
     const int64_t nrb = (m + 31) / 32;
     for (int64_t rb = (int64_t)blockIdx.x * kWaves + w; rb < nrb;
          rb += (int64_t)gridDim.x * kWaves) {
         const int64_t row0 = rb * 32 + nr, row1 = row0 + 16;
-        v4d acc0[4], acc1[4];
+        // indices clamped into range, out-of-range values multiplied by zero
+        // (a product, not a select: the loads then stay unconditional)
+        const int64_t rc0 = row0 < m ? row0 : m - 1, rc1 = row1 < m ? row1 : m - 1;
+        double bv0[NS], bv1[NS];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-            acc0[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
-            acc1[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
-        }
-        for (int s = 0; s < nks; ++s) {
+        for (int s = 0; s < NS; ++s) {
             const int k = 4 * s + kq;
-            const double bv0 = (k < work && row0 < m) ? X[(int64_t)k * m + row0] : 0.0;
-            const double bv1 = (k < work && row1 < m) ? X[(int64_t)k * m + row1] : 0.0;
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                if (cb < ncb) {
-                    const double av = cfl[(cb * 16 + nr) * LD + k];
-                    acc0[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv0, acc0[cb], 0, 0, 0);
-                    acc1[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv1, acc1[cb], 0, 0, 0);
-                }
-            }
+            const int64_t kc = (int64_t)(k < work ? k : work - 1) * m;
+            bv0[s] = X[kc + rc0] * ((k < work && row0 < m) ? 1.0 : 0.0);
+            bv1[s] = X[kc + rc1] * ((k < work && row1 < m) ? 1.0 : 0.0);
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
+            if (cb >= ncb) break;
+            v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double av = cfl[(cb * 16 + nr) * LD + 4 * s + kq];
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv0[s], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv1[s], acc1, 0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int c = cb * 16 + kq + 4 * i;
-                if (cb < ncb && c < kout) {
-                    if (row0 < m) Y[(int64_t)c * m + row0] = acc0[cb][i];
-                    if (row1 < m) Y[(int64_t)c * m + row1] = acc1[cb][i];
+                if (c < kout) {
+                    if (row0 < m) Y[(int64_t)c * m + row0] = acc0[i];
+                    if (row1 < m) Y[(int64_t)c * m + row1] = acc1[i];
                 }
             }
         }
@@ -883,7 +899,7 @@ __device__ __forceinline__ double sum16(double v) {
     return v;
 }
 
-__device__ void small_svd_body(int n, const double *B, double *U, double *s, double *V,
+__device__ __forceinline__ void small_svd_body(int n, const double *B, double *U, double *s, double *V,
                                int *status) {
     extern __shared__ double sm[];
     double *G = sm;
@@ -1055,7 +1071,7 @@ k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status
 // [diag(s_0..s_k-1) res_0..k-1 in column k].  info: [0] converged,
 // [1] k, [2] ||F|| <= 0 (cannot restart), [3] nconv, [4] SVD needs the host.
 // --------------------------------------------------------------------------
-__device__ void restart_check_body(int work, int nu, double tol, double svtol, const double *BU,
+__device__ __forceinline__ void restart_check_body(int work, int nu, double tol, double svtol, const double *BU,
                                    const double *BS, const double *scal, const int *status,
                                    double *svr, double *B, int *info, int last) {
     __shared__ int s_go;  // rebuild B for a restart
@@ -1374,13 +1390,41 @@ void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, 
     SEV_LAUNCH_CHECK();
 }
 
+// Calls f(std::integral_constant<int, ns>) for ns = nks in 1..kMaxWork/4.
+template <class F>
+void switch_steps(int nks, F &&f) {
+    switch (nks) {
+    case 1: f(std::integral_constant<int, 1>()); break;
+    case 2: f(std::integral_constant<int, 2>()); break;
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    case 5: f(std::integral_constant<int, 5>()); break;
+    case 6: f(std::integral_constant<int, 6>()); break;
+    case 7: f(std::integral_constant<int, 7>()); break;
+    case 8: f(std::integral_constant<int, 8>()); break;
+    case 9: f(std::integral_constant<int, 9>()); break;
+    case 10: f(std::integral_constant<int, 10>()); break;
+    case 11: f(std::integral_constant<int, 11>()); break;
+    case 12: f(std::integral_constant<int, 12>()); break;
+    case 13: f(std::integral_constant<int, 13>()); break;
+    case 14: f(std::integral_constant<int, 14>()); break;
+    case 15: f(std::integral_constant<int, 15>()); break;
+    case 16: f(std::integral_constant<int, 16>()); break;
+    default: throw std::runtime_error("restart product: work size out of range");
+    }
+}
+
 void launch_gemm(const char *name, int64_t m, int work, const double *X, const double *Cf,
                  int kout, double *Y, const double *extra, const double *scal, hipStream_t st) {
     ProfScope ps(name, st, 8.0 * m * (work + kout), 2.0 * m * work * kout);
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(m, 32), kWaves),
                                                       (int64_t)device_cu_count() * 8);
-    hipLaunchKernelGGL(k_gemm_mfma, dim3(grid), dim3(kBlock), 0, st, m, work, X, Cf, kout, Y,
-                       extra, scal);
+    const int nks = (work + 3) / 4;
+    auto go = [&](auto ns) {
+        hipLaunchKernelGGL(k_gemm_mfma<decltype(ns)::value>, dim3(grid), dim3(kBlock), 0, st, m,
+                           work, X, Cf, kout, Y, extra, scal);
+    };
+    switch_steps(nks, go);
     SEV_LAUNCH_CHECK();
 }
 
@@ -1846,9 +1890,15 @@ k_b_dot_w(BDev d, const double *W, int j, const double *y, double *part) {
         for (int i = 0; i < 32; ++i) acc[i] = 0.0;
         for (int64_t r = rb + threadIdx.x; r < re; r += kDotThreads) {
             const double yr = y[r];
+            // every load is issued (columns past jc read column jc - 1; their
+            // sums are never stored), so the 32 loads of a row are in flight
+            // together
+            double wv[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i)
-                if (c0 + i < jc) acc[i] = fma(W[(int64_t)(c0 + i) * m + r], yr, acc[i]);
+            for (int i = 0; i < 32; ++i) wv[i] = W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
+            __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the FMAs
+#pragma unroll
+            for (int i = 0; i < 32; ++i) acc[i] = fma(wv[i], yr, acc[i]);
         }
         const double t = butterfly32(acc);
         if ((lane & 1) == 0) red[w][lane >> 1] = t;
@@ -2104,6 +2154,7 @@ k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel,
 // The same on MFMA (the tiles of k_gemm_mfma): W side (vmode 0) one block
 // per A v row block (rows of one sample, ld M); V side (vmode 1) block
 // (x, s) takes rows 256x .. of sample s's n x work block (ld n).
+template <int NS>
 __global__ void __launch_bounds__(kBlock)
 k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
               int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
@@ -2144,33 +2195,37 @@ k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *
     const int w = wave_index(), lane = lane_id();
     const int nr = lane & 15, kq = lane >> 4;
     const int ncb = (kout + 15) / 16;
-    const int nks = (work + 3) / 4;
     for (int64_t t0 = rb0 + (int64_t)w * 16; t0 < re0; t0 += (int64_t)kWaves * 16) {
         const int64_t row = t0 + nr;
         const bool ok = row < re0;
-        v4d acc[4];
+        // all NS = ceil(work / 4) loads of the 16 x work slice first
+        // (clamped indices, out-of-range values multiplied by zero), then
+        // the MFMA chains
+        const int64_t rc = ok ? row : re0 - 1;
+        double bv[NS];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
-        for (int st = 0; st < nks; ++st) {
+        for (int st = 0; st < NS; ++st) {
             const int k = 4 * st + kq;
-            const double bv = (k < work && ok) ? X[(int64_t)k * ld + row] : 0.0;
+            bv[st] = X[(int64_t)(k < work ? k : work - 1) * ld + rc] * ((k < work && ok) ? 1.0 : 0.0);
+        }
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                if (cb < ncb) {
-                    const double av = cfl[(cb * 16 + nr) * LD + k];
-                    acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[cb], 0, 0, 0);
+        for (int cb = 0; cb < 4; ++cb) {
+            if (cb >= ncb) break;
+            v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int st = 0; st < NS; ++st) {
+                const double av = cfl[(cb * 16 + nr) * LD + 4 * st + kq];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[st], acc, 0, 0, 0);
+            }
+            if (ok) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int c = cb * 16 + kq + 4 * i;
+                    if (c < kout) Ys[(int64_t)c * ld + row] = acc[i];
                 }
             }
         }
         if (ok) {
-#pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int c = cb * 16 + kq + 4 * i;
-                    if (cb < ncb && c < kout) Ys[(int64_t)c * ld + row] = acc[cb][i];
-                }
-            }
             if (vmode && restart && kq == 0)
                 Ys[(int64_t)kout * ld + row] = Fall[(int64_t)s * d.n + row] / scal[4 * s + 2];
         }
@@ -2503,14 +2558,17 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                            w.B.get(), w.info.get(), w.flags.get());
         SEV_LAUNCH_CHECK();
         // restart of the continuing samples into the other buffers
-        hipLaunchKernelGGL(k_b_gemm_mfma, dim3((unsigned)ceil_div(n, 256), (unsigned)S),
-                           dim3(kBlock), 0, st, d, 1, Vc, Vc, (const int *)nullptr, Vn,
-                           (int64_t)work, w.BV.get(), 0, w.F.get(), w.scal.get(), true);
-        SEV_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_b_gemm_mfma, dim3((unsigned)nblk_av), dim3(kBlock), 0, st, d, 0, Wc,
-                           Wc, (const int *)nullptr, Wn, M, w.BU.get(), 0,
-                           (const double *)nullptr, w.scal.get(), true);
-        SEV_LAUNCH_CHECK();
+        switch_steps((work + 3) / 4, [&](auto ns) {
+            constexpr int NS = decltype(ns)::value;
+            hipLaunchKernelGGL(k_b_gemm_mfma<NS>, dim3((unsigned)ceil_div(n, 256), (unsigned)S),
+                               dim3(kBlock), 0, st, d, 1, Vc, Vc, (const int *)nullptr, Vn,
+                               (int64_t)work, w.BV.get(), 0, w.F.get(), w.scal.get(), true);
+            SEV_LAUNCH_CHECK();
+            hipLaunchKernelGGL(k_b_gemm_mfma<NS>, dim3((unsigned)nblk_av), dim3(kBlock), 0, st, d,
+                               0, Wc, Wc, (const int *)nullptr, Wn, M, w.BU.get(), 0,
+                               (const double *)nullptr, w.scal.get(), true);
+            SEV_LAUNCH_CHECK();
+        });
     };
 
     std::vector<int> fin_buf((size_t)S, -1);
