@@ -855,7 +855,7 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
 // SVD of the projected matrix B (work x work, column-major) on the device:
 // one-sided Jacobi on G = B' with the round-robin (circle) pair schedule, so
 // that the ne/2 pairs of a round touch disjoint columns and run in parallel,
-// one 32-lane group per pair.  Rotations accumulate in J; at convergence
+// one 16-lane group per pair.  Rotations accumulate in J; at convergence
 // B' J = G, the singular values are the column norms of G, U = J and
 // V = G diag(1/s) (ordered by descending s, ties by column index).
 // status[0] = 1 if a singular value vanished (V then needs completing; the
@@ -1060,6 +1060,195 @@ __device__ __forceinline__ void small_svd_body(int n, const double *B, double *U
 __global__ void __launch_bounds__(kSvdThreads)
 k_small_svd(int n, const double *B, double *U, double *s, double *V, int *status) {
     small_svd_body(n, B, U, s, V, status);
+}
+
+// The same SVD with 8 lanes per column pair (ne/2 <= 31 groups in four
+// wavefronts): columns of G and J sit in LDS with stride kSvdLd, rows n..63
+// held at zero, and lane l of a group holds rows 16k + 2l + h (k < 4, h < 2),
+// so that each of its four 16-byte LDS reads per column is one contiguous
+// 128-byte segment across the group.  Half the lanes of the 16-lane form for
+// the same rounds: the per-pair reductions and rotation formulas, repeated on
+// every lane of a group, are the bulk of a round.  For work <= kSvd8MaxWork
+// (the two arrays fit the 64 KiB of LDS a block may allocate by default).
+constexpr int kSvd8Threads = 256;
+constexpr int kSvdLd = 64;
+constexpr int kSvd8MaxWork = 62;
+
+// Sum over each group of 8 lanes (quad xor 1, quad xor 2, half-row mirror):
+// commutative steps, so the 8 lanes end with bit-identical totals.
+__device__ __forceinline__ double sum8(double v) {
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    return v;
+}
+
+__device__ __forceinline__ void small_svd8_body(int n, const double *B, double *U, double *s,
+                                                double *V, int *status) {
+    extern __shared__ double2 svd8_lds[];  // 16-byte aligned for the paired reads
+    const int ne = n + (n & 1);
+    double *G = (double *)svd8_lds;
+    // B itself (for U = B V diag(1/s)), or the accumulated rotations when B
+    // is too ill-conditioned for that product
+    double *J = G + ne * kSvdLd;
+    __shared__ int rotated, need_j, nsweeps;
+    __shared__ double sv[kMaxWork];
+    const int tid = threadIdx.x;
+    const int pr = tid >> 3, gl = tid & 7;
+    const int npair = ne / 2;
+    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    if (tid == 0) {
+        need_j = 0;
+        nsweeps = 0;
+    }
+    auto load = [&](bool withj) {
+        for (int e = tid; e < ne * kSvdLd; e += kSvd8Threads) {
+            const int c = e / kSvdLd, i = e % kSvdLd;
+            const bool in = c < n && i < n;
+            G[e] = in ? B[(int64_t)i * n + c] : 0.0;  // G(i, c) = B(c, i)
+            J[e] = withj ? (in && i == c ? 1.0 : 0.0) : (in ? B[(int64_t)c * n + i] : 0.0);
+        }
+        __syncthreads();
+    };
+    auto sweeps = [&](bool withj) {
+        for (int sweep = 0; sweep < 100; ++sweep) {
+            if (tid == 0) rotated = 0;
+            __syncthreads();
+            // circle schedule: round r pairs (r + pr, r - pr) mod (ne - 1),
+            // group 0 pairs r with the fixed column ne - 1
+            int pp = pr, qq = ne - 1 - pr;
+            for (int r = 0; r < ne - 1; ++r) {
+                if (pr < npair) {
+                    const int p = pp < qq ? pp : qq, q = pp < qq ? qq : pp;
+                    const bool valid = q < n;
+                    double2 *gp = (double2 *)(G + p * kSvdLd) + gl;
+                    double2 *gq = (double2 *)(G + q * kSvdLd) + gl;
+                    double2 x[4], y[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        x[k] = gp[8 * k];
+                        y[k] = gq[8 * k];
+                    }
+                    double app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        app = fma(x[k].x, x[k].x, app);
+                        aqq = fma(y[k].x, y[k].x, aqq);
+                        apq = fma(x[k].x, y[k].x, apq);
+                        app = fma(x[k].y, x[k].y, app);
+                        aqq = fma(y[k].y, y[k].y, aqq);
+                        apq = fma(x[k].y, y[k].y, apq);
+                    }
+                    app = sum8(app);
+                    aqq = sum8(aqq);
+                    apq = sum8(apq);
+                    if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
+                        if (gl == 0) rotated = 1;
+                        // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2)
+                        const double zeta = (aqq - app) * rcp_nr(2.0 * apq);
+                        const double az = fabs(zeta);
+                        double t;
+                        if (az > 1e150) {
+                            t = 0.5 * rcp_nr(zeta);
+                        } else {
+                            const double xx = fma(az, az, 1.0);
+                            t = copysign(rcp_nr(az + xx * rsq_nr(xx)), zeta);
+                        }
+                        const double c = rsq_nr(fma(t, t, 1.0));
+                        const double sn = c * t;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            gp[8 * k] = make_double2(c * x[k].x - sn * y[k].x, c * x[k].y - sn * y[k].y);
+                            gq[8 * k] = make_double2(sn * x[k].x + c * y[k].x, sn * x[k].y + c * y[k].y);
+                        }
+                        if (withj) {
+                            double2 *jp = (double2 *)(J + p * kSvdLd) + gl;
+                            double2 *jq = (double2 *)(J + q * kSvdLd) + gl;
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                const double2 u = jp[8 * k], v = jq[8 * k];
+                                jp[8 * k] = make_double2(c * u.x - sn * v.x, c * u.y - sn * v.y);
+                                jq[8 * k] = make_double2(sn * u.x + c * v.x, sn * u.y + c * v.y);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                pp = pp + 1 == ne - 1 ? 0 : pp + 1;
+                if (pr > 0) qq = qq + 1 == ne - 1 ? 0 : qq + 1;
+            }
+            const int any = rotated;
+            __syncthreads();
+            if (tid == 0) nsweeps = nsweeps + 1;
+            if (!any) break;
+        }
+    };
+    auto norms = [&]() {
+        if (tid < n) {
+            double t = 0.0;
+            for (int i = 0; i < n; ++i) t += G[tid * kSvdLd + i] * G[tid * kSvdLd + i];
+            sv[tid] = sqrt(t);
+        }
+        __syncthreads();
+    };
+    load(false);
+    sweeps(false);
+    norms();
+    // U = B V diag(1/s) is accurate to eps s_max / s_j: without the
+    // accumulated rotations only while s_min >= 1e-5 s_max
+    if (tid == 0) {
+        double mx = 0.0, mn = __builtin_huge_val();
+        for (int i = 0; i < n; ++i) {
+            mx = fmax(mx, sv[i]);
+            mn = fmin(mn, sv[i]);
+        }
+        need_j = !(mn >= 1e-5 * mx);
+    }
+    __syncthreads();
+    const bool withj = need_j != 0;
+    if (withj) {
+        load(true);
+        sweeps(true);
+        norms();
+    }
+    if (tid == 0) {
+        status[0] = 0;
+        status[1] = nsweeps;
+    }
+    __syncthreads();
+    for (int j = tid >> 6; j < n; j += kSvd8Threads / 64) {
+        const int l = tid & 63;
+        const double sj = sv[j];
+        int rk = 0;
+        for (int i = 0; i < n; ++i) rk += (sv[i] > sj || (sv[i] == sj && i < j)) ? 1 : 0;
+        if (l == 0) {
+            s[rk] = sj;
+            if (!(sj > 0.0)) status[0] = 1;
+        }
+        for (int i = l; i < n; i += 64) {
+            if (withj) {
+                U[(int64_t)rk * n + i] = J[j * kSvdLd + i];
+            } else {
+                // (B g_j)_i / s_j^2, g_j = s_j v_j the column of G
+                double t = 0.0;
+                for (int c = 0; c < n; ++c) t = fma(J[c * kSvdLd + i], G[j * kSvdLd + c], t);
+                U[(int64_t)rk * n + i] = t / (sj * sj);
+            }
+            V[(int64_t)rk * n + i] = sj > 0.0 ? G[j * kSvdLd + i] / sj : 0.0;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSvd8Threads)
+k_small_svd8(int n, const double *B, double *U, double *s, double *V, int *status) {
+    small_svd8_body(n, B, U, s, V, status);
+}
+
+// LDS bytes and launch of the projected SVD for work columns
+size_t svd_lds_bytes(int work) {
+    const size_t ne = (size_t)(work + (work & 1));
+    return work <= kSvd8MaxWork ? sizeof(double) * 2 * ne * kSvdLd
+                                : sizeof(double) * 2 * (size_t)work * (size_t)work;
 }
 
 // --------------------------------------------------------------------------
@@ -1614,7 +1803,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     int info_h[8] = {0, (int)nu, 0, 0, 0, 0, 0, 0};
     SEV_HIP(hipMemcpyAsync(w.info.get(), info_h, sizeof info_h, hipMemcpyHostToDevice, st));
     SEV_HIP(hipMemsetAsync(w.svr.get(), 0, sizeof(double) * kMaxWork, st));
-    const size_t svd_lds = sizeof(double) * 2 * (size_t)work * (size_t)work;
+    const size_t svd_lds = svd_lds_bytes(work);
     SEV_REQUIRE(svd_lds + 1024 <= (size_t)device_lds_per_block(),
                 "irlba: projected matrix too large for the device SVD");
     int64_t iter = 0, nmult = 0, k = nu;
@@ -1685,8 +1874,14 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         // reads the decision (converged, k) back
         {
             ProfScope ps("small_svd", st);
-            hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kSvdThreads), svd_lds, st, work,
-                               w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get() + 8);
+            if (work <= kSvd8MaxWork)
+                hipLaunchKernelGGL(k_small_svd8, dim3(1), dim3(kSvd8Threads), svd_lds, st, work,
+                                   w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(),
+                                   w.info.get() + 8);
+            else
+                hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kSvdThreads), svd_lds, st, work,
+                                   w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(),
+                                   w.info.get() + 8);
             SEV_LAUNCH_CHECK();
         }
         const int last = iter + 1 >= p.maxit ? 1 : 0;
@@ -2064,6 +2259,17 @@ k_b_small_svd(BDev d, const double *Ball, double *BUall, double *BSall, double *
                    info + 16 * s + 8);
 }
 
+__global__ void __launch_bounds__(kSvd8Threads)
+k_b_small_svd8(BDev d, const double *Ball, double *BUall, double *BSall, double *BVall,
+               int *info) {
+    const int s = blockIdx.x;
+    if (d.done[s]) return;
+    const int work = d.work;
+    const int64_t o = (int64_t)s * work * work;
+    small_svd8_body(work, Ball + o, BUall + o, BSall + (int64_t)s * kMaxWork, BVall + o,
+                    info + 16 * s + 8);
+}
+
 // The restart decision of each sample; then its state for the next round:
 // j0 = k, iters + 1, done on convergence, failure or the last iteration.
 __global__ void __launch_bounds__(kBlock)
@@ -2399,7 +2605,7 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     w.flags.ensure((size_t)(4 * S));
     w.info.ensure((size_t)(16 * S));
     w.state.ensure((size_t)(4 * S));
-    const size_t svd_lds = sizeof(double) * 2 * (size_t)work * (size_t)work;
+    const size_t svd_lds = svd_lds_bytes(work);
     SEV_REQUIRE(svd_lds + 1024 <= (size_t)device_lds_per_block(),
                 "batched irlba: projected matrix too large for the device SVD");
 
@@ -2550,8 +2756,12 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                                w.Fy.get(), Wc, w.scal.get());
             SEV_LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL(k_b_small_svd, dim3((unsigned)S), dim3(kSvdThreads), svd_lds, st, d,
-                           w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get());
+        if (work <= kSvd8MaxWork)
+            hipLaunchKernelGGL(k_b_small_svd8, dim3((unsigned)S), dim3(kSvd8Threads), svd_lds, st,
+                               d, w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get());
+        else
+            hipLaunchKernelGGL(k_b_small_svd, dim3((unsigned)S), dim3(kSvdThreads), svd_lds, st,
+                               d, w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(), w.info.get());
         SEV_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_b_restart_check, dim3((unsigned)S), dim3(kBlock), 0, st, d, nu, maxit,
                            tol, svtol, w.BU.get(), w.BS.get(), w.scal.get(), w.svr.get(),
