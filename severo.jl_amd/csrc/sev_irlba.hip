@@ -581,8 +581,17 @@ k_red_slabs(int64_t nslab, int64_t ncol, const double *part, double *part2) {
     if (c >= ncol) return;
     const int64_t per = (nslab + kRedChunks - 1) / kRedChunks;
     const int64_t sb = q * per, se = sb + per < nslab ? sb + per : nslab;
+    // 16 slabs at a time: all loads issued first (indices clamped, values
+    // past the chunk multiplied by zero), then added in slab order
     double t = 0.0;
-    for (int64_t s = sb; s < se; ++s) t += part[s * ncol + c];
+    for (int64_t s0 = sb; s0 < se; s0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = part[(s0 + i < se ? s0 + i : se - 1) * ncol + c];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t += v[i] * (s0 + i < se ? 1.0 : 0.0);
+    }
     part2[q * ncol + c] = t;
 }
 

@@ -234,8 +234,24 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
                 s = s + delta * (v - mu);
             };
             if (nt == kWave) {
+                // full chunk without the per-element branch: tiny deltas are
+                // only flagged (off the recurrence chain), and a flagged
+                // chunk is redone with the exact division
+                const float mu0 = mu, s0 = s;
+                bool tiny = false;
 #pragma unroll
-                for (int t = 0; t < kWave; ++t) step(t);
+                for (int t = 0; t < kWave; ++t) {
+                    const float v = lane_value(vl, t);
+                    const float delta = v - mu;
+                    tiny = tiny || (fabsf(delta) < 0x1p-100f && delta != 0.0f);
+                    mu = mu + (float)((double)delta * lane_value(rcl, t));
+                    s = s + delta * (v - mu);
+                }
+                if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)tiny), 0)) {
+                    mu = mu0;
+                    s = s0;
+                    for (int t = 0; t < kWave; ++t) step(t);
+                }
             } else {
                 for (int t = 0; t < nt; ++t) step(t);
             }
