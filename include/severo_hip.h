@@ -267,6 +267,9 @@ int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void
  * aggregated entries: name (32 bytes each, NUL-terminated), launches, total
  * milliseconds, and the algorithmic bytes and flops of those launches. */
 int sev_pipeline_profile(sev_pipeline *p, int enable);
+/* The same, bracketing only the regions called name (e.g. "spmv_av"): the
+ * other regions then add no events to the stream.  Clears the records. */
+int sev_pipeline_profile_only(sev_pipeline *p, const char *name);
 int sev_pipeline_profile_read(sev_pipeline *p, int max, char *names, int64_t *launches,
                               double *ms, double *bytes, double *flops, int *count);
 

@@ -68,6 +68,7 @@ SIGNATURES = {
     "sev_pipeline_snn": (_int, [_p, _dbl, _int, _pi64]),
     "sev_pipeline_get_snn": (_int, [_p, _p, _p, _p]),
     "sev_pipeline_profile": (_int, [_p, _int]),
+    "sev_pipeline_profile_only": (_int, [_p, C.c_char_p]),
     "sev_pipeline_profile_read": (_int, [_p, _int, _p, _p, _p, _p, _p, C.POINTER(_int)]),
 }
 

@@ -878,6 +878,17 @@ int sev_pipeline_profile(sev_pipeline *p, int enable) {
         SEV_HIP(hipStreamSynchronize(p->st()));
         p->prof.reset();
         p->prof.on = enable != 0;
+        p->prof.only.clear();
+    });
+}
+
+int sev_pipeline_profile_only(sev_pipeline *p, const char *name) {
+    return guarded([&] {
+        SEV_REQUIRE(name && name[0], "profile_only: empty region name");
+        SEV_HIP(hipStreamSynchronize(p->st()));
+        p->prof.reset();
+        p->prof.on = true;
+        p->prof.only = name;
     });
 }
 

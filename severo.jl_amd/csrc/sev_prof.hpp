@@ -19,6 +19,7 @@ struct Prof {
         double bytes, flops;
     };
     bool on = false;
+    std::string only;  // when set, only the regions of this name are recorded
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
 
@@ -77,7 +78,7 @@ struct ProfScope {
     hipStream_t st;
     Prof::Rec rec;
     ProfScope(const char *name, hipStream_t s, double bytes = 0, double flops = 0) : st(s) {
-        if (t_prof && t_prof->on) {
+        if (t_prof && t_prof->on && (t_prof->only.empty() || t_prof->only == name)) {
             p = t_prof;
             rec.name = name;
             rec.bytes = bytes;

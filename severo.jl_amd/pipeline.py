@@ -48,10 +48,14 @@ class Pipeline:
     def synchronize(self):
         check(self._L.sev_pipeline_synchronize(self._h))
 
-    def profile(self, enable=True):
-        """Bracket every kernel of this handle with HIP events (clears the
-        previous records)."""
-        check(self._L.sev_pipeline_profile(self._h, int(enable)))
+    def profile(self, enable=True, only=None):
+        """Bracket every kernel of this handle with HIP events, or with
+        only="name" just the regions of that name (clears the previous
+        records)."""
+        if enable and only:
+            check(self._L.sev_pipeline_profile_only(self._h, only.encode()))
+        else:
+            check(self._L.sev_pipeline_profile(self._h, int(enable)))
 
     def profile_read(self, max_entries=64):
         """{name: (launches, total_ms, algorithmic_bytes, algorithmic_flops)}"""
