@@ -110,6 +110,12 @@ void csc_to_device(int64_t m, int64_t n, const int64_t *colptr, const int64_t *r
         SEV_HIP(hipMemcpyAsync(&bad_h, bad.get(), sizeof bad_h, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipStreamSynchronize(st));
         SEV_REQUIRE(bad_h == 0, "row index out of range");
+        // a SparseMatrixCSC keeps the rows of each column strictly increasing;
+        // the row-range searches of the later stages rely on it
+        check_sorted(n, dcp.get(), drow.get(), bad.get(), st);
+        SEV_HIP(hipMemcpyAsync(&bad_h, bad.get(), sizeof bad_h, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        SEV_REQUIRE(bad_h == 0, "rows of a column not strictly increasing");
     }
     SEV_HIP(hipStreamSynchronize(st));
 }
@@ -494,7 +500,7 @@ int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor) {
         hipStream_t st = p->st();
         p->tot.ensure((size_t)p->m);
         p->norm.ensure((size_t)p->nnz);
-        row_totals(p->row.get(), p->cnt.get(), p->nnz, p->m, p->tot.get(), st);
+        row_totals(p->cp.get(), p->row.get(), p->cnt.get(), p->nnz, p->m, p->g, p->tot.get(), st);
         lognorm_f32(p->row.get(), p->cnt.get(), p->tot.get(), p->nnz, (float)scale_factor,
                     method == SEV_LOGNORMALIZE, p->norm.get(), st);
         p->have_norm = true;
@@ -964,7 +970,7 @@ int sev_normalize_cells(int64_t m, int64_t n, const int64_t *colptr, const int64
         }
         upload(cnt, c.data(), (size_t)nnz, s.st);
         tot.ensure((size_t)m);
-        row_totals(row.get(), cnt.get(), nnz, m, tot.get(), s.st);
+        row_totals(cp.get(), row.get(), cnt.get(), nnz, m, n, tot.get(), s.st);
         const int dolog = method == SEV_LOGNORMALIZE;
         if (dtype == SEV_FLOAT32) {
             DevBuf<float> out;

@@ -695,8 +695,20 @@ __device__ __forceinline__ void right_b_body(int64_t n, int j, int nblk, const d
     const int64_t i = (int64_t)bid * kRsBlock + threadIdx.x;
     double f = 0.0;
     if (i < n) {
+        // 16 columns at a time: loads first (clamped column index, the extra
+        // terms multiplied by zero), then the FMA chain in column order
         double d = 0.0;
-        for (int c = 0; c <= j; ++c) d = fma(V[(int64_t)c * n + i], T[c], d);
+        for (int c0 = 0; c0 <= j; c0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) v[c] = V[(int64_t)(c0 + c <= j ? c0 + c : j) * n + i];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const double t = T[c0 + c <= j ? c0 + c : j];
+                d = fma(v[c], c0 + c <= j ? t : 0.0, d);
+            }
+        }
         f = Fv[i] - d;
         Fv[i] = f;
     }

@@ -44,8 +44,12 @@ void synth_fill(const SynthPlan &plan, uint64_t seed, double r, const double *p0
 void synth_classes(int64_t C, uint64_t seed, double boost_frac, uint8_t *cls_d, hipStream_t st);
 
 // Per-cell totals and the per-nonzero map of normalize_cells.
-void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t ncells,
-                uint32_t *tot, hipStream_t st);
+// colptr/row: CSC with strictly increasing rows in every column
+void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
+                int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st);
+// bad |= 1 when some column's rows are not strictly increasing
+void check_sorted(int64_t n, const int64_t *colptr, const int32_t *row, int32_t *bad,
+                  hipStream_t st);
 void lognorm_f32(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,
                  float sf, int dolog, float *out, hipStream_t st);
 void lognorm_f64(const int32_t *row, const int32_t *cnt, const uint32_t *tot, int64_t nnz,

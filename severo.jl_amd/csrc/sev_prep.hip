@@ -110,11 +110,84 @@ k_synth_fill(int64_t C, int64_t nchunk, int64_t chunk, uint64_t seedm, double r,
 // --------------------------------------------------------------------------
 
 // s = sum(A, dims=2): integer row (cell) totals over the CSC nonzeros.
+// Block (x, y) sums the entries of columns [x*kTotCols, ...) whose rows fall
+// in [y*kTotRows, ...) into an LDS histogram (one lane per column finds its
+// segment by binary search in the column's sorted rows; the wave then walks
+// the 64 segments, four strides of loads in flight at a time), and adds the
+// histogram to the totals with one global atomic per nonzero row.
+constexpr int kTotThreads = 1024;
+constexpr int kTotWaves = kTotThreads / kWave;
+constexpr int64_t kTotRows = 16384;  // rows per histogram (64 KiB of LDS)
+constexpr int64_t kTotCols = 2048;   // columns per block
+
+__device__ __forceinline__ int64_t lower_row(const int32_t *row, int64_t lo, int64_t hi,
+                                             int64_t key) {
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (row[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kTotThreads)
+k_row_totals(int64_t m, int64_t n, const int64_t *colptr, const int32_t *row, const int32_t *cnt,
+             uint32_t *tot) {
+    __shared__ uint32_t h[kTotRows];
+    const int64_t r0 = (int64_t)blockIdx.y * kTotRows;
+    const int64_t r1 = r0 + kTotRows < m ? r0 + kTotRows : m;
+    const int64_t c0 = (int64_t)blockIdx.x * kTotCols;
+    const int64_t c1 = c0 + kTotCols < n ? c0 + kTotCols : n;
+    for (int i = threadIdx.x; i < kTotRows; i += kTotThreads) h[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    for (int64_t cb = c0 + (int64_t)w * kWave; cb < c1; cb += (int64_t)kTotWaves * kWave) {
+        const int64_t c = cb + lane;
+        int64_t a = 0, b = 0;
+        if (c < c1) {
+            const int64_t lo = colptr[c], hi = colptr[c + 1];
+            a = lower_row(row, lo, hi, r0);
+            b = lower_row(row, a, hi, r1);
+        }
+        const int nc = (int)(c1 - cb < kWave ? c1 - cb : kWave);
+        for (int t = 0; t < nc; ++t) {
+            const int64_t sa = __shfl(a, t, kWave), sb = __shfl(b, t, kWave);
+            for (int64_t p = sa + lane; p < sb; p += 4 * kWave) {
+                // clamped indices: every load is issued, extra ones add nothing
+                int32_t rr[4], cc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t q = p + u * kWave < sb ? p + u * kWave : sb - 1;
+                    rr[u] = row[q];
+                    cc[u] = cnt[q];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint64_t lr = (uint64_t)((int64_t)rr[u] - r0);
+                    if (p + u * kWave < sb && lr < (uint64_t)kTotRows)
+                        atomicAdd(&h[lr], (uint32_t)cc[u]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < r1 - r0; i += kTotThreads)
+        if (h[i]) atomicAdd(&tot[r0 + i], h[i]);
+}
+
+// bad = 1 if the rows of some column are not strictly increasing (one
+// wavefront per column)
 __global__ void __launch_bounds__(kBlock)
-k_row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, uint32_t *tot) {
-    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
-         p += (int64_t)gridDim.x * kBlock)
-        atomicAdd(&tot[row[p]], (uint32_t)cnt[p]);
+k_check_sorted(int64_t n, const int64_t *colptr, const int32_t *row, int32_t *bad) {
+    const int64_t j = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (j >= n) return;
+    const int lane = lane_id();
+    const int64_t b = colptr[j], e = colptr[j + 1];
+    bool ok = true;
+    for (int64_t p = b + 1 + lane; p < e; p += kWave) ok = ok && row[p] > row[p - 1];
+    if (__ballot(!ok) && lane == 0) atomicOr(bad, 1);
 }
 
 // nzB = sf * nzA / s[row] in Float32, left to right; then log1p, defined as
@@ -375,13 +448,22 @@ void synth_fill(const SynthPlan &plan, uint64_t seed, double r, const double *p0
     SEV_LAUNCH_CHECK();
 }
 
-void row_totals(const int32_t *row, const int32_t *cnt, int64_t nnz, int64_t ncells,
-                uint32_t *tot, hipStream_t st) {
+void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
+                int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st) {
     SEV_HIP(hipMemsetAsync(tot, 0, sizeof(uint32_t) * (size_t)(ncells > 0 ? ncells : 1), st));
     if (nnz == 0) return;
     ProfScope ps("row_totals", st, 8.0 * nnz + 4.0 * ncells);
-    hipLaunchKernelGGL(k_row_totals, dim3(grid_for(nnz, 16)), dim3(kBlock), 0, st, row, cnt, nnz,
-                       tot);
+    const dim3 grid((unsigned)ceil_div(ngenes, kTotCols), (unsigned)ceil_div(ncells, kTotRows));
+    hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, ncells, ngenes, colptr, row,
+                       cnt, tot);
+    SEV_LAUNCH_CHECK();
+}
+
+void check_sorted(int64_t n, const int64_t *colptr, const int32_t *row, int32_t *bad,
+                  hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_check_sorted, dim3((unsigned)ceil_div(n * kWave, kBlock)), dim3(kBlock),
+                       0, st, n, colptr, row, bad);
     SEV_LAUNCH_CHECK();
 }
 

@@ -64,6 +64,43 @@ def test_normalize_cells_errors(sev):
         sev.normalize_cells(X, method="bogus")
 
 
+@pytest.mark.parametrize("bad", ["unsorted", "duplicate"])
+def test_load_rejects_unsorted_rows(sev, bad):
+    """The C-ABI refuses a column whose rows are not strictly increasing
+    (a SparseMatrixCSC invariant the row-range searches rely on)."""
+    import ctypes as C
+    _lib = sev._lib
+    colptr = np.array([1, 3, 5], np.int64)  # 1-based, two columns of 2 entries
+    rowval = np.array([1, 3, 2, 4] if bad == "unsorted" else [1, 3, 2, 2], np.int64)
+    if bad == "unsorted":
+        rowval[:2] = [3, 1]
+    counts = np.array([1, 2, 3, 4], np.int64)
+    out = np.empty(4, np.float64)
+    L = _lib.load()
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+    rc = L.sev_normalize_cells(4, 2, ptr(colptr), ptr(rowval), ptr(counts), _lib.SEV_INT64,
+                               _lib.SEV_LOGNORMALIZE, 1.0, _lib.SEV_FLOAT64, ptr(out))
+    assert rc != 0
+    assert b"strictly increasing" in L.sev_last_error()
+
+
+def test_row_totals_many_ranges(sev):
+    """Cell totals over several 16384-row ranges and 2048-column chunks
+    (ragged at both ends) against scipy's integer row sums."""
+    rng = np.random.default_rng(9)
+    m, n = 40000, 4500
+    X = sp.random(m, n, density=0.004, format="csc", random_state=rng,
+                  data_rvs=lambda k: rng.integers(1, 50, k)).astype(np.int64)
+    # plus a dense row
+    X = sp.csc_matrix(X + sp.csc_matrix((np.full(n, 7), (np.full(n, 5), np.arange(n))),
+                                        shape=(m, n)))
+    Y = sev.normalize_cells(X, method="relativecounts", scale_factor=1.0, dtype=np.float64)
+    tot = np.asarray(X.sum(axis=1)).ravel().astype(np.float64)
+    rows = X.indices
+    expect = X.data.astype(np.float64) / tot[rows]
+    assert np.array_equal(Y.data, expect)
+
+
 def test_normalize_f64_vs_oracle(sev, orc):
     t = sev.nb_tables(800, seed=5, density=0.08)
     cp, rv, cnt = orc.synth_csc(1500, t)
