@@ -160,13 +160,6 @@ __global__ void __launch_bounds__(kBlock) k_to_bf16(const float *in, uint16_t *o
         out[i] = cvt_store(in[i], (uint16_t *)nullptr);
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_row_count(const int32_t *row, int64_t nnz, int32_t *cnt) {
-    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < nnz;
-         p += (int64_t)gridDim.x * kBlock)
-        atomicAdd(&cnt[row[p]], 1);
-}
-
 // row lengths rounded up to a multiple of R (a power of two)
 __global__ void __launch_bounds__(kBlock) k_round_up(int32_t *cnt, int64_t m, int R) {
     for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
@@ -1365,9 +1358,10 @@ unsigned grid_rows(int64_t m, int per_cu = 4) {
 int val_bytes(int vtype) { return vtype == kValBF16 ? 2 : (vtype == kValF64 ? 8 : 4); }
 
 template <class ST, class DT>
-void launch_csr_fill(int64_t m, int64_t n, int64_t nbs, const int64_t *boff, const int32_t *row,
-                     const void *val, const int64_t *rowptr, uint16_t *ccol, void *cval,
-                     hipStream_t st) {
+void launch_csr_fill(int64_t m, int64_t n, int64_t nbs, int64_t nnz, const int64_t *boff,
+                     const int32_t *row, const void *val, const int64_t *rowptr, uint16_t *ccol,
+                     void *cval, hipStream_t st) {
+    if (nnz == 0) return;
     hipLaunchKernelGGL((k_csr_fill<ST, DT>), dim3((unsigned)ceil_div(nbs, kWaves)), dim3(kBlock),
                        0, st, m, n, nbs, boff, row, (const ST *)val, rowptr, ccol, (DT *)cval);
     SEV_LAUNCH_CHECK();
@@ -1430,12 +1424,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, nbs, kBuildRows,
                        colptr, row, b.boff.get(), (uint16_t *)nullptr);
     SEV_LAUNCH_CHECK();
-    SEV_HIP(hipMemsetAsync(b.cnt.get(), 0, sizeof(int32_t) * m, st));
-    if (op.nnz > 0) {
-        hipLaunchKernelGGL(k_row_count, dim3(grid_rows(op.nnz, 16)), dim3(kBlock), 0, st, row,
-                           op.nnz, b.cnt.get());
-        SEV_LAUNCH_CHECK();
-    }
+    row_counts(colptr, row, op.nnz, m, n, b.cnt.get(), st);
     // BF16 rows are padded to 8 entries (16-byte loads of values and columns)
     const int ralign = bf16 ? 8 : 4;
     hipLaunchKernelGGL(k_round_up, dim3(grid_rows(m)), dim3(kBlock), 0, st, b.cnt.get(), m,
@@ -1450,15 +1439,16 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     op.cval.ensure(((size_t)op.csr_nnz + 8) * vb);
     SEV_HIP(hipMemsetAsync(op.ccol.get(), 0, sizeof(uint16_t) * (size_t)op.csr_nnz, st));
     SEV_HIP(hipMemsetAsync(op.cval.get(), 0, vb * (size_t)op.csr_nnz, st));
+    const int64_t *bo = b.boff.get(), *rptr = op.rowptr.get();
     if (op.vtype == kValBF16)
-        launch_csr_fill<float, uint16_t>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
-                                         op.ccol.get(), op.cval.get(), st);
+        launch_csr_fill<float, uint16_t>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
+                                         op.cval.get(), st);
     else if (op.vtype == kValF64)
-        launch_csr_fill<double, double>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
-                                        op.ccol.get(), op.cval.get(), st);
+        launch_csr_fill<double, double>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
+                                        op.cval.get(), st);
     else
-        launch_csr_fill<float, float>(m, n, nbs, b.boff.get(), row, val, op.rowptr.get(),
-                                      op.ccol.get(), op.cval.get(), st);
+        launch_csr_fill<float, float>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
+                                      op.cval.get(), st);
     op.col12 = op.vtype == kValBF16 && n <= 4096;
     if (op.col12) pack12(op.ccol.get(), op.csr_nnz, op.ccol12, st);
     const double avg = (double)op.csr_nnz / (double)m;

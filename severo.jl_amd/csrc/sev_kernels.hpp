@@ -47,6 +47,9 @@ void synth_classes(int64_t C, uint64_t seed, double boost_frac, uint8_t *cls_d, 
 // colptr/row: CSC with strictly increasing rows in every column
 void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
                 int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st);
+// cnt[r] = number of stored entries of row r (same CSC requirement)
+void row_counts(const int64_t *colptr, const int32_t *row, int64_t nnz, int64_t m, int64_t n,
+                int32_t *cnt, hipStream_t st);
 // bad |= 1 when some column's rows are not strictly increasing
 void check_sorted(int64_t n, const int64_t *colptr, const int32_t *row, int32_t *bad,
                   hipStream_t st);

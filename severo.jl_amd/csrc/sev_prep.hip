@@ -109,8 +109,8 @@ k_synth_fill(int64_t C, int64_t nchunk, int64_t chunk, uint64_t seedm, double r,
 // normalize_cells
 // --------------------------------------------------------------------------
 
-// s = sum(A, dims=2): integer row (cell) totals over the CSC nonzeros.
-// Block (x, y) sums the entries of columns [x*kTotCols, ...) whose rows fall
+// s = sum(A, dims=2): integer row (cell) totals over the CSC nonzeros (or
+// the number of stored entries per row when cnt is null).  Block (x, y) sums the entries of columns [x*kTotCols, ...) whose rows fall
 // in [y*kTotRows, ...) into an LDS histogram (one lane per column finds its
 // segment by binary search in the column's sorted rows; the wave then walks
 // the 64 segments, four strides of loads in flight at a time), and adds the
@@ -161,7 +161,7 @@ k_row_totals(int64_t m, int64_t n, const int64_t *colptr, const int32_t *row, co
                 for (int u = 0; u < 4; ++u) {
                     const int64_t q = p + u * kWave < sb ? p + u * kWave : sb - 1;
                     rr[u] = row[q];
-                    cc[u] = cnt[q];
+                    cc[u] = cnt ? cnt[q] : 1;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -456,6 +456,16 @@ void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, i
     const dim3 grid((unsigned)ceil_div(ngenes, kTotCols), (unsigned)ceil_div(ncells, kTotRows));
     hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, ncells, ngenes, colptr, row,
                        cnt, tot);
+    SEV_LAUNCH_CHECK();
+}
+
+void row_counts(const int64_t *colptr, const int32_t *row, int64_t nnz, int64_t m, int64_t n,
+                int32_t *cnt, hipStream_t st) {
+    SEV_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)(m > 0 ? m : 1), st));
+    if (nnz == 0) return;
+    const dim3 grid((unsigned)ceil_div(n, kTotCols), (unsigned)ceil_div(m, kTotRows));
+    hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, m, n, colptr, row,
+                       (const int32_t *)nullptr, (uint32_t *)cnt);
     SEV_LAUNCH_CHECK();
 }
 
