@@ -110,15 +110,17 @@ k_synth_fill(int64_t C, int64_t nchunk, int64_t chunk, uint64_t seedm, double r,
 // --------------------------------------------------------------------------
 
 // s = sum(A, dims=2): integer row (cell) totals over the CSC nonzeros (or
-// the number of stored entries per row when cnt is null).  Block (x, y) sums the entries of columns [x*kTotCols, ...) whose rows fall
-// in [y*kTotRows, ...) into an LDS histogram (one lane per column finds its
+// the number of stored entries per row when cnt is null).  Block (x, y)
+// sums the entries of columns [x*cpb, ...) whose rows fall in
+// [y*kTotRows, ...) into an LDS histogram (one lane per column finds its
 // segment by binary search in the column's sorted rows; the wave then walks
 // the 64 segments, four strides of loads in flight at a time), and adds the
-// histogram to the totals with one global atomic per nonzero row.
+// histogram to the totals with one global atomic per nonzero row.  The
+// launch picks cpb (columns per block).
 constexpr int kTotThreads = 1024;
 constexpr int kTotWaves = kTotThreads / kWave;
 constexpr int64_t kTotRows = 16384;  // rows per histogram (64 KiB of LDS)
-constexpr int64_t kTotCols = 2048;   // columns per block
+constexpr int64_t kTotBlocks = 256;
 
 __device__ __forceinline__ int64_t lower_row(const int32_t *row, int64_t lo, int64_t hi,
                                              int64_t key) {
@@ -133,14 +135,14 @@ __device__ __forceinline__ int64_t lower_row(const int32_t *row, int64_t lo, int
 }
 
 __global__ void __launch_bounds__(kTotThreads)
-k_row_totals(int64_t m, int64_t n, const int64_t *colptr, const int32_t *row, const int32_t *cnt,
-             uint32_t *tot) {
+k_row_totals(int64_t m, int64_t n, int64_t cpb, const int64_t *colptr, const int32_t *row,
+             const int32_t *cnt, uint32_t *tot) {
     __shared__ uint32_t h[kTotRows];
     const int64_t r0 = (int64_t)blockIdx.y * kTotRows;
     const int64_t r1 = r0 + kTotRows < m ? r0 + kTotRows : m;
-    const int64_t c0 = (int64_t)blockIdx.x * kTotCols;
-    const int64_t c1 = c0 + kTotCols < n ? c0 + kTotCols : n;
-    for (int i = threadIdx.x; i < kTotRows; i += kTotThreads) h[i] = 0;
+    const int64_t c0 = (int64_t)blockIdx.x * cpb;
+    const int64_t c1 = c0 + cpb < n ? c0 + cpb : n;
+    for (int64_t i = threadIdx.x; i < r1 - r0; i += kTotThreads) h[i] = 0;
     __syncthreads();
     const int w = threadIdx.x / kWave, lane = lane_id();
     for (int64_t cb = c0 + (int64_t)w * kWave; cb < c1; cb += (int64_t)kTotWaves * kWave) {
@@ -448,25 +450,34 @@ void synth_fill(const SynthPlan &plan, uint64_t seed, double r, const double *p0
     SEV_LAUNCH_CHECK();
 }
 
+// row ranges x column chunks: a chunk is a multiple of 64 columns (one per
+// lane), at most one per wavefront of the block, and small enough for about
+// kTotBlocks blocks in all
+static void launch_row_totals(int64_t m, int64_t n, const int64_t *colptr, const int32_t *row,
+                              const int32_t *cnt, uint32_t *tot, hipStream_t st) {
+    if (m == 0 || n == 0) return;
+    const int64_t nrange = ceil_div(m, kTotRows);
+    int64_t cpb = ceil_div(ceil_div(n * nrange, kTotBlocks), (int64_t)kWave) * kWave;
+    cpb = std::max<int64_t>(kWave, std::min<int64_t>(cpb, (int64_t)kTotWaves * kWave));
+    const dim3 grid((unsigned)ceil_div(n, cpb), (unsigned)nrange);
+    hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, m, n, cpb, colptr, row, cnt,
+                       tot);
+    SEV_LAUNCH_CHECK();
+}
+
 void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
                 int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st) {
     SEV_HIP(hipMemsetAsync(tot, 0, sizeof(uint32_t) * (size_t)(ncells > 0 ? ncells : 1), st));
     if (nnz == 0) return;
     ProfScope ps("row_totals", st, 8.0 * nnz + 4.0 * ncells);
-    const dim3 grid((unsigned)ceil_div(ngenes, kTotCols), (unsigned)ceil_div(ncells, kTotRows));
-    hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, ncells, ngenes, colptr, row,
-                       cnt, tot);
-    SEV_LAUNCH_CHECK();
+    launch_row_totals(ncells, ngenes, colptr, row, cnt, tot, st);
 }
 
 void row_counts(const int64_t *colptr, const int32_t *row, int64_t nnz, int64_t m, int64_t n,
                 int32_t *cnt, hipStream_t st) {
     SEV_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)(m > 0 ? m : 1), st));
     if (nnz == 0) return;
-    const dim3 grid((unsigned)ceil_div(n, kTotCols), (unsigned)ceil_div(m, kTotRows));
-    hipLaunchKernelGGL(k_row_totals, grid, dim3(kTotThreads), 0, st, m, n, colptr, row,
-                       (const int32_t *)nullptr, (uint32_t *)cnt);
-    SEV_LAUNCH_CHECK();
+    launch_row_totals(m, n, colptr, row, nullptr, (uint32_t *)cnt, st);
 }
 
 void check_sorted(int64_t n, const int64_t *colptr, const int32_t *row, int32_t *bad,
