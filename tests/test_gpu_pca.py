@@ -59,6 +59,19 @@ def test_irlba_dense(sev, m, n, nu):
     assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
 
 
+@pytest.mark.parametrize("nu", [55, 57])
+def test_irlba_wide_basis(sev, nu):
+    """work = nu + 7 at the top of each projected-SVD kernel's range: 62
+    (8 lanes per column pair) and 64 (16 lanes per pair)."""
+    rng = np.random.default_rng(nu)
+    m, n = 400, 160
+    X = rng.standard_normal((m, n)) * (1.0 + 3.0 * np.exp(-np.arange(n) / 20.0))
+    S = sev.irlba(sp.csc_matrix(X), nu, tol=1e-6, init=rng.standard_normal(n))
+    ss = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(S.S, ss[:nu], rtol=1e-8)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-6
+
+
 def test_irlba_graded_spectrum(sev):
     """Singular values spread over eight decades: the projected matrices are
     too ill-conditioned for U = B V / s, and the device SVD takes its
