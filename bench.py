@@ -328,8 +328,8 @@ def main():
                          "eigendecomposition (host BLAS; adds about a minute at 1M cells)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of the path "
-                          "is described in DESIGN.md but not implemented", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the path is not sharded across "
+                          "GPUs", "n_gpus": args.gpus}))
         sys.exit(2)
 
     sev = entry.load_package()

@@ -136,6 +136,17 @@ void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int 
 void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int ntables,
                                int include_self, int seed, int32_t *nn_index, double *distances);
 
+/* Exact cosine kNN with the native libcell signature (the CosineDist
+ * methods of src/neighbours.jl:41-50 and :63-72).  Each row is first scaled
+ * to unit length in its precision (sum of squares in t order, IEEE sqrt and
+ * division; an all-zero row stays zero); the Euclidean search above then
+ * runs on the unit rows, and the distance reported is D / 2 of the unit
+ * rows (1 - cos in exact arithmetic). */
+void FindNeighboursCosine32(const float *X, int n, int d, int ld, int k, int ntables,
+                            int include_self, int seed, int32_t *nn_index, float *distances);
+void FindNeighboursCosine64(const double *X, int n, int d, int ld, int k, int ntables,
+                            int include_self, int seed, int32_t *nn_index, double *distances);
+
 /* Same search with a status and a precision form: SEV_PREC_FP32 (FP32
  * candidates on MFMA), SEV_PREC_BF16 (BF16 candidates on MFMA) -- both
  * followed by an exact FP32 re-rank and a certificate, so the result equals
@@ -145,6 +156,11 @@ void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int
 int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
             int include_self, int precision, int32_t *nn_index, void *distances,
             int64_t *n_fallback);
+/* The cosine form of sev_knn (unit rows, distance D / 2 of the unit rows),
+ * with the same arguments and precision forms. */
+int sev_knn_cosine(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
+                   int include_self, int precision, int32_t *nn_index, void *distances,
+                   int64_t *n_fallback);
 
 /* Jaccard SNN graph (src/neighbours.jl:79, :88-93): from nn_index (n x k
  * column-major, 1-based), S = nn'nn, w = x/(k + (k - x)) in dtype, entries

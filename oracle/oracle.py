@@ -226,6 +226,47 @@ def knn(X, k, include_self=True, qidx=None, nthreads=None):
     return idx, dist
 
 
+def unit_rows(X):
+    """Rows of X scaled to unit length in X's precision: sum of squares in
+    column order (each square and sum rounded), IEEE sqrt and division; an
+    all-zero row stays zero.  The cosine form of ann! (CosineDist,
+    src/neighbours.jl:41-50, :63-72) is defined here on these rows."""
+    X = np.asfortranarray(X)
+    dt = X.dtype.type
+    s = np.zeros(X.shape[0], X.dtype)
+    for t in range(X.shape[1]):
+        v = X[:, t]
+        s = (s + v * v).astype(X.dtype)
+    nrm = np.sqrt(s).astype(X.dtype)
+    U = X.copy(order="F")
+    nz = s > dt(0)
+    U[nz, :] = (X[nz, :] / nrm[nz, None]).astype(X.dtype)
+    return U
+
+
+def half_pair_dist(U, nn_index):
+    """D / 2 of the unit rows for each (i, nn_index[i, t]) (1-based), with D
+    the Euclidean definition of knn(): squares of differences, accumulated in
+    column order, no FMA."""
+    U = np.asfortranarray(U)
+    n, k = nn_index.shape
+    j = np.asarray(nn_index, np.int64) - 1
+    D = np.zeros((n, k), U.dtype)
+    rows = np.arange(n)[:, None]
+    for t in range(U.shape[1]):
+        diff = (U[rows, t] - U[j, t]).astype(U.dtype)
+        D = (D + diff * diff).astype(U.dtype)
+    return (U.dtype.type(0.5) * D).astype(U.dtype)
+
+
+def knn_cosine(X, k, include_self=True, nthreads=None):
+    """Exact cosine kNN: the exact Euclidean search on unit_rows(X), ties to
+    the lower index, distance D / 2 (1 - cos in exact arithmetic)."""
+    U = unit_rows(X)
+    idx, _ = knn(U, k, include_self=include_self, nthreads=nthreads)
+    return idx, np.asfortranarray(half_pair_dist(U, idx))
+
+
 def snn(nn_index, k, prune=1.0 / 15, dtype=np.float64):
     """Jaccard SNN graph as CSC (colptr, rowval 1-based int64, nzval)."""
     nn_index = np.asfortranarray(nn_index, np.int32)

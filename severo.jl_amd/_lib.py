@@ -40,6 +40,9 @@ SIGNATURES = {
     "FindNeighboursEuclidean32": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "FindNeighboursEuclidean64": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "sev_knn": (_int, [_p, _i64, _i64, _i64, _int, _i64, _int, _int, _p, _p, _pi64]),
+    "sev_knn_cosine": (_int, [_p, _i64, _i64, _i64, _int, _i64, _int, _int, _p, _p, _pi64]),
+    "FindNeighboursCosine32": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
+    "FindNeighboursCosine64": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "sev_snn_jaccard": (_int, [_i64, _i64, _p, _dbl, _int, _pi64, _p, _p, _p]),
     "sev_find_variable_features": (_int, [_i64, _i64, _p, _p, _p, _i64, _dbl, _p, _p]),
     "sev_pipeline_find_variable_features": (_int, [_p, _i64, _dbl, _p, _p]),

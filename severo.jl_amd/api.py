@@ -312,12 +312,16 @@ def _coords(X, dims):
     return X
 
 
-def ann(X, k: int, include_self=True, ntables=None, precision="fp32", return_fallback=False):
-    """Exact Euclidean kNN (the search behind ann!/FindNeighboursEuclidean,
-    src/neighbours.jl:19-63).  Returns (nn_index n x k 1-based Int32,
-    distances n x k), column-major like the reference.  ``precision``
-    selects how candidates are proposed ("fp32" / "bf16" matrix cores with an
-    exact re-rank, or "exact" exhaustive); the result is the same."""
+def ann(X, k: int, include_self=True, ntables=None, precision="fp32", return_fallback=False,
+        metric="euclidean"):
+    """Exact kNN (the search behind ann!/FindNeighboursEuclidean and
+    FindNeighboursCosine, src/neighbours.jl:19-72).  Returns (nn_index n x k
+    1-based Int32, distances n x k), column-major like the reference.
+    ``precision`` selects how candidates are proposed ("fp32" / "bf16"
+    matrix cores with an exact re-rank, or "exact" exhaustive); the result is
+    the same.  ``metric="cosine"`` (Distances.CosineDist) searches the rows
+    scaled to unit length and reports D / 2 of the unit rows (1 - cos in
+    exact arithmetic)."""
     X = np.asfortranarray(X)
     if X.dtype not in (np.float32, np.float64):
         X = X.astype(np.float64, order="F")
@@ -327,8 +331,12 @@ def ann(X, k: int, include_self=True, ntables=None, precision="fp32", return_fal
     nfb = C.c_int64(0)
     prec = {"fp32": _lib.SEV_PREC_FP32, "bf16": _lib.SEV_PREC_BF16,
             "exact": _lib.SEV_PREC_EXACT, "fp8": _lib.SEV_PREC_FP8}[precision]
-    check(_lib.load().sev_knn(_ptr(X), n, d, n, _dtype_code(X.dtype), k, int(include_self), prec,
-                              _ptr(idx), _ptr(dist), C.byref(nfb)))
+    metrics = {"euclidean": "sev_knn", "cosine": "sev_knn_cosine"}
+    if metric not in metrics:
+        raise ValueError(f"unknown metric: {metric}")
+    fn = getattr(_lib.load(), metrics[metric])
+    check(fn(_ptr(X), n, d, n, _dtype_code(X.dtype), k, int(include_self), prec, _ptr(idx),
+             _ptr(dist), C.byref(nfb)))
     if return_fallback:
         return idx, dist, nfb.value
     return idx, dist

@@ -247,7 +247,8 @@ void scale_host(int64_t m, int64_t n, const DevBuf<int64_t> &cp, int64_t nnz, co
 namespace {
 template <class T>
 void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
-              int precision, int32_t *nn_index, T *distances, int64_t *n_fallback) {
+              int precision, int32_t *nn_index, T *distances, int64_t *n_fallback,
+              bool cosine = false) {
     SEV_REQUIRE(n >= 1 && d >= 1 && ld >= n, "bad X shape");
     SEV_REQUIRE(k >= 1 && k <= (include_self ? n : n - 1), "k larger than the number of points");
     Stream s;
@@ -255,6 +256,7 @@ void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int inclu
     dx.ensure((size_t)(n * d));
     SEV_HIP(hipMemcpy2DAsync(dx.get(), sizeof(T) * n, X, sizeof(T) * ld, sizeof(T) * n, d,
                              hipMemcpyHostToDevice, s.st));
+    if (cosine) knn_unit_rows<T>(dx.get(), n, d, s.st);
     DevBuf<int32_t> idx;
     DevBuf<T> dist;
     idx.ensure((size_t)(n * k));
@@ -271,6 +273,7 @@ void knn_host(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int inclu
     } else {
         knn_exact<T>(dx.get(), n, d, n, k, include_self, nullptr, n, idx.get(), dist.get(), s.st);
     }
+    if (cosine) knn_half_pair_dist<T>(dx.get(), n, d, idx.get(), k, dist.get(), s.st);
     if (n_fallback) *n_fallback = nfb;
     std::vector<int32_t> hi((size_t)(n * k));
     std::vector<T> hd((size_t)(n * k));
@@ -1198,6 +1201,25 @@ int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t 
     });
 }
 
+int sev_knn_cosine(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
+                   int include_self, int precision, int32_t *nn_index, void *distances,
+                   int64_t *n_fallback) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
+                        precision == SEV_PREC_EXACT || precision == SEV_PREC_FP8,
+                    "bad precision");
+        if (dtype == SEV_FLOAT32)
+            knn_host<float>((const float *)X, n, d, ld, k, include_self, precision, nn_index,
+                            (float *)distances, n_fallback, true);
+        else if (dtype == SEV_FLOAT64)
+            knn_host<double>((const double *)X, n, d, ld, k, include_self, precision, nn_index,
+                             (double *)distances, n_fallback, true);
+        else
+            fail(kErrArg, "bad dtype");
+    });
+}
+
 void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int ntables,
                                int include_self, int seed, int32_t *nn_index, float *distances) {
     (void)ntables;
@@ -1212,6 +1234,22 @@ void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int
     (void)seed;
     (void)sev_knn(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index, distances,
                   nullptr);
+}
+
+void FindNeighboursCosine32(const float *X, int n, int d, int ld, int k, int ntables,
+                            int include_self, int seed, int32_t *nn_index, float *distances) {
+    (void)ntables;
+    (void)seed;
+    (void)sev_knn_cosine(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32, nn_index,
+                         distances, nullptr);
+}
+
+void FindNeighboursCosine64(const double *X, int n, int d, int ld, int k, int ntables,
+                            int include_self, int seed, int32_t *nn_index, double *distances) {
+    (void)ntables;
+    (void)seed;
+    (void)sev_knn_cosine(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index,
+                         distances, nullptr);
 }
 
 int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,

@@ -284,6 +284,13 @@ constexpr int kMaxK = 64;
 template <class T>
 void knn_exact(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                const int64_t *qidx, int64_t nq, int32_t *idx, T *dist, hipStream_t st);
+// Cosine form: X (n x d, column-major, ld = n) scaled in place to unit rows
+// (zero rows stay zero); then D / 2 of the pairs (i, idx[i*k + t]) into dist.
+template <class T>
+void knn_unit_rows(T *X, int64_t n, int64_t d, hipStream_t st);
+template <class T>
+void knn_half_pair_dist(const T *X, int64_t n, int64_t d, const int32_t *idx, int64_t k, T *dist,
+                        hipStream_t st);
 // Same, but result row r is written at row qidx[r] of idx/dist (n x k).
 template <class T>
 void knn_exact_rows(const T *X, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,

@@ -19,6 +19,7 @@
 // lane (insertion by ballot + shift).
 #include "sev_kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 
@@ -194,6 +195,78 @@ void knn_exact_rows_batch(const float *X, int64_t n, int64_t d, int64_t ld, int6
     knn_exact_impl<float>(X, n, d, ld, k, include_self, qidx, nq, 1, idx, dist, st, seg,
                           row_sample);
 }
+
+// --------------------------------------------------------------------------
+// Cosine neighbours (the CosineDist form of src/neighbours.jl:41-50 and
+// :63-72).  Each row is scaled to unit length in T (sum of squares in t
+// order, IEEE sqrt and division; an all-zero row stays zero); the exact
+// Euclidean search then runs on the unit rows, whose D ranks pairs as
+// 1 - cos does in exact arithmetic; the reported distance is D / 2, with D
+// recomputed for the found pairs by the same definition.
+// --------------------------------------------------------------------------
+namespace {
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_unit_rows(T *X, int64_t n, int64_t d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        T s = 0;
+        for (int64_t t = 0; t < d; ++t) {
+            const T v = X[t * n + i];
+            s = s + v * v;
+        }
+        if (s > T(0)) {
+            const T nrm = sqrt(s);
+            for (int64_t t = 0; t < d; ++t) X[t * n + i] = X[t * n + i] / nrm;
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_half_pair_dist(const T *X, int64_t n, int64_t d, const int32_t *idx, int64_t k, T *dist) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n * k;
+         p += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = p / k;
+        const int32_t j = idx[p];
+        if (j < 0 || j >= n) continue;
+        T D = 0;
+        for (int64_t t = 0; t < d; ++t) {
+            const T diff = X[t * n + i] - X[t * n + j];
+            D = D + diff * diff;
+        }
+        dist[p] = T(0.5) * D;
+    }
+}
+
+unsigned rows_grid(int64_t n) {
+    const int64_t g = ceil_div(n > 0 ? n : 1, (int64_t)kBlock);
+    return (unsigned)std::min<int64_t>(g, (int64_t)device_cu_count() * 16);
+}
+
+}  // namespace
+
+template <class T>
+void knn_unit_rows(T *X, int64_t n, int64_t d, hipStream_t st) {
+    hipLaunchKernelGGL(k_unit_rows<T>, dim3(rows_grid(n)), dim3(kBlock), 0, st, X, n, d);
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T>
+void knn_half_pair_dist(const T *X, int64_t n, int64_t d, const int32_t *idx, int64_t k, T *dist,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_half_pair_dist<T>, dim3(rows_grid(n * k)), dim3(kBlock), 0, st, X, n, d,
+                       idx, k, dist);
+    SEV_LAUNCH_CHECK();
+}
+
+template void knn_unit_rows<float>(float *, int64_t, int64_t, hipStream_t);
+template void knn_unit_rows<double>(double *, int64_t, int64_t, hipStream_t);
+template void knn_half_pair_dist<float>(const float *, int64_t, int64_t, const int32_t *, int64_t,
+                                        float *, hipStream_t);
+template void knn_half_pair_dist<double>(const double *, int64_t, int64_t, const int32_t *,
+                                         int64_t, double *, hipStream_t);
 
 template void knn_exact<float>(const float *, int64_t, int64_t, int64_t, int64_t, int,
                                const int64_t *, int64_t, int32_t *, float *, hipStream_t);

@@ -3,8 +3,9 @@
 #
 # Two ways to use it:
 #   1. Drop-in at the native boundary: point Severo's `libcell` at this
-#      library (it exports FindNeighboursEuclidean32/64 with libcell's
-#      signature, src/neighbours.jl:32-63); see INTEGRATION.md.
+#      library (it exports irlba and FindNeighboursEuclidean32/64 and
+#      FindNeighboursCosine32/64 with libcell's signatures,
+#      src/irlba.jl:66-71, src/neighbours.jl:32-72).
 #   2. The functions below, which take and return the same types as
 #      Severo's API (normalize_cells, scale_features, embedding/pca,
 #      nearest_neighbours, jaccard_index, shared_nearest_neighbours) and run
@@ -133,19 +134,31 @@ embedding(X, n::Integer=50; method=:pca, kw...) =
     Symbol(method) == :pca ? pca(X, n; kw...) : error("unknown reduction method: $method")
 
 # ------------------------------------------------------------- neighbours
-# exact kNN behind ann! (src/neighbours.jl:19-63); indices 1-based, n x k
+# exact kNN behind ann! (src/neighbours.jl:19-72); indices 1-based, n x k;
+# metric :euclidean, or :cosine (unit rows, distance D / 2 of the unit rows)
 function ann(X::StridedMatrix{T}, k::Integer; include_self::Bool=true,
-             precision=:fp32) where {T<:Union{Float32,Float64}}
+             precision=:fp32, metric=:euclidean) where {T<:Union{Float32,Float64}}
     n, d = size(X)
     @assert stride(X, 1) == 1
     idx = Matrix{Int32}(undef, n, k)
     dist = Matrix{T}(undef, n, k)
     nfb = Ref{Int64}(0)
-    check(ccall((:sev_knn, libsev), Cint,
-                (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Int64, Cint, Cint, Ptr{Int32}, Ptr{Cvoid},
-                 Ref{Int64}),
-                X, n, d, stride(X, 2), dtcode(T), k, include_self, SEV_PREC[precision], idx, dist,
-                nfb))
+    m = Symbol(metric)
+    if m == :euclidean
+        check(ccall((:sev_knn, libsev), Cint,
+                    (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Int64, Cint, Cint, Ptr{Int32},
+                     Ptr{Cvoid}, Ref{Int64}),
+                    X, n, d, stride(X, 2), dtcode(T), k, include_self, SEV_PREC[precision], idx,
+                    dist, nfb))
+    elseif m == :cosine
+        check(ccall((:sev_knn_cosine, libsev), Cint,
+                    (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Int64, Cint, Cint, Ptr{Int32},
+                     Ptr{Cvoid}, Ref{Int64}),
+                    X, n, d, stride(X, 2), dtcode(T), k, include_self, SEV_PREC[precision], idx,
+                    dist, nfb))
+    else
+        error("unknown metric: $metric")
+    end
     idx, dist
 end
 

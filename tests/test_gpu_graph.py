@@ -42,6 +42,37 @@ def test_find_neighbours_abi_strided(sev, orc):
     assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("include_self", [True, False])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "exact"])
+def test_ann_cosine_exact(sev, orc, dtype, include_self, precision):
+    """Cosine form (unit rows, D / 2) bit for bit against the oracle, with
+    scaled copies of rows (distance 0 ties) and an all-zero row."""
+    if dtype == np.float64 and precision == "bf16":
+        pytest.skip("the BF16 candidate form is for Float32 points")
+    rng = np.random.default_rng(21)
+    base = rng.standard_normal((2500, 30)).astype(dtype)
+    X = np.concatenate([base, 3.0 * base[:200], np.zeros((1, 30), dtype)])
+    idx, dist = sev.ann(X, 12, include_self=include_self, precision=precision, metric="cosine")
+    ridx, rdist = orc.knn_cosine(X, 12, include_self=include_self)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist)
+
+
+def test_find_neighbours_cosine_abi(sev, orc):
+    """FindNeighboursCosine32/64 with ld > n (libcell's signature)."""
+    rng = np.random.default_rng(22)
+    for dt, name in ((np.float32, "FindNeighboursCosine32"), (np.float64, "FindNeighboursCosine64")):
+        full = np.asfortranarray(rng.standard_normal((900, 16)).astype(dt))
+        n, d, k = 700, 16, 10
+        idx = np.zeros((n, k), np.int32, order="F")
+        dist = np.zeros((n, k), dt, order="F")
+        getattr(sev.load(), name)(full.ctypes.data, n, d, 900, k, 32, 0, 7, idx.ctypes.data,
+                                  dist.ctypes.data)
+        ridx, rdist = orc.knn_cosine(np.asfortranarray(full[:n]), k, include_self=False)
+        assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist), name
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 @pytest.mark.parametrize("k,prune", [(20, 1 / 15), (15, 1 / 15), (30, 1 / 15), (10, 0.0)])
 def test_snn_vs_oracle(sev, orc, dt, k, prune):

@@ -293,6 +293,37 @@ def test_knn_ties_lower_index_wins(orc):
     assert idx[40, 0] == 1 and idx[40, 1] == 41 and idx[80, 0] == 1
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 3e-7), (np.float64, 1e-15)])
+def test_knn_cosine_vs_float64_cosine(orc, dtype, tol):
+    """The cosine form ranks like 1 - cos of a float64 brute force (points
+    without near ties) and reports 1 - cos to the rounding of dtype."""
+    rng = np.random.default_rng(15)
+    X = (rng.standard_normal((400, 12)) * rng.uniform(0.1, 10.0, (400, 1))).astype(dtype)
+    idx, dist = orc.knn_cosine(X, 8)
+    Xd = X.astype(np.float64)
+    U = Xd / np.linalg.norm(Xd, axis=1)[:, None]
+    C = 1.0 - U @ U.T
+    ridx = np.stack([np.lexsort((np.arange(400), C[i]))[:8] for i in range(400)]) + 1
+    assert np.array_equal(idx, ridx)
+    assert np.max(np.abs(dist - np.take_along_axis(C, idx - 1, 1))) < tol * 4
+    assert dist.dtype == dtype
+
+
+def test_knn_cosine_scale_invariance_and_zero_row(orc):
+    """Positive multiples of a row are at distance 0 (lower index first);
+    an all-zero row stays zero, at D / 2 = |unit row|^2 / 2 (0.5 up to
+    rounding) from every unit row."""
+    rng = np.random.default_rng(16)
+    base = rng.standard_normal((50, 6)).astype(np.float32)
+    X = np.concatenate([base, 4.0 * base[:10], np.zeros((1, 6), np.float32)])
+    idx, dist = orc.knn_cosine(X, 3)
+    assert idx[50, 0] == 1 and idx[50, 1] == 51 and dist[50, 1] <= 1e-7
+    assert np.all(np.abs(dist[60, 1:] - 0.5) < 1e-6)  # |unit row|^2 / 2, rounded
+    # a planted mutation (no scaling to unit length) ranks differently
+    ridx, _ = orc.knn(X, 3)
+    assert not np.array_equal(idx, ridx)
+
+
 def test_knn_reference_recall_property(orc):
     # test/test_nn.jl:31-38: 30th percentile Jaccard recall vs brute force == 1
     rng = np.random.default_rng(14)
