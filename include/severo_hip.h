@@ -52,6 +52,12 @@ extern "C" {
 #define SEV_PREC_FP8 3   /* kNN only: FP8 e4m3 candidates (per-128-row scaling), then
                           * BF16 / FP32 candidates for rows without a certificate */
 
+/* find_variable_features methods besides :vst */
+#define SEV_HVG_DISPERSION 1
+#define SEV_HVG_MEANVARPLOT 2
+#define SEV_HVG_SAUNDERS 3
+#define SEV_BIN_WIDTH 0 /* binning_method=:width (the only one supported) */
+
 /* ---- library ----------------------------------------------------------- */
 const char *sev_last_error(void);
 int sev_version(void);
@@ -182,6 +188,29 @@ int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune,
 int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
                                const int32_t *nzval, int64_t nfeatures, double loess_span,
                                int64_t *selected, double *metric);
+
+/* find_variable_features(counts, nfeatures; method=:dispersion,
+ * :meanvarplot or :saunders) (src/variablefeatures.jl:60-91,128-161) on
+ * norm = row_norm(counts, 1.0): the per-gene FP64 mean and variance of the
+ * relative counts (src/scaling.jl:18-34), then
+ *   SEV_HVG_DISPERSION  log(var/mean) (NaN -> 0);
+ *   SEV_HVG_MEANVARPLOT the z-score of log(var/mean) within num_bins bins of
+ *                       equal width of log1p(mean) (binning SEV_BIN_WIDTH;
+ *                       the reference's :frequency binning places the
+ *                       smallest mean in bin 0, outside its bin statistics,
+ *                       and is refused);
+ *   SEV_HVG_SAUNDERS    log10(var) - log10(mean nolan) for genes whose
+ *                       var/nolan exceeds mean + q sqrt(mean nolan / m),
+ *                       nolan = mean(1 ./ cell totals), q the standard
+ *                       normal quantile at 1 - alpha_thresh/n/2 (the
+ *                       reference's default alpha_thresh is 0.1); 0 else.
+ * rowval: 1-based, strictly increasing within each column.  Outputs as in
+ * sev_find_variable_features.  num_bins is read for MEANVARPLOT,
+ * alpha_thresh for SAUNDERS. */
+int sev_find_variable_features_method(int64_t m, int64_t n, const int64_t *colptr,
+                                      const int64_t *rowval, const int32_t *nzval, int method,
+                                      int64_t nfeatures, int64_t num_bins, int binning,
+                                      double alpha_thresh, int64_t *selected, double *metric);
 
 /* ---- device-resident pipeline ----------------------------------------- */
 typedef struct sev_pipeline sev_pipeline;

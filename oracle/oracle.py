@@ -59,6 +59,7 @@ def lib():
         L.so_snn_jaccard.restype = _i64
         L.so_snn_jaccard.argtypes = [_i64, _i64, _p, _dbl, C.c_int, _p, _p, _p]
         L.so_count_mean_std.argtypes = [_i64, _i64, _p, _p, _p, _p]
+        L.so_rel_mean_var.argtypes = [_i64, _i64, _p, _p, _p, _p, _p]
         L.so_loess_fit.restype = C.c_int
         L.so_loess_fit.argtypes = [_i64, _p, _p, _dbl, _p]
         L.so_standardized_var.argtypes = [_i64, _i64, _p, _p, _p, _p, _p]
@@ -292,6 +293,139 @@ def count_mean_std(m, colptr, counts):
     sd = np.empty(n)
     lib().so_count_mean_std(m, n, _ptr(colptr), _ptr(counts), _ptr(mu), _ptr(sd))
     return mu, sd
+
+
+def rel_mean_var(m, colptr, rows, counts):
+    """mean_var(Float64, row_norm(A, 1.0)) of an Int32 count CSC: per gene
+    Welford in FP64 over x / s_row (src/scaling.jl:18-34, src/normalize.jl)."""
+    n = len(colptr) - 1
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rows = np.ascontiguousarray(rows, np.int32)
+    counts = np.ascontiguousarray(counts, np.int32)
+    mu = np.empty(n)
+    var = np.empty(n)
+    lib().so_rel_mean_var(m, n, _ptr(colptr), _ptr(rows), _ptr(counts), _ptr(mu), _ptr(var))
+    return mu, var
+
+
+def _nan2zero(v):
+    return np.where(np.isnan(v), 0.0, v)
+
+
+def _libm(fn, v):
+    """fn (a C library function through Python's math module) per element;
+    log of 0 is -inf and of a negative or NaN argument NaN, as in C."""
+    import math
+
+    def one(x):
+        if x != x or x < 0.0:
+            return float("nan")
+        if x == 0.0 and fn in (math.log, math.log10):
+            return float("-inf")
+        if x == float("inf"):
+            return x
+        return fn(x)
+    return np.array([one(float(x)) for x in np.asarray(v, np.float64)])
+
+
+def hvg_dispersion(mu, var):
+    """select_dispersion (src/variablefeatures.jl:74-77): log(var / mu) with
+    NaN -> 0 (log_VMR, src/scaling.jl:190-192)."""
+    import math
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return _nan2zero(_libm(math.log, var / mu))
+
+
+def width_breaks(v, nbins):
+    """cut(v, nbins; method=:width) breaks (src/utils.jl:140-148): nbins + 1
+    evenly spaced points from min to max, widened by (max - min)/1000 at
+    both ends (interior points from an extended-precision interpolation)."""
+    lo, hi = float(np.min(v)), float(np.max(v))
+    dx = hi - lo
+    i = np.arange(nbins + 1, dtype=np.longdouble)
+    br = ((np.longdouble(lo) * (nbins - i) + np.longdouble(hi) * i) / nbins).astype(np.float64)
+    br[0], br[-1] = lo, hi
+    br[0] -= dx / 1000
+    br[-1] += dx / 1000
+    return br
+
+
+def cut_labels(v, breaks):
+    """_cut! with right=true (src/utils.jl:109-125): searchsortedlast, minus
+    one when v equals that break; 1-based labels."""
+    idx = np.searchsorted(breaks, v, side="right")  # searchsortedlast, 1-based
+    on = breaks[np.clip(idx - 1, 0, len(breaks) - 1)] == v
+    return np.where(on, idx - 1, idx)
+
+
+def hvg_meanvarplot(mu, var, nbins=20):
+    """select_meanvarplot (src/variablefeatures.jl:79-91): log1p(mu) and
+    log(var/mu) (NaN -> 0), width bins of log1p(mu), then the z-score of
+    the dispersion in its bin; per-bin mean and sd by the labelled Welford
+    in gene order (src/scaling.jl:90-112), NaN -> 0."""
+    import math
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lmu = _nan2zero(_libm(math.log1p, mu))
+        disp = _nan2zero(_libm(math.log, var / mu))
+        lab = cut_labels(lmu, width_breaks(lmu, nbins))
+        bm = np.zeros(nbins + 1)
+        bv = np.zeros(nbins + 1)
+        bn = np.zeros(nbins + 1, np.int64)
+        for v, k in zip(disp.tolist(), lab.tolist()):
+            bn[k] += 1
+            delta = v - bm[k]
+            bm[k] = bm[k] + delta / bn[k]
+            bv[k] = bv[k] + delta * (v - bm[k])
+        bs = np.sqrt(bv / (bn - 1))
+        z = (disp - bm[lab]) / bs[lab]
+    return _nan2zero(z)
+
+
+def norm_quantile_upper(tail):
+    """x with P(Z > x) = tail for a standard normal Z, by Newton steps on
+    erfc (the quantile at 1 - tail)."""
+    import math
+    x = math.sqrt(-2.0 * math.log(tail))  # start above the root
+    for _ in range(100):
+        f = 0.5 * math.erfc(x / math.sqrt(2.0)) - tail
+        dfdx = -math.exp(-0.5 * x * x) / math.sqrt(2.0 * math.pi)
+        step = f / dfdx
+        x -= step
+        if abs(step) <= 1e-16 * abs(x):
+            break
+    return x
+
+
+def hvg_saunders(m, colptr, rows, counts, mu, var, alpha=0.1):
+    """select_features_saunders (src/variablefeatures.jl:60-72): the Nolan
+    constant mean(1 ./ cell totals) (summed in cell order here), the
+    Bonferroni-corrected upper bound of a Poisson-like gene mean, and
+    log10(var) - log10(mu * nolan) for genes above it, else 0."""
+    G = len(colptr) - 1
+    tot = np.zeros(m, np.int64)
+    np.add.at(tot, np.asarray(rows, np.int64), np.asarray(counts, np.int64))
+    acc = 0.0
+    for t in tot.tolist():
+        acc += 1.0 / t
+    nolan = acc / m
+    a = alpha / G
+    p = 1.0 - a / 2.0
+    q = norm_quantile_upper(1.0 - p)
+    upper = mu + q * np.sqrt(mu * nolan / m)
+    import math
+    with np.errstate(divide="ignore", invalid="ignore"):
+        base = _libm(math.log10, mu * nolan)
+        J = (var / nolan) > upper
+        metric = np.zeros(G)
+        metric[J] = _libm(math.log10, var[J]) - base[J]
+    return metric
+
+
+def top_features(metric, nfeatures):
+    """partialsortperm(metric, 1:nfeatures, rev=true), ties to the lower
+    gene (0-based)."""
+    order = np.lexsort((np.arange(len(metric)), -np.asarray(metric)))
+    return order[:nfeatures]
 
 
 def loess_fit(x, y, span):

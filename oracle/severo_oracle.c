@@ -25,6 +25,8 @@
  *                       crafted duplicate-point ties.
  *   so_snn_jaccard      Python set intersections on small graphs.
  *   so_count_mean_std   numpy mean / var(ddof=1) of the densified columns.
+ *   so_rel_mean_var     numpy mean / var(ddof=1) of the densified relative
+ *                       counts, including all-zero genes.
  *   so_loess_fit        exact reproduction of quadratics, and a numpy
  *                       weighted least-squares fit (lstsq) per point.
  *   so_standardized_var numpy closed form of the clipped standardized values.
@@ -849,6 +851,33 @@ SO_EXPORT void so_count_mean_std(int64_t m, int64_t n, const int64_t *colptr, co
         mu_out[j] = mu;
         sd_out[j] = sqrt(s / (double)(m - 1));
     }
+}
+
+/* mean_var(Float64, row_norm(A, 1.0)) (src/scaling.jl:18-34,132-142 on
+ * src/normalize.jl:24-36): per column, Welford in FP64 over the relative
+ * counts v = x / s_row (s_row the Int row total; 1.0 * x is exact), count
+ * starting at m - nnz; var = s / (m - 1).  The statistics of the
+ * :dispersion, :meanvarplot and :saunders selections
+ * (src/variablefeatures.jl:60-91). */
+SO_EXPORT void so_rel_mean_var(int64_t m, int64_t n, const int64_t *colptr, const int32_t *row,
+                               const int32_t *x, double *mu_out, double *var_out)
+{
+    int64_t *tot = (int64_t *)calloc((size_t)(m > 0 ? m : 1), sizeof(int64_t));
+    for (int64_t p = 0; p < colptr[n]; ++p) tot[row[p]] += x[p];
+    for (int64_t j = 0; j < n; ++j) {
+        int64_t count = m - (colptr[j + 1] - colptr[j]);
+        double mu = 0.0, s = 0.0;
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+            const double v = (double)x[p] / (double)tot[row[p]];
+            count += 1;
+            const double delta = v - mu;
+            mu = mu + delta / (double)count;
+            s = s + delta * (v - mu);
+        }
+        mu_out[j] = mu;
+        var_out[j] = s / (double)(m - 1);
+    }
+    free(tot);
 }
 
 /* Loess (Cleveland's local regression, degree 2, tricube weights) evaluated

@@ -12,6 +12,8 @@ SEV_FLOAT32, SEV_FLOAT64 = 0, 1
 SEV_INT32, SEV_INT64 = 0, 1
 SEV_LOGNORMALIZE, SEV_RELATIVECOUNTS = 0, 1
 SEV_PREC_FP32, SEV_PREC_BF16, SEV_PREC_EXACT, SEV_PREC_FP8 = 0, 1, 2, 3
+SEV_HVG_DISPERSION, SEV_HVG_MEANVARPLOT, SEV_HVG_SAUNDERS = 1, 2, 3
+SEV_BIN_WIDTH = 0
 
 _i64 = C.c_int64
 _int = C.c_int
@@ -45,6 +47,8 @@ SIGNATURES = {
     "FindNeighboursCosine64": (None, [_p, _int, _int, _int, _int, _int, _int, _int, _p, _p]),
     "sev_snn_jaccard": (_int, [_i64, _i64, _p, _dbl, _int, _pi64, _p, _p, _p]),
     "sev_find_variable_features": (_int, [_i64, _i64, _p, _p, _p, _i64, _dbl, _p, _p]),
+    "sev_find_variable_features_method": (_int, [_i64, _i64, _p, _p, _p, _int, _i64, _i64, _int, _dbl,
+                                              _p, _p]),
     "sev_pipeline_find_variable_features": (_int, [_p, _i64, _dbl, _p, _p]),
     "sev_pipeline_create": (_int, [_pp]),
     "sev_pipeline_destroy": (None, [_p]),

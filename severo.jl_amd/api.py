@@ -70,17 +70,29 @@ def normalize_cells(X, method="lognormalize", scale_factor=1.0, dtype=np.float64
 # ---------------------------------------------------------------------------
 # find_variable_features  (src/variablefeatures.jl:128-161)
 # ---------------------------------------------------------------------------
-def find_variable_features(X, nfeatures=2000, method="vst", loess_span=0.5,
-                           return_metric=False):
+def find_variable_features(X, nfeatures=2000, method="vst", loess_span=0.5, num_bins=20,
+                           binning_method="width", alpha_thresh=0.1, return_metric=False):
     """The ``nfeatures`` genes (0-based, best first) with the largest
-    clipped standardized variance after a loess fit of log10 sd on log10
-    mean (``:vst``, the reference's default method).  Ties go to the lower
-    gene index.  The loess is the exact local fit at every gene (Loess.jl
-    interpolates between kd-tree vertices)."""
+    selection metric (src/variablefeatures.jl:128-161); ties go to the lower
+    gene index.
+
+    ``:vst`` (default): clipped standardized variance after a loess fit of
+    log10 sd on log10 mean (the exact local fit at every gene; Loess.jl
+    interpolates between kd-tree vertices).  ``:dispersion``,
+    ``:meanvarplot`` (``num_bins`` bins of equal width) and ``:saunders``
+    (``alpha_thresh``, 0.1 as in the reference's keyword default) work on
+    the relative counts row_norm(counts, 1.0)."""
     if isinstance(method, str):
         method = method.lstrip(":")
-    if method != "vst":
-        raise ValueError(f"unknown selection method: {method} (only vst is provided)")
+    if isinstance(binning_method, str):
+        binning_method = binning_method.lstrip(":")
+    codes = {"dispersion": _lib.SEV_HVG_DISPERSION, "meanvarplot": _lib.SEV_HVG_MEANVARPLOT,
+             "saunders": _lib.SEV_HVG_SAUNDERS}
+    if method != "vst" and method not in codes:
+        raise ValueError(f"unknown selection method: {method}")
+    if binning_method != "width":
+        raise ValueError("only width binning is provided: frequency binning puts the smallest "
+                         "mean in bin 0, outside the reference's bin statistics")
     X, colptr, rowval = _csc_julia(X)
     if not np.issubdtype(X.dtype, np.integer):
         raise TypeError("find_variable_features expects an integer count matrix")
@@ -89,8 +101,15 @@ def find_variable_features(X, nfeatures=2000, method="vst", loess_span=0.5,
     sel = np.zeros(nfeatures, np.int64)
     metric = np.zeros(n)
     L = _lib.load()
-    check(L.sev_find_variable_features(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts), nfeatures,
-                                       float(loess_span), _ptr(sel), _ptr(metric)))
+    if method == "vst":
+        check(L.sev_find_variable_features(m, n, _ptr(colptr), _ptr(rowval), _ptr(counts),
+                                           nfeatures, float(loess_span), _ptr(sel),
+                                           _ptr(metric)))
+    else:
+        check(L.sev_find_variable_features_method(m, n, _ptr(colptr), _ptr(rowval),
+                                                  _ptr(counts), codes[method], nfeatures,
+                                                  int(num_bins), _lib.SEV_BIN_WIDTH,
+                                                  float(alpha_thresh), _ptr(sel), _ptr(metric)))
     sel -= 1
     return (sel, metric) if return_metric else sel
 

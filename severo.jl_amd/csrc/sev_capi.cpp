@@ -1182,6 +1182,33 @@ int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, cons
     });
 }
 
+int sev_find_variable_features_method(int64_t m, int64_t n, const int64_t *colptr,
+                                      const int64_t *rowval, const int32_t *nzval, int method,
+                                      int64_t nfeatures, int64_t num_bins, int binning,
+                                      double alpha_thresh, int64_t *selected, double *metric) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(m >= 2 && n >= 1 && colptr && nzval && selected, "bad arguments");
+        SEV_REQUIRE(method == SEV_HVG_DISPERSION || method == SEV_HVG_MEANVARPLOT ||
+                        method == SEV_HVG_SAUNDERS,
+                    "unknown selection method (the vst method is sev_find_variable_features)");
+        SEV_REQUIRE(binning == SEV_BIN_WIDTH,
+                    "only width binning: frequency binning puts the smallest mean in bin 0 "
+                    "(right-closed cut at the first quantile), outside the bin statistics");
+        Stream s;
+        DevBuf<int64_t> dcp;
+        DevBuf<int32_t> drow, dcnt;
+        int64_t nnz = 0;
+        csc_to_device(m, n, colptr, rowval, dcp, drow, nnz, s.st);
+        for (int64_t p = 0; p < nnz; ++p)
+            SEV_REQUIRE(nzval[p] >= 0, "counts must be non-negative");
+        upload(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+        find_variable_features_method(m, n, dcp.get(), drow.get(), dcnt.get(), method, nfeatures,
+                                      num_bins, alpha_thresh, selected, metric, s.st);
+        for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;
+    });
+}
+
 int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t k,
             int include_self, int precision, int32_t *nn_index, void *distances,
             int64_t *n_fallback) {

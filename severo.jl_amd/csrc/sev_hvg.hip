@@ -154,6 +154,19 @@ k_standardized_var(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t
     }
 }
 
+// partialsortperm(metric, 1:nfeatures, rev=true) (src/variablefeatures.jl:159);
+// ties to the lower gene index
+void select_top(const std::vector<double> &mh, int64_t nfeat, int64_t *selected) {
+    const int64_t G = (int64_t)mh.size();
+    std::vector<int64_t> idx((size_t)G);
+    std::iota(idx.begin(), idx.end(), (int64_t)0);
+    std::partial_sort(idx.begin(), idx.begin() + nfeat, idx.end(), [&](int64_t a, int64_t b) {
+        const double ma = mh[(size_t)a], mb = mh[(size_t)b];
+        return ma > mb || (ma == mb && a < b);
+    });
+    for (int64_t i = 0; i < nfeat; ++i) selected[i] = idx[(size_t)i];
+}
+
 }  // namespace
 
 void find_variable_features(int64_t m, int64_t G, const int64_t *colptr_d, const int32_t *cnt_d,
@@ -230,15 +243,130 @@ void find_variable_features(int64_t m, int64_t G, const int64_t *colptr_d, const
     std::vector<double> mh((size_t)G);
     SEV_HIP(hipMemcpyAsync(mh.data(), met.get(), sizeof(double) * G, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
-    // partialsortperm(metric, 1:nfeatures, rev=true) (:159); ties to the
-    // lower gene index
-    std::vector<int64_t> idx((size_t)G);
-    std::iota(idx.begin(), idx.end(), (int64_t)0);
-    std::partial_sort(idx.begin(), idx.begin() + nfeat, idx.end(), [&](int64_t a, int64_t b) {
-        const double ma = mh[(size_t)a], mb = mh[(size_t)b];
-        return ma > mb || (ma == mb && a < b);
-    });
-    for (int64_t i = 0; i < nfeat; ++i) selected[i] = idx[(size_t)i];
+    select_top(mh, nfeat, selected);
+    if (metric_out) std::copy(mh.begin(), mh.end(), metric_out);
+}
+
+namespace {
+
+double nan2zero(double v) { return std::isnan(v) ? 0.0 : v; }
+
+// x with P(Z > x) = tail for a standard normal Z: Newton steps on erfc from
+// above the root (the quantile of Normal() at 1 - tail)
+double norm_quantile_upper(double tail) {
+    double x = std::sqrt(-2.0 * std::log(tail));
+    for (int it = 0; it < 100; ++it) {
+        const double f = 0.5 * std::erfc(x / std::sqrt(2.0)) - tail;
+        const double dfdx = -std::exp(-0.5 * x * x) / std::sqrt(2.0 * 3.141592653589793);
+        const double step = f / dfdx;
+        x -= step;
+        if (std::fabs(step) <= 1e-16 * std::fabs(x)) break;
+    }
+    return x;
+}
+
+}  // namespace
+
+// :dispersion, :meanvarplot and :saunders (src/variablefeatures.jl:60-91,
+// :128-161) on row_norm(A, 1.0): the per-gene FP64 mean and variance of the
+// relative counts on the device, then the O(genes) selection rules:
+//   dispersion   log(var/mu), NaN -> 0 (log_VMR, src/scaling.jl:190-192);
+//   meanvarplot  log1p(mu) cut into nbins bins of equal width (breaks from
+//                min to max, widened by (max - min)/1000 at both ends,
+//                right-closed, src/utils.jl:109-148), the dispersion's
+//                z-score within its bin from the labelled Welford in gene
+//                order (src/scaling.jl:90-112), NaN -> 0;
+//   saunders     nolan = mean(1 ./ cell totals) (summed in cell order),
+//                q = quantile(Normal(), 1 - alpha/G/2), genes with
+//                var/nolan > mu + q sqrt(mu nolan / m) get
+//                log10(var) - log10(mu nolan), the others 0.
+void find_variable_features_method(int64_t m, int64_t G, const int64_t *colptr_d,
+                                   const int32_t *row_d, const int32_t *cnt_d, int method,
+                                   int64_t nfeat, int64_t nbins, double alpha, int64_t *selected,
+                                   double *metric_out, hipStream_t st) {
+    SEV_REQUIRE(m >= 2 && G >= 1, "find_variable_features: need at least two cells");
+    SEV_REQUIRE(nfeat >= 1 && nfeat <= G, "find_variable_features: nfeatures out of range");
+    SEV_REQUIRE(method >= 1 && method <= 3, "find_variable_features: unknown method");
+    SEV_REQUIRE(method != 2 || nbins >= 1, "find_variable_features: num_bins must be positive");
+    SEV_REQUIRE(method != 3 || alpha > 0.0, "find_variable_features: alpha_thresh must be positive");
+    int64_t nnz = 0;
+    SEV_HIP(hipMemcpyAsync(&nnz, colptr_d + G, sizeof nnz, hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipStreamSynchronize(st));
+    DevBuf<uint32_t> tot;
+    DevBuf<double> mu, var;
+    tot.ensure((size_t)m);
+    mu.ensure((size_t)G);
+    var.ensure((size_t)G);
+    row_totals(colptr_d, row_d, cnt_d, nnz, m, G, tot.get(), st);
+    rel_mean_var(m, G, colptr_d, row_d, cnt_d, tot.get(), mu.get(), var.get(), st);
+    std::vector<double> muh((size_t)G), varh((size_t)G), mh((size_t)G, 0.0);
+    std::vector<uint32_t> toth;
+    SEV_HIP(hipMemcpyAsync(muh.data(), mu.get(), sizeof(double) * G, hipMemcpyDeviceToHost, st));
+    SEV_HIP(hipMemcpyAsync(varh.data(), var.get(), sizeof(double) * G, hipMemcpyDeviceToHost, st));
+    if (method == 3) {
+        toth.resize((size_t)m);
+        SEV_HIP(hipMemcpyAsync(toth.data(), tot.get(), sizeof(uint32_t) * m,
+                               hipMemcpyDeviceToHost, st));
+    }
+    SEV_HIP(hipStreamSynchronize(st));
+    if (method == 1) {
+        for (int64_t g = 0; g < G; ++g)
+            mh[(size_t)g] = nan2zero(std::log(varh[(size_t)g] / muh[(size_t)g]));
+    } else if (method == 2) {
+        std::vector<double> lmu((size_t)G), disp((size_t)G);
+        double lo = 0.0, hi = 0.0;
+        for (int64_t g = 0; g < G; ++g) {
+            lmu[(size_t)g] = nan2zero(std::log1p(muh[(size_t)g]));
+            disp[(size_t)g] = nan2zero(std::log(varh[(size_t)g] / muh[(size_t)g]));
+            if (g == 0 || lmu[(size_t)g] < lo) lo = lmu[(size_t)g];
+            if (g == 0 || lmu[(size_t)g] > hi) hi = lmu[(size_t)g];
+        }
+        const double dx = hi - lo;
+        std::vector<double> br((size_t)nbins + 1);
+        for (int64_t i = 0; i <= nbins; ++i)
+            br[(size_t)i] = (double)(((long double)lo * (long double)(nbins - i) +
+                                      (long double)hi * (long double)i) /
+                                     (long double)nbins);
+        br[0] = lo - dx / 1000;
+        br[(size_t)nbins] = hi + dx / 1000;
+        std::vector<int64_t> lab((size_t)G);
+        for (int64_t g = 0; g < G; ++g) {
+            const double x = lmu[(size_t)g];
+            // searchsortedlast, then one down when x sits on that break
+            int64_t idx = std::upper_bound(br.begin(), br.end(), x) - br.begin();
+            if (idx >= 1 && br[(size_t)idx - 1] == x) idx -= 1;
+            lab[(size_t)g] = idx;
+        }
+        std::vector<double> bm((size_t)nbins + 2, 0.0), bv((size_t)nbins + 2, 0.0);
+        std::vector<int64_t> bn((size_t)nbins + 2, 0);
+        for (int64_t g = 0; g < G; ++g) {
+            const int64_t k = lab[(size_t)g];
+            const double v = disp[(size_t)g];
+            bn[(size_t)k] += 1;
+            const double delta = v - bm[(size_t)k];
+            bm[(size_t)k] = bm[(size_t)k] + delta / (double)bn[(size_t)k];
+            bv[(size_t)k] = bv[(size_t)k] + delta * (v - bm[(size_t)k]);
+        }
+        for (int64_t g = 0; g < G; ++g) {
+            const int64_t k = lab[(size_t)g];
+            const double sd = std::sqrt(bv[(size_t)k] / (double)(bn[(size_t)k] - 1));
+            mh[(size_t)g] = nan2zero((disp[(size_t)g] - bm[(size_t)k]) / sd);
+        }
+    } else {
+        double acc = 0.0;
+        for (int64_t i = 0; i < m; ++i) acc += 1.0 / (double)toth[(size_t)i];
+        const double nolan = acc / (double)m;
+        const double a = alpha / (double)G;
+        const double p = 1.0 - a / 2.0;
+        const double q = norm_quantile_upper(1.0 - p);
+        for (int64_t g = 0; g < G; ++g) {
+            const double mu_g = muh[(size_t)g], var_g = varh[(size_t)g];
+            const double upper = mu_g + q * std::sqrt(mu_g * nolan / (double)m);
+            if (var_g / nolan > upper)
+                mh[(size_t)g] = std::log10(var_g) - std::log10(mu_g * nolan);
+        }
+    }
+    select_top(mh, nfeat, selected);
     if (metric_out) std::copy(mh.begin(), mh.end(), metric_out);
 }
 

@@ -77,6 +77,11 @@ void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, con
 void count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *x, double *mu,
                     double *sd, hipStream_t st);
 
+// mean_var(Float64, row_norm(A, 1.0)) per column: FP64 Welford over x / s_row
+// (tot = the per-row totals).
+void rel_mean_var(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *row,
+                  const int32_t *x, const uint32_t *tot, double *mu, double *var, hipStream_t st);
+
 // ---- sev_hvg.hip ---------------------------------------------------------
 // find_variable_features(:vst) on device-resident counts (cells x genes CSC,
 // rows sorted within columns): selected[0..nfeat) = 0-based genes by
@@ -85,6 +90,13 @@ void count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_
 void find_variable_features(int64_t m, int64_t G, const int64_t *colptr_d, const int32_t *cnt_d,
                             int64_t nfeat, double span, int64_t *selected, double *metric_out,
                             hipStream_t st);
+// The :dispersion (method 1), :meanvarplot (2, width bins) and :saunders (3)
+// selections on device-resident counts (row_d: 0-based rows, sorted within
+// columns); same outputs.
+void find_variable_features_method(int64_t m, int64_t G, const int64_t *colptr_d,
+                                   const int32_t *row_d, const int32_t *cnt_d, int method,
+                                   int64_t nfeat, int64_t nbins, double alpha, int64_t *selected,
+                                   double *metric_out, hipStream_t st);
 
 // Batches of S samples stacked by rows of one cells x genes CSC.
 // soff[s*G + g] (s = 0..S): first entry of column g in sample >= s.

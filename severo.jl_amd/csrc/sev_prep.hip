@@ -383,6 +383,37 @@ k_count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *
     }
 }
 
+// mean_var(Float64, row_norm(A, 1.0)) (src/scaling.jl:18-34,132-142 on
+// src/normalize.jl:24-36): per column, FP64 Welford over the relative counts
+// x / s_row in row order (1.0 * x is exact; IEEE division), the count
+// starting at m - nnz; var = s / (m - 1).  One wavefront per column.
+__global__ void __launch_bounds__(kBlock)
+k_rel_mean_var(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *row,
+               const int32_t *x, const uint32_t *tot, double *muv, double *varv) {
+    const int64_t col = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+    if (col >= ncol) return;
+    const int lane = lane_id();
+    const int64_t b = colptr[col], e = colptr[col + 1];
+    int64_t count = m - (e - b);
+    double mu = 0.0, s = 0.0;
+    for (int64_t base = b; base < e; base += kWave) {
+        const int64_t p = base + lane;
+        const double vl = p < e ? (double)x[p] / (double)tot[row[p]] : 0.0;
+        const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
+        for (int t = 0; t < nt; ++t) {
+            const double v = lane_value(vl, t);
+            const double delta = v - mu;
+            mu = mu + delta / (double)(count + t + 1);
+            s = s + delta * (v - mu);
+        }
+        count += nt;
+    }
+    if (lane == 0) {
+        muv[col] = mu;
+        varv[col] = s / (double)(m - 1);
+    }
+}
+
 // b = min(x / sigma, clip) with only the upper clip (src/scaling.jl:210-213).
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -557,6 +588,15 @@ void count_mean_std(int64_t m, int64_t ncol, const int64_t *colptr, const int32_
     ProfScope ps("count_stats", st);
     hipLaunchKernelGGL(k_count_mean_std, dim3((unsigned)ceil_div(ncol * kWave, kBlock)),
                        dim3(kBlock), 0, st, m, ncol, colptr, x, mu, sd);
+    SEV_LAUNCH_CHECK();
+}
+
+void rel_mean_var(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *row,
+                  const int32_t *x, const uint32_t *tot, double *mu, double *var, hipStream_t st) {
+    if (ncol == 0) return;
+    ProfScope ps("rel_stats", st);
+    hipLaunchKernelGGL(k_rel_mean_var, dim3((unsigned)ceil_div(ncol * kWave, kBlock)),
+                       dim3(kBlock), 0, st, m, ncol, colptr, row, x, tot, mu, var);
     SEV_LAUNCH_CHECK();
 }
 

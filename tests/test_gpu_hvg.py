@@ -54,3 +54,43 @@ def test_vst_constant_genes_rank_last(sev):
     sel, metric = sev.find_variable_features(sp.csc_matrix(X), 40, return_metric=True)
     assert np.all(metric[:8] == 0)
     assert set(sel[-8:].tolist()) == set(range(8))
+
+
+def _oracle_method(orc, X, method, nbins=20, alpha=0.1):
+    m = X.shape[0]
+    mu, var = orc.rel_mean_var(m, X.indptr, X.indices, X.data)
+    if method == "dispersion":
+        return orc.hvg_dispersion(mu, var)
+    if method == "meanvarplot":
+        return orc.hvg_meanvarplot(mu, var, nbins)
+    return orc.hvg_saunders(m, X.indptr, X.indices, X.data, mu, var, alpha=alpha)
+
+
+@pytest.mark.parametrize("method", ["dispersion", "meanvarplot", "saunders"])
+@pytest.mark.parametrize("cells,genes,density,nfeat",
+                         [(5000, 3000, 0.05, 300), (800, 1200, 0.1, 1200)])
+def test_methods_match_oracle(sev, orc, method, cells, genes, density, nfeat):
+    """:dispersion, :meanvarplot (20 width bins) and :saunders (alpha 0.1):
+    the per-gene metric bit for bit (the device Welford of x / s runs in the
+    oracle's order) and the selected genes in order."""
+    _, X = _counts(sev, orc, cells, genes, density, seed=genes + 7)
+    X = sp.csc_matrix(X)
+    X.sort_indices()
+    sel, metric = sev.find_variable_features(X, nfeat, method=method, return_metric=True)
+    rmetric = _oracle_method(orc, X, method)
+    assert np.array_equal(metric, rmetric)
+    assert np.array_equal(sel, orc.top_features(rmetric, nfeat))
+
+
+def test_meanvarplot_bins_and_errors(sev, orc):
+    """Other bin counts; frequency binning and unknown methods are refused."""
+    _, X = _counts(sev, orc, 2000, 800, 0.08, seed=31)
+    X = sp.csc_matrix(X)
+    for nb in (5, 37):
+        _, metric = sev.find_variable_features(X, 100, method="meanvarplot", num_bins=nb,
+                                               return_metric=True)
+        assert np.array_equal(metric, _oracle_method(orc, X, "meanvarplot", nbins=nb))
+    with pytest.raises(ValueError):
+        sev.find_variable_features(X, 100, method="meanvarplot", binning_method="frequency")
+    with pytest.raises(ValueError):
+        sev.find_variable_features(X, 100, method="bogus")

@@ -481,3 +481,83 @@ def test_vst_selection_order(orc):
     assert np.array_equal(sel, np.lexsort((np.arange(80), -metric))[:20])
     assert metric[5] == 0.0 and metric[7] == 0.0
     assert np.all(np.diff(metric[sel]) <= 0)
+
+
+# ---------------------------------------------------------------------------
+# find_variable_features(:dispersion / :meanvarplot / :saunders) pieces
+# (src/variablefeatures.jl:60-91, src/scaling.jl:18-34,90-112,190-192)
+# ---------------------------------------------------------------------------
+def _rel(X):
+    tot = X.sum(axis=1).astype(np.float64)
+    return X / np.where(tot == 0, 1.0, tot)[:, None]
+
+
+def test_rel_mean_var_vs_numpy(orc):
+    S, X = _count_csc(11, m=500, n=70)
+    mu, var = orc.rel_mean_var(X.shape[0], S.indptr, S.indices, S.data)
+    R = _rel(X)
+    assert np.allclose(mu, R.mean(axis=0), rtol=1e-13, atol=1e-18)
+    assert np.allclose(var, R.var(axis=0, ddof=1), rtol=1e-11, atol=1e-20)
+    assert mu[5] == 0.0 and var[5] == 0.0
+
+
+def test_dispersion_closed_form(orc):
+    S, X = _count_csc(12, m=500, n=70)
+    mu, var = orc.rel_mean_var(X.shape[0], S.indptr, S.indices, S.data)
+    d = orc.hvg_dispersion(mu, var)
+    R = _rel(X)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.log(R.var(axis=0, ddof=1) / R.mean(axis=0))
+    ok = np.isfinite(ref)
+    assert np.allclose(d[ok], ref[ok], rtol=1e-11)
+    assert d[5] == 0.0  # 0/0 -> NaN -> 0
+
+
+def test_width_bins_right_closed(orc):
+    v = np.array([0.0, 1.0, 2.0, 3.0, 4.0, 2.0])
+    br = orc.width_breaks(v, 4)
+    assert br[0] < 0.0 and br[-1] > 4.0 and np.allclose(br[1:-1], [1.0, 2.0, 3.0])
+    lab = orc.cut_labels(v, br)
+    # (b0, 1], (1, 2], (2, 3], (3, b4]: a value on an interior break goes down
+    assert lab.tolist() == [1, 1, 2, 3, 4, 2]
+    # planted mutation: left-closed bins differ
+    assert np.searchsorted(br, v, side="right").tolist() != lab.tolist()
+
+
+def test_meanvarplot_vs_numpy_bins(orc):
+    S, X = _count_csc(13, m=800, n=150)
+    mu, var = orc.rel_mean_var(X.shape[0], S.indptr, S.indices, S.data)
+    z = orc.hvg_meanvarplot(mu, var, 20)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lmu = np.nan_to_num(np.log1p(mu), nan=0.0)
+        disp = np.log(var / mu)
+        disp = np.where(np.isnan(disp), 0.0, disp)
+    lab = orc.cut_labels(lmu, orc.width_breaks(lmu, 20))
+    ref = np.zeros_like(z)
+    for k in np.unique(lab):
+        sel = lab == k
+        if sel.sum() < 2:
+            continue  # sd of one value is NaN: z -> 0
+        ref[sel] = (disp[sel] - disp[sel].mean()) / disp[sel].std(ddof=1)
+    ref = np.where(np.isnan(ref), 0.0, ref)
+    fin = np.isfinite(disp)
+    assert np.allclose(z[fin], ref[fin], rtol=1e-9, atol=1e-12)
+
+
+def test_saunders_closed_form(orc):
+    import scipy.stats as st
+    S, X = _count_csc(14, m=700, n=90)
+    X[X.sum(axis=1) == 0, 0] = 1  # every cell has a count
+    S = sp.csc_matrix(X)
+    m = X.shape[0]
+    mu, var = orc.rel_mean_var(m, S.indptr, S.indices, S.data)
+    met = orc.hvg_saunders(m, S.indptr, S.indices, S.data, mu, var, alpha=0.1)
+    nolan = np.mean(1.0 / X.sum(axis=1))
+    q = st.norm.ppf(1 - 0.1 / 90 / 2)
+    assert abs(orc.norm_quantile_upper(0.1 / 90 / 2) - q) <= 1e-14 * q
+    with np.errstate(divide="ignore", invalid="ignore"):
+        upper = mu + q * np.sqrt(mu * nolan / m)
+        J = var / nolan > upper
+        ref = np.where(J, np.log10(var) - np.log10(mu * nolan), 0.0)
+    assert np.allclose(met, ref, rtol=1e-11, atol=0)
+    assert (met > 0).sum() > 0 and met[5] == 0.0
