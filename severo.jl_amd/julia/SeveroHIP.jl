@@ -55,19 +55,34 @@ normalize_cells(X::NamedArray; kw...) = NamedArray(normalize_cells(X.array; kw..
                                                    X.dimnames)
 
 # ------------------------------------------------------ variable features
-# src/variablefeatures.jl:128-161 (method :vst; the loess is the exact local
-# fit at every gene)
+# src/variablefeatures.jl:128-161: method :vst (the loess is the exact local
+# fit at every gene), :dispersion, :meanvarplot (num_bins, width binning) or
+# :saunders (alpha_thresh, 0.1 as the reference's keyword default)
+const SEV_HVG = Dict(:dispersion => Cint(1), :meanvarplot => Cint(2), :saunders => Cint(3))
+
 function find_variable_features(counts::NamedArray, nfeatures::Integer=2000; method=:vst,
-                                loess_span::Real=0.5)
-    Symbol(method) == :vst || error("unknown selection method: $method")
+                                loess_span::Real=0.5, num_bins::Integer=20,
+                                binning_method=:width, alpha_thresh::Real=0.1)
+    method = Symbol(method)
     X = counts.array
     m, n = size(X)
     nz = convert(Vector{Int32}, nonzeros(X))
     selected = Vector{Int64}(undef, nfeatures)
-    check(ccall((:sev_find_variable_features, libsev), Cint,
-                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}, Int64, Cdouble, Ptr{Int64},
-                 Ptr{Cdouble}),
-                m, n, X.colptr, X.rowval, nz, nfeatures, loess_span, selected, C_NULL))
+    if method == :vst
+        check(ccall((:sev_find_variable_features, libsev), Cint,
+                    (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}, Int64, Cdouble, Ptr{Int64},
+                     Ptr{Cdouble}),
+                    m, n, X.colptr, X.rowval, nz, nfeatures, loess_span, selected, C_NULL))
+    elseif haskey(SEV_HVG, method)
+        Symbol(binning_method) == :width || error("only width binning is provided")
+        check(ccall((:sev_find_variable_features_method, libsev), Cint,
+                    (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}, Cint, Int64, Int64, Cint,
+                     Cdouble, Ptr{Int64}, Ptr{Cdouble}),
+                    m, n, X.colptr, X.rowval, nz, SEV_HVG[method], nfeatures, num_bins, Cint(0),
+                    alpha_thresh, selected, C_NULL))
+    else
+        error("unknown selection method: $method")
+    end
     NamedArray(selected, (names(counts, 2)[selected],), (dimnames(counts, 2),))
 end
 
