@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/severo_hip.h"
@@ -70,6 +72,88 @@ void ensure_device() {
     ok = true;
 }
 
+// Host-to-device transfer of large arrays through two reusable pinned
+// buffers: host threads fill one (converting as they copy) while the DMA of
+// the other runs.  The buffers are allocated at the first large transfer and
+// kept until the process ends; transfers are serialised by a mutex.
+struct PinnedStage {
+    static constexpr size_t kBytes = (size_t)256 << 20;  // per buffer
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    std::mutex mu;
+    void ensure() {
+        for (int u = 0; u < 2; ++u) {
+            if (!buf[u]) SEV_HIP(hipHostMalloc(&buf[u], kBytes, hipHostMallocDefault));
+            if (!done[u]) SEV_HIP(hipEventCreateWithFlags(&done[u], hipEventDisableTiming));
+        }
+    }
+};
+
+PinnedStage &pinned_stage() {
+    static PinnedStage *ps = new PinnedStage();  // lives until the process ends
+    return *ps;
+}
+
+constexpr size_t kStageMinBytes = (size_t)64 << 20;  // smaller arrays go directly
+
+// f(first, count, thread) over [0, n) split across host threads
+template <class F>
+void host_parallel(int64_t n, F &&f) {
+    const int64_t nt = std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)std::thread::hardware_concurrency(), 16,
+                              n / ((int64_t)1 << 20) + 1}));
+    std::vector<std::thread> th;
+    const int64_t per = (n + nt - 1) / nt;
+    for (int64_t t = 1; t < nt; ++t) {
+        const int64_t a = t * per, b = std::min<int64_t>(n, a + per);
+        if (a < b) th.emplace_back([&f, a, b, t] { f(a, b - a, (int)t); });
+    }
+    f(0, std::min<int64_t>(n, per), 0);
+    for (auto &x : th) x.join();
+}
+
+// dst[0..n) = convert(src[0..n)) through the pinned buffers; convert(in,
+// out, count, thread) may flag errors through its own state.
+template <class S, class D, class C>
+void staged_upload(D *dst, const S *src, int64_t n, C &&convert, hipStream_t st) {
+    PinnedStage &ps = pinned_stage();
+    std::lock_guard<std::mutex> lock(ps.mu);
+    ps.ensure();
+    const int64_t per = (int64_t)(PinnedStage::kBytes / sizeof(D));
+    int u = 0;
+    for (int64_t p0 = 0; p0 < n; p0 += per, u ^= 1) {
+        const int64_t len = std::min<int64_t>(per, n - p0);
+        if (ps.pending[u]) SEV_HIP(hipEventSynchronize(ps.done[u]));
+        D *h = (D *)ps.buf[u];
+        host_parallel(len, [&](int64_t a, int64_t c, int t) { convert(src + p0 + a, h + a, c, t); });
+        SEV_HIP(hipMemcpyAsync(dst + p0, h, sizeof(D) * (size_t)len, hipMemcpyHostToDevice, st));
+        SEV_HIP(hipEventRecord(ps.done[u], st));
+        ps.pending[u] = true;
+    }
+    for (int v = 0; v < 2; ++v)
+        if (ps.pending[v]) {
+            SEV_HIP(hipEventSynchronize(ps.done[v]));
+            ps.pending[v] = false;
+        }
+}
+
+// Int32 counts to the device (staged when large)
+void upload_counts(DevBuf<int32_t> &b, const int32_t *h, size_t n, hipStream_t st) {
+    b.ensure(n);
+    if (n == 0) return;
+    if (n * sizeof(int32_t) < kStageMinBytes) {
+        SEV_HIP(hipMemcpyAsync(b.get(), h, sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+        return;
+    }
+    staged_upload<int32_t, int32_t>(
+        b.get(), h, (int64_t)n,
+        [](const int32_t *in, int32_t *out, int64_t c, int) {
+            std::memcpy(out, in, sizeof(int32_t) * (size_t)c);
+        },
+        st);
+}
+
 // Julia 1-based Int64 CSC structure -> device 0-based int64 colptr and int32
 // rows (validated).
 void csc_to_device(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
@@ -89,9 +173,35 @@ void csc_to_device(int64_t m, int64_t n, const int64_t *colptr, const int64_t *r
     drow.ensure((size_t)nnz);
     SEV_HIP(hipMemcpyAsync(dcp.get(), cp.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice,
                            st));
-    // the 1-based Int64 rows go up in chunks through two device staging
-    // buffers and are converted (and range-checked) on the device
-    if (nnz) {
+    if (nnz && (size_t)nnz * sizeof(int64_t) >= kStageMinBytes) {
+        // large: converted to 0-based int32 (and range-checked) by host
+        // threads into the pinned buffers, half the bytes over the link
+        std::vector<char> badt(16, 0);
+        staged_upload<int64_t, int32_t>(
+            drow.get(), rowval, nnz,
+            [&](const int64_t *in, int32_t *out, int64_t c, int t) {
+                bool bad = false;
+                for (int64_t i = 0; i < c; ++i) {
+                    const int64_t r = in[i] - 1;
+                    bad |= (r < 0) | (r >= m);
+                    out[i] = (int32_t)r;
+                }
+                if (bad) badt[(size_t)t] = 1;
+            },
+            st);
+        SEV_REQUIRE(std::find(badt.begin(), badt.end(), 1) == badt.end(),
+                    "row index out of range");
+        DevBuf<int32_t> bad;
+        bad.ensure(1);
+        SEV_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int32_t), st));
+        check_sorted(n, dcp.get(), drow.get(), bad.get(), st);
+        int32_t bad_h = 0;
+        SEV_HIP(hipMemcpyAsync(&bad_h, bad.get(), sizeof bad_h, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        SEV_REQUIRE(bad_h == 0, "rows of a column not strictly increasing");
+    } else if (nnz) {
+        // the 1-based Int64 rows go up in chunks through two device staging
+        // buffers and are converted (and range-checked) on the device
         const int64_t chunk = std::min<int64_t>(nnz, (int64_t)1 << 25);  // 256 MB of Int64
         DevBuf<int64_t> stage[2];
         DevBuf<int32_t> bad;
@@ -334,7 +444,7 @@ int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_
         ProfInstall pi_(&p->prof);
         hipStream_t st = p->st();
         csc_to_device(m, n, colptr, rowval, p->cp, p->row, p->nnz, st);
-        upload(p->cnt, nzval, (size_t)p->nnz, st);
+        upload_counts(p->cnt, nzval, (size_t)p->nnz, st);
         SEV_HIP(hipStreamSynchronize(st));
         p->m = m;
         p->g = n;
@@ -971,7 +1081,7 @@ int sev_normalize_cells(int64_t m, int64_t n, const int64_t *colptr, const int64
                         "counts must be non-negative and fit in Int32");
             c[(size_t)i] = (int32_t)v;
         }
-        upload(cnt, c.data(), (size_t)nnz, s.st);
+        upload_counts(cnt, c.data(), (size_t)nnz, s.st);
         tot.ensure((size_t)m);
         row_totals(cp.get(), row.get(), cnt.get(), nnz, m, n, tot.get(), s.st);
         const int dolog = method == SEV_LOGNORMALIZE;
@@ -1175,7 +1285,7 @@ int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, cons
         DevBuf<int64_t> dcp;
         DevBuf<int32_t> dcnt;
         upload(dcp, cp.data(), cp.size(), s.st);
-        upload(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+        upload_counts(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
         find_variable_features(m, n, dcp.get(), dcnt.get(), nfeatures, loess_span, selected,
                                metric, s.st);
         for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;
@@ -1202,7 +1312,7 @@ int sev_find_variable_features_method(int64_t m, int64_t n, const int64_t *colpt
         csc_to_device(m, n, colptr, rowval, dcp, drow, nnz, s.st);
         for (int64_t p = 0; p < nnz; ++p)
             SEV_REQUIRE(nzval[p] >= 0, "counts must be non-negative");
-        upload(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+        upload_counts(dcnt, nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
         find_variable_features_method(m, n, dcp.get(), drow.get(), dcnt.get(), method, nfeatures,
                                       num_bins, alpha_thresh, selected, metric, s.st);
         for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;

@@ -145,3 +145,33 @@ def test_scale_features_vs_oracle(sev, orc, dt):
     S = sev.scale_features(X, scale_max=4.0)
     out, mu, _ = orc.scale(m, X.indptr, X.data, dt(4.0))
     assert np.array_equal(S.A.data, out) and np.array_equal(S.mu, mu)
+
+
+def test_large_load_roundtrip_and_checks(sev):
+    """Counts above the pinned-staging threshold (rows 160 MB as Int64,
+    values 80 MB) come back unchanged; a row out of range or a column out
+    of order anywhere in them is refused."""
+    rng = np.random.default_rng(17)
+    m, n, per = 400_000, 2000, 10_000  # 20M stored counts
+    rows = np.sort(rng.choice(m, size=(n, per), replace=True), axis=1)
+    # strictly increasing rows per column: drop repeats column by column
+    cols = [np.unique(r) for r in rows]
+    colptr = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64) + 1
+    rowval = (np.concatenate(cols).astype(np.int64) + 1)
+    vals = rng.integers(1, 50, len(rowval)).astype(np.int32)
+    p = sev.Pipeline()
+    p.load_counts_julia(m, n, colptr, rowval, vals)
+    mm, nn, cp2, rv2, v2 = p.get_counts_julia()
+    assert (mm, nn) == (m, n)
+    assert np.array_equal(cp2, colptr) and np.array_equal(rv2, rowval)
+    assert np.array_equal(v2, vals)
+    bad = rowval.copy()
+    bad[len(bad) - 3] = 6 + (1 << 32)  # row 5 + 2^32: would wrap to a valid int32 row
+    with pytest.raises(sev.SeveroError, match="out of range"):
+        p.load_counts_julia(m, n, colptr, bad, vals)
+    uns = rowval.copy()
+    j = colptr[1500] - 1
+    uns[j], uns[j + 1] = uns[j + 1], uns[j]
+    with pytest.raises(sev.SeveroError, match="strictly increasing"):
+        p.load_counts_julia(m, n, colptr, uns, vals)
+    p.close()
