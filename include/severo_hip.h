@@ -296,6 +296,11 @@ int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const floa
  * (indices stay those of the stacked rows). */
 int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
                             int precision, int64_t *n_fallback);
+/* The same with metric=CosineDist() (src/neighbours.jl:159-204): the search
+ * runs on the embedding's rows scaled to unit length and the distances are
+ * D / 2 of the unit rows, as in FindNeighboursCosine32. */
+int sev_pipeline_neighbours_cosine(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                                   int precision, int64_t *n_fallback);
 int sev_pipeline_get_neighbours(sev_pipeline *p, int32_t *nn_index, float *distances);
 /* FP8 form only: rows left without a certificate after the FP8, BF16 and
  * FP32 candidate passes of the last search (passes[3]). */

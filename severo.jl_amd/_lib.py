@@ -70,6 +70,7 @@ SIGNATURES = {
     "sev_pipeline_get_pca": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_set_embedding": (_int, [_p, _i64, _i64, _p, _i64]),
     "sev_pipeline_neighbours": (_int, [_p, _i64, _int, _i64, _int, _pi64]),
+    "sev_pipeline_neighbours_cosine": (_int, [_p, _i64, _int, _i64, _int, _pi64]),
     "sev_pipeline_get_neighbours": (_int, [_p, _p, _p]),
     "sev_pipeline_knn_passes": (_int, [_p, _p]),
     "sev_pipeline_snn": (_int, [_p, _dbl, _int, _pi64]),

@@ -862,8 +862,11 @@ int sev_pipeline_set_embedding(sev_pipeline *p, int64_t m, int64_t d, const floa
     });
 }
 
-int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
-                            int precision, int64_t *n_fallback) {
+// The pipeline's kNN search on its embedding; cosine: on a copy of the
+// embedding's first d columns scaled to unit rows, then D / 2 of the found
+// pairs (the CosineDist form of the kNN entry points).
+static int pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                               int precision, int64_t *n_fallback, bool cosine) {
     return guarded([&] {
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_embedding, "no embedding");
@@ -878,6 +881,15 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
         hipStream_t st = p->st();
         p->nn.ensure((size_t)(m * k));
         p->dist.ensure((size_t)(m * k));
+        DevBuf<float> zc;
+        const float *Zs = p->Z.get();
+        if (cosine) {
+            zc.ensure((size_t)(m * d));
+            SEV_HIP(hipMemcpyAsync(zc.get(), p->Z.get(), sizeof(float) * (size_t)(m * d),
+                                   hipMemcpyDeviceToDevice, st));
+            knn_unit_rows<float>(zc.get(), m, d, st);
+            Zs = zc.get();
+        }
         if (p->batched) {
             for (int64_t s = 0; s < p->bl.S; ++s)
                 SEV_REQUIRE(p->bl.row0[(size_t)s + 1] - p->bl.row0[(size_t)s] >=
@@ -893,30 +905,41 @@ int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_
                 }
                 DevBuf<int64_t> qd;
                 upload(qd, q.data(), q.size(), st);
-                knn_exact_rows_batch(p->Z.get(), m, d, m, k, include_self, qd.get(),
+                knn_exact_rows_batch(Zs, m, d, m, k, include_self, qd.get(),
                                      (int64_t)q.size(), p->bl.row0_d.get(),
                                      p->bl.row_sample_d.get(), p->nn.get(), p->dist.get(), st);
                 SEV_HIP(hipStreamSynchronize(st));
                 if (n_fallback) *n_fallback = m;
             } else {
-                knn_mfma_batch(p->Z.get(), m, d, m, k, include_self, precision == SEV_PREC_BF16,
+                knn_mfma_batch(Zs, m, d, m, k, include_self, precision == SEV_PREC_BF16,
                                p->bl, p->kw, p->nn.get(), p->dist.get(), st, n_fallback);
             }
         } else if (precision == SEV_PREC_EXACT) {
-            knn_exact<float>(p->Z.get(), m, d, m, k, include_self, nullptr, m, p->nn.get(),
+            knn_exact<float>(Zs, m, d, m, k, include_self, nullptr, m, p->nn.get(),
                              p->dist.get(), st);
             if (n_fallback) *n_fallback = m;
         } else if (precision == SEV_PREC_FP8) {
-            knn_mfma_fp8(p->Z.get(), m, d, m, k, include_self, p->kw, p->nn.get(), p->dist.get(),
+            knn_mfma_fp8(Zs, m, d, m, k, include_self, p->kw, p->nn.get(), p->dist.get(),
                          st, n_fallback, p->knn_passes);
         } else {
-            knn_mfma(p->Z.get(), m, d, m, k, include_self, precision == SEV_PREC_BF16, p->kw,
+            knn_mfma(Zs, m, d, m, k, include_self, precision == SEV_PREC_BF16, p->kw,
                      p->nn.get(), p->dist.get(), st, n_fallback);
         }
+        if (cosine) knn_half_pair_dist<float>(Zs, m, d, p->nn.get(), k, p->dist.get(), st);
         p->k = k;
         p->have_nn = true;
         p->have_snn = false;
     });
+}
+
+int sev_pipeline_neighbours(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                            int precision, int64_t *n_fallback) {
+    return pipeline_neighbours(p, k, include_self, ndims, precision, n_fallback, false);
+}
+
+int sev_pipeline_neighbours_cosine(sev_pipeline *p, int64_t k, int include_self, int64_t ndims,
+                                   int precision, int64_t *n_fallback) {
+    return pipeline_neighbours(p, k, include_self, ndims, precision, n_fallback, true);
 }
 
 int sev_pipeline_find_variable_features(sev_pipeline *p, int64_t nfeatures, double loess_span,

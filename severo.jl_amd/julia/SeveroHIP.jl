@@ -150,7 +150,13 @@ embedding(X, n::Integer=50; method=:pca, kw...) =
 
 # ------------------------------------------------------------- neighbours
 # exact kNN behind ann! (src/neighbours.jl:19-72); indices 1-based, n x k;
-# metric :euclidean, or :cosine (unit rows, distance D / 2 of the unit rows)
+# metric :euclidean or Euclidean(), :cosine or CosineDist() (unit rows,
+# distance D / 2 of the unit rows)
+_metric(m::Symbol) = m
+_metric(m::AbstractString) = Symbol(m)
+_metric(::Severo.Euclidean) = :euclidean
+_metric(::Severo.CosineDist) = :cosine
+
 function ann(X::StridedMatrix{T}, k::Integer; include_self::Bool=true,
              precision=:fp32, metric=:euclidean) where {T<:Union{Float32,Float64}}
     n, d = size(X)
@@ -158,7 +164,7 @@ function ann(X::StridedMatrix{T}, k::Integer; include_self::Bool=true,
     idx = Matrix{Int32}(undef, n, k)
     dist = Matrix{T}(undef, n, k)
     nfb = Ref{Int64}(0)
-    m = Symbol(metric)
+    m = _metric(metric)
     if m == :euclidean
         check(ccall((:sev_knn, libsev), Cint,
                     (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Int64, Cint, Cint, Ptr{Int32},

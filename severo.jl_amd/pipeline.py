@@ -264,10 +264,15 @@ class Pipeline:
         self.npcs = Z.shape[1]
         self._m = Z.shape[0]
 
-    def neighbours(self, k=20, include_self=True, ndims=0, precision="fp32"):
+    def neighbours(self, k=20, include_self=True, ndims=0, precision="fp32", metric="euclidean"):
+        """Exact kNN of the embedding (metric "euclidean" or "cosine")."""
+        fns = {"euclidean": self._L.sev_pipeline_neighbours,
+               "cosine": self._L.sev_pipeline_neighbours_cosine}
+        if metric not in fns:
+            raise ValueError(f"unknown metric: {metric}")
         nfb = C.c_int64()
-        check(self._L.sev_pipeline_neighbours(self._h, k, int(include_self), ndims,
-                                              PRECISION[precision], C.byref(nfb)))
+        check(fns[metric](self._h, k, int(include_self), ndims, PRECISION[precision],
+                          C.byref(nfb)))
         self.k = k
         return nfb.value
 

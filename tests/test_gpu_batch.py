@@ -132,3 +132,20 @@ def test_batch_knn_snn_per_sample(batch, sev, orc, precision, include_self):
     assert np.array_equal(Sg.indptr, Sr.indptr)
     assert np.array_equal(Sg.indices, Sr.indices)
     assert np.array_equal(Sg.data, Sr.data)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "exact"])
+def test_batch_knn_cosine_per_sample(batch, orc, precision):
+    """Cosine neighbours of a batch: each sample's lists are the oracle's
+    cosine search on that sample alone."""
+    p = batch["p"]
+    Z = batch["pca"][3]
+    p.set_embedding(Z)
+    M = int(sum(CELLS))
+    p.neighbours(K, include_self=True, precision=precision, metric="cosine")
+    idx, dist = p.get_neighbours(M)
+    row0 = batch["row0"]
+    for s in range(len(CELLS)):
+        ri, rd = orc.knn_cosine(np.asfortranarray(Z[row0[s]:row0[s + 1]]), K)
+        assert np.array_equal(idx[row0[s]:row0[s + 1]], ri + row0[s]), (s, precision)
+        assert np.array_equal(dist[row0[s]:row0[s + 1]], rd), (s, precision)

@@ -190,3 +190,24 @@ def test_pipeline_neighbours_forms(sev, orc, precision):
         left = p.knn_passes()
         assert left[0] >= left[1] >= left[2] == nfb
         print(f"fp8 passes: rows left after FP8 / BF16 / FP32 candidates: {left.tolist()}")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "exact", "fp8"])
+def test_pipeline_neighbours_cosine(sev, orc, precision):
+    """The device-resident pipeline's cosine form (metric=CosineDist() of
+    nearest_neighbours) equals the oracle's on the embedding's first ndims
+    columns, and its SNN is built from those lists."""
+    rng = np.random.default_rng(23)
+    Z = np.asfortranarray(rng.standard_normal((3000, 40)).astype(np.float32))
+    Z[:, :5] *= 4.0
+    p = sev.Pipeline()
+    p.set_embedding(Z)
+    p.neighbours(15, include_self=True, ndims=30, precision=precision, metric="cosine")
+    idx, dist = p.get_neighbours(3000)
+    ridx, rdist = orc.knn_cosine(np.asfortranarray(Z[:, :30]), 15)
+    assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
+    p.snn(1 / 15, np.float32)
+    S = p.get_snn(3000)
+    cp, rv, nz = orc.snn(ridx, 15, 1 / 15, np.float32)
+    assert np.array_equal(S.indptr, cp - 1) and np.array_equal(S.data, nz)
+    p.close()
