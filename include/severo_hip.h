@@ -249,6 +249,13 @@ int sev_pipeline_get_counts(sev_pipeline *p, int64_t *colptr, int64_t *rowval, i
 int sev_pipeline_find_variable_features(sev_pipeline *p, int64_t nfeatures, double loess_span,
                                         int64_t *selected, double *metric);
 
+/* find_variable_features(:dispersion | :meanvarplot | :saunders) on the
+ * loaded counts (as sev_find_variable_features_method, same codes and
+ * errors); src/variablefeatures.jl:128-161. */
+int sev_pipeline_find_variable_features_method(sev_pipeline *p, int method, int64_t nfeatures,
+                                               int64_t num_bins, int binning, double alpha_thresh,
+                                               int64_t *selected, double *metric);
+
 /* Stage 1: normalize_cells in Float32 (method SEV_LOGNORMALIZE or
  * SEV_RELATIVECOUNTS). */
 int sev_pipeline_normalize(sev_pipeline *p, int method, double scale_factor);

@@ -954,6 +954,27 @@ int sev_pipeline_find_variable_features(sev_pipeline *p, int64_t nfeatures, doub
     });
 }
 
+int sev_pipeline_find_variable_features_method(sev_pipeline *p, int method, int64_t nfeatures,
+                                               int64_t num_bins, int binning, double alpha_thresh,
+                                               int64_t *selected, double *metric) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_counts, "no counts loaded");
+        SEV_REQUIRE(p->m >= 2 && selected != nullptr, "bad arguments");
+        SEV_REQUIRE(method == SEV_HVG_DISPERSION || method == SEV_HVG_MEANVARPLOT ||
+                        method == SEV_HVG_SAUNDERS,
+                    "unknown selection method (the vst method is "
+                    "sev_pipeline_find_variable_features)");
+        SEV_REQUIRE(binning == SEV_BIN_WIDTH,
+                    "only width binning: frequency binning puts the smallest mean in bin 0 "
+                    "(right-closed cut at the first quantile), outside the bin statistics");
+        find_variable_features_method(p->m, p->g, p->cp.get(), p->row.get(), p->cnt.get(), method,
+                                      nfeatures, num_bins, alpha_thresh, selected, metric,
+                                      p->st());
+        for (int64_t i = 0; i < nfeatures; ++i) selected[i] += 1;
+    });
+}
+
 int sev_pipeline_knn_passes(sev_pipeline *p, int64_t *passes) {
     return guarded([&] {
         SEV_REQUIRE(passes != nullptr, "passes is NULL");

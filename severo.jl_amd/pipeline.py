@@ -195,14 +195,32 @@ class Pipeline:
         check(self._L.sev_pipeline_get_counts(self._h, _ptr(colptr), _ptr(rowval), _ptr(vals)))
         return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, n))
 
-    def find_variable_features(self, nfeatures=2000, loess_span=0.5, return_metric=False):
-        """find_variable_features(:vst) on the loaded counts: 0-based genes,
-        best first."""
+    def find_variable_features(self, nfeatures=2000, method="vst", loess_span=0.5, num_bins=20,
+                               binning_method="width", alpha_thresh=0.1, return_metric=False):
+        """find_variable_features on the loaded counts (src/variablefeatures.jl:
+        128-161): 0-based genes, best first; methods as the one-shot
+        find_variable_features."""
+        if isinstance(method, str):
+            method = method.lstrip(":")
+        if isinstance(binning_method, str):
+            binning_method = binning_method.lstrip(":")
+        codes = {"dispersion": _lib.SEV_HVG_DISPERSION, "meanvarplot": _lib.SEV_HVG_MEANVARPLOT,
+                 "saunders": _lib.SEV_HVG_SAUNDERS}
+        if method != "vst" and method not in codes:
+            raise ValueError(f"unknown selection method: {method}")
+        if binning_method != "width":
+            raise ValueError("only width binning is provided")
         _, n, _ = self.counts_info()
         sel = np.zeros(nfeatures, np.int64)
         metric = np.zeros(n)
-        check(self._L.sev_pipeline_find_variable_features(self._h, nfeatures, float(loess_span),
-                                                          _ptr(sel), _ptr(metric)))
+        if method == "vst":
+            check(self._L.sev_pipeline_find_variable_features(self._h, nfeatures,
+                                                              float(loess_span), _ptr(sel),
+                                                              _ptr(metric)))
+        else:
+            check(self._L.sev_pipeline_find_variable_features_method(
+                self._h, codes[method], nfeatures, int(num_bins), _lib.SEV_BIN_WIDTH,
+                float(alpha_thresh), _ptr(sel), _ptr(metric)))
         sel -= 1
         return (sel, metric) if return_metric else sel
 

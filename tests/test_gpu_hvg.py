@@ -94,3 +94,18 @@ def test_meanvarplot_bins_and_errors(sev, orc):
         sev.find_variable_features(X, 100, method="meanvarplot", binning_method="frequency")
     with pytest.raises(ValueError):
         sev.find_variable_features(X, 100, method="bogus")
+
+
+@pytest.mark.parametrize("method", ["dispersion", "meanvarplot", "saunders"])
+def test_methods_pipeline_equals_host_entry(sev, orc, method):
+    """The pipeline's selection on its resident counts is the one-shot
+    entry's on the same counts, bit for bit."""
+    t, _ = _counts(sev, orc, 3000, 1500, 0.05, seed=17)
+    p = sev.Pipeline()
+    p.synth_counts(3000, t)
+    sel_p, m_p = p.find_variable_features(150, method=method, return_metric=True)
+    sel_h, m_h = sev.find_variable_features(p.get_counts(), 150, method=method,
+                                            return_metric=True)
+    assert np.array_equal(sel_p, sel_h) and np.array_equal(m_p, m_h)
+    with pytest.raises(ValueError):
+        p.find_variable_features(10, method="bogus")
