@@ -176,6 +176,27 @@ int sev_knn_cosine(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, i
 int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,
                     int64_t *nnz, int64_t *colptr, int64_t *rowval, void *nzval);
 
+/* cluster(SNN; algorithm=:louvain, resolution, nrandomstarts, niterations,
+ * group_singletons) (src/clustering.jl:59-104, src/modularity.jl:165-380) on
+ * a square graph in 1-based CSC (rows sorted within columns; weights of
+ * dtype, finite and >= 0; self-links ignored as Network(snn) does).
+ * Louvain with nrandomstarts starts from singletons, each repeated while it
+ * gains at least 1e-4 modularity, at most niterations further times; the
+ * start of largest modularity is kept; group_singletons != 0 then moves each
+ * member of a one-node cluster as group_singletons! does.  Local moving runs
+ * in parallel sweeps on the device (seed picks the random label orders), so
+ * labels are one outcome of the reference's randomised search, not its
+ * exact output; the same seed gives the same labels.  labels[n] receive
+ * 1-based clusters in order of first occurrence; *modularity (optional) the
+ * best start's modularity before singleton grouping (NaN without edges);
+ * *nclusters (optional) the number of clusters.  Fails when group_singletons
+ * is set and every cluster has one member (the reference's findmax over no
+ * clusters). */
+int sev_cluster_louvain(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int dtype, double resolution, int64_t nrandomstarts,
+                        int64_t niterations, uint64_t seed, int group_singletons,
+                        int64_t *labels, double *modularity, int64_t *nclusters);
+
 /* find_variable_features(counts, nfeatures; method=:vst, loess_span)
  * (src/variablefeatures.jl:34-50,128-161): per-gene mean and sd of the
  * counts (mean_std(Float64, A), src/scaling.jl:119-130), a degree-2 loess of

@@ -63,6 +63,12 @@ def lib():
         L.so_loess_fit.restype = C.c_int
         L.so_loess_fit.argtypes = [_i64, _p, _p, _dbl, _p]
         L.so_standardized_var.argtypes = [_i64, _i64, _p, _p, _p, _p, _p]
+        L.so_louvain.restype = _i64
+        L.so_louvain.argtypes = [_i64, _p, _p, _p, _dbl, _i64, _i64, _dbl, _u64, _p, _p]
+        L.so_modularity.restype = _dbl
+        L.so_modularity.argtypes = [_i64, _p, _p, _p, _p, _dbl]
+        L.so_group_singletons.restype = _i64
+        L.so_group_singletons.argtypes = [_i64, _p, _p, _p, _p]
         _LIB = L
     return _LIB
 
@@ -465,3 +471,40 @@ def vst(m, colptr, counts, nfeatures=2000, loess_span=0.5):
     metric = standardized_var(m, colptr, counts, mu, esd)
     sel = np.lexsort((np.arange(len(metric)), -metric))[:nfeatures]
     return sel, metric, mu, sd, esd
+
+
+def _graph(colptr, rows, w):
+    return (np.ascontiguousarray(colptr, np.int64), np.ascontiguousarray(rows, np.int32),
+            np.ascontiguousarray(w, np.float64))
+
+
+def louvain(n, colptr, rows, w, resolution=0.8, nstarts=10, max_iter=10, min_mod=1e-4,
+            seed=0):
+    """Multi-start Louvain of the reference's cluster(SNN) in FP64 on a
+    square CSC graph (src/clustering.jl:59-77, src/modularity.jl:255-380);
+    node orders from a seeded shuffle.  Returns (0-based labels in order of
+    first occurrence, modularity)."""
+    colptr, rows, w = _graph(colptr, rows, w)
+    lab = np.empty(n, np.int64)
+    q = C.c_double()
+    lib().so_louvain(n, _ptr(colptr), _ptr(rows), _ptr(w), resolution, nstarts, max_iter,
+                     min_mod, seed, _ptr(lab), C.byref(q))
+    return lab, q.value
+
+
+def modularity(n, colptr, rows, w, labels, resolution=0.8):
+    """modularity of compact 0-based labels on Network(SNN) (self-links
+    dropped; src/modularity.jl:125-148,165-194)."""
+    colptr, rows, w = _graph(colptr, rows, w)
+    labels = np.ascontiguousarray(labels, np.int64)
+    return lib().so_modularity(n, _ptr(colptr), _ptr(rows), _ptr(w), _ptr(labels), resolution)
+
+
+def group_singletons(n, colptr, rows, w, labels):
+    """group_singletons! then relabel! (src/clustering.jl:79-98,
+    src/utils.jl:91-106) on compact 0-based labels; None when every cluster
+    is a singleton (the reference's findmax over an empty set)."""
+    colptr, rows, w = _graph(colptr, rows, w)
+    lab = np.array(labels, np.int64)
+    r = lib().so_group_singletons(n, _ptr(colptr), _ptr(rows), _ptr(w), _ptr(lab))
+    return None if r < 0 else lab

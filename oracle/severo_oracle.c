@@ -1003,3 +1003,428 @@ SO_EXPORT void so_standardized_var(int64_t m, int64_t n, const int64_t *colptr, 
         out[j] = (t + (double)(m - nz) * (z * z)) / (double)(m - 1);
     }
 }
+
+/* ---- Louvain clustering of the SNN graph (src/clustering.jl, src/modularity.jl) ----
+ *
+ * A sequential restatement of the reference's multi-start Louvain in FP64.
+ * The node visiting order comes from a Fisher-Yates shuffle driven by
+ * so_mix64 (Julia's default_rng stream cannot be reproduced here), so a run
+ * is one sample of the reference's randomised search, not its exact output.
+ */
+typedef struct {
+    int64_t n;
+    int64_t *ptr;   /* n + 1 */
+    int64_t *adj;   /* edge targets, self-links excluded */
+    double *w;
+    double *k;      /* node weights */
+    double totw, tot_self;
+} so_net;
+
+typedef struct {
+    const so_net *net;
+    double res;
+    int64_t *c;     /* cluster of each node, 0-based */
+    double *wtot;   /* length net->n, zero beyond nc */
+    int64_t nc;
+} so_clu;
+
+static void so_net_free(so_net *g)
+{
+    free(g->ptr);
+    free(g->adj);
+    free(g->w);
+    free(g->k);
+}
+
+/* Network(snn) (src/modularity.jl:165-194): column i's entries are the edges
+ * of node i; the self-link (i, i) is skipped and counted nowhere; totw is the
+ * sum of the remaining weights, in column order. */
+static void so_net_from_csc(int64_t n, const int64_t *colptr, const int32_t *row, const double *w,
+                            so_net *g)
+{
+    g->n = n;
+    g->ptr = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    const int64_t nnz = colptr[n];
+    g->adj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnz > 0 ? nnz : 1));
+    g->w = (double *)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+    g->k = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    g->totw = 0.0;
+    g->tot_self = 0.0;
+    int64_t e = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        g->ptr[i] = e;
+        double wi = 0.0;
+        for (int64_t p = colptr[i]; p < colptr[i + 1]; ++p) {
+            if (row[p] == i) continue;
+            g->totw += w[p];
+            wi += w[p];
+            g->adj[e] = row[p];
+            g->w[e] = w[p];
+            ++e;
+        }
+        g->k[i] = wi;
+    }
+    g->ptr[n] = e;
+}
+
+static void so_clu_singletons(const so_net *g, double res, so_clu *cl)
+{
+    cl->net = g;
+    cl->res = res;
+    cl->c = (int64_t *)malloc(sizeof(int64_t) * (size_t)(g->n > 0 ? g->n : 1));
+    cl->wtot = (double *)malloc(sizeof(double) * (size_t)(g->n > 0 ? g->n : 1));
+    for (int64_t i = 0; i < g->n; ++i) {
+        cl->c[i] = i;
+        cl->wtot[i] = g->k[i];
+    }
+    cl->nc = g->n;
+}
+
+/* modularity(clustering) (src/modularity.jl:125-147): (tot_self + weight of
+ * the edges inside clusters) / totw - res * sum_i (w_tot[i] / totw)^2. */
+static double so_clu_modularity(const so_clu *cl)
+{
+    const so_net *g = cl->net;
+    double in = g->tot_self;
+    for (int64_t v = 0; v < g->n; ++v)
+        for (int64_t e = g->ptr[v]; e < g->ptr[v + 1]; ++e)
+            if (cl->c[g->adj[e]] == cl->c[v]) in += g->w[e];
+    double sq = 0.0;
+    for (int64_t i = 0; i < g->n; ++i) {
+        const double x = cl->wtot[i] / g->totw;
+        sq += x * x;
+    }
+    return in / g->totw - cl->res * sq;
+}
+
+/* local_move! (src/modularity.jl:255-287) with best_local_move (:243-253):
+ * one shuffled order per call; each node leaves its cluster and joins the
+ * neighbouring cluster with the largest kin - res * w_tot * ki / totw (ties
+ * to the smaller label, myfindmax of src/utils.jl:197-212) when that beats
+ * staying; sweeps repeat until a sweep gains less than min_mod or moves
+ * nothing. */
+static double so_local_move(uint64_t *rng, so_clu *cl, double min_mod)
+{
+    const so_net *g = cl->net;
+    const int64_t n = g->n;
+    const double totw = g->totw;
+    int64_t *order = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    for (int64_t i = n - 1; i > 0; --i) {
+        const uint64_t r = so_mix64((*rng)++);
+        const int64_t j = (int64_t)(r % (uint64_t)(i + 1));
+        const int64_t t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+    }
+    double *kin = (double *)calloc((size_t)n, sizeof(double));
+    int64_t *nbr = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t nn = 0;
+    double delta_mod = min_mod, total = 0.0;
+    int stable = 0;
+    while (!stable && delta_mod >= min_mod) {
+        delta_mod = 0.0;
+        stable = 1;
+        for (int64_t oi = 0; oi < n; ++oi) {
+            const int64_t v = order[oi];
+            const double ki = g->k[v];
+            const int64_t ci = cl->c[v];
+            cl->wtot[ci] -= ki;
+            for (int64_t t = 0; t < nn; ++t) kin[nbr[t]] = 0.0;
+            nn = 0;
+            for (int64_t e = g->ptr[v]; e < g->ptr[v + 1]; ++e) {
+                const int64_t cj = cl->c[g->adj[e]];
+                if (kin[cj] == 0.0) nbr[nn++] = cj;
+                kin[cj] += g->w[e];
+            }
+            int64_t best = ci;
+            double gain = 0.0;
+            if (nn > 0) {
+                const double init = kin[ci] - cl->res * (cl->wtot[ci] * ki) / totw;
+                int64_t m = nbr[0];
+                double fm = kin[m] - cl->res * (cl->wtot[m] * ki) / totw;
+                for (int64_t t = 1; t < nn; ++t) {
+                    const int64_t x = nbr[t];
+                    const double fx = kin[x] - cl->res * (cl->wtot[x] * ki) / totw;
+                    if (fm < fx || (fm == fx && x < m)) {
+                        m = x;
+                        fm = fx;
+                    }
+                }
+                gain = 2.0 * (fm - init) / totw;
+                best = m;
+            }
+            if (gain > 0.0) {
+                cl->wtot[best] += ki;
+                cl->c[v] = best;
+                delta_mod += gain;
+                stable = 0;
+            } else {
+                cl->wtot[ci] += ki;
+                cl->c[v] = ci;
+            }
+        }
+        total += delta_mod;
+    }
+    free(order);
+    free(kin);
+    free(nbr);
+    return total;
+}
+
+/* renumber! (src/modularity.jl:289-315): labels in order of first
+ * occurrence, w_tot carried along. */
+static void so_renumber(so_clu *cl)
+{
+    const int64_t n = cl->net->n;
+    int64_t *lab = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    double *w = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    for (int64_t i = 0; i < n; ++i) lab[i] = -1;
+    int64_t id = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t c = cl->c[i];
+        if (lab[c] < 0) {
+            lab[c] = id++;
+            w[lab[c]] = cl->wtot[c];
+        }
+        cl->c[i] = lab[c];
+    }
+    memcpy(cl->wtot, w, sizeof(double) * (size_t)n);
+    cl->nc = id;
+    free(lab);
+    free(w);
+}
+
+/* reduced_network (src/modularity.jl:196-241): one node per cluster with
+ * the summed node weights; edges between clusters summed (members in node
+ * order, their edges in order) and emitted by ascending target; edges
+ * inside a cluster added to tot_self. */
+static void so_reduce(const so_clu *cl, so_net *r)
+{
+    const so_net *g = cl->net;
+    const int64_t n = g->n, nc = cl->nc;
+    int64_t *cnt = (int64_t *)calloc((size_t)nc + 1, sizeof(int64_t));
+    int64_t *mem = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t v = 0; v < n; ++v) cnt[cl->c[v] + 1]++;
+    for (int64_t i = 0; i < nc; ++i) cnt[i + 1] += cnt[i];
+    int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nc > 0 ? nc : 1));
+    for (int64_t i = 0; i < nc; ++i) cur[i] = cnt[i];
+    for (int64_t v = 0; v < n; ++v) mem[cur[cl->c[v]]++] = v;
+    double *acc = (double *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(double));
+    char *seen = (char *)calloc((size_t)(nc > 0 ? nc : 1), 1);
+    int64_t *touched = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nc > 0 ? nc : 1));
+    const int64_t cap0 = g->ptr[n] > 0 ? g->ptr[n] : 1;
+    r->n = nc;
+    r->ptr = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nc + 1));
+    r->adj = (int64_t *)malloc(sizeof(int64_t) * (size_t)cap0);
+    r->w = (double *)malloc(sizeof(double) * (size_t)cap0);
+    r->k = (double *)malloc(sizeof(double) * (size_t)(nc > 0 ? nc : 1));
+    r->totw = g->totw;
+    r->tot_self = g->tot_self;
+    int64_t ne = 0;
+    for (int64_t i = 0; i < nc; ++i) {
+        int64_t nt = 0;
+        double wi = 0.0;
+        for (int64_t t = cnt[i]; t < cnt[i + 1]; ++t) {
+            const int64_t v = mem[t];
+            wi += g->k[v];
+            for (int64_t e = g->ptr[v]; e < g->ptr[v + 1]; ++e) {
+                const int64_t j = cl->c[g->adj[e]];
+                if (j == i) {
+                    r->tot_self += g->w[e];
+                } else {
+                    if (!seen[j]) {
+                        seen[j] = 1;
+                        touched[nt++] = j;
+                    }
+                    acc[j] += g->w[e];
+                }
+            }
+        }
+        qsort(touched, (size_t)nt, sizeof(int64_t), so_cmp_i64);
+        r->ptr[i] = ne;
+        for (int64_t t = 0; t < nt; ++t) {
+            const int64_t j = touched[t];
+            if (acc[j] != 0.0) {
+                r->adj[ne] = j;
+                r->w[ne] = acc[j];
+                ++ne;
+            }
+            acc[j] = 0.0;
+            seen[j] = 0;
+        }
+        r->k[i] = wi;
+    }
+    r->ptr[nc] = ne;
+    free(cnt);
+    free(mem);
+    free(cur);
+    free(acc);
+    free(seen);
+    free(touched);
+}
+
+/* louvain! (src/modularity.jl:326-340): local moving, renumbering, and the
+ * same on the reduced network, whose clusters are merged back. */
+static double so_louvain_level(uint64_t *rng, so_clu *cl, double min_mod)
+{
+    if (cl->net->n <= 1) return 0.0;
+    double gain = so_local_move(rng, cl, min_mod);
+    so_renumber(cl);
+    if (cl->nc < cl->net->n) {
+        so_net r;
+        so_reduce(cl, &r);
+        so_clu rc;
+        so_clu_singletons(&r, cl->res, &rc);
+        gain += so_louvain_level(rng, &rc, min_mod);
+        for (int64_t v = 0; v < cl->net->n; ++v) cl->c[v] = rc.c[cl->c[v]];
+        memset(cl->wtot, 0, sizeof(double) * (size_t)cl->net->n);
+        memcpy(cl->wtot, rc.wtot, sizeof(double) * (size_t)r.n);
+        cl->nc = rc.nc;
+        free(rc.c);
+        free(rc.wtot);
+        so_net_free(&r);
+    }
+    return gain;
+}
+
+/* cluster(rng, SNN; resolution, nrandomstarts, niterations) with
+ * louvain_multi (src/clustering.jl:59-77, src/modularity.jl:344-380): per
+ * start, louvain! from singletons and again while it gains at least
+ * min_mod, at most max_iter further times; the start with the largest
+ * modularity is kept (the first on ties).  labels[n] receive 0-based
+ * clusters in order of first occurrence; returns their number. */
+SO_EXPORT int64_t so_louvain(int64_t n, const int64_t *colptr, const int32_t *row, const double *w,
+                             double res, int64_t nstarts, int64_t max_iter, double min_mod,
+                             uint64_t seed, int64_t *labels, double *q_out)
+{
+    so_net g;
+    so_net_from_csc(n, colptr, row, w, &g);
+    uint64_t rng = so_mix64(seed);
+    double qbest = 0.0;
+    int64_t ncbest = 0;
+    for (int64_t s = 0; s < nstarts; ++s) {
+        so_clu cl;
+        so_clu_singletons(&g, res, &cl);
+        double gain = so_louvain_level(&rng, &cl, min_mod);
+        for (int64_t it = 1; gain >= min_mod && it <= max_iter; ++it)
+            gain = so_louvain_level(&rng, &cl, min_mod);
+        const double q = so_clu_modularity(&cl);
+        if (s == 0 || qbest < q) {
+            qbest = q;
+            ncbest = cl.nc;
+            memcpy(labels, cl.c, sizeof(int64_t) * (size_t)n);
+        }
+        free(cl.c);
+        free(cl.wtot);
+    }
+    so_net_free(&g);
+    if (q_out) *q_out = qbest;
+    return ncbest;
+}
+
+/* modularity of given 0-based labels (compact, 0..nc-1) on Network(snn):
+ * Clustering(network, nodecluster, resolution) (src/modularity.jl:52-66)
+ * followed by modularity (:145-148). */
+SO_EXPORT double so_modularity(int64_t n, const int64_t *colptr, const int32_t *row,
+                               const double *w, const int64_t *labels, double res)
+{
+    so_net g;
+    so_net_from_csc(n, colptr, row, w, &g);
+    so_clu cl;
+    cl.net = &g;
+    cl.res = res;
+    cl.c = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    cl.wtot = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    int64_t nc = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        cl.c[v] = labels[v];
+        if (labels[v] + 1 > nc) nc = labels[v] + 1;
+    }
+    for (int64_t ci = 0; ci < nc; ++ci)
+        for (int64_t v = 0; v < n; ++v)
+            if (labels[v] == ci) cl.wtot[ci] += g.k[v];
+    cl.nc = nc;
+    const double q = so_clu_modularity(&cl);
+    free(cl.c);
+    free(cl.wtot);
+    so_net_free(&g);
+    return q;
+}
+
+/* group_singletons! (src/clustering.jl:79-98) on the SNN itself (self-links
+ * included): each member of a cluster of size 1, in node order, moves to the
+ * cluster of size > 1 (ascending label order) with the largest mean weight
+ * of its entries into that cluster, labels as updated so far.  The mean over
+ * no entries is NaN, and findmax returns the first NaN it meets, so the
+ * first such cluster without an entry from the node wins.  Labels are then
+ * renumbered by first occurrence (relabel!, src/utils.jl:91-106).  labels are
+ * 0-based and compact; returns the new number of clusters, or -1 when no
+ * cluster has more than one member (findmax over an empty set). */
+SO_EXPORT int64_t so_group_singletons(int64_t n, const int64_t *colptr, const int32_t *row,
+                                      const double *w, int64_t *labels)
+{
+    int64_t nc = 0;
+    for (int64_t v = 0; v < n; ++v)
+        if (labels[v] + 1 > nc) nc = labels[v] + 1;
+    int64_t *cnt = (int64_t *)calloc((size_t)(nc > 0 ? nc : 1), sizeof(int64_t));
+    for (int64_t v = 0; v < n; ++v) cnt[labels[v]]++;
+    int64_t nothers = 0, nsingle = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+        if (cnt[c] > 1) ++nothers;
+        else ++nsingle;
+    }
+    if (nsingle == 0) {
+        free(cnt);
+        return nc;
+    }
+    if (nothers == 0) {
+        free(cnt);
+        return -1;
+    }
+    int64_t *others = (int64_t *)malloc(sizeof(int64_t) * (size_t)nothers);
+    int64_t *single = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
+    nothers = 0;
+    for (int64_t c = 0; c < nc; ++c)
+        if (cnt[c] > 1) others[nothers++] = c;
+    int64_t ns = 0;
+    for (int64_t v = 0; v < n; ++v)
+        if (cnt[labels[v]] <= 1) single[ns++] = v;
+    for (int64_t t = 0; t < ns; ++t) {
+        const int64_t j = single[t];
+        int64_t closest = -1;
+        double best = 0.0;
+        for (int64_t o = 0; o < nothers; ++o) {
+            const int64_t cl = others[o];
+            double s = 0.0;
+            int64_t c = 0;
+            for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p)
+                if (labels[row[p]] == cl) {
+                    s += w[p];
+                    ++c;
+                }
+            if (c == 0) { /* NaN: the first one wins */
+                closest = cl;
+                break;
+            }
+            const double mean = s / (double)c;
+            if (closest < 0 || best < mean) {
+                closest = cl;
+                best = mean;
+            }
+        }
+        labels[j] = closest;
+    }
+    int64_t *lab = (int64_t *)malloc(sizeof(int64_t) * (size_t)nc);
+    for (int64_t c = 0; c < nc; ++c) lab[c] = -1;
+    int64_t id = 0;
+    for (int64_t v = 0; v < n; ++v) {
+        if (lab[labels[v]] < 0) lab[labels[v]] = id++;
+        labels[v] = lab[labels[v]];
+    }
+    free(lab);
+    free(cnt);
+    free(others);
+    free(single);
+    return id;
+}

@@ -16,6 +16,7 @@ SEV_HVG_DISPERSION, SEV_HVG_MEANVARPLOT, SEV_HVG_SAUNDERS = 1, 2, 3
 SEV_BIN_WIDTH = 0
 
 _i64 = C.c_int64
+_u64 = C.c_uint64
 _int = C.c_int
 _dbl = C.c_double
 _p = C.c_void_p
@@ -50,6 +51,8 @@ SIGNATURES = {
     "sev_find_variable_features_method": (_int, [_i64, _i64, _p, _p, _p, _int, _i64, _i64, _int, _dbl,
                                               _p, _p]),
     "sev_pipeline_find_variable_features": (_int, [_p, _i64, _dbl, _p, _p]),
+    "sev_cluster_louvain": (_int, [_i64, _p, _p, _p, _int, _dbl, _i64, _i64, _u64, _int, _p, _p,
+                                   _p]),
     "sev_pipeline_find_variable_features_method": (_int, [_p, _int, _i64, _i64, _int, _dbl, _p, _p]),
     "sev_pipeline_create": (_int, [_pp]),
     "sev_pipeline_destroy": (None, [_p]),

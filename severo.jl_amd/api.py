@@ -416,6 +416,38 @@ def shared_nearest_neighbours(X, k: int, dims=None, include_self=True, ntables=N
     return _snn_from_index(idx, k, prune, Xc.dtype)
 
 
+# ---------------------------------------------------------------------------
+# cluster  (src/clustering.jl:59-104, src/modularity.jl:165-380)
+# ---------------------------------------------------------------------------
+def cluster(SNN, algorithm="louvain", resolution=0.8, nrandomstarts=10, niterations=10,
+            group_singletons=True, seed=0, return_modularity=False):
+    """Louvain clusters of a neighbour graph (square sparse, weights >= 0;
+    self-links ignored), Severo's cluster(SNN): 1-based cluster ids in order
+    of first occurrence.  Local moving runs in parallel sweeps on the device,
+    so the labels are one outcome of the reference's randomised search;
+    ``seed`` fixes it.  ``return_modularity`` adds the best start's
+    modularity (before singleton grouping)."""
+    if isinstance(algorithm, str):
+        algorithm = algorithm.lstrip(":")
+    if algorithm != "louvain":
+        raise ValueError(f"unknown clustering algorithm: {algorithm}")
+    if SNN.shape[0] != SNN.shape[1]:
+        raise ValueError(f"matrix is not square: dimensions are {SNN.shape}")
+    X, colptr, rowval = _csc_julia(SNN)
+    dt = X.dtype if X.dtype in (np.float32, np.float64) else np.dtype(np.float64)
+    vals = np.ascontiguousarray(X.data, dt)
+    n = X.shape[0]
+    labels = np.zeros(n, np.int64)
+    q = C.c_double(0.0)
+    nc = C.c_int64(0)
+    check(_lib.load().sev_cluster_louvain(n, _ptr(colptr), _ptr(rowval), _ptr(vals),
+                                          _dtype_code(dt), float(resolution), int(nrandomstarts),
+                                          int(niterations), int(seed) & (2**64 - 1),
+                                          1 if group_singletons else 0, _ptr(labels),
+                                          C.byref(q), C.byref(nc)))
+    return (labels, q.value) if return_modularity else labels
+
+
 def copy_probe() -> float:
     """Achievable HBM copy rate of the device in GB/s (1 GiB read + 1 GiB
     written per launch, best of 5)."""

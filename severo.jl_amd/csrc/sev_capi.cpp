@@ -1433,6 +1433,49 @@ void FindNeighboursCosine64(const double *X, int n, int d, int ld, int k, int nt
                          distances, nullptr);
 }
 
+int sev_cluster_louvain(int64_t n, const int64_t *colptr, const int64_t *rowval,
+                        const void *nzval, int dtype, double resolution, int64_t nrandomstarts,
+                        int64_t niterations, uint64_t seed, int group_singletons,
+                        int64_t *labels, double *modularity, int64_t *nclusters) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+        SEV_REQUIRE(n >= 1 && labels != nullptr, "bad arguments");
+        SEV_REQUIRE(nrandomstarts >= 1 && niterations >= 0,
+                    "nrandomstarts must be >= 1 and niterations >= 0");
+        Stream s;
+        DevBuf<int64_t> dcp;
+        DevBuf<int32_t> drow;
+        int64_t nnz = 0;
+        csc_to_device(n, n, colptr, rowval, dcp, drow, nnz, s.st);
+        SEV_REQUIRE(nnz == 0 || nzval != nullptr, "nzval is NULL");
+        std::vector<double> w((size_t)std::max<int64_t>(nnz, 1));
+        double wsum = 0.0;
+        for (int64_t p = 0; p < nnz; ++p) {
+            const double x = dtype == SEV_FLOAT32 ? (double)((const float *)nzval)[p]
+                                                  : ((const double *)nzval)[p];
+            SEV_REQUIRE(std::isfinite(x) && x >= 0.0, "graph weights must be finite and >= 0");
+            w[(size_t)p] = x;
+            wsum += x;
+        }
+        DevBuf<double> dw;
+        upload(dw, w.data(), w.size(), s.st);
+        DevBuf<int32_t> dlab;
+        dlab.ensure((size_t)n);
+        double q = 0.0;
+        const int64_t nc = louvain_cluster(n, dcp.get(), drow.get(), dw.get(), wsum, resolution,
+                                           nrandomstarts, niterations, 1e-4, seed,
+                                           group_singletons != 0, dlab.get(), &q, s.st);
+        std::vector<int32_t> lab((size_t)n);
+        SEV_HIP(hipMemcpyAsync(lab.data(), dlab.get(), sizeof(int32_t) * (size_t)n,
+                               hipMemcpyDeviceToHost, s.st));
+        SEV_HIP(hipStreamSynchronize(s.st));
+        for (int64_t i = 0; i < n; ++i) labels[i] = (int64_t)lab[(size_t)i] + 1;
+        if (modularity) *modularity = q;
+        if (nclusters) *nclusters = nc;
+    });
+}
+
 int sev_snn_jaccard(int64_t n, int64_t k, const int32_t *nn_index, double prune, int dtype,
                     int64_t *nnz, int64_t *colptr, int64_t *rowval, void *nzval) {
     return guarded([&] {

@@ -98,6 +98,17 @@ void find_variable_features_method(int64_t m, int64_t G, const int64_t *colptr_d
                                    int64_t nfeat, int64_t nbins, double alpha, int64_t *selected,
                                    double *metric_out, hipStream_t st);
 
+// ---- sev_louvain.hip -----------------------------------------------------
+// cluster(SNN; algorithm=:louvain) on a square CSC (0-based rows, sorted
+// within columns; val_d FP64 weights >= 0 summing to wsum): labels_d[n]
+// receive 0-based clusters in order of first occurrence, *q_out the
+// modularity of the best start (NaN without edges); returns the number of
+// clusters.
+int64_t louvain_cluster(int64_t n, const int64_t *colptr_d, const int32_t *row_d,
+                        const double *val_d, double wsum, double res, int64_t nstarts,
+                        int64_t niter, double min_mod, uint64_t seed, bool group,
+                        int32_t *labels_d, double *q_out, hipStream_t st);
+
 // Batches of S samples stacked by rows of one cells x genes CSC.
 // soff[s*G + g] (s = 0..S): first entry of column g in sample >= s.
 void sample_offsets(int64_t G, int64_t S, const int32_t *row_sample, const int64_t *colptr,
