@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive step")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
+    ap.add_argument("--no-cluster", action="store_true",
+                    help="skip cluster(SNN) (Louvain) on the last step's graph")
     ap.add_argument("--batch-direct", action="store_true",
                     help="launch the batch's restart rounds directly instead of from HIP graphs")
     ap.add_argument("--errors", action="store_true",
@@ -419,6 +421,16 @@ def main():
     out["find_variable_features_ms"] = round(hvg_ms, 2)
     if errs:
         out["errors_vs_fp64"] = errs
+    if not args.no_cluster:
+        # Louvain (10 random starts) on the SNN graph of the last timed step,
+        # downstream of the metric's path and outside its timing
+        t0 = time.perf_counter()
+        lab, q = p.cluster(m, return_modularity=True)
+        cl_s = time.perf_counter() - t0
+        out["cluster"] = {"seconds": round(cl_s, 3), "nrandomstarts": 10, "resolution": 0.8,
+                          "clusters": int(lab.max()), "modularity": round(q, 6),
+                          "graph": f"SNN of the {forms[-1]} step"}
+        print(f"# cluster: {out['cluster']}", file=sys.stderr, flush=True)
     if not args.no_e2e:
         out["end_to_end"] = end_to_end(sev, p, feats, init, head, m)
         print(f"# end to end ({head}): {out['end_to_end']}", file=sys.stderr, flush=True)

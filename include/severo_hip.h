@@ -336,6 +336,13 @@ int sev_pipeline_knn_passes(sev_pipeline *p, int64_t *passes);
 
 /* Stage 5: Jaccard SNN of the current neighbours (dtype of the weights). */
 int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz);
+
+/* cluster(SNN; algorithm=:louvain, ...) on the graph built by
+ * sev_pipeline_snn, as sev_cluster_louvain (same arguments, outputs and
+ * errors; labels[m] for the m cells). */
+int sev_pipeline_cluster(sev_pipeline *p, double resolution, int64_t nrandomstarts,
+                         int64_t niterations, uint64_t seed, int group_singletons,
+                         int64_t *labels, double *modularity, int64_t *nclusters);
 int sev_pipeline_get_snn(sev_pipeline *p, int64_t *colptr, int64_t *rowval, void *nzval);
 
 /* Kernel timing (no counterpart in the reference, which has no tracing):

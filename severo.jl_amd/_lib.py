@@ -53,6 +53,7 @@ SIGNATURES = {
     "sev_pipeline_find_variable_features": (_int, [_p, _i64, _dbl, _p, _p]),
     "sev_cluster_louvain": (_int, [_i64, _p, _p, _p, _int, _dbl, _i64, _i64, _u64, _int, _p, _p,
                                    _p]),
+    "sev_pipeline_cluster": (_int, [_p, _dbl, _i64, _i64, _u64, _int, _p, _p, _p]),
     "sev_pipeline_find_variable_features_method": (_int, [_p, _int, _i64, _i64, _int, _dbl, _p, _p]),
     "sev_pipeline_create": (_int, [_pp]),
     "sev_pipeline_destroy": (None, [_p]),

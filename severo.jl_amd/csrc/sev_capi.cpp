@@ -1036,6 +1036,35 @@ int sev_pipeline_snn(sev_pipeline *p, double prune, int dtype, int64_t *nnz) {
     });
 }
 
+int sev_pipeline_cluster(sev_pipeline *p, double resolution, int64_t nrandomstarts,
+                         int64_t niterations, uint64_t seed, int group_singletons,
+                         int64_t *labels, double *modularity, int64_t *nclusters) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        SEV_REQUIRE(p->have_snn, "the SNN graph has not been built");
+        SEV_REQUIRE(labels != nullptr, "labels is NULL");
+        SEV_REQUIRE(nrandomstarts >= 1 && niterations >= 0,
+                    "nrandomstarts must be >= 1 and niterations >= 0");
+        hipStream_t st = p->st();
+        const int64_t m = p->m;
+        const bool f32 = p->snn_dtype == SEV_FLOAT32;
+        const void *val = f32 ? (const void *)p->sval32.get() : (const void *)p->sval64.get();
+        DevBuf<int32_t> dlab;
+        dlab.ensure((size_t)m);
+        double q = 0.0;
+        const int64_t nc = louvain_cluster(m, p->scp.get(), p->srow.get(), val, f32, p->snn_nnz,
+                                           resolution, nrandomstarts, niterations, 1e-4, seed,
+                                           group_singletons != 0, dlab.get(), &q, st);
+        std::vector<int32_t> lab((size_t)m);
+        SEV_HIP(hipMemcpyAsync(lab.data(), dlab.get(), sizeof(int32_t) * (size_t)m,
+                               hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+        for (int64_t i = 0; i < m; ++i) labels[i] = (int64_t)lab[(size_t)i] + 1;
+        if (modularity) *modularity = q;
+        if (nclusters) *nclusters = nc;
+    });
+}
+
 int sev_pipeline_profile(sev_pipeline *p, int enable) {
     return guarded([&] {
         SEV_HIP(hipStreamSynchronize(p->st()));
@@ -1449,21 +1478,25 @@ int sev_cluster_louvain(int64_t n, const int64_t *colptr, const int64_t *rowval,
         int64_t nnz = 0;
         csc_to_device(n, n, colptr, rowval, dcp, drow, nnz, s.st);
         SEV_REQUIRE(nnz == 0 || nzval != nullptr, "nzval is NULL");
-        std::vector<double> w((size_t)std::max<int64_t>(nnz, 1));
-        double wsum = 0.0;
+        const bool f32 = dtype == SEV_FLOAT32;
         for (int64_t p = 0; p < nnz; ++p) {
-            const double x = dtype == SEV_FLOAT32 ? (double)((const float *)nzval)[p]
-                                                  : ((const double *)nzval)[p];
+            const double x = f32 ? (double)((const float *)nzval)[p] : ((const double *)nzval)[p];
             SEV_REQUIRE(std::isfinite(x) && x >= 0.0, "graph weights must be finite and >= 0");
-            w[(size_t)p] = x;
-            wsum += x;
         }
-        DevBuf<double> dw;
-        upload(dw, w.data(), w.size(), s.st);
+        DevBuf<float> dw32;
+        DevBuf<double> dw64;
+        const void *dw = nullptr;
+        if (f32) {
+            upload(dw32, (const float *)nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+            dw = dw32.get();
+        } else {
+            upload(dw64, (const double *)nzval, (size_t)std::max<int64_t>(nnz, 1), s.st);
+            dw = dw64.get();
+        }
         DevBuf<int32_t> dlab;
         dlab.ensure((size_t)n);
         double q = 0.0;
-        const int64_t nc = louvain_cluster(n, dcp.get(), drow.get(), dw.get(), wsum, resolution,
+        const int64_t nc = louvain_cluster(n, dcp.get(), drow.get(), dw, f32, nnz, resolution,
                                            nrandomstarts, niterations, 1e-4, seed,
                                            group_singletons != 0, dlab.get(), &q, s.st);
         std::vector<int32_t> lab((size_t)n);

@@ -100,12 +100,12 @@ void find_variable_features_method(int64_t m, int64_t G, const int64_t *colptr_d
 
 // ---- sev_louvain.hip -----------------------------------------------------
 // cluster(SNN; algorithm=:louvain) on a square CSC (0-based rows, sorted
-// within columns; val_d FP64 weights >= 0 summing to wsum): labels_d[n]
+// within columns; nnz weights >= 0 in FP32 (f32) or FP64): labels_d[n]
 // receive 0-based clusters in order of first occurrence, *q_out the
 // modularity of the best start (NaN without edges); returns the number of
 // clusters.
 int64_t louvain_cluster(int64_t n, const int64_t *colptr_d, const int32_t *row_d,
-                        const double *val_d, double wsum, double res, int64_t nstarts,
+                        const void *val_d, bool f32, int64_t nnz, double res, int64_t nstarts,
                         int64_t niter, double min_mod, uint64_t seed, bool group,
                         int32_t *labels_d, double *q_out, hipStream_t st);
 

@@ -16,13 +16,14 @@
 // reference.
 //
 // Edge weights are held in fixed point (w * 2^S rounded, S <= 40 chosen so
-// that the total stays below 2^61), so every accumulation is an exact integer
+// that nnz * max(w) * 2^S stays below 2^61), so every accumulation is an exact integer
 // sum: results do not depend on the order of atomic updates, and repeated
 // runs give identical labels.
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -47,14 +48,11 @@ inline unsigned grid_of(int64_t n, int threads = kLvThreads) {
     return (unsigned)std::max<int64_t>(1, ceil_div(n, threads));
 }
 
-__device__ __forceinline__ uint32_t slot_of(int key, uint32_t cap) {
-    return ((uint32_t)key * 2654435761u) % cap;
-}
-
 // Open addressing with linear probing; cap exceeds twice the number of
-// distinct keys, so a free slot is always reached.
+// distinct keys, so a free slot is always reached.  Power-of-two tables
+// (shift = 32 - log2(cap)) take the high bits of the multiplicative hash.
 __device__ __forceinline__ void hash_add(int *keys, u64 *vals, uint32_t cap, int key, u64 w) {
-    uint32_t h = slot_of(key, cap);
+    uint32_t h = ((uint32_t)key * 2654435761u) % cap;
     while (true) {
         const int prev = atomicCAS(&keys[h], -1, key);
         if (prev == -1 || prev == key) {
@@ -62,6 +60,19 @@ __device__ __forceinline__ void hash_add(int *keys, u64 *vals, uint32_t cap, int
             return;
         }
         h = h + 1 == cap ? 0 : h + 1;
+    }
+}
+
+__device__ __forceinline__ void hash_add_pow2(int *keys, u64 *vals, uint32_t cap, int shift,
+                                              int key, u64 w) {
+    uint32_t h = ((uint32_t)key * 2654435761u) >> shift;
+    while (true) {
+        const int prev = atomicCAS(&keys[h], -1, key);
+        if (prev == -1 || prev == key) {
+            atomicAdd(&vals[h], w);
+            return;
+        }
+        h = (h + 1) & (cap - 1);
     }
 }
 
@@ -149,14 +160,21 @@ k_move_light(int64_t n, const int64_t *ptr, const int32_t *adj, const u64 *w, co
     }
     int *keys = skey[wv];
     u64 *vals = sval[wv];
-    for (int s = lane; s < kLightCap; s += kWave) {
+    // table of the smallest power of two >= 2 deg, at least one slot a lane
+    uint32_t cap = kWave;
+    int shift = 26;
+    while (cap < 2 * (uint32_t)deg) {
+        cap <<= 1;
+        --shift;
+    }
+    for (uint32_t s = lane; s < cap; s += kWave) {
         keys[s] = -1;
         vals[s] = 0;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     for (int64_t e = lane; e < deg; e += kWave)
-        hash_add(keys, vals, kLightCap, c[adj[p0 + e]], w[p0 + e]);
+        hash_add_pow2(keys, vals, cap, shift, c[adj[p0 + e]], w[p0 + e]);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const u64 kvi = k[v];
@@ -164,7 +182,7 @@ k_move_light(int64_t n, const int64_t *ptr, const int32_t *adj, const u64 *w, co
     double bs = -INFINITY;
     int bc = INT_MAX;
     u64 kcv = 0;
-    for (int s = lane; s < kLightCap; s += kWave) {
+    for (uint32_t s = lane; s < cap; s += kWave) {
         const int C = keys[s];
         if (C >= 0) r.consider(C, vals[s], wtot, bs, bc, kcv);
     }
@@ -265,18 +283,17 @@ __global__ void k_wtot(int64_t n, const int32_t *c, const u64 *k, u64 *wtot) {
     if (v < n) atomicAdd(&wtot[c[v]], k[v]);
 }
 
-// weight of the edges inside clusters (sum_w_in without tot_self)
+// weight of the edges inside clusters (sum_w_in without tot_self), one
+// edge per thread over a fixed grid
+constexpr int kInBlocks = 2048;
 __global__ void __launch_bounds__(kLvThreads)
-k_mod_in(int64_t n, const int64_t *ptr, const int32_t *adj, const u64 *w, const int32_t *c,
+k_mod_in(int64_t m, const int32_t *src, const int32_t *adj, const u64 *w, const int32_t *c,
          u64 *acc) {
     __shared__ u64 part[kLvWaves];
-    const int64_t v = (int64_t)blockIdx.x * kLvThreads + threadIdx.x;
     u64 s = 0;
-    if (v < n) {
-        const int cv = c[v];
-        for (int64_t e = ptr[v]; e < ptr[v + 1]; ++e)
-            if (c[adj[e]] == cv) s += w[e];
-    }
+    for (int64_t e = (int64_t)blockIdx.x * kLvThreads + threadIdx.x; e < m;
+         e += (int64_t)kInBlocks * kLvThreads)
+        if (c[src[e]] == c[adj[e]]) s += w[e];
     s = wave_sum_u64(s);
     if (lane_id() == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
@@ -385,7 +402,8 @@ __global__ void k_run_heads(int64_t m, const u64 *key, int b, int32_t *flag) {
 
 __global__ void __launch_bounds__(kLvThreads)
 k_run_emit(int64_t m, const u64 *key, const u64 *val, int b, const int32_t *flag,
-           const int64_t *pos, int32_t *adj_r, u64 *w_r, int32_t *rowcnt, u64 *self_acc) {
+           const int64_t *pos, int32_t *adj_r, int32_t *src_r, u64 *w_r, int32_t *rowcnt,
+           u64 *self_acc) {
     const int64_t i = (int64_t)blockIdx.x * kLvThreads + threadIdx.x;
     u64 self = 0;
     if (i < m) {
@@ -398,6 +416,7 @@ k_run_emit(int64_t m, const u64 *key, const u64 *val, int b, const int32_t *flag
             const int64_t g = pos[i] + flag[i] - 1;
             if (flag[i]) {
                 adj_r[g] = (int32_t)col;
+                src_r[g] = (int32_t)row;
                 atomicAdd(&rowcnt[row], 1);
             }
             atomicAdd(&w_r[g], val[i]);
@@ -422,7 +441,7 @@ __global__ void k_count_nonself(int64_t n, const int64_t *colptr, const int32_t 
 
 __global__ void k_build_net(int64_t n, const int64_t *colptr, const int32_t *row,
                             const double *val, double scale, const int64_t *ptr, int32_t *adj,
-                            u64 *w, u64 *k) {
+                            int32_t *src, u64 *w, u64 *k) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t e = ptr[i];
@@ -431,11 +450,27 @@ __global__ void k_build_net(int64_t n, const int64_t *colptr, const int32_t *row
         if (row[p] == (int32_t)i) continue;
         const u64 q = (u64)llrint(val[p] * scale);
         adj[e] = row[p];
+        src[e] = (int32_t)i;
         w[e] = q;
         s += q;
         ++e;
     }
     k[i] = s;
+}
+
+// largest weight as its bit pattern (ordered like the value for w >= 0)
+template <class T>
+__global__ void __launch_bounds__(kLvThreads) k_max_bits(int64_t m, const T *val, u64 *acc) {
+    const int64_t i = (int64_t)blockIdx.x * kLvThreads + threadIdx.x;
+    const double x = i < m ? (double)val[i] : 0.0;
+    const u64 b = wave_max_u64((u64)__double_as_longlong(x));
+    if (lane_id() == 0 && b) atomicMax(acc, b);
+}
+
+template <class T>
+__global__ void k_to_f64(int64_t m, const T *in, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] = (double)in[i];
 }
 
 __global__ void __launch_bounds__(kLvThreads) k_sum_u64(int64_t n, const u64 *a, u64 *acc) {
@@ -463,6 +498,97 @@ __global__ void k_flag_single(int64_t n, const int32_t *lab, const int32_t *cnt,
 __global__ void k_compact(int64_t L, const int32_t *flag, const int64_t *pos, int32_t *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < L && flag[i]) out[pos[i]] = (int32_t)i;
+}
+
+// Singles joined to another single (in either direction) keep the reference's
+// sequential order; the others read no label that moves, so they are placed
+// in parallel.
+__global__ void k_single_dep(int64_t ns, const int32_t *singles, const int64_t *colptr,
+                             const int32_t *row, const int32_t *lab, const int32_t *cnt,
+                             int32_t *dep) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ns) return;
+    const int32_t j = singles[t];
+    for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+        const int32_t i = row[p];
+        if (i != j && cnt[lab[i]] <= 1) {
+            dep[j] = 1;
+            dep[i] = 1;
+        }
+    }
+}
+
+// One wave per independent single; the choice goes to choice[t] (-1 for
+// dependent ones) and is applied afterwards.
+__global__ void __launch_bounds__(kLvThreads)
+k_group_indep(int64_t ns, const int32_t *singles, const int32_t *dep, const int64_t *colptr,
+              const int32_t *row, const double *val, const int32_t *others, int64_t nothers,
+              const int32_t *lab, int32_t *choice) {
+    const int lane = lane_id();
+    const int64_t t = (int64_t)blockIdx.x * kLvWaves + (threadIdx.x >> 6);
+    if (t >= ns) return;
+    const int32_t j = singles[t];
+    if (dep[j]) {
+        if (lane == 0) choice[t] = -1;
+        return;
+    }
+    const int64_t p0 = colptr[j], p1 = colptr[j + 1];
+    int64_t found = INT64_MAX;
+    for (int64_t base = 0; base < nothers && found == INT64_MAX; base += kWave) {
+        const int64_t o = base + lane;
+        int64_t mine = INT64_MAX;
+        if (o < nothers) {
+            const int cl = others[o];
+            bool none = true;
+            for (int64_t p = p0; p < p1 && none; ++p) none = lab[row[p]] != cl;
+            if (none) mine = o;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const int64_t other = __shfl_xor(mine, off, kWave);
+            mine = other < mine ? other : mine;
+        }
+        found = mine;
+    }
+    int closest;
+    if (found != INT64_MAX) {
+        closest = others[found];
+    } else {
+        double bs = -INFINITY;
+        int bo = INT_MAX;
+        for (int64_t o = lane; o < nothers; o += kWave) {
+            const int cl = others[o];
+            double sum = 0.0;
+            int64_t cnt = 0;
+            for (int64_t p = p0; p < p1; ++p)
+                if (lab[row[p]] == cl) {
+                    sum += val[p];
+                    ++cnt;
+                }
+            best_merge(bs, bo, sum / (double)cnt, (int)o);
+        }
+        wave_best(bs, bo);
+        closest = others[bo];
+    }
+    if (lane == 0) choice[t] = closest;
+}
+
+__global__ void k_apply_choice(int64_t ns, const int32_t *singles, const int32_t *choice,
+                               int32_t *lab) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns && choice[t] >= 0) lab[singles[t]] = choice[t];
+}
+
+__global__ void k_flag_dep(int64_t ns, const int32_t *singles, const int32_t *dep,
+                           int32_t *flag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ns) flag[t] = dep[singles[t]];
+}
+
+__global__ void k_compact_vals(int64_t L, const int32_t *flag, const int64_t *pos,
+                               const int32_t *vals, int32_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L && flag[i]) out[pos[i]] = vals[i];
 }
 
 // One block moves singles[s0..s1) in order: the first cluster of size > 1
@@ -547,7 +673,7 @@ struct LvGraph {
     int64_t n = 0, nnz = 0;
     u64 tot_self = 0;
     DevBuf<int64_t> ptr;
-    DevBuf<int32_t> adj;
+    DevBuf<int32_t> adj, src;  // target and source node of each edge
     DevBuf<u64> w, k;
     DevBuf<int32_t> heavy;
     int64_t nheavy = 0;
@@ -612,8 +738,9 @@ struct Louvain {
         part.ensure(kSqBlocks);
         sq.ensure(1);
         k_fill_u64<<<1, 1, 0, st>>>(acc.get(), 1, 0);
-        k_mod_in<<<grid_of(g.n), kLvThreads, 0, st>>>(g.n, g.ptr.get(), g.adj.get(), g.w.get(),
-                                                       c, acc.get());
+        if (g.nnz > 0)
+            k_mod_in<<<(unsigned)std::min<int64_t>(kInBlocks, grid_of(g.nnz)), kLvThreads, 0, st>>>(
+                g.nnz, g.src.get(), g.adj.get(), g.w.get(), c, acc.get());
         k_mod_sq<<<kSqBlocks, kLvThreads, 0, st>>>(g.n, wt, totw_d, part.get());
         k_mod_sq_final<<<1, kSqBlocks, 0, st>>>(part.get(), sq.get());
         SEV_LAUNCH_CHECK();
@@ -730,6 +857,7 @@ struct Louvain {
             SEV_HIP(hipMemsetAsync(r.ptr.get(), 0, sizeof(int64_t) * (size_t)(nc + 1), st));
             r.nnz = 0;
             r.adj.ensure(1);
+            r.src.ensure(1);
             r.w.ensure(1);
             return;
         }
@@ -752,6 +880,7 @@ struct Louvain {
         const int64_t ne = scan_i32(flag.get(), m);
         r.nnz = ne;
         r.adj.ensure((size_t)std::max<int64_t>(ne, 1));
+        r.src.ensure((size_t)std::max<int64_t>(ne, 1));
         r.w.ensure((size_t)std::max<int64_t>(ne, 1));
         DevBuf<int32_t> rowcnt;
         rowcnt.ensure((size_t)nc);
@@ -760,8 +889,8 @@ struct Louvain {
         k_fill_i32<<<grid_of(nc), kLvThreads, 0, st>>>(rowcnt.get(), nc, 0);
         k_fill_u64<<<1, 1, 0, st>>>(acc.get(), 1, 0);
         k_run_emit<<<grid_of(m), kLvThreads, 0, st>>>(m, key2.get(), val2.get(), b, flag.get(),
-                                                      pos.get(), r.adj.get(), r.w.get(),
-                                                      rowcnt.get(), acc.get());
+                                                      pos.get(), r.adj.get(), r.src.get(),
+                                                      r.w.get(), rowcnt.get(), acc.get());
         SEV_LAUNCH_CHECK();
         u64 self = 0;
         SEV_HIP(hipMemcpyAsync(&self, acc.get(), sizeof self, hipMemcpyDeviceToHost, st));
@@ -796,15 +925,36 @@ struct Louvain {
 }  // namespace
 
 int64_t louvain_cluster(int64_t n, const int64_t *colptr_d, const int32_t *row_d,
-                        const double *val_d, double wsum, double res,
-                        int64_t nstarts, int64_t niter, double min_mod, uint64_t seed,
-                        bool group, int32_t *labels_d, double *q_out, hipStream_t st) {
+                        const void *val_in, bool f32, int64_t nnz, double res, int64_t nstarts,
+                        int64_t niter, double min_mod, uint64_t seed, bool group,
+                        int32_t *labels_d, double *q_out, hipStream_t st) {
     SEV_REQUIRE(n >= 1 && n < INT_MAX, "number of nodes out of range");
     SEV_REQUIRE(nstarts >= 1 && niter >= 0, "nrandomstarts must be >= 1, niterations >= 0");
     SEV_REQUIRE(std::isfinite(res), "resolution must be finite");
-    SEV_REQUIRE(std::isfinite(wsum) && wsum >= 0.0, "weights must be finite and >= 0");
+    DevBuf<double> val64;
+    const double *val_d = (const double *)val_in;
+    u64 maxbits = 0;
+    {
+        DevBuf<u64> mx;
+        mx.ensure(1);
+        k_fill_u64<<<1, 1, 0, st>>>(mx.get(), 1, 0);
+        if (f32) {
+            val64.ensure((size_t)std::max<int64_t>(nnz, 1));
+            k_to_f64<<<grid_of(nnz), kLvThreads, 0, st>>>(nnz, (const float *)val_in,
+                                                          val64.get());
+            val_d = val64.get();
+        }
+        k_max_bits<<<grid_of(nnz), kLvThreads, 0, st>>>(nnz, val_d, mx.get());
+        SEV_LAUNCH_CHECK();
+        SEV_HIP(hipMemcpyAsync(&maxbits, mx.get(), sizeof maxbits, hipMemcpyDeviceToHost, st));
+        SEV_HIP(hipStreamSynchronize(st));
+    }
+    double wmax = 0.0;
+    std::memcpy(&wmax, &maxbits, sizeof wmax);
+    const double wbound = wmax * (double)nnz;
+    SEV_REQUIRE(std::isfinite(wbound), "graph weights must be finite");
     int e2 = 0;
-    if (wsum > 0.0) (void)std::frexp(wsum, &e2);
+    if (wbound > 0.0) (void)std::frexp(wbound, &e2);
     const int S = std::min(40, 61 - e2);
     SEV_REQUIRE(S >= 0, "total edge weight too large for the fixed-point sums");
     Louvain lv;
@@ -832,10 +982,12 @@ int64_t louvain_cluster(int64_t n, const int64_t *colptr_d, const int32_t *row_d
         g.nnz = last + lc;
         k_set_i64<<<1, 1, 0, st>>>(g.ptr.get() + n, g.nnz);
         g.adj.ensure((size_t)std::max<int64_t>(g.nnz, 1));
+        g.src.ensure((size_t)std::max<int64_t>(g.nnz, 1));
         g.w.ensure((size_t)std::max<int64_t>(g.nnz, 1));
         k_build_net<<<grid_of(n), kLvThreads, 0, st>>>(n, colptr_d, row_d, val_d,
                                                        std::ldexp(1.0, S), g.ptr.get(),
-                                                       g.adj.get(), g.w.get(), g.k.get());
+                                                       g.adj.get(), g.src.get(), g.w.get(),
+                                                       g.k.get());
         lv.acc.ensure(1);
         k_fill_u64<<<1, 1, 0, st>>>(lv.acc.get(), 1, 0);
         k_sum_u64<<<grid_of(n), kLvThreads, 0, st>>>(n, g.k.get(), lv.acc.get());
@@ -892,10 +1044,31 @@ int64_t louvain_cluster(int64_t n, const int64_t *colptr_d, const int32_t *row_d
             k_compact<<<grid_of(nc), kLvThreads, 0, st>>>(nc, lv.flag.get(), lv.pos.get(),
                                                           others.get());
             SEV_LAUNCH_CHECK();
-            for (int64_t s0 = 0; s0 < nsingle; s0 += kSinglesPerLaunch) {
-                const int64_t s1 = std::min(nsingle, s0 + kSinglesPerLaunch);
-                k_group_singletons<<<1, kLvThreads, 0, st>>>(s0, s1, singles.get(), colptr_d,
-                                                             row_d, val_d, others.get(), nothers,
+            DevBuf<int32_t> dep, choice, seq;
+            dep.ensure((size_t)n);
+            choice.ensure((size_t)nsingle);
+            k_fill_i32<<<grid_of(n), kLvThreads, 0, st>>>(dep.get(), n, 0);
+            k_single_dep<<<grid_of(nsingle), kLvThreads, 0, st>>>(
+                nsingle, singles.get(), colptr_d, row_d, labels_d, cnt.get(), dep.get());
+            k_group_indep<<<grid_of(nsingle, kLvWaves), kLvThreads, 0, st>>>(
+                nsingle, singles.get(), dep.get(), colptr_d, row_d, val_d, others.get(), nothers,
+                labels_d, choice.get());
+            k_apply_choice<<<grid_of(nsingle), kLvThreads, 0, st>>>(nsingle, singles.get(),
+                                                                    choice.get(), labels_d);
+            k_flag_dep<<<grid_of(nsingle), kLvThreads, 0, st>>>(nsingle, singles.get(), dep.get(),
+                                                                lv.flag.get());
+            SEV_LAUNCH_CHECK();
+            const int64_t ndep = lv.scan_i32(lv.flag.get(), nsingle);
+            if (ndep > 0) {
+                seq.ensure((size_t)ndep);
+                k_compact_vals<<<grid_of(nsingle), kLvThreads, 0, st>>>(
+                    nsingle, lv.flag.get(), lv.pos.get(), singles.get(), seq.get());
+                SEV_LAUNCH_CHECK();
+            }
+            for (int64_t s0 = 0; s0 < ndep; s0 += kSinglesPerLaunch) {
+                const int64_t s1 = std::min(ndep, s0 + kSinglesPerLaunch);
+                k_group_singletons<<<1, kLvThreads, 0, st>>>(s0, s1, seq.get(), colptr_d, row_d,
+                                                             val_d, others.get(), nothers,
                                                              labels_d);
                 SEV_LAUNCH_CHECK();
             }

@@ -8,9 +8,9 @@
 #      src/irlba.jl:66-71, src/neighbours.jl:32-72).
 #   2. The functions below, which take and return the same types as
 #      Severo's API (normalize_cells, scale_features, embedding/pca,
-#      nearest_neighbours, jaccard_index, shared_nearest_neighbours) and run
-#      every stage on the GPU.  `Pipeline` keeps a dataset resident on the
-#      device across stages.
+#      nearest_neighbours, jaccard_index, shared_nearest_neighbours,
+#      cluster) and run every stage on the GPU.  (The Python package's
+#      `Pipeline` keeps a dataset resident on the device across stages.)
 module SeveroHIP
 
 using SparseArrays, LinearAlgebra
@@ -232,5 +232,27 @@ function shared_nearest_neighbours(X::NamedMatrix{T}, k::Integer; dims=:, includ
 end
 shared_nearest_neighbours(em::Severo.LinearEmbedding, k::Integer; kw...) =
     shared_nearest_neighbours(em.coordinates, k; kw...)
+
+# cluster(SNN; algorithm=:louvain, resolution, nrandomstarts, niterations,
+# group_singletons) (src/clustering.jl:59-104): Louvain with parallel local
+# moving on the device; `seed` fixes the random label orders.
+function cluster(SNN::NamedArray{T,2,SparseMatrixCSC{T,Int64}}; algorithm=:louvain,
+                 resolution::Real=0.8, nrandomstarts::Integer=10, niterations::Integer=10,
+                 group_singletons::Bool=true, seed::Integer=0) where {T<:Union{Float32,Float64}}
+    algorithm = Symbol(algorithm)
+    algorithm == :louvain || error("unknown clustering algorithm: $algorithm")
+    A = SNN.array
+    n = Severo.checksquare(A)
+    labels = Vector{Int64}(undef, n)
+    q = Ref{Float64}(0.0)
+    nc = Ref{Int64}(0)
+    check(ccall((:sev_cluster_louvain, libsev), Cint,
+                (Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Cint, Cdouble, Int64, Int64, UInt64,
+                 Cint, Ptr{Int64}, Ref{Float64}, Ref{Int64}),
+                n, A.colptr, A.rowval, A.nzval, T == Float32 ? SEV_FLOAT32 : SEV_FLOAT64,
+                resolution, nrandomstarts, niterations, UInt64(seed), group_singletons ? 1 : 0,
+                labels, q, nc))
+    NamedArray(labels, (SNN.dicts[1],), (:cells,))
+end
 
 end # module

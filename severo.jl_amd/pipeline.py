@@ -314,6 +314,19 @@ class Pipeline:
         self._snn = (nnz.value, np.dtype(dtype))
         return nnz.value
 
+    def cluster(self, m, resolution=0.8, nrandomstarts=10, niterations=10,
+                group_singletons=True, seed=0, return_modularity=False):
+        """cluster(SNN) (src/clustering.jl:59-104) on the resident SNN graph
+        of the m cells: 1-based cluster ids, as the one-shot cluster."""
+        labels = np.zeros(m, np.int64)
+        q = C.c_double(0.0)
+        nc = C.c_int64(0)
+        check(self._L.sev_pipeline_cluster(self._h, float(resolution), int(nrandomstarts),
+                                           int(niterations), int(seed) & (2**64 - 1),
+                                           1 if group_singletons else 0, _ptr(labels),
+                                           C.byref(q), C.byref(nc)))
+        return (labels, q.value) if return_modularity else labels
+
     def get_snn(self, m):
         nnz, dt = self._snn
         colptr = np.zeros(m + 1, np.int64)

@@ -181,3 +181,54 @@ def test_no_edges_and_errors(sev):
     bad = sp.csc_matrix(np.array([[0.0, -1.0], [-1.0, 0.0]]))
     with pytest.raises(sev.SeveroError):
         sev.cluster(bad)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pipeline_cluster_equals_one_shot(sev, orc, dtype):
+    """Clustering the device-resident SNN gives the one-shot entry's labels
+    on the same graph, bit for bit."""
+    rng = np.random.default_rng(31)
+    C = rng.normal(0, 5.0, (9, 30))
+    Z = np.asfortranarray((C[rng.integers(0, 9, 4000)] + rng.normal(0, 1.0, (4000, 30)))
+                          .astype(np.float32))
+    p = sev.Pipeline()
+    p.set_embedding(Z)
+    p.neighbours(20, include_self=True, ndims=30)
+    p.snn(1 / 15, dtype)
+    lab_p, q_p = p.cluster(4000, return_modularity=True)
+    S = p.get_snn(4000)
+    p.close()
+    lab_h, q_h = sev.cluster(S, return_modularity=True)
+    assert np.array_equal(lab_p, lab_h) and q_p == q_h
+    assert lab_p.max() == 9
+
+
+def test_dependent_singletons_in_order(sev, orc):
+    """Two singles joined by a stored zero-weight entry (weight 0 gives them
+    no reason to move): the first goes to the first cluster, and the second,
+    which then has an entry into that cluster, to the next one, as the
+    reference's in-order loop places them."""
+    M, _ = planted([40, 60, 30], 0.5, 0.01, seed=12)
+    A = M.toarray()
+    n0 = A.shape[0]
+    n = n0 + 4
+    B = np.zeros((n, n))
+    B[:n0, :n0] = A
+    B[np.arange(n0, n), np.arange(n0, n)] = 1.0
+    cols, rows, vals = [], [], []
+    for j in range(n):
+        r = list(np.nonzero(B[:, j])[0])
+        if j in (n0 + 1, n0 + 2):  # the stored zero between nodes n0+1 and n0+2
+            r.append(n0 + 2 if j == n0 + 1 else n0 + 1)
+        r.sort()
+        rows += r
+        vals += [B[i, j] for i in r]
+        cols.append(len(r))
+    indptr = np.concatenate([[0], np.cumsum(cols)])
+    Ms = sp.csc_matrix((np.array(vals), np.array(rows), indptr), shape=(n, n))
+    assert Ms.nnz == np.count_nonzero(B) + 2
+    ungrouped = sev.cluster(Ms, group_singletons=False)
+    grouped = sev.cluster(Ms)
+    ref = orc.group_singletons(n, Ms.indptr, Ms.indices, Ms.data, ungrouped - 1)
+    assert np.array_equal(grouped - 1, ref)
+    assert grouped[n0 + 1] != grouped[n0 + 2]
