@@ -39,7 +39,7 @@ using u64 = unsigned long long;
 constexpr int kLvThreads = 256;
 constexpr int kLvWaves = kLvThreads / kWave;
 constexpr int kLightDeg = 256;   // nodes with at most this many edges hash in LDS
-constexpr int kLightCap = 512;   // LDS slots per wave (load factor <= 1/2)
+constexpr int kSmallDeg = 128;   // ... in 256-slot tables (12 KB a block, 8 waves a SIMD)
 constexpr int kMaxSweeps = 200;  // per local-moving phase
 constexpr int kSqBlocks = 256;   // fixed grid of the sum of squared totals
 constexpr int64_t kSinglesPerLaunch = 4096;
@@ -121,9 +121,10 @@ __device__ __forceinline__ bool active(int64_t v, uint64_t akey, uint32_t athres
 
 // Score of joining cluster C (its total w_tot excluding the node):
 // kin - res * (w_tot * k) / totw, best_local_move (src/modularity.jl:243-253).
+// f = res * k / totw is formed once per node.
 struct MoveRule {
     int cv, dir;
-    double kv, totw, res;
+    double f;
     __device__ __forceinline__ void consider(int C, u64 kin, const u64 *wtot, double &bs, int &bc,
                                              u64 &kcv) const {
         if (C == cv) {
@@ -131,32 +132,44 @@ struct MoveRule {
             return;
         }
         if (dir > 0 ? C < cv : C > cv) return;
-        const double sc = (double)kin - res * (((double)wtot[C] * kv) / totw);
+        const double sc = (double)kin - (double)wtot[C] * f;
         best_merge(bs, bc, sc, C);
     }
     // move when the best allowed cluster beats staying (gain > 0)
     __device__ __forceinline__ int decide(double bs, int bc, u64 kcv, u64 wcv, u64 kvi) const {
-        const double init = (double)kcv - res * (((double)(wcv - kvi) * kv) / totw);
+        const double init = (double)kcv - (double)(wcv - kvi) * f;
         return (bc != INT_MAX && bs > init) ? bc : cv;
     }
 };
 
+// One wave per node of at most kCap / 2 edges: all nodes (list == nullptr;
+// nodes with more edges are left to the listed launches) or the nodes of a
+// list.
+template <int kCap>
 __global__ void __launch_bounds__(kLvThreads)
-k_move_light(int64_t n, const int64_t *ptr, const int32_t *adj, const u64 *w, const u64 *k,
-             const int32_t *c, const u64 *wtot, int32_t *cnew, double totw, double res, int dir,
-             uint64_t akey, uint32_t athresh, int *moved) {
-    __shared__ int skey[kLvWaves][kLightCap];
-    __shared__ u64 sval[kLvWaves][kLightCap];
+k_move_light(int64_t n, const int32_t *list, const int64_t *ptr, const int32_t *adj,
+             const u64 *w, const u64 *k, const int32_t *c, const u64 *wtot, int32_t *cnew,
+             double totw, double res, int dir, uint64_t akey, uint32_t athresh, int *moved) {
+    __shared__ int skey[kLvWaves][kCap];
+    __shared__ u64 sval[kLvWaves][kCap];
     const int lane = lane_id(), wv = (int)(threadIdx.x >> 6);
-    const int64_t v = (int64_t)blockIdx.x * kLvWaves + wv;
-    if (v >= n) return;
+    const int64_t t = (int64_t)blockIdx.x * kLvWaves + wv;
+    if (t >= n) return;
+    const int64_t v = list ? (int64_t)list[t] : t;
     const int64_t p0 = ptr[v];
     const int64_t deg = ptr[v + 1] - p0;
-    if (deg > kLightDeg) return;
+    if (deg > kCap / 2) return;
     const int cv = c[v];
     if (deg == 0 || !active(v, akey, athresh)) {
         if (lane == 0) cnew[v] = cv;
         return;
+    }
+    // each lane's first edge is gathered while the table is cleared
+    int c0 = -1;
+    u64 w0 = 0;
+    if (lane < deg) {
+        c0 = c[adj[p0 + lane]];
+        w0 = w[p0 + lane];
     }
     int *keys = skey[wv];
     u64 *vals = sval[wv];
@@ -173,12 +186,13 @@ k_move_light(int64_t n, const int64_t *ptr, const int32_t *adj, const u64 *w, co
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    for (int64_t e = lane; e < deg; e += kWave)
+    if (lane < deg) hash_add_pow2(keys, vals, cap, shift, c0, w0);
+    for (int64_t e = lane + kWave; e < deg; e += kWave)
         hash_add_pow2(keys, vals, cap, shift, c[adj[p0 + e]], w[p0 + e]);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const u64 kvi = k[v];
-    const MoveRule r{cv, dir, (double)kvi, totw, res};
+    const MoveRule r{cv, dir, res * (double)kvi / totw};
     double bs = -INFINITY;
     int bc = INT_MAX;
     u64 kcv = 0;
@@ -227,7 +241,7 @@ k_move_heavy(const int32_t *heavy, const int64_t *ptr, const int32_t *adj, const
     __syncthreads();
     const int cv = c[v];
     const u64 kvi = k[v];
-    const MoveRule r{cv, dir, (double)kvi, totw, res};
+    const MoveRule r{cv, dir, res * (double)kvi / totw};
     double bs = -INFINITY;
     int bc = INT_MAX;
     u64 kcv = 0;
@@ -257,9 +271,14 @@ k_move_heavy(const int32_t *heavy, const int64_t *ptr, const int32_t *adj, const
     }
 }
 
-__global__ void k_heavy_list(int64_t n, const int64_t *ptr, int32_t *heavy, int *count) {
+// nodes of kSmallDeg < deg <= kLightDeg (medium) and deg > kLightDeg (heavy)
+__global__ void k_deg_lists(int64_t n, const int64_t *ptr, int32_t *medium, int32_t *heavy,
+                            int *count) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n && ptr[v + 1] - ptr[v] > kLightDeg) heavy[atomicAdd(count, 1)] = (int32_t)v;
+    if (v >= n) return;
+    const int64_t deg = ptr[v + 1] - ptr[v];
+    if (deg > kLightDeg) heavy[atomicAdd(&count[1], 1)] = (int32_t)v;
+    else if (deg > kSmallDeg) medium[atomicAdd(&count[0], 1)] = (int32_t)v;
 }
 
 __global__ void k_fill_u64(u64 *a, int64_t n, u64 val) {
@@ -675,8 +694,8 @@ struct LvGraph {
     DevBuf<int64_t> ptr;
     DevBuf<int32_t> adj, src;  // target and source node of each edge
     DevBuf<u64> w, k;
-    DevBuf<int32_t> heavy;
-    int64_t nheavy = 0;
+    DevBuf<int32_t> medium, heavy;
+    int64_t nmedium = 0, nheavy = 0;
 };
 
 struct LvLevel {
@@ -714,16 +733,18 @@ struct Louvain {
     }
 
     void heavy_list(LvGraph &g) {
+        g.medium.ensure((size_t)g.n);
         g.heavy.ensure((size_t)g.n);
-        counter.ensure(1);
-        SEV_HIP(hipMemsetAsync(counter.get(), 0, sizeof(int), st));
-        k_heavy_list<<<grid_of(g.n), kLvThreads, 0, st>>>(g.n, g.ptr.get(), g.heavy.get(),
-                                                          counter.get());
+        counter.ensure(2);
+        SEV_HIP(hipMemsetAsync(counter.get(), 0, 2 * sizeof(int), st));
+        k_deg_lists<<<grid_of(g.n), kLvThreads, 0, st>>>(g.n, g.ptr.get(), g.medium.get(),
+                                                         g.heavy.get(), counter.get());
         SEV_LAUNCH_CHECK();
-        int h = 0;
-        SEV_HIP(hipMemcpyAsync(&h, counter.get(), sizeof h, hipMemcpyDeviceToHost, st));
+        int h[2] = {0, 0};
+        SEV_HIP(hipMemcpyAsync(h, counter.get(), sizeof h, hipMemcpyDeviceToHost, st));
         SEV_HIP(hipStreamSynchronize(st));
-        g.nheavy = h;
+        g.nmedium = h[0];
+        g.nheavy = h[1];
     }
 
     void compute_wtot(const LvGraph &g, const int32_t *c, u64 *wt) {
@@ -782,9 +803,13 @@ struct Louvain {
             const int dir = (sweep & 1) ? -1 : 1;
             const uint64_t akey = seed * 0xD1B54A32D192ED03ull + (uint64_t)sweep;
             SEV_HIP(hipMemsetAsync(counter.get(), 0, sizeof(int), st));
-            k_move_light<<<grid_of(n, kLvWaves), kLvThreads, 0, st>>>(
-                n, g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(), cnew.get(),
-                totw_d, res, dir, akey, athresh, counter.get());
+            k_move_light<2 * kSmallDeg><<<grid_of(n, kLvWaves), kLvThreads, 0, st>>>(
+                n, nullptr, g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(),
+                cnew.get(), totw_d, res, dir, akey, athresh, counter.get());
+            if (g.nmedium > 0)
+                k_move_light<2 * kLightDeg><<<grid_of(g.nmedium, kLvWaves), kLvThreads, 0, st>>>(
+                    g.nmedium, g.medium.get(), g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c,
+                    wtot.get(), cnew.get(), totw_d, res, dir, akey, athresh, counter.get());
             if (g.nheavy > 0)
                 k_move_heavy<<<(unsigned)g.nheavy, kLvThreads, 0, st>>>(
                     g.heavy.get(), g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(),
@@ -847,7 +872,9 @@ struct Louvain {
         r.n = nc;
         r.tot_self = g.tot_self;
         r.heavy.release();
+        r.medium.release();
         r.nheavy = 0;
+        r.nmedium = 0;
         r.ptr.ensure((size_t)nc + 1);
         r.k.ensure((size_t)nc);
         k_fill_u64<<<grid_of(nc), kLvThreads, 0, st>>>(r.k.get(), nc, 0);
