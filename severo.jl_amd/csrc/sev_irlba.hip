@@ -1455,7 +1455,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     if (op.vtype == kValBF16)
         op.lanes_per_row = avg > 256 ? 16 : (avg > 64 ? 8 : 4);
     else
-        op.lanes_per_row = avg > 192 ? 64 : (avg > 96 ? 32 : (avg > 48 ? 16 : 8));
+        op.lanes_per_row = avg > 96 ? 16 : 8;
     // the build scratch is released when b goes out of scope: wait for the fill
     SEV_HIP(hipStreamSynchronize(st));
 }
@@ -1682,18 +1682,12 @@ void autotune_operator(SlabCsc &op, hipStream_t st) {
     t_prof = nullptr;  // keep the trial launches out of the kernel profile
     TuneEntry best;
     float best_ms = 1e30f;
-    const std::vector<int> lanes_opts = op.vtype == kValBF16 ? std::vector<int>{4, 8, 16, 32}
-                                                              : std::vector<int>{8, 16, 32, 64};
-    for (int g : lanes_opts) {
-        op.lanes_per_row = g;
-        const float ms = time_launches(st, [&] { launch_av(op, v.get(), nullptr, nullptr, y.get(), st); }, 3);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best.lanes = g;
-        }
-    }
-    op.lanes_per_row = best.lanes;
-    best_ms = 1e30f;
+    // The lanes per CSR row of A v keep the build's rule: they set how each
+    // row's entries are split into partial sums, so a choice made by timing
+    // would let the rounding, and so the results, differ between runs.  The
+    // column groups of A' w are timed: every column segment is still summed
+    // by one wavefront in the same order.
+    best.lanes = op.lanes_per_row;
     const int cap = (int)std::max<int64_t>(1, op.n / (4 * kWaves));
     for (int g : {1, 2, 3, 4, 6, 8, 12, 16}) {
         if (g > cap) break;
