@@ -302,6 +302,19 @@ __global__ void k_wtot(int64_t n, const int32_t *c, const u64 *k, u64 *wtot) {
     if (v < n) atomicAdd(&wtot[c[v]], k[v]);
 }
 
+// totals after a sweep from those before it (wtot_new holds a copy): only
+// the nodes that moved change them (exact in two's-complement integers)
+__global__ void k_wtot_moves(int64_t n, const int32_t *c, const int32_t *cnew, const u64 *k,
+                             u64 *wtot_new) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int32_t a = c[v], b = cnew[v];
+    if (a != b) {
+        atomicAdd(&wtot_new[a], (u64)0 - k[v]);
+        atomicAdd(&wtot_new[b], k[v]);
+    }
+}
+
 // weight of the edges inside clusters (sum_w_in without tot_self), one
 // edge per thread over a fixed grid
 constexpr int kInBlocks = 2048;
@@ -825,7 +838,11 @@ struct Louvain {
                 if (athresh >= kActAll && ++still >= 2) break;
             } else {
                 still = 0;
-                compute_wtot(g, cnew.get(), wtot_new.get());
+                SEV_HIP(hipMemcpyAsync(wtot_new.get(), wtot.get(), sizeof(u64) * (size_t)n,
+                                       hipMemcpyDeviceToDevice, st));
+                k_wtot_moves<<<grid_of(n), kLvThreads, 0, st>>>(n, c, cnew.get(), g.k.get(),
+                                                                wtot_new.get());
+                SEV_LAUNCH_CHECK();
                 const double qn = modularity(g, cnew.get(), wtot_new.get());
                 if (qn > q) {
                     SEV_HIP(hipMemcpyAsync(c, cnew.get(), sizeof(int32_t) * (size_t)n,
