@@ -3,12 +3,12 @@
 // group_singletons) of src/clustering.jl:59-104 with the modularity search of
 // src/modularity.jl:165-380.
 //
-// The reference moves one node at a time in a shuffled order.  Here every
-// node of a level chooses its best neighbouring cluster at once against the
-// same cluster totals; sweeps alternate between moves to larger and to
-// smaller cluster labels (two nodes can then never swap clusters in one
-// sweep), and a sweep that does not raise the modularity is discarded and
-// retried with a pseudo-random half of the nodes (down to 1/64).  The labels
+// The reference moves one node at a time in a shuffled order.  Here, in each
+// sweep, a keyed pseudo-random half of the nodes of a level choose their best
+// neighbouring cluster at once against the same cluster totals (moving half
+// the nodes keeps simultaneous moves from undoing each other); a sweep that
+// does not raise the modularity is discarded and retried with fewer nodes
+// (down to 1/64).  The labels
 // are permuted at random before each local-moving phase, which is what
 // differs between random starts.  The level structure (local moving,
 // renumbering, the reduced network, merging, and repetition while the gain
@@ -107,8 +107,8 @@ __device__ __forceinline__ u64 wave_sum_u64(u64 v) {
     return v;
 }
 
-// Whether node v may move in a sweep: all nodes at athresh = 2^24; fewer
-// (a keyed pseudo-random subset) after a sweep had to be discarded.
+// Whether node v may move in a sweep: a keyed pseudo-random subset of about
+// athresh / 2^24 of the nodes (all of them at athresh = 2^24).
 constexpr uint32_t kActAll = 1u << 24;
 __device__ __forceinline__ bool active(int64_t v, uint64_t akey, uint32_t athresh) {
     if (athresh >= kActAll) return true;
@@ -123,7 +123,7 @@ __device__ __forceinline__ bool active(int64_t v, uint64_t akey, uint32_t athres
 // kin - res * (w_tot * k) / totw, best_local_move (src/modularity.jl:243-253).
 // f = res * k / totw is formed once per node.
 struct MoveRule {
-    int cv, dir;
+    int cv;
     double f;
     __device__ __forceinline__ void consider(int C, u64 kin, const u64 *wtot, double &bs, int &bc,
                                              u64 &kcv) const {
@@ -131,7 +131,6 @@ struct MoveRule {
             kcv = kin;
             return;
         }
-        if (dir > 0 ? C < cv : C > cv) return;
         const double sc = (double)kin - (double)wtot[C] * f;
         best_merge(bs, bc, sc, C);
     }
@@ -149,7 +148,7 @@ template <int kCap>
 __global__ void __launch_bounds__(kLvThreads)
 k_move_light(int64_t n, const int32_t *list, const int64_t *ptr, const int32_t *adj,
              const u64 *w, const u64 *k, const int32_t *c, const u64 *wtot, int32_t *cnew,
-             double totw, double res, int dir, uint64_t akey, uint32_t athresh, int *moved) {
+             double totw, double res, uint64_t akey, uint32_t athresh, int *moved) {
     __shared__ int skey[kLvWaves][kCap];
     __shared__ u64 sval[kLvWaves][kCap];
     const int lane = lane_id(), wv = (int)(threadIdx.x >> 6);
@@ -192,7 +191,7 @@ k_move_light(int64_t n, const int32_t *list, const int64_t *ptr, const int32_t *
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const u64 kvi = k[v];
-    const MoveRule r{cv, dir, res * (double)kvi / totw};
+    const MoveRule r{cv, res * (double)kvi / totw};
     double bs = -INFINITY;
     int bc = INT_MAX;
     u64 kcv = 0;
@@ -214,8 +213,7 @@ k_move_light(int64_t n, const int32_t *list, const int64_t *ptr, const int32_t *
 __global__ void __launch_bounds__(kLvThreads)
 k_move_heavy(const int32_t *heavy, const int64_t *ptr, const int32_t *adj, const u64 *w,
              const u64 *k, const int32_t *c, const u64 *wtot, int32_t *cnew, int *hkeys,
-             u64 *hvals, double totw, double res, int dir, uint64_t akey, uint32_t athresh,
-             int *moved) {
+             u64 *hvals, double totw, double res, uint64_t akey, uint32_t athresh, int *moved) {
     __shared__ double rs[kLvWaves];
     __shared__ int rc[kLvWaves];
     __shared__ u64 rk[kLvWaves];
@@ -241,7 +239,7 @@ k_move_heavy(const int32_t *heavy, const int64_t *ptr, const int32_t *adj, const
     __syncthreads();
     const int cv = c[v];
     const u64 kvi = k[v];
-    const MoveRule r{cv, dir, res * (double)kvi / totw};
+    const MoveRule r{cv, res * (double)kvi / totw};
     double bs = -INFINITY;
     int bc = INT_MAX;
     u64 kcv = 0;
@@ -809,33 +807,32 @@ struct Louvain {
         compute_wtot(g, c, wtot.get());
         const double q0 = modularity(g, c, wtot.get());
         double q = q0, round_gain = 0.0;
-        uint32_t athresh = kActAll;
+        uint32_t athresh = kActAll >> 1;
         int still = 0;
         bool rejected = false;
         for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-            const int dir = (sweep & 1) ? -1 : 1;
             const uint64_t akey = seed * 0xD1B54A32D192ED03ull + (uint64_t)sweep;
             SEV_HIP(hipMemsetAsync(counter.get(), 0, sizeof(int), st));
             k_move_light<2 * kSmallDeg><<<grid_of(n, kLvWaves), kLvThreads, 0, st>>>(
                 n, nullptr, g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(),
-                cnew.get(), totw_d, res, dir, akey, athresh, counter.get());
+                cnew.get(), totw_d, res, akey, athresh, counter.get());
             if (g.nmedium > 0)
                 k_move_light<2 * kLightDeg><<<grid_of(g.nmedium, kLvWaves), kLvThreads, 0, st>>>(
                     g.nmedium, g.medium.get(), g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c,
-                    wtot.get(), cnew.get(), totw_d, res, dir, akey, athresh, counter.get());
+                    wtot.get(), cnew.get(), totw_d, res, akey, athresh, counter.get());
             if (g.nheavy > 0)
                 k_move_heavy<<<(unsigned)g.nheavy, kLvThreads, 0, st>>>(
                     g.heavy.get(), g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(),
-                    cnew.get(), hkeys.get(), hvals.get(), totw_d, res, dir, akey, athresh,
+                    cnew.get(), hkeys.get(), hvals.get(), totw_d, res, akey, athresh,
                     counter.get());
             SEV_LAUNCH_CHECK();
             int moved = 0;
             SEV_HIP(hipMemcpyAsync(&moved, counter.get(), sizeof moved, hipMemcpyDeviceToHost, st));
             SEV_HIP(hipStreamSynchronize(st));
             if (moved == 0) {
-                // nothing wants to move in this direction; two such sweeps in a
-                // row (both directions) end the phase
-                if (athresh >= kActAll && ++still >= 2) break;
+                // no chosen node wants to move; four such sweeps in a row end
+                // the phase
+                if (athresh >= (kActAll >> 1) && ++still >= 4) break;
             } else {
                 still = 0;
                 SEV_HIP(hipMemcpyAsync(wtot_new.get(), wtot.get(), sizeof(u64) * (size_t)n,
@@ -850,10 +847,10 @@ struct Louvain {
                     wtot.swap(wtot_new);
                     round_gain += qn - q;
                     q = qn;
-                    athresh = kActAll;
+                    athresh = kActAll >> 1;
                 } else {
-                    // simultaneous moves interfered: retry with a random half
-                    // of the nodes, down to 1/64 of them
+                    // simultaneous moves interfered: retry with half as many
+                    // nodes, down to 1/64 of them
                     rejected = true;
                     athresh >>= 1;
                     if (athresh < (kActAll >> 6)) break;
