@@ -1360,7 +1360,13 @@ SO_EXPORT double so_modularity(int64_t n, const int64_t *colptr, const int32_t *
  * first such cluster without an entry from the node wins.  Labels are then
  * renumbered by first occurrence (relabel!, src/utils.jl:91-106).  labels are
  * 0-based and compact; returns the new number of clusters, or -1 when no
- * cluster has more than one member (findmax over an empty set). */
+ * cluster has more than one member (findmax over an empty set).
+ * The node takes the cluster label that findmax(f, others) returns, as Julia
+ * 1.6 (the version the reference's CI pins, .github/workflows/CI.yml:12-14)
+ * defines findmax(f, domain) -> (f(x), x).  Julia 1.7 and later return the
+ * position in `others` instead, so under those versions src/clustering.jl:89-96
+ * assigns a position, not a label; that behaviour is not reproduced (parity
+ * unpinned for Julia >= 1.7). */
 SO_EXPORT int64_t so_group_singletons(int64_t n, const int64_t *colptr, const int32_t *row,
                                       const double *w, int64_t *labels)
 {

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -1396,6 +1397,8 @@ int sev_knn(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, int64_t 
             int include_self, int precision, int32_t *nn_index, void *distances,
             int64_t *n_fallback) {
     return guarded([&] {
+        SEV_REQUIRE(k >= 1 && k <= 64,
+                    "knn: k must be in [1, 64] (the longest neighbour list the search keeps)");
         ensure_device();
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
                         precision == SEV_PREC_EXACT || precision == SEV_PREC_FP8,
@@ -1415,6 +1418,8 @@ int sev_knn_cosine(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, i
                    int include_self, int precision, int32_t *nn_index, void *distances,
                    int64_t *n_fallback) {
     return guarded([&] {
+        SEV_REQUIRE(k >= 1 && k <= 64,
+                    "knn: k must be in [1, 64] (the longest neighbour list the search keeps)");
         ensure_device();
         SEV_REQUIRE(precision == SEV_PREC_FP32 || precision == SEV_PREC_BF16 ||
                         precision == SEV_PREC_EXACT || precision == SEV_PREC_FP8,
@@ -1430,36 +1435,57 @@ int sev_knn_cosine(const void *X, int64_t n, int64_t d, int64_t ld, int dtype, i
     });
 }
 
+// The libcell drop-ins have no status (src/neighbours.jl:32-74 reads the
+// outputs directly).  On failure the message goes to stderr, every index is
+// set to 0 (outside Julia's 1-based range, so the sparse() of
+// src/neighbours.jl:79 raises instead of consuming undefined memory) and the
+// distances to NaN.
+extern "C++" template <class T>
+static void dropin_knn_status(int rc, const char *name, int n, int k, int32_t *nn_index,
+                              T *distances) {
+    if (rc == SEV_OK) return;
+    std::fprintf(stderr, "[severo_hip] %s: %s\n", name, sev_last_error());
+    if (n > 0 && k > 0) {
+        const size_t cnt = (size_t)n * (size_t)k;
+        if (nn_index) std::fill(nn_index, nn_index + cnt, 0);
+        if (distances) std::fill(distances, distances + cnt, std::numeric_limits<T>::quiet_NaN());
+    }
+}
+
 void FindNeighboursEuclidean32(const float *X, int n, int d, int ld, int k, int ntables,
                                int include_self, int seed, int32_t *nn_index, float *distances) {
     (void)ntables;
     (void)seed;
-    (void)sev_knn(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32, nn_index, distances,
-                  nullptr);
+    dropin_knn_status(sev_knn(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32, nn_index,
+                              distances, nullptr),
+                      "FindNeighboursEuclidean32", n, k, nn_index, distances);
 }
 
 void FindNeighboursEuclidean64(const double *X, int n, int d, int ld, int k, int ntables,
                                int include_self, int seed, int32_t *nn_index, double *distances) {
     (void)ntables;
     (void)seed;
-    (void)sev_knn(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index, distances,
-                  nullptr);
+    dropin_knn_status(sev_knn(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index,
+                              distances, nullptr),
+                      "FindNeighboursEuclidean64", n, k, nn_index, distances);
 }
 
 void FindNeighboursCosine32(const float *X, int n, int d, int ld, int k, int ntables,
                             int include_self, int seed, int32_t *nn_index, float *distances) {
     (void)ntables;
     (void)seed;
-    (void)sev_knn_cosine(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32, nn_index,
-                         distances, nullptr);
+    dropin_knn_status(sev_knn_cosine(X, n, d, ld, SEV_FLOAT32, k, include_self, SEV_PREC_FP32,
+                                     nn_index, distances, nullptr),
+                      "FindNeighboursCosine32", n, k, nn_index, distances);
 }
 
 void FindNeighboursCosine64(const double *X, int n, int d, int ld, int k, int ntables,
                             int include_self, int seed, int32_t *nn_index, double *distances) {
     (void)ntables;
     (void)seed;
-    (void)sev_knn_cosine(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32, nn_index,
-                         distances, nullptr);
+    dropin_knn_status(sev_knn_cosine(X, n, d, ld, SEV_FLOAT64, k, include_self, SEV_PREC_FP32,
+                                     nn_index, distances, nullptr),
+                      "FindNeighboursCosine64", n, k, nn_index, distances);
 }
 
 int sev_cluster_louvain(int64_t n, const int64_t *colptr, const int64_t *rowval,

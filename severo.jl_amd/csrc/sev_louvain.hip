@@ -808,10 +808,13 @@ struct Louvain {
         const double q0 = modularity(g, c, wtot.get());
         double q = q0, round_gain = 0.0;
         uint32_t athresh = kActAll >> 1;
-        int still = 0;
         bool rejected = false;
+        // the active subsets differ between sweeps, and between the phases
+        // (levels and repeats) of one start
+        const uint64_t pkey = (seed ^ (0xC2B2AE3D27D4EB4Full * (uint64_t)calls)) *
+                              0xD1B54A32D192ED03ull;
         for (int sweep = 0; sweep < kMaxSweeps; ++sweep) {
-            const uint64_t akey = seed * 0xD1B54A32D192ED03ull + (uint64_t)sweep;
+            const uint64_t akey = pkey + (uint64_t)sweep;
             SEV_HIP(hipMemsetAsync(counter.get(), 0, sizeof(int), st));
             k_move_light<2 * kSmallDeg><<<grid_of(n, kLvWaves), kLvThreads, 0, st>>>(
                 n, nullptr, g.ptr.get(), g.adj.get(), g.w.get(), g.k.get(), c, wtot.get(),
@@ -829,12 +832,9 @@ struct Louvain {
             int moved = 0;
             SEV_HIP(hipMemcpyAsync(&moved, counter.get(), sizeof moved, hipMemcpyDeviceToHost, st));
             SEV_HIP(hipStreamSynchronize(st));
-            if (moved == 0) {
-                // no chosen node wants to move; four such sweeps in a row end
-                // the phase
-                if (athresh >= (kActAll >> 1) && ++still >= 4) break;
-            } else {
-                still = 0;
+            // a sweep in which no chosen node wants to move gains nothing; the
+            // pair test below then ends the phase
+            if (moved != 0) {
                 SEV_HIP(hipMemcpyAsync(wtot_new.get(), wtot.get(), sizeof(u64) * (size_t)n,
                                        hipMemcpyDeviceToDevice, st));
                 k_wtot_moves<<<grid_of(n), kLvThreads, 0, st>>>(n, c, cnew.get(), g.k.get(),

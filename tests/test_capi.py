@@ -63,3 +63,24 @@ def test_synth_tables_calibration(sev):
     f = sev.top_mean_features(t, 2000)
     assert len(set(f.tolist())) == 2000
     assert np.all(np.diff(t.gene_mean[f]) <= 0)
+
+
+@pytest.mark.parametrize("name,dt", [("FindNeighboursEuclidean32", np.float32),
+                                     ("FindNeighboursCosine64", np.float64)])
+@pytest.mark.parametrize("k", [65, 5])
+def test_dropin_knn_failure_is_flagged(sev, name, dt, k):
+    """The libcell-signature searches return nothing, so a failure (here k
+    beyond the longest list, or no device) leaves indices that Julia's
+    sparse() rejects (0) and NaN distances instead of undefined memory."""
+    import torch
+    L = sev.load()
+    n, d = 40, 3
+    X = np.asfortranarray(np.random.default_rng(0).standard_normal((n, d)).astype(dt))
+    idx = np.full((n, k), 7, np.int32, order="F")
+    dist = np.zeros((n, k), dt, order="F")
+    if k <= 64 and torch.cuda.is_available():
+        pytest.skip("a GPU is present: the k <= 64 call succeeds")
+    getattr(L, name)(X.ctypes.data, n, d, n, k, 0, 1, 0, idx.ctypes.data, dist.ctypes.data)
+    assert np.all(idx == 0) and np.all(np.isnan(dist))
+    if k > 64:
+        assert b"k must be in [1, 64]" in L.sev_last_error()

@@ -92,6 +92,91 @@ def test_normalize_operation_order(orc):
 
 
 # ---------------------------------------------------------------------------
+# column selection Y[:, features]  (docs/src/pbmc.md:121, src/scaling.jl:350-352;
+# the order of the features is the order of src/variablefeatures.jl:159)
+# ---------------------------------------------------------------------------
+def _sparse_with_empty_columns(seed, m=300, n=60):
+    rng = np.random.default_rng(seed)
+    X = sp.random(m, n, density=0.1, format="csc", random_state=rng, dtype=np.float32)
+    X.data[:] = rng.uniform(0.5, 5.0, X.nnz).astype(np.float32)
+    X = X.tolil()
+    for j in (0, 7, 31, n - 1):
+        X[:, j] = 0
+    X = X.tocsc()
+    X.sort_indices()
+    return X
+
+
+@pytest.mark.parametrize("order", ["unsorted", "descending", "single", "empty_cols", "none"])
+def test_gather_vs_scipy_column_indexing(orc, order):
+    X = _sparse_with_empty_columns(3)
+    n = X.shape[1]
+    feats = {
+        "unsorted": np.random.default_rng(9).permutation(n)[:25],
+        "descending": np.arange(n - 1, -1, -3),
+        "single": np.array([17]),
+        "empty_cols": np.array([31, 2, 0, n - 1, 7, 40]),
+        "none": np.array([], np.int64),
+    }[order]
+    cp, rows, vals = orc.gather(feats, X.indptr, X.indices, X.data)
+    ref = X[:, feats]  # scipy's own column indexing keeps the given order
+    ref = ref.tocsc()
+    ref.sort_indices()
+    assert np.array_equal(cp, ref.indptr)
+    assert np.array_equal(rows, ref.indices) and np.array_equal(vals, ref.data)
+    # dense cross-check through a second independent route
+    D = X.toarray()
+    G = sp.csc_matrix((vals, rows, cp), shape=(X.shape[0], len(feats))).toarray()
+    assert np.array_equal(G, D[:, feats])
+
+
+def test_gather_planted_mistakes_differ(orc):
+    X = _sparse_with_empty_columns(4)
+    feats = np.array([40, 3, 22, 9, 55, 1])
+    cp, rows, vals = orc.gather(feats, X.indptr, X.indices, X.data)
+    ref = X[:, feats].toarray()
+    G = sp.csc_matrix((vals, rows, cp), shape=(X.shape[0], len(feats))).toarray()
+    assert np.array_equal(G, ref)
+    # a gather that sorts the features, or that is off by one gene, fails
+    for wrong in (np.sort(feats), feats + 1):
+        cpw, rw, vw = orc.gather(wrong, X.indptr, X.indices, X.data)
+        Gw = sp.csc_matrix((vw, rw, cpw), shape=(X.shape[0], len(feats))).toarray()
+        assert not np.array_equal(Gw, ref)
+
+
+def _brute_top(metric, nfeatures):
+    """Selection by repeated scans: the largest remaining value, the lowest
+    index among equal values (partialsortperm(metric, 1:n, rev=true) with
+    ties in index order)."""
+    left = list(range(len(metric)))
+    out = []
+    for _ in range(nfeatures):
+        best = left[0]
+        for i in left[1:]:
+            if metric[i] > metric[best]:
+                best = i
+        out.append(best)
+        left.remove(best)
+    return np.array(out, np.int64)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_top_features_vs_brute_force_with_ties(orc, seed):
+    rng = np.random.default_rng(seed)
+    metric = rng.integers(0, 12, 150).astype(np.float64)  # many planted ties
+    metric[rng.choice(150, 10, replace=False)] = 0.0
+    for nf in (1, 7, 40, 150):
+        got = orc.top_features(metric, nf)
+        assert np.array_equal(got, _brute_top(metric, nf))
+    # a selection that breaks ties towards the higher index differs
+    rev = np.lexsort((-np.arange(150), -metric))[:40]
+    assert not np.array_equal(rev, _brute_top(metric, 40))
+    # continuous values: the order equals a plain descending sort
+    v = rng.standard_normal(300)
+    assert np.array_equal(orc.top_features(v, 50), np.argsort(-v, kind="stable")[:50])
+
+
+# ---------------------------------------------------------------------------
 # scale_features  (src/scaling.jl)
 # ---------------------------------------------------------------------------
 def test_scale_golden_closed_form(orc, golden):
