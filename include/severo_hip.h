@@ -124,7 +124,11 @@ int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const in
  * entries / columns of s, U, V (src/irlba.jl:87-99).  The Lanczos vectors,
  * reorthogonalisation and restarts run on the GPU; every product copies one
  * vector to the host and back.  Returns 0, or a nonzero status where the
- * reference raises "convergence failed". */
+ * reference raises "convergence failed"; the cause is also printed to stderr.
+ * Limits: work <= 128 (nu <= 121 with the reference's work = nu + 7).  The
+ * integer arguments are 64-bit: this binds an ILP64 Severo build (BlasInt =
+ * Int64, deps/build.jl:17-57); an LP64 build passes 32-bit BlasInt and needs
+ * its ccall argument types widened to Int64 (the values are unchanged). */
 typedef void (*sev_irlba_randv_fn)(double *y, int32_t n, void *data);
 typedef void (*sev_irlba_matmul_fn)(double *y, char trans, double *x, double *temp, void *data);
 int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t restart,
@@ -299,9 +303,11 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
                             float *mu);
 
 /* Stage 3: embedding(S, npcs; method=:pca, algorithm=:irlba)
- * (src/embedding.jl:46-94): IRLBA with work = npcs + 7, then
- * Z = U diag(s) kept on the device in Float32 for the neighbour search. */
-int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+ * (src/embedding.jl:46-94): IRLBA, then Z = U diag(s) kept on the device in
+ * Float32 for the neighbour search.  work is the number of Lanczos vectors:
+ * 0 takes the reference's npcs + 7 (src/irlba.jl:50); a larger basis (at
+ * most 128) converges to the same tolerance in fewer restarts. */
+int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit, double tol,
                      const double *init, int precision, int64_t *iters, int64_t *nmult);
 /* Batch form: one IRLBA per sample (src/irlba.jl:47-85 on the sample's
  * centered matrix), all run in lockstep over the Lanczos steps with BF16

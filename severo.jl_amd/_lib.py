@@ -70,7 +70,7 @@ SIGNATURES = {
     "sev_pipeline_select_scale_batch": (_int, [_p, _i64, _p, _dbl]),
     "sev_pipeline_scaled_nnz": (_int, [_p, _pi64]),
     "sev_pipeline_get_scaled": (_int, [_p, _p, _p, _p, _p]),
-    "sev_pipeline_pca": (_int, [_p, _i64, _i64, _dbl, _p, _int, _pi64, _pi64]),
+    "sev_pipeline_pca": (_int, [_p, _i64, _i64, _i64, _dbl, _p, _int, _pi64, _pi64]),
     "sev_pipeline_pca_batch": (_int, [_p, _i64, _i64, _dbl, _p, _int, _p, _pi64]),
     "sev_pipeline_get_pca": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_set_embedding": (_int, [_p, _i64, _i64, _p, _i64]),

@@ -73,6 +73,16 @@ void ensure_device() {
     ok = true;
 }
 
+}  // namespace
+
+int sev::guarded_call(void (*fn)(void *), void *ctx) {
+    return guarded([&] { fn(ctx); });
+}
+
+void sev::require_device() { ensure_device(); }
+
+namespace {
+
 // Host-to-device transfer of large arrays through two reusable pinned
 // buffers: host threads fill one (converting as they copy) while the DMA of
 // the other runs.  The buffers are allocated at the first large transfer and
@@ -750,8 +760,8 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
     });
 }
 
-int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, const double *init,
-                     int precision, int64_t *iters, int64_t *nmult) {
+int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit, double tol,
+                     const double *init, int precision, int64_t *iters, int64_t *nmult) {
     return guarded([&] {
         ProfInstall pi_(&p->prof);
         SEV_REQUIRE(p->have_scaled, "scale has not run");
@@ -765,9 +775,11 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol, c
         build_slab_csc(p->op, m, n, p->fcp.get(), p->frow.get(), p->bval.get(), false,
                        p->mu.get(), precision == SEV_PREC_BF16, st);
         autotune_operator(p->op, st);
+        SEV_REQUIRE(work == 0 || (work > nu && work <= 128),
+                    "work must be 0 (npcs + 7) or in (npcs, 128]");
         IrlbaParams prm;
         prm.nu = nu;
-        prm.work = nu + 7;
+        prm.work = work > 0 ? work : nu + 7;
         prm.maxit = maxit;
         prm.tol = tol;
         prm.svtol = tol;
@@ -1293,7 +1305,9 @@ int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const in
 int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t restart,
           double tol, double *init, double *s, double *U, double *V, sev_irlba_randv_fn randv,
           sev_irlba_matmul_fn matmul, void *data) {
-    return guarded([&] {
+    // src/irlba.jl:73 turns any nonzero status into error("convergence
+    // failed"); the actual cause goes to stderr
+    const int rc = guarded([&] {
         ensure_device();
         SEV_REQUIRE(m >= 1 && n >= 1 && nu >= 1, "irlba: empty operator or nu < 1");
         SEV_REQUIRE(matmul && s && U && V, "irlba: NULL matmul or output");
@@ -1339,6 +1353,8 @@ int irlba(int64_t m, int64_t n, int64_t nu, int64_t work, int64_t maxit, int64_t
         download(V, dV.get(), (size_t)(n * nu), st.st);
         SEV_HIP(hipStreamSynchronize(st.st));
     });
+    if (rc != SEV_OK) std::fprintf(stderr, "[severo_hip] irlba: %s\n", sev_last_error());
+    return rc;
 }
 
 int sev_find_variable_features(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,

@@ -40,7 +40,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
-constexpr int kMaxWork = 64;
+constexpr int kMaxWork = 128;  // Lanczos vectors (single operator)
+constexpr int kMaxWorkB = 64;  // Lanczos vectors per sample of a batch
 constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v, FP64)
 constexpr int64_t kMaxFL32 = 14336;  // the same for the FP32 copy of the BF16 form
 constexpr int kRedChunks = 16;       // first-level chunks of the slab partials
@@ -800,31 +801,37 @@ k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
         dst[r] = src[r] / S;
 }
 
-// Y[:, c] = X[:, 0:work] Cf[:, c] for c < kout (X, Y column-major, ld = m),
-// computed as the 16x16 tiles of D = Cf' X' on v_mfma_f64_16x16x4_f64: one
-// wavefront per 32 rows (two 16-row tiles sharing every A operand), lanes
-// (n = row, k-group) read 16 consecutive rows of one column of X, and each
-// lane writes 16 consecutive rows of an output column.  In this instruction
-// lane l holds D[l/16 + 4i][l%16], i = 0..3.  NS = ceil(work / 4) k-steps
-// (the launch picks the instantiation): the whole 32 x work slice of X is
-// loaded before the MFMA chain, so its 2 NS loads per lane are in flight
-// together.  Optionally Y[:, kout] = extra / scal[2] (the normalised
-// residual).
+// Y[:, c0 + c] = X[:, 0:work] Cf[:, c0 + c] for c < kout (X, Y column-major,
+// ld = m), computed as the 16x16 tiles of D = Cf' X' on
+// v_mfma_f64_16x16x4_f64: one wavefront per 32 rows (two 16-row tiles sharing
+// every A operand), lanes (n = row, k-group) read 16 consecutive rows of one
+// column of X, and each lane writes 16 consecutive rows of an output column.
+// In this instruction lane l holds D[l/16 + 4i][l%16], i = 0..3.  NS =
+// ceil(work / 4) k-steps (the launch picks the instantiation): the whole
+// 32 x work slice of X is loaded before the MFMA chain, so its 2 NS loads per
+// lane are in flight together.  One launch covers at most gemm_cols(NS)
+// output columns (their coefficients sit in LDS, rows padded to an odd
+// stride); wider outputs take several launches.  Optionally Y[:, c0 + kout] =
+// extra / scal[2] (the normalised residual).
+__host__ __device__ constexpr int gemm_cols(int ns) { return ns <= 31 ? 64 : 48; }
+
 template <int NS>
 __global__ void __launch_bounds__(kBlock)
-k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, double *Y,
+k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int c0, int kout, double *Y,
             const double *extra, const double *scal) {
-    constexpr int LD = kMaxWork + 1;
-    __shared__ double cfl[kMaxWork * LD];  // cfl[c*LD + l] = Cf[c*work + l], zero padded
-    for (int i = threadIdx.x; i < kMaxWork * kMaxWork; i += kBlock) {
-        const int c = i / kMaxWork, l = i % kMaxWork;
-        cfl[c * LD + l] = (c < kout && l < work) ? Cf[c * work + l] : 0.0;
+    constexpr int LD = 4 * NS + 1;
+    constexpr int GC = gemm_cols(NS);
+    __shared__ double cfl[GC * LD];  // cfl[c*LD + l] = Cf[(c0 + c)*work + l], zero padded
+    for (int i = threadIdx.x; i < GC * (LD - 1); i += kBlock) {
+        const int c = i / (LD - 1), l = i % (LD - 1);
+        cfl[c * LD + l] = (c < kout && l < work) ? Cf[(int64_t)(c0 + c) * work + l] : 0.0;
     }
     __syncthreads();
     const int w = wave_index(), lane = lane_id();
     const int nr = lane & 15, kq = lane >> 4;
     const int ncb = (kout + 15) / 16;
     const int64_t nrb = (m + 31) / 32;
+    double *Yc = Y + (int64_t)c0 * m;
     for (int64_t rb = (int64_t)blockIdx.x * kWaves + w; rb < nrb;
          rb += (int64_t)gridDim.x * kWaves) {
         const int64_t row0 = rb * 32 + nr, row1 = row0 + 16;
@@ -840,7 +847,7 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
             bv1[s] = X[kc + rc1] * ((k < work && row1 < m) ? 1.0 : 0.0);
         }
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
+        for (int cb = 0; cb < GC / 16; ++cb) {
             if (cb >= ncb) break;
             v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -853,14 +860,14 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int kout, do
             for (int i = 0; i < 4; ++i) {
                 const int c = cb * 16 + kq + 4 * i;
                 if (c < kout) {
-                    if (row0 < m) Y[(int64_t)c * m + row0] = acc0[i];
-                    if (row1 < m) Y[(int64_t)c * m + row1] = acc1[i];
+                    if (row0 < m) Yc[(int64_t)c * m + row0] = acc0[i];
+                    if (row1 < m) Yc[(int64_t)c * m + row1] = acc1[i];
                 }
             }
         }
         if (extra && kq == 0) {
-            if (row0 < m) Y[(int64_t)kout * m + row0] = extra[row0] / scal[2];
-            if (row1 < m) Y[(int64_t)kout * m + row1] = extra[row1] / scal[2];
+            if (row0 < m) Yc[(int64_t)kout * m + row0] = extra[row0] / scal[2];
+            if (row1 < m) Yc[(int64_t)kout * m + row1] = extra[row1] / scal[2];
         }
     }
 }
@@ -1258,11 +1265,158 @@ k_small_svd8(int n, const double *B, double *U, double *s, double *V, int *statu
     small_svd8_body(n, B, U, s, V, status);
 }
 
+// The same SVD for 64 < work <= kMaxWork: 64 groups of 16 lanes (one column
+// pair each, circle schedule), lane l of a group holding rows l + 16e (e < 8)
+// of its two columns; only G = B' sits in LDS (n x n FP64), so U = B V
+// diag(1/s) reads B from global memory.  When B is too ill-conditioned for
+// that product (s_min < 1e-5 s_max) the kernel sets status[0] and the caller
+// redoes the SVD on the host (the accumulated rotations would need a second
+// n x n array).
+constexpr int kSvdBigThreads = 1024;
+
+__global__ void __launch_bounds__(kSvdBigThreads)
+k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *status) {
+    extern __shared__ double svdb_lds[];
+    double *G = svdb_lds;  // G(i, c) = B(c, i) at G[c * n + i]
+    __shared__ int rotated, nsweeps, ill;
+    __shared__ double sv[kMaxWork];
+    const int tid = threadIdx.x;
+    const int ne = n + (n & 1);
+    const int npair = ne / 2;
+    const int pr = tid >> 4, gl = tid & 15;
+    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    if (tid == 0) {
+        nsweeps = 0;
+        ill = 0;
+    }
+    for (int e = tid; e < n * n; e += kSvdBigThreads) {
+        const int i = e % n, c = e / n;
+        G[e] = B[(int64_t)i * n + c];
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        if (tid == 0) rotated = 0;
+        __syncthreads();
+        // circle schedule: round r pairs (r + pr, r - pr) mod (ne - 1), group
+        // 0 pairs r with the fixed column ne - 1
+        int pp = pr, qq = ne - 1 - pr;
+        for (int r = 0; r < ne - 1; ++r) {
+            if (pr < npair) {
+                const int p = pp < qq ? pp : qq, q = pp < qq ? qq : pp;
+                const bool valid = q < n;
+                double *gp = G + p * n, *gq = G + (valid ? q : p) * n;
+                double x[8], y[8];
+                double app = 0.0, aqq = 0.0, apq = 0.0;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int i = gl + 16 * e;
+                    const bool in = valid && i < n;
+                    x[e] = in ? gp[i] : 0.0;
+                    y[e] = in ? gq[i] : 0.0;
+                    app = fma(x[e], x[e], app);
+                    aqq = fma(y[e], y[e], aqq);
+                    apq = fma(x[e], y[e], apq);
+                }
+                app = sum16(app);
+                aqq = sum16(aqq);
+                apq = sum16(apq);
+                if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
+                    if (gl == 0) rotated = 1;
+                    // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2)
+                    const double zeta = (aqq - app) * rcp_nr(2.0 * apq);
+                    const double az = fabs(zeta);
+                    double t;
+                    if (az > 1e150) {
+                        t = 0.5 * rcp_nr(zeta);
+                    } else {
+                        const double xx = fma(az, az, 1.0);
+                        t = copysign(rcp_nr(az + xx * rsq_nr(xx)), zeta);
+                    }
+                    const double c = rsq_nr(fma(t, t, 1.0));
+                    const double sn = c * t;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int i = gl + 16 * e;
+                        if (i < n) {
+                            gp[i] = c * x[e] - sn * y[e];
+                            gq[i] = sn * x[e] + c * y[e];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            pp = pp + 1 == ne - 1 ? 0 : pp + 1;
+            if (pr > 0) qq = qq + 1 == ne - 1 ? 0 : qq + 1;
+        }
+        const int any = rotated;
+        __syncthreads();
+        if (tid == 0) nsweeps = nsweeps + 1;
+        if (!any) break;
+    }
+    if (tid < n) {
+        double t = 0.0;
+        for (int i = 0; i < n; ++i) t += G[tid * n + i] * G[tid * n + i];
+        sv[tid] = sqrt(t);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double mx = 0.0, mn = __builtin_huge_val();
+        for (int i = 0; i < n; ++i) {
+            mx = fmax(mx, sv[i]);
+            mn = fmin(mn, sv[i]);
+        }
+        ill = !(mn >= 1e-5 * mx);
+        status[0] = ill;
+        status[1] = nsweeps;
+    }
+    __syncthreads();
+    if (ill) return;
+    for (int j = tid >> 6; j < n; j += kSvdBigThreads / 64) {
+        const int l = tid & 63;
+        const double sj = sv[j];
+        int rk = 0;
+        for (int i = 0; i < n; ++i) rk += (sv[i] > sj || (sv[i] == sj && i < j)) ? 1 : 0;
+        if (l == 0) s[rk] = sj;
+        for (int i = l; i < n; i += 64) {
+            // (B g_j)_i / s_j^2, g_j = s_j v_j the column of G
+            double t = 0.0;
+            for (int c = 0; c < n; ++c) t = fma(B[(int64_t)c * n + i], G[j * n + c], t);
+            U[(int64_t)rk * n + i] = t / (sj * sj);
+            V[(int64_t)rk * n + i] = G[j * n + i] / sj;
+        }
+    }
+}
+
 // LDS bytes and launch of the projected SVD for work columns
 size_t svd_lds_bytes(int work) {
     const size_t ne = (size_t)(work + (work & 1));
-    return work <= kSvd8MaxWork ? sizeof(double) * 2 * ne * kSvdLd
-                                : sizeof(double) * 2 * (size_t)work * (size_t)work;
+    if (work <= kSvd8MaxWork) return sizeof(double) * 2 * ne * kSvdLd;
+    if (work <= 64) return sizeof(double) * 2 * (size_t)work * (size_t)work;
+    return sizeof(double) * (size_t)work * (size_t)work;
+}
+
+// The projected SVD of the single-operator IRLBA for any work <= kMaxWork.
+void launch_small_svd(int work, const double *B, double *BU, double *BS, double *BV, int *status,
+                      hipStream_t st) {
+    const size_t lds = svd_lds_bytes(work);
+    if (work <= kSvd8MaxWork) {
+        hipLaunchKernelGGL(k_small_svd8, dim3(1), dim3(kSvd8Threads), lds, st, work, B, BU, BS,
+                           BV, status);
+    } else if (work <= 64) {
+        hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kSvdThreads), lds, st, work, B, BU, BS, BV,
+                           status);
+    } else {
+        static bool attr = false;
+        if (!attr) {
+            SEV_HIP(hipFuncSetAttribute((const void *)k_small_svd_big,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(sizeof(double) * kMaxWork * kMaxWork)));
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_small_svd_big, dim3(1), dim3(kSvdBigThreads), lds, st, work, B, BU,
+                           BS, BV, status);
+    }
+    SEV_LAUNCH_CHECK();
 }
 
 // --------------------------------------------------------------------------
@@ -1287,17 +1441,22 @@ __device__ __forceinline__ void restart_check_body(int work, int nu, double tol,
         double RF = scal[2];
         const double RFn = RF;
         if (RF < 2 * eps) RF = 0.0;
-        double bs = lane < work ? BS[lane] : 0.0;
-        double smax = bs;
+        // lane l looks after Ritz values l and l + 64
+        const int i0 = lane, i1 = lane + kWave;
+        const double bs0 = i0 < work ? BS[i0] : 0.0, bs1 = i1 < work ? BS[i1] : 0.0;
+        double smax = fmax(bs0, bs1);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) smax = fmax(smax, __shfl_xor(smax, o, kWave));
-        bool conv = false;
-        if (lane < work) {
-            const double r = RF * BU[(int64_t)lane * work + (work - 1)];
-            const double ratio = fabs(svr[lane] - bs) / bs;
-            conv = fabs(r) < tol * smax && ratio < svtol;
+        bool conv0 = false, conv1 = false;
+        if (i0 < work) {
+            const double r = RF * BU[(int64_t)i0 * work + (work - 1)];
+            conv0 = fabs(r) < tol * smax && fabs(svr[i0] - bs0) / bs0 < svtol;
         }
-        const int nconv = __popcll(__ballot(conv));
+        if (i1 < work) {
+            const double r = RF * BU[(int64_t)i1 * work + (work - 1)];
+            conv1 = fabs(r) < tol * smax && fabs(svr[i1] - bs1) / bs1 < svtol;
+        }
+        const int nconv = __popcll(__ballot(conv0)) + __popcll(__ballot(conv1));
         int go = 0, k = info[1];
         if (!st && nconv < nu && !last) {
             go = 1;
@@ -1590,30 +1749,50 @@ void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, 
     SEV_LAUNCH_CHECK();
 }
 
-// Calls f(std::integral_constant<int, ns>) for ns = nks in 1..kMaxWork/4.
-template <class F>
+// Calls f(std::integral_constant<int, ns>) for ns = nks in 1..MAXNS.
+template <int MAXNS, class F>
 void switch_steps(int nks, F &&f) {
+    static_assert(MAXNS <= 32, "instantiated up to 32 k-steps");
+    if (nks < 1 || nks > MAXNS) throw std::runtime_error("restart product: work size out of range");
     switch (nks) {
-    case 1: f(std::integral_constant<int, 1>()); break;
-    case 2: f(std::integral_constant<int, 2>()); break;
-    case 3: f(std::integral_constant<int, 3>()); break;
-    case 4: f(std::integral_constant<int, 4>()); break;
-    case 5: f(std::integral_constant<int, 5>()); break;
-    case 6: f(std::integral_constant<int, 6>()); break;
-    case 7: f(std::integral_constant<int, 7>()); break;
-    case 8: f(std::integral_constant<int, 8>()); break;
-    case 9: f(std::integral_constant<int, 9>()); break;
-    case 10: f(std::integral_constant<int, 10>()); break;
-    case 11: f(std::integral_constant<int, 11>()); break;
-    case 12: f(std::integral_constant<int, 12>()); break;
-    case 13: f(std::integral_constant<int, 13>()); break;
-    case 14: f(std::integral_constant<int, 14>()); break;
-    case 15: f(std::integral_constant<int, 15>()); break;
-    case 16: f(std::integral_constant<int, 16>()); break;
-    default: throw std::runtime_error("restart product: work size out of range");
+    case 1: if constexpr (1 <= MAXNS) f(std::integral_constant<int, 1>()); break;
+    case 2: if constexpr (2 <= MAXNS) f(std::integral_constant<int, 2>()); break;
+    case 3: if constexpr (3 <= MAXNS) f(std::integral_constant<int, 3>()); break;
+    case 4: if constexpr (4 <= MAXNS) f(std::integral_constant<int, 4>()); break;
+    case 5: if constexpr (5 <= MAXNS) f(std::integral_constant<int, 5>()); break;
+    case 6: if constexpr (6 <= MAXNS) f(std::integral_constant<int, 6>()); break;
+    case 7: if constexpr (7 <= MAXNS) f(std::integral_constant<int, 7>()); break;
+    case 8: if constexpr (8 <= MAXNS) f(std::integral_constant<int, 8>()); break;
+    case 9: if constexpr (9 <= MAXNS) f(std::integral_constant<int, 9>()); break;
+    case 10: if constexpr (10 <= MAXNS) f(std::integral_constant<int, 10>()); break;
+    case 11: if constexpr (11 <= MAXNS) f(std::integral_constant<int, 11>()); break;
+    case 12: if constexpr (12 <= MAXNS) f(std::integral_constant<int, 12>()); break;
+    case 13: if constexpr (13 <= MAXNS) f(std::integral_constant<int, 13>()); break;
+    case 14: if constexpr (14 <= MAXNS) f(std::integral_constant<int, 14>()); break;
+    case 15: if constexpr (15 <= MAXNS) f(std::integral_constant<int, 15>()); break;
+    case 16: if constexpr (16 <= MAXNS) f(std::integral_constant<int, 16>()); break;
+    case 17: if constexpr (17 <= MAXNS) f(std::integral_constant<int, 17>()); break;
+    case 18: if constexpr (18 <= MAXNS) f(std::integral_constant<int, 18>()); break;
+    case 19: if constexpr (19 <= MAXNS) f(std::integral_constant<int, 19>()); break;
+    case 20: if constexpr (20 <= MAXNS) f(std::integral_constant<int, 20>()); break;
+    case 21: if constexpr (21 <= MAXNS) f(std::integral_constant<int, 21>()); break;
+    case 22: if constexpr (22 <= MAXNS) f(std::integral_constant<int, 22>()); break;
+    case 23: if constexpr (23 <= MAXNS) f(std::integral_constant<int, 23>()); break;
+    case 24: if constexpr (24 <= MAXNS) f(std::integral_constant<int, 24>()); break;
+    case 25: if constexpr (25 <= MAXNS) f(std::integral_constant<int, 25>()); break;
+    case 26: if constexpr (26 <= MAXNS) f(std::integral_constant<int, 26>()); break;
+    case 27: if constexpr (27 <= MAXNS) f(std::integral_constant<int, 27>()); break;
+    case 28: if constexpr (28 <= MAXNS) f(std::integral_constant<int, 28>()); break;
+    case 29: if constexpr (29 <= MAXNS) f(std::integral_constant<int, 29>()); break;
+    case 30: if constexpr (30 <= MAXNS) f(std::integral_constant<int, 30>()); break;
+    case 31: if constexpr (31 <= MAXNS) f(std::integral_constant<int, 31>()); break;
+    case 32: if constexpr (32 <= MAXNS) f(std::integral_constant<int, 32>()); break;
+    default: break;
     }
 }
 
+// Y[:, 0:kout] = X[:, 0:work] Cf[:, 0:kout] in launches of at most
+// gemm_cols(NS) columns (extra, when given, follows the last column).
 void launch_gemm(const char *name, int64_t m, int work, const double *X, const double *Cf,
                  int kout, double *Y, const double *extra, const double *scal, hipStream_t st) {
     ProfScope ps(name, st, 8.0 * m * (work + kout), 2.0 * m * work * kout);
@@ -1621,10 +1800,16 @@ void launch_gemm(const char *name, int64_t m, int work, const double *X, const d
                                                       (int64_t)device_cu_count() * 8);
     const int nks = (work + 3) / 4;
     auto go = [&](auto ns) {
-        hipLaunchKernelGGL(k_gemm_mfma<decltype(ns)::value>, dim3(grid), dim3(kBlock), 0, st, m,
-                           work, X, Cf, kout, Y, extra, scal);
+        constexpr int NS = decltype(ns)::value;
+        constexpr int GC = gemm_cols(NS);
+        for (int c0 = 0; c0 < kout; c0 += GC) {
+            const int kc = kout - c0 < GC ? kout - c0 : GC;
+            const bool last = c0 + GC >= kout;
+            hipLaunchKernelGGL(k_gemm_mfma<NS>, dim3(grid), dim3(kBlock), 0, st, m, work, X, Cf,
+                               c0, kc, Y, last ? extra : nullptr, scal);
+        }
     };
-    switch_steps(nks, go);
+    switch_steps<kMaxWork / 4>(nks, go);
     SEV_LAUNCH_CHECK();
 }
 
@@ -1719,7 +1904,7 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     if (workl > n) workl = n;
     const int work = (int)workl;
     SEV_REQUIRE(nu >= 1 && nu <= work, "irlba: need 1 <= nu <= min(m, n)");
-    SEV_REQUIRE(work <= kMaxWork, "irlba: work (nu + 7) exceeds 64 on the device path");
+    SEV_REQUIRE(work <= kMaxWork, "irlba: work (nu + 7) exceeds 128 Lanczos vectors, the most the device path keeps (nu <= 121)");
     const double eps = 2.220446049250313e-16;
 
     w.W.ensure((size_t)(m * work));
@@ -1879,15 +2064,8 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         // reads the decision (converged, k) back
         {
             ProfScope ps("small_svd", st);
-            if (work <= kSvd8MaxWork)
-                hipLaunchKernelGGL(k_small_svd8, dim3(1), dim3(kSvd8Threads), svd_lds, st, work,
-                                   w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(),
-                                   w.info.get() + 8);
-            else
-                hipLaunchKernelGGL(k_small_svd, dim3(1), dim3(kSvdThreads), svd_lds, st, work,
-                                   w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(),
-                                   w.info.get() + 8);
-            SEV_LAUNCH_CHECK();
+            launch_small_svd(work, w.B.get(), w.BU.get(), w.BS.get(), w.BV.get(),
+                             w.info.get() + 8, st);
         }
         const int last = iter + 1 >= p.maxit ? 1 : 0;
         hipLaunchKernelGGL(k_restart_check, dim3(1), dim3(kBlock), 0, st, work, (int)nu, p.tol,
@@ -2370,8 +2548,8 @@ __global__ void __launch_bounds__(kBlock)
 k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
               int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
               bool restart) {
-    constexpr int LD = kMaxWork + 1;
-    __shared__ double cfl[kMaxWork * LD];
+    constexpr int LD = kMaxWorkB + 1;
+    __shared__ double cfl[kMaxWorkB * LD];
     const int work = d.work;
     int s;
     int64_t rb0, re0, ld;
@@ -2398,8 +2576,8 @@ k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *
     }
     const int kout = kfix > 0 ? kfix : d.j0[s];
     const double *Cf = Call + (int64_t)s * work * work;
-    for (int i = threadIdx.x; i < kMaxWork * kMaxWork; i += kBlock) {
-        const int c = i / kMaxWork, l = i % kMaxWork;
+    for (int i = threadIdx.x; i < kMaxWorkB * kMaxWorkB; i += kBlock) {
+        const int c = i / kMaxWorkB, l = i % kMaxWorkB;
         cfl[c * LD + l] = (c < kout && l < work) ? Cf[c * work + l] : 0.0;
     }
     __syncthreads();
@@ -2587,7 +2765,7 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     if (workl > n) workl = n;
     const int work = (int)workl;
     SEV_REQUIRE(nu >= 1 && nu <= work, "batched irlba: need 1 <= nu <= min(cells, features)");
-    SEV_REQUIRE(work <= kMaxWork, "batched irlba: work exceeds 64");
+    SEV_REQUIRE(work <= kMaxWorkB, "batched irlba: work exceeds 64");
     const int64_t nslab = (int64_t)L.slab_sample.size();
     const int nblk_rs = (int)ceil_div(n, kRsBlock);
     w.W[0].ensure((size_t)(M * work));
@@ -2773,7 +2951,7 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                            w.B.get(), w.info.get(), w.flags.get());
         SEV_LAUNCH_CHECK();
         // restart of the continuing samples into the other buffers
-        switch_steps((work + 3) / 4, [&](auto ns) {
+        switch_steps<kMaxWorkB / 4>((work + 3) / 4, [&](auto ns) {
             constexpr int NS = decltype(ns)::value;
             hipLaunchKernelGGL(k_b_gemm_mfma<NS>, dim3((unsigned)ceil_div(n, 256), (unsigned)S),
                                dim3(kBlock), 0, st, d, 1, Vc, Vc, (const int *)nullptr, Vn,

@@ -12,6 +12,18 @@
 
 namespace sev {
 
+// ---- sev_capi.cpp ---------------------------------------------------------
+// Runs fn(ctx) and turns a thrown error into the status and message the C
+// ABI reports (sev_last_error).
+int guarded_call(void (*fn)(void *), void *ctx);
+// Throws unless a gfx950 device is current.
+void require_device();
+
+// ---- sev_noalloc.hip -------------------------------------------------------
+// Resolves the device address of the allocation-free entry points' static
+// scratch array on the current device (once per device).
+void noalloc_setup();
+
 // ---- sev_scan.hip --------------------------------------------------------
 // out[i] = sum_{j<i} in[j] for i in [0, n]; out has n+1 entries.  Uses the
 // caller-provided scratch of scan_scratch_elems(n) int64 values.  in may
@@ -47,6 +59,9 @@ void synth_classes(int64_t C, uint64_t seed, double boost_frac, uint8_t *cls_d, 
 // colptr/row: CSC with strictly increasing rows in every column
 void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
                 int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st);
+// The same with the totals cleared by a kernel instead of a memset.
+void row_totals_fill(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
+                     int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st);
 // cnt[r] = number of stored entries of row r (same CSC requirement)
 void row_counts(const int64_t *colptr, const int32_t *row, int64_t nnz, int64_t m, int64_t n,
                 int32_t *cnt, hipStream_t st);

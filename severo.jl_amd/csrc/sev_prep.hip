@@ -504,6 +504,23 @@ void row_totals(const int64_t *colptr, const int32_t *row, const int32_t *cnt, i
     launch_row_totals(ncells, ngenes, colptr, row, cnt, tot, st);
 }
 
+__global__ void __launch_bounds__(kBlock) k_zero_u32(uint32_t *a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        a[i] = 0u;
+}
+
+void row_totals_fill(const int64_t *colptr, const int32_t *row, const int32_t *cnt, int64_t nnz,
+                     int64_t ncells, int64_t ngenes, uint32_t *tot, hipStream_t st) {
+    if (ncells > 0) {
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ncells, 4)), dim3(kBlock), 0, st, tot,
+                           ncells);
+        SEV_LAUNCH_CHECK();
+    }
+    if (nnz == 0) return;
+    launch_row_totals(ncells, ngenes, colptr, row, cnt, tot, st);
+}
+
 void row_counts(const int64_t *colptr, const int32_t *row, int64_t nnz, int64_t m, int64_t n,
                 int32_t *cnt, hipStream_t st) {
     SEV_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)(m > 0 ? m : 1), st));

@@ -255,13 +255,14 @@ class Pipeline:
                                               _ptr(mu)))
         return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, self.nfeat)), mu
 
-    def pca(self, npcs=50, init=None, tol=1e-5, maxit=1000, precision="fp32", rng=None):
+    def pca(self, npcs=50, init=None, tol=1e-5, maxit=1000, precision="fp32", rng=None, work=0):
+        """work: Lanczos vectors (0: npcs + 7, the reference's choice)."""
         if init is None:
             rng = np.random.default_rng(0) if rng is None else rng
             init = rng.standard_normal(self.nfeat)
         init = np.ascontiguousarray(init, np.float64)
         it, nm = C.c_int64(), C.c_int64()
-        check(self._L.sev_pipeline_pca(self._h, npcs, maxit, float(tol), _ptr(init),
+        check(self._L.sev_pipeline_pca(self._h, npcs, int(work), maxit, float(tol), _ptr(init),
                                        PRECISION[precision], C.byref(it), C.byref(nm)))
         self.npcs = npcs
         return it.value, nm.value

@@ -59,10 +59,12 @@ def test_irlba_dense(sev, m, n, nu):
     assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
 
 
-@pytest.mark.parametrize("nu", [55, 57])
+@pytest.mark.parametrize("nu", [55, 57, 58, 90, 121])
 def test_irlba_wide_basis(sev, nu):
     """work = nu + 7 at the top of each projected-SVD kernel's range: 62
-    (8 lanes per column pair) and 64 (16 lanes per pair)."""
+    (8 lanes per column pair), 64 (16 lanes per pair, both arrays in LDS),
+    then 65, 97 and 128 (one array in LDS; restart products in several
+    column groups)."""
     rng = np.random.default_rng(nu)
     m, n = 400, 160
     X = rng.standard_normal((m, n)) * (1.0 + 3.0 * np.exp(-np.arange(n) / 20.0))
@@ -86,6 +88,30 @@ def test_irlba_graded_spectrum(sev):
     assert np.allclose(S.S, sig[:nu], rtol=1e-7)
     assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-7
     assert _angle(S.U, Q1[:, :nu]) < 1e-6 and _angle(S.V, Q2[:, :nu]) < 1e-6
+
+
+@pytest.mark.parametrize("nu", [10, 60])
+def test_irlba_graded_spectrum_wide(sev, nu):
+    """The graded spectrum with work = nu + 7 up to 67: beyond 64 the device
+    SVD hands ill-conditioned projected matrices to the host SVD."""
+    rng = np.random.default_rng(13)
+    m, n = 300, 120
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    sig = 10.0 ** (-0.1 * np.arange(n))
+    X = (Q1 * sig) @ Q2.T
+    S = sev.irlba(sp.csc_matrix(X), nu, tol=1e-8, init=rng.standard_normal(n))
+    assert np.allclose(S.S, sig[:nu], rtol=1e-7)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-7
+
+
+def test_irlba_work_limit_reported(sev):
+    """work above 128 is refused with the cause in the message (the libcell
+    entry also prints it, since Severo only sees a nonzero status)."""
+    rng = np.random.default_rng(2)
+    X = sp.csc_matrix(rng.standard_normal((300, 200)))
+    with pytest.raises(sev.SeveroError, match="128"):
+        sev.irlba(X, 122, tol=1e-5, init=rng.standard_normal(200))
 
 
 def test_irlba_sparse_two(sev):
@@ -181,6 +207,29 @@ def test_pipeline_pca_vs_gram_oracle(sev, precision, tol_s):
     assert res < (1e-5 if precision == "fp32" else 1e-3)
 
 
+@pytest.mark.parametrize("work", [0, 64, 80, 128])
+def test_pipeline_pca_work_sizes(sev, work):
+    """A larger Lanczos basis than the reference's npcs + 7 converges to the
+    same tolerance: singular values within 1e-5 of the FP64 Gram oracle, the
+    leading right vectors within 1e-3 rad, and the residual below tol."""
+    t = sev.nb_tables(4000, seed=3, density=0.05)
+    C = 6000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    p.normalize("lognormalize", 1e4)
+    feats = sev.top_mean_features(t, 900)
+    p.select_scale(feats, 10.0)
+    B, mu = p.get_scaled()
+    it, nm = p.pca(50, init=np.random.default_rng(1).standard_normal(900), work=work)
+    s, U, V, Z = p.get_pca()
+    sg, Vg = _gram_oracle(B, mu)
+    assert np.max(np.abs(s - sg[:50]) / sg[:50]) < 1e-5
+    assert _angle(V[:, :20], Vg[:, :20]) < 1e-3
+    Q = B.toarray().astype(np.float64) - mu.astype(np.float64)
+    assert np.linalg.norm(Q.T @ U - V * s) / np.linalg.norm(Q) < 1e-5
+    p.close()
+
+
 @pytest.mark.parametrize("m,n,dens", [(3000, 600, 0.5), (9000, 400, 0.3), (4000, 900, 0.08),
                                       (2500, 8000, 0.02), (70000, 50, 0.2)])
 def test_centered_mul_layouts(sev, orc, m, n, dens):
@@ -211,6 +260,19 @@ def test_irlba_callback_dense(sev, m, n, nu):
     ss = np.linalg.svd(X, compute_uv=False)
     assert np.allclose(S.S, ss[:nu], rtol=1e-8)
     assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-5
+
+
+def test_irlba_callback_nu58(sev):
+    """nu = 58 (work 65) through libcell's callback entry point, which
+    Severo's irlba!/pca reach with any npcs (src/irlba.jl:47-76)."""
+    from scipy.sparse.linalg import aslinearoperator
+    rng = np.random.default_rng(58)
+    m, n, nu = 500, 200, 58
+    X = rng.standard_normal((m, n)) * (1.0 + 2.0 * np.exp(-np.arange(n) / 30.0))
+    S = sev.irlba_operator(aslinearoperator(X), nu, tol=1e-6, init=rng.standard_normal(n))
+    ss = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(S.S, ss[:nu], rtol=1e-8)
+    assert np.linalg.norm(X.T @ S.U - S.V * S.S) / np.linalg.norm(X) < 1e-6
 
 
 def test_irlba_callback_matches_device_operator(sev):
