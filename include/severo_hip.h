@@ -82,6 +82,24 @@ int sev_normalize_cells(int64_t m, int64_t n, const int64_t *colptr, const int64
                         const void *nzval, int count_type, int method, double scale_factor,
                         int dtype, void *out_nzval);
 
+/* Allocation-free normalize_cells.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It computes what sev_normalize_cells computes, on a count
+ * matrix already in device memory: colptr[n+1] (int64, 0-based), rowval[nnz]
+ * (int32, 0-based, below m and strictly increasing within each column) and
+ * counts[nnz] (int32, >= 0); out_nzval[nnz] (device, dtype) receives the
+ * values.  The work is enqueued on `stream` (a hipStream_t; NULL for the
+ * null stream) and the call returns without waiting.  The per-cell totals
+ * live in a static device array of 8,388,608 entries (32 MiB on every
+ * device the library runs on, held until the process ends); m above that
+ * is refused with SEV_ERR_ARG.  Every call on one device shares that array,
+ * so calls that overlap on one device (other threads or streams) race.  No
+ * hipMalloc, hipFree or other allocating call is made. */
+int sev_normalize_cells_noalloc(int64_t m, int64_t n, int64_t nnz, const int64_t *colptr,
+                                const int32_t *rowval, const int32_t *counts, int method,
+                                double scale_factor, int dtype, void *out_nzval, void *stream);
+
 /* scale_features -> scale_data (src/scaling.jl:199-217, :335-357) with the
  * Welford mean/var of src/scaling.jl:18-34 in dtype.  Writes the clipped
  * scaled values (same structure) and mu' = mu / sigma per column (the

@@ -35,6 +35,8 @@ SIGNATURES = {
     "sev_selftest": (_int, [_int, C.POINTER(_dbl)]),
     "sev_device_init": (_int, [_int]),
     "sev_normalize_cells": (_int, [_i64, _i64, _p, _p, _p, _int, _int, _dbl, _int, _p]),
+    "sev_normalize_cells_noalloc": (_int, [_i64, _i64, _i64, _p, _p, _p, _int, _dbl, _int, _p,
+                                           _p]),
     "sev_scale_features": (_int, [_i64, _i64, _p, _p, _p, _int, _dbl, _p, _p]),
     "sev_centered_mul": (_int, [_i64, _i64, _p, _p, _p, _p, _int, C.c_char, _p, _p, _dbl, _dbl]),
     "sev_irlba_centered_csc": (_int, [_i64, _i64, _p, _p, _p, _p, _int, _i64, _i64, _i64, _dbl, _p,

@@ -432,6 +432,7 @@ int sev_device_init(int device) {
         SEV_REQUIRE(device >= 0 && device < count, "device index out of range");
         SEV_HIP(hipSetDevice(device));
         ensure_device();
+        noalloc_setup();
     });
 }
 
