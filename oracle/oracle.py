@@ -45,6 +45,7 @@ def lib():
         L.so_scale_f64.argtypes = [_i64, _i64, _p, _p, _dbl, _p, _p, _p]
         L.so_centered_mul.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, C.c_int, _p, _p,
                                       _dbl, _dbl]
+        L.so_centered_tmul_mat.argtypes = [_i64, _i64, _p, _p, _p, _p, _p, _i64, _p, C.c_int]
         L.so_svd_jacobi.argtypes = [_i64, _p, _p, _p, _p]
         L.so_irlba_csc.restype = C.c_int
         L.so_irlba_csc.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, _i64, _i64, _i64, _dbl,
@@ -160,6 +161,22 @@ def centered_mul(m, colptr, rowval, vals, mu, x, trans=False, y=None, alpha=1.0,
     lib().so_centered_mul(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), f64, _ptr(mu),
                           int(trans), _ptr(x), _ptr(y), alpha, beta)
     return y
+
+
+def centered_tmul_mat(m, colptr, rowval, vals, mu, U, nthreads=None):
+    """A'U for the centered matrix A = B - 1 mu' (Float32 B and mu), U m x k
+    FP64; returns n x k FP64 (threaded over the columns of B)."""
+    n = len(colptr) - 1
+    U = np.ascontiguousarray(U, np.float64)
+    k = U.shape[1]
+    Y = np.zeros((n, k))
+    colptr = np.ascontiguousarray(colptr, np.int64)
+    rowval = np.ascontiguousarray(rowval, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    mu = None if mu is None else np.ascontiguousarray(mu, np.float32)
+    lib().so_centered_tmul_mat(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), _ptr(mu), _ptr(U),
+                               k, _ptr(Y), int(nthreads or os.cpu_count() or 1))
+    return Y
 
 
 def svd_jacobi(A):

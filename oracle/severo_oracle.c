@@ -588,6 +588,36 @@ done:
     return ret;
 }
 
+/* Y = A' U for A = B - 1 mu' (B: m x n CSC, Float32 values and mu) and U
+ * m x k row-major FP64: Y (n x k row-major) row j = sum over the column's
+ * entries of B[p] U[row_p, :] - mu_j (sum of U's rows).  The vector form
+ * of src/scaling.jl:252-257 applied to k columns at once (the matrix method
+ * at :266-272 adds the mu term with the opposite sign; that is not
+ * reproduced here).  Columns of B are split over nthreads OpenMP threads;
+ * each output row is summed in entry order. */
+SO_EXPORT void so_centered_tmul_mat(int64_t m, int64_t n, const int64_t *colptr,
+                                    const int32_t *rowval, const float *nzval, const float *mu,
+                                    const double *U, int64_t k, double *Y, int nthreads)
+{
+    double *colsum = (double *)calloc((size_t)(k > 0 ? k : 1), sizeof(double));
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t c = 0; c < k; ++c) colsum[c] += U[i * k + c];
+    if (nthreads < 1) nthreads = 1;
+    _Pragma("omp parallel for schedule(dynamic, 8) num_threads(nthreads)")
+    for (int64_t j = 0; j < n; ++j) {
+        double *y = Y + j * k;
+        for (int64_t c = 0; c < k; ++c) y[c] = 0.0;
+        for (int64_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+            const double v = (double)nzval[p];
+            const double *u = U + (int64_t)rowval[p] * k;
+            for (int64_t c = 0; c < k; ++c) y[c] += v * u[c];
+        }
+        if (mu)
+            for (int64_t c = 0; c < k; ++c) y[c] -= (double)mu[j] * colsum[c];
+    }
+    free(colsum);
+}
+
 typedef struct {
     int64_t m, n;
     const int64_t *colptr;

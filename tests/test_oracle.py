@@ -646,3 +646,22 @@ def test_saunders_closed_form(orc):
         ref = np.where(J, np.log10(var) - np.log10(mu * nolan), 0.0)
     assert np.allclose(met, ref, rtol=1e-11, atol=0)
     assert (met > 0).sum() > 0 and met[5] == 0.0
+
+
+def test_centered_tmul_mat_vs_dense(orc):
+    """A'U of the centered matrix for several vectors at once (the checker
+    of the IRLBA residual at full size) against the dense product; a
+    planted mistake (the mu term added instead of subtracted) differs."""
+    rng = np.random.default_rng(21)
+    X = sp.random(700, 60, 0.15, format="csc", random_state=2, dtype=np.float64)
+    X = X.astype(np.float32)
+    mu = rng.random(60).astype(np.float32)
+    U = rng.standard_normal((700, 7))
+    A = X.toarray().astype(np.float64) - mu.astype(np.float64)
+    Y = orc.centered_tmul_mat(700, X.indptr, X.indices, X.data, mu, U, nthreads=3)
+    assert np.allclose(Y, A.T @ U, rtol=1e-12, atol=1e-11)
+    wrong = X.toarray().astype(np.float64).T @ U + np.outer(mu, U.sum(axis=0))
+    assert not np.allclose(Y, wrong, rtol=1e-6)
+    # one column equals the single-vector product
+    y0 = orc.centered_mul(700, X.indptr, X.indices, X.data, mu, U[:, 3], trans=True)
+    assert np.allclose(Y[:, 3], y0, rtol=1e-13, atol=1e-12)
