@@ -478,44 +478,64 @@ __device__ __forceinline__ int64_t xcd_slab(int64_t x, int64_t nslab) {
     return xcd * q + (xcd < r ? xcd : r) + i;
 }
 
+// Sum over the G lanes of each lane group (G a power of two); lane 0 of the
+// group holds the result.  G = 64 is the wave sum (result in every lane).
+template <int G, class T>
+__device__ __forceinline__ T group_sum(T v) {
+    if constexpr (G == kWave) {
+        return wave_sum(v);
+    } else {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        return v;
+    }
+}
+
 // --------------------------------------------------------------------------
 // Slab partials of B' y: block (slab s, column group g) stages the slab of y
-// in LDS and computes the dots of the columns of its group, one wavefront per
-// column segment.  Each lane reads 4 consecutive entries per step from the
-// 4-aligned cover of the segment (entries outside it are masked; the value
-// and row arrays are padded so the cover stays in bounds); two accumulators
-// alternate and are combined in order.
+// in LDS and computes the dots of the columns of its group; each column
+// segment is summed by a group of G lanes (kWave / G segments per wavefront
+// at once, G set from the mean segment length when the operator is built).
+// Lane u of a group reads 4 consecutive entries per step from the 4-aligned
+// cover of the segment (entries outside it are masked; the value and row
+// arrays are padded so the cover stays in bounds); two accumulators
+// alternate and are combined in order, then the group is summed.
 // --------------------------------------------------------------------------
-template <class VT>
+template <class VT, int G>
 __global__ void __launch_bounds__(kBlock)
 k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *val,
       const double *y, double *part) {
+    constexpr int C = kWave / G;  // segments per wavefront
     __shared__ double wl[kSlabRows];
     const int64_t s = xcd_slab(blockIdx.x, gridDim.x);
     const int g = blockIdx.y, ng = gridDim.y;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
     const int w = wave_index(), lane = lane_id();
+    const int u = lane % G, c = lane / G;
     for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock)
         wl[lr] = lr < nrows ? y[r0 + lr] : 0.0;
     __syncthreads();
     double *prow = part + s * (n + 2);
-    for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
-        const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
+    for (int64_t j0 = ((int64_t)g * kWaves + w) * C; j0 < n; j0 += (int64_t)ng * kWaves * C) {
+        const int64_t j = j0 + c;
         double acc0 = 0.0, acc1 = 0.0;
-        for (int64_t p = (a & ~(int64_t)3) + 4 * lane; p < b; p += 4 * kWave) {
-            const ushort4 l4 = *(const ushort4 *)(lrow + p);
-            double x[4];
-            ld4(val, p, x);
-            const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b, v2 = p + 2 >= a && p + 2 < b,
-                       v3 = p + 3 < b;
-            acc0 = fma(v0 ? x[0] : 0.0, wl[v0 ? l4.x : 0], acc0);
-            acc1 = fma(v1 ? x[1] : 0.0, wl[v1 ? l4.y : 0], acc1);
-            acc0 = fma(v2 ? x[2] : 0.0, wl[v2 ? l4.z : 0], acc0);
-            acc1 = fma(v3 ? x[3] : 0.0, wl[v3 ? l4.w : 0], acc1);
+        if (j < n) {
+            const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
+            for (int64_t p = (a & ~(int64_t)3) + 4 * u; p < b; p += 4 * G) {
+                const ushort4 l4 = *(const ushort4 *)(lrow + p);
+                double x[4];
+                ld4(val, p, x);
+                const bool v0 = p >= a, v1 = p + 1 >= a && p + 1 < b,
+                           v2 = p + 2 >= a && p + 2 < b, v3 = p + 3 < b;
+                acc0 = fma(v0 ? x[0] : 0.0, wl[v0 ? l4.x : 0], acc0);
+                acc1 = fma(v1 ? x[1] : 0.0, wl[v1 ? l4.y : 0], acc1);
+                acc0 = fma(v2 ? x[2] : 0.0, wl[v2 ? l4.z : 0], acc0);
+                acc1 = fma(v3 ? x[3] : 0.0, wl[v3 ? l4.w : 0], acc1);
+            }
         }
-        const double acc = wave_sum(acc0 + acc1);
-        if (lane == 0) prow[j] = acc;
+        const double acc = group_sum<G>(acc0 + acc1);
+        if (u == 0 && j < n) prow[j] = acc;
     }
 }
 
@@ -524,10 +544,12 @@ k_atw(int64_t m, int64_t n, const int64_t *off, const uint16_t *lrow, const VT *
 // in-slab rows are 12-bit packed; each lane reads the 8-aligned groups of 8
 // entries covering the segment [a, b) (entries outside it are masked; the
 // value and index arrays are padded so the cover stays in bounds).
+// (Lane u of a group of G; the whole segment for G = kWave, u = lane.)
+template <int G = kWave>
 __device__ __forceinline__ float atw_seg_bf16(int64_t a, int64_t b, const uint32_t *lrow12,
-                                              const uint16_t *val, const float *wl, int lane) {
+                                              const uint16_t *val, const float *wl, int u) {
     float acc0 = 0.0f, acc1 = 0.0f;
-    for (int64_t p = (a & ~(int64_t)7) + 8 * lane; p < b; p += 8 * kWave) {
+    for (int64_t p = (a & ~(int64_t)7) + 8 * u; p < b; p += 8 * G) {
         uint32_t l[8];
         load_idx8<true>(lrow12, p, l);
         const uint4 v8 = *(const uint4 *)(val + p);
@@ -546,23 +568,31 @@ __device__ __forceinline__ float atw_seg_bf16(int64_t a, int64_t b, const uint32
     return acc0 + acc1;
 }
 
+template <int G>
 __global__ void __launch_bounds__(kBlock)
 k_atw_bf16(int64_t m, int64_t n, const int64_t *off, const uint32_t *lrow12, const uint16_t *val,
            const double *y, double *part) {
+    constexpr int C = kWave / G;  // segments per wavefront
     __shared__ float wl[kSlabRows];
     const int64_t s = xcd_slab(blockIdx.x, gridDim.x);
     const int g = blockIdx.y, ng = gridDim.y;
     const int64_t r0 = s * kSlabRows;
     const int64_t nrows = (m - r0) < kSlabRows ? (m - r0) : kSlabRows;
     const int w = wave_index(), lane = lane_id();
+    const int u = lane % G, c = lane / G;
     for (int64_t lr = threadIdx.x; lr < kSlabRows; lr += kBlock)
         wl[lr] = lr < nrows ? (float)y[r0 + lr] : 0.0f;
     __syncthreads();
     double *prow = part + s * (n + 2);
-    for (int64_t j = (int64_t)g * kWaves + w; j < n; j += (int64_t)ng * kWaves) {
-        const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
-        const float acc = wave_sum(atw_seg_bf16(a, b, lrow12, val, wl, lane));
-        if (lane == 0) prow[j] = (double)acc;
+    for (int64_t j0 = ((int64_t)g * kWaves + w) * C; j0 < n; j0 += (int64_t)ng * kWaves * C) {
+        const int64_t j = j0 + c;
+        float acc = 0.0f;
+        if (j < n) {
+            const int64_t a = off[s * n + j], b = off[(s + 1) * n + j];
+            acc = atw_seg_bf16<G>(a, b, lrow12, val, wl, u);
+        }
+        acc = group_sum<G>(acc);
+        if (u == 0 && j < n) prow[j] = (double)acc;
     }
 }
 
@@ -1615,6 +1645,14 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         op.lanes_per_row = avg > 256 ? 16 : (avg > 64 ? 8 : 4);
     else
         op.lanes_per_row = avg > 96 ? 16 : 8;
+    // A' w: lanes per column segment, enough for one pass over a segment of
+    // the mean length (8 entries per lane and step in the BF16 form, 4
+    // otherwise); a rule of the shape, like the lanes per row, since it
+    // fixes how each segment is split into partial sums
+    const double seg = (double)op.nnz / ((double)op.nslab * (double)n);
+    const double per = op.vtype == kValBF16 ? 8.0 : 4.0;
+    op.atw_lanes = 4;
+    while (op.atw_lanes < kWave && op.atw_lanes * per < seg) op.atw_lanes <<= 1;
     // the build scratch is released when b goes out of scope: wait for the fill
     SEV_HIP(hipStreamSynchronize(st));
 }
@@ -1697,21 +1735,32 @@ int atw_groups_default(const SlabCsc &op) {
 
 int atw_groups(const SlabCsc &op) { return op.atw_groups > 0 ? op.atw_groups : atw_groups_default(op); }
 
+template <int G>
+void launch_atw_g(const SlabCsc &op, const double *y, double *part, hipStream_t st) {
+    const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
+    if (op.vtype == kValBF16)
+        hipLaunchKernelGGL(k_atw_bf16<G>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
+                           op.lrow12.get(), (const uint16_t *)op.vals, y, part);
+    else if (op.vtype == kValF64)
+        hipLaunchKernelGGL((k_atw<double, G>), grid, dim3(kBlock), 0, st, op.m, op.n,
+                           op.off.get(), op.lrow.get(), (const double *)op.vals, y, part);
+    else
+        hipLaunchKernelGGL((k_atw<float, G>), grid, dim3(kBlock), 0, st, op.m, op.n,
+                           op.off.get(), op.lrow.get(), (const float *)op.vals, y, part);
+    SEV_LAUNCH_CHECK();
+}
+
 void launch_atw_only(const SlabCsc &op, const double *y, double *part, hipStream_t st) {
     ProfScope ps("spmv_atw", st,
                  (double)op.nnz * (val_bytes(op.vtype) + (op.vtype == kValBF16 ? 1.5 : 2.0)) +
                      8.0 * op.m + 8.0 * (op.nslab + 1) * op.n);
-    const dim3 grid((unsigned)op.nslab, (unsigned)atw_groups(op));
-    if (op.vtype == kValBF16)
-        hipLaunchKernelGGL(k_atw_bf16, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow12.get(), (const uint16_t *)op.vals, y, part);
-    else if (op.vtype == kValF64)
-        hipLaunchKernelGGL(k_atw<double>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const double *)op.vals, y, part);
-    else
-        hipLaunchKernelGGL(k_atw<float>, grid, dim3(kBlock), 0, st, op.m, op.n, op.off.get(),
-                           op.lrow.get(), (const float *)op.vals, y, part);
-    SEV_LAUNCH_CHECK();
+    switch (op.atw_lanes) {
+        case 4: launch_atw_g<4>(op, y, part, st); break;
+        case 8: launch_atw_g<8>(op, y, part, st); break;
+        case 16: launch_atw_g<16>(op, y, part, st); break;
+        case 32: launch_atw_g<32>(op, y, part, st); break;
+        default: launch_atw_g<64>(op, y, part, st); break;
+    }
 }
 
 // y' = y - W[:,0:jc] T into yout (plus norm/sum partials), then the slab

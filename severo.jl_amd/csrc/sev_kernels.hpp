@@ -183,6 +183,7 @@ struct SlabCsc {
     DevBuf<unsigned char> cval;  // csr_nnz values of vtype
     int lanes_per_row = 64;       // A v: lanes per CSR row (8/16/32/64)
     int atw_groups = 0;           // A' w: column groups per slab (0: heuristic)
+    int atw_lanes = 64;           // A' w: lanes per column segment (4..64)
     const void *vals = nullptr;  // CSC values read by A' w (not owned unless
                                  // they point into vbf16)
     const void *mu = nullptr;    // mu' (not owned), same float type as the

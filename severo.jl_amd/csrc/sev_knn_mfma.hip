@@ -397,15 +397,18 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
 #pragma unroll
         for (int rb = 0; rb < QW; ++rb) {
             // one test per output row for the whole tile: the largest of its
-            // 4 x 16 products against the row's threshold
-            bool any = false;
+            // 4 x 16 products against the row's threshold (IEEE maximum, so
+            // the MFMA results are not canonicalised first; the tests are
+            // combined as wave masks)
+            uint64_t any = 0;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float m = fmaxf(fmaxf(acc[0][rb][i], acc[1][rb][i]),
-                                      fmaxf(acc[2][rb][i], acc[3][rb][i]));
-                any |= m > thr[rb][i];
+                const float m = __builtin_elementwise_maximum(
+                    __builtin_elementwise_maximum(acc[0][rb][i], acc[1][rb][i]),
+                    __builtin_elementwise_maximum(acc[2][rb][i], acc[3][rb][i]));
+                any |= __ballot(m > thr[rb][i]);
             }
-            if (!__ballot(any)) continue;
+            if (!any) continue;
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
                 // products as integers for readlane (on vector elements it was
