@@ -5,8 +5,9 @@ A step is one pass of normalize_cells -> Y[:, features] -> scale_features
 Jaccard SNN (prune 1/15) over a synthetic negative-binomial count matrix that
 is generated on the device before timing starts (inputs resident in HBM).
 
-Workloads (cells x genes, density; features = the 2000 genes with the
-largest generator mean):
+Workloads (cells x genes, density; features = the 2000 genes that the
+device's find_variable_features(:vst) selects, computed once before timing;
+--features top-mean takes the 2000 genes of largest generator mean instead):
   small : 5,000 x 20,000, 5%
   100k  : 100,000 x 33,538, 4%, 5% of cells with 3x library size
   1m    : 1,000,000 x 27,998, 3%   (default: the quoted metric's workload)
@@ -15,7 +16,9 @@ largest generator mean):
 
 The headline value is the precision form (BF16 operator values for the
 Lanczos products, BF16 candidate search for kNN with exact FP32 re-rank);
-the plain FP32 form is timed in the same run.  The batching form (64 samples
+the plain FP32 form is timed in the same run.  IRLBA keeps --work Lanczos
+vectors (default 128; the reference's npcs + 7 = 57 is timed beside it in
+the BF16 form, and so is the top-mean feature set).  The batching form (64 samples
 of uneven size, 20,000 genes, 30 PCs, k = 15) is timed after it, against the
 single-sample pipeline looped over the same samples.  Prints one JSON line.
 """
@@ -45,16 +48,19 @@ WORKLOADS = {
 }
 NFEAT, NPCS, K, PRUNE, SCALE_MAX, SF = 2000, 50, 20, 1.0 / 15, 10.0, 1e4
 MAXIT = 1000  # IRLBA restarts (the reference's default)
+DEFAULT_WORK = 128  # Lanczos vectors kept by IRLBA (0: npcs + 7, the reference's choice)
+WORK = DEFAULT_WORK
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
 
 
-def run_step(p, feats, init, form):
+def run_step(p, feats, init, form, work=None):
     """One step; the fp8 form is the bf16 form with FP8 kNN candidates."""
     p.normalize("lognormalize", SF)
     p.select_scale(feats, SCALE_MAX)
     it, nm = p.pca(NPCS, init=init, tol=1e-5, maxit=MAXIT,
-                   precision="bf16" if form == "fp8" else form)
+                   precision="bf16" if form == "fp8" else form,
+                   work=WORK if work is None else work)
     nfb = p.neighbours(K, include_self=True, precision=form)
     nnz = p.snn(PRUNE, np.float32)
     p.synchronize()
@@ -87,6 +93,23 @@ def time_form(p, feats, init, form, steps, warmup, profile):
     if form == "fp8":
         info["knn_rows_left_after_fp8_bf16_fp32"] = [int(x) for x in p.knn_passes()]
     return dt, info, prof, breakdown
+
+
+def time_variant(p, feats, init, form, steps, work=None):
+    """A secondary timing (no profile): one warm-up step, then `steps`.
+    An IRLBA that reaches maxit is reported, not raised."""
+    try:
+        run_step(p, feats, init, form, work)
+    except RuntimeError as e:
+        p.synchronize()
+        return None, {"failed": str(e)}
+    p.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, nm, nfb, nnz = run_step(p, feats, init, form, work)
+    p.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, dict(irlba_iters=it, irlba_nmult=nm, knn_fallback_rows=nfb, snn_nnz=nnz)
 
 
 def measured_traffic():
@@ -281,15 +304,44 @@ def batch_leg(sev, steps, warmup, graph=True):
     return out
 
 
-def cpu_baseline(sev, orc, w, sample_cells, threads=1):
+def cpu_stages_full(orc, p, feats, m):
+    """The oracle's linear stages on the whole workload, single-threaded:
+    normalise, gather + scale, and one centered product pair (A v, A'w) --
+    the unit of work an IRLBA product count multiplies.  Its kNN is
+    O(cells^2) and is timed on the sample instead.  The counts are the
+    device's, copied to the host."""
+    X = p.get_counts()
+    cp = np.ascontiguousarray(X.indptr, np.int64)
+    rv = np.ascontiguousarray(X.indices, np.int32)
+    cnt = np.ascontiguousarray(X.data, np.int32)
+    nnz = len(rv)
+    del X
+    t0 = time.perf_counter()
+    norm = orc.normalize(m, cp, rv, cnt, SF, True, np.float32)
+    t_norm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fcp, frow, fval = orc.gather(feats, cp, rv, norm)
+    b, mu, _ = orc.scale(m, fcp, fval, np.float32(SCALE_MAX))
+    t_scale = time.perf_counter() - t0
+    del norm, cp, rv, cnt, fval
+    x = np.random.default_rng(1).standard_normal(len(feats))
+    t0 = time.perf_counter()
+    y = orc.centered_mul(m, fcp, frow, b, mu, x)
+    orc.centered_mul(m, fcp, frow, b, mu, y, trans=True)
+    t_pair = time.perf_counter() - t0
+    return {"cells": m, "nnz": nnz, "selected_nnz": int(fcp[-1]), "cores": 1, "kind": "port",
+            "normalise_s": round(t_norm, 3), "gather_scale_s": round(t_scale, 3),
+            "product_pair_s": round(t_pair, 3)}
+
+
+def cpu_baseline(sev, orc, w, sample_cells, feats, threads=1):
     """The oracle's scalar C restatement of the same path on a bounded
     sample of the workload (same generator tables, first `sample_cells`
-    cells' worth of draws).  threads > 1 runs the kNN stage (the only
-    threaded one) on that many threads; the other stages stay scalar, as
-    the reference's Julia loops are."""
+    cells' worth of draws, the timed step's features).  threads > 1 runs the
+    kNN stage (the only threaded one) on that many threads; the other stages
+    stay scalar, as the reference's Julia loops are."""
     t = sev.nb_tables(w["genes"], seed=0, density=w["density"], boost_frac=w["boost"])
     cp, rv, cnt = orc.synth_csc(sample_cells, t)
-    feats = sev.top_mean_features(t, NFEAT)
     init = np.random.default_rng(0).standard_normal(NFEAT)
     t0 = time.perf_counter()
     norm = orc.normalize(sample_cells, cp, rv, cnt, SF, True, np.float32)
@@ -318,6 +370,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8000)
     ap.add_argument("--maxit", type=int, default=None,
                     help="IRLBA restart limit (default 1000; 4m: 5000)")
+    ap.add_argument("--work", type=int, default=DEFAULT_WORK,
+                    help="Lanczos vectors kept by IRLBA (0: npcs + 7, the reference's choice)")
+    ap.add_argument("--features", default="vst", choices=["vst", "top-mean"],
+                    help="feature set of the timed step")
+    ap.add_argument("--variant-steps", type=int, default=3,
+                    help="steps of the secondary timings (reference work size, other feature "
+                         "set); 0 skips them")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-e2e", action="store_true", help="skip the PCIe-inclusive step")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
@@ -341,7 +400,8 @@ def main():
     copy_gbs = sev.copy_probe()
     print(f"# HBM copy probe: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     w = WORKLOADS[args.workload]
-    global K, MAXIT
+    global K, MAXIT, WORK
+    WORK = args.work
     K = w.get("k", K)
     MAXIT = args.maxit if args.maxit is not None else w.get("maxit", 1000)
     if args.forms is None:
@@ -352,21 +412,26 @@ def main():
     p.synth_counts(w["cells"], t)
     m, g, nnz = p.counts_info()
     gen_s = time.perf_counter() - t0
-    feats = sev.top_mean_features(t, NFEAT)
+    top = sev.top_mean_features(t, NFEAT)
     init = np.random.default_rng(0).standard_normal(NFEAT)
     print(f"# generated {m} x {g} counts, nnz {nnz} ({nnz / (m * g):.4f}) in {gen_s:.1f} s",
           file=sys.stderr, flush=True)
 
     # find_variable_features(:vst) on the same counts, outside the timed
     # step (the feature list of the step is fixed in advance)
-    p.find_variable_features(NFEAT)
-    p.synchronize()
-    t0 = time.perf_counter()
-    hvf = p.find_variable_features(NFEAT)
-    hvg_ms = (time.perf_counter() - t0) * 1e3
-    print(f"# find_variable_features(:vst, {NFEAT}): {hvg_ms:.1f} ms, "
-          f"{len(set(hvf.tolist()) & set(feats.tolist()))} of its genes among the top-mean ones",
-          file=sys.stderr, flush=True)
+    if g > NFEAT:
+        p.find_variable_features(NFEAT)
+        p.synchronize()
+        t0 = time.perf_counter()
+        hvf = p.find_variable_features(NFEAT)
+        hvg_ms = (time.perf_counter() - t0) * 1e3
+        print(f"# find_variable_features(:vst, {NFEAT}): {hvg_ms:.1f} ms, "
+              f"{len(set(hvf.tolist()) & set(top.tolist()))} of its genes among the top-mean "
+              "ones", file=sys.stderr, flush=True)
+    else:
+        hvf, hvg_ms = top, 0.0  # every gene is a feature
+    feats, other = (hvf, top) if args.features == "vst" else (top, hvf)
+    fname = {"vst": "find_variable_features(:vst)", "top-mean": "largest generator mean"}
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
@@ -407,9 +472,12 @@ def main():
         "dtype": head,
         "data": "synthetic negative-binomial counts generated on the device",
         "config": {"workload": f"{args.workload}: {m} cells x {g} genes, nnz {nnz}, "
-                               f"{NFEAT} features, {NPCS} PCs, k={K}, prune 1/15, form {head}",
-                   "cells": m, "genes": g, "nnz": nnz, "features": NFEAT, "npcs": NPCS, "k": K,
-                   "form": head},
+                               f"{NFEAT} features ({fname[args.features] if g > NFEAT else 'all genes'}), "
+                               f"{NPCS} PCs (IRLBA, {WORK or NPCS + 7} Lanczos vectors), k={K}, "
+                               f"prune 1/15, form {head}",
+                   "cells": m, "genes": g, "nnz": nnz, "features": NFEAT,
+                   "feature_set": args.features if g > NFEAT else "all", "npcs": NPCS,
+                   "irlba_work": WORK or NPCS + 7, "k": K, "form": head},
         "roofline": dict(roofline(prof, head, args.steps, args.workload) or {}, copy_probe_gbs=round(copy_gbs, 1)),
         "forms": {f: {"cells_per_s": round(m / res[f][0], 2), "ms_per_step": round(res[f][0] * 1e3, 3),
                       **res[f][1]} for f in forms},
@@ -439,16 +507,46 @@ def main():
     if not args.no_e2e:
         out["end_to_end"] = end_to_end(sev, p, feats, init, head, m)
         print(f"# end to end ({head}): {out['end_to_end']}", file=sys.stderr, flush=True)
+    # secondary timings in the headline form: the reference's work size, and
+    # the other feature set
+    variants = {}
+    if args.variant_steps > 0:
+        if WORK != 0:
+            vdt, vinfo = time_variant(p, feats, init, head, args.variant_steps, work=0)
+            variants["reference_work"] = {"irlba_work": NPCS + 7, "maxit": MAXIT, **vinfo}
+            if vdt is not None:
+                variants["reference_work"].update(ms_per_step=round(vdt * 1e3, 3),
+                                                  cells_per_s=round(m / vdt, 2))
+            print(f"# {head} with work = npcs + 7: {variants['reference_work']}", file=sys.stderr,
+                  flush=True)
+        if g > NFEAT:
+            vdt, vinfo = time_variant(p, other, init, head, args.variant_steps)
+            oname = "top_mean_features" if args.features == "vst" else "vst_features"
+            variants[oname] = dict(vinfo)
+            if vdt is not None:
+                variants[oname].update(ms_per_step=round(vdt * 1e3, 3),
+                                       cells_per_s=round(m / vdt, 2))
+            print(f"# {head} on the other feature set: {variants[oname]}", file=sys.stderr,
+                  flush=True)
+    if variants:
+        out["variants"] = variants
+    cpu_full = None
+    if not args.no_cpu and args.workload in ("1m", "100k", "small"):
+        cpu_full = cpu_stages_full(entry.load_oracle(), p, feats, m)
+        print(f"# oracle stages on the whole workload: {cpu_full}", file=sys.stderr, flush=True)
     p.close()
     if not args.no_batch:
         out["batch"] = batch_leg(sev, max(1, min(args.steps, 2)), max(1, args.warmup),
                                  graph=not args.batch_direct)
     if not args.no_cpu:
         orc = entry.load_oracle()
-        out["cpu_baseline"] = cpu_baseline(sev, orc, w, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(sev, orc, w, args.cpu_sample, feats)
+        if cpu_full:
+            out["cpu_baseline"]["stages_full_workload"] = cpu_full
         nthr = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
         if nthr > 1:
-            out["cpu_baseline_all_cores"] = cpu_baseline(sev, orc, w, args.cpu_sample, nthr)
+            out["cpu_baseline_all_cores"] = cpu_baseline(sev, orc, w, args.cpu_sample, feats,
+                                                         nthr)
         if not args.no_cluster:
             # the oracle's sequential Louvain (one start, one core) on the same graph
             S = snn_host.tocsc()

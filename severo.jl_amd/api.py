@@ -297,24 +297,57 @@ class LinearEmbedding:
     basis: np.ndarray  # F x npcs (loadings)
 
 
+DENSE_MAX = 128  # largest min(m, n) of the dense route (a full Lanczos basis)
+
+
+def pca_algorithm(m, n, npcs, algorithm="arpack"):
+    """The algorithm _pca runs (src/embedding.jl:46-53): more than half the
+    possible components switch :arpack and :irlba to the dense SVD, with the
+    reference's warning.  Returns (algorithm, work): :arpack and :irlba both
+    run the device IRLBA (svds and irlba compute the same truncated SVD to
+    tolerance), work None meaning npcs + 7; :svd runs it with a full basis
+    of min(m, n) Lanczos vectors, which is the Golub-Kahan bidiagonalisation
+    followed by the SVD of the bidiagonal -- a dense SVD to working
+    precision -- for min(m, n) <= 128.  :tssvd is not provided."""
+    import warnings
+    if isinstance(algorithm, str):
+        algorithm = algorithm.lstrip(":")
+    if algorithm not in ("arpack", "irlba", "svd", "tssvd"):
+        raise ValueError(f"unknown algorithm: {algorithm}")
+    if npcs > 0.5 * min(m, n) and algorithm in ("arpack", "irlba"):
+        warnings.warn("Computing too large a percentage of principal components, using "
+                      "standard svd instead")
+        algorithm = "svd"
+    if algorithm == "tssvd":
+        raise NotImplementedError("algorithm=:tssvd is not provided on the device")
+    if algorithm == "svd":
+        if min(m, n) > DENSE_MAX:
+            raise NotImplementedError(f"the dense SVD route needs min(m, n) <= {DENSE_MAX}")
+        return "svd", min(m, n)
+    return algorithm, None
+
+
 def pca(X: CenteredMatrix, npcs: int = 50, tol=1e-5, maxit=1000, init=None, rng=None,
-        precision="fp32") -> LinearEmbedding:
-    """_pca with algorithm=:irlba (src/embedding.jl:46-76)."""
+        precision="fp32", algorithm="arpack", work=None) -> LinearEmbedding:
+    """_pca (src/embedding.jl:46-76): the truncated SVD of the algorithm
+    pca_algorithm picks, Z = U diag(S), stdev = S / sqrt(max(1, m - 1)),
+    basis = V."""
     m, n = X.shape
     npcs = min(min(m, n), npcs)
-    S = irlba(X, npcs, tol=tol, maxit=maxit, init=init, rng=rng, precision=precision)
+    algorithm, dense_work = pca_algorithm(m, n, npcs, algorithm)
+    S = irlba(X, npcs, tol=tol, maxit=maxit, init=init, rng=rng, precision=precision,
+              work=dense_work if dense_work is not None else work)
     Z = S.U[:, :npcs] * S.S[:npcs][None, :]
     stdev = S.S[:npcs] / np.sqrt(max(1, m - 1))
     return LinearEmbedding(Z, stdev, S.V[:, :npcs])
 
 
 def embedding(X, ncomponents: int = 50, method="pca", **kw) -> LinearEmbedding:
-    """src/embedding.jl:202-212."""
+    """src/embedding.jl:202-212 (algorithm= passes on to _pca)."""
     if isinstance(method, str):
         method = method.lstrip(":")
     if method != "pca":
         raise ValueError(f"unknown reduction method: {method}")
-    kw.pop("algorithm", None)
     return pca(X, ncomponents, **kw)
 
 

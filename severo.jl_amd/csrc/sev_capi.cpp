@@ -776,8 +776,8 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit,
         build_slab_csc(p->op, m, n, p->fcp.get(), p->frow.get(), p->bval.get(), false,
                        p->mu.get(), precision == SEV_PREC_BF16, st);
         autotune_operator(p->op, st);
-        SEV_REQUIRE(work == 0 || (work > nu && work <= 128),
-                    "work must be 0 (npcs + 7) or in (npcs, 128]");
+        SEV_REQUIRE(work == 0 || (work >= nu && work <= 128),
+                    "work must be 0 (npcs + 7) or in [npcs, 128]");
         IrlbaParams prm;
         prm.nu = nu;
         prm.work = work > 0 ? work : nu + 7;

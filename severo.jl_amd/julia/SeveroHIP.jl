@@ -110,7 +110,7 @@ end
 # -------------------------------------------------------------------- PCA
 # src/irlba.jl:47-85 and src/embedding.jl:46-94 (algorithm = :irlba)
 function irlba(C::Severo.CenteredMatrix, nu::Integer; tol=1e-5, maxit=1000, init=nothing,
-               precision=:fp32)
+               precision=:fp32, work=nu + 7)
     A, mu = Severo._A_mu(C)
     m, n = size(A)
     init = init === nothing ? randn(n) : convert(Vector{Float64}, init)
@@ -126,15 +126,30 @@ function irlba(C::Severo.CenteredMatrix, nu::Integer; tol=1e-5, maxit=1000, init
                 (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Int64, Int64,
                  Int64, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint,
                  Ref{Int64}, Ref{Int64}),
-                m, n, A.colptr, A.rowval, vals, muv, dtcode(T), nu, nu + 7, maxit, tol, init, s, U,
+                m, n, A.colptr, A.rowval, vals, muv, dtcode(T), nu, work, maxit, tol, init, s, U,
                 V, SEV_PREC[precision], iters, nmult))
     SVD(U, s, V')
 end
 
-function pca(X::Severo.CenteredMatrix, npcs::Integer; kw...)
+# _pca's algorithm rule (src/embedding.jl:46-53): :arpack and :irlba both run
+# the device IRLBA; more than half the possible components take the dense
+# route, the full Golub-Kahan basis of min(m, n) <= 128 Lanczos vectors
+function pca(X::Severo.CenteredMatrix, npcs::Integer; algorithm=:arpack, kw...)
     m, n = size(X)
     npcs = min(min(m, n), npcs)
-    S = irlba(X, npcs; kw...)
+    if (npcs > 0.5 * min(m, n)) && (algorithm == :arpack || algorithm == :irlba)
+        @warn "Computing too large a percentage of principal components, using standard svd instead"
+        algorithm = :svd
+    end
+    algorithm == :tssvd && error("algorithm=:tssvd is not provided on the device")
+    S = if algorithm == :svd
+        min(m, n) <= 128 || error("the dense SVD route needs min(m, n) <= 128")
+        irlba(X, npcs; work=min(m, n), kw...)
+    elseif algorithm == :arpack || algorithm == :irlba
+        irlba(X, npcs; kw...)
+    else
+        error("unknown algorithm: $algorithm")
+    end
     Z = S.U[:, 1:npcs] * Diagonal(S.S[1:npcs])
     stdev = S.S[1:npcs] ./ sqrt(max(1, m - 1))
     latent = map(x -> string("PC-", x), 1:npcs)

@@ -255,8 +255,17 @@ class Pipeline:
                                               _ptr(mu)))
         return sp.csc_matrix((vals, rowval - 1, colptr - 1), shape=(m, self.nfeat)), mu
 
-    def pca(self, npcs=50, init=None, tol=1e-5, maxit=1000, precision="fp32", rng=None, work=0):
-        """work: Lanczos vectors (0: npcs + 7, the reference's choice)."""
+    def pca(self, npcs=50, init=None, tol=1e-5, maxit=1000, precision="fp32", rng=None, work=0,
+            algorithm="irlba"):
+        """work: Lanczos vectors (0: npcs + 7, the reference's choice).
+        algorithm follows _pca's rule (src/embedding.jl:46-53; see
+        api.pca_algorithm): more than half the possible components take the
+        dense route."""
+        from .api import pca_algorithm
+        m, _, _ = self.counts_info()
+        _, dense_work = pca_algorithm(m, self.nfeat, min(npcs, m, self.nfeat), algorithm)
+        if dense_work is not None:
+            work = dense_work
         if init is None:
             rng = np.random.default_rng(0) if rng is None else rng
             init = rng.standard_normal(self.nfeat)

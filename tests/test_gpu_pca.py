@@ -207,6 +207,77 @@ def test_pipeline_pca_vs_gram_oracle(sev, precision, tol_s):
     assert res < (1e-5 if precision == "fp32" else 1e-3)
 
 
+def _clustered_counts(C, G, ncl, seed):
+    """Negative-binomial counts of cells from ncl clusters whose gene means
+    differ (log-normal, 60% of genes shifted per cluster): the PCA of such
+    data has ncl - 1 leading components well separated from each other and
+    from the noise."""
+    rng = np.random.default_rng(seed)
+    base = np.exp(rng.normal(-1.0, 1.2, G))
+    means = np.stack([base * np.where(rng.random(G) < 0.6, np.exp(rng.normal(0, 1.0, G)), 1.0)
+                      for _ in range(ncl)])
+    lab = rng.integers(0, ncl, C)
+    size = rng.lognormal(0.0, 0.2, C)[:, None]
+    mu = means[lab] * size * 0.6
+    r = 1.0 / 0.3
+    X = rng.negative_binomial(r, r / (r + mu)).astype(np.int32)
+    X[:, 0] += 1  # no empty cell
+    return sp.csc_matrix(X)
+
+
+@pytest.mark.parametrize("precision,tol_s,tol_angle",
+                         [("fp32", 1e-5, 1e-4), ("bf16", 1e-4, 1e-2)])
+def test_pipeline_pca_singular_vectors_svd_flip(sev, precision, tol_s, tol_angle):
+    """Right singular vectors of well-separated components, sign-aligned by
+    svd_flip! (src/utils.jl:214-228) on both sides, against the FP64 Gram
+    eigenvectors: the angle of each component whose singular value lies 1%
+    or more from its neighbours is below 1e-4 rad (FP32 form) / 1e-2 rad
+    (BF16 form, whose operator is the BF16-rounded matrix), and the signs
+    agree; the leading 5-dimensional subspace is within the same bound."""
+    import _gpu_ref as ref
+    X = _clustered_counts(6000, 1500, 7, seed=4)
+    p = sev.Pipeline()
+    p.load_counts(X)
+    p.normalize("lognormalize", 1e4)
+    p.select_scale(np.arange(1500), 10.0)
+    B, mu = p.get_scaled()
+    p.pca(20, init=np.random.default_rng(2).standard_normal(1500), precision=precision,
+          tol=1e-8 if precision == "fp32" else 1e-6)
+    s, U, V, _ = p.get_pca()
+    p.close()
+    sg, Vg = ref.gram_svd(B, mu, 21)
+    assert np.max(np.abs(s - sg[:20]) / sg[:20]) < tol_s
+    keep, ang, sign = ref.svd_flip_angles(V, Vg, sg)
+    assert len(keep) >= 4, keep
+    assert ang.max() < tol_angle and sign.all(), (keep, ang)
+    assert ref.angle(V[:, :5], Vg[:, :5]) < tol_angle
+
+
+@pytest.mark.parametrize("m,n,npcs", [(300, 60, 40), (80, 120, 70), (500, 128, 65)])
+def test_pca_dense_route(sev, m, n, npcs):
+    """_pca with more than half of the possible components switches to the
+    dense SVD (src/embedding.jl:50-53), with the reference's warning: here
+    the full Golub-Kahan basis of min(m, n) Lanczos vectors.  All npcs
+    singular values match numpy's SVD of the centered matrix (1e-9), the
+    embedding is U diag(S) and the loadings span the same subspace."""
+    import _gpu_ref as ref
+    rng = np.random.default_rng(m + n)
+    X = sp.random(m, n, 0.3, format="csc", random_state=m, dtype=np.float64)
+    mu = np.asarray(X.mean(axis=0)).ravel()
+    Cm = sev.CenteredMatrix(X, mu)
+    with pytest.warns(UserWarning, match="standard svd"):
+        E = sev.pca(Cm, npcs, init=rng.standard_normal(n))
+    Q = X.toarray() - mu
+    U, s, Vt = np.linalg.svd(Q, full_matrices=False)
+    sd = E.stdev * np.sqrt(max(1, m - 1))
+    assert np.allclose(sd, s[:npcs], rtol=1e-9)
+    assert ref.angle(E.basis, Vt[:npcs].T) < 1e-6
+    assert np.allclose(E.coordinates, Q @ E.basis, rtol=1e-8, atol=1e-9)
+    # explicitly requested: the same route without the warning
+    E2 = sev.embedding(Cm, 10, algorithm=":svd", init=rng.standard_normal(n))
+    assert np.allclose(E2.stdev * np.sqrt(max(1, m - 1)), s[:10], rtol=1e-9)
+
+
 @pytest.mark.parametrize("work", [0, 64, 80, 128])
 def test_pipeline_pca_work_sizes(sev, work):
     """A larger Lanczos basis than the reference's npcs + 7 converges to the
