@@ -161,16 +161,32 @@ __device__ int64_t column_setup(const int32_t *nn, int64_t n, int64_t j, int k,
     return total;
 }
 
-// Map a flat candidate index e (< P) to (owner lane) by binary search over
-// the lanes' exclusive prefixes (which are nondecreasing in lane order).
-__device__ __forceinline__ int owner_lane(int64_t e, int64_t mypre) {
-    int lo = 0;
+// Insert the candidates e = first, first + stride, ... < P of a column into
+// tab.  Entry e belongs to the neighbour list t with pre[t] <= e < pre[t+1]
+// (found by binary search over the 64 prefixes; lists dropped as duplicates
+// have zero length and are skipped by taking the last such t) and is the row
+// rev[beg[t] + e - pre[t]].  Eight entries per thread are looked up and their
+// rows loaded before any is inserted, so the loads overlap.
+template <class S>
+__device__ __forceinline__ void insert_entries(S *tab, uint32_t nslots, int64_t P, int64_t first,
+                                               int64_t stride, const int64_t *beg,
+                                               const int64_t *pre, const int32_t *rev) {
+    constexpr int R = 8;
+    for (int64_t base = first; base < P; base += R * stride) {
+        uint32_t key[R];
 #pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-        const int64_t p = __shfl(mypre, lo + step, kWave);
-        if (lo + step < kWave && p <= e) lo += step;
+        for (int r = 0; r < R; ++r) {
+            const int64_t e = base + r * stride;
+            int t = 0;
+#pragma unroll
+            for (int step = kMaxK / 2; step > 0; step >>= 1)
+                if (pre[t + step] <= e) t += step;
+            key[r] = e < P ? (uint32_t)rev[beg[t] + (e - pre[t])] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (base + r * stride < P) tab_insert<S>(tab, nslots, key[r]);
     }
-    return lo;
 }
 
 // Bitonic sort of n (power of two) values ascending by one wavefront.
@@ -225,6 +241,8 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
     constexpr int NS = kTableBytes / (int)sizeof(S);
     __shared__ S tabs[kWaves][NS];
     __shared__ S kbufs[kWaves][kWaveKept];
+    __shared__ int64_t wbeg[kWaves][kMaxK], wpre[kWaves][kMaxK];
+    static_assert(kMaxK == kWave, "one neighbour list per lane");
     const int lane = lane_id(), w = threadIdx.x / kWave;
     S *tab = tabs[w];
     S *kbuf = kbufs[w];
@@ -242,23 +260,23 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
             continue;
         }
         const uint32_t T_ = table_size(P);
-        // every lane runs the shuffles each round; lanes past P insert nothing
-        for (int64_t base = 0; base < P; base += kWave) {
-            const int64_t e = base + lane;
-            const int o = owner_lane(e < P ? e : P - 1, mypre);
-            const int64_t ob = __shfl(mybeg, o, kWave);
-            const int64_t op = __shfl(mypre, o, kWave);
-            if (e < P) tab_insert<S>(tab, T_, (uint32_t)rev[ob + (e - op)]);
+        // lanes past k hold prefix P, so the search never selects them
+        wbeg[w][lane] = mybeg;
+        wpre[w][lane] = mypre;
+        insert_entries<S>(tab, T_, P, lane, kWave, wbeg[w], wpre[w], rev);
+        // one pass over the table: the kept slots go to kbuf in slot order
+        // (the first kWaveKept of them) and every slot is cleared
+        const uint64_t ltmask = (1ull << lane) - 1;
+        int L = 0;
+        for (uint32_t i0 = 0; i0 < T_; i0 += kWave) {
+            const S v = tab[i0 + lane];
+            const bool kept = slot_kept<S>(v, xmin);
+            const uint64_t km = __ballot(kept);
+            const int pos = L + __popcll(km & ltmask);
+            if (kept && pos < kWaveKept) kbuf[pos] = v;
+            L += __popcll(km);
+            tab[i0 + lane] = 0;
         }
-        int nk = 0;
-        for (uint32_t i = lane; i < T_; i += kWave) nk += slot_kept<S>(tab[i], xmin) ? 1 : 0;
-        int pre = nk;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int y = __shfl_up(pre, o, kWave);
-            if (lane >= o) pre += y;
-        }
-        const int L = __shfl(pre, kWave - 1, kWave);
         if (L > kWaveKept || (!rows && !stage)) {
             if (!rows && lane == 0) {
                 if (L > kWaveKept) {
@@ -269,14 +287,7 @@ k_snn_wave(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
                     cnt[j] = L;
                 }
             }
-            for (uint32_t i = lane; i < T_; i += kWave) tab[i] = 0;
             continue;
-        }
-        int pos = pre - nk;
-        for (uint32_t i = lane; i < T_; i += kWave) {
-            const S v = tab[i];
-            if (slot_kept<S>(v, xmin)) kbuf[pos++] = v;
-            tab[i] = 0;
         }
         int Lp = 1;
         while (Lp < L) Lp <<= 1;
@@ -329,14 +340,7 @@ __device__ int block_count(S *tab, uint32_t nslots, int64_t P, int k, const int3
     const int lane = lane_id(), w = threadIdx.x / kWave;
     for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) tab[i] = 0;
     __syncthreads();
-    for (int64_t e = threadIdx.x; e < P; e += kBlock) {
-        // owner = last neighbour slot whose prefix is <= e (deduplicated
-        // slots have zero length and share the next slot's prefix)
-        int lo = 0;
-        for (int t = 1; t < k; ++t)
-            if (spre[t] <= e) lo = t;
-        tab_insert<S>(tab, nslots, (uint32_t)rev[sbeg[lo] + (e - spre[lo])]);
-    }
+    insert_entries<S>(tab, nslots, P, threadIdx.x, kBlock, sbeg, spre, rev);
     __syncthreads();
     int nk = 0;
     for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) nk += slot_kept<S>(tab[i], xmin);
@@ -389,59 +393,96 @@ __device__ void block_emit(S *tab, uint32_t nslots, S *kept, int64_t kcap, int L
     }
 }
 
-// Heavy columns: one block per column, table in LDS (P <= half the LDS table)
-// or in global memory at gtab + gtab_off[h] (64-bit slots: 2/3 of the region
-// is the table, 1/3 the sort buffer).  The fill pass recounts LDS-table
-// columns and reuses the global tables filled by the count pass.
+// Heavy columns: persistent blocks take one column at a time, table in LDS
+// (P <= half the LDS table) or in global memory at gtab + gtab_off[h] (64-bit
+// slots: 2/3 of the region is the table, 1/3 the sort buffer).  With stage given, the count pass
+// writes the sorted kept slots of LDS-table columns with at most kWaveKept
+// of them to stage (staged[j] = 1), as the wave path does, and the fill pass
+// skips those; it recounts the other LDS-table columns and reuses the global
+// tables filled by the count pass.
 template <class T, class S>
 __global__ void __launch_bounds__(kBlock)
 k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
             const int32_t *rev, const int32_t *heavy, int64_t nheavy, int32_t *cnt,
             uint64_t *gtab, const int64_t *gtab_off, const T *wtab, const int64_t *colptr,
-            int32_t *rows, T *vals) {
+            int32_t *rows, T *vals, S *stage, int32_t *staged) {
     constexpr int NS = kBlockTableBytes / (int)sizeof(S);
     __shared__ S ltab[NS];
     __shared__ S lkept[kBlockKept];
     __shared__ int64_t sbeg[kMaxK], spre[kMaxK + 1];
     __shared__ int wk[kWaves];
     __shared__ int nkept;
-    const int64_t h = blockIdx.x;
-    if (h >= nheavy) return;
-    const int64_t j = heavy[h];
     const int lane = lane_id(), w = threadIdx.x / kWave;
-    if (w == 0) {
-        int64_t mybeg, mypre, mylen;
-        const int64_t P = column_setup(nn, n, j, k, rptr, mybeg, mypre, mylen);
-        if (lane < kMaxK) {
-            sbeg[lane] = mybeg;
-            spre[lane] = mypre;
+    // the LDS table is clear at the start of every column: cleared here
+    // once, and after each column by the pass that reads it
+    for (int i = threadIdx.x; i < NS; i += kBlock) ltab[i] = 0;
+    for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const int64_t j = heavy[h];
+        if (rows && staged[j]) continue;  // emitted by the count pass
+        __syncthreads();  // the previous column is done with the shared state
+        if (w == 0) {
+            int64_t mybeg, mypre, mylen;
+            const int64_t P = column_setup(nn, n, j, k, rptr, mybeg, mypre, mylen);
+            if (lane < kMaxK) {
+                sbeg[lane] = mybeg;
+                spre[lane] = mypre;
+            }
+            if (lane == 0) {
+                spre[kMaxK] = P;
+                nkept = 0;
+            }
         }
-        if (lane == 0) spre[kMaxK] = P;
-    }
-    __syncthreads();
-    const int64_t P = spre[kMaxK];
-    if (P <= NS / 2) {
-        const uint32_t T_ = table_size(P);
-        const int L = block_count<S>(ltab, T_, P, k, rev, sbeg, spre, xmin, wk);
-        if (!rows) {
-            if (threadIdx.x == 0) cnt[j] = L;
-            return;
+        __syncthreads();
+        const int64_t P = spre[kMaxK];
+        if (P <= NS / 2) {
+            const uint32_t T_ = table_size(P);
+            if (!rows) {
+                insert_entries<S>(ltab, T_, P, threadIdx.x, kBlock, sbeg, spre, rev);
+                __syncthreads();
+                // one pass: count the kept slots, keep the first kBlockKept
+                // of them, clear the table
+                for (uint32_t i = threadIdx.x; i < T_; i += kBlock) {
+                    const S v = ltab[i];
+                    if (slot_kept<S>(v, xmin)) {
+                        const int q = atomicAdd(&nkept, 1);
+                        if (q < kBlockKept) lkept[q] = v;
+                    }
+                    ltab[i] = 0;
+                }
+                __syncthreads();
+                const int L = nkept;
+                if (threadIdx.x == 0) cnt[j] = L;
+                if (stage && L <= kWaveKept && w == 0) {
+                    // sort the kept slots by row (one wavefront) and stage them
+                    int Lp = 1;
+                    while (Lp < L) Lp <<= 1;
+                    for (int i = L + lane; i < Lp; i += kWave) lkept[i] = ~(S)0;
+                    wave_bitonic<S>(lkept, Lp);
+                    S *dst = stage + j * kWaveKept;
+                    for (int i = lane; i < L; i += kWave) dst[i] = lkept[i];
+                    if (lane == 0) staged[j] = 1;
+                }
+                continue;
+            }
+            const int L = block_count<S>(ltab, T_, P, k, rev, sbeg, spre, xmin, wk);
+            block_emit<S, T>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab,
+                             rows + colptr[j], vals + colptr[j]);
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < T_; i += kBlock) ltab[i] = 0;
+        } else {
+            // the global table of this column keeps its counts from the count
+            // pass (its region is private to the column): the fill only emits
+            uint64_t *tab = gtab + gtab_off[h];
+            const uint32_t T_ = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
+            if (!rows) {
+                const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
+                if (threadIdx.x == 0) cnt[j] = L;
+                continue;
+            }
+            const int L = (int)(colptr[j + 1] - colptr[j]);
+            block_emit<uint64_t, T>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
+                                    rows + colptr[j], vals + colptr[j]);
         }
-        block_emit<S, T>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab, rows + colptr[j],
-                         vals + colptr[j]);
-    } else {
-        // the global table of this column keeps its counts from the count
-        // pass (its region is private to the column): the fill only emits
-        uint64_t *tab = gtab + gtab_off[h];
-        const uint32_t T_ = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
-        if (!rows) {
-            const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
-            if (threadIdx.x == 0) cnt[j] = L;
-            return;
-        }
-        const int L = (int)(colptr[j + 1] - colptr[j]);
-        block_emit<uint64_t, T>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
-                                rows + colptr[j], vals + colptr[j]);
     }
 }
 
@@ -503,16 +544,21 @@ template <class T>
 void launch_block(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
                   const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
     if (w.nheavy_h <= 0) return;
+    const bool stage = use_stage(n);
+    // persistent blocks (two per CU fit the LDS table) loop over the columns
+    const unsigned grid = (unsigned)std::min<int64_t>(w.nheavy_h, 2 * (int64_t)device_cu_count());
     if (small_slots(n))
-        hipLaunchKernelGGL((k_snn_block<T, uint32_t>), dim3((unsigned)w.nheavy_h), dim3(kBlock),
+        hipLaunchKernelGGL((k_snn_block<T, uint32_t>), dim3(grid), dim3(kBlock),
                            0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
                            w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
-                           count ? nullptr : rows, vals);
+                           count ? nullptr : rows, vals,
+                           stage ? (uint32_t *)w.stage.get() : nullptr, w.staged.get());
     else
-        hipLaunchKernelGGL((k_snn_block<T, uint64_t>), dim3((unsigned)w.nheavy_h), dim3(kBlock),
+        hipLaunchKernelGGL((k_snn_block<T, uint64_t>), dim3(grid), dim3(kBlock),
                            0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
                            w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
-                           count ? nullptr : rows, vals);
+                           count ? nullptr : rows, vals,
+                           stage ? (uint64_t *)w.stage.get() : nullptr, w.staged.get());
     SEV_LAUNCH_CHECK();
 }
 
