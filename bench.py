@@ -48,7 +48,7 @@ WORKLOADS = {
 }
 NFEAT, NPCS, K, PRUNE, SCALE_MAX, SF = 2000, 50, 20, 1.0 / 15, 10.0, 1e4
 MAXIT = 1000  # IRLBA restarts (the reference's default)
-DEFAULT_WORK = 128  # Lanczos vectors kept by IRLBA (0: npcs + 7, the reference's choice)
+DEFAULT_WORK = 112  # Lanczos vectors kept by IRLBA (0: npcs + 7, the reference's choice)
 WORK = DEFAULT_WORK
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
