@@ -656,9 +656,12 @@ __device__ double hash_normal(uint64_t seed, uint32_t i) {
 // --------------------------------------------------------------------------
 constexpr int kRsBlock = 256;
 
+// Block bid of the rows; column groups cg, cg + ncg, ... of 32 (every group
+// recomputes f for its rows, group 0 stores it).
 template <class MT>
 __device__ __forceinline__ void right_a_body(int64_t n, int j, const double *part2, const MT *mu, const double *V,
-                             double *Fv, double *rpart, unsigned bid) {
+                             double *Fv, double *rpart, unsigned bid, int cg = 0,
+                             int ncg = 1) {
     __shared__ double red[kRsBlock / kWave][kMaxWork];
     const int64_t ncol = n + 2;
     double nrm = 0.0, sumy = 0.0;
@@ -675,11 +678,11 @@ __device__ __forceinline__ void right_a_body(int64_t n, int j, const double *par
         for (int q = 0; q < kRedChunks; ++q) t += part2[q * ncol + i];
         if (mu) t -= sumy * ldmu(mu, i);
         f = t / S - S * V[(int64_t)j * n + i];
-        Fv[i] = f;
+        if (cg == 0) Fv[i] = f;
     }
     const int w = threadIdx.x / kWave, lane = lane_id();
     const int64_t ic = i < n ? i : n - 1;
-    for (int c0 = 0; c0 <= j; c0 += 32) {
+    for (int c0 = 32 * cg; c0 <= j; c0 += 32 * ncg) {
         double v[32];
         // unconditional (clamped) loads: all 32 in flight together
 #pragma unroll
@@ -693,6 +696,7 @@ __device__ __forceinline__ void right_a_body(int64_t n, int j, const double *par
     }
     __syncthreads();
     for (int c = threadIdx.x; c <= j; c += kRsBlock) {
+        if ((c / 32) % ncg != cg) continue;
         double t = 0.0;
         for (int q = 0; q < kRsBlock / kWave; ++q) t += red[q][c];
         rpart[(int64_t)bid * kMaxWork + c] = t;
@@ -703,7 +707,7 @@ template <class MT>
 __global__ void __launch_bounds__(kRsBlock)
 k_right_a(int64_t n, int j, const double *part2, const MT *mu, const double *V, double *Fv,
           double *rpart) {
-    right_a_body<MT>(n, j, part2, mu, V, Fv, rpart, blockIdx.x);
+    right_a_body<MT>(n, j, part2, mu, V, Fv, rpart, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 __device__ __forceinline__ void right_b_body(int64_t n, int j, int nblk, const double *rpart, const double *V,
@@ -1783,11 +1787,13 @@ void launch_right_step(const SlabCsc &op, int j, int work, const double *part2, 
     const int64_t n = op.n;
     const int nblk = (int)ceil_div(n, kRsBlock);
     double *npart = rpart + (size_t)nblk * kMaxWork;
+    // the dots against V[:, 0:j] in column groups of 32, one block each
+    const dim3 ga(nblk, (unsigned)((j + 32) / 32));
     if (op.mu64)
-        hipLaunchKernelGGL(k_right_a<double>, dim3(nblk), dim3(kRsBlock), 0, st, n, j, part2,
+        hipLaunchKernelGGL(k_right_a<double>, ga, dim3(kRsBlock), 0, st, n, j, part2,
                            (const double *)op.mu, V, Fv, rpart);
     else
-        hipLaunchKernelGGL(k_right_a<float>, dim3(nblk), dim3(kRsBlock), 0, st, n, j, part2,
+        hipLaunchKernelGGL(k_right_a<float>, ga, dim3(kRsBlock), 0, st, n, j, part2,
                            (const float *)op.mu, V, Fv, rpart);
     SEV_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_right_b, dim3(nblk), dim3(kRsBlock), 0, st, n, j, nblk, rpart, V, Fv,

@@ -481,18 +481,84 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
 // ---------------------------------------------------------------------------
 // Exact re-rank and certificate (one wavefront per query).
 // ---------------------------------------------------------------------------
+// (v, id) ranks before (ov, oid): larger value, ties by lower id.
+__device__ __forceinline__ bool ranks_before(float v, int32_t id, float ov, int32_t oid) {
+    return v > ov || (v == ov && id < oid);
+}
+
+// Bitonic sort of the 128 pairs held two per lane (position e*64 + lane)
+// into rank order (ranks_before first).
+__device__ __forceinline__ void wave_rank_sort2(float (&v)[2], int32_t (&id)[2]) {
+    const int lane = lane_id();
+    for (int size = 2; size <= 2 * kWave; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            if (stride == kWave) {
+                // partner in the other register of the lane (size 128: the
+                // lower position keeps the better pair)
+                if (ranks_before(v[1], id[1], v[0], id[0])) {
+                    const float tv = v[0];
+                    const int32_t ti = id[0];
+                    v[0] = v[1];
+                    id[0] = id[1];
+                    v[1] = tv;
+                    id[1] = ti;
+                }
+                continue;
+            }
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int pos = e * kWave + lane;
+                const float ov = __shfl_xor(v[e], stride, kWave);
+                const int32_t oi = __shfl_xor(id[e], stride, kWave);
+                const bool up = (pos & size) == 0;
+                const bool lower = (pos & stride) == 0;
+                const bool take = (lower == up) ? ranks_before(ov, oi, v[e], id[e])
+                                                : ranks_before(v[e], id[e], ov, oi);
+                if (take) {
+                    v[e] = ov;
+                    id[e] = oi;
+                }
+            }
+        }
+    }
+}
+
 // Merge the per-chunk candidate lists of each query (one wavefront per list
 // index): the running top KL (lanes 0..KL-1) and the next chunk's list
 // (lanes KL..2KL-1) are sorted together by (value desc, id asc) and the first
 // KL kept.  The merged threshold is the KL-th value when KL candidates were
 // kept over all chunks.  Every candidate outside the merged list lies below
-// its chunk's threshold, which is at most the merged one.
+// its chunk's threshold, which is at most the merged one.  For KL = 64 the
+// running list is one register of the lanes and the chunk's list another.
 template <int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, const int32_t *pcnt,
             int32_t *cidx, float *cthr, int32_t *ccnt) {
-    static_assert(2 * KL <= kWave, "two lists must fit a wavefront");
+    static_assert(2 * KL <= kWave || KL == kWave, "two lists must fit a wavefront");
     const int lane = lane_id();
+    if constexpr (KL == kWave) {
+        const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
+        if (qi >= nq) return;
+        float v[2] = {ninf(), ninf()};
+        int32_t id[2] = {INT_MAX, INT_MAX};
+        int tot = 0;
+        for (int c = 0; c < nchunk; ++c) {
+            const int64_t o = (int64_t)c * nq + qi;
+            const int cnt = pcnt[o];
+            tot += cnt;
+            v[1] = lane < cnt ? pval[o * KL + lane] : ninf();
+            id[1] = lane < cnt ? pidx[o * KL + lane] : INT_MAX;
+            wave_rank_sort2(v, id);
+        }
+        const int cnt = tot < KL ? tot : KL;
+        cidx[qi * KL + lane] = lane < cnt ? id[0] : -1;
+        const float last = __shfl(v[0], KL - 1, kWave);
+        if (lane == 0) {
+            ccnt[qi] = cnt;
+            cthr[qi] = tot >= KL ? last : ninf();
+        }
+        return;
+    }
     const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
     if (qi >= nq) return;
     float v = ninf();
@@ -717,14 +783,16 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     // split the candidates when the query blocks alone cannot fill the device
     int nchunk = 1;
     const int64_t want = (int64_t)device_cu_count() * 8;
-    if (2 * KL <= kWave && qblocks < want)
+    if (KL <= kWave && qblocks < want)
         nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, want / qblocks));
     nchunk = (int)std::min<int64_t>(nchunk, std::max<int64_t>(1, n / (4 * kCT)));
     w.cidx.ensure((size_t)nq * KL);
     w.cthr.ensure((size_t)nq);
     w.ccnt.ensure((size_t)nq);
     {
-        ProfScope ps(MODE == 0 ? "knn_cand_fp32" : (MODE == 1 ? "knn_cand_bf16" : "knn_cand_fp8"),
+        ProfScope ps(MODE == 0 ? "knn_cand_fp32"
+                               : (MODE == 1 ? (KL > 32 ? "knn_cand_bf16_kl64" : "knn_cand_bf16")
+                                            : "knn_cand_fp8"),
                      st, 0.0,
                      2.0 * (double)nq * (double)n * (double)d);
         if (nchunk == 1) {
@@ -734,7 +802,7 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
                                w.cthr.get(), w.ccnt.get(), (float *)nullptr, seg, row_sample,
                                w.scf.get(), w.hn.get());
             SEV_LAUNCH_CHECK();
-        } else if constexpr (2 * KL <= kWave) {
+        } else if constexpr (KL <= kWave) {
             w.pidx.ensure((size_t)nchunk * nq * KL);
             w.pval.ensure((size_t)nchunk * nq * KL);
             w.pthr.ensure((size_t)nchunk * nq);
@@ -769,18 +837,33 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
 template <int MODE, int KP, int QW, int KL, int KP32>
 void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
               KnnWork &w, int32_t *idx, float *dist, hipStream_t st, int64_t *n_fallback) {
-    w.nfail.ensure(2);
+    w.nfail.ensure(3);
     w.fail.ensure((size_t)n);
-    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 2 * sizeof(int32_t), st));
+    SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 3 * sizeof(int32_t), st));
     cand_rerank<MODE, KP, QW, KL>(Z, n, d, ld, k, include_self, nullptr, n, w, idx, dist,
                                   w.nfail.get(), w.fail.get(), st);
     int32_t nf = 0;
     SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
     const int64_t *left = w.fail.get();
+    if constexpr (MODE == 1 && KL == 32 && KP32 > 0) {
+        if (nf > 0) {
+            // BF16 rows without a certificate: BF16 candidates again with a
+            // list of 64 (the operands of the first pass), whose larger gap
+            // to the k-th neighbour certifies most of them
+            w.fail2.ensure((size_t)nf);
+            SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
+                                   hipMemcpyDeviceToDevice, st));
+            cand_rerank<1, KP, 1, 64>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
+                                      dist, w.nfail.get() + 2, w.fail.get(), st, nullptr,
+                                      nullptr, false);
+            SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 2, sizeof nf, hipMemcpyDeviceToHost, st));
+            SEV_HIP(hipStreamSynchronize(st));
+        }
+    }
     if constexpr (MODE == 1 && KP32 > 0) {
         if (nf > 0) {
-            // BF16 rows without a certificate: FP32 candidates for those rows
+            // rows still without a certificate: FP32 candidates for those rows
             w.fail2.ensure((size_t)nf);
             SEV_HIP(hipMemcpyAsync(w.fail2.get(), w.fail.get(), sizeof(int64_t) * nf,
                                    hipMemcpyDeviceToDevice, st));
