@@ -255,6 +255,19 @@ int sev_find_variable_features_method(int64_t m, int64_t n, const int64_t *colpt
                                       int64_t nfeatures, int64_t num_bins, int binning,
                                       double alpha_thresh, int64_t *selected, double *metric);
 
+/* ---- MatrixMarket (10X matrix.mtx[.gz]) --------------------------------- */
+/* readMM (src/input.jl:26-115) up to its sparse() call: checks the header
+ * ("%%MatrixMarket matrix coordinate real|integer general"), skips blank and
+ * '%' lines, reads the size line.  rows/cols/entries as written; *is_real is
+ * 1 for the real field (values as Float64), 0 for integer (Int64).  A path
+ * ending in ".gz" is inflated (GzipDecompressorStream in maybe_gzip). */
+int sev_mtx_info(const char *path, int64_t *rows, int64_t *cols, int64_t *entries,
+                 int32_t *is_real);
+/* The entries in file order: I[i], J[i] 1-based row and column, the value in
+ * Vi (integer field) or Vf (real field); the other may be NULL.  Arrays of
+ * `entries` elements.  Errors carry readMM's messages (sev_last_error). */
+int sev_mtx_read(const char *path, int64_t *I, int64_t *J, int64_t *Vi, double *Vf);
+
 /* ---- device-resident pipeline ----------------------------------------- */
 typedef struct sev_pipeline sev_pipeline;
 
@@ -265,6 +278,14 @@ int sev_pipeline_synchronize(sev_pipeline *p);
 /* Load a cells x genes count matrix (1-based Int64 structure, Int32 counts). */
 int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_t *colptr,
                              const int64_t *rowval, const int32_t *nzval);
+/* Load a 10X matrix.mtx[.gz] (genes x cells) as the cells x genes counts:
+ * readMM, sparse(I, J, V) (duplicates summed, stored zeros kept) and the
+ * copy(X') of _read_10X (src/input.jl:199-208), the transpose and the sums
+ * built on the device from the parsed entries.  Counts must fit Int32 (a
+ * real field must hold integers, as convert_counts converts it).  *out_m
+ * (cells), *out_n (genes) and *out_nnz may be NULL. */
+int sev_pipeline_load_mtx(sev_pipeline *p, const char *path, int64_t *out_m, int64_t *out_n,
+                          int64_t *out_nnz);
 /* Generate the seeded negative-binomial matrix on the device: entry (g, c)
  * is 0 when u < p0[2g+class(c)], else drawn by inverse CDF with
  * q[2g+class(c)] and size r; class(c) = 1 for a boost_frac share of cells. */

@@ -11,7 +11,8 @@ The directory name is not a Python identifier; load it with
 module name ``severo_jl_amd``.
 """
 from ._lib import ConvergenceError, SeveroError, load  # noqa: F401
-from .api import (CenteredMatrix, LinearEmbedding, SVD, ann, cluster, copy_probe, embedding, irlba,  # noqa: F401
+from .api import (CenteredMatrix, LinearEmbedding, NamedCounts, SVD, ann, cluster, copy_probe,  # noqa: F401
+                  embedding, irlba, make_unique, read_10X, read_mm,
                   find_variable_features, irlba_operator,
                   jaccard_index, nearest_neighbours, normalize_cells, pca, scale_features,
                   shared_nearest_neighbours)

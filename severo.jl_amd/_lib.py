@@ -61,6 +61,9 @@ SIGNATURES = {
     "sev_pipeline_destroy": (None, [_p]),
     "sev_pipeline_synchronize": (_int, [_p]),
     "sev_pipeline_load_counts": (_int, [_p, _i64, _i64, _p, _p, _p]),
+    "sev_pipeline_load_mtx": (_int, [_p, C.c_char_p, _pi64, _pi64, _pi64]),
+    "sev_mtx_info": (_int, [C.c_char_p, _pi64, _pi64, _pi64, C.POINTER(C.c_int32)]),
+    "sev_mtx_read": (_int, [C.c_char_p, _p, _p, _p, _p]),
     "sev_pipeline_synth_counts": (_int, [_p, _i64, _i64, C.c_uint64, _dbl, _p, _p, _dbl]),
     "sev_pipeline_synth_batch": (_int, [_p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
     "sev_pipeline_set_batch": (_int, [_p, _i64, _p]),

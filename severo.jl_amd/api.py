@@ -487,3 +487,111 @@ def copy_probe() -> float:
     r = C.c_double(0.0)
     check(_lib.load().sev_selftest(3, C.byref(r)))
     return r.value
+
+
+# ---------------------------------------------------------------------------
+# 10X input  (src/input.jl:26-115 readMM, :199-208 _read_10X, :634-646
+# read_10X, :776-790 convert_counts, src/utils.jl make_unique!)
+# ---------------------------------------------------------------------------
+def read_mm(fname):
+    """readMM: the MatrixMarket coordinate file (``.gz`` inflated) as a CSC
+    matrix of its own orientation, built as Julia's ``sparse(I, J, V, rows,
+    cols)`` builds it (duplicates summed, stored zeros kept).  Int64 values
+    for the integer field, Float64 for real.  The text is parsed by the
+    library's host threads."""
+    import os
+    path = os.fsencode(fname)
+    L = _lib.load()
+    rows, cols, entries, is_real = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+    check(L.sev_mtx_info(path, C.byref(rows), C.byref(cols), C.byref(entries),
+                         C.byref(is_real)))
+    E = entries.value
+    I = np.empty(E, np.int64)
+    J = np.empty(E, np.int64)
+    V = np.empty(E, np.float64 if is_real.value else np.int64)
+    check(L.sev_mtx_read(path, _ptr(I), _ptr(J), None if is_real.value else _ptr(V),
+                         _ptr(V) if is_real.value else None))
+    if E and (I.min() < 1 or I.max() > rows.value or J.min() < 1 or J.max() > cols.value):
+        raise ValueError("matrix entry index out of range")
+    X = sp.coo_matrix((V, (I - 1, J - 1)), shape=(rows.value, cols.value)).tocsc()
+    X.sort_indices()
+    return X
+
+
+def make_unique(names, sep="."):
+    """make_unique!: the second and later copies of a name get ``sep`` and a
+    counter, skipping names already present."""
+    n = len(names)
+    seen = {}
+    dup = [False] * n
+    for i, x in enumerate(names):
+        if x not in seen:
+            seen[x] = 1
+        else:
+            dup[i] = True
+    out = list(names)
+    for i, x in enumerate(names):
+        if dup[i]:
+            cnt = seen[x]
+            y = f"{x}{sep}{cnt}"
+            while y in seen and cnt <= n:
+                cnt += 1
+                y = f"{x}{sep}{cnt}"
+            out[i] = y
+            seen[x] = cnt + 1
+            seen[y] = 1
+    return out
+
+
+@dataclass
+class NamedCounts:
+    """A cells x genes count matrix with its barcodes and feature names
+    (Severo's NamedCountMatrix).  ``X`` is None when the counts were loaded
+    into a pipeline and left on the device."""
+    X: sp.csc_matrix | None
+    barcodes: list
+    features: list
+
+
+def _read_lines(fname):
+    import gzip
+    opener = gzip.open if str(fname).endswith(".gz") else open
+    with opener(fname, "rt") as f:
+        return [ln.rstrip("\r\n") for ln in f]
+
+
+def read_10X(dirname, gene_column: int = 2, unique_features: bool = True, pipeline=None):
+    """read_10X: matrix.mtx, genes.tsv and barcodes.tsv (or the v3 names
+    matrix.mtx.gz, features.tsv.gz, barcodes.tsv.gz) of a 10X directory.
+    The counts are parsed by host threads and transposed to cells x genes on
+    the GPU; with ``pipeline`` they stay there (X is None), otherwise they are
+    copied back.  Feature names take column ``gene_column`` with '_' replaced
+    by '-' and, with ``unique_features``, made unique."""
+    import os
+    if not os.path.isdir(dirname):
+        raise ValueError(f"Directory {dirname} does not exist")
+    v3 = os.path.isfile(os.path.join(dirname, "features.tsv.gz"))
+    feature_file = os.path.join(dirname, "features.tsv.gz" if v3 else "genes.tsv")
+    barcodes_file = os.path.join(dirname, "barcodes.tsv.gz" if v3 else "barcodes.tsv")
+    matrix_file = os.path.join(dirname, "matrix.mtx.gz" if v3 else "matrix.mtx")
+    for f in (feature_file, barcodes_file, matrix_file):
+        if not os.path.isfile(f):
+            raise ValueError(f"Couldn't locate all components: {feature_file}, "
+                             f"{barcodes_file}, {matrix_file}")
+    barcodes = _read_lines(barcodes_file)
+    features = [ln.split("\t")[gene_column - 1] for ln in _read_lines(feature_file)]
+    features = [f.replace("_", "-") for f in features]
+    if unique_features:
+        features = make_unique(features)
+    from .pipeline import Pipeline
+    p = pipeline if pipeline is not None else Pipeline()
+    try:
+        m, n, _ = p.load_mtx(matrix_file)
+        if len(barcodes) != m or len(features) != n:
+            raise ValueError(f"matrix is {n} x {m} (genes x cells) but there are "
+                             f"{len(features)} features and {len(barcodes)} barcodes")
+        X = None if pipeline is not None else p.get_counts()
+    finally:
+        if pipeline is None:
+            p.close()
+    return NamedCounts(X, barcodes, features)

@@ -20,6 +20,8 @@
 #include "../../include/severo_hip.h"
 #include "sev_kernels.hpp"
 
+#include <atomic>
+
 using namespace sev;
 
 thread_local Prof *sev::t_prof = nullptr;
@@ -463,6 +465,85 @@ int sev_pipeline_load_counts(sev_pipeline *p, int64_t m, int64_t n, const int64_
         p->have_counts = true;
         p->batched = false;
         p->invalidate_from_counts();
+    });
+}
+
+int sev_pipeline_load_mtx(sev_pipeline *p, const char *path, int64_t *out_m, int64_t *out_n,
+                          int64_t *out_nnz) {
+    return guarded([&] {
+        ProfInstall pi_(&p->prof);
+        hipStream_t st = p->st();
+        MtxReader rd(path);
+        const int64_t genes = rd.rows(), cells = rd.cols(), E = rd.entries();
+        SEV_REQUIRE(cells < (int64_t)0x7FFFFFFF && genes < (int64_t)0x7FFFFFFF,
+                    "matrix too large for int32 cell and gene indices");
+        SEV_REQUIRE(E < (int64_t)0x7FFFFFFF, "more than 2^31 - 1 entries");
+        // (gene << 32 | cell, value) on the host, then to the device
+        std::vector<uint64_t> hk((size_t)E);
+        std::vector<int64_t> hv((size_t)E);
+        std::atomic<int64_t> bad_index(-1), inexact(-1);
+        const bool real = rd.is_real();
+        rd.parse(
+            [&](int64_t i, int64_t r, int64_t c, int64_t vi, double vf) {
+                if (r < 1 || r > genes || c < 1 || c > cells) {
+                    int64_t none = -1;
+                    bad_index.compare_exchange_strong(none, i);
+                    return;
+                }
+                int64_t v = vi;
+                if (real) {
+                    // convert_counts turns a real matrix into Int32 counts
+                    if (!(vf == std::floor(vf)) || !(std::fabs(vf) < 9.0e18)) {
+                        int64_t none = -1;
+                        inexact.compare_exchange_strong(none, i);
+                        return;
+                    }
+                    v = (int64_t)vf;
+                }
+                hk[(size_t)i] = ((uint64_t)(r - 1) << 32) | (uint64_t)(c - 1);
+                hv[(size_t)i] = v;
+            },
+            nullptr);
+        SEV_REQUIRE(bad_index.load() < 0,
+                    "matrix entry " + std::to_string(bad_index.load() + 1) +
+                        ": row or column index out of range");
+        SEV_REQUIRE(inexact.load() < 0, "matrix entry " + std::to_string(inexact.load() + 1) +
+                                            ": a real count that is not an integer");
+        DevBuf<uint64_t> dk;
+        DevBuf<int64_t> dv;
+        dk.ensure((size_t)E);
+        dv.ensure((size_t)E);
+        if (E > 0 && (size_t)E * sizeof(uint64_t) >= kStageMinBytes) {
+            staged_upload<uint64_t, uint64_t>(
+                dk.get(), hk.data(), E,
+                [](const uint64_t *in, uint64_t *out, int64_t c, int) {
+                    std::memcpy(out, in, sizeof(uint64_t) * (size_t)c);
+                },
+                st);
+            staged_upload<int64_t, int64_t>(
+                dv.get(), hv.data(), E,
+                [](const int64_t *in, int64_t *out, int64_t c, int) {
+                    std::memcpy(out, in, sizeof(int64_t) * (size_t)c);
+                },
+                st);
+        } else if (E > 0) {
+            SEV_HIP(hipMemcpyAsync(dk.get(), hk.data(), sizeof(uint64_t) * (size_t)E,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipMemcpyAsync(dv.get(), hv.data(), sizeof(int64_t) * (size_t)E,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipStreamSynchronize(st));
+        }
+        int64_t nnz = 0;
+        coo_to_csc(genes, dk.get(), dv.get(), E, p->cp, p->row, p->cnt, nnz, st);
+        p->m = cells;
+        p->g = genes;
+        p->nnz = nnz;
+        p->have_counts = true;
+        p->batched = false;
+        p->invalidate_from_counts();
+        if (out_m) *out_m = cells;
+        if (out_n) *out_n = genes;
+        if (out_nnz) *out_nnz = nnz;
     });
 }
 

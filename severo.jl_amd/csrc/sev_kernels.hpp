@@ -6,7 +6,10 @@
 #include "sev_common.hpp"
 #include "sev_prof.hpp"
 
+#include <functional>
 #include <map>
+#include <memory>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -16,8 +19,49 @@ namespace sev {
 // Runs fn(ctx) and turns a thrown error into the status and message the C
 // ABI reports (sev_last_error).
 int guarded_call(void (*fn)(void *), void *ctx);
+// guarded_call for a callable
+template <class F>
+int guarded_fn(F &&f) {
+    using G = typename std::remove_reference<F>::type;
+    return guarded_call([](void *c) { (*(G *)c)(); }, (void *)&f);
+}
 // Throws unless a gfx950 device is current.
 void require_device();
+
+// ---- sev_mtx.cpp ------------------------------------------------------------
+// A MatrixMarket coordinate file (plain or .gz): the header, comment and
+// size lines are read on construction; parse() calls emit(i, row, col,
+// ivalue, fvalue) for the entries i in [0, entries) (1-based row/col as
+// written; ivalue for the integer field, fvalue for the real one) from
+// several host threads, and done(first, count, thread) after each thread's
+// range.  Errors throw std::runtime_error with readMM's messages.
+class MtxReader {
+  public:
+    explicit MtxReader(const char *path);
+    ~MtxReader();
+    int64_t rows() const { return rows_; }
+    int64_t cols() const { return cols_; }
+    int64_t entries() const { return entries_; }
+    bool is_real() const { return real_; }
+    void parse(const std::function<void(int64_t, int64_t, int64_t, int64_t, double)> &emit,
+               const std::function<void(int64_t, int64_t, int)> &done);
+
+  private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+    int64_t rows_ = 0, cols_ = 0, entries_ = 0;
+    bool real_ = false;
+};
+
+// ---- sev_coo.hip -------------------------------------------------------------
+// The cells x genes CSC of the transpose of a genes x cells coordinate list:
+// keys[i] = gene << 32 | cell and vals[i] (both overwritten), n entries.
+// Entries with equal keys are summed (Julia's sparse(I, J, V) combines
+// duplicates with +; stored zeros stay).  Outputs colptr (ngenes + 1), the
+// cell rows (ascending in each column) and the int32 sums; *nnz receives
+// the number of distinct keys.  Throws if a sum leaves the int32 range.
+void coo_to_csc(int64_t ngenes, uint64_t *keys, int64_t *vals, int64_t n, DevBuf<int64_t> &cp,
+                DevBuf<int32_t> &row, DevBuf<int32_t> &val, int64_t &nnz, hipStream_t st);
 
 // ---- sev_noalloc.hip -------------------------------------------------------
 // Resolves the device address of the allocation-free entry points' static

@@ -270,4 +270,60 @@ function cluster(SNN::NamedArray{T,2,SparseMatrixCSC{T,Int64}}; algorithm=:louva
     NamedArray(labels, (SNN.dicts[1],), (:cells,))
 end
 
+# ---------------------------------------------------------------- 10X input
+# src/input.jl:26-115 readMM: the text parsed by the library's host threads,
+# then sparse(I, J, V, rows, cols) as the reference builds it
+function readMM(fname::AbstractString)
+    rows, cols, entries, isreal = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0), Ref{Int32}(0)
+    check(ccall((:sev_mtx_info, libsev), Cint,
+                (Cstring, Ref{Int64}, Ref{Int64}, Ref{Int64}, Ref{Int32}),
+                fname, rows, cols, entries, isreal))
+    I = Vector{Int64}(undef, entries[])
+    J = Vector{Int64}(undef, entries[])
+    if isreal[] != 0
+        V = Vector{Float64}(undef, entries[])
+        check(ccall((:sev_mtx_read, libsev), Cint,
+                    (Cstring, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                    fname, I, J, C_NULL, V))
+    else
+        V = Vector{Int64}(undef, entries[])
+        check(ccall((:sev_mtx_read, libsev), Cint,
+                    (Cstring, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                    fname, I, J, V, C_NULL))
+    end
+    sparse(I, J, V, rows[], cols[])
+end
+
+# src/input.jl:199-208 and :634-646: the counts matrix parsed on the host and
+# transposed (duplicates summed) on the GPU, as Int32 counts
+function read_10X(dirname::AbstractString; gene_column::Int64=2, unique_features::Bool=true)
+    isdir(dirname) || throw(Severo.ParseError_10X("Directory $dirname does not exist"))
+    is_v3 = isfile(joinpath(dirname, "features.tsv.gz"))
+    feature_file = joinpath(dirname, is_v3 ? "features.tsv.gz" : "genes.tsv")
+    barcodes_file = joinpath(dirname, is_v3 ? "barcodes.tsv.gz" : "barcodes.tsv")
+    matrix_file = joinpath(dirname, is_v3 ? "matrix.mtx.gz" : "matrix.mtx")
+    if !(isfile(feature_file) && isfile(barcodes_file) && isfile(matrix_file))
+        throw(Severo.ParseError_10X("Couldn't locate all components: $feature_file, $barcodes_file, $matrix_file"))
+    end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:sev_pipeline_create, libsev), Cint, (Ref{Ptr{Cvoid}},), h))
+    X = try
+        m, n, nnz = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+        check(ccall((:sev_pipeline_load_mtx, libsev), Cint,
+                    (Ptr{Cvoid}, Cstring, Ref{Int64}, Ref{Int64}, Ref{Int64}),
+                    h[], matrix_file, m, n, nnz))
+        colptr = Vector{Int64}(undef, n[] + 1)
+        rowval = Vector{Int64}(undef, nnz[])
+        nzval = Vector{Int32}(undef, nnz[])
+        check(ccall((:sev_pipeline_get_counts, libsev), Cint,
+                    (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}), h[], colptr, rowval, nzval))
+        SparseMatrixCSC(m[], n[], colptr, rowval, nzval)
+    finally
+        ccall((:sev_pipeline_destroy, libsev), Cvoid, (Ptr{Cvoid},), h[])
+    end
+    barcodes = Severo._readlines(barcodes_file)
+    features = Severo.readDelim(feature_file, header=false)[:, gene_column]
+    Severo.convert_counts(X, features, barcodes, unique_features=unique_features)
+end
+
 end # module

@@ -8,6 +8,7 @@ normalize_cells -> Y[:, features] -> scale_features -> PCA (IRLBA)
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 import scipy.sparse as sp
@@ -84,6 +85,16 @@ class Pipeline:
         check(self._L.sev_pipeline_load_counts(self._h, X.shape[0], X.shape[1], _ptr(colptr),
                                                _ptr(rowval), _ptr(vals)))
         self.cells = None
+
+    def load_mtx(self, path):
+        """Load a 10X matrix.mtx[.gz] (genes x cells) as the cells x genes
+        counts, built on the device (readMM + sparse + copy(X') of
+        _read_10X, src/input.jl:78-115,199-208).  Returns (cells, genes, nnz)."""
+        m, n, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self._L.sev_pipeline_load_mtx(self._h, os.fsencode(path), C.byref(m), C.byref(n),
+                                            C.byref(nnz)))
+        self.cells = None
+        return m.value, n.value, nnz.value
 
     def load_counts_julia(self, m, n, colptr, rowval, nzval):
         """Upload counts given in Julia's layout as they are: 1-based Int64
