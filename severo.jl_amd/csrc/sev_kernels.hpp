@@ -432,6 +432,7 @@ struct SnnWork {
     DevBuf<int64_t> rptr, scratch, htab_off, heavyP;
     DevBuf<uint64_t> htab;
     int64_t nheavy_h = 0;
+    int64_t nheavy_cls[3] = {0, 0, 0};  // heavy columns per table class
 };
 template <class T>
 int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,

@@ -334,16 +334,16 @@ k_snn_stage_copy(int64_t n, const S *stage, const int32_t *staged, const T *wtab
 
 // Count the candidates of one column into tab (nslots slots, zeroed here)
 // with the whole block, and return the number of kept entries.
-template <class S>
+template <class S, int BS>
 __device__ int block_count(S *tab, uint32_t nslots, int64_t P, int k, const int32_t *rev,
                            const int64_t *sbeg, const int64_t *spre, int xmin, int *wk) {
     const int lane = lane_id(), w = threadIdx.x / kWave;
-    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) tab[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nslots; i += BS) tab[i] = 0;
     __syncthreads();
-    insert_entries<S>(tab, nslots, P, threadIdx.x, kBlock, sbeg, spre, rev);
+    insert_entries<S>(tab, nslots, P, threadIdx.x, BS, sbeg, spre, rev);
     __syncthreads();
     int nk = 0;
-    for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) nk += slot_kept<S>(tab[i], xmin);
+    for (uint32_t i = threadIdx.x; i < nslots; i += BS) nk += slot_kept<S>(tab[i], xmin);
     int wsum = nk;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
@@ -353,7 +353,7 @@ __device__ int block_count(S *tab, uint32_t nslots, int64_t P, int k, const int3
     if (lane == kWave - 1) wk[w] = wsum;
     __syncthreads();
     int L = 0;
-    for (int q = 0; q < kWaves; ++q) L += wk[q];
+    for (int q = 0; q < BS / kWave; ++q) L += wk[q];
     __syncthreads();
     return L;
 }
@@ -361,7 +361,7 @@ __device__ int block_count(S *tab, uint32_t nslots, int64_t P, int k, const int3
 // Write the kept entries of tab sorted by row: gathered into kept (when
 // given and large enough) and sorted there, or sorted in place in the table
 // after the other slots are set to the largest key.
-template <class S, class T>
+template <class S, class T, int BS>
 __device__ void block_emit(S *tab, uint32_t nslots, S *kept, int64_t kcap, int L, int xmin,
                            int *nkept, const T *wtab, int32_t *rows, T *vals) {
     S *src;
@@ -370,52 +370,55 @@ __device__ void block_emit(S *tab, uint32_t nslots, S *kept, int64_t kcap, int L
     if (kept && L <= kcap) {
         if (threadIdx.x == 0) *nkept = 0;
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < nslots; i += kBlock) {
+        for (uint32_t i = threadIdx.x; i < nslots; i += BS) {
             const S v = tab[i];
             if (slot_kept<S>(v, xmin)) kept[atomicAdd(nkept, 1)] = v;
         }
         __syncthreads();
-        for (int64_t i = L + threadIdx.x; i < Lp; i += kBlock) kept[i] = ~(S)0;
+        for (int64_t i = L + threadIdx.x; i < Lp; i += BS) kept[i] = ~(S)0;
         __syncthreads();
         block_bitonic<S>(kept, Lp);
         src = kept;
     } else {
-        for (uint32_t i = threadIdx.x; i < nslots; i += kBlock)
+        for (uint32_t i = threadIdx.x; i < nslots; i += BS)
             if (!slot_kept<S>(tab[i], xmin)) tab[i] = ~(S)0;
         __syncthreads();
         block_bitonic<S>(tab, nslots);
         src = tab;
     }
-    for (int64_t i = threadIdx.x; i < L; i += kBlock) {
+    for (int64_t i = threadIdx.x; i < L; i += BS) {
         const S v = src[i];
         rows[i] = slot_row<S>(v);
         vals[i] = wtab[slot_count<S>(v)];
     }
 }
 
-// Heavy columns: persistent blocks take one column at a time, table in LDS
-// (P <= half the LDS table) or in global memory at gtab + gtab_off[h] (64-bit
-// slots: 2/3 of the region is the table, 1/3 the sort buffer).  With stage given, the count pass
-// writes the sorted kept slots of LDS-table columns with at most kWaveKept
-// of them to stage (staged[j] = 1), as the wave path does, and the fill pass
-// skips those; it recounts the other LDS-table columns and reuses the global
-// tables filled by the count pass.
-template <class T, class S>
-__global__ void __launch_bounds__(kBlock)
+// Heavy columns: persistent blocks of BS threads take one column at a time
+// from their list, with a hash table of nslots_max slots in dynamic LDS for
+// columns with P <= lds_maxP (table size table_size(P), at most nslots_max,
+// so at most 3/4 full), else in global memory at gtab + gtab_off[h] (64-bit
+// slots: 2/3 of the region is the table, 1/3 the sort buffer).  With stage
+// given, the count pass writes the sorted kept slots of LDS-table columns
+// with at most kWaveKept of them to stage (staged[j] = 1), as the wave path
+// does, and the fill pass skips those; it recounts the other LDS-table
+// columns and reuses the global tables filled by the count pass.
+template <class T, class S, int BS>
+__global__ void __launch_bounds__(BS)
 k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
-            const int32_t *rev, const int32_t *heavy, int64_t nheavy, int32_t *cnt,
-            uint64_t *gtab, const int64_t *gtab_off, const T *wtab, const int64_t *colptr,
-            int32_t *rows, T *vals, S *stage, int32_t *staged) {
-    constexpr int NS = kBlockTableBytes / (int)sizeof(S);
-    __shared__ S ltab[NS];
+            const int32_t *rev, const int32_t *heavy, int64_t nheavy, uint32_t nslots_max,
+            int64_t lds_maxP, int32_t *cnt, uint64_t *gtab,
+            const int64_t *gtab_off, const T *wtab, const int64_t *colptr, int32_t *rows,
+            T *vals, S *stage, int32_t *staged) {
+    extern __shared__ unsigned char snn_lds[];
+    S *ltab = (S *)snn_lds;
     __shared__ S lkept[kBlockKept];
     __shared__ int64_t sbeg[kMaxK], spre[kMaxK + 1];
-    __shared__ int wk[kWaves];
+    __shared__ int wk[BS / kWave];
     __shared__ int nkept;
     const int lane = lane_id(), w = threadIdx.x / kWave;
     // the LDS table is clear at the start of every column: cleared here
     // once, and after each column by the pass that reads it
-    for (int i = threadIdx.x; i < NS; i += kBlock) ltab[i] = 0;
+    for (uint32_t i = threadIdx.x; i < nslots_max; i += BS) ltab[i] = 0;
     for (int64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
         const int64_t j = heavy[h];
         if (rows && staged[j]) continue;  // emitted by the count pass
@@ -434,14 +437,14 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
         }
         __syncthreads();
         const int64_t P = spre[kMaxK];
-        if (P <= NS / 2) {
-            const uint32_t T_ = table_size(P);
+        if (P <= lds_maxP) {
+            const uint32_t T_ = table_size(P) < nslots_max ? table_size(P) : nslots_max;
             if (!rows) {
-                insert_entries<S>(ltab, T_, P, threadIdx.x, kBlock, sbeg, spre, rev);
+                insert_entries<S>(ltab, T_, P, threadIdx.x, BS, sbeg, spre, rev);
                 __syncthreads();
                 // one pass: count the kept slots, keep the first kBlockKept
                 // of them, clear the table
-                for (uint32_t i = threadIdx.x; i < T_; i += kBlock) {
+                for (uint32_t i = threadIdx.x; i < T_; i += BS) {
                     const S v = ltab[i];
                     if (slot_kept<S>(v, xmin)) {
                         const int q = atomicAdd(&nkept, 1);
@@ -464,24 +467,25 @@ k_snn_block(const int32_t *nn, int64_t n, int k, int xmin, const int64_t *rptr,
                 }
                 continue;
             }
-            const int L = block_count<S>(ltab, T_, P, k, rev, sbeg, spre, xmin, wk);
-            block_emit<S, T>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab,
-                             rows + colptr[j], vals + colptr[j]);
+            const int L = block_count<S, BS>(ltab, T_, P, k, rev, sbeg, spre, xmin, wk);
+            block_emit<S, T, BS>(ltab, T_, lkept, kBlockKept, L, xmin, &nkept, wtab,
+                                 rows + colptr[j], vals + colptr[j]);
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < T_; i += kBlock) ltab[i] = 0;
+            for (uint32_t i = threadIdx.x; i < T_; i += BS) ltab[i] = 0;
         } else {
             // the global table of this column keeps its counts from the count
             // pass (its region is private to the column): the fill only emits
             uint64_t *tab = gtab + gtab_off[h];
             const uint32_t T_ = (uint32_t)((gtab_off[h + 1] - gtab_off[h]) / 3 * 2);
             if (!rows) {
-                const int L = block_count<uint64_t>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
+                const int L =
+                    block_count<uint64_t, BS>(tab, T_, P, k, rev, sbeg, spre, xmin, wk);
                 if (threadIdx.x == 0) cnt[j] = L;
                 continue;
             }
             const int L = (int)(colptr[j + 1] - colptr[j]);
-            block_emit<uint64_t, T>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
-                                    rows + colptr[j], vals + colptr[j]);
+            block_emit<uint64_t, T, BS>(tab, T_, tab + T_, T_ / 2, L, xmin, &nkept, wtab,
+                                        rows + colptr[j], vals + colptr[j]);
         }
     }
 }
@@ -495,9 +499,18 @@ unsigned grid_for(int64_t n, int per_cu = 8) {
 // rows below 2^25 (and counts below 128) fit 32-bit slots
 bool small_slots(int64_t n) { return n < ((int64_t)1 << 25) - 1; }
 
-int64_t block_lds_maxP(int64_t n) {
-    return small_slots(n) ? kBlockTableBytes / 4 / 2 : kBlockTableBytes / 8 / 2;
-}
+// The heavy columns in three classes: an LDS table of kBlockTableBytes in
+// blocks of kBlock threads (two per CU) for P up to half its slots, one of
+// kBigTableBytes in blocks of 1024 threads (one per CU) for P up to 3/4 of
+// its slots, and global-memory tables above that.
+constexpr int64_t kBigTableBytes = 131072;
+constexpr int kBigBlock = 1024;
+
+int64_t slot_bytes(int64_t n) { return small_slots(n) ? 4 : 8; }
+uint32_t small_table_slots(int64_t n) { return (uint32_t)(kBlockTableBytes / slot_bytes(n)); }
+uint32_t big_table_slots(int64_t n) { return (uint32_t)(kBigTableBytes / slot_bytes(n)); }
+int64_t small_maxP(int64_t n) { return small_table_slots(n) / 2; }
+int64_t big_maxP(int64_t n) { return (int64_t)big_table_slots(n) * 3 / 4; }
 
 // Staging of the wave columns' kept entries (n x kWaveKept slots) when it
 // fits this budget; otherwise the fill recounts those columns.
@@ -540,26 +553,56 @@ void launch_stage_copy(int64_t n, SnnWork &w, const T *wtab, const int64_t *colp
     SEV_LAUNCH_CHECK();
 }
 
+template <class T, class S, int BS>
+void launch_block_class(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
+                        int64_t first, int64_t cnt_cols, uint32_t slots, int64_t maxP,
+                        int per_cu, const T *wtab, const int64_t *colptr, int32_t *rows,
+                        T *vals, hipStream_t st) {
+    if (cnt_cols <= 0) return;
+    const size_t lds = (size_t)slots * sizeof(S);
+    if (lds > 65536) {
+        static bool attr = false;
+        if (!attr) {
+            SEV_HIP(hipFuncSetAttribute((const void *)k_snn_block<T, S, BS>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            attr = true;
+        }
+    }
+    const bool stage = use_stage(n);
+    const unsigned grid =
+        (unsigned)std::min<int64_t>(cnt_cols, (int64_t)per_cu * device_cu_count());
+    hipLaunchKernelGGL((k_snn_block<T, S, BS>), dim3(grid), dim3(BS), lds, st, nn, n, k, xmin,
+                       w.rptr.get(), w.rev.get(), w.heavy.get() + first, cnt_cols, slots, maxP,
+                       w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
+                       count ? nullptr : rows, vals, stage ? (S *)w.stage.get() : nullptr,
+                       w.staged.get());
+    SEV_LAUNCH_CHECK();
+}
+
+template <class T, class S>
+void launch_block_s(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
+                    const T *wtab, const int64_t *colptr, int32_t *rows, T *vals,
+                    hipStream_t st) {
+    // persistent blocks: two per CU fit the small table, one the big one
+    launch_block_class<T, S, kBlock>(count, nn, n, k, xmin, w, 0, w.nheavy_cls[0],
+                                     small_table_slots(n), small_maxP(n), 2, wtab, colptr, rows,
+                                     vals, st);
+    launch_block_class<T, S, kBigBlock>(count, nn, n, k, xmin, w, w.nheavy_cls[0],
+                                        w.nheavy_cls[1], big_table_slots(n), big_maxP(n), 1,
+                                        wtab, colptr, rows, vals, st);
+    launch_block_class<T, S, kBlock>(count, nn, n, k, xmin, w,
+                                     w.nheavy_cls[0] + w.nheavy_cls[1], w.nheavy_cls[2], 0, -1, 2,
+                                     wtab, colptr, rows, vals, st);
+}
+
 template <class T>
 void launch_block(bool count, const int32_t *nn, int64_t n, int k, int xmin, SnnWork &w,
                   const T *wtab, const int64_t *colptr, int32_t *rows, T *vals, hipStream_t st) {
     if (w.nheavy_h <= 0) return;
-    const bool stage = use_stage(n);
-    // persistent blocks (two per CU fit the LDS table) loop over the columns
-    const unsigned grid = (unsigned)std::min<int64_t>(w.nheavy_h, 2 * (int64_t)device_cu_count());
     if (small_slots(n))
-        hipLaunchKernelGGL((k_snn_block<T, uint32_t>), dim3(grid), dim3(kBlock),
-                           0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
-                           w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
-                           count ? nullptr : rows, vals,
-                           stage ? (uint32_t *)w.stage.get() : nullptr, w.staged.get());
+        launch_block_s<T, uint32_t>(count, nn, n, k, xmin, w, wtab, colptr, rows, vals, st);
     else
-        hipLaunchKernelGGL((k_snn_block<T, uint64_t>), dim3(grid), dim3(kBlock),
-                           0, st, nn, n, k, xmin, w.rptr.get(), w.rev.get(), w.heavy.get(),
-                           w.nheavy_h, w.cnt.get(), w.htab.get(), w.htab_off.get(), wtab, colptr,
-                           count ? nullptr : rows, vals,
-                           stage ? (uint64_t *)w.stage.get() : nullptr, w.staged.get());
-    SEV_LAUNCH_CHECK();
+        launch_block_s<T, uint64_t>(count, nn, n, k, xmin, w, wtab, colptr, rows, vals, st);
 }
 
 }  // namespace
@@ -598,32 +641,52 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
     SEV_HIP(hipStreamSynchronize(st));
     w.nheavy_h = nh;
     if (nh > 0) {
-        // columns above the LDS capacity get a global table of
-        // 2 * next_pow2(P) slots plus a sort buffer of half that
+        // the heavy columns ordered by class (small LDS table, big LDS table,
+        // global table); the global ones get a table of 2 * next_pow2(P)
+        // slots plus a sort buffer of half that
         std::vector<int64_t> hp(nh);
+        std::vector<int32_t> hj(nh);
         SEV_HIP(hipMemcpy(hp.data(), w.heavyP.get(), sizeof(int64_t) * nh, hipMemcpyDeviceToHost));
-        const int64_t lds_maxP = block_lds_maxP(n);
-        std::vector<int64_t> offs(nh + 1, 0);
-        int64_t big = 0, mx = 0;
-        for (int32_t h = 0; h < nh; ++h) {
-            int64_t slots = 0;
-            if (hp[h] > lds_maxP) {
-                slots = 1;
-                while (slots < 2 * hp[h]) slots <<= 1;
-                slots += slots / 2;
-                big++;
+        SEV_HIP(hipMemcpy(hj.data(), w.heavy.get(), sizeof(int32_t) * nh, hipMemcpyDeviceToHost));
+        const int64_t lim[2] = {small_maxP(n), big_maxP(n)};
+        std::vector<int32_t> order;
+        order.reserve((size_t)nh);
+        std::vector<int64_t> gp;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t before = (int64_t)order.size();
+            for (int32_t h = 0; h < nh; ++h) {
+                const int cls = hp[h] <= lim[0] ? 0 : (hp[h] <= lim[1] ? 1 : 2);
+                if (cls != c) continue;
+                order.push_back(hj[h]);
+                if (c == 2) gp.push_back(hp[h]);
             }
-            mx = std::max(mx, hp[h]);
-            offs[h + 1] = offs[h] + slots;
+            w.nheavy_cls[c] = (int64_t)order.size() - before;
+        }
+        // columns are independent, so the order within a class is free: kept
+        // as found for determinism of the layout (every result is sorted)
+        SEV_HIP(hipMemcpy(w.heavy.get(), order.data(), sizeof(int32_t) * nh,
+                          hipMemcpyHostToDevice));
+        const int64_t ng = w.nheavy_cls[2];
+        std::vector<int64_t> offs(ng + 1, 0);
+        int64_t mx = 0;
+        for (int64_t h = 0; h < ng; ++h) {
+            int64_t slots = 1;
+            while (slots < 2 * gp[(size_t)h]) slots <<= 1;
+            slots += slots / 2;
+            mx = std::max(mx, gp[(size_t)h]);
+            offs[(size_t)h + 1] = offs[(size_t)h] + slots;
         }
         if (std::getenv("SEV_VERBOSE"))
-            std::fprintf(stderr, "[severo_hip] snn n=%lld k=%lld: %d block columns (max P %lld, "
-                         "%lld with a global table)\n", (long long)n, (long long)k, nh,
-                         (long long)mx, (long long)big);
-        w.htab_off.ensure((size_t)nh + 1);
-        SEV_HIP(hipMemcpy(w.htab_off.get(), offs.data(), sizeof(int64_t) * (nh + 1),
+            std::fprintf(stderr, "[severo_hip] snn n=%lld k=%lld: %d block columns (%lld with "
+                         "P <= %lld, %lld with P <= %lld, %lld with a global table, max P "
+                         "%lld)\n", (long long)n, (long long)k, nh,
+                         (long long)w.nheavy_cls[0], (long long)lim[0],
+                         (long long)w.nheavy_cls[1], (long long)lim[1], (long long)ng,
+                         (long long)mx);
+        w.htab_off.ensure((size_t)ng + 1);
+        SEV_HIP(hipMemcpy(w.htab_off.get(), offs.data(), sizeof(int64_t) * (ng + 1),
                           hipMemcpyHostToDevice));
-        w.htab.ensure((size_t)(offs[nh] > 0 ? offs[nh] : 1));
+        w.htab.ensure((size_t)(offs[(size_t)ng] > 0 ? offs[(size_t)ng] : 1));
         launch_block<T>(true, nn, n, (int)k, xmin, w, nullptr, nullptr, nullptr, nullptr, st);
     }
     scan_exclusive_i32(w.cnt.get(), colptr, n, w.scratch.get(), st);
