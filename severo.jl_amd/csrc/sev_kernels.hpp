@@ -392,10 +392,10 @@ void knn_exact_rows_batch(const float *X, int64_t n, int64_t d, int64_t ld, int6
 // recomputed exhaustively.  Z is n x d column-major (ld).  Outputs row-major
 // n x k, 0-based.  *n_fallback = rows recomputed exhaustively.
 struct KnnWork {
-    DevBuf<unsigned char> zq, zc;
+    DevBuf<unsigned char> zq, zc, zcp, stmp;
     DevBuf<float> zr, nhat, eps, cthr, pval, pthr, scf, hn;
-    DevBuf<unsigned> gmax, bmax;
-    DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt;
+    DevBuf<unsigned> gmax, bmax, skey, skey2;
+    DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt, perm, sval;
     DevBuf<int64_t> fail, fail2, qpad;
 };
 void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,

@@ -30,6 +30,8 @@
 // exact.
 #include "sev_kernels.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <climits>
 #include <type_traits>
 #include <cstdlib>
@@ -197,13 +199,31 @@ struct Elem<2> {
 // tile), or, with seg, of the rows seg[s] .. seg[s+1]-1 of the sample s of
 // the block's first query (batches: every query of a block is from s).
 // Outputs at [chunk][list index]: cidx (KL ids), cthr, ccnt, and the list
-// values in cval when it is given.
+// values in cval when it is given.  cthr bounds every candidate left out of
+// the list: dot' <= cthr (-inf: every candidate is in the list).
+//
+// Candidate order.  With perm, row p of Zc is the point perm[p] (the points
+// by ascending |c|^2, so the candidates with the largest dot' for a typical
+// query come first) and the lists hold perm[p]; positions past the chunk
+// carry id -1.  The lists are the KL best under (dot' desc, id asc) in any
+// order; the order only changes how many insertions it takes to get there.
+//
+// Seed.  With nseed > 0 every query first sweeps the head [0, nseed) of the
+// candidates without lists, keeping the largest dot' of each of the 64
+// residues of the tile position, and starts its threshold at the 32nd
+// largest of those 64 maxima: a value reached by 32 real candidates, so
+// admitting only candidates above it loses none of the final list whenever
+// the list fills, and the list's threshold stays a bound for the candidates
+// left out when it does not.
+constexpr int kSeedRank = 32;
 template <int MODE, int KP, int QW, int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const int64_t *qidx,
            int64_t nq, int32_t *cidx, float *cthr, int32_t *ccnt, float *cval,
-           const int64_t *seg, const int32_t *row_sample, const float *scf, const float *hn) {
+           const int64_t *seg, const int32_t *row_sample, const float *scf, const float *hn,
+           const int32_t *perm, int64_t nseed) {
     typedef typename Elem<MODE>::T E;
+    static_assert(KL == 32 || KL == 64, "lists of 32 or 64");
     constexpr int QB = kNW * QW * 16;
     // LDS row stride (elements): odd for FP32, 16 bytes of padding otherwise
     constexpr int ROW = MODE == 0 ? KP + 1 : (MODE == 1 ? KP + 8 : KP + 16);
@@ -216,6 +236,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     __shared__ int32_t listI[QB][KL];
     __shared__ int listN[QB];
     __shared__ int32_t qg[QB];  // query point of each list row (-1: none)
+    __shared__ float seedv[QB];  // starting threshold of each list row
+    __shared__ int32_t tileid[2][kCT];  // with perm: the point of each tile row
 
     const E *Zq = (const E *)Zq_;
     const E *Zc = (const E *)Zc_;
@@ -230,6 +252,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     }
     for (int i = threadIdx.x; i < QB; i += kBlock) {
         listN[i] = 0;
+        seedv[i] = ninf();
         const int64_t qi = q0 + i;
         qg[i] = qi < nq ? (int32_t)(qidx ? qidx[qi] : qi) : -1;
     }
@@ -294,26 +317,32 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     static_assert(KP % EPL == 0, "row length must be a multiple of 16 bytes");
     uint4 stage[NLD];
     float fstage = 1.0f, hstage = 0.0f;
-    auto fetch = [&](int64_t t0) {
+    int32_t idstage = -1;
+    auto fetch = [&](int64_t t0, int64_t lim) {
 #pragma unroll
         for (int u = 0; u < NLD; ++u) {
             const int e = threadIdx.x + u * kBlock;
             const int r = e / CH, part = e % CH;
             const int64_t c = t0 + r;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (e < kCT * CH && c < cend)
+            if (e < kCT * CH && c < lim)
                 v = *(const uint4 *)((const E *)Zc + c * KP + part * EPL);
             stage[u] = v;
+        }
+        if (perm && threadIdx.x < kCT) {
+            const int64_t c = t0 + threadIdx.x;
+            idstage = c < lim ? perm[c] : -1;
         }
         if constexpr (MODE == 2) {
             const int64_t c = t0 + threadIdx.x;
             if (threadIdx.x < kCT) {
-                fstage = c < cend ? scf[c] : 1.0f;
-                hstage = c < cend ? hn[c] : 0.0f;
+                fstage = c < lim ? scf[c] : 1.0f;
+                hstage = c < lim ? hn[c] : 0.0f;
             }
         }
     };
     auto deposit = [&](E *tile, int b) {
+        if (perm && threadIdx.x < kCT) tileid[b][threadIdx.x] = idstage;
         if constexpr (MODE == 2) {
             if (threadIdx.x < kCT) {
                 tilef[b][threadIdx.x] = fstage;
@@ -336,25 +365,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             }
         }
     };
-    fetch(cbeg);
-    deposit(tiles[0], 0);
-    // every global load issued so far (query operands, first tile) is
-    // complete here, so the loop only waits for its own prefetches
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    int buf = 0;
-    for (int64_t t0 = cbeg; t0 < cend; t0 += kCT, buf ^= 1) {
-        const E *tile = tiles[buf];
-        const bool more = t0 + kCT < cend;
-        if (more) fetch(t0 + kCT);
-        // the tile holds padding columns, or (self pairs excluded) may hold
-        // this block's own queries: such pairs are rejected in the insertion
-        // path (their products may pass the threshold test)
-        const bool edge = t0 + kCT > cend ||
-                          (!include_self && (qidx || (t0 < q0 + QB && q0 < t0 + kCT)));
-        const int t32 = (int)t0, n32 = (int)cend;
-        // all products of the tile: acc[cb][rb] = 16 queries x 16 candidates
-        v4f acc[NCB][QW];
+    // all products of a tile: acc[cb][rb] = 16 queries x 16 candidates
+    auto products = [&](const E *tile, int b, v4f (&acc)[NCB][QW]) {
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
@@ -387,13 +399,112 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 if constexpr (MODE == 2) {
                     // score = sigma_q sigma_c (q^'.c^') - |c^|^2/2 (the scales are
                     // powers of two: their product is exact)
-                    const float fc = tilef[buf][cb * 16 + col], hc = tileh[buf][cb * 16 + col];
+                    const float fc = tilef[b][cb * 16 + col], hc = tileh[b][cb * 16 + col];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) a[i] = fmaf(a[i], sq[rb][i] * fc, hc);
                 }
                 acc[cb][rb] = a;
             }
         }
+    };
+
+    // seed: the head of the candidates without lists (see above)
+    if (nseed > 0) {
+        float mx[NCB][QW][4];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) mx[cb][rb][i] = ninf();
+        fetch(0, nseed);
+        deposit(tiles[0], 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        int sb = 0;
+        for (int64_t t0 = 0; t0 < nseed; t0 += kCT, sb ^= 1) {
+            const bool more = t0 + kCT < nseed;
+            if (more) fetch(t0 + kCT, nseed);
+            v4f acc[NCB][QW];
+            products(tiles[sb], sb, acc);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        mx[cb][rb][i] = fmaxf(mx[cb][rb][i], acc[cb][rb][i]);
+            if (more) deposit(tiles[sb ^ 1], sb ^ 1);
+            __syncthreads();
+        }
+        // the 64 maxima of each row into its (still empty) list storage:
+        // residue cb*16 + col at slot cb*16 + col of listV, continuing in
+        // listI for lists of 32
+#pragma unroll
+        for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = w * QW * 16 + rb * 16 + 4 * grp + i;
+#pragma unroll
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const int slot = cb * 16 + col;
+                    if (slot < KL)
+                        listV[row][slot] = mx[cb][rb][i];
+                    else
+                        listI[row][slot - KL] = __float_as_int(mx[cb][rb][i]);
+                }
+            }
+        __syncthreads();
+        // each wavefront: the kSeedRank-th largest of its rows' maxima
+        for (int r = 0; r < QW * 16; ++r) {
+            const int row = w * QW * 16 + r;
+            float v = lane < KL ? listV[row][lane] : __int_as_float(listI[row][lane - KL]);
+            for (int size = 2; size <= kWave; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    const float ov = __shfl_xor(v, stride, kWave);
+                    const bool up = (lane & size) == 0;
+                    const bool lower = (lane & stride) == 0;
+                    // descending: the lower position of an ascending pair
+                    // keeps the larger value
+                    if ((lower == up) ? ov > v : v > ov) v = ov;
+                }
+            }
+            const float sd = __shfl(v, kSeedRank - 1, kWave);
+            if (lane == 0) seedv[row] = qg[row] >= 0 ? sd : ninf();
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < QB * KL; e += kBlock) {
+            (&listV[0][0])[e] = ninf();
+            (&listI[0][0])[e] = INT_MAX;
+        }
+#pragma unroll
+        for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = w * QW * 16 + rb * 16 + 4 * grp + i;
+                if (qg[row] >= 0) thr[rb][i] = seedv[row];
+            }
+        __syncthreads();
+    }
+    fetch(cbeg, cend);
+    deposit(tiles[0], 0);
+    // every global load issued so far (query operands, first tile) is
+    // complete here, so the loop only waits for its own prefetches
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    int buf = 0;
+    for (int64_t t0 = cbeg; t0 < cend; t0 += kCT, buf ^= 1) {
+        const E *tile = tiles[buf];
+        const bool more = t0 + kCT < cend;
+        if (more) fetch(t0 + kCT, cend);
+        // the tile holds padding columns, or (self pairs excluded) may hold
+        // this block's own queries: such pairs are rejected in the insertion
+        // path (their products may pass the threshold test)
+        const bool edge = t0 + kCT > cend ||
+                          (!include_self && (qidx || (t0 < q0 + QB && q0 < t0 + kCT)));
+        const int t32 = (int)t0, n32 = (int)cend;
+        v4f acc[NCB][QW];
+        products(tile, buf, acc);
 #pragma unroll
         for (int rb = 0; rb < QW; ++rb) {
             // one test per output row for the whole tile: the largest of its
@@ -411,6 +522,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             if (!any) continue;
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
+                // the candidate id of this lane's column
+                const int32_t cid = perm ? tileid[buf][cb * 16 + col] : t32 + cb * 16 + col;
                 // products as integers for readlane (on vector elements it was
                 // seen to read element 0 for every element)
                 int ai[4];
@@ -430,8 +543,12 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                                                                      16 * g));
                         if (!(v > ts)) continue;
                         const int qlo = w * QW * 16 + rb * 16 + 4 * g + i;
-                        const int32_t cand = t32 + cb * 16 + (src & 15);
-                        if (edge && (cand >= n32 || (!include_self && cand == qg[qlo]))) continue;
+                        const int32_t cand = __builtin_amdgcn_readlane(cid, src);
+                        if (perm) {
+                            if (cand < 0 || (!include_self && cand == qg[qlo])) continue;
+                        } else if (edge && (cand >= n32 || (!include_self && cand == qg[qlo]))) {
+                            continue;
+                        }
                         // insert (v, cand) into the descending list of qlo:
                         // entries from pos on move one slot up by LDS stores
                         const int cnt = listN[qlo];
@@ -452,7 +569,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                             pos == KL - 1 ? v
                                           : __int_as_float(__builtin_amdgcn_readlane(
                                                 __float_as_int(lv), KL - 2));
-                        const float nthr = ncnt == KL ? last : ninf();
+                        // a list that is not full keeps its starting threshold
+                        const float nthr = ncnt == KL ? last : ts;
                         if (lane == 0) listN[qlo] = ncnt;
                         if (grp == g) thr[rb][i] = nthr;
                     }
@@ -472,7 +590,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             if (cval) cval[o * KL + lane] = listV[ql][lane];
         }
         if (lane == 0) {
-            cthr[o] = cnt == KL ? listV[ql][KL - 1] : ninf();
+            cthr[o] = cnt == KL ? listV[ql][KL - 1] : seedv[ql];
             ccnt[o] = cnt;
         }
     }
@@ -528,12 +646,14 @@ __device__ __forceinline__ void wave_rank_sort2(float (&v)[2], int32_t (&id)[2])
 // (lanes KL..2KL-1) are sorted together by (value desc, id asc) and the first
 // KL kept.  The merged threshold is the KL-th value when KL candidates were
 // kept over all chunks.  Every candidate outside the merged list lies below
-// its chunk's threshold, which is at most the merged one.  For KL = 64 the
+// its chunk's threshold, which is at most the merged one when KL were kept;
+// otherwise the merged threshold is the largest chunk threshold (a chunk's
+// list that is not full has its starting threshold).  For KL = 64 the
 // running list is one register of the lanes and the chunk's list another.
 template <int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, const int32_t *pcnt,
-            int32_t *cidx, float *cthr, int32_t *ccnt) {
+            const float *pthr, int32_t *cidx, float *cthr, int32_t *ccnt) {
     static_assert(2 * KL <= kWave || KL == kWave, "two lists must fit a wavefront");
     const int lane = lane_id();
     if constexpr (KL == kWave) {
@@ -542,10 +662,12 @@ k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, cons
         float v[2] = {ninf(), ninf()};
         int32_t id[2] = {INT_MAX, INT_MAX};
         int tot = 0;
+        float floor = ninf();
         for (int c = 0; c < nchunk; ++c) {
             const int64_t o = (int64_t)c * nq + qi;
             const int cnt = pcnt[o];
             tot += cnt;
+            floor = fmaxf(floor, pthr[o]);
             v[1] = lane < cnt ? pval[o * KL + lane] : ninf();
             id[1] = lane < cnt ? pidx[o * KL + lane] : INT_MAX;
             wave_rank_sort2(v, id);
@@ -555,7 +677,7 @@ k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, cons
         const float last = __shfl(v[0], KL - 1, kWave);
         if (lane == 0) {
             ccnt[qi] = cnt;
-            cthr[qi] = tot >= KL ? last : ninf();
+            cthr[qi] = tot >= KL ? fmaxf(last, floor) : floor;
         }
         return;
     }
@@ -564,10 +686,12 @@ k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, cons
     float v = ninf();
     int32_t id = INT_MAX;
     int tot = 0;
+    float floor = ninf();
     for (int c = 0; c < nchunk; ++c) {
         const int64_t o = (int64_t)c * nq + qi;
         const int cnt = pcnt[o];
         tot += cnt;
+        floor = fmaxf(floor, pthr[o]);
         if (lane >= KL && lane < 2 * KL) {
             const int l = lane - KL;
             v = l < cnt ? pval[o * KL + l] : ninf();
@@ -597,7 +721,7 @@ k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, cons
     const float last = __shfl(v, KL - 1, kWave);
     if (lane == 0) {
         ccnt[qi] = cnt;
-        cthr[qi] = tot >= KL ? last : ninf();
+        cthr[qi] = tot >= KL ? fmaxf(last, floor) : floor;
     }
 }
 
@@ -645,14 +769,15 @@ k_knn_rerank(const float *Zr, int64_t nq, const int64_t *qidx, int d, int k, con
         }
     }
     const float Dk = __shfl(D, k - 1, kWave);
+    const float th = cthr[qi];
     bool ok;
-    if (cnt < KL) {
+    if (th == ninf()) {
         ok = cnt >= k;  // every admissible candidate was seen and re-ranked
     } else {
         const double ncmax = (double)__uint_as_float(gmax[0]);
         const double emax = (double)__uint_as_float(gmax[1]);
         const double nq_ = (double)nhat[q];
-        const double dt = nq_ - 2.0 * (double)cthr[qi];
+        const double dt = nq_ - 2.0 * (double)th;
         const double lb2 = dt * (1.0 - 0x1p-22) - eta * (nq_ + ncmax) - 1e-30;
         const double lb = lb2 > 0.0 ? sqrt(lb2) - (double)eps[q] - emax : -1.0;
         ok = lb > 0.0 && lb * lb * spec_factor > (double)Dk;
@@ -742,13 +867,71 @@ double mfma_selftest(int mode) {
 
 namespace {
 
+__global__ void __launch_bounds__(kBlock)
+k_norm_keys(const float *nhat, int64_t n, unsigned *key, int32_t *id) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        key[i] = __float_as_uint(nhat[i]);  // positive: the bits order as the values
+        id[i] = (int32_t)i;
+    }
+}
+
+// dst row p = src row perm[p]; rows of ch 16-byte chunks
+__global__ void __launch_bounds__(kBlock)
+k_gather_rows(const uint4 *src, const int32_t *perm, int64_t n, int ch, uint4 *dst) {
+    const int64_t tot = n * ch;
+    for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < tot;
+         e += (int64_t)gridDim.x * kBlock) {
+        const int64_t p = e / ch;
+        const int part = (int)(e - p * ch);
+        dst[e] = src[(int64_t)perm[p] * ch + part];
+    }
+}
+
+// Candidate order for the search: w.perm = the points by ascending |c|^2
+// (the stable radix sort keeps ties in index order) and w.zc's rows put in
+// that order.
+void norm_order(int64_t n, size_t row_bytes, KnnWork &w, hipStream_t st) {
+    SEV_REQUIRE(row_bytes % 16 == 0, "knn: operand rows must be 16-byte multiples");
+    ProfScope ps("knn_order", st, 2.0 * n * row_bytes + 24.0 * n);
+    w.skey.ensure((size_t)n);
+    w.skey2.ensure((size_t)n);
+    w.sval.ensure((size_t)n);
+    w.perm.ensure((size_t)n);
+    const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
+    hipLaunchKernelGGL(k_norm_keys, dim3(g), dim3(kBlock), 0, st, w.nhat.get(), n, w.skey.get(),
+                       w.sval.get());
+    SEV_LAUNCH_CHECK();
+    size_t tb = 0;
+    SEV_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, w.skey.get(), w.skey2.get(),
+                                               w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
+    w.stmp.ensure(tb);
+    SEV_HIP(hipcub::DeviceRadixSort::SortPairs(w.stmp.get(), tb, w.skey.get(), w.skey2.get(),
+                                               w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
+    const int ch = (int)(row_bytes / 16);
+    w.zcp.ensure((size_t)n * row_bytes);
+    const unsigned g2 =
+        (unsigned)std::min<int64_t>(ceil_div(n * ch, kBlock), device_cu_count() * 16);
+    hipLaunchKernelGGL(k_gather_rows, dim3(g2), dim3(kBlock), 0, st,
+                       (const uint4 *)w.zc.get(), w.perm.get(), n, ch, (uint4 *)w.zcp.get());
+    SEV_LAUNCH_CHECK();
+    w.zc.swap(w.zcp);
+}
+
+// Head of the ordered candidates swept for the seed thresholds: 1/64 of the
+// points in whole tiles (none below one tile).
+int64_t seed_head(int64_t n) { return (n / 64) / kCT * kCT; }
+
 // Candidate search + merge (chunked) + re-rank for a query list.  Rows that
-// fail the certificate are appended to w.fail (count in w.nfail).
+// fail the certificate are appended to w.fail (count in w.nfail).  order:
+// 0 = candidates in index order; 1 = candidates by ascending norm (made
+// here, prep required) with seed thresholds; 2 = the norm order the last
+// order-1 call made on the same operands, without seeds.
 template <int MODE, int KP, int QW, int KL>
 void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                  const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
                  int32_t *nfail, int64_t *fail, hipStream_t st, const int64_t *seg = nullptr,
-                 const int32_t *row_sample = nullptr, bool prep = true) {
+                 const int32_t *row_sample = nullptr, bool prep = true, int order = 0) {
     const size_t es = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
     w.zq.ensure((size_t)n * KP * es);
     if (MODE != 2) w.zc.ensure((size_t)n * KP * es);  // FP8: one operand array for both
@@ -777,6 +960,17 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
                            (const unsigned *)w.bmax.get(), w.scf.get(), w.hn.get());
         SEV_LAUNCH_CHECK();
     }
+    const int32_t *perm = nullptr;
+    int64_t nseed = 0;
+    if (order != 0) {
+        SEV_REQUIRE(MODE != 2 && seg == nullptr, "knn: norm order needs separate operands");
+        SEV_REQUIRE(order == 2 || prep, "knn: norm order is made with the operands");
+        if (order == 1) {
+            norm_order(n, (size_t)KP * es, w, st);
+            nseed = seed_head(n);
+        }
+        perm = w.perm.get();
+    }
     const void *zc_op = MODE == 2 ? (const void *)w.zq.get() : (const void *)w.zc.get();
     constexpr int QB = kNW * QW * 16;
     const int64_t qblocks = ceil_div(nq, QB);
@@ -800,7 +994,7 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
                                dim3(kBlock), 0, st, (const void *)w.zq.get(), zc_op, n,
                                include_self, qidx, nq, w.cidx.get(),
                                w.cthr.get(), w.ccnt.get(), (float *)nullptr, seg, row_sample,
-                               w.scf.get(), w.hn.get());
+                               w.scf.get(), w.hn.get(), perm, nseed);
             SEV_LAUNCH_CHECK();
         } else if constexpr (KL <= kWave) {
             w.pidx.ensure((size_t)nchunk * nq * KL);
@@ -811,11 +1005,12 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
                                dim3(kBlock), 0, st, (const void *)w.zq.get(), zc_op, n,
                                include_self, qidx, nq, w.pidx.get(),
                                w.pthr.get(), w.pcnt.get(), w.pval.get(), seg, row_sample,
-                               w.scf.get(), w.hn.get());
+                               w.scf.get(), w.hn.get(), perm, nseed);
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL((k_knn_merge<KL>), dim3((unsigned)ceil_div(nq * kWave, kBlock)),
                                dim3(kBlock), 0, st, nq, nchunk, w.pidx.get(), w.pval.get(),
-                               w.pcnt.get(), w.cidx.get(), w.cthr.get(), w.ccnt.get());
+                               w.pcnt.get(), w.pthr.get(), w.cidx.get(), w.cthr.get(),
+                               w.ccnt.get());
             SEV_LAUNCH_CHECK();
         }
     }
@@ -841,7 +1036,7 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     w.fail.ensure((size_t)n);
     SEV_HIP(hipMemsetAsync(w.nfail.get(), 0, 3 * sizeof(int32_t), st));
     cand_rerank<MODE, KP, QW, KL>(Z, n, d, ld, k, include_self, nullptr, n, w, idx, dist,
-                                  w.nfail.get(), w.fail.get(), st);
+                                  w.nfail.get(), w.fail.get(), st, nullptr, nullptr, true, 1);
     int32_t nf = 0;
     SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
@@ -856,7 +1051,7 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
                                    hipMemcpyDeviceToDevice, st));
             cand_rerank<1, KP, 1, 64>(Z, n, d, ld, k, include_self, w.fail2.get(), nf, w, idx,
                                       dist, w.nfail.get() + 2, w.fail.get(), st, nullptr,
-                                      nullptr, false);
+                                      nullptr, false, 2);
             SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get() + 2, sizeof nf, hipMemcpyDeviceToHost, st));
             SEV_HIP(hipStreamSynchronize(st));
         }

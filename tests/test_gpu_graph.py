@@ -171,6 +171,25 @@ def test_ann_mfma_duplicates(sev, orc):
         assert np.array_equal(idx, ridx) and np.array_equal(dist, rdist)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("include_self", [True, False])
+def test_ann_norm_order_ties(sev, orc, precision, include_self):
+    """Candidates searched by ascending norm with seeded thresholds (enough
+    points for a seeded head): exact duplicates, copies with the coordinates
+    reversed (equal norms, other points) and a block of zero vectors at the
+    head of the order.  The lists depend on the order only through ties, and
+    the certified result must not."""
+    rng = np.random.default_rng(31)
+    base = rng.standard_normal((3000, 16)).astype(np.float32)
+    X = np.concatenate([base, base[:, ::-1], base[:2000], np.zeros((50, 16), np.float32)])
+    idx, dist, nfb = sev.ann(X, 20, include_self=include_self, precision=precision,
+                             return_fallback=True)
+    ridx, rdist = orc.knn(X, 20, include_self=include_self)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(dist, rdist)
+    print(f"{precision} include_self={include_self}: {nfb} fallback rows")
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8"])
 def test_pipeline_neighbours_forms(sev, orc, precision):
     t = sev.nb_tables(3000, seed=1, density=0.05)
