@@ -117,6 +117,30 @@ int sev_centered_mul(int64_t m, int64_t n, const int64_t *colptr, const int64_t 
                      const void *nzval, const void *mu, int dtype, char trans, const double *x,
                      double *y, double alpha, double beta);
 
+/* mul!(C, S, X, α, β) and mul!(C, S', X, α, β) of a CenteredMatrix S with a
+ * dense matrix X (src/scaling.jl:259-271).  S = A - 1 mu' with A as in
+ * sev_centered_mul.  trans 'N': Y (m x p) = alpha S X + beta Y, X n x p;
+ * 'T': Y (n x p) = alpha S' X + beta Y, X m x p.  X and Y are column-major
+ * FP64 with leading dimensions ldx, ldy; beta == 0 overwrites Y.  The 'T'
+ * form subtracts mu (1' X) like the vector form (src/scaling.jl:252-257);
+ * the reference's matrix form adds it (src/scaling.jl:268-269). */
+int sev_centered_mul_mat(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                         const void *nzval, const void *mu, int dtype, char trans, int64_t p,
+                         const double *X, int64_t ldx, double *Y, int64_t ldy, double alpha,
+                         double beta);
+
+/* mul!(C, Sl', Sr, α, β) of two CenteredMatrix with m rows
+ * (src/scaling.jl:273-294): C (nl x nr, column-major FP64, leading
+ * dimension ldc) = alpha Sl' Sr + beta C, computed as the reference does:
+ * Al'Ar - mul (1'Ar) - (1'Al - m mul')' mur'.  colptr_r == NULL takes
+ * Sr = Sl (the Gram matrix S'S; nr and the other _r arguments are then
+ * ignored).  Both matrices have the same dtype. */
+int sev_centered_gram(int64_t m, int64_t nl, const int64_t *colptr_l, const int64_t *rowval_l,
+                      const void *nzval_l, const void *mu_l, int64_t nr,
+                      const int64_t *colptr_r, const int64_t *rowval_r, const void *nzval_r,
+                      const void *mu_r, int dtype, double *C, int64_t ldc, double alpha,
+                      double beta);
+
 /* IRLBA on the centered matrix held on the device (replaces the libcell
  * irlba call of src/irlba.jl:66-71 for the CenteredMatrix operator of
  * src/scaling.jl:219-257, without per-product callbacks).  B (dtype values,

@@ -46,6 +46,9 @@ def lib():
         L.so_centered_mul.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, C.c_int, _p, _p,
                                       _dbl, _dbl]
         L.so_centered_tmul_mat.argtypes = [_i64, _i64, _p, _p, _p, _p, _p, _i64, _p, C.c_int]
+        L.so_centered_gram.restype = C.c_int
+        L.so_centered_gram.argtypes = [_i64, _i64, _p, _p, _p, _p, _i64, _p, _p, _p, _p, C.c_int,
+                                       _p, _dbl, _dbl]
         L.so_svd_jacobi.argtypes = [_i64, _p, _p, _p, _p]
         L.so_irlba_csc.restype = C.c_int
         L.so_irlba_csc.argtypes = [_i64, _i64, _p, _p, _p, C.c_int, _p, _i64, _i64, _i64, _dbl,
@@ -161,6 +164,44 @@ def centered_mul(m, colptr, rowval, vals, mu, x, trans=False, y=None, alpha=1.0,
     lib().so_centered_mul(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), f64, _ptr(mu),
                           int(trans), _ptr(x), _ptr(y), alpha, beta)
     return y
+
+
+def centered_mul_mat(m, colptr, rowval, vals, mu, X, trans=False, Y=None, alpha=1.0, beta=0.0):
+    """mul!(Y, S, X, alpha, beta) / mul!(Y, S', X, alpha, beta) with a matrix
+    X (src/scaling.jl:259-271): the vector form column by column (the
+    reference's matrix form runs the same SparseArrays loop per column of X
+    and subtracts the same centering term for S X; for S' X the vector
+    form's -mu (1'x) is used, the reference's matrix form adds it)."""
+    n = len(colptr) - 1
+    X = np.asarray(X, np.float64)
+    p = X.shape[1]
+    out = np.zeros(((n if trans else m), p)) if Y is None else np.array(Y, np.float64)
+    for c in range(p):
+        out[:, c] = centered_mul(m, colptr, rowval, vals, mu, X[:, c], trans=trans, y=out[:, c],
+                                 alpha=alpha, beta=beta)
+    return out
+
+
+def centered_gram(m, cpl, rvl, vl, mul, cpr=None, rvr=None, vr=None, mur=None, C=None,
+                  alpha=1.0, beta=0.0):
+    """mul!(C, Sl', Sr, alpha, beta) (src/scaling.jl:273-294); cpr None: Sr = Sl."""
+    if cpr is None:
+        cpr, rvr, vr, mur = cpl, rvl, vl, mul
+    vl = np.ascontiguousarray(vl)
+    f64 = int(vl.dtype == np.float64)
+    vr = np.ascontiguousarray(vr, vl.dtype)
+    mul = None if mul is None else np.ascontiguousarray(mul, vl.dtype)
+    mur = None if mur is None else np.ascontiguousarray(mur, vl.dtype)
+    nl, nr = len(cpl) - 1, len(cpr) - 1
+    out = np.zeros((nl, nr), order="F") if C is None else np.array(C, np.float64, order="F")
+    args = [np.ascontiguousarray(a, t) for a, t in ((cpl, np.int64), (rvl, np.int32),
+                                                   (cpr, np.int64), (rvr, np.int32))]
+    rc = lib().so_centered_gram(m, nl, _ptr(args[0]), _ptr(args[1]), _ptr(vl), _ptr(mul), nr,
+                                _ptr(args[2]), _ptr(args[3]), _ptr(vr), _ptr(mur), f64,
+                                _ptr(out), alpha, beta)
+    if rc != 0:
+        raise MemoryError("so_centered_gram")
+    return out
 
 
 def centered_tmul_mat(m, colptr, rowval, vals, mu, U, nthreads=None):

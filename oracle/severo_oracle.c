@@ -19,6 +19,8 @@
  *                       a pure-Python float32 Welford loop, numpy clip.
  *   so_centered_mul     known-answer vectors of test/test_scaling.jl:70-113
  *                       (dense closed form computed by numpy).
+ *   so_centered_gram    numpy (A - 1 mu')'(B - 1 nu') of the densified
+ *                       operands, alpha/beta, uncentered and self cases.
  *   so_irlba_*          numpy/LAPACK dense SVD, residual invariant
  *                       ||A'U - V S||_F / ||A||_F < tol (test/test_irlba.jl).
  *   so_knn_*            numpy float32 brute force with stable argsort,
@@ -286,6 +288,70 @@ SO_EXPORT void so_centered_mul(int64_t m, int64_t n, const int64_t *colptr,
     }
 #undef SO_VAL
 #undef SO_MU
+}
+
+/* C (nl x nr, column-major) = alpha (Al - 1 mul')'(Ar - 1 mur') + beta C,
+ * in the steps of mul!(C, Sl', Sr, α, β) (src/scaling.jl:273-294):
+ *   1. C = β C, then the sparse-sparse loop of src/mul.jl:82-113 on
+ *      copy(Al') and Ar: for each column c of Ar and each stored (j, b) of
+ *      it (rows ascending), each stored (r, a) of row j of Al:
+ *      C[r, c] += (α a) b;
+ *   2. q = colsums(Ar); C[r, c] += (-α) (mul[r] q[c]);
+ *   3. q' = colsums(Al) (q when Sl is Sr) - m mul; C[r, c] += (-α) (q'[r] mur[c]).
+ * Column sums are sequential over the stored entries.  mul or mur NULL: that
+ * matrix is uncentered.  The rows of Al are gathered by a counting sort. */
+SO_EXPORT int so_centered_gram(int64_t m, int64_t nl, const int64_t *cpl, const int32_t *rvl,
+                               const void *vl, const void *mul, int64_t nr, const int64_t *cpr,
+                               const int32_t *rvr, const void *vr, const void *mur, int f64,
+                               double *C, double alpha, double beta)
+{
+#define SO_V(v, p) (f64 ? ((const double *)(v))[p] : (double)((const float *)(v))[p])
+    const int64_t nnzl = cpl[nl];
+    int64_t *rp = (int64_t *)calloc((size_t)m + 1, sizeof(int64_t));
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * ((size_t)m + 1));
+    int64_t *rc = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nnzl > 0 ? nnzl : 1));
+    double *ra = (double *)malloc(sizeof(double) * (size_t)(nnzl > 0 ? nnzl : 1));
+    double *q = (double *)malloc(sizeof(double) * (size_t)(nr > 0 ? nr : 1));
+    double *ql = (double *)malloc(sizeof(double) * (size_t)(nl > 0 ? nl : 1));
+    if (!rp || !fill || !rc || !ra || !q || !ql) {
+        free(rp); free(fill); free(rc); free(ra); free(q); free(ql);
+        return 1;
+    }
+    for (int64_t e = 0; e < nnzl; ++e) rp[rvl[e] + 1]++;
+    for (int64_t i = 0; i < m; ++i) rp[i + 1] += rp[i];
+    memcpy(fill, rp, sizeof(int64_t) * ((size_t)m + 1));
+    for (int64_t c = 0; c < nl; ++c)
+        for (int64_t e = cpl[c]; e < cpl[c + 1]; ++e) {
+            const int64_t u = fill[rvl[e]]++;
+            rc[u] = c;
+            ra[u] = SO_V(vl, e);
+        }
+    for (int64_t e = 0; e < nl * nr; ++e) C[e] = beta == 0.0 ? 0.0 : beta * C[e];
+    for (int64_t c = 0; c < nr; ++c)
+        for (int64_t t = cpr[c]; t < cpr[c + 1]; ++t) {
+            const int64_t j = rvr[t];
+            const double b = SO_V(vr, t);
+            for (int64_t u = rp[j]; u < rp[j + 1]; ++u) C[c * nl + rc[u]] += (alpha * ra[u]) * b;
+        }
+    for (int64_t c = 0; c < nr; ++c) {
+        double a = 0.0;
+        for (int64_t t = cpr[c]; t < cpr[c + 1]; ++t) a += SO_V(vr, t);
+        q[c] = a;
+    }
+    for (int64_t r = 0; r < nl; ++r) {
+        double a = 0.0;
+        for (int64_t t = cpl[r]; t < cpl[r + 1]; ++t) a += SO_V(vl, t);
+        ql[r] = a - (mul ? (double)m * SO_V(mul, r) : 0.0);
+    }
+    if (mul)
+        for (int64_t c = 0; c < nr; ++c)
+            for (int64_t r = 0; r < nl; ++r) C[c * nl + r] += -alpha * (SO_V(mul, r) * q[c]);
+    if (mur)
+        for (int64_t c = 0; c < nr; ++c)
+            for (int64_t r = 0; r < nl; ++r) C[c * nl + r] += -alpha * (ql[r] * SO_V(mur, c));
+    free(rp); free(fill); free(rc); free(ra); free(q); free(ql);
+    return 0;
+#undef SO_V
 }
 
 /* ------------------------------------------------------------------------ */

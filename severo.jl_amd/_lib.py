@@ -39,6 +39,10 @@ SIGNATURES = {
                                            _p]),
     "sev_scale_features": (_int, [_i64, _i64, _p, _p, _p, _int, _dbl, _p, _p]),
     "sev_centered_mul": (_int, [_i64, _i64, _p, _p, _p, _p, _int, C.c_char, _p, _p, _dbl, _dbl]),
+    "sev_centered_mul_mat": (_int, [_i64, _i64, _p, _p, _p, _p, _int, C.c_char, _i64, _p, _i64, _p,
+                                    _i64, _dbl, _dbl]),
+    "sev_centered_gram": (_int, [_i64, _i64, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _int, _p, _i64,
+                                 _dbl, _dbl]),
     "sev_irlba_centered_csc": (_int, [_i64, _i64, _p, _p, _p, _p, _int, _i64, _i64, _i64, _dbl, _p,
                                       _p, _p, _p, _int, _pi64, _pi64]),
     "irlba": (_int, [_i64, _i64, _i64, _i64, _i64, _i64, _dbl, _p, _p, _p, _p, _p, _p, _p]),

@@ -136,16 +136,23 @@ class CenteredMatrix:
     def dtype(self):
         return np.promote_types(self.A.dtype, self.mu.dtype)
 
+    def _julia(self, dt=None):
+        A, colptr, rowval = _csc_julia(self.A)
+        dt = np.dtype(self.dtype if dt is None else dt)
+        return (A.shape, colptr, rowval, np.ascontiguousarray(A.data, dt),
+                np.ascontiguousarray(self.mu, dt), dt)
+
     def mul(self, y, x, alpha=1.0, beta=0.0, trans=False):
         """mul!(y, C, x, alpha, beta) (src/scaling.jl:245-250) or, with
-        ``trans``, mul!(y, C', x, alpha, beta) (:252-257).  FP64 vectors."""
-        A, colptr, rowval = _csc_julia(self.A)
-        dt = np.dtype(self.dtype)
-        vals = np.ascontiguousarray(A.data, dt)
-        mu = np.ascontiguousarray(self.mu, dt)
-        x = np.ascontiguousarray(x, np.float64)
+        ``trans``, mul!(y, C', x, alpha, beta) (:252-257).  FP64 vectors, or
+        FP64 matrices (the matrix forms of :259-271, with the adjoint form's
+        centering subtracted as in the vector form)."""
+        (m, n), colptr, rowval, vals, mu, dt = self._julia()
+        x = np.asarray(x, np.float64)
+        if x.ndim == 2:
+            return self._mul_mat(y, x, alpha, beta, trans)
+        x = np.ascontiguousarray(x)
         y = np.ascontiguousarray(y, np.float64)
-        m, n = A.shape
         if x.shape[0] != (m if trans else n) or y.shape[0] != (n if trans else m):
             raise ValueError("dimension mismatch")
         check(_lib.load().sev_centered_mul(m, n, _ptr(colptr), _ptr(rowval), _ptr(vals),
@@ -153,7 +160,49 @@ class CenteredMatrix:
                                            _ptr(x), _ptr(y), float(alpha), float(beta)))
         return y
 
+    def _mul_mat(self, Y, X, alpha, beta, trans):
+        (m, n), colptr, rowval, vals, mu, dt = self._julia()
+        X = np.asfortranarray(X, np.float64)
+        Y = np.asfortranarray(Y, np.float64)
+        p = X.shape[1]
+        if X.shape[0] != (m if trans else n) or Y.shape != ((n if trans else m), p):
+            raise ValueError("dimension mismatch")
+        check(_lib.load().sev_centered_mul_mat(
+            m, n, _ptr(colptr), _ptr(rowval), _ptr(vals), _ptr(mu), _dtype_code(dt),
+            b"T" if trans else b"N", p, _ptr(X), max(X.shape[0], 1), _ptr(Y), max(Y.shape[0], 1),
+            float(alpha), float(beta)))
+        return Y
+
+    def gram(self, other=None, C=None, alpha=1.0, beta=0.0):
+        """mul!(C, Sl', Sr, alpha, beta) (src/scaling.jl:273-294) with Sl =
+        self and Sr = other (self when None): the FP64 (nl x nr) matrix
+        Sl' Sr, never densifying either operand.  Both are read in the dtype
+        promoted over the two."""
+        Sr = self if other is None else other
+        if Sr.shape[0] != self.shape[0]:
+            raise ValueError("dimension mismatch")
+        dt = np.promote_types(self.dtype, Sr.dtype)
+        (m, nl), cpl, rvl, vl, mul, _ = self._julia(dt)
+        nr = Sr.shape[1]
+        if C is None:
+            C = np.zeros((nl, nr), order="F")
+        C = np.asfortranarray(C, np.float64)
+        if C.shape != (nl, nr):
+            raise ValueError("dimension mismatch")
+        if other is None:
+            right = (None, None, None, None)
+        else:
+            _, cpr, rvr, vr, mur, _ = Sr._julia(dt)
+            right = (cpr, rvr, vr, mur)
+        check(_lib.load().sev_centered_gram(
+            m, nl, _ptr(cpl), _ptr(rvl), _ptr(vl), _ptr(mul), nr, *(_ptr(a) for a in right),
+            _dtype_code(dt), _ptr(C), max(nl, 1), float(alpha), float(beta)))
+        return C
+
     def __matmul__(self, x):
+        x = np.asarray(x)
+        if x.ndim == 2:
+            return self.mul(np.zeros((self.shape[0], x.shape[1]), order="F"), x)
         return self.mul(np.zeros(self.shape[0]), x)
 
     def rmatvec(self, x):

@@ -1331,6 +1331,72 @@ int sev_centered_mul(int64_t m, int64_t n, const int64_t *colptr, const int64_t 
     });
 }
 
+// A CSC matrix of the caller (Julia layout) and its mu on the device.
+struct HeldCsc {
+    DevBuf<int64_t> cp;
+    DevBuf<int32_t> row;
+    DevBuf<unsigned char> vals;
+    DevBuf<double> mu;
+    DevCsc view;
+};
+
+void hold_csc(HeldCsc &h, int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+              const void *nzval, const void *mu, int dtype, hipStream_t st) {
+    SEV_REQUIRE(dtype == SEV_FLOAT32 || dtype == SEV_FLOAT64, "bad dtype");
+    int64_t nnz = 0;
+    csc_to_device(m, n, colptr, rowval, h.cp, h.row, nnz, st);
+    const size_t es = dtype == SEV_FLOAT32 ? 4 : 8;
+    SEV_REQUIRE(nnz == 0 || nzval != nullptr, "nzval is NULL");
+    h.vals.ensure((size_t)nnz * es);
+    if (nnz)
+        SEV_HIP(hipMemcpyAsync(h.vals.get(), nzval, (size_t)nnz * es, hipMemcpyHostToDevice, st));
+    if (mu) {
+        std::vector<double> m64((size_t)n);
+        for (int64_t j = 0; j < n; ++j)
+            m64[(size_t)j] = dtype == SEV_FLOAT32 ? (double)((const float *)mu)[j]
+                                                  : ((const double *)mu)[j];
+        upload(h.mu, m64.data(), (size_t)n, st);
+        SEV_HIP(hipStreamSynchronize(st));
+    }
+    h.view.m = m;
+    h.view.n = n;
+    h.view.nnz = nnz;
+    h.view.cp = h.cp.get();
+    h.view.row = h.row.get();
+    h.view.val = h.vals.get();
+    h.view.f64 = dtype == SEV_FLOAT64;
+    h.view.mu = mu ? h.mu.get() : nullptr;
+}
+
+int sev_centered_mul_mat(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
+                         const void *nzval, const void *mu, int dtype, char trans, int64_t p,
+                         const double *X, int64_t ldx, double *Y, int64_t ldy, double alpha,
+                         double beta) {
+    return guarded([&] {
+        ensure_device();
+        SEV_REQUIRE(trans == 'N' || trans == 'T', "trans must be 'N' or 'T'");
+        Stream s;
+        HeldCsc A;
+        hold_csc(A, m, n, colptr, rowval, nzval, mu, dtype, s.st);
+        centered_mul_mat(A.view, trans == 'T', p, X, ldx, Y, ldy, alpha, beta, s.st);
+    });
+}
+
+int sev_centered_gram(int64_t m, int64_t nl, const int64_t *colptr_l, const int64_t *rowval_l,
+                      const void *nzval_l, const void *mu_l, int64_t nr,
+                      const int64_t *colptr_r, const int64_t *rowval_r, const void *nzval_r,
+                      const void *mu_r, int dtype, double *C, int64_t ldc, double alpha,
+                      double beta) {
+    return guarded([&] {
+        ensure_device();
+        Stream s;
+        HeldCsc L, R;
+        hold_csc(L, m, nl, colptr_l, rowval_l, nzval_l, mu_l, dtype, s.st);
+        if (colptr_r) hold_csc(R, m, nr, colptr_r, rowval_r, nzval_r, mu_r, dtype, s.st);
+        centered_gram(L.view, colptr_r ? &R.view : nullptr, C, ldc, alpha, beta, s.st);
+    });
+}
+
 int sev_irlba_centered_csc(int64_t m, int64_t n, const int64_t *colptr, const int64_t *rowval,
                            const void *B, const void *mu, int dtype, int64_t nu, int64_t work,
                            int64_t maxit, double tol, const double *init, double *s, double *U,

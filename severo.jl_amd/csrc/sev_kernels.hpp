@@ -251,6 +251,25 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
 // heuristics).
 void autotune_operator(SlabCsc &op, hipStream_t st);
 
+// ---- sev_centered.hip --------------------------------------------------------
+// A (m x n) on the device as CSC: cp (n + 1, 0-based), row (int32), val
+// (double when f64, else float); mu (n doubles) centres it, nullptr: none.
+struct DevCsc {
+    int64_t m = 0, n = 0, nnz = 0;
+    const int64_t *cp = nullptr;
+    const int32_t *row = nullptr;
+    const void *val = nullptr;
+    bool f64 = false;
+    const double *mu = nullptr;
+};
+// Y = alpha S X + beta Y (Y m x p, X n x p) or, with trans, alpha S' X +
+// beta Y (Y n x p, X m x p), S = A - 1 mu'; X and Y host column-major FP64.
+void centered_mul_mat(const DevCsc &A, bool trans, int64_t p, const double *X, int64_t ldx,
+                      double *Y, int64_t ldy, double alpha, double beta, hipStream_t st);
+// C = alpha Sl' Sr + beta C (C nl x nr, host column-major); Ar null: Sr = Sl.
+void centered_gram(const DevCsc &Al, const DevCsc *Ar, double *C, int64_t ldc, double alpha,
+                   double beta, hipStream_t st);
+
 // y = A x (trans == 0, y length m) or y = A' x (trans == 1, y length n) on
 // the device, A = B - 1 mu'.  Uses w.part / w.part2 as scratch.
 struct IrlbaWork;

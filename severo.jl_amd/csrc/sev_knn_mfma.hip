@@ -925,8 +925,8 @@ int64_t seed_head(int64_t n) { return (n / 64) / kCT * kCT; }
 // Candidate search + merge (chunked) + re-rank for a query list.  Rows that
 // fail the certificate are appended to w.fail (count in w.nfail).  order:
 // 0 = candidates in index order; 1 = candidates by ascending norm (made
-// here, prep required) with seed thresholds; 2 = the norm order the last
-// order-1 call made on the same operands, without seeds.
+// here, prep required) with seed thresholds; 2 = the same with the norm
+// order the last order-1 call made on the same operands.
 template <int MODE, int KP, int QW, int KL>
 void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                  const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
@@ -965,11 +965,9 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     if (order != 0) {
         SEV_REQUIRE(MODE != 2 && seg == nullptr, "knn: norm order needs separate operands");
         SEV_REQUIRE(order == 2 || prep, "knn: norm order is made with the operands");
-        if (order == 1) {
-            norm_order(n, (size_t)KP * es, w, st);
-            nseed = seed_head(n);
-        }
+        if (order == 1) norm_order(n, (size_t)KP * es, w, st);
         perm = w.perm.get();
+        nseed = seed_head(n);
     }
     const void *zc_op = MODE == 2 ? (const void *)w.zq.get() : (const void *)w.zc.get();
     constexpr int QB = kNW * QW * 16;

@@ -107,6 +107,47 @@ function scale_features(X::NamedArray{T,2,SparseMatrixCSC{T,Int64}}; scale_max::
                           NamedArray(mu, (X.dicts[2],), (X.dimnames[2],)))
 end
 
+# ------------------------------------------------ CenteredMatrix products
+# The matrix forms of mul! (src/scaling.jl:259-294) on the GPU:
+# mul!(Y, S, X, α, β), mul!(Y, S', X, α, β) and mul!(C, Sl', Sr, α, β).
+# Named functions rather than mul! methods on Severo's types.
+function centered_mul!(Y::StridedMatrix{Float64}, S::Severo.CenteredMatrix,
+                       X::StridedMatrix{Float64}, α::Real=1.0, β::Real=0.0; trans::Bool=false)
+    A, mu = Severo._A_mu(S)
+    m, n = size(A)
+    T = eltype(nonzeros(A))
+    stride(X, 1) == 1 && stride(Y, 1) == 1 || throw(ArgumentError("column-major X and Y"))
+    size(X, 1) == (trans ? m : n) && size(Y) == ((trans ? n : m), size(X, 2)) ||
+        throw(DimensionMismatch())
+    check(ccall((:sev_centered_mul_mat, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Cchar, Int64,
+                 Ptr{Float64}, Int64, Ptr{Float64}, Int64, Cdouble, Cdouble),
+                m, n, A.colptr, A.rowval, nonzeros(A), convert(Vector{T}, mu), dtcode(T),
+                trans ? Cchar('T') : Cchar('N'), size(X, 2), X, max(stride(X, 2), 1), Y,
+                max(stride(Y, 2), 1), α, β))
+    Y
+end
+
+function centered_gram!(C::StridedMatrix{Float64}, Sl::Severo.CenteredMatrix,
+                        Sr::Severo.CenteredMatrix, α::Real=1.0, β::Real=0.0)
+    Al, mul = Severo._A_mu(Sl)
+    Ar, mur = Severo._A_mu(Sr)
+    size(Al, 1) == size(Ar, 1) && size(C) == (size(Al, 2), size(Ar, 2)) ||
+        throw(DimensionMismatch())
+    T = promote_type(eltype(nonzeros(Al)), eltype(nonzeros(Ar)))
+    same = Sl === Sr
+    check(ccall((:sev_centered_gram, libsev), Cint,
+                (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Int64, Ptr{Int64},
+                 Ptr{Int64}, Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Float64}, Int64, Cdouble, Cdouble),
+                size(Al, 1), size(Al, 2), Al.colptr, Al.rowval, convert(Vector{T}, nonzeros(Al)),
+                convert(Vector{T}, mul), size(Ar, 2), same ? C_NULL : pointer(Ar.colptr),
+                same ? C_NULL : pointer(Ar.rowval),
+                same ? C_NULL : pointer(convert(Vector{T}, nonzeros(Ar))),
+                same ? C_NULL : pointer(convert(Vector{T}, mur)), dtcode(T), C,
+                max(stride(C, 2), 1), α, β))
+    C
+end
+
 # -------------------------------------------------------------------- PCA
 # src/irlba.jl:47-85 and src/embedding.jl:46-94 (algorithm = :irlba)
 function irlba(C::Severo.CenteredMatrix, nu::Integer; tol=1e-5, maxit=1000, init=nothing,

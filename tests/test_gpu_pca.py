@@ -319,6 +319,90 @@ def test_centered_mul_layouts(sev, orc, m, n, dens):
 
 
 # ---------------------------------------------------------------------------
+# matrix and Gram forms of mul! on CenteredMatrix (src/scaling.jl:259-294)
+# ---------------------------------------------------------------------------
+def _close(got, ref, rtol=1e-12):
+    scale = max(np.abs(ref).max(), 1.0)
+    assert np.allclose(got, ref, rtol=rtol, atol=rtol * scale)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("m,n,p,dens", [(300, 40, 7, 0.2), (20000, 500, 70, 0.03),
+                                        (5000, 3, 1, 0.5)])
+def test_centered_mul_mat_vs_oracle(sev, orc, dt, m, n, p, dens):
+    """S X and S' X with dense matrices; p = 70 takes two lane passes, an
+    empty column and an empty row are planted."""
+    rng = np.random.default_rng(m + p)
+    A = sp.random(m, n, dens, format="lil", random_state=m, dtype=np.float64)
+    A[:, 0] = 0.0
+    A[1, :] = 0.0
+    A = sp.csc_matrix(A).astype(dt)
+    mu = rng.standard_normal(n).astype(dt)
+    Cm = sev.CenteredMatrix(A, mu)
+    X = rng.standard_normal((n, p))
+    W = rng.standard_normal((m, p))
+    Y0 = rng.standard_normal((m, p))
+    Z0 = rng.standard_normal((n, p))
+    _close(Cm @ X, orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, X))
+    _close(Cm.mul(Y0.copy(order="F"), X, 2.0, -0.5),
+           orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, X, Y=Y0, alpha=2.0, beta=-0.5))
+    _close(Cm.mul(np.zeros((n, p), order="F"), W, trans=True),
+           orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, W, trans=True))
+    _close(Cm.mul(Z0.copy(order="F"), W, 0.5, 3.0, trans=True),
+           orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, W, trans=True, Y=Z0,
+                                alpha=0.5, beta=3.0))
+
+
+def test_centered_mul_mat_leading_dimensions(sev, orc):
+    """The C ABI with ldx, ldy larger than the rows (submatrices of Julia
+    arrays): the padding rows are neither read nor written."""
+    import ctypes as C
+    rng = np.random.default_rng(3)
+    m, n, p = 400, 30, 5
+    A = sp.random(m, n, 0.2, format="csc", random_state=3, dtype=np.float64)
+    A.sort_indices()
+    mu = rng.standard_normal(n)
+    Xp = np.asfortranarray(rng.standard_normal((n + 9, p)))
+    Yp = np.asfortranarray(np.full((m + 5, p), 7.0))
+    cp = (A.indptr.astype(np.int64) + 1)
+    rv = (A.indices.astype(np.int64) + 1)
+    vals = np.ascontiguousarray(A.data)
+    sev._lib.check(sev.load().sev_centered_mul_mat(
+        m, n, cp.ctypes.data, rv.ctypes.data, vals.ctypes.data, mu.ctypes.data, 1, b"N", p,
+        Xp.ctypes.data, n + 9, Yp.ctypes.data, m + 5, C.c_double(1.0), C.c_double(0.0)))
+    _close(Yp[:m], orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, Xp[:n]))
+    assert np.all(Yp[m:] == 7.0)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("case", ["self", "pair", "wide", "uncentered"])
+def test_centered_gram_vs_oracle(sev, orc, dt, case):
+    """Sl' Sr for the Gram matrix of one operand, two operands with the same
+    rows, a right operand wider than one LDS pass (5000 columns) and an
+    uncentered right operand."""
+    rng = np.random.default_rng(len(case))
+    m = 30000
+    Al = sp.random(m, 200, 0.05, format="csc", random_state=1, dtype=np.float64).astype(dt)
+    mul = rng.standard_normal(200).astype(dt)
+    Sl = sev.CenteredMatrix(Al, mul)
+    if case == "self":
+        got = Sl.gram(alpha=0.25)
+        ref = orc.centered_gram(m, Al.indptr, Al.indices, Al.data, mul, alpha=0.25)
+    else:
+        nr = 5000 if case == "wide" else 120
+        Ar = sp.random(m, nr, 0.004 if case == "wide" else 0.1, format="csc", random_state=2,
+                       dtype=np.float64).astype(dt)
+        mur = rng.standard_normal(nr).astype(dt)
+        if case == "uncentered":
+            mur = np.zeros(nr, dt)
+        C0 = rng.standard_normal((200, nr))
+        got = Sl.gram(sev.CenteredMatrix(Ar, mur), C=C0.copy(order="F"), alpha=1.5, beta=-0.5)
+        ref = orc.centered_gram(m, Al.indptr, Al.indices, Al.data, mul, Ar.indptr, Ar.indices,
+                                Ar.data, mur, C=C0, alpha=1.5, beta=-0.5)
+    _close(got, ref, 1e-11)
+
+
+# ---------------------------------------------------------------------------
 # libcell's callback irlba entry point (src/irlba.jl:22-99)
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("m,n,nu", [(50, 50, 20), (100, 50, 20)])

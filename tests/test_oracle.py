@@ -254,6 +254,65 @@ def test_centered_mul_known_answers(orc, golden):
     assert np.allclose(r1, golden["cm_r1"], rtol=1e-14)
 
 
+def _centered(rng, m, n, dens, dt):
+    A = sp.random(m, n, density=dens, format="csc", random_state=rng, dtype=np.float64)
+    A.data = (A.data * 4.0 - 1.0).astype(dt)
+    A = A.astype(dt)
+    A.sort_indices()
+    mu = (rng.standard_normal(n) * 0.7).astype(dt)
+    return A, mu
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_centered_mul_mat_dense_closed_form(orc, dt):
+    """mul! of a CenteredMatrix with matrices (src/scaling.jl:259-271) against
+    (A - 1 mu') X and (A - 1 mu')' X of the densified operator."""
+    rng = np.random.default_rng(5)
+    m, n, p = 120, 30, 7
+    A, mu = _centered(rng, m, n, 0.2, dt)
+    D = A.toarray().astype(np.float64) - mu.astype(np.float64)[None, :]
+    X = rng.standard_normal((n, p))
+    W = rng.standard_normal((m, p))
+    Y0 = rng.standard_normal((m, p))
+    Z0 = rng.standard_normal((n, p))
+    got = orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, X, Y=Y0, alpha=2.0, beta=0.5)
+    assert np.allclose(got, 2.0 * D @ X + 0.5 * Y0, rtol=1e-12, atol=1e-12)
+    got = orc.centered_mul_mat(m, A.indptr, A.indices, A.data, mu, W, trans=True, Y=Z0,
+                               alpha=-1.5, beta=2.0)
+    assert np.allclose(got, -1.5 * D.T @ W + 2.0 * Z0, rtol=1e-12, atol=1e-12)
+    # a planted sign error in the adjoint centering term is caught
+    wrong = -1.5 * (A.toarray().T @ W + np.outer(mu, W.sum(0))) + 2.0 * Z0
+    assert not np.allclose(got, wrong, rtol=1e-6)
+
+
+@pytest.mark.parametrize("same", [True, False])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_centered_gram_dense_closed_form(orc, same, dt):
+    """mul!(C, Sl', Sr, alpha, beta) (src/scaling.jl:273-294) against
+    (Al - 1 mul')'(Ar - 1 mur') of the densified operands, for Sl === Sr,
+    two different matrices and an uncentered right operand."""
+    rng = np.random.default_rng(6 + same)
+    m, nl, nr = 150, 25, 25 if same else 18
+    Al, mul = _centered(rng, m, nl, 0.15, dt)
+    if same:
+        Ar, mur = Al, mul
+        got = orc.centered_gram(m, Al.indptr, Al.indices, Al.data, mul, alpha=0.5)
+    else:
+        Ar, mur = _centered(rng, m, nr, 0.3, dt)
+        C0 = rng.standard_normal((nl, nr))
+        got = orc.centered_gram(m, Al.indptr, Al.indices, Al.data, mul, Ar.indptr, Ar.indices,
+                                Ar.data, mur, C=C0, alpha=0.5, beta=-1.0)
+    Dl = Al.toarray().astype(np.float64) - mul.astype(np.float64)[None, :]
+    Dr = Ar.toarray().astype(np.float64) - mur.astype(np.float64)[None, :]
+    ref = 0.5 * Dl.T @ Dr + (0.0 if same else -1.0 * C0)
+    assert np.allclose(got, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    if not same:
+        got = orc.centered_gram(m, Al.indptr, Al.indices, Al.data, mul, Ar.indptr, Ar.indices,
+                                Ar.data, None)
+        assert np.allclose(got, Dl.T @ Ar.toarray().astype(np.float64), rtol=1e-11,
+                           atol=1e-11 * np.abs(got).max())
+
+
 # ---------------------------------------------------------------------------
 # small SVD and IRLBA  (test/test_irlba.jl cases against LAPACK)
 # ---------------------------------------------------------------------------
