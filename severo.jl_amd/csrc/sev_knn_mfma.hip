@@ -410,13 +410,18 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
 
     // seed: the head of the candidates without lists (see above)
     if (nseed > 0) {
-        float mx[NCB][QW][4];
+        // the 64 residues: column cb*16 + col of tile parity h
+        constexpr int NRES = kWave / kCT;
+        static_assert(NRES * kCT == kWave, "tiles of 16, 32 or 64 candidates");
+        float mx[NRES][NCB][QW][4];
 #pragma unroll
-        for (int cb = 0; cb < NCB; ++cb)
+        for (int h = 0; h < NRES; ++h)
 #pragma unroll
-            for (int rb = 0; rb < QW; ++rb)
+            for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) mx[cb][rb][i] = ninf();
+                for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) mx[h][cb][rb][i] = ninf();
         fetch(0, nseed);
         deposit(tiles[0], 0);
         __builtin_amdgcn_s_waitcnt(0);
@@ -427,18 +432,23 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             if (more) fetch(t0 + kCT, nseed);
             v4f acc[NCB][QW];
             products(tiles[sb], sb, acc);
+            const int par = (int)((t0 / kCT) % NRES);
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb)
+            for (int h = 0; h < NRES; ++h) {
+                if (h != par) continue;
 #pragma unroll
-                for (int rb = 0; rb < QW; ++rb)
+                for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        mx[cb][rb][i] = fmaxf(mx[cb][rb][i], acc[cb][rb][i]);
+                    for (int rb = 0; rb < QW; ++rb)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+                            mx[h][cb][rb][i] = fmaxf(mx[h][cb][rb][i], acc[cb][rb][i]);
+            }
             if (more) deposit(tiles[sb ^ 1], sb ^ 1);
             __syncthreads();
         }
         // the 64 maxima of each row into its (still empty) list storage:
-        // residue cb*16 + col at slot cb*16 + col of listV, continuing in
+        // residue (h*NCB + cb)*16 + col at that slot of listV, continuing in
         // listI for lists of 32
 #pragma unroll
         for (int rb = 0; rb < QW; ++rb)
@@ -446,13 +456,15 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             for (int i = 0; i < 4; ++i) {
                 const int row = w * QW * 16 + rb * 16 + 4 * grp + i;
 #pragma unroll
-                for (int cb = 0; cb < NCB; ++cb) {
-                    const int slot = cb * 16 + col;
-                    if (slot < KL)
-                        listV[row][slot] = mx[cb][rb][i];
-                    else
-                        listI[row][slot - KL] = __float_as_int(mx[cb][rb][i]);
-                }
+                for (int h = 0; h < NRES; ++h)
+#pragma unroll
+                    for (int cb = 0; cb < NCB; ++cb) {
+                        const int slot = (h * NCB + cb) * 16 + col;
+                        if (slot < KL)
+                            listV[row][slot] = mx[h][cb][rb][i];
+                        else
+                            listI[row][slot - KL] = __float_as_int(mx[h][cb][rb][i]);
+                    }
             }
         __syncthreads();
         // each wavefront: the kSeedRank-th largest of its rows' maxima
