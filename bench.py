@@ -138,7 +138,7 @@ def roofline(prof, form, steps, workload):
     traffic = measured_traffic().get((name, form, workload))
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
-        # knn_cand_<form>[_kl64]: the arithmetic of that kernel
+        # knn_cand_<form>[_kl48 or _kl64]: the arithmetic of that kernel
         kform = name.split("_")[2] if name.startswith("knn_cand_") else "fp32"
         peak = PEAK_TFLOPS[kform if kform in PEAK_TFLOPS else "fp32"]
         return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,

@@ -223,7 +223,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
            const int64_t *seg, const int32_t *row_sample, const float *scf, const float *hn,
            const int32_t *perm, int64_t nseed) {
     typedef typename Elem<MODE>::T E;
-    static_assert(KL == 32 || KL == 64, "lists of 32 or 64");
+    static_assert(KL % 16 == 0 && KL >= 32 && KL <= kWave, "lists of 32, 48 or 64");
     constexpr int QB = kNW * QW * 16;
     // LDS row stride (elements): odd for FP32, 16 bytes of padding otherwise
     constexpr int ROW = MODE == 0 ? KP + 1 : (MODE == 1 ? KP + 8 : KP + 16);
@@ -660,15 +660,16 @@ __device__ __forceinline__ void wave_rank_sort2(float (&v)[2], int32_t (&id)[2])
 // kept over all chunks.  Every candidate outside the merged list lies below
 // its chunk's threshold, which is at most the merged one when KL were kept;
 // otherwise the merged threshold is the largest chunk threshold (a chunk's
-// list that is not full has its starting threshold).  For KL = 64 the
-// running list is one register of the lanes and the chunk's list another.
+// list that is not full has its starting threshold).  For KL > 32 the
+// running list is one register of the lanes and the chunk's list another
+// (the running register keeps up to 64 entries; its first KL are the list).
 template <int KL>
 __global__ void __launch_bounds__(kBlock)
 k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, const int32_t *pcnt,
             const float *pthr, int32_t *cidx, float *cthr, int32_t *ccnt) {
-    static_assert(2 * KL <= kWave || KL == kWave, "two lists must fit a wavefront");
+    static_assert(KL <= kWave, "a list must fit a wavefront");
     const int lane = lane_id();
-    if constexpr (KL == kWave) {
+    if constexpr (2 * KL > kWave) {
         const int64_t qi = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kWave;
         if (qi >= nq) return;
         float v[2] = {ninf(), ninf()};
@@ -685,7 +686,7 @@ k_knn_merge(int64_t nq, int nchunk, const int32_t *pidx, const float *pval, cons
             wave_rank_sort2(v, id);
         }
         const int cnt = tot < KL ? tot : KL;
-        cidx[qi * KL + lane] = lane < cnt ? id[0] : -1;
+        if (lane < KL) cidx[qi * KL + lane] = lane < cnt ? id[0] : -1;
         const float last = __shfl(v[0], KL - 1, kWave);
         if (lane == 0) {
             ccnt[qi] = cnt;
@@ -995,7 +996,9 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     w.ccnt.ensure((size_t)nq);
     {
         ProfScope ps(MODE == 0 ? "knn_cand_fp32"
-                               : (MODE == 1 ? (KL > 32 ? "knn_cand_bf16_kl64" : "knn_cand_bf16")
+                               : (MODE == 1 ? (KL == 64 ? "knn_cand_bf16_kl64"
+                                                        : (KL == 48 ? "knn_cand_bf16_kl48"
+                                                                    : "knn_cand_bf16"))
                                             : "knn_cand_fp8"),
                      st, 0.0,
                      2.0 * (double)nq * (double)n * (double)d);
@@ -1051,7 +1054,7 @@ void run_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
     SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
     const int64_t *left = w.fail.get();
-    if constexpr (MODE == 1 && KL == 32 && KP32 > 0) {
+    if constexpr (MODE == 1 && KL < 64 && KP32 > 0) {
         if (nf > 0) {
             // BF16 rows without a certificate: BF16 candidates again with a
             // list of 64 (the operands of the first pass), whose larger gap
@@ -1168,6 +1171,8 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
         SEV_REQUIRE(d + 2 <= 128, "knn: BF16 form supports d <= 126");
         if (k <= 24 && d + 2 <= 64 && d + 1 <= 52)
             run_mfma<1, 64, 2, 32, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
+        else if (k <= 40 && d + 2 <= 64 && d + 1 <= 52)
+            run_mfma<1, 64, 2, 48, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st, n_fallback);
         else if (k <= 24 && d + 2 <= 64)
             run_mfma<1, 64, 2, 32, 128>(Z, n, d, ld, k, include_self, w, idx, dist, st,
                                         n_fallback);
@@ -1186,6 +1191,9 @@ void knn_mfma(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int i
         if (d + 1 <= 52) {
             if (k <= 24)
                 run_mfma<0, 52, 2, 32, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
+                                           n_fallback);
+            else if (k <= 40)
+                run_mfma<0, 52, 2, 48, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,
                                            n_fallback);
             else
                 run_mfma<0, 52, 1, 64, 52>(Z, n, d, ld, k, include_self, w, idx, dist, st,

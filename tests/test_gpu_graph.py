@@ -147,7 +147,8 @@ def test_mfma_lane_layout_selftest(sev, what):
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp8", "exact"])
 @pytest.mark.parametrize("include_self", [True, False])
-@pytest.mark.parametrize("n,d,k", [(5000, 50, 20), (3000, 30, 30), (700, 10, 5), (130, 3, 64)])
+@pytest.mark.parametrize("n,d,k", [(5000, 50, 20), (3000, 30, 30), (6000, 20, 40), (700, 10, 5),
+                                   (130, 3, 64)])
 def test_ann_precision_forms_bit_exact(sev, orc, precision, include_self, n, d, k):
     rng = np.random.default_rng(n * 7 + d)
     # clustered data: neighbours much closer than the norms
