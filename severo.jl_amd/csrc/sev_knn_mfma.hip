@@ -225,8 +225,11 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     typedef typename Elem<MODE>::T E;
     static_assert(KL % 16 == 0 && KL >= 32 && KL <= kWave, "lists of 32, 48 or 64");
     constexpr int QB = kNW * QW * 16;
-    // LDS row stride (elements): odd for FP32, 16 bytes of padding otherwise
-    constexpr int ROW = MODE == 0 ? KP + 1 : (MODE == 1 ? KP + 8 : KP + 16);
+    // LDS row stride (elements): odd for FP32; BF16 rows unpadded, their
+    // 16-byte chunks stored at chunk ^ (row mod chunks per row), so the 8
+    // lanes of a 16-byte read phase meet 8 different bank groups; FP8 rows
+    // padded by 16 bytes
+    constexpr int ROW = MODE == 0 ? KP + 1 : (MODE == 1 ? KP : KP + 16);
     constexpr int KS = MODE == 0 ? KP / 4 : KP / 32;  // MFMA steps per block
     constexpr int NCB = kCT / 16;                     // candidate blocks per tile
     __shared__ __attribute__((aligned(16))) E tiles[2][kCT * ROW];
@@ -315,6 +318,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     constexpr int CH = KP / EPL;                               // loads per candidate row
     constexpr int NLD = (kCT * CH + kBlock - 1) / kBlock;      // loads per thread
     static_assert(KP % EPL == 0, "row length must be a multiple of 16 bytes");
+    static_assert(MODE != 1 || (CH & (CH - 1)) == 0, "BF16 rows of a power-of-two chunk count");
     uint4 stage[NLD];
     float fstage = 1.0f, hstage = 0.0f;
     int32_t idstage = -1;
@@ -361,7 +365,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 d[2] = __uint_as_float(stage[u].z);
                 d[3] = __uint_as_float(stage[u].w);
             } else {
-                *(uint4 *)(&tile[r * ROW + part * EPL]) = stage[u];
+                const int pc = MODE == 1 ? (part ^ (r & (CH - 1))) : part;
+                *(uint4 *)(&tile[r * ROW + pc * EPL]) = stage[u];
             }
         }
     };
@@ -381,7 +386,8 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                     bfv[s] = ((const float *)tile)[(cb * 16 + col) * ROW + 4 * s + grp];
                 else
                     bbv[s] = *(const v8bf *)((const uint16_t *)tile + (cb * 16 + col) * ROW +
-                                             32 * s + 8 * grp);
+                                             ((4 * s + grp) ^ ((cb * 16 + col) & (CH - 1))) *
+                                                 EPL);
             }
 #pragma unroll
             for (int rb = 0; rb < QW; ++rb) {
