@@ -319,10 +319,12 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     constexpr int NLD = (kCT * CH + kBlock - 1) / kBlock;      // loads per thread
     static_assert(KP % EPL == 0, "row length must be a multiple of 16 bytes");
     static_assert(MODE != 1 || (CH & (CH - 1)) == 0, "BF16 rows of a power-of-two chunk count");
-    uint4 stage[NLD];
-    float fstage = 1.0f, hstage = 0.0f;
-    int32_t idstage = -1;
-    auto fetch = [&](int64_t t0, int64_t lim) {
+    // two register stage sets: the main loop keeps the loads of two tiles
+    // in flight (the candidate rows mostly come from the MALL)
+    uint4 stage[2][NLD];
+    float fstage[2] = {1.0f, 1.0f}, hstage[2] = {0.0f, 0.0f};
+    int32_t idstage[2] = {-1, -1};
+    auto fetch = [&](int64_t t0, int64_t lim, int k) __attribute__((always_inline)) {
 #pragma unroll
         for (int u = 0; u < NLD; ++u) {
             const int e = threadIdx.x + u * kBlock;
@@ -331,26 +333,26 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             uint4 v = make_uint4(0, 0, 0, 0);
             if (e < kCT * CH && c < lim)
                 v = *(const uint4 *)((const E *)Zc + c * KP + part * EPL);
-            stage[u] = v;
+            stage[k][u] = v;
         }
         if (perm && threadIdx.x < kCT) {
             const int64_t c = t0 + threadIdx.x;
-            idstage = c < lim ? perm[c] : -1;
+            idstage[k] = c < lim ? perm[c] : -1;
         }
         if constexpr (MODE == 2) {
             const int64_t c = t0 + threadIdx.x;
             if (threadIdx.x < kCT) {
-                fstage = c < lim ? scf[c] : 1.0f;
-                hstage = c < lim ? hn[c] : 0.0f;
+                fstage[k] = c < lim ? scf[c] : 1.0f;
+                hstage[k] = c < lim ? hn[c] : 0.0f;
             }
         }
     };
-    auto deposit = [&](E *tile, int b) {
-        if (perm && threadIdx.x < kCT) tileid[b][threadIdx.x] = idstage;
+    auto deposit = [&](E *tile, int b, int k) __attribute__((always_inline)) {
+        if (perm && threadIdx.x < kCT) tileid[b][threadIdx.x] = idstage[k];
         if constexpr (MODE == 2) {
             if (threadIdx.x < kCT) {
-                tilef[b][threadIdx.x] = fstage;
-                tileh[b][threadIdx.x] = hstage;
+                tilef[b][threadIdx.x] = fstage[k];
+                tileh[b][threadIdx.x] = hstage[k];
             }
         }
 #pragma unroll
@@ -360,18 +362,18 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             const int r = e / CH, part = e % CH;
             if (MODE == 0) {
                 float *d = (float *)tile + r * ROW + part * 4;  // odd row stride: scalar stores
-                d[0] = __uint_as_float(stage[u].x);
-                d[1] = __uint_as_float(stage[u].y);
-                d[2] = __uint_as_float(stage[u].z);
-                d[3] = __uint_as_float(stage[u].w);
+                d[0] = __uint_as_float(stage[k][u].x);
+                d[1] = __uint_as_float(stage[k][u].y);
+                d[2] = __uint_as_float(stage[k][u].z);
+                d[3] = __uint_as_float(stage[k][u].w);
             } else {
                 const int pc = MODE == 1 ? (part ^ (r & (CH - 1))) : part;
-                *(uint4 *)(&tile[r * ROW + pc * EPL]) = stage[u];
+                *(uint4 *)(&tile[r * ROW + pc * EPL]) = stage[k][u];
             }
         }
     };
     // all products of a tile: acc[cb][rb] = 16 queries x 16 candidates
-    auto products = [&](const E *tile, int b, v4f (&acc)[NCB][QW]) {
+    auto products = [&](const E *tile, int b, v4f (&acc)[NCB][QW]) __attribute__((always_inline)) {
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             float bfv[MODE == 0 ? KS : 1];
@@ -428,14 +430,14 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 for (int rb = 0; rb < QW; ++rb)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) mx[h][cb][rb][i] = ninf();
-        fetch(0, nseed);
-        deposit(tiles[0], 0);
+        fetch(0, nseed, 0);
+        deposit(tiles[0], 0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int sb = 0;
         for (int64_t t0 = 0; t0 < nseed; t0 += kCT, sb ^= 1) {
             const bool more = t0 + kCT < nseed;
-            if (more) fetch(t0 + kCT, nseed);
+            if (more) fetch(t0 + kCT, nseed, 0);
             v4f acc[NCB][QW];
             products(tiles[sb], sb, acc);
             const int par = (int)((t0 / kCT) % NRES);
@@ -450,7 +452,7 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                         for (int i = 0; i < 4; ++i)
                             mx[h][cb][rb][i] = fmaxf(mx[h][cb][rb][i], acc[cb][rb][i]);
             }
-            if (more) deposit(tiles[sb ^ 1], sb ^ 1);
+            if (more) deposit(tiles[sb ^ 1], sb ^ 1, 0);
             __syncthreads();
         }
         // the 64 maxima of each row into its (still empty) list storage:
@@ -504,17 +506,21 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             }
         __syncthreads();
     }
-    fetch(cbeg, cend);
-    deposit(tiles[0], 0);
-    // every global load issued so far (query operands, first tile) is
+    // tile t is in LDS buffer t % 2 and its rows were loaded into stage set
+    // t % 2 two tiles earlier
+    fetch(cbeg, cend, 0);
+    if (cbeg + kCT < cend) fetch(cbeg + kCT, cend, 1);
+    deposit(tiles[0], 0, 0);
+    // every global load issued so far (query operands, first two tiles) is
     // complete here, so the loop only waits for its own prefetches
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    int buf = 0;
-    for (int64_t t0 = cbeg; t0 < cend; t0 += kCT, buf ^= 1) {
+    // one tile: buffer and stage set buf; the next tile's rows (stage set
+    // buf ^ 1) go to the other buffer at the end
+    auto tile_step = [&](int64_t t0, int buf) __attribute__((always_inline)) {
         const E *tile = tiles[buf];
         const bool more = t0 + kCT < cend;
-        if (more) fetch(t0 + kCT, cend);
+        if (t0 + 2 * kCT < cend) fetch(t0 + 2 * kCT, cend, buf);
         // the tile holds padding columns, or (self pairs excluded) may hold
         // this block's own queries: such pairs are rejected in the insertion
         // path (their products may pass the threshold test)
@@ -595,8 +601,15 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 }
             }
         }
-        if (more) deposit(tiles[buf ^ 1], buf ^ 1);
+        if (more) deposit(tiles[buf ^ 1], buf ^ 1, buf ^ 1);
         __syncthreads();
+    };
+    for (int64_t t0 = cbeg; t0 < cend;) {
+        tile_step(t0, 0);
+        t0 += kCT;
+        if (t0 >= cend) break;
+        tile_step(t0, 1);
+        t0 += kCT;
     }
     for (int ql = w; ql < QB; ql += kNW) {
         const int64_t qi = q0 + ql;
