@@ -443,17 +443,11 @@ void centered_gram(const DevCsc &Al, const DevCsc *Ar_, double *C, int64_t ldc, 
     AA.ensure((size_t)(nl * nr));
     const int kw = (int)std::min<int64_t>(nr, kGramW);
     const size_t lds = sizeof(double) * kNW * (size_t)kw;
-    static bool attr[2] = {false, false};
-    if (lds > 65536 && !attr[Al.f64]) {
-        if (Al.f64)
-            SEV_HIP(hipFuncSetAttribute((const void *)k_gram_cols<true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(sizeof(double) * kNW * kGramW)));
-        else
-            SEV_HIP(hipFuncSetAttribute((const void *)k_gram_cols<false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(sizeof(double) * kNW * kGramW)));
-        attr[Al.f64] = true;
+    if (lds > 65536) {
+        // set on every call: the attribute belongs to the current device
+        const void *fn = Al.f64 ? (const void *)k_gram_cols<true> : (const void *)k_gram_cols<false>;
+        SEV_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(double) * kNW * kGramW)));
     }
     const unsigned g = (unsigned)std::min<int64_t>(nl, (int64_t)device_cu_count() * 4);
     for (int64_t k0 = 0; k0 < nr; k0 += kw) {
