@@ -223,15 +223,16 @@ def fp64_errors(sev, p, npcs):
 
 
 BATCH = dict(samples=64, genes=20000, cells=(500, 50000), density=(0.02, 0.08), features=2000,
-             npcs=30, k=15, seed=1000)
+             npcs=30, k=15, seed=1000, work=64)
 
 
 def batch_leg(sev, steps, warmup, graph=True):
     """The batching form: 64 samples over one 20,000-gene set, cells
     log-uniform in [500, 50000], densities uniform in [2%, 8%], seed per
     sample = 1000 + s; per-sample normalise, 2000 features (each sample's
-    top generator means), IRLBA 30 components (BF16 operator values, FP64
-    vectors), kNN k = 15, SNN prune 1/15.  The batch runs all samples in one
+    top generator means), IRLBA 30 components with 64 Lanczos vectors per
+    sample (BF16 operator values, FP64 vectors), kNN k = 15, SNN prune 1/15.
+    The batch runs all samples in one
     stacked launch sequence (restart rounds replayed from HIP graphs); the
     comparison loops the single-sample pipeline over the samples."""
     b = BATCH
@@ -253,7 +254,7 @@ def batch_leg(sev, steps, warmup, graph=True):
     def batched():
         pb.normalize("lognormalize", SF)
         pb.select_scale_batch(feats, SCALE_MAX)
-        it, nm = pb.pca_batch(b["npcs"], init=init, tol=1e-5, graph=graph)
+        it, nm = pb.pca_batch(b["npcs"], init=init, tol=1e-5, graph=graph, work=b["work"])
         nfb = pb.neighbours(b["k"], include_self=True, precision="bf16")
         nz = pb.snn(PRUNE, np.float32)
         pb.synchronize()
@@ -288,7 +289,8 @@ def batch_leg(sev, steps, warmup, graph=True):
         for s, q in enumerate(ps):
             q.normalize("lognormalize", SF)
             q.select_scale(feats[s], SCALE_MAX)
-            its.append(q.pca(b["npcs"], init=init[s], tol=1e-5, precision=form)[0])
+            its.append(q.pca(b["npcs"], init=init[s], tol=1e-5, precision=form,
+                             work=min(b["work"], int(cells[s]), b["features"]))[0])
             q.neighbours(b["k"], include_self=True, precision=form)
             q.snn(PRUNE, np.float32)
         for q in ps:
@@ -307,7 +309,8 @@ def batch_leg(sev, steps, warmup, graph=True):
     out = {
         "workload": f"{S} samples x {b['genes']} genes, cells log-uniform in [{lo}, {hi}] "
                     f"(total {M}), density U({b['density'][0]}, {b['density'][1]}), "
-                    f"{b['features']} features, {b['npcs']} PCs, k={b['k']}, prune 1/15",
+                    f"{b['features']} features, {b['npcs']} PCs ({b['work']} Lanczos vectors, "
+                    f"in the batch and in the loops), k={b['k']}, prune 1/15",
         "samples": S, "cells": M, "nnz": int(nnz),
         "ms_per_step": round(dt_b * 1e3, 3), "cells_per_s": round(M / dt_b, 2),
         "form": "bf16 operator values, bf16 kNN candidates, one stacked launch sequence, "

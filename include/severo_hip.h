@@ -376,8 +376,10 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit,
  * centered matrix), all run in lockstep over the Lanczos steps with BF16
  * operator values and FP64 vectors; init is S x nfeat (row per sample);
  * iters[S].  use_graph != 0 records each restart round once as a HIP graph
- * and replays it.  npcs is capped by the smallest sample. */
-int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+ * and replays it.  npcs is capped by the smallest sample.  work: Lanczos
+ * vectors per sample, 0 for the reference's npcs + 7, else npcs + 1 .. 64
+ * (capped by the smallest sample and the feature count). */
+int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit, double tol,
                            const double *init, int use_graph, int64_t *iters, int64_t *nmult);
 /* s[npcs]; U (m x npcs) and V (nfeat x npcs) column-major FP64; Z (m x npcs
  * column-major Float32).  Any pointer may be NULL.  For a batch, s is

@@ -80,7 +80,7 @@ SIGNATURES = {
     "sev_pipeline_scaled_nnz": (_int, [_p, _pi64]),
     "sev_pipeline_get_scaled": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_pca": (_int, [_p, _i64, _i64, _i64, _dbl, _p, _int, _pi64, _pi64]),
-    "sev_pipeline_pca_batch": (_int, [_p, _i64, _i64, _dbl, _p, _int, _p, _pi64]),
+    "sev_pipeline_pca_batch": (_int, [_p, _i64, _i64, _i64, _dbl, _p, _int, _p, _pi64]),
     "sev_pipeline_get_pca": (_int, [_p, _p, _p, _p, _p]),
     "sev_pipeline_set_embedding": (_int, [_p, _i64, _i64, _p, _i64]),
     "sev_pipeline_neighbours": (_int, [_p, _i64, _int, _i64, _int, _pi64]),

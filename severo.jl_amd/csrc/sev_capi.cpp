@@ -882,7 +882,7 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit,
     });
 }
 
-int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t maxit, double tol,
+int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit, double tol,
                            const double *init, int use_graph, int64_t *iters, int64_t *nmult) {
     return guarded([&] {
         ProfInstall pi_(&p->prof);
@@ -904,7 +904,9 @@ int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t maxit, double 
         batch_slabs(p->op, p->bl, p->fcp.get(), p->frow.get(), st);
         IrlbaParams prm;
         prm.nu = nu;
-        prm.work = nu + 7;
+        SEV_REQUIRE(work == 0 || (work >= nu + 1 && work <= 64),
+                    "batch work must be 0 (npcs + 7) or in [npcs + 1, 64]");
+        prm.work = work == 0 ? nu + 7 : work;
         prm.maxit = maxit;
         prm.tol = tol;
         prm.svtol = tol;

@@ -164,9 +164,11 @@ class Pipeline:
                                                       float(scale_max)))
         self.nfeat = f.shape[1]
 
-    def pca_batch(self, npcs=30, init=None, tol=1e-5, maxit=1000, graph=True, rng=None):
+    def pca_batch(self, npcs=30, init=None, tol=1e-5, maxit=1000, graph=True, rng=None, work=0):
         """One IRLBA per sample (BF16 operator values, FP64 vectors), in
-        lockstep; returns (iterations per sample, total products)."""
+        lockstep; returns (iterations per sample, total products).  work:
+        Lanczos vectors per sample (0: npcs + 7, the reference's choice; at
+        most 64, capped by the smallest sample)."""
         S = len(self.cells)
         if init is None:
             rng = np.random.default_rng(0) if rng is None else rng
@@ -176,7 +178,7 @@ class Pipeline:
             raise ValueError("init must be S x F")
         it = np.zeros(S, np.int64)
         nm = C.c_int64()
-        check(self._L.sev_pipeline_pca_batch(self._h, npcs, maxit, float(tol), _ptr(init),
+        check(self._L.sev_pipeline_pca_batch(self._h, npcs, int(work), maxit, float(tol), _ptr(init),
                                              int(bool(graph)), _ptr(it), C.byref(nm)))
         self.npcs = min(npcs, int(self.cells.min()), self.nfeat)
         return it, nm.value

@@ -149,3 +149,21 @@ def test_batch_knn_cosine_per_sample(batch, orc, precision):
         ri, rd = orc.knn_cosine(np.asfortranarray(Z[row0[s]:row0[s + 1]]), K)
         assert np.array_equal(idx[row0[s]:row0[s + 1]], ri + row0[s]), (s, precision)
         assert np.array_equal(dist[row0[s]:row0[s + 1]], rd), (s, precision)
+
+
+def test_batch_pca_wider_basis(batch, orc, sev):
+    """work = 48 Lanczos vectors per sample: the same singular values as the
+    oracle's IRLBA with the reference's npcs + 7, in fewer restarts; a basis
+    outside [npcs + 1, 64] is refused."""
+    p = batch["p"]
+    it, _ = p.pca_batch(NPCS, init=batch["init"], tol=1e-5, graph=True, work=48)
+    s_all = p.get_pca_batch()[0]
+    assert it.max() < batch["it"].max(), (it, batch["it"])
+    for s, r in enumerate(batch["ref"]):
+        rc, so, _, _, _, _ = orc.irlba_csc(CELLS[s], r["fcp"], r["frow"], _bf16(r["B"]), r["mu"],
+                                           NPCS, batch["init"][s])
+        assert rc == 0
+        assert np.allclose(s_all[s], so, rtol=1e-5, atol=0), (s, s_all[s], so)
+    for bad in (NPCS, 65):
+        with pytest.raises(sev.SeveroError, match="batch work"):
+            p.pca_batch(NPCS, init=batch["init"], tol=1e-5, work=bad)
