@@ -414,6 +414,7 @@ struct KnnWork {
     DevBuf<unsigned char> zq, zc, zcp, stmp;
     DevBuf<float> zr, nhat, eps, cthr, pval, pthr, scf, hn;
     DevBuf<unsigned> gmax, bmax, skey, skey2;
+    DevBuf<uint64_t> skey64, skey64b;
     DevBuf<int32_t> cidx, ccnt, nfail, pidx, pcnt, perm, sval;
     DevBuf<int64_t> fail, fail2, qpad;
 };

@@ -279,6 +279,10 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
     const int64_t csz = ((re - rb + gridDim.y - 1) / gridDim.y + kCT - 1) / kCT * kCT;
     const int64_t cbeg = rb + (int64_t)blockIdx.y * csz;
     const int64_t cend = cbeg + csz < re ? cbeg + csz : re;
+    // the seed head: [0, nseed), or with seg the lowest-norm 1/64 of the
+    // block's sample (nseed > 0 then only enables it)
+    const int64_t sh0 = seg ? rb : 0;
+    if (seg && nseed > 0) nseed = ((re - rb) / 64) / kCT * kCT;
 #pragma unroll
     for (int rb = 0; rb < QW; ++rb) {
         const int32_t qpt = qg[w * QW * 16 + rb * 16 + col];
@@ -430,17 +434,17 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
                 for (int rb = 0; rb < QW; ++rb)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) mx[h][cb][rb][i] = ninf();
-        fetch(0, nseed, 0);
+        fetch(sh0, sh0 + nseed, 0);
         deposit(tiles[0], 0, 0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int sb = 0;
-        for (int64_t t0 = 0; t0 < nseed; t0 += kCT, sb ^= 1) {
-            const bool more = t0 + kCT < nseed;
-            if (more) fetch(t0 + kCT, nseed, 0);
+        for (int64_t t0 = sh0; t0 < sh0 + nseed; t0 += kCT, sb ^= 1) {
+            const bool more = t0 + kCT < sh0 + nseed;
+            if (more) fetch(t0 + kCT, sh0 + nseed, 0);
             v4f acc[NCB][QW];
             products(tiles[sb], sb, acc);
-            const int par = (int)((t0 / kCT) % NRES);
+            const int par = (int)(((t0 - sh0) / kCT) % NRES);
 #pragma unroll
             for (int h = 0; h < NRES; ++h) {
                 if (h != par) continue;
@@ -908,6 +912,16 @@ k_norm_keys(const float *nhat, int64_t n, unsigned *key, int32_t *id) {
     }
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_norm_keys_seg(const float *nhat, const int32_t *row_sample, int64_t n, uint64_t *key,
+                int32_t *id) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        key[i] = ((uint64_t)(uint32_t)row_sample[i] << 32) | __float_as_uint(nhat[i]);
+        id[i] = (int32_t)i;
+    }
+}
+
 // dst row p = src row perm[p]; rows of ch 16-byte chunks
 __global__ void __launch_bounds__(kBlock)
 k_gather_rows(const uint4 *src, const int32_t *perm, int64_t n, int ch, uint4 *dst) {
@@ -923,23 +937,43 @@ k_gather_rows(const uint4 *src, const int32_t *perm, int64_t n, int ch, uint4 *d
 // Candidate order for the search: w.perm = the points by ascending |c|^2
 // (the stable radix sort keeps ties in index order) and w.zc's rows put in
 // that order.
-void norm_order(int64_t n, size_t row_bytes, KnnWork &w, hipStream_t st) {
+void norm_order(int64_t n, size_t row_bytes, KnnWork &w, hipStream_t st,
+                const int32_t *row_sample = nullptr, int64_t nsamp = 0) {
     SEV_REQUIRE(row_bytes % 16 == 0, "knn: operand rows must be 16-byte multiples");
     ProfScope ps("knn_order", st, 2.0 * n * row_bytes + 24.0 * n);
-    w.skey.ensure((size_t)n);
-    w.skey2.ensure((size_t)n);
     w.sval.ensure((size_t)n);
     w.perm.ensure((size_t)n);
     const unsigned g = (unsigned)std::min<int64_t>(ceil_div(n, kBlock), device_cu_count() * 8);
-    hipLaunchKernelGGL(k_norm_keys, dim3(g), dim3(kBlock), 0, st, w.nhat.get(), n, w.skey.get(),
-                       w.sval.get());
-    SEV_LAUNCH_CHECK();
     size_t tb = 0;
-    SEV_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, w.skey.get(), w.skey2.get(),
-                                               w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
-    w.stmp.ensure(tb);
-    SEV_HIP(hipcub::DeviceRadixSort::SortPairs(w.stmp.get(), tb, w.skey.get(), w.skey2.get(),
-                                               w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
+    if (row_sample) {
+        // a batch: by (sample, |c|^2), so every sample's rows stay in its
+        // own row range (the samples are stacked in order)
+        w.skey64.ensure((size_t)n);
+        w.skey64b.ensure((size_t)n);
+        hipLaunchKernelGGL(k_norm_keys_seg, dim3(g), dim3(kBlock), 0, st, w.nhat.get(),
+                           row_sample, n, w.skey64.get(), w.sval.get());
+        SEV_LAUNCH_CHECK();
+        int end_bit = 33;
+        while (end_bit < 64 && (nsamp >> (end_bit - 32)) != 0) ++end_bit;
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, w.skey64.get(), w.skey64b.get(),
+                                                   w.sval.get(), w.perm.get(), (int)n, 0,
+                                                   end_bit, st));
+        w.stmp.ensure(tb);
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(w.stmp.get(), tb, w.skey64.get(),
+                                                   w.skey64b.get(), w.sval.get(), w.perm.get(),
+                                                   (int)n, 0, end_bit, st));
+    } else {
+        w.skey.ensure((size_t)n);
+        w.skey2.ensure((size_t)n);
+        hipLaunchKernelGGL(k_norm_keys, dim3(g), dim3(kBlock), 0, st, w.nhat.get(), n,
+                           w.skey.get(), w.sval.get());
+        SEV_LAUNCH_CHECK();
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, w.skey.get(), w.skey2.get(),
+                                                   w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
+        w.stmp.ensure(tb);
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(w.stmp.get(), tb, w.skey.get(), w.skey2.get(),
+                                                   w.sval.get(), w.perm.get(), (int)n, 0, 32, st));
+    }
     const int ch = (int)(row_bytes / 16);
     w.zcp.ensure((size_t)n * row_bytes);
     const unsigned g2 =
@@ -958,12 +992,14 @@ int64_t seed_head(int64_t n) { return (n / 64) / kCT * kCT; }
 // fail the certificate are appended to w.fail (count in w.nfail).  order:
 // 0 = candidates in index order; 1 = candidates by ascending norm (made
 // here, prep required) with seed thresholds; 2 = the same with the norm
-// order the last order-1 call made on the same operands.
+// order the last order-1 call made on the same operands.  With seg (a
+// batch of nsamp samples) the order is by norm within each sample.
 template <int MODE, int KP, int QW, int KL>
 void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, int include_self,
                  const int64_t *qidx, int64_t nq, KnnWork &w, int32_t *idx, float *dist,
                  int32_t *nfail, int64_t *fail, hipStream_t st, const int64_t *seg = nullptr,
-                 const int32_t *row_sample = nullptr, bool prep = true, int order = 0) {
+                 const int32_t *row_sample = nullptr, bool prep = true, int order = 0,
+                 int64_t nsamp = 0) {
     const size_t es = MODE == 0 ? 4 : (MODE == 1 ? 2 : 1);
     w.zq.ensure((size_t)n * KP * es);
     if (MODE != 2) w.zc.ensure((size_t)n * KP * es);  // FP8: one operand array for both
@@ -995,11 +1031,11 @@ void cand_rerank(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k, in
     const int32_t *perm = nullptr;
     int64_t nseed = 0;
     if (order != 0) {
-        SEV_REQUIRE(MODE != 2 && seg == nullptr, "knn: norm order needs separate operands");
+        SEV_REQUIRE(MODE != 2, "knn: norm order needs separate operands");
         SEV_REQUIRE(order == 2 || prep, "knn: norm order is made with the operands");
-        if (order == 1) norm_order(n, (size_t)KP * es, w, st);
+        if (order == 1) norm_order(n, (size_t)KP * es, w, st, seg ? row_sample : nullptr, nsamp);
         perm = w.perm.get();
-        nseed = seed_head(n);
+        nseed = seg ? 1 : seed_head(n);  // with seg the kernel sizes each sample's head
     }
     const void *zc_op = MODE == 2 ? (const void *)w.zq.get() : (const void *)w.zc.get();
     constexpr int QB = kNW * QW * 16;
@@ -1292,7 +1328,8 @@ void run_mfma_batch(const float *Z, int64_t n, int64_t d, int64_t ld, int64_t k,
     const std::vector<int64_t> q = batch_queries(L, nullptr, 0, kNW * QW * 16, false);
     upload_sync(w.qpad, q, st);
     cand_rerank<MODE, KP, QW, KL>(Z, n, d, ld, k, include_self, w.qpad.get(), (int64_t)q.size(),
-                                  w, idx, dist, w.nfail.get(), w.fail.get(), st, seg, rs);
+                                  w, idx, dist, w.nfail.get(), w.fail.get(), st, seg, rs, true,
+                                  1, L.S);
     int32_t nf = 0;
     SEV_HIP(hipMemcpyAsync(&nf, w.nfail.get(), sizeof nf, hipMemcpyDeviceToHost, st));
     SEV_HIP(hipStreamSynchronize(st));
