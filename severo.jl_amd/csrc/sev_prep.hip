@@ -353,6 +353,51 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
     }
 }
 
+// The same statistics for many short columns (the stacked batch, column
+// (j, s) holding feature j of sample s = col % S): one lane per column, each
+// lane running its column's recurrence in the same operations and order as
+// k_scale_stats (delta * RN64(1/F) rounded to Float32, exact division for
+// the tiny deltas), so the results are the same bits.
+__global__ void __launch_bounds__(kBlock)
+k_scale_stats_lane(int64_t ncol, const int64_t *colptr, const float *x, float scale_max,
+                   float *sdv, float *mupv, float *clipv, const int64_t *row0, int64_t S) {
+    const int64_t col = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (col >= ncol) return;
+    const int64_t smp = col % S;
+    const int64_t m = row0[smp + 1] - row0[smp];
+    const int64_t b = colptr[col], e = colptr[col + 1];
+    int64_t count = m - (e - b);
+    float mu = 0.0f, s = 0.0f;
+    auto step = [&](float v) {
+        ++count;
+        const float delta = v - mu;
+        float q;
+        if (fabsf(delta) < 0x1p-100f && delta != 0.0f)
+            q = delta / (float)count;
+        else
+            q = (float)((double)delta * (1.0 / (double)(float)count));
+        mu = mu + q;
+        s = s + delta * (v - mu);
+    };
+    // 16 loads of the lane's column in flight ahead of its recurrence
+    constexpr int U = 16;
+    int64_t p = b;
+    for (; p + U <= e; p += U) {
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = x[p + u];
+#pragma unroll
+        for (int u = 0; u < U; ++u) step(v[u]);
+    }
+    for (; p < e; ++p) step(x[p]);
+    const float var = s / (float)(m - 1);
+    const float sd = sqrtf(var);
+    const float mup = mu / sd;
+    sdv[col] = sd;
+    mupv[col] = mup;
+    clipv[col] = scale_max + mup;
+}
+
 // mean_std(Float64, A) of the Int32 counts (src/scaling.jl:18-38,119-130):
 // the same one-wavefront-per-column Welford recurrence in FP64 (IEEE
 // division by the count), sd = sqrt(s / (m - 1)).
@@ -807,11 +852,9 @@ void batch_gather(int64_t nv, const int64_t *src_b, const int64_t *vcp, const in
 void batch_scale_stats(int64_t ncol, const int64_t *colptr, const float *x, float scale_max,
                        float *sd, float *mup, float *clip, const int64_t *row0, int64_t S,
                        hipStream_t st) {
-    const int64_t threads = ncol * kWave;
     ProfScope ps("scale_stats", st);
-    hipLaunchKernelGGL(k_scale_stats<float>, dim3((unsigned)ceil_div(threads, kBlock)),
-                       dim3(kBlock), 0, st, (int64_t)0, ncol, colptr, x, scale_max, sd, mup, clip,
-                       row0, S);
+    hipLaunchKernelGGL(k_scale_stats_lane, dim3((unsigned)ceil_div(ncol, kBlock)), dim3(kBlock),
+                       0, st, ncol, colptr, x, scale_max, sd, mup, clip, row0, S);
     SEV_LAUNCH_CHECK();
 }
 
