@@ -20,6 +20,8 @@
 // reproducible run to run.
 #include "sev_kernels.hpp"
 
+#include <hipcub/hipcub.hpp>
+
 #include <cstdio>
 #include <cstdlib>
 
@@ -45,7 +47,6 @@ constexpr int kMaxWorkB = 64;  // Lanczos vectors per sample of a batch
 constexpr int64_t kMaxFL = 7168;     // largest feature count kept in LDS (A v, FP64)
 constexpr int64_t kMaxFL32 = 14336;  // the same for the FP32 copy of the BF16 form
 constexpr int kRedChunks = 16;       // first-level chunks of the slab partials
-constexpr int64_t kBuildRows = 256;  // row slabs of the CSR build
 
 __device__ __forceinline__ double ldval(const float *v, int64_t p) { return (double)v[p]; }
 __device__ __forceinline__ double ldval(const double *v, int64_t p) { return v[p]; }
@@ -168,41 +169,60 @@ __global__ void __launch_bounds__(kBlock) k_round_up(int32_t *cnt, int64_t m, in
         cnt[r] = (cnt[r] + R - 1) & ~(R - 1);
 }
 
-// Padded CSR fill: one wavefront per slab of kBuildRows rows walks the
-// columns in order and appends each entry at rowptr[row] + cursor[row]; the
-// rows of one column are distinct, so the LDS cursors need no atomics and
-// every row comes out sorted by column.  Padding slots were zeroed before.
+// Padded CSR placement: the entries sorted by row (a stable radix sort keeps
+// each row's entries in CSC, i.e. column, order) are copied to
+// rowptr[row] + (their rank within the row); each entry's column is the last
+// j with colptr[j] <= e.  Consecutive sorted entries land in consecutive
+// slots, so the writes are contiguous.  Padding slots were zeroed before.
 template <class ST, class DT>
 __global__ void __launch_bounds__(kBlock)
-k_csr_fill(int64_t m, int64_t n, int64_t nbs, const int64_t *boff, const int32_t *row,
-           const ST *val, const int64_t *rowptr, uint16_t *ccol, DT *cval) {
-    __shared__ int32_t cur[kWaves][kBuildRows];
-    __shared__ int64_t rp[kWaves][kBuildRows];
-    const int w = threadIdx.x / kWave, lane = lane_id();
-    const int64_t s = (int64_t)blockIdx.x * kWaves + w;
-    if (s >= nbs) return;
-    const int64_t r0 = s * kBuildRows;
-    for (int i = lane; i < kBuildRows; i += kWave) {
-        cur[w][i] = 0;
-        rp[w][i] = r0 + i < m ? rowptr[r0 + i] : 0;
-    }
-    for (int64_t j0 = 0; j0 < n; j0 += kWave) {
-        const int64_t jl = j0 + lane;
-        const int64_t a_l = jl < n ? boff[s * n + jl] : 0;
-        const int64_t b_l = jl < n ? boff[(s + 1) * n + jl] : 0;
-        const int nj = (int)((n - j0) < kWave ? (n - j0) : kWave);
-        for (int jj = 0; jj < nj; ++jj) {
-            const int64_t a = __shfl(a_l, jj, kWave), b = __shfl(b_l, jj, kWave);
-            for (int64_t p = a + lane; p < b; p += kWave) {
-                const int lr = (int)(row[p] - r0);
-                const int c = cur[w][lr];
-                cur[w][lr] = c + 1;
-                const int64_t pos = rp[w][lr] + c;
-                ccol[pos] = (uint16_t)(j0 + jj);
-                cval[pos] = cvt_store(val[p], (DT *)nullptr);
-            }
+k_csr_place(int64_t nnz, int64_t n, const int64_t *colptr, const uint32_t *srow,
+            const int32_t *sid, const int64_t *cstart, const int64_t *rowptr, const ST *val,
+            uint16_t *ccol, DT *cval) {
+    for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nnz;
+         u += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = srow[u];
+        const int64_t e = sid[u];
+        int64_t lo = 0, hi = n;  // colptr[lo] <= e < colptr[hi]
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (colptr[mid] <= e)
+                lo = mid;
+            else
+                hi = mid;
         }
+        const int64_t pos = rowptr[r] + (u - cstart[r]);
+        ccol[pos] = (uint16_t)lo;
+        cval[pos] = cvt_store(val[e], (DT *)nullptr);
     }
+}
+
+// BF16 operators: one 32-bit word per entry, column << 16 | bf16(value), so
+// the sorted words hold everything the CSR needs
+__global__ void __launch_bounds__(kBlock)
+k_csr_words(int64_t n, const int64_t *colptr, const float *val, uint32_t *word) {
+    for (int64_t j = blockIdx.x; j < n; j += gridDim.x)
+        for (int64_t e = colptr[j] + threadIdx.x; e < colptr[j + 1]; e += kBlock)
+            word[e] = ((uint32_t)j << 16) | cvt_store(val[e], (uint16_t *)nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_csr_place_words(int64_t nnz, const uint32_t *srow, const uint32_t *sword, const int64_t *cstart,
+                  const int64_t *rowptr, uint16_t *ccol, uint16_t *cval) {
+    for (int64_t u = (int64_t)blockIdx.x * kBlock + threadIdx.x; u < nnz;
+         u += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = srow[u];
+        const uint32_t wd = sword[u];
+        const int64_t pos = rowptr[r] + (u - cstart[r]);
+        ccol[pos] = (uint16_t)(wd >> 16);
+        cval[pos] = (uint16_t)(wd & 0xFFFFu);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota32(int32_t *v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        v[i] = (int32_t)i;
 }
 
 // --------------------------------------------------------------------------
@@ -1551,12 +1571,12 @@ unsigned grid_rows(int64_t m, int per_cu = 4) {
 int val_bytes(int vtype) { return vtype == kValBF16 ? 2 : (vtype == kValF64 ? 8 : 4); }
 
 template <class ST, class DT>
-void launch_csr_fill(int64_t m, int64_t n, int64_t nbs, int64_t nnz, const int64_t *boff,
-                     const int32_t *row, const void *val, const int64_t *rowptr, uint16_t *ccol,
-                     void *cval, hipStream_t st) {
+void launch_csr_place(int64_t nnz, int64_t n, const int64_t *colptr, const uint32_t *srow,
+                      const int32_t *sid, const int64_t *cstart, const int64_t *rowptr,
+                      const void *val, uint16_t *ccol, void *cval, hipStream_t st) {
     if (nnz == 0) return;
-    hipLaunchKernelGGL((k_csr_fill<ST, DT>), dim3((unsigned)ceil_div(nbs, kWaves)), dim3(kBlock),
-                       0, st, m, n, nbs, boff, row, (const ST *)val, rowptr, ccol, (DT *)cval);
+    hipLaunchKernelGGL((k_csr_place<ST, DT>), dim3(grid_rows(nnz)), dim3(kBlock), 0, st, nnz, n,
+                       colptr, srow, sid, cstart, rowptr, (const ST *)val, ccol, (DT *)cval);
     SEV_LAUNCH_CHECK();
 }
 
@@ -1608,16 +1628,15 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     }
     // padded CSR for A v
     ProfScope ps("csr_build", st, 2.0 * 10.0 * op.nnz);
-    BuildScratch b;
-    const int64_t nbs = ceil_div(m, kBuildRows);
-    b.boff.ensure((size_t)((nbs + 1) * n));
+    BuildScratch &b = op.build;
+    SEV_REQUIRE(op.nnz < ((int64_t)1 << 31) - 1, "more than 2^31 - 2 selected entries");
     b.cnt.ensure((size_t)m);
     b.scan.ensure(scan_scratch_elems(m));
+    b.cstart.ensure((size_t)m + 1);
     op.rowptr.ensure((size_t)m + 1);
-    hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, nbs, kBuildRows,
-                       colptr, row, b.boff.get(), (uint16_t *)nullptr);
-    SEV_LAUNCH_CHECK();
     row_counts(colptr, row, op.nnz, m, n, b.cnt.get(), st);
+    // the entries' offsets among the sorted entries, then the padded rows
+    scan_exclusive_i32(b.cnt.get(), b.cstart.get(), m, b.scan.get(), st);
     // BF16 rows are padded to 8 entries (16-byte loads of values and columns)
     const int ralign = bf16 ? 8 : 4;
     hipLaunchKernelGGL(k_round_up, dim3(grid_rows(m)), dim3(kBlock), 0, st, b.cnt.get(), m,
@@ -1632,16 +1651,48 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     op.cval.ensure(((size_t)op.csr_nnz + 8) * vb);
     SEV_HIP(hipMemsetAsync(op.ccol.get(), 0, sizeof(uint16_t) * (size_t)op.csr_nnz, st));
     SEV_HIP(hipMemsetAsync(op.cval.get(), 0, vb * (size_t)op.csr_nnz, st));
-    const int64_t *bo = b.boff.get(), *rptr = op.rowptr.get();
-    if (op.vtype == kValBF16)
-        launch_csr_fill<float, uint16_t>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
-                                         op.cval.get(), st);
-    else if (op.vtype == kValF64)
-        launch_csr_fill<double, double>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
-                                        op.cval.get(), st);
-    else
-        launch_csr_fill<float, float>(m, n, nbs, op.nnz, bo, row, val, rptr, op.ccol.get(),
-                                      op.cval.get(), st);
+    const bool words = op.vtype == kValBF16;
+    if (op.nnz > 0) {
+        // entries by row: keys = the CSC rows (non-negative int32); values =
+        // the entry's column and BF16 value (BF16 operators) or its id
+        b.ids.ensure((size_t)op.nnz);
+        b.sids.ensure((size_t)op.nnz);
+        b.srow.ensure((size_t)op.nnz);
+        if (words) {
+            hipLaunchKernelGGL(k_csr_words, dim3((unsigned)std::min<int64_t>(n, 65535)),
+                               dim3(kBlock), 0, st, n, colptr, (const float *)val,
+                               (uint32_t *)b.ids.get());
+        } else {
+            hipLaunchKernelGGL(k_iota32, dim3(grid_rows(op.nnz)), dim3(kBlock), 0, st,
+                               b.ids.get(), op.nnz);
+        }
+        SEV_LAUNCH_CHECK();
+        int end_bit = 1;
+        while (end_bit < 32 && (m >> end_bit) != 0) ++end_bit;
+        size_t tb = 0;
+        const uint32_t *keys = (const uint32_t *)row;
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, b.srow.get(), b.ids.get(),
+                                                   b.sids.get(), (int)op.nnz, 0, end_bit, st));
+        b.tmp.ensure(tb);
+        SEV_HIP(hipcub::DeviceRadixSort::SortPairs(b.tmp.get(), tb, keys, b.srow.get(),
+                                                   b.ids.get(), b.sids.get(), (int)op.nnz, 0,
+                                                   end_bit, st));
+    }
+    const int64_t *rptr = op.rowptr.get(), *cst = b.cstart.get();
+    if (words) {
+        if (op.nnz > 0) {
+            hipLaunchKernelGGL(k_csr_place_words, dim3(grid_rows(op.nnz)), dim3(kBlock), 0, st,
+                               op.nnz, b.srow.get(), (const uint32_t *)b.sids.get(), cst, rptr,
+                               op.ccol.get(), (uint16_t *)op.cval.get());
+            SEV_LAUNCH_CHECK();
+        }
+    } else if (op.vtype == kValF64) {
+        launch_csr_place<double, double>(op.nnz, n, colptr, b.srow.get(), b.sids.get(), cst,
+                                         rptr, val, op.ccol.get(), op.cval.get(), st);
+    } else {
+        launch_csr_place<float, float>(op.nnz, n, colptr, b.srow.get(), b.sids.get(), cst, rptr,
+                                       val, op.ccol.get(), op.cval.get(), st);
+    }
     op.col12 = op.vtype == kValBF16 && n <= 4096;
     if (op.col12) pack12(op.ccol.get(), op.csr_nnz, op.ccol12, st);
     const double avg = (double)op.csr_nnz / (double)m;

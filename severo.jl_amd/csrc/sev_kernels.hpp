@@ -214,6 +214,15 @@ constexpr int64_t kValuePad = 8;
 
 enum ValType : int { kValF32 = 0, kValBF16 = 1, kValF64 = 2 };
 
+// Temporaries of the operator build, kept with the operator so that a
+// rebuild (every step of a pipeline) reuses them.
+struct BuildScratch {
+    DevBuf<int64_t> scan, cstart;
+    DevBuf<int32_t> cnt, ids, sids;
+    DevBuf<uint32_t> srow;
+    DevBuf<unsigned char> tmp;
+};
+
 struct SlabCsc {
     int64_t m = 0, n = 0, nnz = 0, nslab = 0, csr_nnz = 0;
     DevBuf<int64_t> off;      // (nslab + 1) * n
@@ -234,14 +243,10 @@ struct SlabCsc {
                                  // source values; nullptr = uncentered
     int vtype = kValF32;         // type of vals / cval
     bool mu64 = false;           // mu is double
+    BuildScratch build;          // temporaries of build_slab_csc
 };
 
 // val is float (src_f64 == false) or double; bf16 converts float values.
-// scratch holds temporaries of the build (grown as needed).
-struct BuildScratch {
-    DevBuf<int64_t> boff, scan;
-    DevBuf<int32_t> cnt;
-};
 void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
                     const int32_t *row, const void *val, bool src_f64, const void *mu, bool bf16,
                     hipStream_t st);
