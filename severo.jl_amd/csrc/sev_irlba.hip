@@ -2795,7 +2795,7 @@ void batch_layout(BatchLayout &L, const std::vector<int64_t> &cells, hipStream_t
     L.sample_slab0.assign((size_t)L.S + 1, 0);
     L.blk_sample.clear();
     L.blk_r0.clear();
-    constexpr int64_t kBlkRows = 256;  // A v and restart row blocks
+    constexpr int64_t kBlkRows = 128;  // A v and restart row blocks
     for (int64_t s = 0; s < L.S; ++s) {
         L.sample_slab0[(size_t)s] = (int64_t)L.slab_sample.size();
         for (int64_t r = L.row0[(size_t)s]; r < L.row0[(size_t)s + 1]; r += kSlabRows) {
