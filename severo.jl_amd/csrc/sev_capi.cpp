@@ -865,6 +865,8 @@ int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit,
         prm.maxit = maxit;
         prm.tol = tol;
         prm.svtol = tol;
+        // the BF16 form keeps its left Lanczos basis in FP32
+        prm.basis32 = precision == SEV_PREC_BF16;
         p->s.assign((size_t)nu, 0.0);
         p->U.ensure((size_t)(m * nu));
         p->V.ensure((size_t)(n * nu));

@@ -32,6 +32,7 @@
 #include <stdexcept>
 #include <tuple>
 #include <type_traits>
+#include <type_traits>
 #include <vector>
 
 namespace sev {
@@ -396,8 +397,9 @@ k_av_bf16(int64_t m, int64_t n, const int64_t *rowptr, const void *col, const ui
 // --------------------------------------------------------------------------
 constexpr int kDotThreads = 512;
 
+template <class TW>
 __global__ void __launch_bounds__(kDotThreads)
-k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
+k_dot_w(int64_t m, const TW *W, int jc, const double *y, double *part) {
     constexpr int NW = kDotThreads / kWave;
     __shared__ double red[NW][32];
     const int64_t nblk = gridDim.x;
@@ -415,7 +417,8 @@ k_dot_w(int64_t m, const double *W, int jc, const double *y, double *part) {
             // together
             double wv[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) wv[i] = W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
+            for (int i = 0; i < 32; ++i)
+                wv[i] = (double)W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
             __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the FMAs
 #pragma unroll
             for (int i = 0; i < 32; ++i) acc[i] = fma(wv[i], yr, acc[i]);
@@ -455,8 +458,9 @@ __global__ void __launch_bounds__(kBlock) k_red_t(int nblk, const double *part, 
 // --------------------------------------------------------------------------
 constexpr int kApplyThreads = 1024;
 
+template <class TW>
 __global__ void __launch_bounds__(kApplyThreads)
-k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const double *y,
+k_apply(int64_t m, int64_t n, const TW *W, int jc, const double *T, const double *y,
         double *yout, double *part) {
     __shared__ double tl[kMaxWork];
     __shared__ double scratch[kApplyThreads / kWave];
@@ -472,7 +476,7 @@ k_apply(int64_t m, int64_t n, const double *W, int jc, const double *T, const do
         if (jc > 0) {
             double d = 0.0;
 #pragma unroll 8
-            for (int i = 0; i < jc; ++i) d = fma(W[(int64_t)i * m + r], tl[i], d);
+            for (int i = 0; i < jc; ++i) d = fma((double)W[(int64_t)i * m + r], tl[i], d);
             yv -= d;
         }
         yout[r] = yv;
@@ -855,6 +859,19 @@ k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
         dst[r] = src[r] / S;
 }
 
+// The same into a basis stored in FP32 (dst) and an FP64 copy (dst64) that
+// the next product's recurrence term reads.
+__global__ void __launch_bounds__(kBlock)
+k_scale_col32(const double *src, float *dst, double *dst64, int64_t m, const double *scal) {
+    const double S = scal[0];
+    for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * kBlock) {
+        const double v = src[r] / S;
+        dst64[r] = v;
+        dst[r] = (float)v;
+    }
+}
+
 // Y[:, c0 + c] = X[:, 0:work] Cf[:, c0 + c] for c < kout (X, Y column-major,
 // ld = m), computed as the 16x16 tiles of D = Cf' X' on
 // v_mfma_f64_16x16x4_f64: one wavefront per 32 rows (two 16-row tiles sharing
@@ -869,9 +886,9 @@ k_scale_col(const double *src, double *dst, int64_t m, const double *scal) {
 // extra / scal[2] (the normalised residual).
 __host__ __device__ constexpr int gemm_cols(int ns) { return ns <= 31 ? 64 : 48; }
 
-template <int NS>
+template <int NS, class TX = double, class TY = double>
 __global__ void __launch_bounds__(kBlock)
-k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int c0, int kout, double *Y,
+k_gemm_mfma(int64_t m, int work, const TX *X, const double *Cf, int c0, int kout, TY *Y,
             const double *extra, const double *scal) {
     constexpr int LD = 4 * NS + 1;
     constexpr int GC = gemm_cols(NS);
@@ -885,7 +902,7 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int c0, int 
     const int nr = lane & 15, kq = lane >> 4;
     const int ncb = (kout + 15) / 16;
     const int64_t nrb = (m + 31) / 32;
-    double *Yc = Y + (int64_t)c0 * m;
+    TY *Yc = Y + (int64_t)c0 * m;
     for (int64_t rb = (int64_t)blockIdx.x * kWaves + w; rb < nrb;
          rb += (int64_t)gridDim.x * kWaves) {
         const int64_t row0 = rb * 32 + nr, row1 = row0 + 16;
@@ -897,8 +914,8 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int c0, int 
         for (int s = 0; s < NS; ++s) {
             const int k = 4 * s + kq;
             const int64_t kc = (int64_t)(k < work ? k : work - 1) * m;
-            bv0[s] = X[kc + rc0] * ((k < work && row0 < m) ? 1.0 : 0.0);
-            bv1[s] = X[kc + rc1] * ((k < work && row1 < m) ? 1.0 : 0.0);
+            bv0[s] = (double)X[kc + rc0] * ((k < work && row0 < m) ? 1.0 : 0.0);
+            bv1[s] = (double)X[kc + rc1] * ((k < work && row1 < m) ? 1.0 : 0.0);
         }
 #pragma unroll
         for (int cb = 0; cb < GC / 16; ++cb) {
@@ -914,14 +931,14 @@ k_gemm_mfma(int64_t m, int work, const double *X, const double *Cf, int c0, int 
             for (int i = 0; i < 4; ++i) {
                 const int c = cb * 16 + kq + 4 * i;
                 if (c < kout) {
-                    if (row0 < m) Yc[(int64_t)c * m + row0] = acc0[i];
-                    if (row1 < m) Yc[(int64_t)c * m + row1] = acc1[i];
+                    if (row0 < m) Yc[(int64_t)c * m + row0] = (TY)acc0[i];
+                    if (row1 < m) Yc[(int64_t)c * m + row1] = (TY)acc1[i];
                 }
             }
         }
         if (extra && kq == 0) {
-            if (row0 < m) Yc[(int64_t)kout * m + row0] = extra[row0] / scal[2];
-            if (row1 < m) Yc[(int64_t)kout * m + row1] = extra[row1] / scal[2];
+            if (row0 < m) Yc[(int64_t)kout * m + row0] = (TY)(extra[row0] / scal[2]);
+            if (row1 < m) Yc[(int64_t)kout * m + row1] = (TY)(extra[row1] / scal[2]);
         }
     }
 }
@@ -1820,12 +1837,13 @@ void launch_atw_only(const SlabCsc &op, const double *y, double *part, hipStream
 
 // y' = y - W[:,0:jc] T into yout (plus norm/sum partials), then the slab
 // partials of B' y'.
-void launch_upd_atw(const SlabCsc &op, const double *W, int jc, const double *T,
+template <class TW>
+void launch_upd_atw(const SlabCsc &op, const TW *W, int jc, const double *T,
                     const double *y, double *yout, double *part, hipStream_t st) {
     {
-        ProfScope ps("reorth_apply", st, 8.0 * op.m * (jc + 2));
-        hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0, st, op.m,
-                           op.n, W, jc, T, y, yout, part);
+        ProfScope ps("reorth_apply", st, (double)op.m * (sizeof(TW) * jc + 16));
+        hipLaunchKernelGGL(k_apply<TW>, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0, st,
+                           op.m, op.n, W, jc, T, y, yout, part);
         SEV_LAUNCH_CHECK();
     }
     launch_atw_only(op, yout, part, st);
@@ -1899,9 +1917,11 @@ void switch_steps(int nks, F &&f) {
 
 // Y[:, 0:kout] = X[:, 0:work] Cf[:, 0:kout] in launches of at most
 // gemm_cols(NS) columns (extra, when given, follows the last column).
-void launch_gemm(const char *name, int64_t m, int work, const double *X, const double *Cf,
-                 int kout, double *Y, const double *extra, const double *scal, hipStream_t st) {
-    ProfScope ps(name, st, 8.0 * m * (work + kout), 2.0 * m * work * kout);
+template <class TX, class TY>
+void launch_gemm(const char *name, int64_t m, int work, const TX *X, const double *Cf,
+                 int kout, TY *Y, const double *extra, const double *scal, hipStream_t st) {
+    ProfScope ps(name, st, (double)m * (sizeof(TX) * work + sizeof(TY) * kout),
+                 2.0 * m * work * kout);
     const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(ceil_div(m, 32), kWaves),
                                                       (int64_t)device_cu_count() * 8);
     const int nks = (work + 3) / 4;
@@ -1911,8 +1931,8 @@ void launch_gemm(const char *name, int64_t m, int work, const double *X, const d
         for (int c0 = 0; c0 < kout; c0 += GC) {
             const int kc = kout - c0 < GC ? kout - c0 : GC;
             const bool last = c0 + GC >= kout;
-            hipLaunchKernelGGL(k_gemm_mfma<NS>, dim3(grid), dim3(kBlock), 0, st, m, work, X, Cf,
-                               c0, kc, Y, last ? extra : nullptr, scal);
+            hipLaunchKernelGGL((k_gemm_mfma<NS, TX, TY>), dim3(grid), dim3(kBlock), 0, st, m,
+                               work, X, Cf, c0, kc, Y, last ? extra : nullptr, scal);
         }
     };
     switch_steps<kMaxWork / 4>(nks, go);
@@ -1999,9 +2019,14 @@ namespace {
 
 }  // namespace
 
-void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
-                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
-                  const HostOperator *hop) {
+namespace {
+template <class TW>
+void irlba_run(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
+               DevBuf<TW> &Wb, DevBuf<TW> &Wb2, double *s, double *U, double *V, hipStream_t st,
+               IrlbaOut *out, const HostOperator *hop) {
+    // an FP32 basis keeps the current vector and the last normalised one in
+    // FP64 buffers of their own
+    constexpr bool b32 = std::is_same<TW, float>::value;
     const int64_t m = op.m, n = op.n;
     const int64_t nu = p.nu;
     int64_t workl = p.work;
@@ -2013,8 +2038,12 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     SEV_REQUIRE(work <= kMaxWork, "irlba: work (nu + 7) exceeds 128 Lanczos vectors, the most the device path keeps (nu <= 121)");
     const double eps = 2.220446049250313e-16;
 
-    w.W.ensure((size_t)(m * work));
-    w.W2.ensure((size_t)(m * work));
+    Wb.ensure((size_t)(m * work));
+    Wb2.ensure((size_t)(m * work));
+    if (b32) {
+        w.ycur.ensure((size_t)m);
+        w.wprev.ensure((size_t)m);
+    }
     w.V.ensure((size_t)(n * work));
     w.V2.ensure((size_t)(n * work));
     w.B.ensure((size_t)(work * work));
@@ -2063,8 +2092,15 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         // reorthogonalisation (W[:,0:d0]' A v_d0)
         SEV_HIP(hipMemcpyAsync(w.V.get(), p.rV, sizeof(double) * n * d0, hipMemcpyHostToDevice,
                                st));
-        SEV_HIP(hipMemcpyAsync(w.W.get(), p.rU, sizeof(double) * m * d0, hipMemcpyHostToDevice,
-                               st));
+        if constexpr (b32) {
+            const std::vector<float> rUf(p.rU, p.rU + m * d0);
+            SEV_HIP(hipMemcpyAsync(Wb.get(), rUf.data(), sizeof(float) * m * d0,
+                                   hipMemcpyHostToDevice, st));
+            SEV_HIP(hipStreamSynchronize(st));
+        } else {
+            SEV_HIP(hipMemcpyAsync(Wb.get(), p.rU, sizeof(double) * m * d0,
+                                   hipMemcpyHostToDevice, st));
+        }
         std::vector<double> Bd((size_t)work * work, 0.0);
         for (int64_t c = 0; c < d0; ++c) Bd[(size_t)c * work + c] = p.rs[c];
         SEV_HIP(hipMemcpyAsync(w.B.get(), Bd.data(), sizeof(double) * work * work,
@@ -2108,27 +2144,32 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
     while (iter < p.maxit) {
         const int j0 = (iter == 0) ? (int)d0 : (int)k;
         for (int j = j0; j < work; ++j) {
-            double *Wd = w.W.get(), *Vd = w.V.get();
-            double *y = Wd + (int64_t)j * m;
+            TW *Wd = Wb.get();
+            double *Vd = w.V.get();
+            // y = the product; with an FP64 basis it is built in column j
+            double *y = b32 ? w.ycur.get() : (double *)(Wd + (int64_t)j * m);
+            const double *wprev =
+                j > j0 ? (b32 ? w.wprev.get() : (const double *)(Wd + (int64_t)(j - 1) * m))
+                       : nullptr;
             // W[:,j] = A v_j - RF w_{j-1}
             if (hop) {
                 host_mul('N', Vd + (int64_t)j * n, n, y, m);
                 if (j > j0) {
                     hipLaunchKernelGGL(k_sub_rf, dim3(grid_rows(m)), dim3(kBlock), 0, st, y, m,
-                                       w.scal.get() + 1, Wd + (int64_t)(j - 1) * m);
+                                       w.scal.get() + 1, wprev);
                     SEV_LAUNCH_CHECK();
                 }
             } else {
-                launch_av(op, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr,
-                          j > j0 ? Wd + (int64_t)(j - 1) * m : nullptr, y, st);
+                launch_av(op, Vd + (int64_t)j * n, j > j0 ? w.scal.get() + 1 : nullptr, wprev, y,
+                          st);
             }
             nmult++;
             // reorthogonalise against W[:,0:j]
             const int jc = j;
             if (jc > 0) {
-                ProfScope ps("reorth_dot", st, 8.0 * m * (jc + 1));
-                hipLaunchKernelGGL(k_dot_w, dim3(dot_blocks), dim3(kDotThreads), 0, st, m, Wd, jc,
-                                   y, w.red.get());
+                ProfScope ps("reorth_dot", st, (double)m * (sizeof(TW) * jc + 8));
+                hipLaunchKernelGGL(k_dot_w<TW>, dim3(dot_blocks), dim3(kDotThreads), 0, st, m, Wd,
+                                   jc, y, w.red.get());
                 SEV_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_red_t, dim3(jc), dim3(kBlock), 0, st, dot_blocks,
                                    w.red.get(), w.T.get());
@@ -2138,9 +2179,9 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                 SEV_HIP(hipMemcpyAsync(w.B.get() + (int64_t)d0 * work, w.T.get(),
                                        sizeof(double) * d0, hipMemcpyDeviceToDevice, st));
             if (hop) {
-                ProfScope ps("reorth_apply", st, 8.0 * m * (jc + 2));
-                hipLaunchKernelGGL(k_apply, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0, st,
-                                   m, n, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get());
+                ProfScope ps("reorth_apply", st, (double)m * (sizeof(TW) * jc + 16));
+                hipLaunchKernelGGL(k_apply<TW>, dim3((unsigned)op.nslab), dim3(kApplyThreads), 0,
+                                   st, m, n, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get());
                 SEV_LAUNCH_CHECK();
             } else {
                 launch_upd_atw(op, Wd, jc, w.T.get(), y, w.Fv.get() + n, w.part.get(), st);
@@ -2160,9 +2201,16 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                               w.scal.get(), w.flags.get(), (uint64_t)(iter * work + j) + 17u,
                               w.rpart.get(), st);
             {
-                ProfScope ps("scale_col", st, 16.0 * m);
-                hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st,
-                                   w.Fv.get() + n, y, m, w.scal.get());
+                if constexpr (b32) {
+                    ProfScope ps("scale_col", st, 20.0 * m);
+                    hipLaunchKernelGGL(k_scale_col32, dim3(grid_rows(m)), dim3(kBlock), 0, st,
+                                       w.Fv.get() + n, Wd + (int64_t)j * m, w.wprev.get(), m,
+                                       w.scal.get());
+                } else {
+                    ProfScope ps("scale_col", st, 16.0 * m);
+                    hipLaunchKernelGGL(k_scale_col, dim3(grid_rows(m)), dim3(kBlock), 0, st,
+                                       w.Fv.get() + n, y, m, w.scal.get());
+                }
                 SEV_LAUNCH_CHECK();
             }
         }
@@ -2214,12 +2262,12 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
         if (info_h[2]) fail(kErrConvergence, "irlba: zero residual vector at restart");
         // restart: V <- [V BV[:,0:k], F/||F||], W <- W BU[:,0:k]; B was
         // rebuilt by the restart decision
-        launch_gemm("restart_v", n, work, w.V.get(), w.BV.get(), (int)k, w.V2.get(), w.Fv.get(),
-                    w.scal.get(), st);
+        launch_gemm("restart_v", n, work, (const double *)w.V.get(), w.BV.get(), (int)k,
+                    w.V2.get(), (const double *)w.Fv.get(), w.scal.get(), st);
         w.V.swap(w.V2);
-        launch_gemm("restart_w", m, work, w.W.get(), w.BU.get(), (int)k, w.W2.get(), nullptr,
-                    w.scal.get(), st);
-        w.W.swap(w.W2);
+        launch_gemm("restart_w", m, work, Wb.get(), w.BU.get(), (int)k, Wb2.get(),
+                    (const double *)nullptr, w.scal.get(), st);
+        Wb.swap(Wb2);
     }
     if (out) {
         out->iters = iter;
@@ -2236,9 +2284,28 @@ void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, I
                            st));
     SEV_HIP(hipStreamSynchronize(st));
     for (int64_t i = 0; i < nu; ++i) s[i] = BSh[i];
-    launch_gemm("final_u", m, work, w.W.get(), w.BU.get(), (int)nu, U, nullptr, w.scal.get(), st);
-    launch_gemm("final_v", n, work, w.V.get(), w.BV.get(), (int)nu, V, nullptr, w.scal.get(), st);
+    launch_gemm("final_u", m, work, (const TW *)Wb.get(), w.BU.get(), (int)nu, U,
+                (const double *)nullptr, w.scal.get(), st);
+    launch_gemm("final_v", n, work, (const double *)w.V.get(), w.BV.get(), (int)nu, V,
+                (const double *)nullptr, w.scal.get(), st);
     SEV_HIP(hipStreamSynchronize(st));
+}
+}  // namespace
+
+void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
+                  double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
+                  const HostOperator *hop) {
+    if (p.basis32 && !hop) {
+        w.W.release();
+        w.W2.release();
+        irlba_run<float>(op, p, init, w, w.Wf, w.Wf2, s, U, V, st, out, hop);
+    } else {
+        w.Wf.release();
+        w.Wf2.release();
+        w.ycur.release();
+        w.wprev.release();
+        irlba_run<double>(op, p, init, w, w.W, w.W2, s, U, V, st, out, hop);
+    }
 }
 
 void op_apply(const SlabCsc &op, int trans, const double *x, double *y, IrlbaWork &w,
@@ -2251,7 +2318,7 @@ void op_apply(const SlabCsc &op, int trans, const double *x, double *y, IrlbaWor
     w.part.ensure((size_t)(op.nslab * (n + 2)));
     w.part2.ensure((size_t)(kRedChunks * (n + 2)));
     w.Fv.ensure((size_t)(n + op.m));
-    launch_upd_atw(op, nullptr, 0, nullptr, x, w.Fv.get() + n, w.part.get(), st);
+    launch_upd_atw(op, (const double *)nullptr, 0, nullptr, x, w.Fv.get() + n, w.part.get(), st);
     hipLaunchKernelGGL(k_red_slabs, dim3((unsigned)ceil_div(n + 2, kBlock), kRedChunks),
                        dim3(kBlock), 0, st, op.nslab, n + 2, w.part.get(), w.part2.get());
     SEV_LAUNCH_CHECK();

@@ -285,6 +285,9 @@ struct IrlbaParams {
     int64_t nu = 50, work = 57, maxit = 1000;
     double tol = 1e-5, svtol = 1e-5;
     int reorth_passes = 1;  // CGS passes on the W (left) side
+    // Left basis W stored in FP32 (products, dots and the recurrence stay
+    // FP64; the device operator only): halves the reorthogonalisation reads
+    int basis32 = 0;
     // Warm start from `restart` previous singular triplets (host arrays,
     // column-major, leading dimensions m and n): their vectors seed the first
     // columns of the Krylov bases and B starts as diag(rs).
@@ -308,6 +311,8 @@ struct IrlbaOut {
 
 struct IrlbaWork {
     DevBuf<double> W, W2, V, V2, B, BU, BV, BS, svr, T, part, part2, red, Fv, scal, rnd, rpart;
+    DevBuf<float> Wf, Wf2;        // the left basis in FP32 (IrlbaParams::basis32)
+    DevBuf<double> ycur, wprev;  // with Wf: the current product, the last w_j
     DevBuf<int> flags, info;
 };
 
