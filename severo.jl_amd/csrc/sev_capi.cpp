@@ -912,6 +912,7 @@ int sev_pipeline_pca_batch(sev_pipeline *p, int64_t npcs, int64_t work, int64_t 
         prm.maxit = maxit;
         prm.tol = tol;
         prm.svtol = tol;
+        prm.basis32 = 1;  // the BF16 form: left bases in FP32
         p->s.assign((size_t)(S * nu), 0.0);
         p->U.ensure((size_t)(M * nu));
         p->V.ensure((size_t)(S * n * nu));

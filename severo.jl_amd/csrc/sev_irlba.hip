@@ -2368,8 +2368,8 @@ __device__ __forceinline__ bool b_active(const BDev &d, int s, int j) {
 template <int G, bool C12>
 __global__ void __launch_bounds__(kBlock)
 k_b_av_bf16(BDev d, const int64_t *rowptr, const void *col, const uint16_t *val,
-            const float *mu_all, const double *Vall, int j, const double *scal, const double *W,
-            double *y) {
+            const float *mu_all, const double *Vall, int j, const double *scal,
+            const double *wprevp, double *y) {
     extern __shared__ float vf[];
     __shared__ double dmu_s;
     const int s = d.blk_sample[blockIdx.x];
@@ -2389,7 +2389,7 @@ k_b_av_bf16(BDev d, const int64_t *rowptr, const void *col, const uint16_t *val,
     __syncthreads();
     const double dmu = dmu_s;
     const double rfv = first ? 0.0 : scal[4 * s + 1];
-    const double *wprev = first ? nullptr : W + (int64_t)(j - 1) * d.M;
+    const double *wprev = first ? nullptr : wprevp;
     constexpr int RPW = kWave / G;
     const int gl = lane % G, gi = lane / G;
     const int64_t rb = d.blk_r0[blockIdx.x], re = d.blk_r0[blockIdx.x + 1];
@@ -2424,8 +2424,9 @@ k_b_av_bf16(BDev d, const int64_t *rowptr, const void *col, const uint16_t *val,
     }
 }
 
+template <class TW>
 __global__ void __launch_bounds__(kDotThreads)
-k_b_dot_w(BDev d, const double *W, int j, const double *y, double *part) {
+k_b_dot_w(BDev d, const TW *W, int j, const double *y, double *part) {
     constexpr int NW = kDotThreads / kWave;
     __shared__ double red[NW][32];
     const int slab = blockIdx.x;
@@ -2446,7 +2447,8 @@ k_b_dot_w(BDev d, const double *W, int j, const double *y, double *part) {
             // together
             double wv[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) wv[i] = W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
+            for (int i = 0; i < 32; ++i)
+                wv[i] = (double)W[(int64_t)(c0 + i < jc ? c0 + i : jc - 1) * m + r];
             __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the FMAs
 #pragma unroll
             for (int i = 0; i < 32; ++i) acc[i] = fma(wv[i], yr, acc[i]);
@@ -2483,8 +2485,9 @@ k_b_red_t(BDev d, int j, const double *part, double *T) {
     if (threadIdx.x == 0) T[(int64_t)s * kMaxWork + c] = t;
 }
 
+template <class TW>
 __global__ void __launch_bounds__(kApplyThreads)
-k_b_apply(BDev d, const double *W, int j, const double *T, const double *y, double *yout,
+k_b_apply(BDev d, const TW *W, int j, const double *T, const double *y, double *yout,
           double *part) {
     __shared__ double tl[kMaxWork];
     __shared__ double scratch[kApplyThreads / kWave];
@@ -2504,7 +2507,7 @@ k_b_apply(BDev d, const double *W, int j, const double *T, const double *y, doub
         if (jc > 0) {
             double dd = 0.0;
 #pragma unroll 8
-            for (int i = 0; i < jc; ++i) dd = fma(W[(int64_t)i * m + r], tl[i], dd);
+            for (int i = 0; i < jc; ++i) dd = fma((double)W[(int64_t)i * m + r], tl[i], dd);
             yv -= dd;
         }
         yout[r] = yv;
@@ -2594,13 +2597,19 @@ k_b_right_c(BDev d, int j, int nblk, const double *part2, const double *rpart, d
                  flags + 4 * s, (uint64_t)(d.iters[s] * work + j) + 17u);
 }
 
-// W[:, j] = yout / S for the rows of active samples
+// W[:, j] = yout / S for the rows of active samples (and, for a basis in
+// FP32, the FP64 value into w64 for the next product's recurrence term)
+template <class TW>
 __global__ void __launch_bounds__(kBlock)
-k_b_scale_col(BDev d, int j, const double *yout, double *W, const double *scal) {
+k_b_scale_col(BDev d, int j, const double *yout, TW *W, double *w64, const double *scal) {
     for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < d.M;
          r += (int64_t)gridDim.x * kBlock) {
         const int s = d.row_sample[r];
-        if (b_active(d, s, j)) W[(int64_t)j * d.M + r] = yout[r] / scal[4 * s];
+        if (b_active(d, s, j)) {
+            const double v = yout[r] / scal[4 * s];
+            W[(int64_t)j * d.M + r] = (TW)v;
+            if (w64) w64[r] = v;
+        }
     }
 }
 
@@ -2670,8 +2679,9 @@ k_b_restart_check(BDev d, int nu, int maxit, double tol, double svtol, const dou
 // W (ld M, sample from row_sample) or of the per-sample V blocks (n rows,
 // ld n, sample = block).  The extra column kout_s = F_s / ||F_s|| for V.
 // Samples that are done are skipped (restart) or selected by sel (final).
+template <class TX>
 __global__ void __launch_bounds__(kBlock)
-k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
+k_b_gemm(BDev d, int vmode, const TX *X0, const TX *X1, const int *xsel, double *Y,
          int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
          bool restart) {
     const int64_t rows = vmode ? (int64_t)d.S * d.n : d.M;
@@ -2691,8 +2701,8 @@ k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel,
         }
         if (restart && d.done[s]) continue;
         const int kout = kfix > 0 ? kfix : d.j0[s];
-        const double *X = (xsel && xsel[s]) ? X1 : X0;
-        const double *Xs = vmode ? X + (int64_t)s * work * d.n : X;
+        const TX *X = (xsel && xsel[s]) ? X1 : X0;
+        const TX *Xs = vmode ? X + (int64_t)s * work * d.n : X;
         double *Ys = vmode ? Y + (int64_t)s * ldy * d.n : Y;
         const double *Cs = Call + (int64_t)s * work * work;
         for (int c0 = 0; c0 < kout; c0 += 16) {
@@ -2700,7 +2710,7 @@ k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel,
 #pragma unroll
             for (int c = 0; c < 16; ++c) acc[c] = 0.0;
             for (int l = 0; l < work; ++l) {
-                const double x = Xs[(int64_t)l * ld + r];
+                const double x = (double)Xs[(int64_t)l * ld + r];
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
                     if (c0 + c < kout) acc[c] = fma(x, Cs[(int64_t)(c0 + c) * work + l], acc[c]);
@@ -2716,9 +2726,9 @@ k_b_gemm(BDev d, int vmode, const double *X0, const double *X1, const int *xsel,
 // The same on MFMA (the tiles of k_gemm_mfma): W side (vmode 0) one block
 // per A v row block (rows of one sample, ld M); V side (vmode 1) block
 // (x, s) takes rows 256x .. of sample s's n x work block (ld n).
-template <int NS>
+template <int NS, class TX = double>
 __global__ void __launch_bounds__(kBlock)
-k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *xsel, double *Y,
+k_b_gemm_mfma(BDev d, int vmode, const TX *X0, const TX *X1, const int *xsel, TX *Y,
               int64_t ldy, const double *Call, int kfix, const double *Fall, const double *scal,
               bool restart) {
     constexpr int LD = kMaxWorkB + 1;
@@ -2726,8 +2736,8 @@ k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *
     const int work = d.work;
     int s;
     int64_t rb0, re0, ld;
-    const double *X;
-    double *Ys;
+    const TX *X;
+    TX *Ys;
     if (vmode) {
         s = blockIdx.y;
         rb0 = (int64_t)blockIdx.x * 256;
@@ -2768,7 +2778,8 @@ k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *
 #pragma unroll
         for (int st = 0; st < NS; ++st) {
             const int k = 4 * st + kq;
-            bv[st] = X[(int64_t)(k < work ? k : work - 1) * ld + rc] * ((k < work && ok) ? 1.0 : 0.0);
+            bv[st] = (double)X[(int64_t)(k < work ? k : work - 1) * ld + rc] *
+                     ((k < work && ok) ? 1.0 : 0.0);
         }
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
@@ -2783,13 +2794,13 @@ k_b_gemm_mfma(BDev d, int vmode, const double *X0, const double *X1, const int *
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int c = cb * 16 + kq + 4 * i;
-                    if (c < kout) Ys[(int64_t)c * ld + row] = acc[i];
+                    if (c < kout) Ys[(int64_t)c * ld + row] = (TX)acc[i];
                 }
             }
         }
         if (ok) {
             if (vmode && restart && kq == 0)
-                Ys[(int64_t)kout * ld + row] = Fall[(int64_t)s * d.n + row] / scal[4 * s + 2];
+                Ys[(int64_t)kout * ld + row] = (TX)(Fall[(int64_t)s * d.n + row] / scal[4 * s + 2]);
         }
     }
 }
@@ -2923,10 +2934,15 @@ BatchIrlbaWork::~BatchIrlbaWork() {
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
 }
 
-void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
-                 const IrlbaParams &p, const double *init, BatchIrlbaWork &w, double *s_out,
-                 double *U, double *V, int64_t *iters_out, int64_t *nmult_out, bool graph,
-                 hipStream_t st) {
+namespace {
+template <class TW>
+void irlba_batch_run(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
+                     const IrlbaParams &p, const double *init, BatchIrlbaWork &w,
+                     DevBuf<TW> (&Wb)[2], double *s_out, double *U, double *V,
+                     int64_t *iters_out, int64_t *nmult_out, bool graph, hipStream_t st) {
+    // an FP32 basis keeps the current product and the last normalised vector
+    // in FP64 buffers of their own
+    constexpr bool b32 = std::is_same<TW, float>::value;
     SEV_REQUIRE(op.vtype == kValBF16 && op.n <= kMaxFL32,
                 "batched irlba: BF16 operator with at most 14336 features");
     const int64_t S = L.S, M = L.M, n = op.n;
@@ -2941,8 +2957,12 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     SEV_REQUIRE(work <= kMaxWorkB, "batched irlba: work exceeds 64");
     const int64_t nslab = (int64_t)L.slab_sample.size();
     const int nblk_rs = (int)ceil_div(n, kRsBlock);
-    w.W[0].ensure((size_t)(M * work));
-    w.W[1].ensure((size_t)(M * work));
+    Wb[0].ensure((size_t)(M * work));
+    Wb[1].ensure((size_t)(M * work));
+    if (b32) {
+        w.ycur.ensure((size_t)M);
+        w.wprev.ensure((size_t)M);
+    }
     w.V[0].ensure((size_t)(S * n * work));
     w.V[1].ensure((size_t)(S * n * work));
     w.B.ensure((size_t)(S * work * work));
@@ -3023,12 +3043,13 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     // recorded graphs hold every pointer and size below: they are replayed
     // only while all of them are unchanged
     {
-        const double *bufs[] = {w.W[0].get(), w.W[1].get(), w.V[0].get(), w.V[1].get(),
-                                w.B.get(), w.BU.get(), w.BV.get(), w.BS.get(), w.svr.get(),
-                                w.T.get(), w.part.get(), w.part2.get(), w.red.get(), w.Fy.get(),
-                                w.F.get(), w.scal.get(), w.rpart.get()};
+        const void *bufs[] = {Wb[0].get(), Wb[1].get(), w.ycur.get(), w.wprev.get(),
+                              w.V[0].get(), w.V[1].get(),
+                              w.B.get(), w.BU.get(), w.BV.get(), w.BS.get(), w.svr.get(),
+                              w.T.get(), w.part.get(), w.part2.get(), w.red.get(), w.Fy.get(),
+                              w.F.get(), w.scal.get(), w.rpart.get()};
         std::vector<uint64_t> sig;
-        for (const double *b : bufs) sig.push_back((uint64_t)(uintptr_t)b);
+        for (const void *b : bufs) sig.push_back((uint64_t)(uintptr_t)b);
         const void *ptrs[] = {w.flags.get(), w.info.get(), w.state.get(), mu_all,
                               op.rowptr.get(), op.ccol.get(), op.cval.get(), op.off.get(),
                               op.lrow.get(), op.vals, L.row_sample_d.get(), L.slab_row0_d.get(),
@@ -3050,10 +3071,14 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     }
 
     auto round = [&](int jmin, int cur) {
-        double *Wc = w.W[cur].get(), *Wn = w.W[cur ^ 1].get();
+        TW *Wc = Wb[cur].get(), *Wn = Wb[cur ^ 1].get();
         double *Vc = w.V[cur].get(), *Vn = w.V[cur ^ 1].get();
         for (int j = jmin; j < work; ++j) {
-            double *y = Wc + (int64_t)j * M;
+            // the product; with an FP64 basis it is built in column j
+            double *y = b32 ? w.ycur.get() : (double *)(Wc + (int64_t)j * M);
+            const double *wp =
+                j > 0 ? (b32 ? w.wprev.get() : (const double *)(Wc + (int64_t)(j - 1) * M))
+                      : nullptr;
             {
                 const void *colp = op.col12 ? (const void *)op.ccol12.get()
                                             : (const void *)op.ccol.get();
@@ -3064,11 +3089,11 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
         if (op.col12)                                                                      \
             hipLaunchKernelGGL((k_b_av_bf16<GG, true>), gb, dim3(kBlock), lds, st, d,      \
                                op.rowptr.get(), colp, (const uint16_t *)op.cval.get(),     \
-                               mu_all, Vc, j, w.scal.get(), Wc, y);                        \
+                               mu_all, Vc, j, w.scal.get(), wp, y);                        \
         else                                                                               \
             hipLaunchKernelGGL((k_b_av_bf16<GG, false>), gb, dim3(kBlock), lds, st, d,     \
                                op.rowptr.get(), colp, (const uint16_t *)op.cval.get(),     \
-                               mu_all, Vc, j, w.scal.get(), Wc, y);                        \
+                               mu_all, Vc, j, w.scal.get(), wp, y);                        \
     } while (0)
                 switch (G) {
                     case 4: SEV_B_AV(4); break;
@@ -3080,15 +3105,15 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
             }
             SEV_LAUNCH_CHECK();
             if (j > 0) {
-                hipLaunchKernelGGL(k_b_dot_w, dim3((unsigned)nslab), dim3(kDotThreads), 0, st, d, Wc,
-                                   j, y, w.red.get());
+                hipLaunchKernelGGL(k_b_dot_w<TW>, dim3((unsigned)nslab), dim3(kDotThreads), 0, st,
+                                   d, (const TW *)Wc, j, y, w.red.get());
                 SEV_LAUNCH_CHECK();
                 hipLaunchKernelGGL(k_b_red_t, dim3(j, (unsigned)S), dim3(kBlock), 0, st, d, j,
                                    w.red.get(), w.T.get());
                 SEV_LAUNCH_CHECK();
             }
-            hipLaunchKernelGGL(k_b_apply, dim3((unsigned)nslab), dim3(kApplyThreads), 0, st, d, Wc,
-                               j, w.T.get(), y, w.Fy.get(), w.part.get());
+            hipLaunchKernelGGL(k_b_apply<TW>, dim3((unsigned)nslab), dim3(kApplyThreads), 0, st, d,
+                               (const TW *)Wc, j, w.T.get(), y, w.Fy.get(), w.part.get());
             SEV_LAUNCH_CHECK();
             hipLaunchKernelGGL(k_b_atw_bf16, dim3((unsigned)nslab, ng), dim3(kBlock), 0, st, d, j,
                                op.off.get(), op.lrow12.get(), (const uint16_t *)op.vals, w.Fy.get(),
@@ -3108,8 +3133,8 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
                                w.part2.get(), w.rpart.get(), Vc, w.F.get(), w.B.get(),
                                w.scal.get(), w.flags.get());
             SEV_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_b_scale_col, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, j,
-                               w.Fy.get(), Wc, w.scal.get());
+            hipLaunchKernelGGL(k_b_scale_col<TW>, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, j,
+                               w.Fy.get(), Wc, b32 ? w.wprev.get() : nullptr, w.scal.get());
             SEV_LAUNCH_CHECK();
         }
         if (work <= kSvd8MaxWork)
@@ -3127,12 +3152,13 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
         switch_steps<kMaxWorkB / 4>((work + 3) / 4, [&](auto ns) {
             constexpr int NS = decltype(ns)::value;
             hipLaunchKernelGGL(k_b_gemm_mfma<NS>, dim3((unsigned)ceil_div(n, 256), (unsigned)S),
-                               dim3(kBlock), 0, st, d, 1, Vc, Vc, (const int *)nullptr, Vn,
+                               dim3(kBlock), 0, st, d, 1, (const double *)Vc, (const double *)Vc,
+                               (const int *)nullptr, Vn,
                                (int64_t)work, w.BV.get(), 0, w.F.get(), w.scal.get(), true);
             SEV_LAUNCH_CHECK();
-            hipLaunchKernelGGL(k_b_gemm_mfma<NS>, dim3((unsigned)nblk_av), dim3(kBlock), 0, st, d,
-                               0, Wc, Wc, (const int *)nullptr, Wn, M, w.BU.get(), 0,
-                               (const double *)nullptr, w.scal.get(), true);
+            hipLaunchKernelGGL((k_b_gemm_mfma<NS, TW>), dim3((unsigned)nblk_av), dim3(kBlock), 0,
+                               st, d, 0, (const TW *)Wc, (const TW *)Wc, (const int *)nullptr, Wn,
+                               M, w.BU.get(), 0, (const double *)nullptr, w.scal.get(), true);
             SEV_LAUNCH_CHECK();
         });
     };
@@ -3211,12 +3237,12 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     SEV_HIP(hipMemcpyAsync(sel.get(), fin_buf.data(), sizeof(int) * S, hipMemcpyHostToDevice, st));
     {
         ProfScope ps("batch_final", st);
-        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, 0,
-                           (const double *)w.W[0].get(), (const double *)w.W[1].get(),
+        hipLaunchKernelGGL(k_b_gemm<TW>, dim3(grid_rows(M)), dim3(kBlock), 0, st, d, 0,
+                           (const TW *)Wb[0].get(), (const TW *)Wb[1].get(),
                            (const int *)sel.get(), U, M, w.BU.get(), nu, (const double *)nullptr,
                            w.scal.get(), false);
         SEV_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_b_gemm, dim3(grid_rows(S * n)), dim3(kBlock), 0, st, d, 1,
+        hipLaunchKernelGGL(k_b_gemm<double>, dim3(grid_rows(S * n)), dim3(kBlock), 0, st, d, 1,
                            (const double *)w.V[0].get(), (const double *)w.V[1].get(),
                            (const int *)sel.get(), V, (int64_t)nu, w.BV.get(), nu,
                            (const double *)nullptr, w.scal.get(), false);
@@ -3228,6 +3254,26 @@ void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
     SEV_HIP(hipStreamSynchronize(st));
     for (int64_t s = 0; s < S; ++s)
         for (int i = 0; i < nu; ++i) s_out[s * nu + i] = bs[(size_t)(s * kMaxWork + i)];
+}
+}  // namespace
+
+void irlba_batch(const SlabCsc &op, const BatchLayout &L, const float *mu_all,
+                 const IrlbaParams &p, const double *init, BatchIrlbaWork &w, double *s_out,
+                 double *U, double *V, int64_t *iters_out, int64_t *nmult_out, bool graph,
+                 hipStream_t st) {
+    if (p.basis32) {
+        w.W[0].release();
+        w.W[1].release();
+        irlba_batch_run<float>(op, L, mu_all, p, init, w, w.Wf, s_out, U, V, iters_out,
+                               nmult_out, graph, st);
+    } else {
+        w.Wf[0].release();
+        w.Wf[1].release();
+        w.ycur.release();
+        w.wprev.release();
+        irlba_batch_run<double>(op, L, mu_all, p, init, w, w.W, s_out, U, V, iters_out,
+                                nmult_out, graph, st);
+    }
 }
 
 void batch_embedding(const double *U, const double *s_dev, const BatchLayout &L, int64_t dd,

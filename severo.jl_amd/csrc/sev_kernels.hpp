@@ -354,6 +354,8 @@ void batch_slabs(SlabCsc &op, const BatchLayout &L, const int64_t *colptr, const
 
 struct BatchIrlbaWork {
     DevBuf<double> W[2], V[2], B, BU, BV, BS, svr, T, part, part2, red, Fy, F, scal, rpart;
+    DevBuf<float> Wf[2];         // the left bases in FP32 (IrlbaParams::basis32)
+    DevBuf<double> ycur, wprev;  // with Wf: the current product, the last w_j
     DevBuf<int> flags, info, state;  // state: j0, done, iters, fail per sample
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (first step, buffer parity)
     std::vector<uint64_t> sig;  // pointers and sizes the graphs were recorded with
