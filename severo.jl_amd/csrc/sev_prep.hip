@@ -262,14 +262,27 @@ __device__ __forceinline__ double lane_value(double x, int t) {
 }
 
 // delta / T(count) rounded to T (the reference divides by count converted
-// to T).  For Float32 the quotient is delta * (1/F), F = Float32(count),
-// evaluated in FP64 and rounded once to FP32: the FP64 value is within 2^-52
-// (relative) of delta/F, while delta/F is at least 2^-49 (relative) from
-// every FP32 rounding boundary -- it cannot be a tie, since an odd 25-bit
-// significand times F's significand is no 24-bit float -- so the rounding is
-// the IEEE FP32 division's.  Subnormal deltas divide directly.
+// to T): the FP64 form divides.
 __device__ __forceinline__ double welford_quot(double delta, double, double countf, bool) {
     return delta / countf;
+}
+
+// The FP32 quotient RN(delta/F), F = Float32(count), by Markstein's
+// correction step (P. Markstein, IBM J. Res. Dev. 34(1), 1990): with
+// y = RN(1/F) and q = RN(delta*y) within one ulp of delta/F, r = delta - q*F
+// is exact (one fma) and RN(q + r*y) is RN(delta/F), the IEEE FP32
+// division's result.  y = RN32(RN64(1/F)) is RN32(1/F): for an
+// integer-valued F, 1/F is at least 2^-49 (relative) from every FP32
+// rounding boundary, so the second rounding cannot move it.  Every operation
+// of the recurrence's chain is FP32; deltas below 2^-90 in magnitude, whose
+// products could leave the normal range, take the exact division instead.
+__device__ __forceinline__ float welford_quot32(float delta, float F, float y) {
+    const float q = delta * y;
+    const float r = fmaf(-q, F, delta);
+    return fmaf(r, y, q);
+}
+__device__ __forceinline__ bool welford_tiny(float delta) {
+    return fabsf(delta) < 0x1p-90f && delta != 0.0f;
 }
 
 template <class T>
@@ -288,10 +301,18 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
     int64_t count = m - (e - b);
     const bool fast = sizeof(T) == 4;
     T mu = 0, s = 0;
+    // the next chunk's values are loaded while the recurrence runs over the
+    // current one (indices clamped into the column, so every load is issued
+    // and only the current chunk's is waited for; lanes past the end read 0)
+    T vraw = b < e ? x[b + lane < e ? b + lane : e - 1] : T(0);
     for (int64_t base = b; base < e; base += kWave) {
-        const int64_t p = base + lane;
-        const T vl = p < e ? x[p] : T(0);
-        const double rcl = fast ? 1.0 / (double)(float)(count + lane + 1) : 0.0;
+        const int64_t pn = base + kWave + lane;
+        const T vnext = x[pn < e ? pn : e - 1];
+        const T vl = base + lane < e ? vraw : T(0);
+        vraw = vnext;
+        // lane l: F = Float32(count + l + 1) and RN32(1/F) for element l of the chunk
+        const float fl = fast ? (float)(count + lane + 1) : 0.0f;
+        const float yl = fast ? (float)(1.0 / (double)fl) : 0.0f;
         const int nt = (int)((e - base) < kWave ? (e - base) : kWave);
         if constexpr (sizeof(T) == 4) {
             // every lane holds the same mu and delta, so the rare exact
@@ -300,10 +321,8 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
             auto step = [&](int t) {
                 const float v = lane_value(vl, t);
                 const float delta = v - mu;
-                float q = (float)((double)delta * lane_value(rcl, t));
-                if (__builtin_expect(__builtin_amdgcn_readfirstlane(
-                                         (int)(fabsf(delta) < 0x1p-100f && delta != 0.0f)),
-                                     0))
+                float q = welford_quot32(delta, lane_value(fl, t), lane_value(yl, t));
+                if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)welford_tiny(delta)), 0))
                     q = delta / (float)(count + t + 1);
                 mu = mu + q;
                 s = s + delta * (v - mu);
@@ -318,8 +337,8 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
                 for (int t = 0; t < kWave; ++t) {
                     const float v = lane_value(vl, t);
                     const float delta = v - mu;
-                    tiny = tiny || (fabsf(delta) < 0x1p-100f && delta != 0.0f);
-                    mu = mu + (float)((double)delta * lane_value(rcl, t));
+                    tiny = tiny || welford_tiny(delta);
+                    mu = mu + welford_quot32(delta, lane_value(fl, t), lane_value(yl, t));
                     s = s + delta * (v - mu);
                 }
                 if (__builtin_expect(__builtin_amdgcn_readfirstlane((int)tiny), 0)) {
@@ -356,8 +375,8 @@ k_scale_stats(int64_t m, int64_t ncol, const int64_t *colptr, const T *x, T scal
 // The same statistics for many short columns (the stacked batch, column
 // (j, s) holding feature j of sample s = col % S): one lane per column, each
 // lane running its column's recurrence in the same operations and order as
-// k_scale_stats (delta * RN64(1/F) rounded to Float32, exact division for
-// the tiny deltas), so the results are the same bits.
+// k_scale_stats (Markstein's FP32 quotient, exact division for the tiny
+// deltas), so the results are the same bits.
 __global__ void __launch_bounds__(kBlock)
 k_scale_stats_lane(int64_t ncol, const int64_t *colptr, const float *x, float scale_max,
                    float *sdv, float *mupv, float *clipv, const int64_t *row0, int64_t S) {
@@ -370,24 +389,33 @@ k_scale_stats_lane(int64_t ncol, const int64_t *colptr, const float *x, float sc
     float mu = 0.0f, s = 0.0f;
     auto step = [&](float v) {
         ++count;
+        const float F = (float)count;
         const float delta = v - mu;
         float q;
-        if (fabsf(delta) < 0x1p-100f && delta != 0.0f)
-            q = delta / (float)count;
+        if (welford_tiny(delta))
+            q = delta / F;
         else
-            q = (float)((double)delta * (1.0 / (double)(float)count));
+            q = welford_quot32(delta, F, (float)(1.0 / (double)F));
         mu = mu + q;
         s = s + delta * (v - mu);
     };
-    // 16 loads of the lane's column in flight ahead of its recurrence
+    // groups of 16 loads of the lane's column: the next group is in flight
+    // while the recurrence runs over the current one
     constexpr int U = 16;
     int64_t p = b;
-    for (; p + U <= e; p += U) {
-        float v[U];
+    float v[U], vn[U];
+    if (p + U <= e) {
 #pragma unroll
         for (int u = 0; u < U; ++u) v[u] = x[p + u];
+    }
+    for (; p + U <= e; p += U) {
+        // the following group, indices clamped into the column
+#pragma unroll
+        for (int u = 0; u < U; ++u) vn[u] = x[p + U + u < e ? p + U + u : e - 1];
 #pragma unroll
         for (int u = 0; u < U; ++u) step(v[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = vn[u];
     }
     for (; p < e; ++p) step(x[p]);
     const float var = s / (float)(m - 1);

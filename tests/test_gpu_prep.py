@@ -147,6 +147,34 @@ def test_scale_features_vs_oracle(sev, orc, dt):
     assert np.array_equal(S.A.data, out) and np.array_equal(S.mu, mu)
 
 
+def test_scale_stats_extreme_values(sev, orc):
+    """Welford statistics in FP32 on values that stress the quotient: deltas
+    below 2^-90 and subnormal ones (the exact-division branch), large
+    magnitudes, mixed signs, repeated values and columns of 300,000 stored
+    entries (full chunks, counts past 2^16), bit for bit against the
+    oracle's FP32 division.  Every column's variance stays a normal float."""
+    rng = np.random.default_rng(11)
+    m = 300_000
+    h = m // 2
+    g = rng.gamma(2.0, 1.0, m).astype(np.float32)
+    c0 = rng.standard_normal(m).astype(np.float32)
+    c1 = g.copy()
+    c1[:h] = (np.float32(1e-30) * (1 + rng.integers(0, 4, h))).astype(np.float32)
+    c2 = np.where(rng.random(m) < 0.05, g, np.float32(0))
+    c2[:1000] = (np.float32(1e-40) * rng.integers(1, 6, 1000) *
+                 np.where(rng.random(1000) < 0.5, -1, 1)).astype(np.float32)
+    c3 = (rng.standard_normal(m) * 1e15).astype(np.float32)
+    c4 = np.where(rng.random(m) < 0.02, np.float32(2.5), np.float32(0))
+    c5 = np.where(rng.random(m) < 0.5, np.float32(7.0), np.float32(-7.0))
+    X = sp.csc_matrix(np.stack([c0, c1, c2, c3, c4, c5], axis=1))
+    assert X.dtype == np.float32 and X.indptr[-1] > 4 * m
+    S = sev.scale_features(X, scale_max=np.inf)
+    out, mu, _ = orc.scale(m, X.indptr, X.data, np.float32(np.inf))
+    assert np.all(np.isfinite(mu))
+    assert np.array_equal(S.mu, mu)
+    assert np.array_equal(S.A.data, out)
+
+
 def test_large_load_roundtrip_and_checks(sev):
     """Counts above the pinned-staging threshold (rows 160 MB as Int64,
     values 80 MB) come back unchanged; a row out of range or a column out
