@@ -1336,33 +1336,37 @@ k_small_svd8(int n, const double *B, double *U, double *s, double *V, int *statu
     small_svd8_body(n, B, U, s, V, status);
 }
 
-// The same SVD for 64 < work <= kMaxWork: 64 groups of 16 lanes (one column
-// pair each, circle schedule), lane l of a group holding rows l + 16e (e < 8)
-// of its two columns; only G = B' sits in LDS (n x n FP64), so U = B V
-// diag(1/s) reads B from global memory.  When B is too ill-conditioned for
-// that product (s_min < 1e-5 s_max) the kernel sets status[0] and the caller
-// redoes the SVD on the host (the accumulated rotations would need a second
-// n x n array).
-constexpr int kSvdBigThreads = 1024;
+// The same SVD for 64 < work <= kMaxWork with 8 lanes per column pair (64
+// groups in eight wavefronts, circle schedule): lane l of a group holds rows
+// 16k + 2l + h (k < 8, h < 2) of its two columns, read and written as
+// 16-byte pairs; only G = B' sits in LDS (columns of kSvdLdB doubles, rows
+// n.. held at zero), so U = B V diag(1/s) reads B from global memory.  When
+// B is too ill-conditioned for that product (s_min < 1e-5 s_max) the kernel
+// sets status[0] and the caller redoes the SVD on the host (the accumulated
+// rotations would need a second n x n array).  Per round the per-pair
+// reductions and rotation formulas, repeated on every lane of a group, are
+// half those of 16 lanes per pair.
+constexpr int kSvdBigThreads = 512;
+constexpr int kSvdLdB = 130;  // column stride (doubles): 128 rows + 16 bytes
 
 __global__ void __launch_bounds__(kSvdBigThreads)
 k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *status) {
-    extern __shared__ double svdb_lds[];
-    double *G = svdb_lds;  // G(i, c) = B(c, i) at G[c * n + i]
+    extern __shared__ double2 svdb_lds[];
+    double *G = (double *)svdb_lds;  // G(i, c) = B(c, i) at G[c * kSvdLdB + i]
     __shared__ int rotated, nsweeps, ill;
     __shared__ double sv[kMaxWork];
     const int tid = threadIdx.x;
     const int ne = n + (n & 1);
     const int npair = ne / 2;
-    const int pr = tid >> 4, gl = tid & 15;
+    const int pr = tid >> 3, gl = tid & 7;
     const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
     if (tid == 0) {
         nsweeps = 0;
         ill = 0;
     }
-    for (int e = tid; e < n * n; e += kSvdBigThreads) {
-        const int i = e % n, c = e / n;
-        G[e] = B[(int64_t)i * n + c];
+    for (int e = tid; e < ne * kSvdLdB; e += kSvdBigThreads) {
+        const int c = e / kSvdLdB, i = e % kSvdLdB;
+        G[e] = (c < n && i < n) ? B[(int64_t)i * n + c] : 0.0;
     }
     __syncthreads();
     for (int sweep = 0; sweep < 100; ++sweep) {
@@ -1375,22 +1379,27 @@ k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *st
             if (pr < npair) {
                 const int p = pp < qq ? pp : qq, q = pp < qq ? qq : pp;
                 const bool valid = q < n;
-                double *gp = G + p * n, *gq = G + (valid ? q : p) * n;
-                double x[8], y[8];
+                double2 *gp = (double2 *)(G + p * kSvdLdB) + gl;
+                double2 *gq = (double2 *)(G + q * kSvdLdB) + gl;
+                double2 x[8], y[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    x[k] = gp[8 * k];
+                    y[k] = gq[8 * k];
+                }
                 double app = 0.0, aqq = 0.0, apq = 0.0;
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = gl + 16 * e;
-                    const bool in = valid && i < n;
-                    x[e] = in ? gp[i] : 0.0;
-                    y[e] = in ? gq[i] : 0.0;
-                    app = fma(x[e], x[e], app);
-                    aqq = fma(y[e], y[e], aqq);
-                    apq = fma(x[e], y[e], apq);
+                for (int k = 0; k < 8; ++k) {
+                    app = fma(x[k].x, x[k].x, app);
+                    aqq = fma(y[k].x, y[k].x, aqq);
+                    apq = fma(x[k].x, y[k].x, apq);
+                    app = fma(x[k].y, x[k].y, app);
+                    aqq = fma(y[k].y, y[k].y, aqq);
+                    apq = fma(x[k].y, y[k].y, apq);
                 }
-                app = sum16(app);
-                aqq = sum16(aqq);
-                apq = sum16(apq);
+                app = sum8(app);
+                aqq = sum8(aqq);
+                apq = sum8(apq);
                 if (valid && apq != 0.0 && apq * apq > tol2 * app * aqq) {
                     if (gl == 0) rotated = 1;
                     // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2)
@@ -1406,12 +1415,9 @@ k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *st
                     const double c = rsq_nr(fma(t, t, 1.0));
                     const double sn = c * t;
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const int i = gl + 16 * e;
-                        if (i < n) {
-                            gp[i] = c * x[e] - sn * y[e];
-                            gq[i] = sn * x[e] + c * y[e];
-                        }
+                    for (int k = 0; k < 8; ++k) {
+                        gp[8 * k] = make_double2(c * x[k].x - sn * y[k].x, c * x[k].y - sn * y[k].y);
+                        gq[8 * k] = make_double2(sn * x[k].x + c * y[k].x, sn * x[k].y + c * y[k].y);
                     }
                 }
             }
@@ -1426,7 +1432,7 @@ k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *st
     }
     if (tid < n) {
         double t = 0.0;
-        for (int i = 0; i < n; ++i) t += G[tid * n + i] * G[tid * n + i];
+        for (int i = 0; i < n; ++i) t += G[tid * kSvdLdB + i] * G[tid * kSvdLdB + i];
         sv[tid] = sqrt(t);
     }
     __syncthreads();
@@ -1442,18 +1448,45 @@ k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *st
     }
     __syncthreads();
     if (ill) return;
-    for (int j = tid >> 6; j < n; j += kSvdBigThreads / 64) {
-        const int l = tid & 63;
-        const double sj = sv[j];
-        int rk = 0;
-        for (int i = 0; i < n; ++i) rk += (sv[i] > sj || (sv[i] == sj && i < j)) ? 1 : 0;
-        if (l == 0) s[rk] = sj;
-        for (int i = l; i < n; i += 64) {
-            // (B g_j)_i / s_j^2, g_j = s_j v_j the column of G
-            double t = 0.0;
-            for (int c = 0; c < n; ++c) t = fma(B[(int64_t)c * n + i], G[j * n + c], t);
-            U[(int64_t)rk * n + i] = t / (sj * sj);
-            V[(int64_t)rk * n + i] = G[j * n + i] / sj;
+    // U[:, rank(j)] = B g_j / s_j^2 (g_j = s_j v_j, the column of G) for the
+    // columns j = w + 8q of wavefront w: lane l holds rows l and l + 64 of B,
+    // loaded in blocks of 16 columns with every load in flight, and applies
+    // each block to all the wavefront's j (the sums run over c in order)
+    constexpr int kJw = kMaxWork / (kSvdBigThreads / 64);
+    const int w = tid >> 6, l = tid & 63;
+    for (int ih = 0; ih * 64 < n; ++ih) {
+        const int i = l + 64 * ih;
+        double acc[kJw];
+#pragma unroll
+        for (int q = 0; q < kJw; ++q) acc[q] = 0.0;
+        for (int c0 = 0; c0 < n; c0 += 16) {
+            double bv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+                bv[u] = (c0 + u < n && i < n) ? B[(int64_t)(c0 + u) * n + i] : 0.0;
+#pragma unroll
+            for (int q = 0; q < kJw; ++q) {
+                const int j = w + 8 * q;
+                if (j < n) {
+                    // rows n.. of G are zero
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        acc[q] = fma(bv[u], G[j * kSvdLdB + c0 + u], acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kJw; ++q) {
+            const int j = w + 8 * q;
+            if (j >= n) break;
+            const double sj = sv[j];
+            int rk = 0;
+            for (int r = 0; r < n; ++r) rk += (sv[r] > sj || (sv[r] == sj && r < j)) ? 1 : 0;
+            if (ih == 0 && l == 0) s[rk] = sj;
+            if (i < n) {
+                U[(int64_t)rk * n + i] = acc[q] / (sj * sj);
+                V[(int64_t)rk * n + i] = G[j * kSvdLdB + i] / sj;
+            }
         }
     }
 }
@@ -1463,7 +1496,7 @@ size_t svd_lds_bytes(int work) {
     const size_t ne = (size_t)(work + (work & 1));
     if (work <= kSvd8MaxWork) return sizeof(double) * 2 * ne * kSvdLd;
     if (work <= 64) return sizeof(double) * 2 * (size_t)work * (size_t)work;
-    return sizeof(double) * (size_t)work * (size_t)work;
+    return sizeof(double) * ne * kSvdLdB;
 }
 
 // The projected SVD of the single-operator IRLBA for any work <= kMaxWork.
@@ -1481,7 +1514,7 @@ void launch_small_svd(int work, const double *B, double *BU, double *BS, double 
         if (!attr) {
             SEV_HIP(hipFuncSetAttribute((const void *)k_small_svd_big,
                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)(sizeof(double) * kMaxWork * kMaxWork)));
+                                        (int)(sizeof(double) * kMaxWork * kSvdLdB)));
             attr = true;
         }
         hipLaunchKernelGGL(k_small_svd_big, dim3(1), dim3(kSvdBigThreads), lds, st, work, B, BU,
