@@ -369,12 +369,16 @@ int sev_pipeline_get_scaled(sev_pipeline *p, int64_t *colptr, int64_t *rowval, f
  * (src/embedding.jl:46-94): IRLBA, then Z = U diag(s) kept on the device in
  * Float32 for the neighbour search.  work is the number of Lanczos vectors:
  * 0 takes the reference's npcs + 7 (src/irlba.jl:50); a larger basis (at
- * most 128) converges to the same tolerance in fewer restarts. */
+ * most 128) converges to the same tolerance in fewer restarts.  precision
+ * SEV_PREC_BF16 reads the operator as BF16 and stores the left Lanczos
+ * basis (cells x work) in FP32; every dot, norm and the recurrence stay
+ * FP64.  SEV_PREC_FP32 keeps the FP32 operator and an FP64 basis. */
 int sev_pipeline_pca(sev_pipeline *p, int64_t npcs, int64_t work, int64_t maxit, double tol,
                      const double *init, int precision, int64_t *iters, int64_t *nmult);
 /* Batch form: one IRLBA per sample (src/irlba.jl:47-85 on the sample's
  * centered matrix), all run in lockstep over the Lanczos steps with BF16
- * operator values and FP64 vectors; init is S x nfeat (row per sample);
+ * operator values, left bases stored in FP32 and FP64 arithmetic; init is
+ * S x nfeat (row per sample);
  * iters[S].  use_graph != 0 records each restart round once as a HIP graph
  * and replays it.  npcs is capped by the smallest sample.  work: Lanczos
  * vectors per sample, 0 for the reference's npcs + 7, else npcs + 1 .. 64
