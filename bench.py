@@ -231,7 +231,8 @@ def batch_leg(sev, steps, warmup, graph=True):
     log-uniform in [500, 50000], densities uniform in [2%, 8%], seed per
     sample = 1000 + s; per-sample normalise, 2000 features (each sample's
     top generator means), IRLBA 30 components with 64 Lanczos vectors per
-    sample (BF16 operator values, FP64 vectors), kNN k = 15, SNN prune 1/15.
+    sample (BF16 operator values, FP32 left bases, FP64 arithmetic), kNN k = 15,
+    SNN prune 1/15.
     The batch runs all samples in one
     stacked launch sequence (restart rounds replayed from HIP graphs); the
     comparison loops the single-sample pipeline over the samples."""

@@ -165,8 +165,8 @@ class Pipeline:
         self.nfeat = f.shape[1]
 
     def pca_batch(self, npcs=30, init=None, tol=1e-5, maxit=1000, graph=True, rng=None, work=0):
-        """One IRLBA per sample (BF16 operator values, FP64 vectors), in
-        lockstep; returns (iterations per sample, total products).  work:
+        """One IRLBA per sample (BF16 operator values, FP32 left bases, FP64
+        arithmetic), in lockstep; returns (iterations per sample, total products).  work:
         Lanczos vectors per sample (0: npcs + 7, the reference's choice; at
         most 64, capped by the smallest sample)."""
         S = len(self.cells)
