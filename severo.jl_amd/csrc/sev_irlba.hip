@@ -3019,17 +3019,23 @@ void irlba_batch_run(const SlabCsc &op, const BatchLayout &L, const float *mu_al
                 "batched irlba: projected matrix too large for the device SVD");
 
     // starting vectors and state
-    std::vector<double> v0((size_t)(S * n * work), 0.0);
+    // column 0 of each sample's V block (stride work * n); the other
+    // columns start at zero
+    std::vector<double> v0((size_t)(S * n));
     for (int64_t s = 0; s < S; ++s) {
         const double *in = init + s * n;
         double nr = 0.0;
         for (int64_t i = 0; i < n; ++i) nr += in[i] * in[i];
         nr = std::sqrt(nr);
         if (nr < 4.5e-16) fail(kErrConvergence, "batched irlba: a starting vector is zero");
-        for (int64_t i = 0; i < n; ++i) v0[(size_t)(s * work * n + i)] = in[i] / nr;
+        for (int64_t i = 0; i < n; ++i) v0[(size_t)(s * n + i)] = in[i] / nr;
     }
-    SEV_HIP(hipMemcpyAsync(w.V[0].get(), v0.data(), sizeof(double) * v0.size(),
-                           hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_rows(S * n * work)), dim3(kBlock), 0, st,
+                       w.V[0].get(), S * n * work, 0.0);
+    SEV_LAUNCH_CHECK();
+    SEV_HIP(hipMemcpy2DAsync(w.V[0].get(), sizeof(double) * n * work, v0.data(),
+                             sizeof(double) * n, sizeof(double) * n, (size_t)S,
+                             hipMemcpyHostToDevice, st));
     std::vector<int> st_h((size_t)(4 * S), 0);  // j0 | done | iters | fail
     std::vector<int> info_h((size_t)(16 * S), 0);
     for (int64_t s = 0; s < S; ++s) info_h[(size_t)(16 * s + 1)] = nu;
