@@ -87,6 +87,30 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host buffer (hipHostMalloc), for small copies that must
+// not go through the pageable staging path.
+template <class T>
+struct HostPinned {
+    T *ptr = nullptr;
+    size_t count = 0;
+    HostPinned() = default;
+    HostPinned(const HostPinned &) = delete;
+    HostPinned &operator=(const HostPinned &) = delete;
+    ~HostPinned() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    T *ensure(size_t n) {
+        if (n > count || ptr == nullptr) {
+            if (ptr) (void)hipHostFree(ptr);
+            ptr = nullptr;
+            count = 0;
+            SEV_HIP(hipHostMalloc((void **)&ptr, (n > 0 ? n : 1) * sizeof(T), hipHostMallocDefault));
+            count = n > 0 ? n : 1;
+        }
+        return ptr;
+    }
+};
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 inline int device_cu_count() {

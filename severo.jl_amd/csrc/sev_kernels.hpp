@@ -463,6 +463,8 @@ struct SnnWork {
     DevBuf<unsigned char> stage;  // sorted kept slots of the wave-path columns
     DevBuf<int64_t> rptr, scratch, htab_off, heavyP;
     DevBuf<uint64_t> htab;
+    HostPinned<int64_t> hP, hoff;     // host copies: heavy columns' P, table offsets
+    HostPinned<int32_t> hcol, horder;  // heavy columns as found, and by class
     int64_t nheavy_h = 0;
     int64_t nheavy_cls[3] = {0, 0, 0};  // heavy columns per table class
 };

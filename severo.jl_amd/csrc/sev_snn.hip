@@ -644,30 +644,36 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
         // the heavy columns ordered by class (small LDS table, big LDS table,
         // global table); the global ones get a table of 2 * next_pow2(P)
         // slots plus a sort buffer of half that
-        std::vector<int64_t> hp(nh);
-        std::vector<int32_t> hj(nh);
-        SEV_HIP(hipMemcpy(hp.data(), w.heavyP.get(), sizeof(int64_t) * nh, hipMemcpyDeviceToHost));
-        SEV_HIP(hipMemcpy(hj.data(), w.heavy.get(), sizeof(int32_t) * nh, hipMemcpyDeviceToHost));
+        // pinned host copies on the stream (the pageable path's staging cost
+        // milliseconds here)
+        int64_t *hp = w.hP.ensure((size_t)nh);
+        int32_t *hj = w.hcol.ensure((size_t)nh);
+        int32_t *order = w.horder.ensure((size_t)nh);
+        SEV_HIP(hipMemcpyAsync(hp, w.heavyP.get(), sizeof(int64_t) * nh, hipMemcpyDeviceToHost,
+                               st));
+        SEV_HIP(hipMemcpyAsync(hj, w.heavy.get(), sizeof(int32_t) * nh, hipMemcpyDeviceToHost,
+                               st));
+        SEV_HIP(hipStreamSynchronize(st));
         const int64_t lim[2] = {small_maxP(n), big_maxP(n)};
-        std::vector<int32_t> order;
-        order.reserve((size_t)nh);
+        int64_t no = 0;
         std::vector<int64_t> gp;
         for (int c = 0; c < 3; ++c) {
-            const int64_t before = (int64_t)order.size();
+            const int64_t before = no;
             for (int32_t h = 0; h < nh; ++h) {
                 const int cls = hp[h] <= lim[0] ? 0 : (hp[h] <= lim[1] ? 1 : 2);
                 if (cls != c) continue;
-                order.push_back(hj[h]);
+                order[no++] = hj[h];
                 if (c == 2) gp.push_back(hp[h]);
             }
-            w.nheavy_cls[c] = (int64_t)order.size() - before;
+            w.nheavy_cls[c] = no - before;
         }
         // columns are independent, so the order within a class is free: kept
         // as found for determinism of the layout (every result is sorted)
-        SEV_HIP(hipMemcpy(w.heavy.get(), order.data(), sizeof(int32_t) * nh,
-                          hipMemcpyHostToDevice));
+        SEV_HIP(hipMemcpyAsync(w.heavy.get(), order, sizeof(int32_t) * nh,
+                               hipMemcpyHostToDevice, st));
         const int64_t ng = w.nheavy_cls[2];
-        std::vector<int64_t> offs(ng + 1, 0);
+        int64_t *offs = w.hoff.ensure((size_t)ng + 1);
+        offs[0] = 0;
         int64_t mx = 0;
         for (int64_t h = 0; h < ng; ++h) {
             int64_t slots = 1;
@@ -684,8 +690,8 @@ int64_t snn_count(const int32_t *nn, int64_t n, int64_t k, int xmin, SnnWork &w,
                          (long long)w.nheavy_cls[1], (long long)lim[1], (long long)ng,
                          (long long)mx);
         w.htab_off.ensure((size_t)ng + 1);
-        SEV_HIP(hipMemcpy(w.htab_off.get(), offs.data(), sizeof(int64_t) * (ng + 1),
-                          hipMemcpyHostToDevice));
+        SEV_HIP(hipMemcpyAsync(w.htab_off.get(), offs, sizeof(int64_t) * (ng + 1),
+                               hipMemcpyHostToDevice, st));
         w.htab.ensure((size_t)(offs[(size_t)ng] > 0 ? offs[(size_t)ng] : 1));
         launch_block<T>(true, nn, n, (int)k, xmin, w, nullptr, nullptr, nullptr, nullptr, st);
     }
