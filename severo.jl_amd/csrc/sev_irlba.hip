@@ -954,6 +954,12 @@ k_gemm_mfma(int64_t m, int work, const TX *X, const double *Cf, int c0, int kout
 // caller redoes that restart on the host).
 // --------------------------------------------------------------------------
 constexpr int kSvdThreads = 512;  // 32 groups of 16 lanes: one pair each
+// A pair is rotated while its cosine exceeds 1e-12.  Further rotations would
+// move the singular values by under 1e-24 (relative) and the vectors'
+// orthogonality by under 1e-12, far below what a restart needs (each new
+// Lanczos vector is reorthogonalised in FP64); against the eps * sqrt(n)
+// level this saves about one sweep in ten at 112 columns.
+constexpr double kJacobiCos2 = 1e-24;
 
 // 1/x and 1/sqrt(x) from the hardware approximations with two Newton steps
 // (relative error at the FP64 rounding level).
@@ -1003,7 +1009,7 @@ __device__ __forceinline__ void small_svd_body(int n, const double *B, double *U
     const int tid = threadIdx.x;
     const int ne = n + (n & 1);
     const int pr = tid >> 4, gl = tid & 15;
-    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    const double tol2 = kJacobiCos2;
     if (tid == 0) {
         need_j = 0;
         nsweeps = 0;
@@ -1188,7 +1194,7 @@ __device__ __forceinline__ void small_svd8_body(int n, const double *B, double *
     const int tid = threadIdx.x;
     const int pr = tid >> 3, gl = tid & 7;
     const int npair = ne / 2;
-    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    const double tol2 = kJacobiCos2;
     if (tid == 0) {
         need_j = 0;
         nsweeps = 0;
@@ -1359,7 +1365,7 @@ k_small_svd_big(int n, const double *B, double *U, double *s, double *V, int *st
     const int ne = n + (n & 1);
     const int npair = ne / 2;
     const int pr = tid >> 3, gl = tid & 7;
-    const double tol2 = 2.220446049250313e-16 * 2.220446049250313e-16 * (double)n;
+    const double tol2 = kJacobiCos2;
     if (tid == 0) {
         nsweeps = 0;
         ill = 0;
