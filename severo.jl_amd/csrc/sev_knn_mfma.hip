@@ -781,9 +781,23 @@ k_knn_rerank(const float *Zr, int64_t nq, const int64_t *qidx, int d, int k, con
         const float *zq = Zr + q * d;
         const float *zc = Zr + (int64_t)c * d;
         float acc = 0.0f;
-        for (int t = 0; t < d; ++t) {
-            const float diff = zq[t] - zc[t];
-            acc = acc + diff * diff;
+        if ((d & 1) == 0) {
+            // even d: rows start on 8-byte boundaries (Zr is a device
+            // allocation), so the coordinates are read in pairs; the sum
+            // still runs over t in order
+            const float2 *q2 = (const float2 *)zq, *c2 = (const float2 *)zc;
+            for (int t = 0; t < d / 2; ++t) {
+                const float2 a = q2[t], b = c2[t];
+                const float d0 = a.x - b.x;
+                acc = acc + d0 * d0;
+                const float d1 = a.y - b.y;
+                acc = acc + d1 * d1;
+            }
+        } else {
+            for (int t = 0; t < d; ++t) {
+                const float diff = zq[t] - zc[t];
+                acc = acc + diff * diff;
+            }
         }
         D = acc;
     }
