@@ -539,32 +539,39 @@ k_knn_cand(const void *Zq_, const void *Zc_, int64_t n, int include_self, const 
             // 4 x 16 products against the row's threshold (IEEE maximum, so
             // the MFMA results are not canonicalised first; the tests are
             // combined as wave masks)
-            uint64_t any = 0;
+            uint64_t any = 0, anyi[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const float m = __builtin_elementwise_maximum(
                     __builtin_elementwise_maximum(acc[0][rb][i], acc[1][rb][i]),
                     __builtin_elementwise_maximum(acc[2][rb][i], acc[3][rb][i]));
-                any |= __ballot(m > thr[rb][i]);
+                anyi[i] = __ballot(m > thr[rb][i]);
+                any |= anyi[i];
             }
             if (!any) continue;
+            // the candidate id of this lane's column in each candidate block
+            int32_t cids[NCB];
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                // the candidate id of this lane's column
-                const int32_t cid = perm ? tileid[buf][cb * 16 + col] : t32 + cb * 16 + col;
-                // products as integers for readlane (on vector elements it was
-                // seen to read element 0 for every element)
-                int ai[4];
+            for (int cb = 0; cb < NCB; ++cb)
+                cids[cb] = perm ? tileid[buf][cb * 16 + col] : t32 + cb * 16 + col;
+            // only the output rows i whose tile maximum passed can hold a
+            // passing pair (thresholds only rise); their pairs are visited
+            // block by block (the lists do not depend on the order)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) ai[i] = __float_as_int(acc[cb][rb][i]);
+            for (int i = 0; i < 4; ++i) {
+                if (!anyi[i]) continue;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
+                for (int cb = 0; cb < NCB; ++cb) {
+                    const int32_t cid = cids[cb];
+                    // the product as an integer for readlane (on vector
+                    // elements it was seen to read element 0 for every element)
+                    const int ai = __float_as_int(acc[cb][rb][i]);
                     uint64_t mask = __ballot(acc[cb][rb][i] > thr[rb][i]);
                     while (mask) {
                         const int src = __builtin_ctzll(mask);
                         mask &= mask - 1;
                         const int g = src >> 4;
-                        const float v = __int_as_float(__builtin_amdgcn_readlane(ai[i], src));
+                        const float v = __int_as_float(__builtin_amdgcn_readlane(ai, src));
                         // the row's threshold lives in the lanes of group g
                         const float ts =
                             __int_as_float(__builtin_amdgcn_readlane(__float_as_int(thr[rb][i]),
