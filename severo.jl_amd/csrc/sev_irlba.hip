@@ -2131,15 +2131,8 @@ void irlba_run(const SlabCsc &op, const IrlbaParams &p, const double *init, Irlb
         // reorthogonalisation (W[:,0:d0]' A v_d0)
         SEV_HIP(hipMemcpyAsync(w.V.get(), p.rV, sizeof(double) * n * d0, hipMemcpyHostToDevice,
                                st));
-        if constexpr (b32) {
-            const std::vector<float> rUf(p.rU, p.rU + m * d0);
-            SEV_HIP(hipMemcpyAsync(Wb.get(), rUf.data(), sizeof(float) * m * d0,
-                                   hipMemcpyHostToDevice, st));
-            SEV_HIP(hipStreamSynchronize(st));
-        } else {
-            SEV_HIP(hipMemcpyAsync(Wb.get(), p.rU, sizeof(double) * m * d0,
-                                   hipMemcpyHostToDevice, st));
-        }
+        SEV_REQUIRE(!b32, "irlba: a warm restart keeps the FP64 basis");
+        SEV_HIP(hipMemcpyAsync(Wb.get(), p.rU, sizeof(TW) * m * d0, hipMemcpyHostToDevice, st));
         std::vector<double> Bd((size_t)work * work, 0.0);
         for (int64_t c = 0; c < d0; ++c) Bd[(size_t)c * work + c] = p.rs[c];
         SEV_HIP(hipMemcpyAsync(w.B.get(), Bd.data(), sizeof(double) * work * work,
@@ -2334,7 +2327,8 @@ void irlba_run(const SlabCsc &op, const IrlbaParams &p, const double *init, Irlb
 void irlba_device(const SlabCsc &op, const IrlbaParams &p, const double *init, IrlbaWork &w,
                   double *s, double *U, double *V, hipStream_t st, IrlbaOut *out,
                   const HostOperator *hop) {
-    if (p.basis32 && !hop) {
+    // the FP32 left basis serves the device operator from a cold start
+    if (p.basis32 && !hop && p.restart == 0) {
         w.W.release();
         w.W2.release();
         irlba_run<float>(op, p, init, w, w.Wf, w.Wf2, s, U, V, st, out, hop);
