@@ -207,6 +207,33 @@ def test_pipeline_pca_vs_gram_oracle(sev, precision, tol_s):
     assert res < (1e-5 if precision == "fp32" else 1e-3)
 
 
+def test_pipeline_pca_fp32_basis_matches_fp64_basis(sev):
+    """The BF16 pipeline form stores its left Lanczos basis in FP32; the
+    centered IRLBA entry point on the same BF16 operator keeps it in FP64.
+    With the same matrix, start vector, basis size and tolerance the two
+    agree: singular values to 1e-7 relative, the leading 5-dimensional
+    right singular space to 1e-4 rad."""
+    t = sev.nb_tables(5000, seed=0, density=0.05)
+    C = 5000
+    p = sev.Pipeline()
+    p.synth_counts(C, t)
+    p.normalize("lognormalize", 1e4)
+    feats = sev.top_mean_features(t, 1000)
+    p.select_scale(feats, 10.0)
+    B, mu = p.get_scaled()
+    init = np.random.default_rng(0).standard_normal(1000)
+    p.pca(30, init=init, precision="bf16", work=60)
+    s, U, V, Z = p.get_pca()
+    ref = sev.irlba(sev.CenteredMatrix(B, mu), 30, init=init, precision="bf16", work=60)
+    assert np.max(np.abs(s - ref.S) / ref.S) < 1e-7
+    assert _angle(V[:, :5], ref.V[:, :5]) < 1e-4
+    # the residual invariant on the FP64 centered matrix, as for the FP64 basis
+    Q = B.toarray().astype(np.float64) - mu.astype(np.float64)
+    res = np.linalg.norm(Q.T @ U - V * s) / np.linalg.norm(Q)
+    res_ref = np.linalg.norm(Q.T @ ref.U - ref.V * ref.S) / np.linalg.norm(Q)
+    assert res < 2 * res_ref + 1e-9
+
+
 def _clustered_counts(C, G, ncl, seed):
     """Negative-binomial counts of cells from ncl clusters whose gene means
     differ (log-normal, 60% of genes shifted per cluster): the PCA of such
