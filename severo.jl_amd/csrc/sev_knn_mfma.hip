@@ -137,29 +137,60 @@ k_knn_prep(const float *Z, int64_t n, int64_t d, int64_t ld, void *Zq, void *Zc,
                 c[t] = 0.0f;
             }
         } else {
-            uint16_t *q = (uint16_t *)Zq + i * KP;
-            uint16_t *c = (uint16_t *)Zc + i * KP;
-            for (int64_t t = 0; t < d; ++t) {
-                const float z = Z[t * ld + i];
-                const uint16_t b = f2bf(z);
-                const float zb = bf2f(b);
-                q[t] = b;
-                c[t] = b;
-                Zr[i * d + t] = z;
-                nn += (double)zb * (double)zb;
-                const double e = (double)zb - (double)z;
-                ee += e * e;
+            // the row is assembled in registers (pairs of BF16 per word) and
+            // written as 16-byte pieces: 2-byte stores strided by the row
+            // length left most of each written line to be merged
+            uint32_t wz[KP / 2];
+#pragma unroll
+            for (int u = 0; u < KP / 2; ++u) wz[u] = 0u;
+            float *zr = Zr + i * d;
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                if (t < d) {
+                    const float z = Z[(int64_t)t * ld + i];
+                    const uint16_t b = f2bf(z);
+                    const float zb = bf2f(b);
+                    wz[t / 2] |= (uint32_t)b << (16 * (t & 1));
+                    zr[t] = z;
+                    nn += (double)zb * (double)zb;
+                    const double e = (double)zb - (double)z;
+                    ee += e * e;
+                }
             }
             const float half = (float)(-0.5 * nn);
             const uint16_t hi = f2bf(half);
             const uint16_t lo = f2bf((float)((double)half - (double)bf2f(hi)));
-            q[d] = 0x3F80;  // 1.0
-            q[d + 1] = 0x3F80;
-            c[d] = hi;
-            c[d + 1] = lo;
-            for (int64_t t = d + 2; t < KP; ++t) {
-                q[t] = 0;
-                c[t] = 0;
+            uint32_t wq[KP / 2], wc[KP / 2];
+#pragma unroll
+            for (int u = 0; u < KP / 2; ++u) {
+                uint32_t q2 = wz[u], c2 = wz[u];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int t = 2 * u + h;
+                    if (t == d || t == d + 1) {
+                        q2 |= 0x3F80u << (16 * h);  // 1.0
+                        c2 |= (uint32_t)(t == d ? hi : lo) << (16 * h);
+                    }
+                }
+                wq[u] = q2;
+                wc[u] = c2;
+            }
+            if constexpr (KP % 8 == 0) {  // the BF16 row lengths (64, 128)
+                uint4 *q4 = (uint4 *)((uint16_t *)Zq + i * KP);
+                uint4 *c4 = (uint4 *)((uint16_t *)Zc + i * KP);
+#pragma unroll
+                for (int u = 0; u < KP / 8; ++u) {
+                    q4[u] = make_uint4(wq[4 * u], wq[4 * u + 1], wq[4 * u + 2], wq[4 * u + 3]);
+                    c4[u] = make_uint4(wc[4 * u], wc[4 * u + 1], wc[4 * u + 2], wc[4 * u + 3]);
+                }
+            } else {
+                uint32_t *q1 = (uint32_t *)((uint16_t *)Zq + i * KP);
+                uint32_t *c1 = (uint32_t *)((uint16_t *)Zc + i * KP);
+#pragma unroll
+                for (int u = 0; u < KP / 2; ++u) {
+                    q1[u] = wq[u];
+                    c1[u] = wc[u];
+                }
             }
         }
         const float nf = (float)(nn * (1.0 + 1e-6)) + 1e-30f;
