@@ -488,13 +488,28 @@ k_rel_mean_var(int64_t m, int64_t ncol, const int64_t *colptr, const int32_t *ro
 }
 
 // b = min(x / sigma, clip) with only the upper clip (src/scaling.jl:210-213).
+// Column blockIdx.x, shared by the gridDim.y blocks of that column (the
+// densest columns hold most of the entries); four loads of a thread in
+// flight at once.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_scale_map(const int64_t *colptr, const T *x, const T *sdv, const T *clipv, T *out) {
     const int64_t j = blockIdx.x;
     const int64_t b = colptr[j], e = colptr[j + 1];
     const T sd = sdv[j], clip = clipv[j];
-    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+    const int64_t step = (int64_t)gridDim.y * kBlock;
+    int64_t p = b + (int64_t)blockIdx.y * kBlock + threadIdx.x;
+    for (; p + 3 * step < e; p += 4 * step) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = x[p + u * step];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const T q = v[u] / sd;
+            out[p + u * step] = (q > clip) ? clip : q;
+        }
+    }
+    for (; p < e; p += step) {
         const T v = x[p] / sd;
         out[p] = (v > clip) ? clip : v;
     }
@@ -662,8 +677,11 @@ void scale_map(int64_t ncol, const int64_t *colptr, const T *x, const T *sd, con
                T *out, hipStream_t st) {
     if (ncol == 0) return;
     ProfScope ps("scale_map", st);
-    hipLaunchKernelGGL(k_scale_map<T>, dim3((unsigned)ncol), dim3(kBlock), 0, st, colptr, x, sd,
-                       clip, out);
+    // a few long columns (one sample's features) take 8 blocks each; the
+    // many short columns of a batch one
+    const unsigned slices = ncol >= 8192 ? 1u : 8u;
+    hipLaunchKernelGGL(k_scale_map<T>, dim3((unsigned)ncol, slices), dim3(kBlock), 0, st, colptr,
+                       x, sd, clip, out);
     SEV_LAUNCH_CHECK();
 }
 
