@@ -230,10 +230,28 @@ template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row, const T *val,
               const int64_t *fcolptr, int32_t *orow, T *oval) {
+    // column blockIdx.x, shared by the gridDim.y blocks of that column, four
+    // loads of a thread in flight at once
     const int64_t j = blockIdx.x;
     const int64_t g = feat[j];
     const int64_t b = colptr[g], len = colptr[g + 1] - b, ob = fcolptr[j];
-    for (int64_t i = threadIdx.x; i < len; i += kBlock) {
+    const int64_t step = (int64_t)gridDim.y * kBlock;
+    int64_t i = (int64_t)blockIdx.y * kBlock + threadIdx.x;
+    for (; i + 3 * step < len; i += 4 * step) {
+        int32_t r[4];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            r[u] = row[b + i + u * step];
+            v[u] = val[b + i + u * step];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            orow[ob + i + u * step] = r[u];
+            oval[ob + i + u * step] = v[u];
+        }
+    }
+    for (; i < len; i += step) {
         orow[ob + i] = row[b + i];
         oval[ob + i] = val[b + i];
     }
@@ -655,8 +673,9 @@ void gather_copy(const int64_t *feat, const int64_t *colptr, const int32_t *row,
                  int64_t nfeat, const int64_t *fcolptr, int32_t *orow, T *oval, hipStream_t st) {
     if (nfeat == 0) return;
     ProfScope ps("gather", st);
-    hipLaunchKernelGGL(k_gather_copy<T>, dim3((unsigned)nfeat), dim3(kBlock), 0, st, feat, colptr,
-                       row, val, fcolptr, orow, oval);
+    // a sample's selected columns are few and long: 8 blocks each
+    hipLaunchKernelGGL(k_gather_copy<T>, dim3((unsigned)nfeat, 8), dim3(kBlock), 0, st, feat,
+                       colptr, row, val, fcolptr, orow, oval);
     SEV_LAUNCH_CHECK();
 }
 
