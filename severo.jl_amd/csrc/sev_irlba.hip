@@ -140,13 +140,17 @@ __device__ __forceinline__ double butterfly32(double (&v)[32]) {
 __global__ void __launch_bounds__(kBlock)
 k_slab_offsets(int64_t n, int64_t nslab, int64_t R, const int64_t *colptr, const int32_t *row,
                int64_t *off, uint16_t *lrow) {
+    // column blockIdx.x; its entries are shared by the gridDim.y blocks of
+    // that column (each entry's writes are its own)
     const int64_t j = blockIdx.x;
     const int64_t b = colptr[j], e = colptr[j + 1];
     if (b == e) {
-        for (int64_t s = threadIdx.x; s <= nslab; s += kBlock) off[s * n + j] = b;
+        if (blockIdx.y == 0)
+            for (int64_t s = threadIdx.x; s <= nslab; s += kBlock) off[s * n + j] = b;
         return;
     }
-    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+    for (int64_t p = b + (int64_t)blockIdx.y * kBlock + threadIdx.x; p < e;
+         p += (int64_t)gridDim.y * kBlock) {
         const int64_t r = row[p];
         const int64_t sc = r / R;
         const int64_t sp = (p == b) ? -1 : (int64_t)row[p - 1] / R;
@@ -202,8 +206,11 @@ k_csr_place(int64_t nnz, int64_t n, const int64_t *colptr, const uint32_t *srow,
 // the sorted words hold everything the CSR needs
 __global__ void __launch_bounds__(kBlock)
 k_csr_words(int64_t n, const int64_t *colptr, const float *val, uint32_t *word) {
+    // a column's entries are shared by the gridDim.y blocks of its x index
+    const int64_t step = (int64_t)gridDim.y * kBlock;
     for (int64_t j = blockIdx.x; j < n; j += gridDim.x)
-        for (int64_t e = colptr[j] + threadIdx.x; e < colptr[j + 1]; e += kBlock)
+        for (int64_t e = colptr[j] + (int64_t)blockIdx.y * kBlock + threadIdx.x;
+             e < colptr[j + 1]; e += step)
             word[e] = ((uint32_t)j << 16) | cvt_store(val[e], (uint16_t *)nullptr);
 }
 
@@ -1667,8 +1674,8 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
     op.vtype = src_f64 ? kValF64 : kValF32;
     {
         ProfScope ps("slab_build", st, 6.0 * op.nnz + 8.0 * (op.nslab + 1) * n);
-        hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n), dim3(kBlock), 0, st, n, op.nslab,
-                           kSlabRows, colptr, row, op.off.get(), op.lrow.get());
+        hipLaunchKernelGGL(k_slab_offsets, dim3((unsigned)n, 8), dim3(kBlock), 0, st, n,
+                           op.nslab, kSlabRows, colptr, row, op.off.get(), op.lrow.get());
         SEV_LAUNCH_CHECK();
     }
     if (bf16) {
@@ -1715,7 +1722,7 @@ void build_slab_csc(SlabCsc &op, int64_t m, int64_t n, const int64_t *colptr,
         b.sids.ensure((size_t)op.nnz);
         b.srow.ensure((size_t)op.nnz);
         if (words) {
-            hipLaunchKernelGGL(k_csr_words, dim3((unsigned)std::min<int64_t>(n, 65535)),
+            hipLaunchKernelGGL(k_csr_words, dim3((unsigned)std::min<int64_t>(n, 65535), 8),
                                dim3(kBlock), 0, st, n, colptr, (const float *)val,
                                (uint32_t *)b.ids.get());
         } else {
