@@ -2859,13 +2859,17 @@ k_b_embed(const double *U, const double *s_all, const int32_t *row_sample, int64
 __global__ void __launch_bounds__(kBlock)
 k_var_slab_offsets(int64_t n, int64_t nslab, const int32_t *slab_of_row, const int64_t *slab_row0,
                    const int64_t *colptr, const int32_t *row, int64_t *off, uint16_t *lrow) {
+    // column blockIdx.x; its entries are shared by the gridDim.y blocks of
+    // that column (each entry's writes are its own)
     const int64_t j = blockIdx.x;
     const int64_t b = colptr[j], e = colptr[j + 1];
     if (b == e) {
-        for (int64_t s = threadIdx.x; s <= nslab; s += kBlock) off[s * n + j] = b;
+        if (blockIdx.y == 0)
+            for (int64_t s = threadIdx.x; s <= nslab; s += kBlock) off[s * n + j] = b;
         return;
     }
-    for (int64_t p = b + threadIdx.x; p < e; p += kBlock) {
+    for (int64_t p = b + (int64_t)blockIdx.y * kBlock + threadIdx.x; p < e;
+         p += (int64_t)gridDim.y * kBlock) {
         const int64_t r = row[p];
         const int64_t sc = slab_of_row[r];
         const int64_t sp = (p == b) ? -1 : (int64_t)slab_of_row[row[p - 1]];
@@ -2962,7 +2966,7 @@ void batch_slabs(SlabCsc &op, const BatchLayout &L, const int64_t *colptr, const
     SEV_LAUNCH_CHECK();
     op.nslab = nslab;
     op.off.ensure((size_t)((nslab + 1) * op.n));
-    hipLaunchKernelGGL(k_var_slab_offsets, dim3((unsigned)op.n), dim3(kBlock), 0, st, op.n, nslab,
+    hipLaunchKernelGGL(k_var_slab_offsets, dim3((unsigned)op.n, 8), dim3(kBlock), 0, st, op.n, nslab,
                        slab_of_row.get(), L.slab_row0_d.get(), colptr, row, op.off.get(),
                        op.lrow.get());
     SEV_LAUNCH_CHECK();
